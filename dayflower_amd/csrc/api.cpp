@@ -1,0 +1,842 @@
+// api.cpp — the C-ABI of include/dayflower_hip.h: renderer context, scene
+// compiler (BVH + SoA upload), render loops, ray queries.  Batching and the
+// serving loop live in serve.cpp.
+#include "api_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+#include "bvh.h"
+#include "perez_host.h"
+
+using namespace dfr;
+
+namespace dfr {
+
+// ---------------------------------------------------------------------------
+// material compiler: constant textures make every BSDF a fixed lobe list
+// (Material.computeScatteringFunctions of M/scene/material/*Material.java,
+// transport RADIANCE, multiple lobes allowed).
+// ---------------------------------------------------------------------------
+static float roughness_to_alpha(float r) {  // MicrofacetDistribution.convertRoughnessToAlpha
+    float x = std::max(r, 1.0e-3f), y = (float)std::log((double)x);
+    return 1.62142f + 0.819955f * y + 0.1734f * y * y + 0.0171201f * y * y * y + 0.000640711f * y * y * y * y;
+}
+static void set3(float* d, float a, float b, float c) { d[0] = a; d[1] = b; d[2] = c; }
+static void sat3(float* d, const float* s) { for (int k = 0; k < 3; k++) d[k] = std::max(s[k], 0.0f); }
+static bool black3(const float* c) { return c[0] == 0.0f && c[1] == 0.0f && c[2] == 0.0f; }
+
+void compile_material(const dfr_material& m, DevMaterial& out, std::vector<DevLobe>& lobes) {
+    out.first_lobe = (int32_t)lobes.size();
+    out.n_lobes = 0;
+    out.has_bsdf = 1;
+    out.type = m.type;
+    out.eta = 1.0f;
+    for (int k = 0; k < 3; k++) out.emission[k] = m.emission[k];
+    auto add = [&](const DevLobe& l) { lobes.push_back(l); out.n_lobes++; };
+    auto tr = [](DevLobe& l, float ax, float ay, bool sep) {
+        l.alphaX = std::max(ax, 0.001f); l.alphaY = std::max(ay, 0.001f); l.separable = sep ? 1 : 0;
+    };
+    switch (m.type) {
+        case DFR_MAT_MATTE: {
+            DevLobe l{}; l.type = BX_DIFF | BX_REFL; sat3(l.R, m.kd);
+            float deg = std::min(std::max(m.angle, 0.0f), 90.0f);
+            if (deg == 0.0f) l.kind = LOBE_LAMBERT;
+            else {
+                float r = deg * (3.14159265358979323846f / 180.0f), r2 = r * r;
+                l.kind = LOBE_OREN_NAYAR; l.A = 1.0f - (r2 / (2.0f * (r2 + 0.33f))); l.B = 0.45f * r2 / (r2 + 0.09f);
+            }
+            add(l);
+            break;
+        }
+        case DFR_MAT_MIRROR: {
+            DevLobe l{}; l.kind = LOBE_SPECULAR_R; l.type = BX_SPEC | BX_REFL; sat3(l.R, m.kr); l.fresnel = FR_CONST;
+            if (black3(l.R)) { out.has_bsdf = 0; break; }
+            add(l);
+            break;
+        }
+        case DFR_MAT_GLASS: {
+            float kr[3], kt[3]; sat3(kr, m.kr); sat3(kt, m.kt);
+            float eta = m.eta[0];
+            if (black3(kr) && black3(kt)) { out.has_bsdf = 0; break; }
+            out.eta = eta;
+            if (m.roughness_u == 0.0f && m.roughness_v == 0.0f) {
+                DevLobe l{}; l.kind = LOBE_FRESNEL_SPECULAR; l.type = BX_SPEC | BX_REFL | BX_TRANS;
+                std::memcpy(l.R, kr, 12); std::memcpy(l.T, kt, 12); l.etaA = 1.0f; l.etaB = eta;
+                add(l);
+                break;
+            }
+            float ru = m.remap_roughness ? roughness_to_alpha(m.roughness_u) : m.roughness_u;
+            float rv = m.remap_roughness ? roughness_to_alpha(m.roughness_v) : m.roughness_v;
+            if (!black3(kr)) {
+                DevLobe l{}; l.kind = LOBE_TS_R; l.type = BX_GLOSSY | BX_REFL; std::memcpy(l.R, kr, 12);
+                tr(l, ru, rv, false); l.fresnel = FR_DIELECTRIC; l.frEtaI = 1.0f; l.frEtaT = eta;
+                add(l);
+            }
+            if (!black3(kt)) {
+                DevLobe l{}; l.kind = LOBE_TS_T; l.type = BX_GLOSSY | BX_TRANS; std::memcpy(l.T, kt, 12);
+                tr(l, ru, rv, false); l.etaA = 1.0f; l.etaB = eta;
+                add(l);
+            }
+            break;
+        }
+        case DFR_MAT_METAL: {
+            float ru = m.remap_roughness ? roughness_to_alpha(m.roughness_u) : m.roughness_u;
+            float rv = m.remap_roughness ? roughness_to_alpha(m.roughness_v) : m.roughness_v;
+            DevLobe l{}; l.kind = LOBE_TS_R; l.type = BX_GLOSSY | BX_REFL; set3(l.R, 1, 1, 1);
+            l.fresnel = FR_CONDUCTOR; std::memcpy(l.condEta, m.eta, 12); std::memcpy(l.condK, m.k, 12);
+            tr(l, ru, rv, false);
+            add(l);
+            break;
+        }
+        case DFR_MAT_PLASTIC: {
+            float kd[3], ks[3]; sat3(kd, m.kd); sat3(ks, m.ks);
+            if (!black3(kd)) { DevLobe l{}; l.kind = LOBE_LAMBERT; l.type = BX_DIFF | BX_REFL; std::memcpy(l.R, kd, 12); add(l); }
+            if (!black3(ks)) {
+                float r = m.remap_roughness ? roughness_to_alpha(m.roughness_u) : m.roughness_u;
+                DevLobe l{}; l.kind = LOBE_TS_R; l.type = BX_GLOSSY | BX_REFL; std::memcpy(l.R, ks, 12);
+                tr(l, r, r, false); l.fresnel = FR_DIELECTRIC; l.frEtaI = 1.5f; l.frEtaT = 1.0f;
+                add(l);
+            }
+            if (out.n_lobes == 0) out.has_bsdf = 0;
+            break;
+        }
+        case DFR_MAT_DISNEY: {  // thin = false, scatter distance = 0
+            float color[3]; sat3(color, m.kd);
+            float eta = m.eta[0], metal = m.disney_metallic, rough = m.disney_roughness, sheen = m.disney_sheen;
+            float strans = m.disney_spec_trans;
+            float lum = 0.212671f * color[0] + 0.715160f * color[1] + 0.072169f * color[2];
+            float tint[3];
+            for (int k = 0; k < 3; k++) tint[k] = lum > 0.0f ? color[k] / lum : 1.0f;
+            float dw = (1.0f - metal) * (1.0f - strans);
+            auto lerp = [](float a, float b, float t) { return (1.0f - t) * a + t * b; };
+            if (dw > 0.0f) {
+                DevLobe d{}; d.kind = LOBE_DISNEY_DIFFUSE; d.type = BX_DIFF | BX_REFL;
+                for (int k = 0; k < 3; k++) d.R[k] = color[k] * dw;
+                add(d);
+                DevLobe r{}; r.kind = LOBE_DISNEY_RETRO; r.type = BX_DIFF | BX_REFL; r.roughness = rough;
+                for (int k = 0; k < 3; k++) r.R[k] = color[k] * dw;
+                add(r);
+                if (sheen > 0.0f) {
+                    DevLobe s{}; s.kind = LOBE_DISNEY_SHEEN; s.type = BX_DIFF | BX_REFL;
+                    for (int k = 0; k < 3; k++) s.R[k] = lerp(1.0f, tint[k], m.disney_sheen_tint) * (dw * sheen);
+                    add(s);
+                }
+            }
+            float aspect = (float)std::sqrt((double)(1.0f - m.disney_anisotropic * 0.9f));
+            float ax = std::max(0.001f, rough * rough / aspect), ay = std::max(0.001f, rough * rough * aspect);
+            float r0 = ((eta - 1.0f) * (eta - 1.0f)) / ((eta + 1.0f) * (eta + 1.0f));
+            DevLobe s{}; s.kind = LOBE_TS_R; s.type = BX_GLOSSY | BX_REFL; set3(s.R, 1, 1, 1);
+            tr(s, ax, ay, true);
+            s.fresnel = FR_DISNEY; s.disneyEta = eta; s.disneyMetallic = metal;
+            for (int k = 0; k < 3; k++) s.r0[k] = lerp(lerp(1.0f, tint[k], m.disney_specular_tint) * r0, color[k], metal);
+            add(s);
+            if (m.disney_clearcoat > 0.0f) {
+                DevLobe c{}; c.kind = LOBE_DISNEY_CLEARCOAT; c.type = BX_GLOSSY | BX_REFL;
+                c.gloss = lerp(0.1f, 0.001f, m.disney_clearcoat_gloss); c.weight = m.disney_clearcoat;
+                add(c);
+            }
+            if (strans > 0.0f) {
+                DevLobe t{}; t.kind = LOBE_TS_T; t.type = BX_GLOSSY | BX_TRANS;
+                for (int k = 0; k < 3; k++) t.T[k] = (float)std::sqrt((double)color[k]) * strans;
+                tr(t, ax, ay, true); t.etaA = 1.0f; t.etaB = eta;
+                add(t);
+            }
+            break;
+        }
+        default: out.has_bsdf = 0; break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// scene compiler
+// ---------------------------------------------------------------------------
+static float4 f4(float x, float y, float z, float w) { float4 v; v.x = x; v.y = y; v.z = z; v.w = w; return v; }
+
+template <class T>
+static int upload(DeviceBuffer& b, const std::vector<T>& v, hipStream_t s) {
+    size_t bytes = std::max<size_t>(sizeof(T), v.size() * sizeof(T));
+    if (b.alloc(bytes) != hipSuccess) return DFR_E_MEMORY;
+    if (!v.empty() && hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess)
+        return DFR_E_DEVICE;
+    return DFR_OK;
+}
+
+int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, hipStream_t s, std::string& err) {
+    const int64_t nt = (int64_t)in.indices.size() / 3;
+    const int64_t nv = (int64_t)in.positions.size() / 3;
+    for (int64_t i = 0; i < 3 * nt; i++)
+        if (in.indices[i] < 0 || in.indices[i] >= nv) { err = "triangle index out of range"; return DFR_E_ARGUMENT; }
+    std::vector<int32_t> tri_prim(nt, -1);
+    for (size_t p = 0; p < in.prims.size(); p++) {
+        const dfr_primitive& pr = in.prims[p];
+        if (pr.material < 0 || pr.material >= (int)in.mats.size()) { err = "primitive material out of range"; return DFR_E_ARGUMENT; }
+        if (pr.shape == DFR_SHAPE_TRIANGLE_MESH) {
+            if (pr.first_triangle < 0 || pr.triangle_count < 0 || pr.first_triangle + pr.triangle_count > nt) {
+                err = "primitive triangle range out of bounds"; return DFR_E_ARGUMENT;
+            }
+            for (int64_t t = pr.first_triangle; t < pr.first_triangle + pr.triangle_count; t++) tri_prim[t] = (int32_t)p;
+        } else if (pr.shape < DFR_SHAPE_TRIANGLE_MESH || pr.shape > DFR_SHAPE_PLANE) {
+            err = "unknown shape"; return DFR_E_ARGUMENT;
+        }
+    }
+    for (int64_t t = 0; t < nt; t++)
+        if (tri_prim[t] < 0) { err = "triangle not owned by any mesh primitive"; return DFR_E_ARGUMENT; }
+
+    auto t0 = std::chrono::steady_clock::now();
+    int threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    BvhResult bvh = build_bvh(in.positions.data(), in.indices.data(), nt, threads);
+    out.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    out.max_depth = bvh.max_depth + 1;
+    out.n_nodes = (int64_t)bvh.nodes.size();
+    out.n_tris = nt;
+    out.sah_cost = bvh.sah_cost;
+
+    std::vector<float4> tris(3 * nt), vpos(nv), vnrm(nv), tri_ng(nt);
+    std::vector<int4> tri_idx(nt);
+    for (int64_t v = 0; v < nv; v++) {
+        const float* p = &in.positions[3 * v];
+        vpos[v] = f4(p[0], p[1], p[2], 0.0f);
+    }
+    for (int64_t t = 0; t < nt; t++) {
+        const int32_t* ix = &in.indices[3 * t];
+        tri_idx[t].x = ix[0]; tri_idx[t].y = ix[1]; tri_idx[t].z = ix[2]; tri_idx[t].w = tri_prim[t];
+        // Triangle3F surface normal: normalize(cross(normalize(b-a), normalize(c-a)))
+        auto P = [&](int k) { const float* p = &in.positions[3 * k]; return std::array<float, 3>{p[0], p[1], p[2]}; };
+        auto a = P(ix[0]), b = P(ix[1]), c = P(ix[2]);
+        auto nz = [](std::array<float, 3> v) {
+            float l = (float)std::sqrt((double)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+            if (l >= 0.99999982f && l <= 1.00000012f) return v;
+            return std::array<float, 3>{v[0] / l, v[1] / l, v[2] / l};
+        };
+        auto ab = nz({b[0] - a[0], b[1] - a[1], b[2] - a[2]});
+        auto ac = nz({c[0] - a[0], c[1] - a[1], c[2] - a[2]});
+        auto n = nz({ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]});
+        tri_ng[t] = f4(n[0], n[1], n[2], 0.0f);
+        if (in.normals.empty()) {
+            // flat: every vertex of the triangle carries the face normal
+        }
+    }
+    if (!in.normals.empty()) {
+        for (int64_t v = 0; v < nv; v++) vnrm[v] = f4(in.normals[3 * v], in.normals[3 * v + 1], in.normals[3 * v + 2], 0.0f);
+    } else {
+        // no vertex normals: per-vertex normal = sum of adjacent face normals (meshes given without normals
+        // are expected to have unshared vertices, which makes this the face normal)
+        std::vector<double> acc(3 * nv, 0.0);
+        for (int64_t t = 0; t < nt; t++)
+            for (int k = 0; k < 3; k++) {
+                int v = in.indices[3 * t + k];
+                acc[3 * v] += tri_ng[t].x; acc[3 * v + 1] += tri_ng[t].y; acc[3 * v + 2] += tri_ng[t].z;
+            }
+        for (int64_t v = 0; v < nv; v++) {
+            double l = std::sqrt(acc[3 * v] * acc[3 * v] + acc[3 * v + 1] * acc[3 * v + 1] + acc[3 * v + 2] * acc[3 * v + 2]);
+            if (l == 0) l = 1;
+            vnrm[v] = f4((float)(acc[3 * v] / l), (float)(acc[3 * v + 1] / l), (float)(acc[3 * v + 2] / l), 0.0f);
+        }
+    }
+    for (int64_t i = 0; i < nt; i++) {
+        uint32_t t = bvh.tri_order[i];
+        const int32_t* ix = &in.indices[3 * (int64_t)t];
+        const float* a = &in.positions[3 * (int64_t)ix[0]];
+        const float* b = &in.positions[3 * (int64_t)ix[1]];
+        const float* c = &in.positions[3 * (int64_t)ix[2]];
+        float tid, pid;
+        int32_t ti = (int32_t)t, pi = tri_prim[t];
+        std::memcpy(&tid, &ti, 4); std::memcpy(&pid, &pi, 4);
+        tris[3 * i + 0] = f4(a[0], a[1], a[2], tid);
+        tris[3 * i + 1] = f4(b[0], b[1], b[2], pid);
+        tris[3 * i + 2] = f4(c[0], c[1], c[2], 0.0f);
+    }
+    std::vector<DevPrim> prims(in.prims.size());
+    std::vector<int32_t> analytic;
+    for (size_t p = 0; p < in.prims.size(); p++) {
+        const dfr_primitive& pr = in.prims[p];
+        DevPrim& d = prims[p];
+        std::memset(&d, 0, sizeof(d));
+        d.shape = pr.shape; d.material = pr.material; d.area_light = pr.area_light;
+        std::memcpy(d.data, pr.data, sizeof(d.data));
+        if (pr.shape != DFR_SHAPE_TRIANGLE_MESH) analytic.push_back((int32_t)p);
+        if (pr.shape == DFR_SHAPE_RECTANGLE) {
+            auto nz = [](float* v) {
+                float l = (float)std::sqrt((double)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+                if (l >= 0.99999982f && l <= 1.00000012f) return;
+                v[0] /= l; v[1] /= l; v[2] /= l;
+            };
+            float ab[3] = {pr.data[3] - pr.data[0], pr.data[4] - pr.data[1], pr.data[5] - pr.data[2]};
+            float ac[3] = {pr.data[6] - pr.data[0], pr.data[7] - pr.data[1], pr.data[8] - pr.data[2]};
+            nz(ab); nz(ac);
+            float n[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
+            nz(n);
+            d.normal[0] = n[0]; d.normal[1] = n[1]; d.normal[2] = n[2];
+        }
+    }
+    std::vector<DevMaterial> mats(in.mats.size());
+    std::vector<DevLobe> lobes;
+    for (size_t m = 0; m < in.mats.size(); m++) compile_material(in.mats[m], mats[m], lobes);
+    std::vector<DevLight> lights(in.lights.size());
+    std::vector<DevPerez> perez;
+    for (size_t l = 0; l < in.lights.size(); l++) {
+        const dfr_light& L = in.lights[l];
+        DevLight& d = lights[l];
+        std::memset(&d, 0, sizeof(d));
+        d.type = L.type; d.prim = L.primitive; d.tri = (int32_t)L.triangle; d.two_sided = L.two_sided;
+        std::memcpy(d.radiance, L.radiance, 12);
+        std::memcpy(d.position, L.position, 12);
+        d.perez = -1;
+        if (L.type == DFR_LIGHT_DIFFUSE_AREA) {
+            if (L.triangle < 0 || L.triangle >= nt) { err = "area light needs a triangle"; return DFR_E_UNSUPPORTED; }
+            const int32_t* ix = &in.indices[3 * L.triangle];
+            const float* a = &in.positions[3 * (int64_t)ix[0]];
+            const float* b = &in.positions[3 * (int64_t)ix[1]];
+            const float* c = &in.positions[3 * (int64_t)ix[2]];
+            float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+            float x = e1[1] * e2[2] - e1[2] * e2[1], y = e1[2] * e2[0] - e1[0] * e2[2], z = e1[0] * e2[1] - e1[1] * e2[0];
+            d.area = 0.5f * (float)std::sqrt((double)(x * x + y * y + z * z));
+            d.prim = tri_prim[L.triangle];
+        } else if (L.type == DFR_LIGHT_PEREZ) {
+            DevPerez p;
+            perez_build(p, L.turbidity, L.direction);
+            d.perez = (int32_t)perez.size();
+            perez.push_back(p);
+        }
+    }
+    int rc;
+    if ((rc = upload(out.nodes, bvh.nodes, s))) return rc;
+    if ((rc = upload(out.tris, tris, s))) return rc;
+    if ((rc = upload(out.tri_idx, tri_idx, s))) return rc;
+    if ((rc = upload(out.vpos, vpos, s))) return rc;
+    if ((rc = upload(out.vnrm, vnrm, s))) return rc;
+    if ((rc = upload(out.tri_ng, tri_ng, s))) return rc;
+    if ((rc = upload(out.prims, prims, s))) return rc;
+    if ((rc = upload(out.analytic, analytic, s))) return rc;
+    if ((rc = upload(out.mats, mats, s))) return rc;
+    if ((rc = upload(out.lobes, lobes, s))) return rc;
+    if ((rc = upload(out.lights, lights, s))) return rc;
+    if ((rc = upload(out.perez, perez, s))) return rc;
+    DevScene& ds = out.dev;
+    ds.nodes = (const float4*)out.nodes.ptr;
+    ds.tris = (const float4*)out.tris.ptr;
+    ds.tri_idx = (const int4*)out.tri_idx.ptr;
+    ds.vpos = (const float4*)out.vpos.ptr;
+    ds.vnrm = (const float4*)out.vnrm.ptr;
+    ds.tri_ng = (const float4*)out.tri_ng.ptr;
+    ds.prims = (const DevPrim*)out.prims.ptr;
+    ds.analytic = (const int32_t*)out.analytic.ptr;
+    ds.mats = (const DevMaterial*)out.mats.ptr;
+    ds.lobes = (const DevLobe*)out.lobes.ptr;
+    ds.lights = (const DevLight*)out.lights.ptr;
+    ds.perez = (const DevPerez*)out.perez.ptr;
+    ds.n_analytic = (int32_t)analytic.size();
+    ds.n_lights = (int32_t)lights.size();
+    ds.has_bvh = nt > 0 ? 1 : 0;
+    ds.light_pdf_mode = light_pdf_mode;
+    std::memcpy(out.camera, in.camera, sizeof(out.camera));
+    out.bytes = out.nodes.bytes + out.tris.bytes + out.tri_idx.bytes + out.vpos.bytes + out.vnrm.bytes +
+                out.tri_ng.bytes + out.prims.bytes + out.analytic.bytes + out.mats.bytes + out.lobes.bytes +
+                out.lights.bytes + out.perez.bytes;
+    if (hipStreamSynchronize(s) != hipSuccess) { err = "upload failed"; return DFR_E_DEVICE; }
+    return DFR_OK;
+}
+
+int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err) {
+    if (!d) { err = "scene == null"; return DFR_E_NULL; }
+    if (d->n_vertices < 0 || d->n_triangles < 0 || d->n_primitives < 0 || d->n_materials < 0 || d->n_lights < 0) {
+        err = "negative count"; return DFR_E_ARGUMENT;
+    }
+    if ((d->n_vertices && !d->positions) || (d->n_triangles && !d->indices) || (d->n_primitives && !d->primitives) ||
+        (d->n_materials && !d->materials) || (d->n_lights && !d->lights)) {
+        err = "null array"; return DFR_E_NULL;
+    }
+    h.positions.assign(d->positions, d->positions + 3 * d->n_vertices);
+    if (d->normals) h.normals.assign(d->normals, d->normals + 3 * d->n_vertices); else h.normals.clear();
+    h.indices.assign(d->indices, d->indices + 3 * d->n_triangles);
+    h.prims.assign(d->primitives, d->primitives + d->n_primitives);
+    h.mats.assign(d->materials, d->materials + d->n_materials);
+    h.lights.assign(d->lights, d->lights + d->n_lights);
+    std::memcpy(h.camera, d->camera, sizeof(h.camera));
+    return DFR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// context helpers
+// ---------------------------------------------------------------------------
+int Ctx::fail(int code, const std::string& msg) {
+    error = msg;
+    return code;
+}
+
+int Ctx::hip_fail(hipError_t e, const char* what) {
+    error = std::string(what) + ": " + hipGetErrorString(e);
+    return DFR_E_DEVICE;
+}
+
+int Ctx::sync_scene_table() {
+    std::vector<DevScene> table(scenes.size());
+    for (size_t i = 0; i < scenes.size(); i++)
+        if (scenes[i]) table[i] = scenes[i]->dev;
+    if (scene_table.alloc(std::max<size_t>(1, table.size()) * sizeof(DevScene)) != hipSuccess) return fail(DFR_E_MEMORY, "scene table");
+    hipError_t e = hipMemcpyAsync(scene_table.ptr, table.data(), table.size() * sizeof(DevScene), hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return hip_fail(e, "scene table upload");
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "scene table upload");
+    return DFR_OK;
+}
+
+int Ctx::max_depth() const {
+    int d = 1;
+    for (auto& s : scenes) if (s) d = std::max(d, s->max_depth);
+    return d;
+}
+
+int Ctx::ensure_overflow(int64_t threads) {
+    size_t need = (size_t)overflow_entries_per_thread(max_depth()) * (size_t)threads * sizeof(int);
+    if (overflow.bytes >= need) return DFR_OK;
+    if (overflow.alloc(need) != hipSuccess) return fail(DFR_E_MEMORY, "traversal overflow stack");
+    return DFR_OK;
+}
+
+int Ctx::ensure_film(int64_t pixels) {
+    size_t need = (size_t)pixels * sizeof(float4);
+    if (film.bytes < need && film.alloc(need) != hipSuccess) return fail(DFR_E_MEMORY, "film");
+    return DFR_OK;
+}
+
+int Ctx::ensure_wavefront(int capacity) {
+    if (wf_capacity == capacity && wf.ctr) return DFR_OK;
+    const size_t c = (size_t)capacity;
+    bool ok = true;
+    auto A = [&](DeviceBuffer& b, size_t bytes) { ok = ok && b.alloc(bytes) == hipSuccess; return b.ptr; };
+    wf.a.o = (float4*)A(wf_mem[0], c * 16); wf.a.d = (float4*)A(wf_mem[1], c * 16);
+    wf.a.beta = (float4*)A(wf_mem[2], c * 16); wf.a.info = (uint4*)A(wf_mem[3], c * 16);
+    wf.b.o = (float4*)A(wf_mem[4], c * 16); wf.b.d = (float4*)A(wf_mem[5], c * 16);
+    wf.b.beta = (float4*)A(wf_mem[6], c * 16); wf.b.info = (uint4*)A(wf_mem[7], c * 16);
+    wf.hit = (float4*)A(wf_mem[8], c * 16);
+    wf.perm = (int32_t*)A(wf_mem[9], c * 4);
+    wf.keys = (int32_t*)A(wf_mem[10], c * 4);
+    wf.sh.o = (float4*)A(wf_mem[11], 2 * c * 16); wf.sh.d = (float4*)A(wf_mem[12], 2 * c * 16);
+    wf.sh.c = (float4*)A(wf_mem[13], 2 * c * 16); wf.sh.aux = (int4*)A(wf_mem[14], 2 * c * 16);
+    wf.ctr = (Counters*)A(wf_mem[15], sizeof(Counters));
+    if (!ok) { wf.ctr = nullptr; return fail(DFR_E_MEMORY, "wavefront buffers"); }
+    wf.capacity = capacity;
+    wf_capacity = capacity;
+    return DFR_OK;
+}
+
+size_t wavefront_bytes(int capacity) { return (size_t)capacity * (9 * 16 + 8 + 2 * 4 * 16) + sizeof(Counters); }
+
+// Runs `jobs` (already on the device in job_buf) to completion through the
+// wavefront engine.  Counters are polled every `poll` iterations.
+int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counters* out_ctr) {
+    int rc = ensure_wavefront(wf_params.capacity);
+    if (rc) return rc;
+    int64_t threads = std::max<int64_t>(2 * (int64_t)wf_params.capacity, 1);
+    if ((rc = ensure_overflow(threads))) return rc;
+    wf.overflow = (int*)overflow.ptr;
+    wavefront_init(wf, (const DevJob*)job_buf.ptr, n_jobs, work_total, stream);
+    WavefrontParams p{wf_params.block, wf_params.sort_keys, (wf_params.capacity + 255) / 256};
+    Counters host{};
+    const int poll = 4;
+    const int max_iterations = 1 << 20;  // far above max_bounce * (work / capacity + 1)
+    for (int it = 0;; it++) {
+        if (it >= max_iterations) return fail(DFR_E_DEVICE, "wavefront did not drain");
+        wavefront_iteration((const DevScene*)scene_table.ptr, (const DevJob*)job_buf.ptr, n_jobs, film_ptr, wf, p, stream);
+        if ((it + 1) % poll == 0) {
+            hipError_t e = hipMemcpyAsync(&host, wf.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) return hip_fail(e, "wavefront counters");
+            // after an iteration, the paths of the next queue wait in n_next
+            if (host.n_next == 0 && host.work_next >= host.work_total) break;
+            if (stop.load()) return fail(DFR_E_STOPPED, "render stopped");
+        }
+    }
+    if (out_ctr) *out_ctr = host;
+    return DFR_OK;
+}
+
+int Ctx::upload_jobs(const std::vector<DevJob>& jobs) {
+    if (job_buf.alloc(std::max<size_t>(1, jobs.size()) * sizeof(DevJob)) != hipSuccess) return fail(DFR_E_MEMORY, "job table");
+    hipError_t e = hipMemcpyAsync(job_buf.ptr, jobs.data(), jobs.size() * sizeof(DevJob), hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return hip_fail(e, "job upload");
+    return DFR_OK;
+}
+
+DevJob Ctx::main_job(int spp, int sample_base) const {
+    DevJob j{};
+    j.seed = seed;
+    j.width = width; j.height = height;
+    j.spp = spp; j.sample_base = sample_base;
+    j.max_bounce = max_bounce; j.min_rr = min_rr;
+    j.algorithm = algorithm; j.ao_samples = ao_samples;
+    j.max_distance = max_distance;
+    j.film_offset = 0;
+    j.work_begin = 0;
+    j.work_count = (int64_t)width * height * spp;
+    std::memcpy(j.cam, camera, sizeof(j.cam));
+    j.cam[DFR_CAMERA_RES_X] = (float)width;
+    j.cam[DFR_CAMERA_RES_Y] = (float)height;
+    j.scene = main_scene;
+    return j;
+}
+
+}  // namespace dfr
+
+// ===========================================================================
+// extern "C" entry points
+// ===========================================================================
+#define CTX_GUARD(ctx)                                  \
+    if (!(ctx)) return DFR_E_NULL;                      \
+    std::lock_guard<std::mutex> _lock((ctx)->mutex);     \
+    hipSetDevice((ctx)->device)
+
+extern "C" {
+
+dfr_ctx* dfr_create(int32_t device) {
+    int dev = device;
+    if (dev < 0 && hipGetDevice(&dev) != hipSuccess) return nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || dev >= count) return nullptr;
+    if (hipSetDevice(dev) != hipSuccess) return nullptr;
+    Ctx* c = new Ctx();
+    c->device = dev;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return nullptr; }
+    hipEventCreate(&c->ev0);
+    hipEventCreate(&c->ev1);
+    return reinterpret_cast<dfr_ctx*>(c);
+}
+
+void dfr_destroy(dfr_ctx* h) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    if (!c) return;
+    {
+        std::lock_guard<std::mutex> lock(c->mutex);
+        hipSetDevice(c->device);
+        hipStreamSynchronize(c->stream);
+    }
+    hipSetDevice(c->device);
+    c->scenes.clear();
+    hipEventDestroy(c->ev0);
+    hipEventDestroy(c->ev1);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* dfr_last_error(const dfr_ctx* h) {
+    const Ctx* c = reinterpret_cast<const Ctx*>(h);
+    return c ? c->error.c_str() : "context == null";
+}
+
+int32_t dfr_set_scene(dfr_ctx* h, const dfr_scene_desc* scene) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (!scene) return c->fail(DFR_E_NULL, "scene == null");
+    int rc = copy_scene_desc(scene, c->scene_host, c->error);
+    if (rc) return rc;
+    c->has_scene = true;
+    std::memcpy(c->camera, scene->camera, sizeof(c->camera));
+    c->scene_dirty = true;
+    return DFR_OK;
+}
+
+int32_t dfr_set_resolution(dfr_ctx* h, int32_t w, int32_t hh) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (w < 1 || hh < 1) return c->fail(DFR_E_ARGUMENT, "resolution must be positive");
+    if (w != c->width || hh != c->height) { c->width = w; c->height = hh; c->film_dirty = true; }
+    return DFR_OK;
+}
+
+int32_t dfr_set_i32(dfr_ctx* h, int32_t key, int32_t v) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    switch (key) {
+        case DFR_KEY_MAXIMUM_BOUNCE: c->max_bounce = v; break;
+        case DFR_KEY_MINIMUM_BOUNCE_RR: c->min_rr = v; break;
+        case DFR_KEY_RENDERING_ALGORITHM:
+            if (v < 0 || v > 4) return c->fail(DFR_E_ARGUMENT, "rendering algorithm ordinal out of range");
+            c->algorithm = v; break;
+        case DFR_KEY_SAMPLES: c->ao_samples = v; break;
+        case DFR_KEY_TONE_MAPPER: c->tone_mapper = v; break;
+        case DFR_KEY_PREVIEW_MODE: c->preview = v; break;
+        case DFR_KEY_SEED: c->seed = (c->seed & 0xffffffff00000000ull) | (uint32_t)v; break;
+        case DFR_KEY_SEED_HI: c->seed = (c->seed & 0xffffffffull) | ((uint64_t)(uint32_t)v << 32); break;
+        case DFR_KEY_KERNEL: c->kernel = v; break;
+        case DFR_KEY_LIGHT_PDF_MODE:
+            if (v < 0 || v > 1) return c->fail(DFR_E_ARGUMENT, "light pdf mode is 0 or 1");
+            if (v != c->light_pdf_mode) { c->light_pdf_mode = v; c->scene_dirty = true; }
+            break;
+        case DFR_KEY_SPP_PER_PASS:
+            if (v < 1) return c->fail(DFR_E_ARGUMENT, "spp per pass must be >= 1");
+            c->spp_per_pass = v; break;
+        default: return c->fail(DFR_E_ARGUMENT, "unknown integer key");
+    }
+    return DFR_OK;
+}
+
+int32_t dfr_get_i32(const dfr_ctx* h, int32_t key, int32_t* v) {
+    const Ctx* c = reinterpret_cast<const Ctx*>(h);
+    if (!c || !v) return DFR_E_NULL;
+    switch (key) {
+        case DFR_KEY_MAXIMUM_BOUNCE: *v = c->max_bounce; break;
+        case DFR_KEY_MINIMUM_BOUNCE_RR: *v = c->min_rr; break;
+        case DFR_KEY_RENDERING_ALGORITHM: *v = c->algorithm; break;
+        case DFR_KEY_SAMPLES: *v = c->ao_samples; break;
+        case DFR_KEY_TONE_MAPPER: *v = c->tone_mapper; break;
+        case DFR_KEY_PREVIEW_MODE: *v = c->preview; break;
+        case DFR_KEY_SEED: *v = (int32_t)(uint32_t)c->seed; break;
+        case DFR_KEY_SEED_HI: *v = (int32_t)(uint32_t)(c->seed >> 32); break;
+        case DFR_KEY_KERNEL: *v = c->kernel; break;
+        case DFR_KEY_LIGHT_PDF_MODE: *v = c->light_pdf_mode; break;
+        case DFR_KEY_SPP_PER_PASS: *v = c->spp_per_pass; break;
+        default: return DFR_E_ARGUMENT;
+    }
+    return DFR_OK;
+}
+
+int32_t dfr_set_f32(dfr_ctx* h, int32_t key, float v) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (key != DFR_KEY_MAXIMUM_DISTANCE) return c->fail(DFR_E_ARGUMENT, "unknown float key");
+    c->max_distance = v;
+    return DFR_OK;
+}
+
+int32_t dfr_get_f32(const dfr_ctx* h, int32_t key, float* v) {
+    const Ctx* c = reinterpret_cast<const Ctx*>(h);
+    if (!c || !v) return DFR_E_NULL;
+    if (key != DFR_KEY_MAXIMUM_DISTANCE) return DFR_E_ARGUMENT;
+    *v = c->max_distance;
+    return DFR_OK;
+}
+
+static int32_t setup_impl(Ctx* c, int32_t compile_scene_flag) {
+    if (compile_scene_flag || c->scene_dirty) {
+        if (!c->has_scene) return c->fail(DFR_E_STATE, "setScene before setup");
+        auto cs = std::make_unique<CompiledScene>();
+        int rc = compile_scene(c->scene_host, *cs, c->light_pdf_mode, c->stream, c->error);
+        if (rc) return rc;
+        if (c->main_scene < 0) {
+            c->main_scene = (int)c->scenes.size();
+            c->scenes.push_back(std::move(cs));
+        } else {
+            c->scenes[c->main_scene] = std::move(cs);
+        }
+        if ((rc = c->sync_scene_table())) return rc;
+        c->scene_dirty = false;
+    }
+    int rc = c->ensure_film((int64_t)c->width * c->height);
+    if (rc) return rc;
+    hipError_t e = hipMemsetAsync(c->film.ptr, 0, (size_t)c->width * c->height * sizeof(float4), c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "film clear");
+    c->render_pass = 0;
+    c->clearing = false;
+    c->film_dirty = false;
+    c->is_setup = true;
+    return DFR_OK;
+}
+
+int32_t dfr_setup(dfr_ctx* h, int32_t compile_scene_flag) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    return setup_impl(c, compile_scene_flag);
+}
+
+int32_t dfr_render(dfr_ctx* h, int32_t passes) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (passes < 1) return c->fail(DFR_E_ARGUMENT, "passes must be >= 1");
+    if (!c->is_setup || c->main_scene < 0) return c->fail(DFR_E_STATE, "setup before render");
+    if (c->scene_dirty || c->film_dirty) {
+        int rc = setup_impl(c, 0);
+        if (rc) return rc;
+    }
+    const int64_t npix = (int64_t)c->width * c->height;
+    hipError_t e;
+    if (c->clearing) {
+        e = hipMemsetAsync(c->film.ptr, 0, (size_t)npix * sizeof(float4), c->stream);
+        if (e != hipSuccess) return c->hip_fail(e, "film clear");
+        c->render_pass = 0;
+        c->clearing = false;
+    }
+    c->stop.store(false);
+    const int spp = passes * c->spp_per_pass;
+    DevJob job = c->main_job(spp, c->render_pass * c->spp_per_pass);
+    int rc = c->upload_jobs({job});
+    if (rc) return rc;
+    if (c->counters.alloc(64) != hipSuccess) return c->fail(DFR_E_MEMORY, "counters");
+    hipMemsetAsync(c->counters.ptr, 0, 64, c->stream);
+    hipEventRecord(c->ev0, c->stream);
+    dfr_stats st{};
+    bool plain = c->kernel == 1 || c->algorithm != DFR_ALGO_PATH_TRACING;
+    if (plain) {
+        // one launch per pass: each launch adds spp_per_pass samples to every pixel
+        int64_t threads = ((npix + 255) / 256) * 256;
+        if ((rc = c->ensure_overflow(threads))) return rc;
+        for (int p = 0; p < passes; p++) {
+            DevJob jp = c->main_job(c->spp_per_pass, (c->render_pass + p) * c->spp_per_pass);
+            e = hipMemcpyAsync(c->job_buf.ptr, &jp, sizeof(DevJob), hipMemcpyHostToDevice, c->stream);
+            if (e != hipSuccess) return c->hip_fail(e, "job upload");
+            launch_megakernel((const DevScene*)c->scene_table.ptr, (const DevJob*)c->job_buf.ptr, (float4*)c->film.ptr,
+                              (int*)c->overflow.ptr, (int)npix, 256, (int64_t*)c->counters.ptr, c->stream);
+            if (c->stop.load()) break;
+        }
+        hipEventRecord(c->ev1, c->stream);
+        e = hipEventSynchronize(c->ev1);
+        if (e != hipSuccess) return c->hip_fail(e, "megakernel");
+        int64_t cnt[2];
+        hipMemcpy(cnt, c->counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost);
+        st.rays_closest = cnt[0]; st.rays_any = cnt[1];
+    } else {
+        Counters out{};
+        rc = c->run_wavefront(1, job.work_count, (float4*)c->film.ptr, &out);
+        if (rc) return rc;
+        hipEventRecord(c->ev1, c->stream);
+        e = hipEventSynchronize(c->ev1);
+        if (e != hipSuccess) return c->hip_fail(e, "wavefront");
+        st.rays_closest = out.rays_closest; st.rays_any = out.rays_any; st.bounces = out.bounces;
+    }
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    st.seconds = ms * 1e-3;
+    st.path_samples = npix * spp;
+    c->stats = st;
+    c->render_pass += passes;
+    return DFR_OK;
+}
+
+int32_t dfr_clear(dfr_ctx* h) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    c->clearing = true;
+    return DFR_OK;
+}
+
+int32_t dfr_get_render_pass(const dfr_ctx* h) {
+    const Ctx* c = reinterpret_cast<const Ctx*>(h);
+    return c ? c->render_pass : DFR_E_NULL;
+}
+
+int32_t dfr_request_stop(dfr_ctx* h) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    if (!c) return DFR_E_NULL;
+    c->stop.store(true);
+    return DFR_OK;
+}
+
+int32_t dfr_update_camera(dfr_ctx* h, const float* cam) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (!cam) return c->fail(DFR_E_NULL, "camera == null");
+    std::memcpy(c->camera, cam, sizeof(c->camera));
+    return DFR_OK;
+}
+
+int32_t dfr_read_film(dfr_ctx* h, float* rgb, int32_t* count) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (!c->is_setup) return c->fail(DFR_E_STATE, "setup before read");
+    const int64_t n = (int64_t)c->width * c->height;
+    std::vector<float4> f(n);
+    hipError_t e = hipMemcpyAsync(f.data(), c->film.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "film readback");
+    for (int64_t i = 0; i < n; i++) {
+        float w = f[i].w;
+        float inv = w > 0.0f ? 1.0f / w : 0.0f;
+        if (rgb) { rgb[3 * i] = f[i].x * inv; rgb[3 * i + 1] = f[i].y * inv; rgb[3 * i + 2] = f[i].z * inv; }
+        if (count) count[i] = (int32_t)w;
+    }
+    return DFR_OK;
+}
+
+int32_t dfr_read_rgba8(dfr_ctx* h, uint8_t* rgba) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (!rgba) return c->fail(DFR_E_NULL, "rgba == null");
+    if (!c->is_setup) return c->fail(DFR_E_STATE, "setup before read");
+    const int64_t n = (int64_t)c->width * c->height;
+    DeviceBuffer tmp;
+    if (tmp.alloc(n * 4) != hipSuccess) return c->fail(DFR_E_MEMORY, "rgba8");
+    launch_rgba8((const float4*)c->film.ptr, (uint8_t*)tmp.ptr, (int)n, c->tone_mapper, c->stream);
+    hipError_t e = hipMemcpyAsync(rgba, tmp.ptr, n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "rgba8 readback");
+    return DFR_OK;
+}
+
+int32_t dfr_get_stats(const dfr_ctx* h, dfr_stats* st) {
+    const Ctx* c = reinterpret_cast<const Ctx*>(h);
+    if (!c || !st) return DFR_E_NULL;
+    *st = c->stats;
+    return DFR_OK;
+}
+
+int32_t dfr_trace_rays(dfr_ctx* h, const float* rays, int64_t n, int32_t any_hit, int32_t* prim, int64_t* tri, float* t,
+                       int32_t device_ptrs) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (n < 0) return c->fail(DFR_E_ARGUMENT, "n < 0");
+    if (n == 0) return DFR_OK;
+    if (!rays || !prim || !tri || !t) return c->fail(DFR_E_NULL, "null pointer");
+    if (c->main_scene < 0) return c->fail(DFR_E_STATE, "setup before trace");
+    const int grid = std::min<int64_t>((n + 255) / 256, 256 * 64);
+    int rc = c->ensure_overflow((int64_t)grid * 256);
+    if (rc) return rc;
+    const DevScene* sc = (const DevScene*)c->scene_table.ptr + c->main_scene;
+    hipError_t e;
+    hipEventRecord(c->ev0, c->stream);
+    if (device_ptrs) {
+        launch_trace_rays(sc, (const float4*)rays, n, any_hit, prim, tri, t, (int*)c->overflow.ptr, 256, grid, c->stream);
+        hipEventRecord(c->ev1, c->stream);
+        e = hipEventSynchronize(c->ev1);
+        if (e != hipSuccess) return c->hip_fail(e, "trace");
+    } else {
+        DeviceBuffer r, p, tr, tt;
+        if (r.alloc(n * 32) || p.alloc(n * 4) || tr.alloc(n * 8) || tt.alloc(n * 4)) return c->fail(DFR_E_MEMORY, "ray buffers");
+        e = hipMemcpyAsync(r.ptr, rays, n * 32, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) return c->hip_fail(e, "ray upload");
+        hipEventRecord(c->ev0, c->stream);
+        launch_trace_rays(sc, (const float4*)r.ptr, n, any_hit, (int32_t*)p.ptr, (int64_t*)tr.ptr, (float*)tt.ptr,
+                          (int*)c->overflow.ptr, 256, grid, c->stream);
+        hipEventRecord(c->ev1, c->stream);
+        hipMemcpyAsync(prim, p.ptr, n * 4, hipMemcpyDeviceToHost, c->stream);
+        hipMemcpyAsync(tri, tr.ptr, n * 8, hipMemcpyDeviceToHost, c->stream);
+        hipMemcpyAsync(t, tt.ptr, n * 4, hipMemcpyDeviceToHost, c->stream);
+        e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return c->hip_fail(e, "trace");
+    }
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->stats = dfr_stats{};
+    c->stats.seconds = ms * 1e-3;
+    if (any_hit) c->stats.rays_any = n; else c->stats.rays_closest = n;
+    return DFR_OK;
+}
+
+int32_t dfr_cast_primary(dfr_ctx* h, int32_t sample_index, float* rays, int32_t* prim, int64_t* tri, float* t) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (!rays || !prim || !tri || !t) return c->fail(DFR_E_NULL, "null pointer");
+    if (c->main_scene < 0 || !c->is_setup) return c->fail(DFR_E_STATE, "setup before cast");
+    const int64_t n = (int64_t)c->width * c->height;
+    int rc = c->ensure_overflow(((n + 255) / 256) * 256);
+    if (rc) return rc;
+    DevJob job = c->main_job(1, sample_index);
+    if ((rc = c->upload_jobs({job}))) return rc;
+    DeviceBuffer r, p, tr, tt;
+    if (r.alloc(n * 32) || p.alloc(n * 4) || tr.alloc(n * 8) || tt.alloc(n * 4)) return c->fail(DFR_E_MEMORY, "ray buffers");
+    launch_cast_primary((const DevScene*)c->scene_table.ptr + c->main_scene, (const DevJob*)c->job_buf.ptr, (float*)r.ptr,
+                        (int32_t*)p.ptr, (int64_t*)tr.ptr, (float*)tt.ptr, (int*)c->overflow.ptr, (int)n, c->stream);
+    hipMemcpyAsync(rays, r.ptr, n * 32, hipMemcpyDeviceToHost, c->stream);
+    hipMemcpyAsync(prim, p.ptr, n * 4, hipMemcpyDeviceToHost, c->stream);
+    hipMemcpyAsync(tri, tr.ptr, n * 8, hipMemcpyDeviceToHost, c->stream);
+    hipMemcpyAsync(t, tt.ptr, n * 4, hipMemcpyDeviceToHost, c->stream);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "cast primary");
+    return DFR_OK;
+}
+
+}  // extern "C"

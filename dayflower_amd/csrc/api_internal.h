@@ -1,0 +1,124 @@
+// api_internal.h — host-side state behind the C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <atomic>
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dayflower_hip.h"
+#include "device.h"
+#include "kernels.h"
+
+namespace dfr {
+
+// Owning device allocation.
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    void release() {
+        if (ptr) hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    // (Re)allocates when the current block is smaller than `n`; returns
+    // hipSuccess or the allocation error.
+    hipError_t alloc(size_t n) {
+        if (n <= bytes && ptr) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&ptr, n);
+        if (e != hipSuccess) { ptr = nullptr; return e; }
+        bytes = n;
+        return hipSuccess;
+    }
+};
+
+// Host copy of a dfr_scene_desc (setScene copies; setup compiles).
+struct SceneHost {
+    std::vector<float> positions, normals;
+    std::vector<int32_t> indices;
+    std::vector<dfr_primitive> prims;
+    std::vector<dfr_material> mats;
+    std::vector<dfr_light> lights;
+    float camera[DFR_CAMERA_LENGTH];
+};
+
+// A scene compiled into SoA device buffers, resident in HBM.
+struct CompiledScene {
+    DeviceBuffer nodes, tris, tri_idx, vpos, vnrm, tri_ng, prims, analytic, mats, lobes, lights, perez;
+    DevScene dev{};
+    int max_depth = 1;
+    int64_t n_nodes = 0, n_tris = 0;
+    double sah_cost = 0.0, build_seconds = 0.0;
+    size_t bytes = 0;
+    float camera[DFR_CAMERA_LENGTH];
+};
+
+int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err);
+int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, hipStream_t s, std::string& err);
+size_t wavefront_bytes(int capacity);
+
+struct WavefrontConfig {
+    int block = 256;
+    int capacity = 1 << 20;  // path pool size
+    int sort_keys = 8;       // material-sort keys (0 = unsorted)
+};
+
+struct Ctx {
+    std::mutex mutex;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string error;
+    std::atomic<bool> stop{false};
+
+    // GPURenderer settings (AbstractGPURenderer.java:143-159)
+    int width = 800, height = 800;
+    int max_bounce = 20, min_rr = 5;
+    int algorithm = DFR_ALGO_PATH_TRACING;
+    int ao_samples = 10;
+    float max_distance = 20.0f;
+    int tone_mapper = 0, preview = 0;
+    uint64_t seed = 0;
+    int kernel = 0;            // 0 wavefront, 1 plain megakernel
+    int light_pdf_mode = 0;
+    int spp_per_pass = 1;
+    float camera[DFR_CAMERA_LENGTH] = {0};
+
+    // scene state
+    SceneHost scene_host;
+    bool has_scene = false, scene_dirty = false, film_dirty = true, is_setup = false, clearing = false;
+    int render_pass = 0;
+    std::vector<std::unique_ptr<CompiledScene>> scenes;
+    int main_scene = -1;
+    DeviceBuffer scene_table, job_buf, film, overflow, counters;
+
+    // wavefront engine
+    WavefrontConfig wf_params;
+    WavefrontBuffers wf{};
+    DeviceBuffer wf_mem[16];
+    int wf_capacity = 0;
+
+    dfr_stats stats{};
+
+    int fail(int code, const std::string& msg);
+    int hip_fail(hipError_t e, const char* what);
+    int sync_scene_table();
+    int max_depth() const;
+    int ensure_overflow(int64_t threads);
+    int ensure_film(int64_t pixels);
+    int ensure_wavefront(int capacity);
+    int upload_jobs(const std::vector<DevJob>& jobs);
+    int run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counters* out);
+    DevJob main_job(int spp, int sample_base) const;
+};
+
+}  // namespace dfr

@@ -1,0 +1,944 @@
+// device.h — device functions of the path-tracing hot path (gfx950).
+//
+// Semantics follow the reference CPURenderer path (M/ = src/main/java/
+// org/dayflower/ of the reference tree); the data layout is this build's:
+// a two-wide BVH over all triangles with child-pair boxes, triangles as three
+// float4 in BVH order, shading data indexed by the original triangle.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cstdint>
+
+#include "../../include/dayflower_hip.h"
+#include "device_types.h"
+
+namespace dfr {
+
+#define DEV __device__ __forceinline__
+
+// ---------------------------------------------------------------------------
+// small vector type
+// ---------------------------------------------------------------------------
+struct V {
+    float x, y, z;
+};
+DEV V mk(float x, float y, float z) { V v; v.x = x; v.y = y; v.z = z; return v; }
+DEV V operator+(V a, V b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+DEV V operator-(V a, V b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+DEV V operator-(V a) { return mk(-a.x, -a.y, -a.z); }
+DEV V operator*(V a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+DEV V operator*(V a, V b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+DEV V operator/(V a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+DEV float dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+DEV V cross(V a, V b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+DEV float comp(V v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+DEV float len(V v) { return sqrtf(dot(v, v)); }
+// Vector3F.normalize (M/geometry/Vector3F.java:1000-1011): unchanged when the
+// length is already within [nextDown^3(1), nextUp(1)].
+DEV V nrm(V v) {
+    float l = len(v);
+    if (l >= 0.99999982f && l <= 1.00000012f) return v;
+    return v / l;
+}
+DEV V ld3(const float* p) { return mk(p[0], p[1], p[2]); }
+DEV V f4v(float4 a) { return mk(a.x, a.y, a.z); }
+DEV bool black(V c) { return c.x == 0.0f && c.y == 0.0f && c.z == 0.0f; }
+DEV float maxc(V c) { return fmaxf(c.x, fmaxf(c.y, c.z)); }
+
+constexpr float PI_F = 3.14159265358979323846f;
+constexpr float INV_PI = 1.0f / 3.14159265358979323846f;
+constexpr float T_MIN = 0.001f;  // Scene.T_MINIMUM, M/scene/Scene.java:70
+
+// ---------------------------------------------------------------------------
+// RNG: stateless hash keyed by (seed, pixel, sample, dimension).  The
+// integer hash is Wellons' "lowbias32".  Dimension layout: 0,1 pixel offset;
+// path-tracing bounce b uses 2 + 12*b + {0 light choice, 1-2 light sample,
+// 3-4 NEE BSDF sample, 5 Perez sun choice, 6-7 BSDF continuation, 8 Russian
+// roulette}; ambient-occlusion sample i uses 2 + 2*i and 3 + 2*i.
+// ---------------------------------------------------------------------------
+DEV uint32_t lowbias32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+struct Rng {
+    uint32_t a, b;
+    DEV float get(uint32_t dim) const {
+        uint32_t h = lowbias32(a ^ lowbias32(b + dim * 0x9e3779b9u));
+        return (float)(h >> 8) * (1.0f / 16777216.0f);
+    }
+};
+DEV Rng make_rng(uint64_t seed, uint32_t pixel, uint32_t sample) {
+    Rng r;
+    r.a = lowbias32(pixel ^ lowbias32((uint32_t)seed));
+    r.b = lowbias32(sample ^ lowbias32((uint32_t)(seed >> 32) ^ 0x85ebca6bu) ^ r.a);
+    return r;
+}
+constexpr uint32_t DIM_BOUNCE = 2, DIMS_PER_BOUNCE = 12;
+
+// ---------------------------------------------------------------------------
+// Compiled scene
+// ---------------------------------------------------------------------------
+struct DevScene {
+    const float4* nodes;      // 4 float4 per two-wide node
+    const float4* tris;       // 3 float4 per triangle (BVH order); tris[3i].w = original id (int bits)
+    const int4* tri_idx;      // original order: vertices a, b, c and primitive
+    const float4* vpos;       // per vertex
+    const float4* vnrm;       // per vertex shading normal
+    const float4* tri_ng;     // per triangle geometric normal
+    const DevPrim* prims;
+    const int32_t* analytic;  // indices of non-mesh primitives
+    const DevMaterial* mats;
+    const DevLobe* lobes;
+    const DevLight* lights;
+    const DevPerez* perez;
+    int32_t n_analytic;
+    int32_t n_lights;
+    int32_t has_bvh;
+    int32_t light_pdf_mode;   // 0 reference (Shape3F pdf without 1/area), 1 area
+};
+
+constexpr uint32_t LEAF_FLAG_DEV = 0x80000000u;  // leaf reference bit (see bvh.h)
+
+// Per-thread traversal stack: the first SHORT entries live in LDS (entry k of
+// a thread at lds[k * bdim]); deeper entries overflow into an HBM buffer
+// (entry k at glob[(k - SHORT) * gstride]).
+template <int SHORT>
+struct TraversalStack {
+    int* lds;
+    int* glob;
+    int gstride;
+    int bdim;
+    int sp;
+    DEV void push(int v) {
+        if (sp < SHORT) lds[sp * bdim] = v; else glob[(sp - SHORT) * gstride] = v;
+        sp++;
+    }
+    DEV int pop() {
+        sp--;
+        return sp < SHORT ? lds[sp * bdim] : glob[(sp - SHORT) * gstride];
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Shapes
+// ---------------------------------------------------------------------------
+DEV float gamma_n(int n) {
+    const float e = 5.96046448e-08f;  // ulp(1)/2, M/utility/Floats.java:44
+    return (n * e) / (1.0f - n * e);
+}
+
+// Triangle3F.intersectionT (M/geometry/shape/Triangle3F.java:505-600):
+// pbrt-v3 watertight test with the FP64 recomputation of zero edge
+// functions.  The near limit is t > tmin (the reference compares t * det with
+// tmin, which hides small triangles).  Returns 0 on a miss.
+DEV float tri_t(V p0, V p1, V p2, V o, V d, int kx, int ky, int kz, float sx, float sy, float sz, float tmin,
+                float tmax) {
+    V a = p0 - o, b = p1 - o, c = p2 - o;
+    float ax = comp(a, kx), ay = comp(a, ky), az = comp(a, kz);
+    float bx = comp(b, kx), by = comp(b, ky), bz = comp(b, kz);
+    float cx = comp(c, kx), cy = comp(c, ky), cz = comp(c, kz);
+    ax = ax + sx * az; ay = ay + sy * az;
+    bx = bx + sx * bz; by = by + sy * bz;
+    cx = cx + sx * cz; cy = cy + sy * cz;
+    float e0 = bx * cy - by * cx;
+    float e1 = cx * ay - cy * ax;
+    float e2 = ax * by - ay * bx;
+    if (e0 == 0.0f || e1 == 0.0f || e2 == 0.0f) {
+        e0 = (float)((double)cy * (double)bx - (double)cx * (double)by);
+        e1 = (float)((double)ay * (double)cx - (double)ax * (double)cy);
+        e2 = (float)((double)by * (double)ax - (double)bx * (double)ay);
+    }
+    if ((e0 < 0.0f || e1 < 0.0f || e2 < 0.0f) && (e0 > 0.0f || e1 > 0.0f || e2 > 0.0f)) return 0.0f;
+    float det = e0 + e1 + e2;
+    if (det == 0.0f) return 0.0f;
+    az *= sz; bz *= sz; cz *= sz;
+    float ts = e0 * az + e1 * bz + e2 * cz;
+    // t = ts / det must lie in (tmin, tmax]: compare in scaled form
+    if (det < 0.0f && (ts >= tmin * det || ts < tmax * det)) return 0.0f;
+    if (det > 0.0f && (ts <= tmin * det || ts > tmax * det)) return 0.0f;
+    float inv = 1.0f / det;
+    float t = ts * inv;
+    float maxZ = fmaxf(fabsf(az), fmaxf(fabsf(bz), fabsf(cz)));
+    float maxX = fmaxf(fabsf(ax), fmaxf(fabsf(bx), fabsf(cx)));
+    float maxY = fmaxf(fabsf(ay), fmaxf(fabsf(by), fabsf(cy)));
+    float dz = gamma_n(3) * maxZ;
+    float dx = gamma_n(5) * (maxX + maxZ);
+    float dy = gamma_n(5) * (maxY + maxZ);
+    float de = 2.0f * (gamma_n(2) * maxX * maxY + dy * maxX + dx * maxY);
+    float maxE = fmaxf(fabsf(e0), fmaxf(fabsf(e1), fabsf(e2)));
+    float dt = 3.0f * (gamma_n(3) * maxE * maxZ + de * maxZ + dz * maxE) * fabsf(inv);
+    if (t <= dt) return 0.0f;
+    return t;
+}
+
+// Ray prepared for the watertight test and the slab test.
+struct PRay {
+    V o, d;
+    int kx, ky, kz;
+    float sx, sy, sz;
+    V inv, oinv;
+};
+DEV PRay prep_ray(V o, V d) {
+    PRay r;
+    r.o = o; r.d = d;
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    r.kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);  // Vector3F.getMaxDimension
+    r.kx = r.kz + 1; if (r.kx == 3) r.kx = 0;
+    r.ky = r.kx + 1; if (r.ky == 3) r.ky = 0;
+    float dz = comp(d, r.kz);
+    r.sx = -comp(d, r.kx) / dz;
+    r.sy = -comp(d, r.ky) / dz;
+    r.sz = 1.0f / dz;
+    auto safe = [](float v) { return fabsf(v) < 1e-20f ? copysignf(1e-20f, v) : v; };
+    r.inv = mk(1.0f / safe(d.x), 1.0f / safe(d.y), 1.0f / safe(d.z));
+    r.oinv = o * r.inv;
+    return r;
+}
+
+// EFloat interval arithmetic (M/utility/EFloat.java:34-195).
+struct EF { float v, lo, hi; };
+DEV float nup(float v) { return nextafterf(v, INFINITY); }
+DEV float ndn(float v) { return nextafterf(v, -INFINITY); }
+DEV EF ef(float v) { EF e; e.v = v; e.lo = v; e.hi = v; return e; }
+DEV EF ef_add(EF a, EF b) { EF e; e.v = a.v + b.v; e.lo = ndn(a.lo + b.lo); e.hi = nup(a.hi + b.hi); return e; }
+DEV EF ef_sub(EF a, EF b) { EF e; e.v = a.v - b.v; e.lo = ndn(a.lo - b.hi); e.hi = nup(a.hi - b.lo); return e; }
+DEV EF ef_mul(EF a, EF b) {
+    float p0 = a.lo * b.lo, p1 = a.hi * b.lo, p2 = a.lo * b.hi, p3 = a.hi * b.hi;
+    EF e; e.v = a.v * b.v;
+    e.lo = ndn(fminf(fminf(p0, p1), fminf(p2, p3)));
+    e.hi = nup(fmaxf(fmaxf(p0, p1), fmaxf(p2, p3)));
+    return e;
+}
+DEV EF ef_div(EF a, EF b) {
+    EF e; e.v = a.v / b.v;
+    if (b.lo < 0.0f && b.hi > 0.0f) { e.lo = -INFINITY; e.hi = INFINITY; return e; }
+    float d0 = a.lo / b.lo, d1 = a.hi / b.lo, d2 = a.lo / b.hi, d3 = a.hi / b.hi;
+    e.lo = ndn(fminf(fminf(d0, d1), fminf(d2, d3)));
+    e.hi = nup(fmaxf(fmaxf(d0, d1), fmaxf(d2, d3)));
+    return e;
+}
+
+// Sphere3F.intersectionT for a full sphere (M/geometry/shape/Sphere3F.java).
+DEV float sphere_t(const DevPrim& p, V o, V d, float tmin, float tmax) {
+    V c = mk(p.data[0], p.data[1], p.data[2]);
+    float radius = p.data[3];
+    V q = o - c;
+    EF ox = ef(q.x), oy = ef(q.y), oz = ef(q.z), dx = ef(d.x), dy = ef(d.y), dz = ef(d.z);
+    EF A = ef_add(ef_add(ef_mul(dx, dx), ef_mul(dy, dy)), ef_mul(dz, dz));
+    EF B = ef_mul(ef(2.0f), ef_add(ef_add(ef_mul(dx, ox), ef_mul(dy, oy)), ef_mul(dz, oz)));
+    EF C = ef_sub(ef_add(ef_add(ef_mul(ox, ox), ef_mul(oy, oy)), ef_mul(oz, oz)), ef_mul(ef(radius), ef(radius)));
+    double disc = (double)B.v * (double)B.v - 4.0 * (double)A.v * (double)C.v;
+    if (disc < 0.0) return 0.0f;
+    double root = sqrt(disc);
+    float fr = (float)root;
+    EF R; R.v = fr;
+    float err = 5.96046448e-08f * fr;
+    if (err == 0.0f) { R.lo = fr; R.hi = fr; } else { R.lo = ndn(fr - err); R.hi = nup(fr + err); }
+    EF Q = B.v < 0.0f ? ef_mul(ef(-0.5f), ef_sub(B, R)) : ef_mul(ef(-0.5f), ef_add(B, R));
+    EF t0 = ef_div(Q, A), t1 = ef_div(C, Q);
+    if (t0.v > t1.v) { EF tmp = t0; t0 = t1; t1 = tmp; }
+    if (t0.hi > tmax || t1.lo <= tmin) return 0.0f;
+    EF th = t0;
+    if (th.lo <= 0.0f) {
+        th = t1;
+        if (th.hi > tmax) return 0.0f;
+    }
+    return th.v;
+}
+
+// Rectangle3F.intersectionT
+DEV float rect_t(const DevPrim& p, V o, V d, float tmin, float tmax) {
+    V a = ld3(p.data), b = ld3(p.data + 3), c = ld3(p.data + 6);
+    V n = ld3(p.normal);
+    float dp = dot(n, d);
+    if (dp == 0.0f) return 0.0f;
+    float t = dot(a - o, n) / dp;
+    if (t <= tmin || t >= tmax) return 0.0f;
+    V ab = b - a, bc = c - b;
+    V ap = (o + d * t) - a;
+    float dab = dot(ap, nrm(ab)), dbc = dot(ap, nrm(bc));
+    if (dab >= 0.0f && dab <= len(ab) && dbc >= 0.0f && dbc <= len(bc)) return t;
+    return 0.0f;
+}
+
+// Plane3F.intersectionT
+DEV float plane_t(const DevPrim& p, V o, V d, float tmin, float tmax) {
+    V org = ld3(p.data), w = ld3(p.data + 9);
+    float dp = dot(w, d);
+    if (dp == 0.0f) return 0.0f;
+    float t = dot(org - o, w) / dp;
+    if (t > tmin && t < tmax) return t;
+    return 0.0f;
+}
+
+DEV float analytic_t(const DevPrim& p, V o, V d, float tmin, float tmax) {
+    switch (p.shape) {
+        case DFR_SHAPE_SPHERE: return sphere_t(p, o, d, tmin, tmax);
+        case DFR_SHAPE_RECTANGLE: return rect_t(p, o, d, tmin, tmax);
+        case DFR_SHAPE_PLANE: return plane_t(p, o, d, tmin, tmax);
+        default: return 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// BVH traversal: ordered two-wide walk with a short LDS stack that overflows
+// into HBM.  Closest hit shrinks tmax; any hit returns at the first hit.
+// ---------------------------------------------------------------------------
+struct HitRec {
+    float t;
+    int prim;
+    int tri;   // original triangle index or -1
+};
+
+DEV bool slab2(const float4& n0, const float4& n1, const float4& n2, const PRay& r, float tmin, float tmax,
+               float& t0, float& t1, bool& h0, bool& h1) {
+    // child 0
+    float lx0 = n0.x * r.inv.x - r.oinv.x, hx0 = n0.y * r.inv.x - r.oinv.x;
+    float ly0 = n0.z * r.inv.y - r.oinv.y, hy0 = n0.w * r.inv.y - r.oinv.y;
+    float lz0 = n2.x * r.inv.z - r.oinv.z, hz0 = n2.y * r.inv.z - r.oinv.z;
+    float near0 = fmaxf(fmaxf(fminf(lx0, hx0), fminf(ly0, hy0)), fmaxf(fminf(lz0, hz0), tmin));
+    float far0 = fminf(fminf(fmaxf(lx0, hx0), fmaxf(ly0, hy0)), fminf(fmaxf(lz0, hz0), tmax));
+    // child 1
+    float lx1 = n1.x * r.inv.x - r.oinv.x, hx1 = n1.y * r.inv.x - r.oinv.x;
+    float ly1 = n1.z * r.inv.y - r.oinv.y, hy1 = n1.w * r.inv.y - r.oinv.y;
+    float lz1 = n2.z * r.inv.z - r.oinv.z, hz1 = n2.w * r.inv.z - r.oinv.z;
+    float near1 = fmaxf(fmaxf(fminf(lx1, hx1), fminf(ly1, hy1)), fmaxf(fminf(lz1, hz1), tmin));
+    float far1 = fminf(fminf(fmaxf(lx1, hx1), fmaxf(ly1, hy1)), fminf(fmaxf(lz1, hz1), tmax));
+    // widen the far end by 1 + 2*gamma(3) so rounding never culls a grazing hit
+    h0 = near0 <= far0 * 1.00000048f;
+    h1 = near1 <= far1 * 1.00000048f;
+    t0 = near0; t1 = near1;
+    return h0 || h1;
+}
+
+template <bool ANY, int SHORT>
+DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit, TraversalStack<SHORT>& st,
+               int* node_visits = nullptr) {
+    bool found = false;
+    hit.t = tmax; hit.prim = -1; hit.tri = -1;
+    // analytic primitives (few): linear, in primitive order
+    for (int i = 0; i < s.n_analytic; i++) {
+        int pi = s.analytic[i];
+        float t = analytic_t(s.prims[pi], o, d, tmin, hit.t);
+        if (t != 0.0f && t < hit.t) {
+            hit.t = t; hit.prim = pi; hit.tri = -1; found = true;
+            if (ANY) return true;
+        }
+    }
+    if (!s.has_bvh) return found;
+    PRay r = prep_ray(o, d);
+    st.sp = 0;
+    uint32_t cur = 0;  // root node
+    for (;;) {
+        if (!(cur & LEAF_FLAG_DEV)) {
+            const float4* np = s.nodes + 4 * (size_t)cur;
+            float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+            if (node_visits) (*node_visits)++;
+            float t0, t1; bool h0, h1;
+            slab2(n0, n1, n2, r, tmin, hit.t, t0, t1, h0, h1);
+            uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
+            if (c1 == 0xffffffffu) h1 = false;
+            if (h0 && h1) {
+                bool swap = t1 < t0;
+                uint32_t near = swap ? c1 : c0, far = swap ? c0 : c1;
+                st.push((int)far);
+                cur = near;
+                continue;
+            } else if (h0) { cur = c0; continue; }
+            else if (h1) { cur = c1; continue; }
+        } else {
+            uint32_t first = (cur & 0x7fffffffu) >> 3, cnt = (cur & 7u) + 1;
+            for (uint32_t k = 0; k < cnt; k++) {
+                const float4* tp = s.tris + 3 * (size_t)(first + k);
+                float4 a = tp[0], b = tp[1], c = tp[2];
+                float t = tri_t(f4v(a), f4v(b), f4v(c), r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, tmin, hit.t);
+                if (t != 0.0f && t < hit.t) {
+                    hit.t = t; hit.tri = __float_as_int(a.w); hit.prim = __float_as_int(b.w); found = true;
+                    if (ANY) return true;
+                }
+            }
+        }
+        if (st.sp == 0) break;
+        cur = (uint32_t)st.pop();
+    }
+    return found;
+}
+
+// ---------------------------------------------------------------------------
+// Surface reconstruction (world space)
+// ---------------------------------------------------------------------------
+struct ONB { V u, v, w; };
+// M/geometry/Vector3F.java:527-545
+DEV V compute_v(V w) {
+    V n = nrm(w);
+    float ax = fabsf(n.x), ay = fabsf(n.y), az = fabsf(n.z);
+    if (ax < ay && ax < az) return nrm(mk(0.0f, n.z, -n.y));
+    if (ay < az) return nrm(mk(n.z, 0.0f, -n.x));
+    return nrm(mk(n.y, -n.x, 0.0f));
+}
+DEV ONB onb_w(V w) { ONB o; o.w = nrm(w); o.v = compute_v(w); o.u = cross(o.v, o.w); return o; }
+DEV V to_local(V v, const ONB& b) { return nrm(mk(dot(v, b.u), dot(v, b.v), dot(v, b.w))); }
+DEV V to_world(V v, const ONB& b) {
+    return nrm(mk(v.x * b.u.x + v.y * b.v.x + v.z * b.w.x, v.x * b.u.y + v.y * b.v.y + v.z * b.w.y,
+                  v.x * b.u.z + v.y * b.v.z + v.z * b.w.z));
+}
+
+struct Surf {
+    V p;
+    ONB g, s;
+    int prim;
+    int tri;
+};
+
+DEV Surf make_surface(const DevScene& sc, V o, V d, const HitRec& h) {
+    Surf s;
+    s.prim = h.prim; s.tri = h.tri;
+    s.p = o + d * h.t;
+    const DevPrim& p = sc.prims[h.prim];
+    if (h.tri >= 0) {
+        int4 ix = sc.tri_idx[h.tri];
+        V a = f4v(sc.vpos[ix.x]), b = f4v(sc.vpos[ix.y]), c = f4v(sc.vpos[ix.z]);
+        V eab = b - a, eca = a - c;
+        V d1 = cross(eab, eca);
+        float rdet = 1.0f / dot(d, d1);
+        V d3 = cross(a - o, d);
+        float u = dot(d3, eca) * rdet, v = dot(d3, eab) * rdet, w = 1.0f - u - v;
+        s.g = onb_w(f4v(sc.tri_ng[h.tri]));
+        V ns = nrm(f4v(sc.vnrm[ix.x]) * w + f4v(sc.vnrm[ix.y]) * u + f4v(sc.vnrm[ix.z]) * v);
+        s.s = onb_w(ns);  // texture coordinates are absent: UV determinant 0
+    } else if (p.shape == DFR_SHAPE_SPHERE) {
+        V c = ld3(p.data);
+        float radius = p.data[3];
+        V ph = (o - c) + d * h.t;
+        ph = ph * (radius / len(ph));
+        if (ph.x == 0.0f && ph.y == 0.0f) ph.x = 1.0e-5f * radius;
+        float theta = acosf(fminf(fmaxf(ph.z / radius, -1.0f), 1.0f));
+        float zr = sqrtf(ph.x * ph.x + ph.y * ph.y);
+        float cphi = ph.x / zr, sphi = ph.y / zr;
+        V dpdu = mk(-2.0f * PI_F * ph.y, 2.0f * PI_F * ph.x, 0.0f);
+        V dpdv = mk(ph.z * cphi, ph.z * sphi, -radius * sinf(theta)) * (0.0f - PI_F);
+        V n = nrm(cross(dpdu, dpdv));
+        s.g.w = nrm(n); s.g.v = nrm(dpdv); s.g.u = nrm(dpdu);
+        s.s = s.g;
+        s.p = ph + c;
+    } else if (p.shape == DFR_SHAPE_RECTANGLE) {
+        s.g = onb_w(ld3(p.normal));
+        s.s = s.g;
+    } else {
+        s.g.u = nrm(ld3(p.data + 3)); s.g.v = nrm(ld3(p.data + 6)); s.g.w = nrm(ld3(p.data + 9));
+        s.s = s.g;
+    }
+    if (dot(s.s.w, s.g.w) < 0.0f) s.s.w = -s.s.w;
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Camera.createPrimaryRay (M/scene/Camera.java:243-291) over the 19-float
+// camera record; pinhole origin.
+// ---------------------------------------------------------------------------
+DEV bool primary_ray(const float* cam, float ix, float iy, float px, float py, V& o, V& d) {
+    float cx = 2.0f * ((ix + px) / (cam[DFR_CAMERA_RES_X] - 1.0f)) - 1.0f;
+    float cy = 2.0f * ((iy + py) / (cam[DFR_CAMERA_RES_Y] - 1.0f)) - 1.0f;
+    float wf = 1.0f;
+    if ((int)cam[DFR_CAMERA_LENS] == DFR_LENS_FISHEYE) {
+        float dd = cx * cx + cy * cy;
+        if (dd > 1.0f) return false;
+        wf = sqrtf(1.0f - dd);
+    }
+    V U = ld3(cam + 3), Vv = ld3(cam + 6), W = ld3(cam + 9), eye = ld3(cam + 12);
+    V on_plane = (U * (cam[0] * cx) + Vv * (cam[1] * cy)) + (eye + W * wf);
+    V on_image = eye + nrm(on_plane - eye) * cam[DFR_CAMERA_FOCAL];
+    o = eye;
+    d = nrm(on_image - eye);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// BxDFs (M/scene/bxdf/*, M/scene/microfacet/*, M/scene/fresnel/*)
+// ---------------------------------------------------------------------------
+DEV float cos_t(V v) { return v.z; }
+DEV float cos_abs(V v) { return fabsf(v.z); }
+DEV float sin2_t(V v) { return fmaxf(0.0f, 1.0f - v.z * v.z); }
+DEV float sin_t(V v) { return sqrtf(sin2_t(v)); }
+DEV float cos_phi(V v) { float s = sin_t(v); return s == 0.0f ? 1.0f : fminf(fmaxf(v.x / s, -1.0f), 1.0f); }
+DEV float sin_phi(V v) { float s = sin_t(v); return s == 0.0f ? 0.0f : fminf(fmaxf(v.y / s, -1.0f), 1.0f); }
+DEV bool same_hemi(V a, V b) { return a.z * b.z > 0.0f; }
+DEV V face_fwd(V a, V b) { return dot(a, b) < 0.0f ? -a : a; }
+DEV V face_fwd_z(V a, V b) { return a.z < 0.0f ? mk(b.x, b.y, -b.z) : b; }
+DEV V reflect_pbrt(V wo, V n) { return -wo + n * (2.0f * dot(wo, n)); }
+DEV bool refract(V d, V n, float eta, V& out) {
+    float ci = dot(d, n);
+    float s2i = fmaxf(0.0f, 1.0f - ci * ci);
+    float s2t = eta * eta * s2i;
+    if (s2t >= 1.0f) return false;
+    float ct = sqrtf(1.0f - s2t);
+    out = (-d) * eta + n * (eta * ci - ct);
+    return true;
+}
+
+DEV float fr_dielectric(float cosi, float etai, float etat) {
+    float c = fminf(fmaxf(cosi, -1.0f), 1.0f);
+    bool entering = c > 0.0f;
+    float ci = entering ? c : fabsf(c);
+    float ei = entering ? etai : etat, et = entering ? etat : etai;
+    float si = sqrtf(fmaxf(0.0f, 1.0f - ci * ci));
+    float st = ei / et * si;
+    if (st >= 1.0f) return 1.0f;
+    float ct = sqrtf(fmaxf(0.0f, 1.0f - st * st));
+    float rpa = ((et * ci) - (ei * ct)) / ((et * ci) + (ei * ct));
+    float rpe = ((ei * ci) - (et * ct)) / ((ei * ci) + (et * ct));
+    return (rpa * rpa + rpe * rpe) / 2.0f;
+}
+DEV float fr_conductor(float cosi, float eta, float k) {
+    float c = fminf(fmaxf(cosi, -1.0f), 1.0f);
+    float c2 = c * c, s2 = 1.0f - c2;
+    float eta2 = eta * eta, k2 = k * k;
+    float t0 = eta2 - k2 - s2;
+    float a2b2 = sqrtf(t0 * t0 + eta2 * k2 * 4.0f);
+    float t1 = a2b2 + c2;
+    float a = sqrtf((a2b2 + t0) * 0.5f);
+    float t2 = a * (c * 2.0f);
+    float rs = (t1 - t2) / (t1 + t2);
+    float t3 = c2 * a2b2 + s2 * s2;
+    float t4 = t2 * s2;
+    float rp = rs * (t3 - t4) / (t3 + t4);
+    return 0.5f * (rp + rs);
+}
+DEV float schlick_w(float c) {
+    float m = fminf(fmaxf(1.0f - c, 0.0f), 1.0f);
+    return (m * m) * (m * m) * m;
+}
+DEV float lerpf(float a, float b, float t) { return (1.0f - t) * a + t * b; }
+
+DEV V fresnel(const DevLobe& l, float cosi) {
+    switch (l.fresnel) {
+        case FR_CONST: return mk(1.0f, 1.0f, 1.0f);
+        case FR_DIELECTRIC: { float f = fr_dielectric(cosi, l.frEtaI, l.frEtaT); return mk(f, f, f); }
+        case FR_CONDUCTOR: {
+            float c = fabsf(cosi);
+            return mk(fr_conductor(c, l.condEta[0], l.condK[0]), fr_conductor(c, l.condEta[1], l.condK[1]),
+                      fr_conductor(c, l.condEta[2], l.condK[2]));
+        }
+        default: {
+            float die = fr_dielectric(cosi, 1.0f, l.disneyEta);
+            float sw = schlick_w(cosi);
+            V r;
+            r.x = lerpf(die, lerpf(l.r0[0], 1.0f, sw), l.disneyMetallic);
+            r.y = lerpf(die, lerpf(l.r0[1], 1.0f, sw), l.disneyMetallic);
+            r.z = lerpf(die, lerpf(l.r0[2], 1.0f, sw), l.disneyMetallic);
+            return r;
+        }
+    }
+}
+
+// Trowbridge-Reitz (visible-area sampling on)
+DEV float tr_D(const DevLobe& l, V h) {
+    float s2 = sin2_t(h), c2 = h.z * h.z;
+    float t2 = s2 / c2;
+    if (isinf(t2)) return 0.0f;
+    float c4 = c2 * c2;
+    float cp = cos_phi(h), sp = sin_phi(h);
+    float e = (cp * cp / (l.alphaX * l.alphaX) + sp * sp / (l.alphaY * l.alphaY)) * t2;
+    return 1.0f / (PI_F * l.alphaX * l.alphaY * c4 * (1.0f + e) * (1.0f + e));
+}
+DEV float tr_lambda(const DevLobe& l, V w) {
+    float ta = fabsf(sin_t(w) / w.z);
+    if (isinf(ta)) return 0.0f;
+    float cp = cos_phi(w), sp = sin_phi(w);
+    float alpha = sqrtf(cp * cp * l.alphaX * l.alphaX + sp * sp * l.alphaY * l.alphaY);
+    float a2 = (alpha * ta) * (alpha * ta);
+    return (-1.0f + sqrtf(1.0f + a2)) / 2.0f;
+}
+DEV float tr_G1(const DevLobe& l, V w) { return 1.0f / (1.0f + tr_lambda(l, w)); }
+DEV float tr_G(const DevLobe& l, V wo, V wi) {
+    return l.separable ? tr_G1(l, wo) * tr_G1(l, wi) : 1.0f / (1.0f + tr_lambda(l, wo) + tr_lambda(l, wi));
+}
+DEV float tr_pdf(const DevLobe& l, V wo, V h) { return tr_D(l, h) * tr_G1(l, wo) * fabsf(dot(wo, h)) / cos_abs(wo); }
+DEV void tr_slope(float ct, float u, float v, float& sx, float& sy) {
+    if (ct > 0.9999f) {
+        float r = sqrtf(u / (1.0f - u)), phi = 2.0f * PI_F * v;
+        sx = r * cosf(phi); sy = r * sinf(phi); return;
+    }
+    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    float tt = st / ct;
+    float a = 2.0f / (1.0f + sqrtf(1.0f + 1.0f / ((1.0f / tt) * (1.0f / tt))));
+    float b = 2.0f * u / a - 1.0f;
+    float c = fminf(1.0f / (b * b - 1.0f), 1.0e10f);
+    float dd = tt;
+    float e = sqrtf(fmaxf(dd * dd * c * c - (b * b - dd * dd) * c, 0.0f));
+    float f = dd * c - e, g = dd * c + e;
+    float hs = v > 0.5f ? 1.0f : -1.0f;
+    float i = v > 0.5f ? 2.0f * (v - 0.5f) : 2.0f * (0.5f - v);
+    float j = (i * (i * (i * 0.27385f - 0.73369f) + 0.46341f)) / (i * (i * (i * 0.093073f + 0.309420f) - 1.0f) + 0.597999f);
+    float x = (b < 0.0f || g > 1.0f / tt) ? f : g;
+    sx = x; sy = hs * j * sqrtf(1.0f + x * x);
+}
+DEV V tr_sample_vis(const DevLobe& l, V wi, float u, float v) {
+    V s = nrm(mk(wi.x * l.alphaX, wi.y * l.alphaY, wi.z));
+    float sx, sy;
+    tr_slope(s.z, u, v, sx, sy);
+    float cp = cos_phi(s), sp = sin_phi(s);
+    float x = -((cp * sx - sp * sy) * l.alphaX);
+    float y = -((sp * sx + cp * sy) * l.alphaY);
+    return nrm(mk(x, y, 1.0f));
+}
+DEV V tr_sample_h(const DevLobe& l, V wo, float u, float v) {
+    return wo.z >= 0.0f ? tr_sample_vis(l, wo, u, v) : -tr_sample_vis(l, -wo, u, v);
+}
+
+DEV float gtr1(float c, float alpha) {
+    float a = alpha * alpha;
+    return (a - 1.0f) / (PI_F * logf(a) * (1.0f + (a - 1.0f) * c * c));
+}
+DEV float smith_ggx(float c, float alpha) {
+    float a = alpha * alpha, b = c * c;
+    return 1.0f / (c + sqrtf(a + b - a * b));
+}
+
+DEV V lobe_f(const DevLobe& l, V wo, V wi) {
+    switch (l.kind) {
+        case LOBE_LAMBERT: return ld3(l.R) * INV_PI;
+        case LOBE_OREN_NAYAR: {
+            float ci = cos_abs(wi), co = cos_abs(wo), si = sin_t(wi), so = sin_t(wo);
+            float mc = (si > 1.0e-4f && so > 1.0e-4f) ? fmaxf(0.0f, cos_phi(wi) * cos_phi(wo) + sin_phi(wi) * sin_phi(wo)) : 0.0f;
+            float sa = ci > co ? so : si;
+            float tb = ci > co ? si / ci : so / co;
+            return ld3(l.R) * INV_PI * (l.A + l.B * mc * sa * tb);
+        }
+        case LOBE_TS_R: {
+            float co = cos_abs(wo), ci = cos_abs(wi);
+            if (co == 0.0f || ci == 0.0f) return mk(0, 0, 0);
+            V h = wi + wo;
+            if (h.x == 0.0f && h.y == 0.0f && h.z == 0.0f) return mk(0, 0, 0);
+            h = nrm(h);
+            float D = tr_D(l, h), G = tr_G(l, wo, wi);
+            V F = fresnel(l, dot(wi, face_fwd(h, mk(0, 0, 1))));
+            return (ld3(l.R) * D * G) * F / (4.0f * ci * co);
+        }
+        case LOBE_TS_T: {
+            if (same_hemi(wo, wi)) return mk(0, 0, 0);
+            float ci = cos_t(wi), co = cos_t(wo);
+            if (ci == 0.0f || co == 0.0f) return mk(0, 0, 0);
+            float eta = co > 0.0f ? l.etaB / l.etaA : l.etaA / l.etaB;
+            V h = nrm(wi * eta + wo);
+            if (h.z < 0.0f) h = -h;
+            float oh = dot(wo, h), ih = dot(wi, h);
+            if (oh * ih > 0.0f) return mk(0, 0, 0);
+            float F = fr_dielectric(oh, l.etaA, l.etaB);
+            float a = oh + eta * ih;
+            float b = 1.0f / eta;
+            float D = tr_D(l, h), G = tr_G(l, wo, wi);
+            float k = fabsf(D * G * eta * eta * fabsf(ih) * fabsf(oh) * b * b / (ci * co * a * a));
+            return mk((1.0f - F) * l.T[0] * k, (1.0f - F) * l.T[1] * k, (1.0f - F) * l.T[2] * k);
+        }
+        case LOBE_DISNEY_DIFFUSE: {
+            float fo = schlick_w(cos_abs(wo)), fi = schlick_w(cos_abs(wi));
+            float k = INV_PI * (1.0f - fo / 2.0f) * (1.0f - fi / 2.0f);
+            return ld3(l.R) * k;
+        }
+        case LOBE_DISNEY_RETRO: {
+            V h = wi + wo;
+            if (h.x == 0.0f && h.y == 0.0f && h.z == 0.0f) return mk(0, 0, 0);
+            h = nrm(h);
+            float cd = dot(wi, h);
+            float fo = schlick_w(cos_abs(wo)), fi = schlick_w(cos_abs(wi));
+            float a = 2.0f * l.roughness * cd * cd;
+            float b = fo + fi + fo * fi * (a - 1.0f);
+            return ld3(l.R) * (INV_PI * a * b);
+        }
+        case LOBE_DISNEY_SHEEN: {
+            V h = wi + wo;
+            if (h.x == 0.0f && h.y == 0.0f && h.z == 0.0f) return mk(0, 0, 0);
+            h = nrm(h);
+            return ld3(l.R) * schlick_w(dot(wi, h));
+        }
+        case LOBE_DISNEY_CLEARCOAT: {
+            V h = wi + wo;
+            if (h.x == 0.0f && h.y == 0.0f && h.z == 0.0f) return mk(0, 0, 0);
+            h = nrm(h);
+            float D = gtr1(cos_abs(h), l.gloss);
+            float F = lerpf(0.04f, 1.0f, schlick_w(dot(wo, h)));
+            float G = smith_ggx(cos_abs(wo), 0.25f) * smith_ggx(cos_abs(wi), 0.25f);
+            float v = l.weight * G * F * D / 4.0f;
+            return mk(v, v, v);
+        }
+        default: return mk(0, 0, 0);
+    }
+}
+
+DEV float lobe_pdf(const DevLobe& l, V wo, V wi) {
+    switch (l.kind) {
+        case LOBE_LAMBERT: case LOBE_OREN_NAYAR: case LOBE_DISNEY_DIFFUSE: case LOBE_DISNEY_RETRO:
+        case LOBE_DISNEY_SHEEN:
+            return same_hemi(wo, wi) ? cos_abs(wi) * INV_PI : 0.0f;
+        case LOBE_TS_R: {
+            if (!same_hemi(wo, wi)) return 0.0f;
+            V h = nrm(wo + wi);
+            return tr_pdf(l, wo, h) / (4.0f * dot(wo, h));
+        }
+        case LOBE_TS_T: {
+            if (same_hemi(wo, wi)) return 0.0f;
+            float eta = cos_t(wo) > 0.0f ? l.etaB / l.etaA : l.etaA / l.etaB;
+            V h = nrm(wi * eta + wo);
+            float oh = dot(wo, h), ih = dot(wi, h);
+            if (oh * ih > 0.0f) return 0.0f;
+            float a = oh + eta * ih;
+            float b = fabsf((eta * eta * ih) / (a * a));
+            return tr_pdf(l, wo, h) * b;
+        }
+        case LOBE_DISNEY_CLEARCOAT: {
+            if (!same_hemi(wo, wi)) return 0.0f;
+            V h = wi + wo;
+            if (h.x == 0.0f && h.y == 0.0f && h.z == 0.0f) return 0.0f;
+            h = nrm(h);
+            float ch = cos_abs(h);
+            return gtr1(ch, l.gloss) * ch / (4.0f * dot(wo, h));
+        }
+        default: return 0.0f;
+    }
+}
+
+DEV void disk_concentric(float u, float v, float& x, float& y) {
+    if (u == 0.0f && v == 0.0f) { x = 0.0f; y = 0.0f; return; }
+    float a = u * 2.0f - 1.0f, b = v * 2.0f - 1.0f;
+    if (a * a > b * b) {
+        float phi = (PI_F / 4.0f) * (b / a);
+        x = a * cosf(phi); y = a * sinf(phi);
+    } else {
+        float phi = (PI_F / 2.0f) - (PI_F / 4.0f) * (a / b);
+        x = b * cosf(phi); y = b * sinf(phi);
+    }
+}
+DEV V hemi_cos(float u, float v) {
+    float x, y;
+    disk_concentric(u, v, x, y);
+    return mk(x, y, sqrtf(fmaxf(0.0f, 1.0f - x * x - y * y)));
+}
+
+struct LSample { V f; V wi; float pdf; uint32_t type; };
+
+DEV bool lobe_sample(const DevLobe& l, V wo, float u, float v, LSample& out) {
+    out.type = l.type;
+    switch (l.kind) {
+        case LOBE_LAMBERT: case LOBE_OREN_NAYAR: case LOBE_DISNEY_DIFFUSE: case LOBE_DISNEY_RETRO:
+        case LOBE_DISNEY_SHEEN: {
+            V wi = face_fwd_z(wo, hemi_cos(u, v));
+            out.wi = wi; out.f = lobe_f(l, wo, wi); out.pdf = lobe_pdf(l, wo, wi);
+            return true;
+        }
+        case LOBE_SPECULAR_R: {
+            V wi = mk(-wo.x, -wo.y, wo.z);
+            out.wi = wi; out.f = fresnel(l, cos_t(wi)) * ld3(l.R) / cos_abs(wi); out.pdf = 1.0f;
+            return true;
+        }
+        case LOBE_FRESNEL_SPECULAR: {
+            float refl = fr_dielectric(cos_t(wo), l.etaA, l.etaB);
+            if (u < refl) {
+                V wi = mk(-wo.x, -wo.y, wo.z);
+                out.type = BX_SPEC | BX_REFL;
+                out.wi = wi; out.f = ld3(l.R) * refl / cos_abs(wi); out.pdf = refl;
+                return true;
+            }
+            bool entering = cos_t(wo) > 0.0f;
+            float ei = entering ? l.etaA : l.etaB, et = entering ? l.etaB : l.etaA;
+            V wi;
+            if (!refract(wo, face_fwd(mk(0, 0, 1), wo), ei / et, wi)) return false;
+            out.type = BX_SPEC | BX_TRANS;
+            out.wi = wi;
+            out.f = ld3(l.T) * (1.0f - refl) * ((ei * ei) / (et * et)) / cos_abs(wi);
+            out.pdf = 1.0f - refl;
+            return true;
+        }
+        case LOBE_TS_R: {
+            if (wo.z == 0.0f) return false;
+            V h = tr_sample_h(l, wo, u, v);
+            if (dot(wo, h) < 0.0f) return false;
+            V wi = reflect_pbrt(wo, h);
+            if (!same_hemi(wo, wi)) return false;
+            out.wi = wi; out.f = lobe_f(l, wo, wi); out.pdf = tr_pdf(l, wo, h) / (4.0f * dot(wo, h));
+            return true;
+        }
+        case LOBE_TS_T: {
+            if (wo.z == 0.0f) return false;
+            V h = tr_sample_h(l, wo, u, v);
+            if (dot(wo, h) < 0.0f) return false;
+            float eta = cos_t(wo) > 0.0f ? l.etaA / l.etaB : l.etaB / l.etaA;
+            V wi;
+            if (!refract(wo, h, eta, wi)) return false;
+            out.wi = wi; out.f = lobe_f(l, wo, wi); out.pdf = lobe_pdf(l, wo, wi);
+            return true;
+        }
+        case LOBE_DISNEY_CLEARCOAT: {
+            if (wo.z == 0.0f) return false;
+            float a2 = l.gloss * l.gloss;
+            float ct = sqrtf(fmaxf(0.0f, (1.0f - powf(a2, 1.0f - u)) / (1.0f - a2)));
+            float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+            float phi = 2.0f * PI_F * v;
+            V hs = mk(st * cosf(phi), st * sinf(phi), ct);
+            V h = same_hemi(wo, hs) ? hs : -hs;
+            V wi = reflect_pbrt(wo, h);
+            if (!same_hemi(wo, wi)) return false;
+            out.wi = wi; out.f = lobe_f(l, wo, wi); out.pdf = lobe_pdf(l, wo, wi);
+            return true;
+        }
+        default: return false;
+    }
+}
+
+// BSDF over a compiled material (M/scene/BSDF.java:272-540)
+struct Bsdf {
+    const DevLobe* lobes;
+    int n;
+    float eta;
+    ONB s;
+    V ng, wo_w, wo;
+};
+DEV Bsdf make_bsdf(const DevScene& sc, const DevMaterial& m, const Surf& si, V ray_d) {
+    Bsdf b;
+    b.lobes = sc.lobes + m.first_lobe;
+    b.n = m.n_lobes;
+    b.eta = m.eta;
+    b.s = si.s; b.ng = si.g.w;
+    b.wo_w = -ray_d;
+    b.wo = to_local(b.wo_w, si.s);
+    return b;
+}
+DEV bool matches(uint32_t lt, uint32_t t) { return (lt & t) == lt; }
+DEV int count_nonspecular(const Bsdf& b) {
+    int c = 0;
+    for (int i = 0; i < b.n; i++) if (!(b.lobes[i].type & BX_SPEC)) c++;
+    return c;
+}
+DEV V bsdf_f(const Bsdf& b, uint32_t t, V wi_w) {
+    V wi = to_local(wi_w, b.s);
+    bool refl = dot(wi_w, b.ng) * dot(b.wo_w, b.ng) > 0.0f;
+    V r = mk(0, 0, 0);
+    for (int i = 0; i < b.n; i++) {
+        const DevLobe& l = b.lobes[i];
+        if (matches(l.type, t) && ((refl && (l.type & BX_REFL)) || (!refl && (l.type & BX_TRANS))))
+            r = r + lobe_f(l, b.wo, wi);
+    }
+    return r;
+}
+DEV float bsdf_pdf(const Bsdf& b, uint32_t t, V wi_w) {
+    if (b.n == 0) return 0.0f;
+    V wi = to_local(wi_w, b.s);
+    if (b.wo.z == 0.0f) return 0.0f;
+    float p = 0.0f; int m = 0;
+    for (int i = 0; i < b.n; i++) {
+        const DevLobe& l = b.lobes[i];
+        if (matches(l.type, t)) { m++; p += lobe_pdf(l, b.wo, wi); }
+    }
+    if (m > 1) p /= m;
+    return p;
+}
+DEV bool bsdf_sample(const Bsdf& b, uint32_t t, float u, float v, LSample& out) {
+    if (b.wo.z == 0.0f) return false;
+    int m = 0;
+    for (int i = 0; i < b.n; i++) if (matches(b.lobes[i].type, t)) m++;
+    if (m == 0) return false;
+    int k = min((int)floorf(u * m), m - 1);
+    int sel = -1, c = 0;
+    for (int i = 0; i < b.n; i++)
+        if (matches(b.lobes[i].type, t)) { if (c == k) sel = i; c++; }
+    const DevLobe& L = b.lobes[sel];
+    float ur = fminf(u * m - k, 0.99999994f);
+    LSample ls;
+    if (!lobe_sample(L, b.wo, ur, v, ls)) return false;
+    V wi_w = to_world(ls.wi, b.s);
+    V f = ls.f;
+    float pdf = ls.pdf;
+    if (!(L.type & BX_SPEC)) {
+        bool refl = dot(wi_w, b.ng) * dot(b.wo_w, b.ng) > 0.0f;
+        f = mk(0, 0, 0);
+        for (int i = 0; i < b.n; i++) {
+            const DevLobe& B = b.lobes[i];
+            if (!matches(B.type, t)) continue;
+            if (i != sel) pdf += lobe_pdf(B, b.wo, ls.wi);
+            if ((refl && (B.type & BX_REFL)) || (!refl && (B.type & BX_TRANS))) f = f + lobe_f(B, b.wo, ls.wi);
+        }
+    }
+    if (m > 1) pdf /= m;
+    out.f = f; out.wi = wi_w; out.pdf = pdf; out.type = ls.type;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Lights
+// ---------------------------------------------------------------------------
+struct LightSmp { V L; V p; V wi; float pdf; };
+
+#include "perez_device.inc"
+
+// DiffuseAreaLight over Triangle3F, PointLight, PerezLight sampling.
+DEV bool sample_light(const DevScene& sc, const DevLight& L, const Surf& si, float u, float v, float usun,
+                      LightSmp& out) {
+    if (L.type == DFR_LIGHT_POINT) {
+        V pos = ld3(L.position);
+        V dd = si.p - pos;
+        out.L = ld3(L.radiance) / dot(dd, dd);
+        out.p = pos;
+        out.wi = nrm(pos - si.p);
+        out.pdf = 1.0f;
+        return true;
+    }
+    if (L.type == DFR_LIGHT_PEREZ) return perez_sample(sc.perez[L.perez], si, u, v, usun, out);
+    if (L.tri < 0) return false;
+    int4 ix = sc.tri_idx[L.tri];
+    V a = f4v(sc.vpos[ix.x]), b = f4v(sc.vpos[ix.y]), c = f4v(sc.vpos[ix.z]);
+    float su = sqrtf(u);
+    float b0 = 1.0f - su, b1 = v * su, b2 = 1.0f - b0 - b1;
+    V p = a * b0 + b * b1 + c * b2;
+    V n = nrm(f4v(sc.vnrm[ix.x]) * b0 + f4v(sc.vnrm[ix.y]) * b1 + f4v(sc.vnrm[ix.z]) * b2);
+    V inc = p - si.p;
+    if (dot(inc, inc) == 0.0f) return false;
+    V incn = nrm(inc);
+    float pdf = dot(inc, inc) / fabsf(dot(n, -incn));
+    if (sc.light_pdf_mode == 1) pdf *= 1.0f / L.area;
+    if (isinf(pdf) || isnan(pdf)) return false;
+    V wi = nrm(p - si.p);
+    if (pdf > 0.0f && (L.two_sided || dot(n, -wi) > 0.0f)) {
+        out.L = ld3(L.radiance); out.p = p; out.wi = wi; out.pdf = pdf;
+        return true;
+    }
+    return false;
+}
+
+// Shape3F.evaluateProbabilityDensityFunction for a triangle light
+DEV float light_pdf(const DevScene& sc, const DevLight& L, const Surf& si, V wi) {
+    if (L.type == DFR_LIGHT_PEREZ) return perez_pdf(sc.perez[L.perez], wi);
+    if (L.type != DFR_LIGHT_DIFFUSE_AREA || L.tri < 0) return 0.0f;
+    int4 ix = sc.tri_idx[L.tri];
+    V a = f4v(sc.vpos[ix.x]), b = f4v(sc.vpos[ix.y]), c = f4v(sc.vpos[ix.z]);
+    V d = nrm(wi);
+    PRay r = prep_ray(si.p, d);
+    float t = tri_t(a, b, c, r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, T_MIN, FLT_MAX);
+    if (t == 0.0f) return 0.0f;
+    HitRec h; h.t = t; h.prim = L.prim; h.tri = L.tri;
+    Surf s2 = make_surface(sc, si.p, d, h);
+    V dp = s2.p - si.p;
+    float pdf = dot(dp, dp) / (fabsf(dot(s2.s.w, -wi)) * L.area);
+    if (isinf(pdf) || isnan(pdf)) return 0.0f;
+    return pdf;
+}
+
+// Intersection.evaluateRadianceEmitted (M/scene/Intersection.java:89-101)
+DEV V emitted(const DevScene& sc, const Surf& si, V dir) {
+    const DevPrim& p = sc.prims[si.prim];
+    if (p.area_light >= 0) {
+        const DevLight& L = sc.lights[p.area_light];
+        return (L.two_sided || dot(si.s.w, dir) > 0.0f) ? ld3(L.radiance) : mk(0, 0, 0);
+    }
+    return ld3(sc.mats[p.material].emission);
+}
+
+DEV V emitted_ray(const DevScene& sc, const DevLight& L, V d) {
+    if (L.type == DFR_LIGHT_PEREZ) return perez_sky(sc.perez[L.perez], d);
+    return mk(0, 0, 0);
+}
+
+DEV float power_heuristic(float a, float b) { return a * a / (a * a + b * b); }
+
+}  // namespace dfr

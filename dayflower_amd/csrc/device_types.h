@@ -1,0 +1,94 @@
+// device_types.h — records of the compiled scene as they live in HBM.
+// Shared by the host scene compiler (api.cpp) and the kernels.
+#pragma once
+#include <cstdint>
+
+namespace dfr {
+
+// Lobe kinds of the compiled BSDF (M/scene/bxdf/*).
+enum LobeKind : int32_t {
+    LOBE_LAMBERT = 0,          // LambertianBRDF
+    LOBE_OREN_NAYAR = 1,       // OrenNayarBRDF
+    LOBE_SPECULAR_R = 2,       // SpecularBRDF (constant Fresnel)
+    LOBE_FRESNEL_SPECULAR = 3, // FresnelSpecularBXDF
+    LOBE_TS_R = 4,             // TorranceSparrowBRDF
+    LOBE_TS_T = 5,             // TorranceSparrowBTDF
+    LOBE_DISNEY_DIFFUSE = 6,
+    LOBE_DISNEY_RETRO = 7,
+    LOBE_DISNEY_SHEEN = 8,
+    LOBE_DISNEY_CLEARCOAT = 9,
+};
+
+// BXDFType bit flags (M/scene/BXDFType.java).
+enum : uint32_t { BX_REFL = 1, BX_TRANS = 2, BX_DIFF = 4, BX_GLOSSY = 8, BX_SPEC = 16 };
+constexpr uint32_t BX_ALL = 31, BX_ALL_EXCEPT_SPECULAR = 15;
+
+enum FresnelKind : int32_t { FR_CONST = 0, FR_DIELECTRIC = 1, FR_CONDUCTOR = 2, FR_DISNEY = 3 };
+
+struct DevLobe {           // 128 bytes
+    int32_t kind;
+    uint32_t type;
+    int32_t fresnel;
+    int32_t separable;     // microfacet shadowing model
+    float R[3], T[3];
+    float A, B;            // Oren-Nayar
+    float etaA, etaB;      // dielectric pair of transmission lobes
+    float frEtaI, frEtaT;  // dielectric Fresnel
+    float condEta[3], condK[3];
+    float r0[3];           // Disney Fresnel
+    float disneyEta, disneyMetallic;
+    float alphaX, alphaY;
+    float roughness, gloss, weight;
+};
+static_assert(sizeof(DevLobe) == 128, "lobe record size");
+
+struct DevMaterial {       // 32 bytes
+    int32_t first_lobe;
+    int32_t n_lobes;       // 0 with has_bsdf = 0: pass-through surface
+    int32_t has_bsdf;
+    int32_t type;          // DFR_MAT_* (material-sort key)
+    float eta;
+    float emission[3];
+};
+
+struct DevPrim {           // 80 bytes
+    int32_t shape, material, area_light, pad;
+    float data[12];
+    float normal[4];       // rectangle: normalize(cross(normalize(b-a), normalize(c-a)))
+};
+
+struct DevLight {          // 64 bytes
+    int32_t type;
+    int32_t prim;
+    int32_t tri;           // original triangle index of a triangle area light
+    int32_t two_sided;
+    float radiance[3];
+    float area;
+    float position[3];
+    int32_t perez;         // offset of the Perez record in the perez table
+    float pad[4];
+};
+
+// Perez sky record: parameters and the 32x32 importance distribution.
+constexpr int PEREZ_RES = 32;
+struct DevPerez {
+    float sun_dir_world[3];
+    float sun_dir_object[3];
+    float sun_color[3];
+    float theta;            // sun zenith angle
+    float radius;
+    float zenith[3];
+    float perez_y[5], perez_x[5], perez_lum[5];
+    float s0[3], s1[3], s2[3];   // daylight basis in XYZ
+    float world_to_object[9];    // rotation, row major
+    float object_to_world[9];
+    float func_int;              // marginal integral
+    float marginal_func[PEREZ_RES];
+    float marginal_cdf[PEREZ_RES + 1];
+    float marginal_int_unused;
+    float cond_func[PEREZ_RES][PEREZ_RES];
+    float cond_cdf[PEREZ_RES][PEREZ_RES + 1];
+    float cond_int[PEREZ_RES];
+};
+
+}  // namespace dfr

@@ -1,0 +1,91 @@
+// kernels.h — host-side launch interface of the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "device_types.h"
+
+namespace dfr {
+
+struct DevScene;
+
+// One render job as the kernels see it.
+struct DevJob {
+    uint64_t seed;
+    int32_t width, height;
+    int32_t spp;              // samples per pixel of this launch sequence
+    int32_t sample_base;      // first sample index (progressive passes)
+    int32_t max_bounce, min_rr;
+    int32_t algorithm, ao_samples;
+    float max_distance;
+    int32_t pad0;
+    int64_t film_offset;      // first pixel of this job in the film array
+    int64_t work_begin;       // first work item of this job (prefix sum)
+    int64_t work_count;       // width * height * spp
+    float cam[19];
+    int32_t scene;            // index into the scene table
+};
+
+// Wavefront path state (structure of arrays).
+struct PathSoA {
+    float4* o;      // origin.xyz
+    float4* d;      // direction.xyz, w = etaScale
+    float4* beta;   // throughput rgb
+    uint4* info;    // pixel, sample, job, bounce | specular << 8 | passthrough << 16
+};
+
+struct ShadowSoA {
+    float4* o;      // origin.xyz, tmax
+    float4* d;      // direction.xyz, mode (int bits)
+    float4* c;      // contribution rgb, film index (int bits)
+    int4* aux;      // light index, job scene, -, -
+};
+
+struct Counters {
+    int32_t n_active;      // paths in the current queue
+    int32_t n_next;        // continuing paths appended by shade
+    int32_t n_shadow;      // shadow rays appended by shade
+    int32_t taken;         // new paths appended by generate
+    int64_t work_next;     // next work item
+    int64_t work_total;
+    int64_t rays_closest;
+    int64_t rays_any;
+    int64_t bounces;
+    int32_t key_count[8];  // material-sort histogram
+    int32_t key_offset[8];
+};
+
+struct WavefrontBuffers {
+    PathSoA a, b;
+    float4* hit;       // t, prim, tri (int bits)
+    int32_t* perm;     // material-sorted order
+    int32_t* keys;
+    ShadowSoA sh;
+    int* overflow;     // traversal-stack overflow
+    int capacity;      // path pool size
+    Counters* ctr;
+};
+
+// Launch helpers (kernels.hip).  `scenes` points to a device array of
+// DevScene records (one per resident scene); jobs to a device array.
+void launch_megakernel(const DevScene* scenes, const DevJob* job, float4* film, int* overflow, int n_pixels,
+                       int block, int64_t* counters, hipStream_t s);
+void launch_trace_rays(const DevScene* scene, const float4* rays, int64_t n, int any_hit, int32_t* prim,
+                       int64_t* tri, float* t, int* overflow, int block, int grid, hipStream_t s);
+void launch_cast_primary(const DevScene* scene, const DevJob* job, float* rays, int32_t* prim, int64_t* tri,
+                         float* t, int* overflow, int n_pixels, hipStream_t s);
+void launch_rgba8(const float4* film, uint8_t* out, int n, int tone_mapper, hipStream_t s);
+
+struct WavefrontParams {
+    int block;          // threads per block
+    int sort_keys;      // 0: no material sort, else number of sort keys
+    int grid_cap;       // blocks of the queue kernels (capacity / block)
+};
+
+void wavefront_init(WavefrontBuffers& w, const DevJob* jobs, int n_jobs, int64_t work_total, hipStream_t s);
+void wavefront_iteration(const DevScene* scenes, const DevJob* jobs, int n_jobs, float4* film, WavefrontBuffers& w,
+                         const WavefrontParams& p, hipStream_t s);
+int overflow_entries_per_thread(int max_depth);
+constexpr int SHORT_STACK = 16;
+
+}  // namespace dfr

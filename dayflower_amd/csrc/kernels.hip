@@ -1,0 +1,670 @@
+// kernels.hip — HIP kernels of the path-tracing hot path for gfx950.
+//
+//  * megakernel: one work-item per pixel, all bounces in registers (the
+//    plain per-job kernel; GPURenderer.run(), M/renderer/gpu/GPURenderer.java
+//    :65-77, restated over the CPURenderer semantics of M/scene/Scene.java
+//    :272-326, 451-718).
+//  * wavefront: generate -> extend (closest hit) -> material sort -> shade ->
+//    shadow (NEE visibility) -> commit, over a path pool in HBM with
+//    wave-aggregated queue appends (ballot + popcount prefix).
+//  * ray queries: closest/any hit for a ray array (hit-parity checks and the
+//    traversal benchmark).
+#include "device.h"
+#include "kernels.h"
+
+namespace dfr {
+
+// ---------------------------------------------------------------------------
+// wave-level helpers (wave64)
+// ---------------------------------------------------------------------------
+DEV int lane_id() { return __lane_id(); }
+
+// Appends the predicate-true lanes of the wave to a queue with one atomic per
+// wave; returns each lane's slot (or -1).
+DEV int wave_append(int* counter, bool pred) {
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return -1;
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    int rank = __popcll(mask & ((1ull << lane_id()) - 1ull));
+    return pred ? base + rank : -1;
+}
+
+DEV void film_add(float4* film, int64_t idx, V c) {
+    if (!(isfinite(c.x) && isfinite(c.y) && isfinite(c.z))) return;
+    if (c.x != 0.0f) unsafeAtomicAdd(&film[idx].x, c.x);
+    if (c.y != 0.0f) unsafeAtomicAdd(&film[idx].y, c.y);
+    if (c.z != 0.0f) unsafeAtomicAdd(&film[idx].z, c.z);
+}
+
+// ---------------------------------------------------------------------------
+// NEE as deferred visibility requests (Scene.doEstimateDirectLight,
+// M/scene/Scene.java:1530-1633, isSpecular = false).
+//   mode 0: contribute c if nothing is hit in (T_MIN, tmax)          (delta / sky lights)
+//   mode 1: contribute c if the closest hit in (T_MIN, tmax) is the light    (area lights)
+//   mode 2: BSDF-sampled ray: closest hit in (T_MIN, inf); the light's
+//           emitted radiance there (or the sky on a miss) times c
+// ---------------------------------------------------------------------------
+struct ShadowReq {
+    V o, d;
+    float tmax;
+    int mode;
+    int light;
+    V c;
+};
+
+DEV int nee_requests(const DevScene& sc, const Bsdf& b, const Surf& si, int li, const Rng& rng, uint32_t dim,
+                     V scale, ShadowReq* out) {
+    const DevLight& L = sc.lights[li];
+    const uint32_t T = BX_ALL_EXCEPT_SPECULAR;
+    int n = 0;
+    LightSmp ls;
+    bool have = sample_light(sc, L, si, rng.get(dim + 1), rng.get(dim + 2), rng.get(dim + 5), ls);
+    V ns = si.s.w;
+    if (L.type == DFR_LIGHT_POINT) {
+        if (have && !black(ls.L) && ls.pdf > 0.0f) {
+            V f = bsdf_f(b, T, ls.wi) * fabsf(dot(ls.wi, ns));
+            if (!black(f)) {
+                ShadowReq& r = out[n++];
+                r.o = si.p; r.d = nrm(ls.p - si.p); r.tmax = fabsf(len(si.p - ls.p)) + 0.001f;
+                r.mode = 0; r.light = li; r.c = scale * f * ls.L / ls.pdf;
+            }
+        }
+        return n;
+    }
+    if (have && !black(ls.L) && ls.pdf > 0.0f) {
+        V f = bsdf_f(b, T, ls.wi) * fabsf(dot(ls.wi, ns));
+        float sp = bsdf_pdf(b, T, ls.wi);
+        if (!black(f)) {
+            float w = power_heuristic(ls.pdf, sp);
+            ShadowReq& r = out[n++];
+            r.o = si.p; r.d = nrm(ls.p - si.p); r.tmax = fabsf(len(si.p - ls.p)) + 0.001f;
+            r.mode = L.type == DFR_LIGHT_DIFFUSE_AREA ? 1 : 0; r.light = li;
+            r.c = scale * f * ls.L * w / ls.pdf;
+        }
+    }
+    LSample bs;
+    if (bsdf_sample(b, T, rng.get(dim + 3), rng.get(dim + 4), bs)) {
+        V f = bs.f * fabsf(dot(bs.wi, ns));
+        bool spec = (bs.type & BX_SPEC) != 0;
+        if (!black(f) && bs.pdf > 0.0f) {
+            float w = 1.0f;
+            bool ok = true;
+            if (!spec) {
+                float lp = light_pdf(sc, L, si, bs.wi);
+                if (lp == 0.0f) ok = false;
+                else w = power_heuristic(bs.pdf, lp);
+            }
+            if (ok) {
+                ShadowReq& r = out[n++];
+                r.o = si.p; r.d = nrm(bs.wi); r.tmax = FLT_MAX;
+                r.mode = 2; r.light = li; r.c = scale * f * w / bs.pdf;
+            }
+        }
+    }
+    return n;
+}
+
+template <int SHORT>
+DEV V resolve_shadow(const DevScene& sc, const ShadowReq& r, TraversalStack<SHORT>& st) {
+    HitRec h;
+    if (r.mode == 0) {
+        if (trace<true, SHORT>(sc, r.o, r.d, T_MIN, r.tmax, h, st)) return mk(0, 0, 0);
+        return r.c;
+    }
+    bool hit = trace<false, SHORT>(sc, r.o, r.d, T_MIN, r.tmax, h, st);
+    if (r.mode == 1) return (hit && sc.prims[h.prim].area_light == r.light) ? r.c : mk(0, 0, 0);
+    if (hit) {
+        if (sc.prims[h.prim].area_light != r.light) return mk(0, 0, 0);
+        Surf s2 = make_surface(sc, r.o, r.d, h);
+        V Li = emitted(sc, s2, -r.d);
+        return r.c * Li;
+    }
+    return r.c * emitted_ray(sc, sc.lights[r.light], r.d);
+}
+
+// ---------------------------------------------------------------------------
+// Plain megakernel (one work-item per pixel)
+// ---------------------------------------------------------------------------
+template <int SHORT>
+DEV V radiance_pt(const DevScene& sc, V o, V d, const Rng& rng, int max_bounce, int min_rr,
+                  TraversalStack<SHORT>& st, int64_t* cnt) {
+    V L = mk(0, 0, 0), beta = mk(1, 1, 1);
+    bool spec = false;
+    float eta_scale = 1.0f;
+    int pass = 0;
+    int64_t n_closest = 0, n_any = 0;
+    for (int bounce = 0;; bounce++) {
+        HitRec h;
+        n_closest++;
+        if (trace<false, SHORT>(sc, o, d, T_MIN, FLT_MAX, h, st)) {
+            Surf si = make_surface(sc, o, d, h);
+            V wo = -d;
+            if (bounce == 0 || spec) L = L + beta * emitted(sc, si, wo);
+            if (bounce >= max_bounce) break;
+            const DevMaterial& m = sc.mats[sc.prims[h.prim].material];
+            if (!m.has_bsdf) {
+                o = si.p;
+                bounce--;
+                if (++pass > 256) break;
+                continue;
+            }
+            Bsdf b = make_bsdf(sc, m, si, d);
+            const uint32_t dim = DIM_BOUNCE + DIMS_PER_BOUNCE * (uint32_t)bounce;
+            if (sc.n_lights > 0 && count_nonspecular(b) > 0) {
+                int n = sc.n_lights;
+                int li = min((int)(rng.get(dim) * n), n - 1);
+                ShadowReq req[2];
+                int nr = nee_requests(sc, b, si, li, rng, dim, beta * (float)n, req);
+                for (int k = 0; k < nr; k++) {
+                    if (req[k].mode == 0) n_any++; else n_closest++;
+                    L = L + resolve_shadow<SHORT>(sc, req[k], st);
+                }
+            }
+            LSample bs;
+            if (!bsdf_sample(b, BX_ALL, rng.get(dim + 6), rng.get(dim + 7), bs)) break;
+            if (black(bs.f) || bs.pdf == 0.0f) break;
+            beta = beta * (bs.f * fabsf(dot(bs.wi, si.s.w)) / bs.pdf);
+            spec = (bs.type & BX_SPEC) != 0;
+            if ((bs.type & BX_TRANS) && spec) eta_scale *= dot(wo, si.g.w) > 0.0f ? b.eta * b.eta : 1.0f / (b.eta * b.eta);
+            o = si.p;
+            d = nrm(bs.wi);
+            V rr = beta * eta_scale;
+            if (maxc(rr) < 1.0f && bounce >= min_rr) {
+                float q = fmaxf(0.05f, 1.0f - maxc(rr));
+                if (rng.get(dim + 8) < q) break;
+                beta = beta / (1.0f - q);
+            }
+        } else {
+            if (bounce == 0 || spec)
+                for (int i = 0; i < sc.n_lights; i++) L = L + beta * emitted_ray(sc, sc.lights[i], d);
+            break;
+        }
+    }
+    cnt[0] += n_closest;
+    cnt[1] += n_any;
+    return L;
+}
+
+template <int SHORT>
+DEV V radiance_ao(const DevScene& sc, V o, V d, const Rng& rng, float max_dist, int samples,
+                  TraversalStack<SHORT>& st, int64_t* cnt) {
+    HitRec h;
+    cnt[0]++;
+    if (!trace<false, SHORT>(sc, o, d, T_MIN, FLT_MAX, h, st)) return mk(0, 0, 0);
+    Surf si = make_surface(sc, o, d, h);
+    float acc = 0.0f;
+    for (int i = 0; i < samples; i++) {
+        float u = rng.get(2 + 2 * i), v = rng.get(3 + 2 * i);
+        float st_ = sqrtf(fmaxf(0.0f, 1.0f - u * u)), phi = 2.0f * PI_F * v;
+        V hs = mk(st_ * cosf(phi), st_ * sinf(phi), u);
+        V dir = nrm(mk(hs.x * si.g.u.x + hs.y * si.g.v.x + hs.z * si.g.w.x,
+                       hs.x * si.g.u.y + hs.y * si.g.v.y + hs.z * si.g.w.y,
+                       hs.x * si.g.u.z + hs.y * si.g.v.z + hs.z * si.g.w.z));
+        HitRec h2;
+        if (max_dist > 0.0f) {
+            // hits beyond max_dist weigh the same as a miss, so the query may stop there
+            cnt[0]++;
+            if (trace<false, SHORT>(sc, si.p, dir, T_MIN, max_dist, h2, st)) {
+                float t = fminf(fmaxf(h2.t, 0.0f), max_dist);
+                acc += t / max_dist;
+            } else {
+                acc += 1.0f;
+            }
+        } else {
+            cnt[1]++;
+            if (!trace<true, SHORT>(sc, si.p, dir, T_MIN, FLT_MAX, h2, st)) acc += 1.0f;
+        }
+    }
+    float v = acc * (PI_F / samples);
+    v = v / PI_F;
+    return mk(v, v, v);
+}
+
+template <int SHORT>
+DEV V radiance_rc(const DevScene& sc, V o, V d, TraversalStack<SHORT>& st, int64_t* cnt) {
+    HitRec h;
+    cnt[0]++;
+    if (trace<false, SHORT>(sc, o, d, T_MIN, FLT_MAX, h, st)) {
+        Surf si = make_surface(sc, o, d, h);
+        float g = 0.5f * fabsf(dot(si.s.w, d));
+        return mk(g, g, g);
+    }
+    V L = mk(0, 0, 0);
+    for (int i = 0; i < sc.n_lights; i++) L = L + emitted_ray(sc, sc.lights[i], d);
+    return L;
+}
+
+template <int BLOCK, int SHORT>
+__global__ __launch_bounds__(BLOCK) void k_megakernel(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobp,
+                                                      float4* __restrict__ film, int* __restrict__ overflow, int n_pixels,
+                                                      unsigned long long* __restrict__ counters) {
+    __shared__ int lds[SHORT * BLOCK];
+    const DevJob& job = *jobp;
+    const DevScene& sc = scenes[job.scene];
+    int pix = blockIdx.x * BLOCK + threadIdx.x;
+    int64_t cnt[2] = {0, 0};
+    if (pix < n_pixels) {
+        TraversalStack<SHORT> st;
+        st.lds = lds + threadIdx.x; st.bdim = BLOCK;
+        st.glob = overflow + (size_t)blockIdx.x * BLOCK + threadIdx.x; st.gstride = gridDim.x * BLOCK;
+        int x = pix % job.width, y = pix / job.width;
+        V sum = mk(0, 0, 0);
+        int count = 0;
+        for (int k = 0; k < job.spp; k++) {
+            uint32_t sample = (uint32_t)(job.sample_base + k);
+            Rng rng = make_rng(job.seed, (uint32_t)pix, sample);
+            V o, d;
+            if (!primary_ray(job.cam, (float)x, (float)y, rng.get(0), rng.get(1), o, d)) continue;
+            V L;
+            if (job.algorithm == DFR_ALGO_PATH_TRACING) L = radiance_pt<SHORT>(sc, o, d, rng, job.max_bounce, job.min_rr, st, cnt);
+            else if (job.algorithm == DFR_ALGO_AMBIENT_OCCLUSION) L = radiance_ao<SHORT>(sc, o, d, rng, job.max_distance, job.ao_samples, st, cnt);
+            else if (job.algorithm == DFR_ALGO_RAY_CASTING) L = radiance_rc<SHORT>(sc, o, d, st, cnt);
+            else L = mk(0, 0, 0);
+            // AbstractCPURenderer.java:229: drop NaN/Inf and negative-luminance samples
+            float lum = 0.212671f * L.x + 0.715160f * L.y + 0.072169f * L.z;
+            if (!(isfinite(L.x) && isfinite(L.y) && isfinite(L.z)) || lum < -1.0e-5f) continue;
+            sum = sum + L;
+            count++;
+        }
+        int64_t fi = job.film_offset + pix;
+        float4 f = film[fi];
+        f.x += sum.x; f.y += sum.y; f.z += sum.z; f.w += (float)count;
+        film[fi] = f;
+    }
+    // per-wave reduction of the ray counters, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        cnt[0] += __shfl_down(cnt[0], off);
+        cnt[1] += __shfl_down(cnt[1], off);
+    }
+    if (lane_id() == 0 && counters) {
+        atomicAdd(&counters[0], (unsigned long long)cnt[0]);
+        atomicAdd(&counters[1], (unsigned long long)cnt[1]);
+    }
+}
+
+int overflow_entries_per_thread(int max_depth) {
+    int need = max_depth + 2 - SHORT_STACK;
+    return need > 1 ? need : 1;
+}
+
+void launch_megakernel(const DevScene* scenes, const DevJob* job, float4* film, int* overflow, int n_pixels, int block,
+                       int64_t* counters, hipStream_t s) {
+    int grid = (n_pixels + block - 1) / block;
+    auto c = reinterpret_cast<unsigned long long*>(counters);
+    switch (block) {
+        case 64: hipLaunchKernelGGL((k_megakernel<64, SHORT_STACK>), dim3(grid), dim3(64), 0, s, scenes, job, film, overflow, n_pixels, c); break;
+        case 128: hipLaunchKernelGGL((k_megakernel<128, SHORT_STACK>), dim3(grid), dim3(128), 0, s, scenes, job, film, overflow, n_pixels, c); break;
+        default: hipLaunchKernelGGL((k_megakernel<256, SHORT_STACK>), dim3((n_pixels + 255) / 256), dim3(256), 0, s, scenes, job, film, overflow, n_pixels, c); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ray queries
+// ---------------------------------------------------------------------------
+template <int BLOCK, int SHORT>
+__global__ __launch_bounds__(BLOCK) void k_trace_rays(const DevScene* __restrict__ scene, const float4* __restrict__ rays,
+                                                      int64_t n, int any_hit, int32_t* __restrict__ prim,
+                                                      int64_t* __restrict__ tri, float* __restrict__ tout,
+                                                      int* __restrict__ overflow) {
+    __shared__ int lds[SHORT * BLOCK];
+    TraversalStack<SHORT> st;
+    st.lds = lds + threadIdx.x; st.bdim = BLOCK;
+    st.glob = overflow + (size_t)blockIdx.x * BLOCK + threadIdx.x; st.gstride = gridDim.x * BLOCK;
+    const DevScene& sc = *scene;
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        float4 a = rays[2 * i], b = rays[2 * i + 1];
+        HitRec h;
+        bool hit = any_hit ? trace<true, SHORT>(sc, f4v(a), f4v(b), a.w, b.w, h, st)
+                           : trace<false, SHORT>(sc, f4v(a), f4v(b), a.w, b.w, h, st);
+        prim[i] = hit ? h.prim : -1;
+        tri[i] = hit ? h.tri : -1;
+        tout[i] = hit ? h.t : INFINITY;
+    }
+}
+
+void launch_trace_rays(const DevScene* scene, const float4* rays, int64_t n, int any_hit, int32_t* prim, int64_t* tri,
+                       float* t, int* overflow, int block, int grid, hipStream_t s) {
+    (void)block;
+    hipLaunchKernelGGL((k_trace_rays<256, SHORT_STACK>), dim3(grid), dim3(256), 0, s, scene, rays, n, any_hit, prim, tri, t,
+                       overflow);
+}
+
+template <int BLOCK, int SHORT>
+__global__ __launch_bounds__(BLOCK) void k_cast_primary(const DevScene* __restrict__ scene, const DevJob* __restrict__ jobp,
+                                                        float* __restrict__ rays, int32_t* __restrict__ prim,
+                                                        int64_t* __restrict__ tri, float* __restrict__ tout,
+                                                        int* __restrict__ overflow, int n_pixels) {
+    __shared__ int lds[SHORT * BLOCK];
+    const DevJob& job = *jobp;
+    int pix = blockIdx.x * BLOCK + threadIdx.x;
+    if (pix >= n_pixels) return;
+    TraversalStack<SHORT> st;
+    st.lds = lds + threadIdx.x; st.bdim = BLOCK;
+    st.glob = overflow + (size_t)blockIdx.x * BLOCK + threadIdx.x; st.gstride = gridDim.x * BLOCK;
+    int x = pix % job.width, y = pix / job.width;
+    Rng rng = make_rng(job.seed, (uint32_t)pix, (uint32_t)job.sample_base);
+    V o, d;
+    float* r = rays + 8 * (size_t)pix;
+    if (!primary_ray(job.cam, (float)x, (float)y, rng.get(0), rng.get(1), o, d)) {
+        for (int k = 0; k < 8; k++) r[k] = 0.0f;
+        prim[pix] = -1; tri[pix] = -1; tout[pix] = INFINITY;
+        return;
+    }
+    r[0] = o.x; r[1] = o.y; r[2] = o.z; r[3] = T_MIN; r[4] = d.x; r[5] = d.y; r[6] = d.z; r[7] = FLT_MAX;
+    HitRec h;
+    bool hit = trace<false, SHORT>(*scene, o, d, T_MIN, FLT_MAX, h, st);
+    prim[pix] = hit ? h.prim : -1;
+    tri[pix] = hit ? h.tri : -1;
+    tout[pix] = hit ? h.t : INFINITY;
+}
+
+void launch_cast_primary(const DevScene* scene, const DevJob* job, float* rays, int32_t* prim, int64_t* tri, float* t,
+                         int* overflow, int n_pixels, hipStream_t s) {
+    hipLaunchKernelGGL((k_cast_primary<256, SHORT_STACK>), dim3((n_pixels + 255) / 256), dim3(256), 0, s, scene, job, rays,
+                       prim, tri, t, overflow, n_pixels);
+}
+
+// imageBegin..imageEnd without tone mapping (AbstractImageKernel.java
+// :1104-1168, sRGB constants :39-47).
+__global__ void k_rgba8(const float4* __restrict__ film, uint8_t* __restrict__ out, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 f = film[i];
+    float inv = f.w > 0.0f ? 1.0f / f.w : 0.0f;
+    float c[3] = {f.x * inv, f.y * inv, f.z * inv};
+    for (int k = 0; k < 3; k++) {
+        float v = c[k];
+        v = v <= 0.00304f ? v * 12.9230566f : 1.05500054f * powf(v, 0.41666666f) - 0.05500052f;
+        out[4 * i + k] = (uint8_t)(int)(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f + 0.5f);
+    }
+    out[4 * i + 3] = 255;
+}
+
+void launch_rgba8(const float4* film, uint8_t* out, int n, int tone_mapper, hipStream_t s) {
+    (void)tone_mapper;
+    hipLaunchKernelGGL(k_rgba8, dim3((n + 255) / 256), dim3(256), 0, s, film, out, n);
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront path tracer
+// ---------------------------------------------------------------------------
+DEV int find_job(const DevJob* jobs, int n_jobs, int64_t item) {
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].work_begin <= item) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// New camera paths appended after the continuing ones (up to capacity).
+__global__ void k_generate(const DevJob* __restrict__ jobs, int n_jobs, PathSoA out, Counters* __restrict__ ctr,
+                           float4* __restrict__ film, int capacity) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int base = ctr->n_next;
+    int64_t next = ctr->work_next, total = ctr->work_total;
+    int64_t room = (int64_t)capacity - base;
+    int64_t avail = total - next;
+    int64_t take = room < avail ? room : avail;
+    if (i >= take) return;
+    int64_t item = next + i;
+    int j = find_job(jobs, n_jobs, item);
+    const DevJob& job = jobs[j];
+    int64_t local = item - job.work_begin;
+    int npix = job.width * job.height;
+    int pix = (int)(local % npix);
+    uint32_t sample = (uint32_t)(job.sample_base + local / npix);
+    int x = pix % job.width, y = pix / job.width;
+    Rng rng = make_rng(job.seed, (uint32_t)pix, sample);
+    V o, d;
+    int slot = base + i;
+    uint32_t flags = 0;
+    if (!primary_ray(job.cam, (float)x, (float)y, rng.get(0), rng.get(1), o, d)) {
+        flags = 0x80000000u;  // rejected camera sample: the path is dropped at shade
+        o = mk(0, 0, 0); d = mk(0, 0, 1);
+    } else {
+        unsafeAtomicAdd(&film[job.film_offset + pix].w, 1.0f);
+    }
+    out.o[slot] = make_float4(o.x, o.y, o.z, 0.0f);
+    out.d[slot] = make_float4(d.x, d.y, d.z, 1.0f);
+    out.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    out.info[slot] = make_uint4((uint32_t)pix, sample, (uint32_t)j, flags);
+}
+
+__global__ void k_commit(Counters* ctr, int capacity) {
+    int base = ctr->n_next;
+    int64_t room = (int64_t)capacity - base;
+    int64_t avail = ctr->work_total - ctr->work_next;
+    int64_t take = room < avail ? room : avail;
+    if (take < 0) take = 0;
+    ctr->work_next += take;
+    ctr->n_active = base + (int)take;
+    ctr->bounces += ctr->n_active;
+    ctr->n_next = 0;
+    ctr->n_shadow = 0;
+    for (int k = 0; k < 8; k++) ctr->key_count[k] = 0;
+}
+
+template <int BLOCK, int SHORT>
+__global__ __launch_bounds__(BLOCK) void k_extend(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
+                                                  PathSoA q, float4* __restrict__ hits, Counters* __restrict__ ctr,
+                                                  int* __restrict__ overflow) {
+    __shared__ int lds[SHORT * BLOCK];
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int n = ctr->n_active;
+    if (i >= n) return;
+    TraversalStack<SHORT> st;
+    st.lds = lds + threadIdx.x; st.bdim = BLOCK;
+    st.glob = overflow + (size_t)blockIdx.x * BLOCK + threadIdx.x; st.gstride = gridDim.x * BLOCK;
+    uint4 info = q.info[i];
+    if (info.w & 0x80000000u) { hits[i] = make_float4(0.0f, __int_as_float(-2), __int_as_float(-1), 0.0f); return; }
+    const DevScene& sc = scenes[jobs[info.z].scene];
+    float4 o = q.o[i], d = q.d[i];
+    HitRec h;
+    bool hit = trace<false, SHORT>(sc, f4v(o), f4v(d), T_MIN, FLT_MAX, h, st);
+    hits[i] = make_float4(h.t, __int_as_float(hit ? h.prim : -1), __int_as_float(h.tri), 0.0f);
+}
+
+// Material-sort keys: material type of the hit (0..5), 6 = miss, 7 = dropped.
+__global__ void k_sort_keys(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, PathSoA q,
+                            const float4* __restrict__ hits, int32_t* __restrict__ keys, Counters* __restrict__ ctr) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = ctr->n_active;
+    int key = -1;
+    if (i < n) {
+        int prim = __float_as_int(hits[i].y);
+        if (prim == -2) key = 7;
+        else if (prim < 0) key = 6;
+        else {
+            const DevScene& sc = scenes[jobs[q.info[i].z].scene];
+            key = sc.mats[sc.prims[prim].material].type;
+        }
+        keys[i] = key;
+    }
+    // wave-aggregated histogram
+    for (int k = 0; k < 8; k++) {
+        uint64_t m = __ballot(key == k);
+        if (m && lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(&ctr->key_count[k], __popcll(m));
+    }
+}
+
+__global__ void k_sort_scan(Counters* ctr) {
+    int acc = 0;
+    for (int k = 0; k < 8; k++) { ctr->key_offset[k] = acc; acc += ctr->key_count[k]; ctr->key_count[k] = 0; }
+}
+
+__global__ void k_sort_scatter(const int32_t* __restrict__ keys, int32_t* __restrict__ perm, Counters* __restrict__ ctr) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = ctr->n_active;
+    int key = i < n ? keys[i] : -1;
+    for (int k = 0; k < 8; k++) {
+        bool p = key == k;
+        uint64_t m = __ballot(p);
+        if (!m) continue;
+        int leader = __ffsll((unsigned long long)m) - 1;
+        int base = 0;
+        if (lane_id() == leader) base = ctr->key_offset[k] + atomicAdd(&ctr->key_count[k], __popcll(m));
+        base = __shfl(base, leader);
+        if (p) perm[base + __popcll(m & ((1ull << lane_id()) - 1ull))] = i;
+    }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_shade(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
+                                                 PathSoA q, const float4* __restrict__ hits,
+                                                 const int32_t* __restrict__ perm, PathSoA out, ShadowSoA sh,
+                                                 Counters* __restrict__ ctr, float4* __restrict__ film, int use_perm) {
+    int k = blockIdx.x * BLOCK + threadIdx.x;
+    int n = ctr->n_active;
+    bool active = k < n;
+    int i = active ? (use_perm ? perm[k] : k) : 0;
+    bool cont = false;
+    V no = mk(0, 0, 0), nd = mk(0, 0, 1), nbeta = mk(0, 0, 0);
+    float neta = 1.0f;
+    uint4 ninfo = make_uint4(0, 0, 0, 0);
+    ShadowReq req[2];
+    int nr = 0;
+    int64_t film_idx = 0;
+    if (active) {
+        uint4 info = q.info[i];
+        float4 hv = hits[i];
+        int prim = __float_as_int(hv.y);
+        if (prim != -2) {
+            const DevJob& job = jobs[info.z];
+            const DevScene& sc = scenes[job.scene];
+            film_idx = job.film_offset + info.x;
+            float4 o4 = q.o[i], d4 = q.d[i], b4 = q.beta[i];
+            V o = f4v(o4), d = f4v(d4), beta = f4v(b4);
+            float eta_scale = d4.w;
+            int bounce = (int)(info.w & 0xffu);
+            bool spec = (info.w >> 8) & 1u;
+            int pass = (int)((info.w >> 16) & 0x1ffu);
+            Rng rng = make_rng(job.seed, info.x, info.y);
+            V L = mk(0, 0, 0);
+            if (prim >= 0) {
+                HitRec h; h.t = hv.x; h.prim = prim; h.tri = __float_as_int(hv.z);
+                Surf si = make_surface(sc, o, d, h);
+                V wo = -d;
+                if (bounce == 0 || spec) L = L + beta * emitted(sc, si, wo);
+                if (bounce < job.max_bounce) {
+                    const DevMaterial& m = sc.mats[sc.prims[prim].material];
+                    if (!m.has_bsdf) {
+                        if (pass + 1 <= 256) {
+                            cont = true; no = si.p; nd = d; nbeta = beta; neta = eta_scale;
+                            ninfo = make_uint4(info.x, info.y, info.z, (uint32_t)bounce | ((uint32_t)spec << 8) | ((uint32_t)(pass + 1) << 16));
+                        }
+                    } else {
+                        Bsdf b = make_bsdf(sc, m, si, d);
+                        const uint32_t dim = DIM_BOUNCE + DIMS_PER_BOUNCE * (uint32_t)bounce;
+                        if (sc.n_lights > 0 && count_nonspecular(b) > 0) {
+                            int nl = sc.n_lights;
+                            int li = min((int)(rng.get(dim) * nl), nl - 1);
+                            nr = nee_requests(sc, b, si, li, rng, dim, beta * (float)nl, req);
+                        }
+                        LSample bs;
+                        if (bsdf_sample(b, BX_ALL, rng.get(dim + 6), rng.get(dim + 7), bs) && !black(bs.f) && bs.pdf != 0.0f) {
+                            V nb = beta * (bs.f * fabsf(dot(bs.wi, si.s.w)) / bs.pdf);
+                            bool nspec = (bs.type & BX_SPEC) != 0;
+                            float ne = eta_scale;
+                            if ((bs.type & BX_TRANS) && nspec) ne *= dot(wo, si.g.w) > 0.0f ? b.eta * b.eta : 1.0f / (b.eta * b.eta);
+                            V rr = nb * ne;
+                            bool alive = true;
+                            if (maxc(rr) < 1.0f && bounce >= job.min_rr) {
+                                float qv = fmaxf(0.05f, 1.0f - maxc(rr));
+                                if (rng.get(dim + 8) < qv) alive = false;
+                                else nb = nb / (1.0f - qv);
+                            }
+                            if (alive) {
+                                cont = true; no = si.p; nd = nrm(bs.wi); nbeta = nb; neta = ne;
+                                ninfo = make_uint4(info.x, info.y, info.z, (uint32_t)(bounce + 1) | ((uint32_t)nspec << 8) | ((uint32_t)pass << 16));
+                            }
+                        }
+                    }
+                }
+            } else if (bounce == 0 || spec) {
+                for (int l = 0; l < sc.n_lights; l++) L = L + beta * emitted_ray(sc, sc.lights[l], d);
+            }
+            if (!black(L)) film_add(film, film_idx, L);
+            // each NEE request carries the film index of its path
+        }
+    }
+    int slot = wave_append(&ctr->n_next, cont);
+    if (cont) {
+        out.o[slot] = make_float4(no.x, no.y, no.z, 0.0f);
+        out.d[slot] = make_float4(nd.x, nd.y, nd.z, neta);
+        out.beta[slot] = make_float4(nbeta.x, nbeta.y, nbeta.z, 0.0f);
+        out.info[slot] = ninfo;
+    }
+    for (int r = 0; r < 2; r++) {
+        bool has = r < nr;
+        int s = wave_append(&ctr->n_shadow, has);
+        if (has) {
+            const ShadowReq& R = req[r];
+            sh.o[s] = make_float4(R.o.x, R.o.y, R.o.z, R.tmax);
+            sh.d[s] = make_float4(R.d.x, R.d.y, R.d.z, __int_as_float(R.mode));
+            sh.c[s] = make_float4(R.c.x, R.c.y, R.c.z, __int_as_float((int)film_idx));
+            sh.aux[s] = make_int4(R.light, (int)q.info[i].z, 0, 0);
+        }
+    }
+}
+
+template <int BLOCK, int SHORT>
+__global__ __launch_bounds__(BLOCK) void k_shadow(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
+                                                  ShadowSoA sh, Counters* __restrict__ ctr, float4* __restrict__ film,
+                                                  int* __restrict__ overflow) {
+    __shared__ int lds[SHORT * BLOCK];
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int n = ctr->n_shadow;
+    if (i >= n) return;
+    TraversalStack<SHORT> st;
+    st.lds = lds + threadIdx.x; st.bdim = BLOCK;
+    st.glob = overflow + (size_t)blockIdx.x * BLOCK + threadIdx.x; st.gstride = gridDim.x * BLOCK;
+    float4 o = sh.o[i], d = sh.d[i], c = sh.c[i];
+    int4 aux = sh.aux[i];
+    const DevScene& sc = scenes[jobs[aux.y].scene];
+    ShadowReq r;
+    r.o = f4v(o); r.tmax = o.w; r.d = f4v(d); r.mode = __float_as_int(d.w); r.light = aux.x; r.c = f4v(c);
+    V v = resolve_shadow<SHORT>(sc, r, st);
+    if (!black(v)) film_add(film, (int64_t)__float_as_int(c.w), v);
+}
+
+__global__ void k_count_rays(Counters* ctr) {
+    ctr->rays_closest += ctr->n_active;
+    ctr->rays_any += ctr->n_shadow;
+}
+
+void wavefront_init(WavefrontBuffers& w, const DevJob* jobs, int n_jobs, int64_t work_total, hipStream_t s) {
+    (void)jobs; (void)n_jobs;
+    Counters c{};
+    c.work_total = work_total;
+    hipMemcpyAsync(w.ctr, &c, sizeof(Counters), hipMemcpyHostToDevice, s);
+}
+
+// One wavefront iteration; the queue state lives on the device, so a
+// sequence of iterations needs no host round trip.
+void wavefront_iteration(const DevScene* scenes, const DevJob* jobs, int n_jobs, float4* film, WavefrontBuffers& w,
+                         const WavefrontParams& p, hipStream_t s) {
+    const int cap = w.capacity;
+    const int grid = (cap + 255) / 256;
+    // top up the queue with new camera paths, then commit the counters
+    hipLaunchKernelGGL(k_generate, dim3(grid), dim3(256), 0, s, jobs, n_jobs, w.a, w.ctr, film, cap);
+    hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, s, w.ctr, cap);
+    hipLaunchKernelGGL((k_extend<256, SHORT_STACK>), dim3(grid), dim3(256), 0, s, scenes, jobs, w.a, w.hit, w.ctr, w.overflow);
+    int use_perm = p.sort_keys > 0;
+    if (use_perm) {
+        hipLaunchKernelGGL(k_sort_keys, dim3(grid), dim3(256), 0, s, scenes, jobs, w.a, w.hit, w.keys, w.ctr);
+        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1), 0, s, w.ctr);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(256), 0, s, w.keys, w.perm, w.ctr);
+    }
+    hipLaunchKernelGGL((k_shade<256>), dim3(grid), dim3(256), 0, s, scenes, jobs, w.a, w.hit, w.perm, w.b, w.sh, w.ctr, film,
+                       use_perm);
+    hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(1), 0, s, w.ctr);
+    hipLaunchKernelGGL((k_shadow<256, SHORT_STACK>), dim3((2 * cap + 255) / 256), dim3(256), 0, s, scenes, jobs, w.sh, w.ctr,
+                       film, w.overflow);
+    std::swap(w.a, w.b);
+}
+
+}  // namespace dfr

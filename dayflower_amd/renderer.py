@@ -1,0 +1,273 @@
+"""Host-side mirror of the reference renderer API over the HIP C-ABI.
+
+`GPURenderer` keeps the method names, defaults and error behaviour of the
+reference's GPURenderer (M/renderer/gpu/AbstractGPURenderer.java:143-556,
+M/renderer/gpu/GPURenderer.java:55-57; M/ = src/main/java/org/dayflower/):
+setScene, setImage, setMaximumBounce, setMinimumBounceRussianRoulette,
+setRenderingAlgorithm, setSamples, setMaximumDistance, setup, render(n),
+clear, renderShutdown, dispose.  Every call goes through
+`libdayflower_hip.so`; there is no CPU fallback.
+"""
+import ctypes
+import enum
+
+import numpy as np
+
+from . import abi
+
+
+class NullPointerException(TypeError):
+    pass
+
+
+class IllegalArgumentException(ValueError):
+    pass
+
+
+class IllegalStateException(RuntimeError):
+    pass
+
+
+class DeviceException(RuntimeError):
+    pass
+
+
+class RenderingAlgorithm(enum.IntEnum):
+    """M/renderer/RenderingAlgorithm.java:33-53 (ordinals as on the GPU)."""
+    AMBIENT_OCCLUSION = abi.ALGO_AMBIENT_OCCLUSION
+    DEPTH_CAMERA = abi.ALGO_DEPTH_CAMERA
+    PATH_TRACING = abi.ALGO_PATH_TRACING
+    RAY_CASTING = abi.ALGO_RAY_CASTING
+    RAY_TRACING = abi.ALGO_RAY_TRACING
+
+
+def _check(lib, ctx, rc):
+    if rc >= 0:
+        return rc
+    msg = lib.dfr_last_error(ctx)
+    msg = msg.decode() if msg else ""
+    if rc == abi.E_NULL:
+        raise NullPointerException(msg)
+    if rc == abi.E_ARGUMENT:
+        raise IllegalArgumentException(msg)
+    if rc == abi.E_STATE:
+        raise IllegalStateException(msg)
+    if rc == abi.E_STOPPED:
+        return rc
+    raise DeviceException(f"{msg} (code {rc})")
+
+
+class GPURenderer:
+    """Drop-in for the reference GPURenderer (one context per instance)."""
+
+    KERNEL_WAVEFRONT = 0
+    KERNEL_PLAIN = 1
+
+    def __init__(self, renderer_observer=None, device=-1):
+        self.lib = abi.load()
+        self.ctx = self.lib.dfr_create(device)
+        if not self.ctx:
+            raise DeviceException("dfr_create failed (no HIP device?)")
+        self.renderer_observer = renderer_observer
+        self._scene = None
+        self._desc = None
+
+    # -- lifecycle ----------------------------------------------------------
+    def dispose(self):
+        if self.ctx:
+            self.lib.dfr_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.dispose()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        return _check(self.lib, self.ctx, rc)
+
+    # -- setters (AbstractGPURenderer.java:447-526) --------------------------
+    def setScene(self, scene):
+        if scene is None:
+            raise NullPointerException("scene == null")
+        self._scene = scene
+        self._desc = scene.desc()
+        self._c(self.lib.dfr_set_scene(self.ctx, ctypes.byref(self._desc)))
+        w, h = int(scene.camera[abi.CAMERA_LENGTH - 2]), int(scene.camera[abi.CAMERA_LENGTH - 1])
+        self._c(self.lib.dfr_set_resolution(self.ctx, w, h))
+
+    def setImage(self, width=None, height=None):
+        """setImage(): resolution from the scene camera; setImage(w, h) explicit."""
+        if width is None:
+            if self._scene is None:
+                raise NullPointerException("scene == null")
+            width = int(self._scene.camera[abi.CAMERA_LENGTH - 2])
+            height = int(self._scene.camera[abi.CAMERA_LENGTH - 1])
+        self._c(self.lib.dfr_set_resolution(self.ctx, int(width), int(height)))
+
+    def _seti(self, key, v):
+        self._c(self.lib.dfr_set_i32(self.ctx, key, int(v)))
+
+    def _geti(self, key):
+        out = ctypes.c_int32()
+        self._c(self.lib.dfr_get_i32(self.ctx, key, ctypes.byref(out)))
+        return out.value
+
+    def setMaximumBounce(self, v):
+        self._seti(abi.KEY_MAXIMUM_BOUNCE, v)
+
+    def getMaximumBounce(self):
+        return self._geti(abi.KEY_MAXIMUM_BOUNCE)
+
+    def setMinimumBounceRussianRoulette(self, v):
+        self._seti(abi.KEY_MINIMUM_BOUNCE_RR, v)
+
+    def getMinimumBounceRussianRoulette(self):
+        return self._geti(abi.KEY_MINIMUM_BOUNCE_RR)
+
+    def setRenderingAlgorithm(self, algorithm):
+        if algorithm is None:
+            raise NullPointerException("renderingAlgorithm == null")
+        self._seti(abi.KEY_RENDERING_ALGORITHM, int(algorithm))
+
+    def getRenderingAlgorithm(self):
+        return RenderingAlgorithm(self._geti(abi.KEY_RENDERING_ALGORITHM))
+
+    def setSamples(self, v):
+        self._seti(abi.KEY_SAMPLES, v)
+
+    def getSamples(self):
+        return self._geti(abi.KEY_SAMPLES)
+
+    def setMaximumDistance(self, v):
+        self._c(self.lib.dfr_set_f32(self.ctx, abi.KEY_MAXIMUM_DISTANCE, float(v)))
+
+    def getMaximumDistance(self):
+        out = ctypes.c_float()
+        self._c(self.lib.dfr_get_f32(self.ctx, abi.KEY_MAXIMUM_DISTANCE, ctypes.byref(out)))
+        return out.value
+
+    def setToneMapper(self, v):
+        self._seti(abi.KEY_TONE_MAPPER, v)
+
+    def setPreviewMode(self, v):
+        self._seti(abi.KEY_PREVIEW_MODE, int(bool(v)))
+
+    # build-specific settings
+    def setSeed(self, seed):
+        seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        lo, hi = seed & 0xFFFFFFFF, seed >> 32
+        self._seti(abi.KEY_SEED, lo - (1 << 32) if lo >= 1 << 31 else lo)
+        self._seti(abi.KEY_SEED_HI, hi - (1 << 32) if hi >= 1 << 31 else hi)
+
+    def setKernel(self, kernel):
+        self._seti(abi.KEY_KERNEL, kernel)
+
+    def setLightPdfMode(self, mode):
+        self._seti(abi.KEY_LIGHT_PDF_MODE, mode)
+
+    def setSamplesPerPass(self, spp):
+        self._seti(abi.KEY_SPP_PER_PASS, spp)
+
+    # -- rendering -----------------------------------------------------------
+    def setup(self, compile_scene=True):
+        self._c(self.lib.dfr_setup(self.ctx, 1 if compile_scene else 0))
+
+    def render(self, passes=1):
+        """render(n): n < 1 raises IllegalArgumentException; returns True."""
+        rc = self._c(self.lib.dfr_render(self.ctx, int(passes)))
+        return rc != abi.E_STOPPED
+
+    def clear(self):
+        self._c(self.lib.dfr_clear(self.ctx))
+
+    def getRenderPass(self):
+        return self.lib.dfr_get_render_pass(self.ctx)
+
+    def renderShutdown(self):
+        self.lib.dfr_request_stop(self.ctx)
+
+    def updateCamera(self, camera):
+        cam = np.ascontiguousarray(camera, np.float32)
+        self._c(self.lib.dfr_update_camera(self.ctx, cam.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    # -- readback ------------------------------------------------------------
+    def film(self, width, height):
+        """Mean radiance (H, W, 3) float32 and per-pixel sample counts."""
+        rgb = np.zeros((height, width, 3), np.float32)
+        cnt = np.zeros((height, width), np.int32)
+        self._c(self.lib.dfr_read_film(self.ctx, rgb.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return rgb, cnt
+
+    def rgba8(self, width, height):
+        out = np.zeros((height, width, 4), np.uint8)
+        self._c(self.lib.dfr_read_rgba8(self.ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return out
+
+    def stats(self):
+        st = abi.Stats()
+        self._c(self.lib.dfr_get_stats(self.ctx, ctypes.byref(st)))
+        return {"seconds": st.seconds, "path_samples": st.path_samples, "rays_closest": st.rays_closest,
+                "rays_any": st.rays_any, "bounces": st.bounces}
+
+    # -- ray queries -----------------------------------------------------------
+    def trace_rays(self, rays, any_hit=False):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        prim = np.empty(n, np.int32)
+        tri = np.empty(n, np.int64)
+        t = np.empty(n, np.float32)
+        self._c(self.lib.dfr_trace_rays(self.ctx, rays.ctypes.data, n, int(any_hit), prim.ctypes.data,
+                                        tri.ctypes.data, t.ctypes.data, 0))
+        return prim, tri, t
+
+    def trace_rays_device(self, rays_ptr, n, prim_ptr, tri_ptr, t_ptr, any_hit=False):
+        """Device-resident variant: every pointer is a device address."""
+        self._c(self.lib.dfr_trace_rays(self.ctx, rays_ptr, n, int(any_hit), prim_ptr, tri_ptr, t_ptr, 1))
+
+    def cast_primary(self, width, height, sample_index=0):
+        n = width * height
+        rays = np.empty((n, 8), np.float32)
+        prim = np.empty(n, np.int32)
+        tri = np.empty(n, np.int64)
+        t = np.empty(n, np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        self._c(self.lib.dfr_cast_primary(self.ctx, sample_index, rays.ctypes.data_as(fp),
+                                          prim.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                          tri.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), t.ctypes.data_as(fp)))
+        return rays, prim, tri, t
+
+    # -- resident scenes / batching --------------------------------------------
+    def addScene(self, scene):
+        desc = scene.desc()
+        sid = self._c(self.lib.dfr_scene_add(self.ctx, ctypes.byref(desc)))
+        return sid
+
+    def removeScene(self, sid):
+        self._c(self.lib.dfr_scene_remove(self.ctx, int(sid)))
+
+    def batchRender(self, jobs, mode=abi.BATCH_PACKED, readback=True):
+        """Render a list of abi.Job records; returns per-job (H, W, 3) films."""
+        n = len(jobs)
+        arr = (abi.Job * n)(*jobs)
+        films = [np.zeros((j.height, j.width, 3), np.float32) for j in jobs] if readback else []
+        fp = ctypes.POINTER(ctypes.c_float)
+        ptrs = (fp * n)(*[f.ctypes.data_as(fp) for f in films]) if readback else None
+        self._c(self.lib.dfr_batch_render(self.ctx, arr, n, int(mode), ptrs))
+        return films
+
+
+def make_job(scene_id, camera, width, height, spp, seed, maximum_bounce=8, minimum_bounce_rr=5):
+    j = abi.Job()
+    j.scene = int(scene_id)
+    j.width, j.height, j.spp = int(width), int(height), int(spp)
+    j.maximum_bounce, j.minimum_bounce_rr = int(maximum_bounce), int(minimum_bounce_rr)
+    j.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    cam = np.asarray(camera, np.float32)
+    for i in range(abi.CAMERA_LENGTH):
+        j.camera[i] = float(cam[i])
+    j.camera[17] = float(width)
+    j.camera[18] = float(height)
+    return j

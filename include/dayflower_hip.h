@@ -1,0 +1,356 @@
+/*
+ * dayflower_hip.h — C-ABI of the MI355X-native path-tracing hot path.
+ *
+ * The entry points below replace, one for one, the renderer surface that the
+ * reference's GPURenderer exposes to its callers (JavaFX GUI, README usage,
+ * ImageGenerator).  Citations use M/ = src/main/java/org/dayflower/ of the
+ * reference tree.
+ *
+ * Conventions
+ *   - Every function returns 0 (DFR_OK) or a negative DFR_E_* code; the text of
+ *     the last error on a context is available from dfr_last_error().
+ *   - The caller owns every input array; the library copies what it needs
+ *     before returning.  The context owns all device memory.
+ *   - One context serialises its calls on an internal mutex (the reference
+ *     methods are `synchronized`); dfr_request_stop() is lock-free.
+ *   - No torch or HIP types appear here: plain pointers, sizes and PODs.
+ */
+#ifndef DAYFLOWER_HIP_H
+#define DAYFLOWER_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Error codes                                                              */
+/* ------------------------------------------------------------------------ */
+#define DFR_OK                 0
+#define DFR_E_NULL            -1  /* NullPointerException in the reference   */
+#define DFR_E_ARGUMENT        -2  /* IllegalArgumentException                */
+#define DFR_E_STATE           -3  /* call order (e.g. render before setup)   */
+#define DFR_E_DEVICE          -4  /* HIP runtime error                       */
+#define DFR_E_MEMORY          -5  /* allocation failed / over budget         */
+#define DFR_E_STOPPED         -6  /* render stopped by dfr_request_stop      */
+#define DFR_E_UNSUPPORTED     -7
+
+/* ------------------------------------------------------------------------ */
+/* Enumerations (ordinals follow the reference)                              */
+/* ------------------------------------------------------------------------ */
+/* M/renderer/RenderingAlgorithm.java:33-53; GPU ordinals
+ * M/renderer/gpu/AbstractGPURenderer.java:83-87 */
+#define DFR_ALGO_AMBIENT_OCCLUSION 0
+#define DFR_ALGO_DEPTH_CAMERA      1
+#define DFR_ALGO_PATH_TRACING      2
+#define DFR_ALGO_RAY_CASTING       3
+#define DFR_ALGO_RAY_TRACING       4
+
+/* Shapes: M/geometry/shape/{TriangleMesh3F,Sphere3F,Rectangle3F,Plane3F}.java */
+#define DFR_SHAPE_TRIANGLE_MESH 0
+#define DFR_SHAPE_SPHERE        1
+#define DFR_SHAPE_RECTANGLE     2
+#define DFR_SHAPE_PLANE         3
+
+/* Materials: M/scene/material/{Matte,Mirror,Glass,Metal,Plastic,Disney}Material.java */
+#define DFR_MAT_MATTE   0
+#define DFR_MAT_MIRROR  1
+#define DFR_MAT_GLASS   2
+#define DFR_MAT_METAL   3
+#define DFR_MAT_PLASTIC 4
+#define DFR_MAT_DISNEY  5
+
+/* Lights: M/scene/light/{DiffuseAreaLight,PointLight,PerezLight}.java */
+#define DFR_LIGHT_DIFFUSE_AREA 0
+#define DFR_LIGHT_POINT        1
+#define DFR_LIGHT_PEREZ        2
+
+/* Camera lens ordinals: M/scene/Lens.java:31-36 */
+#define DFR_LENS_FISHEYE 0
+#define DFR_LENS_THIN    1
+
+/* Camera record: the 19-float layout of
+ * M/scene/compiler/CompiledCameraCache.java:40-102,158-197 */
+#define DFR_CAMERA_FOV_X        0  /* tan(+fovX/2)                        */
+#define DFR_CAMERA_FOV_Y        1  /* tan(-fovY/2)                        */
+#define DFR_CAMERA_LENS         2
+#define DFR_CAMERA_U            3
+#define DFR_CAMERA_V            6
+#define DFR_CAMERA_W            9
+#define DFR_CAMERA_EYE         12
+#define DFR_CAMERA_APERTURE    15
+#define DFR_CAMERA_FOCAL       16
+#define DFR_CAMERA_RES_X       17
+#define DFR_CAMERA_RES_Y       18
+#define DFR_CAMERA_LENGTH      19
+
+/* ------------------------------------------------------------------------ */
+/* Scene description (input of setScene; compiled into SoA at setup)        */
+/* ------------------------------------------------------------------------ */
+
+/* One material.  Which fields are read depends on `type`:
+ *   MATTE   kd, angle (degrees; 0 => Lambertian, else Oren-Nayar), emission
+ *   MIRROR  kr, emission
+ *   GLASS   kr, kt, eta[0], roughness_u/v, remap_roughness, emission
+ *   METAL   eta[0..2], k[0..2], roughness_u/v, remap_roughness, emission
+ *   PLASTIC kd, ks, roughness_u, remap_roughness, emission
+ *   DISNEY  kd (base color), the disney_* scalars, eta[0], thin, emission
+ * Record size is fixed at 128 bytes. */
+typedef struct dfr_material {
+    int32_t type;
+    int32_t remap_roughness;
+    int32_t thin;
+    int32_t reserved0;
+    float emission[3];
+    float kd[3];
+    float ks[3];
+    float kr[3];
+    float kt[3];
+    float eta[3];
+    float k[3];
+    float angle;
+    float roughness_u;
+    float roughness_v;
+    float disney_metallic;
+    float disney_roughness;
+    float disney_specular_tint;
+    float disney_anisotropic;
+    float disney_sheen;
+    float disney_sheen_tint;
+    float disney_clearcoat;
+    float disney_clearcoat_gloss;
+    float disney_spec_trans;
+    float disney_diff_trans;
+    float disney_flatness;
+} dfr_material;
+
+/* One primitive (M/scene/Primitive.java).  Transforms are folded into world
+ * space by the caller: every shape is given in world coordinates.
+ *   TRIANGLE_MESH  triangles [first_triangle, first_triangle+triangle_count)
+ *   SPHERE         data = center.xyz, radius
+ *   RECTANGLE      data = corners a.xyz, b.xyz, c.xyz  (Rectangle3F(a,b,c))
+ *   PLANE          data = origin.xyz, u.xyz, v.xyz, w.xyz (w = normal)
+ * area_light is an index into the light array or -1. */
+typedef struct dfr_primitive {
+    int32_t shape;
+    int32_t material;
+    int32_t area_light;
+    int32_t reserved0;
+    int64_t first_triangle;
+    int64_t triangle_count;
+    float data[12];
+} dfr_primitive;
+
+/* One light.
+ *   DIFFUSE_AREA  radiance, two_sided, primitive (its shape), triangle
+ *                 (global triangle index when the shape is one triangle of a
+ *                 mesh primitive, else -1)
+ *   POINT         radiance = intensity, position
+ *   PEREZ         turbidity, direction = sun direction (world space) */
+typedef struct dfr_light {
+    int32_t type;
+    int32_t primitive;
+    int64_t triangle;
+    int32_t two_sided;
+    int32_t reserved0;
+    float radiance[3];
+    float position[3];
+    float direction[3];
+    float turbidity;
+} dfr_light;
+
+typedef struct dfr_scene_desc {
+    int64_t n_vertices;
+    const float* positions;   /* 3 * n_vertices                         */
+    const float* normals;     /* 3 * n_vertices, or NULL (flat)         */
+    const float* uvs;         /* 2 * n_vertices, or NULL                */
+    int64_t n_triangles;
+    const int32_t* indices;   /* 3 * n_triangles                        */
+    int32_t n_primitives;
+    const dfr_primitive* primitives;
+    int32_t n_materials;
+    const dfr_material* materials;
+    int32_t n_lights;
+    const dfr_light* lights;
+    float camera[DFR_CAMERA_LENGTH];
+} dfr_scene_desc;
+
+/* ------------------------------------------------------------------------ */
+/* Renderer context: the GPURenderer surface                                */
+/* ------------------------------------------------------------------------ */
+typedef struct dfr_ctx dfr_ctx;
+
+/* Integer / float settings (setters of AbstractGPURenderer.java:447-526). */
+#define DFR_KEY_MAXIMUM_BOUNCE          1  /* default 20                       */
+#define DFR_KEY_MINIMUM_BOUNCE_RR       2  /* default 5                        */
+#define DFR_KEY_RENDERING_ALGORITHM     3  /* default DFR_ALGO_PATH_TRACING    */
+#define DFR_KEY_SAMPLES                 4  /* AO samples, default 10           */
+#define DFR_KEY_TONE_MAPPER             5  /* default 0 (none)                 */
+#define DFR_KEY_PREVIEW_MODE            6  /* default 0                        */
+#define DFR_KEY_SEED                    7  /* 64-bit seed, low 32 bits         */
+#define DFR_KEY_SEED_HI                 8  /* 64-bit seed, high 32 bits        */
+#define DFR_KEY_KERNEL                  9  /* 0 wavefront (default), 1 plain   */
+#define DFR_KEY_LIGHT_PDF_MODE         10  /* 0 reference (default), 1 area    */
+#define DFR_KEY_SPP_PER_PASS           11  /* samples per pixel per pass, 1    */
+#define DFR_KEY_MAXIMUM_DISTANCE      101  /* float, default 20                */
+
+/* new GPURenderer(observer): M/renderer/gpu/GPURenderer.java:55-57 with the
+ * defaults of M/renderer/gpu/AbstractGPURenderer.java:143-159.
+ * device < 0 selects the current HIP device.  Returns NULL on failure. */
+dfr_ctx* dfr_create(int32_t device);
+
+/* dispose(): M/renderer/gpu/AbstractKernel.java:166-168 */
+void dfr_destroy(dfr_ctx* ctx);
+
+/* Text of the last error on this context (empty string if none). */
+const char* dfr_last_error(const dfr_ctx* ctx);
+
+/* setScene(Scene): M/renderer/gpu/AbstractSceneKernel.java:247-250.
+ * Copies the description; nothing is compiled until dfr_setup(). */
+int32_t dfr_set_scene(dfr_ctx* ctx, const dfr_scene_desc* scene);
+
+/* setImage(): M/renderer/gpu/AbstractGPURenderer.java:417-440.  Sets the
+ * film resolution; takes effect at the next setup/render. */
+int32_t dfr_set_resolution(dfr_ctx* ctx, int32_t width, int32_t height);
+
+/* setMaximumBounce, setMinimumBounceRussianRoulette, setRenderingAlgorithm,
+ * setSamples, setToneMapper, setPreviewMode, setMaximumDistance:
+ * M/renderer/gpu/AbstractGPURenderer.java:447-526 */
+int32_t dfr_set_i32(dfr_ctx* ctx, int32_t key, int32_t value);
+int32_t dfr_get_i32(const dfr_ctx* ctx, int32_t key, int32_t* value);
+int32_t dfr_set_f32(dfr_ctx* ctx, int32_t key, float value);
+int32_t dfr_get_f32(const dfr_ctx* ctx, int32_t key, float* value);
+
+/* setup() / setup(boolean): M/renderer/gpu/AbstractGPURenderer.java:532-556.
+ * Compiles the scene (BVH + SoA) when compile_scene != 0, uploads it and
+ * allocates the film. */
+int32_t dfr_setup(dfr_ctx* ctx, int32_t compile_scene);
+
+/* render(n): M/renderer/gpu/AbstractGPURenderer.java:231-332.  n < 1 is
+ * DFR_E_ARGUMENT.  Each pass adds SPP_PER_PASS samples to every pixel. */
+int32_t dfr_render(dfr_ctx* ctx, int32_t passes);
+
+/* clear(): lazy reset of the film at the next render
+ * (M/renderer/gpu/AbstractGPURenderer.java:209-211, 382-411). */
+int32_t dfr_clear(dfr_ctx* ctx);
+int32_t dfr_get_render_pass(const dfr_ctx* ctx);
+
+/* renderShutdown(): M/renderer/gpu/AbstractGPURenderer.java:342-344. */
+int32_t dfr_request_stop(dfr_ctx* ctx);
+
+/* updateCamera(): M/renderer/gpu/AbstractSceneKernel.java:278-300 */
+int32_t dfr_update_camera(dfr_ctx* ctx, const float camera[DFR_CAMERA_LENGTH]);
+
+/* Film readback (the running mean of M/renderer/gpu/AbstractImageKernel.java
+ * :960-995).  rgb: 3 floats per pixel (row-major, y down), count: samples
+ * per pixel.  Either pointer may be NULL. */
+int32_t dfr_read_film(dfr_ctx* ctx, float* rgb, int32_t* count);
+
+/* RGBA8 image (imageBegin..imageEnd, AbstractImageKernel.java:1104-1168):
+ * tone map (if any), sRGB encode, (int)(clamp*255+0.5). */
+int32_t dfr_read_rgba8(dfr_ctx* ctx, uint8_t* rgba);
+
+/* Counters of the last render call. */
+typedef struct dfr_stats {
+    double   seconds;            /* device time of the last render call     */
+    int64_t  path_samples;       /* camera samples traced                   */
+    int64_t  rays_closest;       /* closest-hit queries                     */
+    int64_t  rays_any;           /* any-hit (shadow / AO) queries           */
+    int64_t  bounces;            /* path vertices shaded                    */
+} dfr_stats;
+int32_t dfr_get_stats(const dfr_ctx* ctx, dfr_stats* stats);
+
+/* ------------------------------------------------------------------------ */
+/* Ray queries (ray casting / AO hit parity, BVH bench)                      */
+/* ------------------------------------------------------------------------ */
+/* A ray is 8 floats: origin.xyz, tmin, direction.xyz, tmax.
+ * Hit records: prim (primitive index or -1), tri (global triangle index or
+ * -1), t (world-space distance; undefined on a miss).
+ * When `device_ptrs` != 0 every pointer is a device pointer already resident
+ * in HBM; otherwise they are host pointers copied in and out.
+ * any_hit != 0: stop at the first hit (M/scene/Scene.java intersects). */
+int32_t dfr_trace_rays(dfr_ctx* ctx, const float* rays, int64_t n_rays,
+                       int32_t any_hit, int32_t* prim, int64_t* tri, float* t,
+                       int32_t device_ptrs);
+
+/* Primary rays of the current camera for pass `sample_index` with the
+ * uniform sub-pixel offset of ray casting (GPURenderer.java:338-369), traced
+ * to the closest hit.  Writes the rays used (8 floats each) and hit records,
+ * one per pixel.  Host pointers. */
+int32_t dfr_cast_primary(dfr_ctx* ctx, int32_t sample_index, float* rays,
+                         int32_t* prim, int64_t* tri, float* t);
+
+/* ------------------------------------------------------------------------ */
+/* Resident scenes, batched jobs and serving                                 */
+/* ------------------------------------------------------------------------ */
+/* A render job: one camera over one resident scene at one resolution and
+ * spp.  Jobs never share RNG streams: sample s of pixel p of job j uses the
+ * counter (seed_j, p, s), so a job's film is the same alone or batched. */
+typedef struct dfr_job {
+    int32_t scene;          /* id from dfr_scene_add                        */
+    int32_t width;
+    int32_t height;
+    int32_t spp;
+    int32_t maximum_bounce;
+    int32_t minimum_bounce_rr;
+    uint64_t seed;
+    float camera[DFR_CAMERA_LENGTH];
+} dfr_job;
+
+/* Compile and keep a scene resident on the device; returns its id (>= 0)
+ * or a negative error code. */
+int32_t dfr_scene_add(dfr_ctx* ctx, const dfr_scene_desc* scene);
+int32_t dfr_scene_remove(dfr_ctx* ctx, int32_t scene);
+
+#define DFR_BATCH_PLAIN   0   /* one launch sequence per job, in order      */
+#define DFR_BATCH_PACKED  1   /* all jobs share persistent wavefront launches */
+
+/* Render `n_jobs` jobs to completion.  films[j] receives width*height*3
+ * floats (mean radiance) of job j (host pointers; NULL to skip readback).
+ * Returns 0 or an error; per-batch timing in dfr_get_stats. */
+int32_t dfr_batch_render(dfr_ctx* ctx, const dfr_job* jobs, int32_t n_jobs,
+                         int32_t mode, float* const* films);
+
+/* Autotune the packed wavefront launch (block size, path-pool capacity,
+ * material-sort granularity) on a job list; the chosen values are kept on
+ * the context and reported through `out` (3 ints) when it is not NULL. */
+int32_t dfr_autotune(dfr_ctx* ctx, const dfr_job* jobs, int32_t n_jobs, int32_t* out);
+
+/* Serving: a native request loop with continuous batching. */
+typedef struct dfr_server dfr_server;
+
+typedef struct dfr_server_opts {
+    int64_t hbm_budget_bytes;   /* admission budget for films + queues      */
+    int32_t max_active;         /* at most this many requests on the device */
+    int32_t pool_paths;         /* path pool size of the wavefront engine   */
+    int32_t spp_quantum;        /* samples per pixel per request per round  */
+    int32_t reserved;
+} dfr_server_opts;
+
+dfr_server* dfr_server_create(dfr_ctx* ctx, const dfr_server_opts* opts);
+void dfr_server_destroy(dfr_server* server);
+/* Submit a request (non-blocking); returns its id (>= 0) or an error. */
+int64_t dfr_server_submit(dfr_server* server, const dfr_job* job);
+/* 1 done, 0 pending, <0 error (including rejected by admission). */
+int32_t dfr_server_poll(dfr_server* server, int64_t request);
+/* Block until the request is done; then copy its film (3 floats/pixel). */
+int32_t dfr_server_wait(dfr_server* server, int64_t request, float* film);
+/* Latency (submit -> film ready) in seconds of request `request`. */
+int32_t dfr_server_latency(dfr_server* server, int64_t request, double* seconds);
+typedef struct dfr_server_stats {
+    int64_t completed;
+    int64_t rejected;
+    int64_t path_samples;
+    double  busy_seconds;
+    int64_t peak_bytes;
+    int32_t peak_active;
+    int32_t reserved;
+} dfr_server_stats;
+int32_t dfr_server_get_stats(dfr_server* server, dfr_server_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DAYFLOWER_HIP_H */

@@ -1,0 +1,1604 @@
+// oracle.cpp — TEST INFRASTRUCTURE ONLY: the oracle of the path-tracing hot
+// path.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+// may load it; the product never links or calls it.
+//
+// A float32 CPU restatement of the reference CPURenderer integrators and an
+// FP64 closest-hit intersector.  Citations: M/ = src/main/java/org/dayflower/
+// of the reference tree.  Pins for every function are listed in oracle.h.
+#include "oracle.h"
+#include "oracle_math.h"
+
+#include <atomic>
+#include <cstring>
+#include <memory>
+#include <thread>
+#include <vector>
+
+using namespace orc;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Counter-based RNG.  The reference draws from an unseeded java.util.Random
+// (M/scene/Scene.java:141, M/sampler/NRooksSampler.java:51-53), so no run of
+// it can be repeated; the restatement replaces it with a stateless hash keyed
+// by (seed, pixel, sample, dimension).  The integer hash is Wellons'
+// "lowbias32" (two multiply-xorshift rounds).
+// ---------------------------------------------------------------------------
+inline uint32_t lowbias32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+struct Rng {
+    uint32_t a, b;
+    float get(uint32_t dim) const {
+        uint32_t h = lowbias32(a ^ lowbias32(b + dim * 0x9e3779b9u));
+        return (float)(h >> 8) * (1.0f / 16777216.0f);
+    }
+};
+inline Rng makeRng(uint64_t seed, uint32_t pixel, uint32_t sample) {
+    Rng r;
+    r.a = lowbias32(pixel ^ lowbias32((uint32_t)seed));
+    r.b = lowbias32(sample ^ lowbias32((uint32_t)(seed >> 32) ^ 0x85ebca6bu) ^ r.a);
+    return r;
+}
+// Dimension layout: 0,1 pixel offset; path-tracing bounce b uses
+// 2 + 12*b + {0 light choice, 1-2 light sample, 3-4 NEE BSDF sample,
+// 5 Perez sun choice, 6-7 BSDF continuation, 8 Russian roulette}.
+// Ambient occlusion sample i uses 2 + 2*i, 3 + 2*i.
+const uint32_t DIM_BOUNCE = 2, DIMS_PER_BOUNCE = 12;
+
+// ---------------------------------------------------------------------------
+// Scene (world space).  Transforms are folded in by the caller.
+// ---------------------------------------------------------------------------
+struct Tri {
+    V3 a, b, c;
+    V3 na, nb, nc;
+    float ua, va, ub, vb, uc, vc;
+    V3 ng;        // Vector3F.normalNormalized(a, b, c), Triangle3F.java:102
+    int prim;
+};
+
+struct Prim {
+    int shape, material, areaLight;
+    int64_t first, count;
+    float data[12];
+    V3 rectNormal;
+};
+
+struct Light {
+    int type;
+    int prim;
+    int64_t tri;
+    bool twoSided;
+    C3 radiance;
+    V3 position;
+    // Perez sky (filled by perezInit)
+    struct Perez* perez;
+};
+
+struct Ray { V3 o, d; };
+
+struct Hit {
+    float t;
+    int prim;
+    int64_t tri;
+};
+
+// Surface intersection in world space (M/geometry/SurfaceIntersection3F.java,
+// M/scene/Intersection.java).
+struct Surface {
+    V3 p;
+    ONB g, s;
+    float tu, tv;
+    float t;
+    int prim;
+    int64_t tri;
+    V3 rayDir;
+};
+
+struct BvhNode {
+    float lo[3], hi[3];
+    int left, right;    // children, or -1
+    int64_t first, count;
+};
+
+struct Scene {
+    std::vector<Tri> tris;
+    std::vector<int64_t> triOrder;
+    std::vector<Prim> prims;
+    std::vector<int> analytic;   // primitives that are not meshes
+    std::vector<dfr_material> mats;
+    std::vector<Light> lights;
+    std::vector<BvhNode> nodes;
+    float cam[DFR_CAMERA_LENGTH];
+    ~Scene();
+};
+
+// ---- oracle BVH (its own median split; results are BVH independent) -------
+void bvhBuild(Scene& sc) {
+    const int64_t n = (int64_t)sc.tris.size();
+    sc.triOrder.resize(n);
+    for (int64_t i = 0; i < n; i++) sc.triOrder[i] = i;
+    sc.nodes.clear();
+    if (n == 0) return;
+    std::vector<float> cen(3 * n), lo(3 * n), hi(3 * n);
+    for (int64_t i = 0; i < n; i++) {
+        const Tri& t = sc.tris[i];
+        for (int k = 0; k < 3; k++) {
+            float a = t.a[k], b = t.b[k], c = t.c[k];
+            lo[3 * i + k] = std::min(a, std::min(b, c));
+            hi[3 * i + k] = std::max(a, std::max(b, c));
+            cen[3 * i + k] = 0.5f * (lo[3 * i + k] + hi[3 * i + k]);
+        }
+    }
+    struct Item { int node; int64_t b, e; };
+    std::vector<Item> stack;
+    sc.nodes.push_back(BvhNode());
+    stack.push_back({0, 0, n});
+    while (!stack.empty()) {
+        Item it = stack.back(); stack.pop_back();
+        BvhNode nd;
+        for (int k = 0; k < 3; k++) { nd.lo[k] = INFINITY; nd.hi[k] = -INFINITY; }
+        float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int64_t i = it.b; i < it.e; i++) {
+            int64_t t = sc.triOrder[i];
+            for (int k = 0; k < 3; k++) {
+                nd.lo[k] = std::min(nd.lo[k], lo[3 * t + k]);
+                nd.hi[k] = std::max(nd.hi[k], hi[3 * t + k]);
+                clo[k] = std::min(clo[k], cen[3 * t + k]);
+                chi[k] = std::max(chi[k], cen[3 * t + k]);
+            }
+        }
+        nd.left = nd.right = -1; nd.first = it.b; nd.count = it.e - it.b;
+        if (it.e - it.b <= 4) { sc.nodes[it.node] = nd; continue; }
+        int axis = 0;
+        for (int k = 1; k < 3; k++) if (chi[k] - clo[k] > chi[axis] - clo[axis]) axis = k;
+        int64_t mid = (it.b + it.e) / 2;
+        std::nth_element(sc.triOrder.begin() + it.b, sc.triOrder.begin() + mid, sc.triOrder.begin() + it.e,
+                         [&](int64_t x, int64_t y) { return cen[3 * x + axis] < cen[3 * y + axis]; });
+        int l = (int)sc.nodes.size(); sc.nodes.push_back(BvhNode());
+        int r = (int)sc.nodes.size(); sc.nodes.push_back(BvhNode());
+        nd.left = l; nd.right = r; nd.count = 0;
+        sc.nodes[it.node] = nd;
+        stack.push_back({l, it.b, mid});
+        stack.push_back({r, mid, it.e});
+    }
+}
+
+// Conservative slab test in double (used by both the float and FP64 paths:
+// box tests only prune, they never decide a hit).
+inline bool boxHit(const BvhNode& nd, const double o[3], const double inv[3], double tmin, double tmax) {
+    double t0 = tmin, t1 = tmax;
+    for (int k = 0; k < 3; k++) {
+        double a = ((double)nd.lo[k] - o[k]) * inv[k];
+        double b = ((double)nd.hi[k] - o[k]) * inv[k];
+        if (a > b) std::swap(a, b);
+        if (std::isnan(a)) a = -INFINITY;
+        if (std::isnan(b)) b = INFINITY;
+        // widen by a relative epsilon so that float-rounded boxes never
+        // reject a ray that grazes them
+        double eps = 1e-7 * (std::fabs(a) + std::fabs(b)) + 1e-12;
+        t0 = std::max(t0, a - eps);
+        t1 = std::min(t1, b + eps);
+        if (t0 > t1) return false;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Shapes (float restatement)
+// ---------------------------------------------------------------------------
+// When set, the triangle test compares t * det with tMinimum literally, as
+// M/geometry/shape/Triangle3F.java:570-574 does (tests only).
+bool g_literal_tmin = false;
+
+// Triangle3F.intersectionT, M/geometry/shape/Triangle3F.java:505-600 (pbrt-v3
+// watertight test).  Returns NaN on a miss.
+float triangleT(const Tri& tr, const Ray& ray, float tMin, float tMax) {
+    V3 p0t = tr.a - ray.o, p1t = tr.b - ray.o, p2t = tr.c - ray.o;
+    V3 ad(std::fabs(ray.d.x), std::fabs(ray.d.y), std::fabs(ray.d.z));
+    int kz = ad.x > ad.y ? (ad.x > ad.z ? 0 : 2) : (ad.y > ad.z ? 1 : 2);  // Vector3F.java:406-408
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    V3 d(ray.d[kx], ray.d[ky], ray.d[kz]);
+    p0t = V3(p0t[kx], p0t[ky], p0t[kz]);
+    p1t = V3(p1t[kx], p1t[ky], p1t[kz]);
+    p2t = V3(p2t[kx], p2t[ky], p2t[kz]);
+    float sx = -d.x / d.z, sy = -d.y / d.z, sz = 1.0f / d.z;
+    p0t = V3(p0t.x + sx * p0t.z, p0t.y + sy * p0t.z, p0t.z);
+    p1t = V3(p1t.x + sx * p1t.z, p1t.y + sy * p1t.z, p1t.z);
+    p2t = V3(p2t.x + sx * p2t.z, p2t.y + sy * p2t.z, p2t.z);
+    float e0 = p1t.x * p2t.y - p1t.y * p2t.x;
+    float e1 = p2t.x * p0t.y - p2t.y * p0t.x;
+    float e2 = p0t.x * p1t.y - p0t.y * p1t.x;
+    if (e0 == 0.0f || e1 == 0.0f || e2 == 0.0f) {
+        e0 = (float)((double)p2t.y * (double)p1t.x - (double)p2t.x * (double)p1t.y);
+        e1 = (float)((double)p0t.y * (double)p2t.x - (double)p0t.x * (double)p2t.y);
+        e2 = (float)((double)p1t.y * (double)p0t.x - (double)p1t.x * (double)p0t.y);
+    }
+    if ((e0 < 0.0f || e1 < 0.0f || e2 < 0.0f) && (e0 > 0.0f || e1 > 0.0f || e2 > 0.0f)) return NAN;
+    float det = e0 + e1 + e2;
+    if (det == 0.0f) return NAN;
+    p0t.z *= sz; p1t.z *= sz; p2t.z *= sz;
+    float tScaled = e0 * p0t.z + e1 * p1t.z + e2 * p2t.z;
+    // The reference compares tScaled (= t * det) with tMinimum itself, which
+    // hides every triangle with t * det <= 0.001; the scale-consistent form
+    // compares with tMinimum * det, i.e. t > tMinimum.
+    const float tMinScaled = g_literal_tmin ? tMin : tMin * det;
+    if (det < 0.0f && (tScaled >= tMinScaled || tScaled < tMax * det)) return NAN;
+    if (det > 0.0f && (tScaled <= tMinScaled || tScaled > tMax * det)) return NAN;
+    float invDet = 1.0f / det;
+    float t = tScaled * invDet;
+    float maxZt = std::max(std::fabs(p0t.z), std::max(std::fabs(p1t.z), std::fabs(p2t.z)));
+    float maxXt = std::max(std::fabs(p0t.x), std::max(std::fabs(p1t.x), std::fabs(p2t.x)));
+    float maxYt = std::max(std::fabs(p0t.y), std::max(std::fabs(p1t.y), std::fabs(p2t.y)));
+    float deltaZ = gamma(3) * maxZt;
+    float deltaX = gamma(5) * (maxXt + maxZt);
+    float deltaY = gamma(5) * (maxYt + maxZt);
+    float deltaE = 2.0f * (gamma(2) * maxXt * maxYt + deltaY * maxXt + deltaX * maxYt);
+    float maxE = std::max(std::fabs(e0), std::max(std::fabs(e1), std::fabs(e2)));
+    float deltaT = 3.0f * (gamma(3) * maxE * maxZt + deltaE * maxZt + deltaZ * maxE) * std::fabs(invDet);
+    if (t <= deltaT) return NAN;
+    return t;
+}
+
+// EFloat (M/utility/EFloat.java:34-195): value with a conservative interval.
+struct EF {
+    float v, lo, hi;
+    EF(float x) : v(x), lo(x), hi(x) {}
+    EF(float x, float err) : v(x) {
+        if (err == 0.0f) { lo = hi = x; } else { lo = nextDown(x - err); hi = nextUp(x + err); }
+    }
+    EF(float x, float l, float h) : v(x), lo(l), hi(h) {}
+};
+inline EF efAdd(EF a, EF b) { return EF(a.v + b.v, nextDown(a.lo + b.lo), nextUp(a.hi + b.hi)); }
+inline EF efSub(EF a, EF b) { return EF(a.v - b.v, nextDown(a.lo - b.hi), nextUp(a.hi - b.lo)); }
+inline EF efMul(EF a, EF b) {
+    float p0 = a.lo * b.lo, p1 = a.hi * b.lo, p2 = a.lo * b.hi, p3 = a.hi * b.hi;
+    return EF(a.v * b.v, nextDown(std::min(std::min(p0, p1), std::min(p2, p3))),
+              nextUp(std::max(std::max(p0, p1), std::max(p2, p3))));
+}
+inline EF efDiv(EF a, EF b) {
+    float v = a.v / b.v;
+    if (b.lo < 0.0f && b.hi > 0.0f) return EF(v, -INFINITY, INFINITY);
+    float d0 = a.lo / b.lo, d1 = a.hi / b.lo, d2 = a.lo / b.hi, d3 = a.hi / b.hi;
+    return EF(v, nextDown(std::min(std::min(d0, d1), std::min(d2, d3))),
+              nextUp(std::max(std::max(d0, d1), std::max(d2, d3))));
+}
+bool efQuadratic(EF a, EF b, EF c, EF* t0, EF* t1) {
+    double discrim = (double)b.v * (double)b.v - 4.0 * a.v * c.v;
+    if (discrim < 0.0) return false;
+    double root = std::sqrt(discrim);
+    EF fr((float)root, MACHINE_EPSILON * (float)root);
+    EF q = b.v < 0.0f ? efMul(EF(-0.5f), efSub(b, fr)) : efMul(EF(-0.5f), efAdd(b, fr));
+    *t0 = efDiv(q, a);
+    *t1 = efDiv(c, q);
+    if (t0->v > t1->v) std::swap(*t0, *t1);
+    return true;
+}
+
+// Sphere3F.intersectionT for a full sphere centred at the object origin
+// (M/geometry/shape/Sphere3F.java intersectionT/intersection).  The ray is
+// translated into object space (the primitive's transform is a translation).
+float sphereT(const Prim& p, const Ray& ray, float tMin, float tMax) {
+    V3 c(p.data[0], p.data[1], p.data[2]);
+    float radius = p.data[3];
+    V3 o = ray.o - c;
+    EF ox(o.x), oy(o.y), oz(o.z), dx(ray.d.x), dy(ray.d.y), dz(ray.d.z);
+    EF a = efAdd(efAdd(efMul(dx, dx), efMul(dy, dy)), efMul(dz, dz));
+    EF b = efMul(EF(2.0f), efAdd(efAdd(efMul(dx, ox), efMul(dy, oy)), efMul(dz, oz)));
+    EF cc = efSub(efAdd(efAdd(efMul(ox, ox), efMul(oy, oy)), efMul(oz, oz)), efMul(EF(radius), EF(radius)));
+    EF t0(0.0f), t1(0.0f);
+    if (!efQuadratic(a, b, cc, &t0, &t1)) return NAN;
+    if (t0.hi > tMax || t1.lo <= tMin) return NAN;
+    EF th = t0;
+    if (th.lo <= 0.0f) {
+        th = t1;
+        if (th.hi > tMax) return NAN;
+    }
+    return th.v;
+}
+
+// Rectangle3F.intersectionT, M/geometry/shape/Rectangle3F.java (plane then
+// edge projections).
+float rectangleT(const Prim& p, const Ray& ray, float tMin, float tMax) {
+    V3 a(p.data[0], p.data[1], p.data[2]), b(p.data[3], p.data[4], p.data[5]), c(p.data[6], p.data[7], p.data[8]);
+    V3 n = p.rectNormal;
+    float dp = dot(n, ray.d);
+    if (isZero(dp)) return NAN;
+    float t = dot(a - ray.o, n) / dp;
+    if (t <= tMin || t >= tMax) return NAN;
+    V3 ab = b - a, bc = c - b;
+    V3 ap = (ray.o + ray.d * t) - a;
+    float dAB = dot(ap, normalize(ab)), dBC = dot(ap, normalize(bc));
+    if (dAB >= 0.0f && dAB <= length(ab) && dBC >= 0.0f && dBC <= length(bc)) return t;
+    return NAN;
+}
+
+// Plane3F.intersectionT (z = 0 plane in object space), M/geometry/shape/
+// Plane3F.java; in world space the plane is (origin, w).
+float planeT(const Prim& p, const Ray& ray, float tMin, float tMax) {
+    V3 o(p.data[0], p.data[1], p.data[2]), w(p.data[9], p.data[10], p.data[11]);
+    float dp = dot(w, ray.d);
+    if (isZero(dp)) return NAN;
+    float t = dot(o - ray.o, w) / dp;
+    if (t > tMin && t < tMax) return t;
+    return NAN;
+}
+
+float analyticT(const Prim& p, const Ray& ray, float tMin, float tMax) {
+    switch (p.shape) {
+        case DFR_SHAPE_SPHERE: return sphereT(p, ray, tMin, tMax);
+        case DFR_SHAPE_RECTANGLE: return rectangleT(p, ray, tMin, tMax);
+        case DFR_SHAPE_PLANE: return planeT(p, ray, tMin, tMax);
+        default: return NAN;
+    }
+}
+
+// Scene.intersection (M/scene/Scene.java:874-900) closest hit: every
+// primitive tested with the shrinking tMax; meshes through a BVH
+// (LeafBVHNode3F/TreeBVHNode3F visit order does not change the closest hit).
+bool sceneClosest(const Scene& sc, const Ray& ray, float tMin, float tMax, Hit* hit, bool anyHit) {
+    bool found = false;
+    float best = tMax;
+    for (int pi : sc.analytic) {
+        float t = analyticT(sc.prims[pi], ray, tMin, best);
+        if (!std::isnan(t) && t < best) {
+            best = t; hit->t = t; hit->prim = pi; hit->tri = -1; found = true;
+            if (anyHit) return true;
+        }
+    }
+    if (!sc.nodes.empty()) {
+        double o[3] = {ray.o.x, ray.o.y, ray.o.z};
+        double inv[3] = {1.0 / (double)ray.d.x, 1.0 / (double)ray.d.y, 1.0 / (double)ray.d.z};
+        int stack[128]; int sp = 0; stack[sp++] = 0;
+        while (sp) {
+            const BvhNode& nd = sc.nodes[stack[--sp]];
+            if (!boxHit(nd, o, inv, tMin, best)) continue;
+            if (nd.left < 0) {
+                for (int64_t i = nd.first; i < nd.first + nd.count; i++) {
+                    int64_t ti = sc.triOrder[i];
+                    float t = triangleT(sc.tris[ti], ray, tMin, best);
+                    if (!std::isnan(t) && t < best) {
+                        best = t; hit->t = t; hit->prim = sc.tris[ti].prim; hit->tri = ti; found = true;
+                        if (anyHit) return true;
+                    } else if (!std::isnan(t) && t == best && found && hit->tri >= 0 && ti < hit->tri) {
+                        hit->tri = ti; hit->prim = sc.tris[ti].prim;  // deterministic tie break
+                    }
+                }
+            } else {
+                stack[sp++] = nd.right;
+                stack[sp++] = nd.left;
+            }
+        }
+    }
+    return found;
+}
+
+// Surface data of the winning shape in world space.  Triangle3F.java
+// doCreateSurfaceIntersection / doCreateBarycentricCoordinates (:1075-1150);
+// SurfaceIntersection3F.transform flips S.w to agree with G.w
+// (M/geometry/SurfaceIntersection3F.java:372-392).
+Surface makeSurface(const Scene& sc, const Ray& ray, const Hit& h) {
+    Surface s;
+    s.t = h.t; s.prim = h.prim; s.tri = h.tri; s.rayDir = ray.d;
+    s.p = ray.o + ray.d * h.t;
+    s.tu = s.tv = 0.0f;
+    const Prim& p = sc.prims[h.prim];
+    if (h.tri >= 0) {
+        const Tri& tr = sc.tris[h.tri];
+        V3 eAB = tr.b - tr.a, eCA = tr.a - tr.c;
+        V3 d1 = cross(eAB, eCA);
+        float det = dot(ray.d, d1);
+        float rdet = 1.0f / det;
+        V3 d2 = tr.a - ray.o;
+        V3 d3 = cross(d2, ray.d);
+        float u = dot(d3, eCA) * rdet, v = dot(d3, eAB) * rdet, w = 1.0f - u - v;
+        // barycentric (w, u, v) weights A, B, C
+        s.tu = tr.ua * w + tr.ub * u + tr.uc * v;
+        s.tv = tr.va * w + tr.vb * u + tr.vc * v;
+        s.g = onbFromW(tr.ng);
+        V3 ns = normalize(tr.na * w + tr.nb * u + tr.nc * v);
+        float cax = tr.ua - tr.uc, cay = tr.va - tr.vc, cbx = tr.ub - tr.uc, cby = tr.vb - tr.vc;
+        float udet = cax * cby - cay * cbx;
+        if (isZero(udet)) {
+            s.s = onbFromW(ns);
+        } else {
+            float r = 1.0f / udet;
+            V3 ca = tr.a - tr.c, cb = tr.b - tr.c;
+            V3 vv((-cbx * ca.x + cax * cb.x) * r, (-cbx * ca.y + cax * cb.y) * r, (-cbx * ca.z + cax * cb.z) * r);
+            s.s = onbFromWV(ns, vv);
+        }
+    } else if (p.shape == DFR_SHAPE_SPHERE) {
+        V3 c(p.data[0], p.data[1], p.data[2]);
+        float radius = p.data[3];
+        V3 ph = (ray.o - c) + ray.d * h.t;
+        float sc_ = radius / length(ph);
+        ph = ph * sc_;
+        if (ph.x == 0.0f && ph.y == 0.0f) ph.x = 1.0e-5f * radius;
+        float phiMax = 2.0f * PI;
+        float theta = facos(std::min(std::max(ph.z / radius, -1.0f), 1.0f));
+        float zr = fsqrt(ph.x * ph.x + ph.y * ph.y);
+        float cphi = ph.x / zr, sphi = ph.y / zr;
+        V3 dpdu(-phiMax * ph.y, phiMax * ph.x, 0.0f);
+        V3 dpdv = V3(ph.z * cphi, ph.z * sphi, -radius * fsin(theta)) * (0.0f - PI);  // thetaMax - thetaMin = 0 - pi
+        V3 n = normalize(cross(dpdu, dpdv));
+        // object->world normalises each basis vector (OrthonormalBasis33F.transformTranspose)
+        s.g.w = normalize(n); s.g.v = normalize(dpdv); s.g.u = normalize(dpdu);
+        s.s = s.g;
+        s.p = ph + c;
+        float phi = fatan2(ph.y, ph.x); if (phi < 0.0f) phi += 2.0f * PI;
+        s.tu = phi / phiMax; s.tv = (theta - PI) / (0.0f - PI);
+    } else if (p.shape == DFR_SHAPE_RECTANGLE) {
+        s.g = onbFromW(p.rectNormal);
+        s.s = s.g;
+    } else {  // plane: G = (w, v, u) of the world-space frame
+        s.g.u = normalize(V3(p.data[3], p.data[4], p.data[5]));
+        s.g.v = normalize(V3(p.data[6], p.data[7], p.data[8]));
+        s.g.w = normalize(V3(p.data[9], p.data[10], p.data[11]));
+        s.s = s.g;
+    }
+    if (dot(s.s.w, s.g.w) < 0.0f) s.s.w = -s.s.w;
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Sampling helpers (M/geometry/SampleGeneratorF.java)
+// ---------------------------------------------------------------------------
+void diskConcentric(float u, float v, float* x, float* y) {  // :129-154
+    if (isZero(u) && isZero(v)) { *x = 0; *y = 0; return; }
+    float a = u * 2.0f - 1.0f, b = v * 2.0f - 1.0f;
+    if (a * a > b * b) {
+        float phi = (PI / 4.0f) * (b / a), r = a;
+        *x = r * fcos(phi); *y = r * fsin(phi);
+    } else {
+        float phi = (PI / 2.0f) - (PI / 4.0f) * (a / b), r = b;
+        *x = r * fcos(phi); *y = r * fsin(phi);
+    }
+}
+V3 hemisphereCosine(float u, float v) {  // :286-294
+    float x, y; diskConcentric(u, v, &x, &y);
+    return V3(x, y, fsqrt(std::max(0.0f, 1.0f - x * x - y * y)));
+}
+V3 hemisphereUniform(float u, float v) {  // :429-439
+    float ct = u, st = fsqrt(std::max(0.0f, 1.0f - ct * ct)), phi = 2.0f * PI * v;
+    return V3(st * fcos(phi), st * fsin(phi), ct);
+}
+float powerHeuristic(float a, float b) {  // :548-553
+    return a * a / (a * a + b * b);
+}
+
+// ---------------------------------------------------------------------------
+// Fresnel (M/scene/fresnel/*)
+// ---------------------------------------------------------------------------
+float fresnelDielectric(float cosThetaI, float etaI, float etaT) {  // DielectricFresnel.java:136-159
+    float c = std::min(std::max(cosThetaI, -1.0f), 1.0f);
+    bool entering = c > 0.0f;
+    float ci = entering ? c : std::fabs(c);
+    float ei = entering ? etaI : etaT, et = entering ? etaT : etaI;
+    float si = fsqrt(std::max(0.0f, 1.0f - ci * ci));
+    float st = ei / et * si;
+    if (st >= 1.0f) return 1.0f;
+    float ct = fsqrt(std::max(0.0f, 1.0f - st * st));
+    float rpa = ((et * ci) - (ei * ct)) / ((et * ci) + (ei * ct));
+    float rpe = ((ei * ci) - (et * ct)) / ((ei * ci) + (et * ct));
+    return (rpa * rpa + rpe * rpe) / 2.0f;
+}
+float conductor1(float cosThetaI, float etaI, float etaT, float k) {  // ConductorFresnel.java (pbrt FrConductor)
+    float c = std::min(std::max(cosThetaI, -1.0f), 1.0f);
+    float eta = etaT / etaI, etak = k / etaI;
+    float c2 = c * c, s2 = 1.0f - c2;
+    float eta2 = eta * eta, etak2 = etak * etak;
+    float t0 = eta2 - etak2 - s2;
+    float a2b2 = fsqrt(t0 * t0 + eta2 * etak2 * 4.0f);
+    float t1 = a2b2 + c2;
+    float a = fsqrt((a2b2 + t0) * 0.5f);
+    float t2 = a * (c * 2.0f);
+    float rs = (t1 - t2) / (t1 + t2);
+    float t3 = c2 * a2b2 + s2 * s2;
+    float t4 = t2 * s2;
+    float rp = rs * (t3 - t4) / (t3 + t4);
+    return 0.5f * (rp + rs);
+}
+float schlickWeight(float c) {  // M/scene/fresnel/Schlick.java
+    float m = std::min(std::max(1.0f - c, 0.0f), 1.0f);
+    return (m * m) * (m * m) * m;
+}
+inline float lerpf(float a, float b, float t) { return (1.0f - t) * a + t * b; }
+inline C3 blend(C3 a, C3 b, float t) { return C3(lerpf(a.r, b.r, t), lerpf(a.g, b.g, t), lerpf(a.b, b.b, t)); }
+
+// ---------------------------------------------------------------------------
+// Microfacet: TrowbridgeReitzMicrofacetDistribution (visible-area sampling on,
+// separable model as given), M/scene/microfacet/*.java
+// ---------------------------------------------------------------------------
+struct TR {
+    float ax, ay;
+    bool separable;
+    TR() : ax(0.001f), ay(0.001f), separable(false) {}
+    TR(float x, float y, bool sep) : ax(std::max(x, 0.001f)), ay(std::max(y, 0.001f)), separable(sep) {}
+    float D(V3 h) const {
+        float t2 = tanThetaSquaredRef(h);
+        if (std::isinf(t2)) return 0.0f;
+        float c4 = cosThetaQuartic(h);
+        float e = (cosPhiSquared(h) / (ax * ax) + sinPhiSquared(h) / (ay * ay)) * t2;
+        return 1.0f / (PI * ax * ay * c4 * (1.0f + e) * (1.0f + e));
+    }
+    float lambda(V3 w) const {
+        float ta = tanThetaAbs(w);
+        if (std::isinf(ta)) return 0.0f;
+        float alpha = fsqrt(cosPhiSquared(w) * ax * ax + sinPhiSquared(w) * ay * ay);
+        float a2 = (alpha * ta) * (alpha * ta);
+        return (-1.0f + fsqrt(1.0f + a2)) / 2.0f;
+    }
+    float G1(V3 w) const { return 1.0f / (1.0f + lambda(w)); }
+    float G(V3 wo, V3 wi) const {
+        return separable ? G1(wo) * G1(wi) : 1.0f / (1.0f + lambda(wo) + lambda(wi));
+    }
+    float pdf(V3 wo, V3 h) const {  // visible-area form, MicrofacetDistribution.java
+        return D(h) * G1(wo) * std::fabs(dot(wo, h)) / cosThetaAbs(wo);
+    }
+    static void slope(float ct, float u, float v, float* sx, float* sy) {  // doComputeSlope
+        if (ct > 0.9999f) {
+            float r = fsqrt(u / (1.0f - u)), phi = 2.0f * PI * v;
+            *sx = r * fcos(phi); *sy = r * fsin(phi); return;
+        }
+        float st = fsqrt(std::max(0.0f, 1.0f - ct * ct));
+        float tt = st / ct;
+        float a = 2.0f / (1.0f + fsqrt(1.0f + 1.0f / ((1.0f / tt) * (1.0f / tt))));
+        float b = 2.0f * u / a - 1.0f;
+        float c = std::min(1.0f / (b * b - 1.0f), 1.0e10f);
+        float d = tt;
+        float e = fsqrt(std::max(d * d * c * c - (b * b - d * d) * c, 0.0f));
+        float f = d * c - e, g = d * c + e;
+        float hs = v > 0.5f ? 1.0f : -1.0f;
+        float i = v > 0.5f ? 2.0f * (v - 0.5f) : 2.0f * (0.5f - v);
+        float j = (i * (i * (i * 0.27385f - 0.73369f) + 0.46341f)) / (i * (i * (i * 0.093073f + 0.309420f) - 1.0f) + 0.597999f);
+        float x = (b < 0.0f || g > 1.0f / tt) ? f : g;
+        *sx = x; *sy = hs * j * fsqrt(1.0f + x * x);
+    }
+    V3 sampleVis(V3 wi, float u, float v) const {  // doSample
+        V3 s = normalize(V3(wi.x * ax, wi.y * ay, wi.z));
+        float sx, sy; slope(cosTheta(s), u, v, &sx, &sy);
+        float x = -((cosPhi(s) * sx - sinPhi(s) * sy) * ax);
+        float y = -((sinPhi(s) * sx + cosPhi(s) * sy) * ay);
+        return normalize(V3(x, y, 1.0f));
+    }
+    V3 sampleH(V3 wo, float u, float v) const {
+        return wo.z >= 0.0f ? sampleVis(wo, u, v) : -sampleVis(-wo, u, v);
+    }
+};
+
+// M/scene/microfacet/MicrofacetDistribution.java convertRoughnessToAlpha
+float roughnessToAlpha(float r) {
+    float x = std::max(r, 1.0e-3f), y = flog(x);
+    return 1.62142f + 0.819955f * y + 0.1734f * y * y + 0.0171201f * y * y * y + 0.000640711f * y * y * y * y;
+}
+
+// ---------------------------------------------------------------------------
+// BxDFs (M/scene/bxdf/*)
+// ---------------------------------------------------------------------------
+enum : uint32_t { F_REFL = 1, F_TRANS = 2, F_DIFF = 4, F_GLOSSY = 8, F_SPEC = 16 };
+const uint32_t ALL = 31, ALL_EXCEPT_SPECULAR = 15;
+enum LobeKind { L_LAMBERT, L_OREN, L_SPEC_R, L_FRESNEL_SPEC, L_TS_R, L_TS_T,
+                L_DISNEY_DIFFUSE, L_DISNEY_RETRO, L_DISNEY_SHEEN, L_DISNEY_CLEARCOAT };
+enum FresnelKind { FR_CONST, FR_DIELECTRIC, FR_CONDUCTOR, FR_DISNEY };
+
+struct Lobe {
+    int kind;
+    uint32_t type;
+    C3 R, T;
+    float A, B;          // Oren-Nayar
+    float etaA, etaB;    // dielectric pairs
+    int fresnel;
+    float frEtaI, frEtaT;        // dielectric fresnel
+    C3 condEta, condK;           // conductor
+    C3 disneyR0; float disneyEta, disneyMetallic;
+    TR md;
+    float roughness, gloss, weight;
+};
+inline bool matches(uint32_t lobe, uint32_t t) { return (lobe & t) == lobe; }
+
+C3 fresnelEval(const Lobe& l, float cosI) {
+    switch (l.fresnel) {
+        case FR_CONST: return C3(1.0f);
+        case FR_DIELECTRIC: return C3(fresnelDielectric(cosI, l.frEtaI, l.frEtaT));
+        case FR_CONDUCTOR: {
+            float c = std::fabs(cosI);
+            return C3(conductor1(c, 1.0f, l.condEta.r, l.condK.r), conductor1(c, 1.0f, l.condEta.g, l.condK.g),
+                      conductor1(c, 1.0f, l.condEta.b, l.condK.b));
+        }
+        default: {  // DisneyFresnel.java:67-69
+            C3 die(fresnelDielectric(cosI, 1.0f, l.disneyEta));
+            return blend(die, blend(l.disneyR0, C3(1.0f), schlickWeight(cosI)), l.disneyMetallic);
+        }
+    }
+}
+
+float gtr1(float c, float alpha) {  // DisneyClearCoatBRDF.doGTR1
+    float a = alpha * alpha;
+    return (a - 1.0f) / (PI * flog(a) * (1.0f + (a - 1.0f) * c * c));
+}
+float smithGGX(float c, float alpha) {
+    float a = alpha * alpha, b = c * c;
+    return 1.0f / (c + fsqrt(a + b - a * b));
+}
+
+C3 lobeF(const Lobe& l, V3 wo, V3 wi) {
+    switch (l.kind) {
+        case L_LAMBERT: return l.R * PI_RECIPROCAL;
+        case L_OREN: {  // OrenNayarBRDF.java
+            float cI = cosThetaAbs(wi), cO = cosThetaAbs(wo);
+            float sI = sinTheta(wi), sO = sinTheta(wo);
+            float maxCos = (sI > 1.0e-4f && sO > 1.0e-4f) ? std::max(0.0f, cosPhi(wi) * cosPhi(wo) + sinPhi(wi) * sinPhi(wo)) : 0.0f;
+            float sinA = cI > cO ? sO : sI;
+            float tanB = cI > cO ? sI / cI : sO / cO;
+            return l.R * PI_RECIPROCAL * (l.A + l.B * maxCos * sinA * tanB);
+        }
+        case L_TS_R: {  // TorranceSparrowBRDF.java
+            float cO = cosThetaAbs(wo), cI = cosThetaAbs(wi);
+            if (isZero(cO) || isZero(cI)) return C3(0.0f);
+            V3 h = wi + wo;
+            if (isZero(h.x) && isZero(h.y) && isZero(h.z)) return C3(0.0f);
+            h = normalize(h);
+            float d = l.md.D(h), g = l.md.G(wo, wi);
+            C3 f = fresnelEval(l, dot(wi, faceForward(h, V3(0, 0, 1))));
+            return (l.R * d * g) * f / (4.0f * cI * cO);
+        }
+        case L_TS_T: {  // TorranceSparrowBTDF.java
+            if (sameHemisphereZ(wo, wi)) return C3(0.0f);
+            float cI = cosTheta(wi), cO = cosTheta(wo);
+            if (isZero(cI) || isZero(cO)) return C3(0.0f);
+            float eta = cO > 0.0f ? l.etaB / l.etaA : l.etaA / l.etaB;
+            V3 h = normalize(wi * eta + wo);
+            if (h.z < 0.0f) h = -h;
+            float oh = dot(wo, h), ih = dot(wi, h);
+            if (oh * ih > 0.0f) return C3(0.0f);
+            C3 f(fresnelDielectric(oh, l.etaA, l.etaB));
+            float a = oh + eta * ih;
+            float b = 1.0f / eta;  // transport mode RADIANCE
+            float d = l.md.D(h), g = l.md.G(wo, wi);
+            return (C3(1.0f) - f) * l.T * std::fabs(d * g * eta * eta * std::fabs(ih) * std::fabs(oh) * b * b / (cI * cO * a * a));
+        }
+        case L_DISNEY_DIFFUSE: {
+            float fo = schlickWeight(cosThetaAbs(wo)), fi = schlickWeight(cosThetaAbs(wi));
+            float k = PI_RECIPROCAL * (1.0f - fo / 2.0f) * (1.0f - fi / 2.0f);
+            return C3(l.R.r * k, l.R.g * k, l.R.b * k);
+        }
+        case L_DISNEY_RETRO: {
+            V3 h = wi + wo;
+            if (isZero(h.x) && isZero(h.y) && isZero(h.z)) return C3(0.0f);
+            h = normalize(h);
+            float cd = dot(wi, h);
+            float fo = schlickWeight(cosThetaAbs(wo)), fi = schlickWeight(cosThetaAbs(wi));
+            float a = 2.0f * l.roughness * cd * cd;
+            float b = fo + fi + fo * fi * (a - 1.0f);
+            return C3(l.R.r * PI_RECIPROCAL * a * b, l.R.g * PI_RECIPROCAL * a * b, l.R.b * PI_RECIPROCAL * a * b);
+        }
+        case L_DISNEY_SHEEN: {
+            V3 h = wi + wo;
+            if (isZero(h.x) && isZero(h.y) && isZero(h.z)) return C3(0.0f);
+            h = normalize(h);
+            return l.R * schlickWeight(dot(wi, h));
+        }
+        case L_DISNEY_CLEARCOAT: {
+            V3 h = wi + wo;
+            if (isZero(h.x) && isZero(h.y) && isZero(h.z)) return C3(0.0f);
+            h = normalize(h);
+            float d = gtr1(cosThetaAbs(h), l.gloss);
+            float f = lerpf(0.04f, 1.0f, schlickWeight(dot(wo, h)));
+            float g = smithGGX(cosThetaAbs(wo), 0.25f) * smithGGX(cosThetaAbs(wi), 0.25f);
+            return C3(l.weight * g * f * d / 4.0f);
+        }
+        default: return C3(0.0f);  // specular lobes: delta
+    }
+}
+
+float lobePdf(const Lobe& l, V3 wo, V3 wi) {
+    switch (l.kind) {
+        case L_LAMBERT: case L_OREN: case L_DISNEY_DIFFUSE: case L_DISNEY_RETRO: case L_DISNEY_SHEEN:
+            return sameHemisphereZ(wo, wi) ? cosThetaAbs(wi) * PI_RECIPROCAL : 0.0f;
+        case L_TS_R: {
+            if (!sameHemisphereZ(wo, wi)) return 0.0f;
+            V3 h = normalize(wo + wi);
+            return l.md.pdf(wo, h) / (4.0f * dot(wo, h));
+        }
+        case L_TS_T: {
+            if (sameHemisphereZ(wo, wi)) return 0.0f;
+            float eta = cosTheta(wo) > 0.0f ? l.etaB / l.etaA : l.etaA / l.etaB;
+            V3 h = normalize(wi * eta + wo);
+            float oh = dot(wo, h), ih = dot(wi, h);
+            if (oh * ih > 0.0f) return 0.0f;
+            float a = oh + eta * ih;
+            float b = std::fabs((eta * eta * ih) / (a * a));
+            return l.md.pdf(wo, h) * b;
+        }
+        case L_DISNEY_CLEARCOAT: {
+            if (!sameHemisphereZ(wo, wi)) return 0.0f;
+            V3 h = wi + wo;
+            if (isZero(h.x) && isZero(h.y) && isZero(h.z)) return 0.0f;
+            h = normalize(h);
+            float ch = cosThetaAbs(h);
+            return gtr1(ch, l.gloss) * ch / (4.0f * dot(wo, h));
+        }
+        default: return 0.0f;
+    }
+}
+
+struct LobeSample { C3 f; V3 wi; float pdf; uint32_t type; };
+
+bool lobeSample(const Lobe& l, V3 wo, float u, float v, LobeSample* out) {
+    out->type = l.type;
+    switch (l.kind) {
+        case L_LAMBERT: case L_OREN: case L_DISNEY_DIFFUSE: case L_DISNEY_RETRO: case L_DISNEY_SHEEN: {
+            V3 wi = faceForwardZ(wo, hemisphereCosine(u, v));
+            out->wi = wi; out->f = lobeF(l, wo, wi); out->pdf = lobePdf(l, wo, wi);
+            return true;
+        }
+        case L_SPEC_R: {  // SpecularBRDF.java
+            V3 wi(-wo.x, -wo.y, wo.z);
+            out->wi = wi;
+            out->f = (fresnelEval(l, cosTheta(wi)) * l.R) / cosThetaAbs(wi);
+            out->pdf = 1.0f;
+            return true;
+        }
+        case L_FRESNEL_SPEC: {  // FresnelSpecularBXDF.java
+            float refl = fresnelDielectric(cosTheta(wo), l.etaA, l.etaB);
+            if (u < refl) {
+                V3 wi(-wo.x, -wo.y, wo.z);
+                out->type = F_SPEC | F_REFL;
+                out->wi = wi; out->f = (l.R * refl) / cosThetaAbs(wi); out->pdf = refl;
+                return true;
+            }
+            bool entering = cosTheta(wo) > 0.0f;
+            float ei = entering ? l.etaA : l.etaB, et = entering ? l.etaB : l.etaA;
+            V3 wi;
+            if (!refraction(wo, faceForward(V3(0, 0, 1), wo), ei / et, &wi)) return false;
+            out->type = F_SPEC | F_TRANS;
+            out->wi = wi;
+            out->f = ((l.T * (1.0f - refl)) * ((ei * ei) / (et * et))) / cosThetaAbs(wi);
+            out->pdf = 1.0f - refl;
+            return true;
+        }
+        case L_TS_R: {
+            if (isZero(wo.z)) return false;
+            V3 h = l.md.sampleH(wo, u, v);
+            if (dot(wo, h) < 0.0f) return false;
+            V3 wi = reflectionPBRT(wo, h);
+            if (!sameHemisphereZ(wo, wi)) return false;
+            out->wi = wi; out->f = lobeF(l, wo, wi);
+            out->pdf = l.md.pdf(wo, h) / (4.0f * dot(wo, h));
+            return true;
+        }
+        case L_TS_T: {
+            if (isZero(wo.z)) return false;
+            V3 h = l.md.sampleH(wo, u, v);
+            if (dot(wo, h) < 0.0f) return false;
+            float eta = cosTheta(wo) > 0.0f ? l.etaA / l.etaB : l.etaB / l.etaA;
+            V3 wi;
+            if (!refraction(wo, h, eta, &wi)) return false;
+            out->wi = wi; out->f = lobeF(l, wo, wi); out->pdf = lobePdf(l, wo, wi);
+            return true;
+        }
+        case L_DISNEY_CLEARCOAT: {
+            if (isZero(wo.z)) return false;
+            float a2 = l.gloss * l.gloss;
+            float ct = fsqrt(std::max(0.0f, (1.0f - fpow(a2, 1.0f - u)) / (1.0f - a2)));
+            float st = fsqrt(std::max(0.0f, 1.0f - ct * ct));
+            float phi = 2.0f * PI * v;
+            V3 hs(st * fcos(phi), st * fsin(phi), ct);
+            V3 h = sameHemisphereZ(wo, hs) ? hs : -hs;
+            V3 wi = reflectionPBRT(wo, h);
+            if (!sameHemisphereZ(wo, wi)) return false;
+            out->wi = wi; out->f = lobeF(l, wo, wi); out->pdf = lobePdf(l, wo, wi);
+            return true;
+        }
+        default: return false;
+    }
+}
+
+// BSDF (M/scene/BSDF.java)
+struct BSDF {
+    Lobe lobes[8];
+    int n = 0;
+    float eta = 1.0f;
+    ONB s;
+    V3 ng, woWorld, woLocal;
+    void add(const Lobe& l) { if (n < 8) lobes[n++] = l; }
+    int countSpecular(bool spec) const {
+        int c = 0;
+        for (int i = 0; i < n; i++) if (((lobes[i].type & F_SPEC) != 0) == spec) c++;
+        return c;
+    }
+    C3 f(uint32_t t, V3 wiWorld) const {  // :272-300
+        V3 wi = toLocal(wiWorld, s);
+        bool reflect = dot(wiWorld, ng) * dot(woWorld, ng) > 0.0f;
+        C3 r;
+        for (int i = 0; i < n; i++) {
+            uint32_t lt = lobes[i].type;
+            if (matches(lt, t) && ((reflect && (lt & F_REFL)) || (!reflect && (lt & F_TRANS))))
+                r = r + lobeF(lobes[i], woLocal, wi);
+        }
+        return r;
+    }
+    float pdf(uint32_t t, V3 wiWorld) const {  // :468-518
+        if (n == 0) return 0.0f;
+        V3 wi = toLocal(wiWorld, s);
+        if (isZero(woLocal.z)) return 0.0f;
+        float p = 0.0f; int m = 0;
+        for (int i = 0; i < n; i++) {
+            if (matches(lobes[i].type, t)) { m++; p += lobePdf(lobes[i], woLocal, wi); }
+        }
+        if (m > 1) p /= m;
+        return p;
+    }
+    bool sample(uint32_t t, float u, float v, LobeSample* out) const {  // :302-379
+        if (isZero(woLocal.z)) return false;
+        int idx[8]; int m = 0;
+        for (int i = 0; i < n; i++) if (matches(lobes[i].type, t)) idx[m++] = i;
+        if (m == 0) return false;
+        int k = std::min((int)std::floor(u * m), m - 1);
+        const Lobe& L = lobes[idx[k]];
+        float ur = std::min(u * m - k, 0.99999994f);
+        LobeSample ls;
+        if (!lobeSample(L, woLocal, ur, v, &ls)) return false;
+        V3 wiW = toWorld(ls.wi, s);
+        C3 f = ls.f;
+        float pdf = ls.pdf;
+        if (!(L.type & F_SPEC)) {
+            bool reflect = dot(wiW, ng) * dot(woWorld, ng) > 0.0f;
+            f = C3(0.0f);
+            for (int i = 0; i < m; i++) {
+                const Lobe& B = lobes[idx[i]];
+                if (i != k) pdf += lobePdf(B, woLocal, ls.wi);
+                if ((reflect && (B.type & F_REFL)) || (!reflect && (B.type & F_TRANS))) f = f + lobeF(B, woLocal, ls.wi);
+            }
+        }
+        if (m > 1) pdf /= m;
+        out->f = f; out->wi = wiW; out->pdf = pdf; out->type = ls.type;
+        return true;
+    }
+};
+
+C3 col(const float* c) { return C3(c[0], c[1], c[2]); }
+
+// Material.computeScatteringFunctions (transport RADIANCE, multiple lobes on).
+bool buildBSDF(const dfr_material& m, const Surface& si, BSDF* b) {
+    b->n = 0; b->eta = 1.0f;
+    b->s = si.s; b->ng = si.g.w;
+    b->woWorld = -si.rayDir;
+    b->woLocal = toLocal(b->woWorld, si.s);
+    switch (m.type) {
+        case DFR_MAT_MATTE: {  // MatteMaterial.java:285
+            C3 kd = saturate0(col(m.kd));
+            float deg = std::min(std::max(m.angle, 0.0f), 90.0f);
+            Lobe l{}; l.type = F_DIFF | F_REFL; l.R = kd;
+            if (isZero(deg)) { l.kind = L_LAMBERT; }
+            else {
+                float r = deg * (PI / 180.0f), r2 = r * r;
+                l.kind = L_OREN; l.A = 1.0f - (r2 / (2.0f * (r2 + 0.33f))); l.B = 0.45f * r2 / (r2 + 0.09f);
+            }
+            b->add(l);
+            return true;
+        }
+        case DFR_MAT_MIRROR: {  // MirrorMaterial.java:235
+            C3 kr = saturate0(col(m.kr));
+            if (isBlack(kr)) return false;
+            Lobe l{}; l.kind = L_SPEC_R; l.type = F_SPEC | F_REFL; l.R = kr; l.fresnel = FR_CONST;
+            b->add(l);
+            return true;
+        }
+        case DFR_MAT_GLASS: {  // GlassMaterial.java:501
+            C3 kr = saturate0(col(m.kr)), kt = saturate0(col(m.kt));
+            float eta = m.eta[0];
+            if (isBlack(kr) && isBlack(kt)) return false;
+            b->eta = eta;
+            if (isZero(m.roughness_u) && isZero(m.roughness_v)) {
+                Lobe l{}; l.kind = L_FRESNEL_SPEC; l.type = F_SPEC | F_REFL | F_TRANS;
+                l.R = kr; l.T = kt; l.etaA = 1.0f; l.etaB = eta;
+                b->add(l);
+                return true;
+            }
+            float ru = m.remap_roughness ? roughnessToAlpha(m.roughness_u) : m.roughness_u;
+            float rv = m.remap_roughness ? roughnessToAlpha(m.roughness_v) : m.roughness_v;
+            TR md(ru, rv, false);
+            if (!isBlack(kr)) {
+                Lobe l{}; l.kind = L_TS_R; l.type = F_GLOSSY | F_REFL; l.R = kr; l.md = md;
+                l.fresnel = FR_DIELECTRIC; l.frEtaI = 1.0f; l.frEtaT = eta;
+                b->add(l);
+            }
+            if (!isBlack(kt)) {
+                Lobe l{}; l.kind = L_TS_T; l.type = F_GLOSSY | F_TRANS; l.T = kt; l.md = md; l.etaA = 1.0f; l.etaB = eta;
+                b->add(l);
+            }
+            return true;
+        }
+        case DFR_MAT_METAL: {  // MetalMaterial.java:436
+            float ru = m.remap_roughness ? roughnessToAlpha(m.roughness_u) : m.roughness_u;
+            float rv = m.remap_roughness ? roughnessToAlpha(m.roughness_v) : m.roughness_v;
+            Lobe l{}; l.kind = L_TS_R; l.type = F_GLOSSY | F_REFL; l.R = C3(1.0f);
+            l.fresnel = FR_CONDUCTOR; l.condEta = col(m.eta); l.condK = col(m.k); l.md = TR(ru, rv, false);
+            b->add(l);
+            return true;
+        }
+        case DFR_MAT_PLASTIC: {  // PlasticMaterial.java:389
+            C3 kd = saturate0(col(m.kd)), ks = saturate0(col(m.ks));
+            if (!isBlack(kd)) { Lobe l{}; l.kind = L_LAMBERT; l.type = F_DIFF | F_REFL; l.R = kd; b->add(l); }
+            if (!isBlack(ks)) {
+                float r = m.remap_roughness ? roughnessToAlpha(m.roughness_u) : m.roughness_u;
+                Lobe l{}; l.kind = L_TS_R; l.type = F_GLOSSY | F_REFL; l.R = ks; l.md = TR(r, r, false);
+                l.fresnel = FR_DIELECTRIC; l.frEtaI = 1.5f; l.frEtaT = 1.0f;
+                b->add(l);
+            }
+            return b->n > 0;
+        }
+        case DFR_MAT_DISNEY: {  // DisneyMaterial.java:1121 (thin = false, scatter distance = 0)
+            float aniso = m.disney_anisotropic, cc = m.disney_clearcoat, eta = m.eta[0];
+            float metal = m.disney_metallic, rough = m.disney_roughness, sheen = m.disney_sheen;
+            float stint = m.disney_specular_tint, strans = m.disney_spec_trans;
+            C3 color = saturate0(col(m.kd));
+            float lum = relativeLuminance(color);
+            C3 tint = lum > 0.0f ? color / lum : C3(1.0f);
+            C3 csheen = sheen > 0.0f ? blend(C3(1.0f), tint, m.disney_sheen_tint) : C3(0.0f);
+            float dw = (1.0f - metal) * (1.0f - strans);
+            if (dw > 0.0f) {
+                Lobe d{}; d.kind = L_DISNEY_DIFFUSE; d.type = F_DIFF | F_REFL; d.R = color * dw; b->add(d);
+                Lobe r{}; r.kind = L_DISNEY_RETRO; r.type = F_DIFF | F_REFL; r.R = color * dw; r.roughness = rough; b->add(r);
+                if (sheen > 0.0f) { Lobe s{}; s.kind = L_DISNEY_SHEEN; s.type = F_DIFF | F_REFL; s.R = csheen * (dw * sheen); b->add(s); }
+            }
+            float aspect = fsqrt(1.0f - aniso * 0.9f);
+            float axx = std::max(0.001f, rough * rough / aspect), ayy = std::max(0.001f, rough * rough * aspect);
+            TR md(axx, ayy, true);
+            float r0 = ((eta - 1.0f) * (eta - 1.0f)) / ((eta + 1.0f) * (eta + 1.0f));
+            C3 specR0 = blend(blend(C3(1.0f), tint, stint) * r0, color, metal);
+            Lobe s{}; s.kind = L_TS_R; s.type = F_GLOSSY | F_REFL; s.R = C3(1.0f); s.md = md;
+            s.fresnel = FR_DISNEY; s.disneyR0 = specR0; s.disneyEta = eta; s.disneyMetallic = metal;
+            b->add(s);
+            if (cc > 0.0f) {
+                Lobe c{}; c.kind = L_DISNEY_CLEARCOAT; c.type = F_GLOSSY | F_REFL;
+                c.gloss = lerpf(0.1f, 0.001f, m.disney_clearcoat_gloss); c.weight = cc; b->add(c);
+            }
+            if (strans > 0.0f) {
+                C3 ts(fsqrt(color.r) * strans, fsqrt(color.g) * strans, fsqrt(color.b) * strans);
+                Lobe t{}; t.kind = L_TS_T; t.type = F_GLOSSY | F_TRANS; t.T = ts; t.md = md; t.etaA = 1.0f; t.etaB = eta;
+                b->add(t);
+            }
+            return true;
+        }
+        default: return false;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lights
+// ---------------------------------------------------------------------------
+struct LightSample { C3 L; V3 p; V3 wi; float pdf; };
+
+}  // namespace
+
+#include "oracle_perez.inc"
+
+namespace {
+
+Scene::~Scene() { for (auto& l : lights) delete l.perez; }
+
+float triArea(const Tri& t) { return 0.5f * length(cross(t.b - t.a, t.c - t.a)); }
+
+// DiffuseAreaLight.sampleRadianceIncoming (M/scene/light/DiffuseAreaLight.java
+// :252-282) over Triangle3F.sample (M/geometry/shape/Triangle3F.java:174-199)
+// and Shape3F.sample(sample, surfaceIntersection) (M/geometry/Shape3F.java
+// :123-147).  Reference mode keeps Shape3F's pdf d^2/|cos| (the triangle's
+// 1/area factor is not applied); area mode multiplies it in.
+bool sampleLight(const Scene& sc, const Light& L, const Surface& si, float u, float v, float uSun, int pdfMode,
+                 LightSample* out) {
+    if (L.type == DFR_LIGHT_POINT) {  // PointLight.java:144-152
+        float d2 = lengthSquared(si.p - L.position);
+        out->L = L.radiance / d2;
+        out->p = L.position;
+        out->wi = normalize(L.position - si.p);
+        out->pdf = 1.0f;
+        return true;
+    }
+    if (L.type == DFR_LIGHT_PEREZ) return perezSample(*L.perez, si, u, v, uSun, out);
+    if (L.tri < 0) return false;
+    const Tri& tr = sc.tris[L.tri];
+    float su = fsqrt(u);
+    float b0 = 1.0f - su, b1 = v * su, b2 = 1.0f - b0 - b1;
+    V3 p = tr.a * b0 + tr.b * b1 + tr.c * b2;
+    V3 n = normalize(tr.na * b0 + tr.nb * b1 + tr.nc * b2);
+    V3 inc = p - si.p;
+    if (isZero(lengthSquared(inc))) return false;
+    V3 incN = normalize(inc);
+    float pdf = lengthSquared(p - si.p) / std::fabs(dot(n, -incN));
+    if (pdfMode == 1) pdf *= 1.0f / triArea(tr);
+    if (std::isinf(pdf) || std::isnan(pdf)) return false;
+    V3 wi = normalize(p - si.p);
+    if (pdf > 0.0f && (L.twoSided || dot(n, -wi) > 0.0f)) {
+        out->L = L.radiance; out->p = p; out->wi = wi; out->pdf = pdf;
+        return true;
+    }
+    return false;
+}
+
+// DiffuseAreaLight.evaluateProbabilityDensityFunctionRadianceIncoming ->
+// Shape3F.evaluateProbabilityDensityFunction (M/geometry/Shape3F.java:199-217).
+float lightPdf(const Scene& sc, const Light& L, const Surface& si, V3 wi) {
+    if (L.type == DFR_LIGHT_PEREZ) return perezPdf(*L.perez, wi);
+    if (L.type != DFR_LIGHT_DIFFUSE_AREA || L.tri < 0) return 0.0f;
+    const Tri& tr = sc.tris[L.tri];
+    Ray r{si.p, normalize(wi)};
+    float t = triangleT(tr, r, 0.001f, FLT_MAX);
+    if (std::isnan(t)) return 0.0f;
+    Hit h{t, tr.prim, L.tri};
+    Surface s2 = makeSurface(sc, r, h);
+    float pdf = lengthSquared(s2.p - si.p) / (std::fabs(dot(s2.s.w, -wi)) * triArea(tr));
+    if (std::isinf(pdf) || std::isnan(pdf)) return 0.0f;
+    return pdf;
+}
+
+// Intersection.evaluateRadianceEmitted (M/scene/Intersection.java:89-101)
+C3 emitted(const Scene& sc, const Surface& si, V3 dir) {
+    const Prim& p = sc.prims[si.prim];
+    if (p.areaLight >= 0) {
+        const Light& L = sc.lights[p.areaLight];
+        return (L.twoSided || dot(si.s.w, dir) > 0.0f) ? L.radiance : C3(0.0f);
+    }
+    return col(sc.mats[p.material].emission);
+}
+
+// Light.evaluateRadianceEmitted(ray): only the Perez sky emits on a miss.
+C3 emittedRay(const Light& L, V3 d) {
+    if (L.type == DFR_LIGHT_PEREZ) return perezSky(*L.perez, d);
+    return C3(0.0f);
+}
+
+bool sceneAny(const Scene& sc, const Ray& r, float tMin, float tMax) {
+    Hit h; return sceneClosest(sc, r, tMin, tMax, &h, true);
+}
+
+// Scene.checkLightVisibility (M/scene/Scene.java:1066-1090)
+bool visible(const Scene& sc, const Surface& si, int lightIndex, const LightSample& ls) {
+    const Light& L = sc.lights[lightIndex];
+    Ray r{si.p, normalize(ls.p - si.p)};
+    float tMax = std::fabs(length(si.p - ls.p)) + 0.001f;
+    if (L.type == DFR_LIGHT_DIFFUSE_AREA) {
+        Hit h;
+        if (sceneClosest(sc, r, 0.001f, tMax, &h, false)) return sc.prims[h.prim].areaLight == lightIndex;
+        return false;
+    }
+    return !sceneAny(sc, r, 0.001f, tMax);
+}
+
+// Scene.doEstimateDirectLight (M/scene/Scene.java:1530-1633), isSpecular false.
+C3 estimateDirect(const Scene& sc, const BSDF& bsdf, const Surface& si, int li, float ua, float va, float ub,
+                  float vb, float uSun, int pdfMode) {
+    const Light& L = sc.lights[li];
+    const uint32_t T = ALL_EXCEPT_SPECULAR;
+    C3 Ld;
+    LightSample ls;
+    bool have = sampleLight(sc, L, si, ua, va, uSun, pdfMode, &ls);
+    V3 n = si.s.w;
+    if (L.type == DFR_LIGHT_POINT) {
+        if (have && !isBlack(ls.L) && ls.pdf > 0.0f) {
+            C3 f = bsdf.f(T, ls.wi) * dotAbs(ls.wi, n);
+            if (!isBlack(f) && visible(sc, si, li, ls)) Ld = Ld + f * ls.L / ls.pdf;
+        }
+        return Ld;
+    }
+    if (have && !isBlack(ls.L) && ls.pdf > 0.0f) {
+        C3 f = bsdf.f(T, ls.wi) * dotAbs(ls.wi, n);
+        float sp = bsdf.pdf(T, ls.wi);
+        if (!isBlack(f) && visible(sc, si, li, ls)) {
+            float w = powerHeuristic(ls.pdf, sp);
+            Ld = Ld + f * ls.L * w / ls.pdf;
+        }
+    }
+    LobeSample bs;
+    if (bsdf.sample(T, ub, vb, &bs)) {
+        C3 f = bs.f * dotAbs(bs.wi, n);
+        bool spec = (bs.type & F_SPEC) != 0;
+        if (!isBlack(f) && bs.pdf > 0.0f) {
+            float w = 1.0f;
+            if (!spec) {
+                float lp = lightPdf(sc, L, si, bs.wi);
+                if (isZero(lp)) return Ld;
+                w = powerHeuristic(bs.pdf, lp);
+            }
+            Ray r{si.p, normalize(bs.wi)};
+            Hit h;
+            if (sceneClosest(sc, r, 0.001f, FLT_MAX, &h, false)) {
+                if (sc.prims[h.prim].areaLight == li) {
+                    Surface s2 = makeSurface(sc, r, h);
+                    C3 Li = emitted(sc, s2, -bs.wi);
+                    if (!isBlack(Li)) Ld = Ld + f * Li * w / bs.pdf;
+                }
+            } else {
+                C3 Li = emittedRay(L, r.d);
+                if (!isBlack(Li)) Ld = Ld + f * Li * w / bs.pdf;
+            }
+        }
+    }
+    return Ld;
+}
+
+// Camera.createPrimaryRay (M/scene/Camera.java:243-291) over the compiled
+// 19-float camera (CompiledCameraCache.java:158-197).
+bool primaryRay(const float* cam, float imageX, float imageY, float px, float py, Ray* out) {
+    float fovX = cam[DFR_CAMERA_FOV_X], fovY = cam[DFR_CAMERA_FOV_Y];
+    float resX = cam[DFR_CAMERA_RES_X], resY = cam[DFR_CAMERA_RES_Y];
+    V3 U(cam[3], cam[4], cam[5]), Vv(cam[6], cam[7], cam[8]), W(cam[9], cam[10], cam[11]);
+    V3 eye(cam[12], cam[13], cam[14]);
+    float cx = 2.0f * ((imageX + px) / (resX - 1.0f)) - 1.0f;
+    float cy = 2.0f * ((imageY + py) / (resY - 1.0f)) - 1.0f;
+    float wf = 1.0f;
+    if ((int)cam[DFR_CAMERA_LENS] == DFR_LENS_FISHEYE) {
+        float d = cx * cx + cy * cy;
+        if (d > 1.0f) return false;
+        wf = fsqrt(1.0f - d);
+    }
+    V3 onPlane = (U * (fovX * cx) + Vv * (fovY * cy)) + (eye + W * wf);
+    V3 onImage = eye + normalize(onPlane - eye) * cam[DFR_CAMERA_FOCAL];
+    out->o = eye;  // aperture radius 0 (pinhole); thin-lens origin offset not modelled
+    out->d = normalize(onImage - eye);
+    return true;
+}
+
+// Scene.radiancePathTracer (M/scene/Scene.java:451-636)
+C3 radiancePT(const Scene& sc, Ray ray, const Rng& rng, int maxBounce, int minRR, int pdfMode) {
+    C3 L, beta(1.0f);
+    bool specBounce = false;
+    float etaScale = 1.0f;
+    int passThrough = 0;
+    for (int bounce = 0;; bounce++) {
+        Hit h;
+        if (sceneClosest(sc, ray, 0.001f, FLT_MAX, &h, false)) {
+            Surface si = makeSurface(sc, ray, h);
+            V3 wo = -ray.d;
+            if (bounce == 0 || specBounce) L = L + beta * emitted(sc, si, wo);
+            if (bounce >= maxBounce) break;
+            BSDF bsdf;
+            if (!buildBSDF(sc.mats[sc.prims[h.prim].material], si, &bsdf)) {
+                ray = Ray{si.p, ray.d};
+                bounce--;
+                if (++passThrough > 256) break;  // guard against an endless chain of null surfaces
+                continue;
+            }
+            const uint32_t dim = DIM_BOUNCE + DIMS_PER_BOUNCE * (uint32_t)bounce;
+            if (bsdf.countSpecular(false) > 0 && !sc.lights.empty()) {
+                int n = (int)sc.lights.size();
+                int li = std::min((int)(rng.get(dim + 0) * n), n - 1);
+                C3 d = estimateDirect(sc, bsdf, si, li, rng.get(dim + 1), rng.get(dim + 2), rng.get(dim + 3),
+                                      rng.get(dim + 4), rng.get(dim + 5), pdfMode);
+                L = L + beta * (d / (1.0f / n));
+            }
+            LobeSample bs;
+            if (!bsdf.sample(ALL, rng.get(dim + 6), rng.get(dim + 7), &bs)) break;
+            if (isBlack(bs.f) || isZero(bs.pdf)) break;
+            beta = beta * (bs.f * dotAbs(bs.wi, si.s.w) / bs.pdf);
+            specBounce = (bs.type & F_SPEC) != 0;
+            if ((bs.type & F_TRANS) && (bs.type & F_SPEC))
+                etaScale *= dot(wo, si.g.w) > 0.0f ? bsdf.eta * bsdf.eta : 1.0f / (bsdf.eta * bsdf.eta);
+            ray = Ray{si.p, normalize(bs.wi)};
+            C3 rr = beta * etaScale;
+            if (maxc(rr) < 1.0f && bounce >= minRR) {
+                float q = std::max(0.05f, 1.0f - maxc(rr));
+                if (rng.get(dim + 8) < q) break;
+                beta = beta / (1.0f - q);
+            }
+        } else {
+            if (bounce == 0 || specBounce)
+                for (const Light& Lt : sc.lights) L = L + beta * emittedRay(Lt, ray.d);
+            break;
+        }
+    }
+    return L;
+}
+
+// Scene.radianceAmbientOcclusion (M/scene/Scene.java:272-326)
+C3 radianceAO(const Scene& sc, const Ray& ray, const Rng& rng, float maxDist, int samples) {
+    Hit h;
+    if (!sceneClosest(sc, ray, 0.001f, FLT_MAX, &h, false)) return C3(0.0f);
+    Surface si = makeSurface(sc, ray, h);
+    float acc = 0.0f;
+    for (int i = 0; i < samples; i++) {
+        V3 hs = hemisphereUniform(rng.get(2 + 2 * i), rng.get(3 + 2 * i));
+        V3 d = normalize(V3(hs.x * si.g.u.x + hs.y * si.g.v.x + hs.z * si.g.w.x,
+                            hs.x * si.g.u.y + hs.y * si.g.v.y + hs.z * si.g.w.y,
+                            hs.x * si.g.u.z + hs.y * si.g.v.z + hs.z * si.g.w.z));
+        Ray r{si.p, d};
+        if (maxDist > 0.0f) {
+            Hit h2;
+            if (sceneClosest(sc, r, 0.001f, FLT_MAX, &h2, false)) {
+                Surface s2 = makeSurface(sc, r, h2);
+                float t = length(s2.p - si.p);
+                float tc = std::min(std::max(t, 0.0f), maxDist);
+                acc += (tc - 0.0f) / (maxDist - 0.0f);
+            } else {
+                acc += 1.0f;
+            }
+        } else if (!sceneAny(sc, r, 0.001f, FLT_MAX)) {
+            acc += 1.0f;
+        }
+    }
+    float v = acc * (PI / samples);
+    v = v / PI;
+    return C3(v);
+}
+
+// Scene.radianceRayCaster (M/scene/Scene.java:675-718)
+C3 radianceRC(const Scene& sc, const Ray& ray) {
+    Hit h;
+    if (sceneClosest(sc, ray, 0.001f, FLT_MAX, &h, false)) {
+        Surface si = makeSurface(sc, ray, h);
+        return C3(0.5f) * std::fabs(dot(si.s.w, ray.d));
+    }
+    C3 L;
+    for (const Light& Lt : sc.lights) L = L + emittedRay(Lt, ray.d);
+    return L;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 intersector (the hit oracle): the same watertight algorithm in double
+// (no FP64 fallback needed), analytic shapes by closed forms in double.
+// ---------------------------------------------------------------------------
+struct D3 { double x, y, z; };
+inline double dget(const D3& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+double triangleT64(const Tri& tr, const D3& o, const D3& d, double tMin, double tMax) {
+    double ad[3] = {std::fabs(d.x), std::fabs(d.y), std::fabs(d.z)};
+    int kz = ad[0] > ad[1] ? (ad[0] > ad[2] ? 0 : 2) : (ad[1] > ad[2] ? 1 : 2);
+    int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+    D3 P[3] = {{(double)tr.a.x - o.x, (double)tr.a.y - o.y, (double)tr.a.z - o.z},
+               {(double)tr.b.x - o.x, (double)tr.b.y - o.y, (double)tr.b.z - o.z},
+               {(double)tr.c.x - o.x, (double)tr.c.y - o.y, (double)tr.c.z - o.z}};
+    double dz = dget(d, kz), sx = -dget(d, kx) / dz, sy = -dget(d, ky) / dz, sz = 1.0 / dz;
+    double X[3], Y[3], Z[3];
+    for (int i = 0; i < 3; i++) {
+        double px = dget(P[i], kx), py = dget(P[i], ky), pz = dget(P[i], kz);
+        X[i] = px + sx * pz; Y[i] = py + sy * pz; Z[i] = pz * sz;
+    }
+    double e0 = X[1] * Y[2] - Y[1] * X[2];
+    double e1 = X[2] * Y[0] - Y[2] * X[0];
+    double e2 = X[0] * Y[1] - Y[0] * X[1];
+    if ((e0 < 0 || e1 < 0 || e2 < 0) && (e0 > 0 || e1 > 0 || e2 > 0)) return NAN;
+    double det = e0 + e1 + e2;
+    if (det == 0.0) return NAN;
+    double t = (e0 * Z[0] + e1 * Z[1] + e2 * Z[2]) / det;
+    if (!(t > tMin && t < tMax)) return NAN;
+    return t;
+}
+double analyticT64(const Prim& p, const D3& o, const D3& d, double tMin, double tMax) {
+    if (p.shape == DFR_SHAPE_SPHERE) {
+        double ox = o.x - p.data[0], oy = o.y - p.data[1], oz = o.z - p.data[2], r = p.data[3];
+        double a = d.x * d.x + d.y * d.y + d.z * d.z;
+        double b = 2 * (d.x * ox + d.y * oy + d.z * oz);
+        double c = ox * ox + oy * oy + oz * oz - r * r;
+        double disc = b * b - 4 * a * c;
+        if (disc < 0) return NAN;
+        double q = b < 0 ? -0.5 * (b - std::sqrt(disc)) : -0.5 * (b + std::sqrt(disc));
+        double t0 = q / a, t1 = c / q;
+        if (t0 > t1) std::swap(t0, t1);
+        if (t0 > tMin && t0 < tMax) return t0;
+        if (t1 > tMin && t1 < tMax) return t1;
+        return NAN;
+    }
+    if (p.shape == DFR_SHAPE_RECTANGLE || p.shape == DFR_SHAPE_PLANE) {
+        double nx, ny, nz, ax, ay, az;
+        if (p.shape == DFR_SHAPE_RECTANGLE) { nx = p.rectNormal.x; ny = p.rectNormal.y; nz = p.rectNormal.z; }
+        else { nx = p.data[9]; ny = p.data[10]; nz = p.data[11]; }
+        ax = p.data[0]; ay = p.data[1]; az = p.data[2];
+        double dp = nx * d.x + ny * d.y + nz * d.z;
+        if (dp == 0.0) return NAN;
+        double t = ((ax - o.x) * nx + (ay - o.y) * ny + (az - o.z) * nz) / dp;
+        if (!(t > tMin && t < tMax)) return NAN;
+        if (p.shape == DFR_SHAPE_PLANE) return t;
+        double px = o.x + d.x * t - ax, py = o.y + d.y * t - ay, pz = o.z + d.z * t - az;
+        double abx = p.data[3] - ax, aby = p.data[4] - ay, abz = p.data[5] - az;
+        double bcx = p.data[6] - p.data[3], bcy = p.data[7] - p.data[4], bcz = p.data[8] - p.data[5];
+        double lab = std::sqrt(abx * abx + aby * aby + abz * abz), lbc = std::sqrt(bcx * bcx + bcy * bcy + bcz * bcz);
+        double dab = (px * abx + py * aby + pz * abz) / lab, dbc = (px * bcx + py * bcy + pz * bcz) / lbc;
+        if (dab >= 0 && dab <= lab && dbc >= 0 && dbc <= lbc) return t;
+        return NAN;
+    }
+    return NAN;
+}
+
+bool closest64(const Scene& sc, const D3& o, const D3& d, double tMin, double tMax, bool any, double* tOut,
+               int* prim, int64_t* tri, int64_t* nNodes, int64_t* nTris) {
+    bool found = false;
+    double best = tMax;
+    for (int pi : sc.analytic) {
+        double t = analyticT64(sc.prims[pi], o, d, tMin, best);
+        if (!std::isnan(t) && t < best) { best = t; *prim = pi; *tri = -1; found = true; if (any) { *tOut = t; return true; } }
+    }
+    if (!sc.nodes.empty()) {
+        double oo[3] = {o.x, o.y, o.z};
+        double inv[3] = {1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
+        int stack[128]; int sp = 0; stack[sp++] = 0;
+        while (sp) {
+            const BvhNode& nd = sc.nodes[stack[--sp]];
+            if (nNodes) (*nNodes)++;
+            if (!boxHit(nd, oo, inv, tMin, best)) continue;
+            if (nd.left < 0) {
+                for (int64_t i = nd.first; i < nd.first + nd.count; i++) {
+                    int64_t ti = sc.triOrder[i];
+                    if (nTris) (*nTris)++;
+                    double t = triangleT64(sc.tris[ti], o, d, tMin, best);
+                    if (!std::isnan(t) && (t < best || (t == best && found && *tri >= 0 && ti < *tri))) {
+                        best = t; *prim = sc.tris[ti].prim; *tri = ti; found = true;
+                        if (any) { *tOut = t; return true; }
+                    }
+                }
+            } else {
+                stack[sp++] = nd.right;
+                stack[sp++] = nd.left;
+            }
+        }
+    }
+    *tOut = best;
+    return found;
+}
+
+template <class F>
+void parallelFor(int64_t n, int threads, F f) {
+    if (threads <= 1 || n < 64) { for (int64_t i = 0; i < n; i++) f(i); return; }
+    std::atomic<int64_t> next(0);
+    std::vector<std::thread> pool;
+    for (int k = 0; k < threads; k++)
+        pool.emplace_back([&]() {
+            for (;;) {
+                int64_t b = next.fetch_add(16);
+                if (b >= n) break;
+                for (int64_t i = b; i < std::min(n, b + 16); i++) f(i);
+            }
+        });
+    for (auto& t : pool) t.join();
+}
+
+}  // namespace
+
+// ===========================================================================
+// C API
+// ===========================================================================
+struct orc_scene { Scene s; };
+
+extern "C" {
+
+orc_scene* orc_scene_create(const dfr_scene_desc* d) {
+    if (!d) return nullptr;
+    std::unique_ptr<orc_scene> h(new orc_scene());
+    Scene& sc = h->s;
+    sc.tris.resize((size_t)d->n_triangles);
+    for (int64_t i = 0; i < d->n_triangles; i++) {
+        Tri& t = sc.tris[i];
+        const int32_t* ix = d->indices + 3 * i;
+        auto P = [&](int k) { return V3(d->positions[3 * k], d->positions[3 * k + 1], d->positions[3 * k + 2]); };
+        t.a = P(ix[0]); t.b = P(ix[1]); t.c = P(ix[2]);
+        t.ng = normalize(cross(normalize(t.b - t.a), normalize(t.c - t.a)));
+        if (d->normals) {
+            auto N = [&](int k) { return V3(d->normals[3 * k], d->normals[3 * k + 1], d->normals[3 * k + 2]); };
+            t.na = N(ix[0]); t.nb = N(ix[1]); t.nc = N(ix[2]);
+        } else {
+            t.na = t.nb = t.nc = t.ng;
+        }
+        if (d->uvs) {
+            t.ua = d->uvs[2 * ix[0]]; t.va = d->uvs[2 * ix[0] + 1];
+            t.ub = d->uvs[2 * ix[1]]; t.vb = d->uvs[2 * ix[1] + 1];
+            t.uc = d->uvs[2 * ix[2]]; t.vc = d->uvs[2 * ix[2] + 1];
+        } else {
+            t.ua = t.va = t.ub = t.vb = t.uc = t.vc = 0.0f;
+        }
+        t.prim = -1;
+    }
+    sc.prims.resize(d->n_primitives);
+    for (int i = 0; i < d->n_primitives; i++) {
+        const dfr_primitive& p = d->primitives[i];
+        Prim& q = sc.prims[i];
+        q.shape = p.shape; q.material = p.material; q.areaLight = p.area_light;
+        q.first = p.first_triangle; q.count = p.triangle_count;
+        std::memcpy(q.data, p.data, sizeof(q.data));
+        if (p.shape == DFR_SHAPE_TRIANGLE_MESH) {
+            for (int64_t k = p.first_triangle; k < p.first_triangle + p.triangle_count; k++) sc.tris[k].prim = i;
+        } else {
+            sc.analytic.push_back(i);
+        }
+        if (p.shape == DFR_SHAPE_RECTANGLE) {
+            V3 a(p.data[0], p.data[1], p.data[2]), b(p.data[3], p.data[4], p.data[5]), c(p.data[6], p.data[7], p.data[8]);
+            q.rectNormal = normalize(cross(normalize(b - a), normalize(c - a)));
+        }
+    }
+    sc.mats.assign(d->materials, d->materials + d->n_materials);
+    for (int i = 0; i < d->n_lights; i++) {
+        const dfr_light& l = d->lights[i];
+        Light L;
+        L.type = l.type; L.prim = l.primitive; L.tri = l.triangle; L.twoSided = l.two_sided != 0;
+        L.radiance = C3(l.radiance[0], l.radiance[1], l.radiance[2]);
+        L.position = V3(l.position[0], l.position[1], l.position[2]);
+        L.perez = nullptr;
+        if (l.type == DFR_LIGHT_PEREZ) {
+            L.perez = new Perez();
+            perezInit(*L.perez, l.turbidity, V3(l.direction[0], l.direction[1], l.direction[2]));
+        }
+        sc.lights.push_back(L);
+    }
+    std::memcpy(sc.cam, d->camera, sizeof(sc.cam));
+    bvhBuild(sc);
+    return h.release();
+}
+
+void orc_scene_destroy(orc_scene* s) { delete s; }
+
+int32_t orc_render(const orc_scene* h, const orc_params* prm, double* sum, double* sumsq, int32_t* count) {
+    if (!h || !prm) return -1;
+    const Scene& sc = h->s;
+    const int W = prm->width, H = prm->height;
+    float cam[DFR_CAMERA_LENGTH];
+    std::memcpy(cam, sc.cam, sizeof(cam));
+    cam[DFR_CAMERA_RES_X] = (float)W; cam[DFR_CAMERA_RES_Y] = (float)H;
+    int x0 = prm->x0, y0 = prm->y0, x1 = prm->x1 > 0 ? prm->x1 : W, y1 = prm->y1 > 0 ? prm->y1 : H;
+    int64_t n = (int64_t)(x1 - x0) * (y1 - y0);
+    parallelFor(n, prm->threads, [&](int64_t i) {
+        int x = x0 + (int)(i % (x1 - x0)), y = y0 + (int)(i / (x1 - x0));
+        uint32_t pix = (uint32_t)(y * W + x);
+        double s[3] = {0, 0, 0}, q[3] = {0, 0, 0};
+        int c = 0;
+        for (int k = 0; k < prm->spp; k++) {
+            Rng rng = makeRng(prm->seed, pix, (uint32_t)(prm->spp_offset + k));
+            Ray ray;
+            if (!primaryRay(cam, (float)x, (float)y, rng.get(0), rng.get(1), &ray)) continue;
+            C3 L;
+            switch (prm->algorithm) {
+                case DFR_ALGO_PATH_TRACING: L = radiancePT(sc, ray, rng, prm->max_bounce, prm->min_rr, prm->light_pdf_mode); break;
+                case DFR_ALGO_AMBIENT_OCCLUSION: L = radianceAO(sc, ray, rng, prm->max_distance, prm->samples); break;
+                case DFR_ALGO_RAY_CASTING: L = radianceRC(sc, ray); break;
+                default: L = C3(0.0f); break;
+            }
+            // AbstractCPURenderer.java:229: drop NaN/Inf/negative-luminance samples
+            if (hasInf(L) || hasNaN(L) || relativeLuminance(L) < -1.0e-5f) continue;
+            s[0] += L.r; s[1] += L.g; s[2] += L.b;
+            q[0] += (double)L.r * L.r; q[1] += (double)L.g * L.g; q[2] += (double)L.b * L.b;
+            c++;
+        }
+        int64_t o = (int64_t)y * W + x;
+        for (int k = 0; k < 3; k++) {
+            if (sum) sum[3 * o + k] = s[k];
+            if (sumsq) sumsq[3 * o + k] = q[k];
+        }
+        if (count) count[o] = c;
+    });
+    return 0;
+}
+
+int32_t orc_trace_f32(const orc_scene* h, const float* rays, int64_t n, int32_t any, int32_t threads, int32_t* prim,
+                      int64_t* tri, float* t) {
+    if (!h) return -1;
+    const Scene& sc = h->s;
+    parallelFor(n, threads, [&](int64_t i) {
+        const float* r = rays + 8 * i;
+        Ray ray{V3(r[0], r[1], r[2]), V3(r[4], r[5], r[6])};
+        Hit hit;
+        if (sceneClosest(sc, ray, r[3], r[7], &hit, any != 0)) {
+            prim[i] = hit.prim; tri[i] = hit.tri; t[i] = hit.t;
+        } else {
+            prim[i] = -1; tri[i] = -1; t[i] = INFINITY;
+        }
+    });
+    return 0;
+}
+
+int32_t orc_trace_f64(const orc_scene* h, const float* rays, int64_t n, int32_t any, int32_t threads, int32_t* prim,
+                      int64_t* tri, double* t, int64_t* nodes, int64_t* tris) {
+    if (!h) return -1;
+    const Scene& sc = h->s;
+    parallelFor(n, threads, [&](int64_t i) {
+        const float* r = rays + 8 * i;
+        D3 o{r[0], r[1], r[2]}, d{r[4], r[5], r[6]};
+        double tt; int p = -1; int64_t tr = -1;
+        int64_t nn = 0, nt = 0;
+        if (closest64(sc, o, d, r[3], r[7], any != 0, &tt, &p, &tr, &nn, &nt)) {
+            prim[i] = p; tri[i] = tr; t[i] = tt;
+        } else {
+            prim[i] = -1; tri[i] = -1; t[i] = INFINITY;
+        }
+        if (nodes) nodes[i] = nn;
+        if (tris) tris[i] = nt;
+    });
+    return 0;
+}
+
+int32_t orc_primary_rays(const orc_scene* h, int32_t width, int32_t height, uint64_t seed, int32_t sample, float* rays) {
+    if (!h) return -1;
+    float cam[DFR_CAMERA_LENGTH];
+    std::memcpy(cam, h->s.cam, sizeof(cam));
+    cam[DFR_CAMERA_RES_X] = (float)width; cam[DFR_CAMERA_RES_Y] = (float)height;
+    for (int y = 0; y < height; y++)
+        for (int x = 0; x < width; x++) {
+            uint32_t pix = (uint32_t)(y * width + x);
+            Rng rng = makeRng(seed, pix, (uint32_t)sample);
+            Ray r;
+            float* o = rays + 8 * (int64_t)pix;
+            if (primaryRay(cam, (float)x, (float)y, rng.get(0), rng.get(1), &r)) {
+                o[0] = r.o.x; o[1] = r.o.y; o[2] = r.o.z; o[3] = 0.001f;
+                o[4] = r.d.x; o[5] = r.d.y; o[6] = r.d.z; o[7] = FLT_MAX;
+            } else {
+                for (int k = 0; k < 8; k++) o[k] = 0.0f;
+            }
+        }
+    return 0;
+}
+
+int32_t orc_camera_ray(const float* cam, float ix, float iy, float px, float py, float* out) {
+    Ray r;
+    if (!primaryRay(cam, ix, iy, px, py, &r)) return 0;
+    out[0] = r.o.x; out[1] = r.o.y; out[2] = r.o.z; out[3] = r.d.x; out[4] = r.d.y; out[5] = r.d.z;
+    return 1;
+}
+
+void orc_set_literal_tmin(int32_t on) { g_literal_tmin = on != 0; }
+
+float orc_rng(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t dim) {
+    return makeRng(seed, pixel, sample).get(dim);
+}
+
+// Unit-level entry points used by the known-answer tests.
+float orc_triangle_t(const float* a, const float* b, const float* c, const float* ray, float tMin, float tMax) {
+    Tri t{};
+    t.a = V3(a[0], a[1], a[2]); t.b = V3(b[0], b[1], b[2]); t.c = V3(c[0], c[1], c[2]);
+    Ray r{V3(ray[0], ray[1], ray[2]), V3(ray[3], ray[4], ray[5])};
+    return triangleT(t, r, tMin, tMax);
+}
+float orc_sphere_t(const float* center, float radius, const float* ray, float tMin, float tMax) {
+    Prim p{}; p.shape = DFR_SHAPE_SPHERE;
+    p.data[0] = center[0]; p.data[1] = center[1]; p.data[2] = center[2]; p.data[3] = radius;
+    Ray r{V3(ray[0], ray[1], ray[2]), normalize(V3(ray[3], ray[4], ray[5]))};
+    return sphereT(p, r, tMin, tMax);
+}
+float orc_plane_t(const float* origin, const float* normal, const float* ray, float tMin, float tMax) {
+    Prim p{}; p.shape = DFR_SHAPE_PLANE;
+    for (int k = 0; k < 3; k++) { p.data[k] = origin[k]; p.data[9 + k] = normal[k]; }
+    Ray r{V3(ray[0], ray[1], ray[2]), normalize(V3(ray[3], ray[4], ray[5]))};
+    return planeT(p, r, tMin, tMax);
+}
+float orc_fresnel_dielectric(float cosThetaI, float etaI, float etaT) { return fresnelDielectric(cosThetaI, etaI, etaT); }
+float orc_fresnel_conductor(float cosThetaI, float eta, float k) { return conductor1(std::fabs(cosThetaI), 1.0f, eta, k); }
+
+// BSDF probes: evaluate f and pdf of a material for local outgoing/incoming
+// directions with the shading frame equal to the world axes.
+int32_t orc_bsdf_eval(const dfr_material* m, const float* wo, const float* wi, float* f, float* pdf) {
+    Surface si{};
+    si.s.u = V3(1, 0, 0); si.s.v = V3(0, 1, 0); si.s.w = V3(0, 0, 1);
+    si.g = si.s;
+    si.rayDir = -V3(wo[0], wo[1], wo[2]);
+    BSDF b;
+    if (!buildBSDF(*m, si, &b)) return 0;
+    V3 w(wi[0], wi[1], wi[2]);
+    C3 c = b.f(ALL, w);
+    f[0] = c.r; f[1] = c.g; f[2] = c.b;
+    *pdf = b.pdf(ALL, w);
+    return 1;
+}
+int32_t orc_bsdf_sample(const dfr_material* m, const float* wo, float u, float v, float* wi, float* f, float* pdf,
+                        int32_t* type) {
+    Surface si{};
+    si.s.u = V3(1, 0, 0); si.s.v = V3(0, 1, 0); si.s.w = V3(0, 0, 1);
+    si.g = si.s;
+    si.rayDir = -V3(wo[0], wo[1], wo[2]);
+    BSDF b;
+    if (!buildBSDF(*m, si, &b)) return 0;
+    LobeSample s;
+    if (!b.sample(ALL, u, v, &s)) return 0;
+    wi[0] = s.wi.x; wi[1] = s.wi.y; wi[2] = s.wi.z;
+    f[0] = s.f.r; f[1] = s.f.g; f[2] = s.f.b;
+    *pdf = s.pdf; *type = (int32_t)s.type;
+    return 1;
+}
+int32_t orc_perez_sky(float turbidity, const float* sun, const float* dir, float* rgb) {
+    Perez p;
+    perezInit(p, turbidity, V3(sun[0], sun[1], sun[2]));
+    C3 c = perezSky(p, V3(dir[0], dir[1], dir[2]));
+    rgb[0] = c.r; rgb[1] = c.g; rgb[2] = c.b;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+/*
+ * oracle.h — TEST INFRASTRUCTURE ONLY.
+ *
+ * The oracle of the path-tracing hot path: a float32 CPU restatement of the
+ * reference CPURenderer integrators (M/renderer/cpu/AbstractCPURenderer.java
+ * :170-258 over M/scene/Scene.java:272-326, 451-718, 797-900, 1066-1090,
+ * 1530-1633) and an FP64 closest-hit intersector.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product never links or calls it.
+ *
+ * How each part is pinned (tests/test_oracle_*.py):
+ *   triangle test   reference golden vectors are absent for Triangle3F; pinned
+ *                   by the FP64 brute force, closed-form hits, watertightness
+ *                   on closed meshes and shared edges
+ *   sphere / plane  T/geometry/shape/Sphere3DUnitTests.java:149-197 and
+ *                   T/geometry/shape/Plane3FUnitTests.java:190-215 vectors
+ *   camera          closed form (centre pixel -> W, corners)
+ *   Fresnel         closed forms ((eta-1)/(eta+1))^2, conductor at normal
+ *                   incidence, total internal reflection
+ *   BxDFs           white furnace (energy <= 1), reciprocity, pdf integrates
+ *                   to one, sampled-direction histograms against the pdf
+ *   integrators     analytic furnace scene (Le/(1-a)), AO of an open plane
+ *                   and of a box interior in closed form
+ *   Perez sky       parity unpinned (no reference test, no runnable JVM)
+ *
+ * M/ = src/main/java/org/dayflower/ and T/ = src/test/java/org/dayflower/ of
+ * the reference tree.
+ */
+#ifndef DAYFLOWER_ORACLE_H
+#define DAYFLOWER_ORACLE_H
+
+#include <stdint.h>
+#include "../include/dayflower_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_scene orc_scene;
+
+typedef struct orc_params {
+    int32_t width, height, spp, spp_offset;
+    int32_t algorithm, max_bounce, min_rr, samples;
+    int32_t light_pdf_mode, threads;
+    int32_t x0, y0, x1, y1;      /* pixel window; x1/y1 <= 0 means full    */
+    float max_distance;
+    float reserved;
+    uint64_t seed;
+} orc_params;
+
+orc_scene* orc_scene_create(const dfr_scene_desc* desc);
+void orc_scene_destroy(orc_scene* scene);
+
+/* Per-pixel sum and sum of squares of the accepted samples and their count
+ * (unfiltered film, one bin per pixel).  Arrays are width*height*3 doubles
+ * and width*height ints; any may be NULL. */
+int32_t orc_render(const orc_scene* scene, const orc_params* params, double* sum, double* sumsq,
+                   int32_t* count);
+
+/* Closest/any hit for rays of 8 floats (origin, tmin, direction, tmax). */
+int32_t orc_trace_f32(const orc_scene* scene, const float* rays, int64_t n, int32_t any_hit,
+                      int32_t threads, int32_t* prim, int64_t* tri, float* t);
+int32_t orc_trace_f64(const orc_scene* scene, const float* rays, int64_t n, int32_t any_hit,
+                      int32_t threads, int32_t* prim, int64_t* tri, double* t, int64_t* nodes,
+                      int64_t* tris);
+int32_t orc_primary_rays(const orc_scene* scene, int32_t width, int32_t height, uint64_t seed,
+                         int32_t sample, float* rays);
+int32_t orc_camera_ray(const float* camera19, float ix, float iy, float px, float py, float* out6);
+/* Tests only: 1 makes the float triangle test compare t * det with tmin
+ * literally, as the reference does (Triangle3F.java:570-574). */
+void orc_set_literal_tmin(int32_t on);
+float orc_rng(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t dim);
+
+float orc_triangle_t(const float* a, const float* b, const float* c, const float* ray, float tmin, float tmax);
+float orc_sphere_t(const float* center, float radius, const float* ray, float tmin, float tmax);
+float orc_plane_t(const float* origin, const float* normal, const float* ray, float tmin, float tmax);
+float orc_fresnel_dielectric(float cos_theta_i, float eta_i, float eta_t);
+float orc_fresnel_conductor(float cos_theta_i, float eta, float k);
+int32_t orc_bsdf_eval(const dfr_material* m, const float* wo, const float* wi, float* f, float* pdf);
+int32_t orc_bsdf_sample(const dfr_material* m, const float* wo, float u, float v, float* wi, float* f,
+                        float* pdf, int32_t* type);
+int32_t orc_perez_sky(float turbidity, const float* sun, const float* dir, float* rgb);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,135 @@
+"""ctypes binding of the oracle (oracle/liboracle.so) for the tests and the
+bench's CPU baseline.  The oracle is test infrastructure: the product never
+imports this module."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from dayflower_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("spp", ctypes.c_int32),
+                ("spp_offset", ctypes.c_int32), ("algorithm", ctypes.c_int32), ("max_bounce", ctypes.c_int32),
+                ("min_rr", ctypes.c_int32), ("samples", ctypes.c_int32), ("light_pdf_mode", ctypes.c_int32),
+                ("threads", ctypes.c_int32), ("x0", ctypes.c_int32), ("y0", ctypes.c_int32),
+                ("x1", ctypes.c_int32), ("y1", ctypes.c_int32), ("max_distance", ctypes.c_float),
+                ("reserved", ctypes.c_float), ("seed", ctypes.c_uint64)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(ORACLE_SO):
+        build()
+    lib = ctypes.CDLL(ORACLE_SO)
+    P, f, i32, i64, vp = ctypes.POINTER, ctypes.c_float, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
+    lib.orc_scene_create.restype = vp
+    lib.orc_scene_create.argtypes = [P(abi.SceneDesc)]
+    lib.orc_scene_destroy.argtypes = [vp]
+    lib.orc_render.argtypes = [vp, P(Params), vp, vp, vp]
+    lib.orc_trace_f32.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
+    lib.orc_trace_f64.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp, vp, vp]
+    lib.orc_primary_rays.argtypes = [vp, i32, i32, ctypes.c_uint64, i32, vp]
+    lib.orc_camera_ray.argtypes = [vp, f, f, f, f, vp]
+    lib.orc_set_literal_tmin.argtypes = [i32]
+    lib.orc_rng.restype = f
+    lib.orc_rng.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+    for name in ("orc_triangle_t",):
+        getattr(lib, name).restype = f
+        getattr(lib, name).argtypes = [vp, vp, vp, vp, f, f]
+    lib.orc_sphere_t.restype = f
+    lib.orc_sphere_t.argtypes = [vp, f, vp, f, f]
+    lib.orc_plane_t.restype = f
+    lib.orc_plane_t.argtypes = [vp, vp, vp, f, f]
+    lib.orc_fresnel_dielectric.restype = f
+    lib.orc_fresnel_dielectric.argtypes = [f, f, f]
+    lib.orc_fresnel_conductor.restype = f
+    lib.orc_fresnel_conductor.argtypes = [f, f, f]
+    lib.orc_bsdf_eval.argtypes = [P(abi.Material), vp, vp, vp, vp]
+    lib.orc_bsdf_sample.argtypes = [P(abi.Material), vp, f, f, vp, vp, vp, vp]
+    lib.orc_perez_sky.argtypes = [f, vp, vp, vp]
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OracleScene:
+    def __init__(self, scene):
+        self.lib = load()
+        self.desc = scene.desc()
+        self.h = self.lib.orc_scene_create(ctypes.byref(self.desc))
+        if not self.h:
+            raise RuntimeError("orc_scene_create failed")
+
+    def close(self):
+        if self.h:
+            self.lib.orc_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def render(self, width, height, spp, algorithm=abi.ALGO_PATH_TRACING, max_bounce=20, min_rr=5, samples=10,
+               max_distance=20.0, seed=0, spp_offset=0, light_pdf_mode=0, threads=os.cpu_count(), window=None):
+        """Returns (sum, sumsq, count) per pixel (H,W,3 doubles; H,W ints)."""
+        p = Params()
+        p.width, p.height, p.spp, p.spp_offset = width, height, spp, spp_offset
+        p.algorithm, p.max_bounce, p.min_rr, p.samples = algorithm, max_bounce, min_rr, samples
+        p.light_pdf_mode, p.threads = light_pdf_mode, threads
+        if window is not None:
+            p.x0, p.y0, p.x1, p.y1 = window
+        p.max_distance = max_distance
+        p.seed = seed
+        s = np.zeros((height, width, 3), np.float64)
+        q = np.zeros((height, width, 3), np.float64)
+        c = np.zeros((height, width), np.int32)
+        rc = self.lib.orc_render(self.h, ctypes.byref(p), _ptr(s), _ptr(q), _ptr(c))
+        if rc != 0:
+            raise RuntimeError("orc_render failed")
+        return s, q, c
+
+    def trace_f32(self, rays, any_hit=False, threads=os.cpu_count()):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        prim = np.empty(n, np.int32)
+        tri = np.empty(n, np.int64)
+        t = np.empty(n, np.float32)
+        self.lib.orc_trace_f32(self.h, _ptr(rays), n, int(any_hit), threads, _ptr(prim), _ptr(tri), _ptr(t))
+        return prim, tri, t
+
+    def trace_f64(self, rays, any_hit=False, threads=os.cpu_count(), counts=False):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        prim = np.empty(n, np.int32)
+        tri = np.empty(n, np.int64)
+        t = np.empty(n, np.float64)
+        nodes = np.zeros(n, np.int64) if counts else None
+        tris = np.zeros(n, np.int64) if counts else None
+        self.lib.orc_trace_f64(self.h, _ptr(rays), n, int(any_hit), threads, _ptr(prim), _ptr(tri), _ptr(t),
+                               _ptr(nodes) if counts else None, _ptr(tris) if counts else None)
+        if counts:
+            return prim, tri, t, nodes, tris
+        return prim, tri, t
+
+    def primary_rays(self, width, height, seed, sample):
+        rays = np.zeros((width * height, 8), np.float32)
+        self.lib.orc_primary_rays(self.h, width, height, seed, sample, _ptr(rays))
+        return rays
