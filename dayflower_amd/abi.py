@@ -28,6 +28,8 @@ KEY_SEED_HI = 8
 KEY_KERNEL = 9
 KEY_LIGHT_PDF_MODE = 10
 KEY_SPP_PER_PASS = 11
+KEY_PROFILE = 12
+KEY_COUNT_TRAVERSAL = 13
 KEY_MAXIMUM_DISTANCE = 101
 
 E_OK, E_NULL, E_ARGUMENT, E_STATE, E_DEVICE, E_MEMORY, E_STOPPED, E_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6, -7
@@ -77,7 +79,9 @@ class SceneDesc(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [("seconds", c_double), ("path_samples", c_int64), ("rays_closest", c_int64),
-                ("rays_any", c_int64), ("bounces", c_int64)]
+                ("rays_any", c_int64), ("bounces", c_int64), ("extend_seconds", c_double),
+                ("shade_seconds", c_double), ("shadow_seconds", c_double), ("extend_launches", c_int64),
+                ("extend_rays", c_int64), ("extend_nodes", c_int64), ("extend_tris", c_int64)]
 
 
 class Job(ctypes.Structure):

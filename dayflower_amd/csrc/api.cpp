@@ -440,9 +440,29 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
     Counters host{};
     const int poll = 4;
     const int max_iterations = 1 << 20;  // far above max_bounce * (work / capacity + 1)
+    unsigned long long* trav = nullptr;
+    if (count_traversal) {
+        if (counters.alloc(64) != hipSuccess) return fail(DFR_E_MEMORY, "counters");
+        trav = (unsigned long long*)counters.ptr + 4;
+        hipMemsetAsync(trav, 0, 16, stream);
+    }
+    int iterations = 0;
     for (int it = 0;; it++) {
         if (it >= max_iterations) return fail(DFR_E_DEVICE, "wavefront did not drain");
-        wavefront_iteration((const DevScene*)scene_table.ptr, (const DevJob*)job_buf.ptr, n_jobs, film_ptr, wf, p, stream);
+        IterProfile prof{};
+        prof.timing = profile != 0;
+        prof.trav = trav;
+        if (prof.timing) {
+            while (prof_events.size() < (size_t)6 * (it + 1)) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return fail(DFR_E_DEVICE, "event create");
+                prof_events.push_back(e);
+            }
+            for (int k = 0; k < 6; k++) prof.ev[k] = prof_events[6 * it + k];
+        }
+        wavefront_iteration((const DevScene*)scene_table.ptr, (const DevJob*)job_buf.ptr, n_jobs, film_ptr, wf, p,
+                            (prof.timing || trav) ? &prof : nullptr, stream);
+        iterations = it + 1;
         if ((it + 1) % poll == 0) {
             hipError_t e = hipMemcpyAsync(&host, wf.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream);
             if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -452,6 +472,27 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
             if (stop.load()) return fail(DFR_E_STOPPED, "render stopped");
         }
     }
+    prof_stats = dfr_stats{};
+    if (profile) {
+        hipStreamSynchronize(stream);
+        for (int it = 0; it < iterations; it++) {
+            float a = 0, b = 0, d = 0;
+            hipEventElapsedTime(&a, prof_events[6 * it + 0], prof_events[6 * it + 1]);
+            hipEventElapsedTime(&b, prof_events[6 * it + 2], prof_events[6 * it + 3]);
+            hipEventElapsedTime(&d, prof_events[6 * it + 4], prof_events[6 * it + 5]);
+            prof_stats.extend_seconds += a * 1e-3;
+            prof_stats.shade_seconds += b * 1e-3;
+            prof_stats.shadow_seconds += d * 1e-3;
+        }
+        prof_stats.extend_launches = iterations;
+    }
+    if (trav) {
+        unsigned long long v[2];
+        hipMemcpy(v, trav, 16, hipMemcpyDeviceToHost);
+        prof_stats.extend_nodes = (int64_t)v[0];
+        prof_stats.extend_tris = (int64_t)v[1];
+    }
+    prof_stats.extend_rays = host.rays_closest;
     if (out_ctr) *out_ctr = host;
     return DFR_OK;
 }
@@ -517,6 +558,7 @@ void dfr_destroy(dfr_ctx* h) {
     }
     hipSetDevice(c->device);
     c->scenes.clear();
+    for (hipEvent_t e : c->prof_events) hipEventDestroy(e);
     hipEventDestroy(c->ev0);
     hipEventDestroy(c->ev1);
     hipStreamDestroy(c->stream);
@@ -570,6 +612,8 @@ int32_t dfr_set_i32(dfr_ctx* h, int32_t key, int32_t v) {
         case DFR_KEY_SPP_PER_PASS:
             if (v < 1) return c->fail(DFR_E_ARGUMENT, "spp per pass must be >= 1");
             c->spp_per_pass = v; break;
+        case DFR_KEY_PROFILE: c->profile = v != 0; break;
+        case DFR_KEY_COUNT_TRAVERSAL: c->count_traversal = v != 0; break;
         default: return c->fail(DFR_E_ARGUMENT, "unknown integer key");
     }
     return DFR_OK;
@@ -590,6 +634,8 @@ int32_t dfr_get_i32(const dfr_ctx* h, int32_t key, int32_t* v) {
         case DFR_KEY_KERNEL: *v = c->kernel; break;
         case DFR_KEY_LIGHT_PDF_MODE: *v = c->light_pdf_mode; break;
         case DFR_KEY_SPP_PER_PASS: *v = c->spp_per_pass; break;
+        case DFR_KEY_PROFILE: *v = c->profile; break;
+        case DFR_KEY_COUNT_TRAVERSAL: *v = c->count_traversal; break;
         default: return DFR_E_ARGUMENT;
     }
     return DFR_OK;
@@ -697,6 +743,10 @@ int32_t dfr_render(dfr_ctx* h, int32_t passes) {
         e = hipEventSynchronize(c->ev1);
         if (e != hipSuccess) return c->hip_fail(e, "wavefront");
         st.rays_closest = out.rays_closest; st.rays_any = out.rays_any; st.bounces = out.bounces;
+        st.extend_seconds = c->prof_stats.extend_seconds; st.shade_seconds = c->prof_stats.shade_seconds;
+        st.shadow_seconds = c->prof_stats.shadow_seconds; st.extend_launches = c->prof_stats.extend_launches;
+        st.extend_rays = c->prof_stats.extend_rays; st.extend_nodes = c->prof_stats.extend_nodes;
+        st.extend_tris = c->prof_stats.extend_tris;
     }
     float ms = 0.0f;
     hipEventElapsedTime(&ms, c->ev0, c->ev1);

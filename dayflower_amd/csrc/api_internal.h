@@ -91,6 +91,8 @@ struct Ctx {
     int kernel = 0;            // 0 wavefront, 1 plain megakernel
     int light_pdf_mode = 0;
     int spp_per_pass = 1;
+    int profile = 0;           // time kernel launches with HIP events
+    int count_traversal = 0;   // count BVH visits of the extend kernel
     float camera[DFR_CAMERA_LENGTH] = {0};
 
     // scene state
@@ -106,8 +108,10 @@ struct Ctx {
     WavefrontBuffers wf{};
     DeviceBuffer wf_mem[16];
     int wf_capacity = 0;
+    std::vector<hipEvent_t> prof_events;   // 6 per wavefront iteration, reused
 
     dfr_stats stats{};
+    dfr_stats prof_stats{};   // instrumentation of the last wavefront run
 
     int fail(int code, const std::string& msg);
     int hip_fail(hipError_t e, const char* what);

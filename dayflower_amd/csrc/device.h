@@ -313,9 +313,13 @@ DEV bool slab2(const float4& n0, const float4& n1, const float4& n2, const PRay&
     return h0 || h1;
 }
 
+struct VisitCount {
+    int nodes = 0, tris = 0;
+};
+
 template <bool ANY, int SHORT>
 DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit, TraversalStack<SHORT>& st,
-               int* node_visits = nullptr) {
+               VisitCount& vc) {
     bool found = false;
     hit.t = tmax; hit.prim = -1; hit.tri = -1;
     // analytic primitives (few): linear, in primitive order
@@ -335,7 +339,7 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
         if (!(cur & LEAF_FLAG_DEV)) {
             const float4* np = s.nodes + 4 * (size_t)cur;
             float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
-            if (node_visits) (*node_visits)++;
+            vc.nodes++;
             float t0, t1; bool h0, h1;
             slab2(n0, n1, n2, r, tmin, hit.t, t0, t1, h0, h1);
             uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
@@ -351,6 +355,7 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
         } else {
             uint32_t first = (cur & 0x7fffffffu) >> 3, cnt = (cur & 7u) + 1;
             for (uint32_t k = 0; k < cnt; k++) {
+                vc.tris++;
                 const float4* tp = s.tris + 3 * (size_t)(first + k);
                 float4 a = tp[0], b = tp[1], c = tp[2];
                 float t = tri_t(f4v(a), f4v(b), f4v(c), r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, tmin, hit.t);
@@ -364,6 +369,12 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
         cur = (uint32_t)st.pop();
     }
     return found;
+}
+
+template <bool ANY, int SHORT>
+DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit, TraversalStack<SHORT>& st) {
+    VisitCount vc;
+    return trace<ANY, SHORT>(s, o, d, tmin, tmax, hit, st, vc);
 }
 
 // ---------------------------------------------------------------------------

@@ -72,20 +72,20 @@ struct DevLight {          // 64 bytes
 // Perez sky record: parameters and the 32x32 importance distribution.
 constexpr int PEREZ_RES = 32;
 struct DevPerez {
+    double zenith[3];
+    double perez_y[5], perez_x[5], perez_lum[5];
     float sun_dir_world[3];
     float sun_dir_object[3];
     float sun_color[3];
-    float theta;            // sun zenith angle
+    float theta;                 // sun zenith angle
     float radius;
-    float zenith[3];
-    float perez_y[5], perez_x[5], perez_lum[5];
     float s0[3], s1[3], s2[3];   // daylight basis in XYZ
+    float xyz_to_rgb[9];         // sRGB, row major
     float world_to_object[9];    // rotation, row major
     float object_to_world[9];
-    float func_int;              // marginal integral
+    float func_int;              // integral of the marginal
     float marginal_func[PEREZ_RES];
     float marginal_cdf[PEREZ_RES + 1];
-    float marginal_int_unused;
     float cond_func[PEREZ_RES][PEREZ_RES];
     float cond_cdf[PEREZ_RES][PEREZ_RES + 1];
     float cond_int[PEREZ_RES];

@@ -64,6 +64,7 @@ struct WavefrontBuffers {
     int* overflow;     // traversal-stack overflow
     int capacity;      // path pool size
     Counters* ctr;
+    int* job_done;     // optional: paths finished per job (serving)
 };
 
 // Launch helpers (kernels.hip).  `scenes` points to a device array of
@@ -82,9 +83,17 @@ struct WavefrontParams {
     int grid_cap;       // blocks of the queue kernels (capacity / block)
 };
 
+// Optional instrumentation of one wavefront iteration: HIP events around the
+// extend, shade and shadow launches, and BVH visit counters for extend.
+struct IterProfile {
+    hipEvent_t ev[6];
+    bool timing;
+    unsigned long long* trav;  // [nodes, triangles] or null
+};
+
 void wavefront_init(WavefrontBuffers& w, const DevJob* jobs, int n_jobs, int64_t work_total, hipStream_t s);
 void wavefront_iteration(const DevScene* scenes, const DevJob* jobs, int n_jobs, float4* film, WavefrontBuffers& w,
-                         const WavefrontParams& p, hipStream_t s);
+                         const WavefrontParams& p, const IterProfile* prof, hipStream_t s);
 int overflow_entries_per_thread(int max_depth);
 constexpr int SHORT_STACK = 16;
 

@@ -449,13 +449,8 @@ __global__ void k_commit(Counters* ctr, int capacity) {
 }
 
 template <int BLOCK, int SHORT>
-__global__ __launch_bounds__(BLOCK) void k_extend(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
-                                                  PathSoA q, float4* __restrict__ hits, Counters* __restrict__ ctr,
-                                                  int* __restrict__ overflow) {
-    __shared__ int lds[SHORT * BLOCK];
-    int i = blockIdx.x * BLOCK + threadIdx.x;
-    int n = ctr->n_active;
-    if (i >= n) return;
+DEV void extend_one(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, PathSoA q,
+                    float4* __restrict__ hits, int* __restrict__ overflow, int* lds, int i, VisitCount& vc) {
     TraversalStack<SHORT> st;
     st.lds = lds + threadIdx.x; st.bdim = BLOCK;
     st.glob = overflow + (size_t)blockIdx.x * BLOCK + threadIdx.x; st.gstride = gridDim.x * BLOCK;
@@ -464,8 +459,24 @@ __global__ __launch_bounds__(BLOCK) void k_extend(const DevScene* __restrict__ s
     const DevScene& sc = scenes[jobs[info.z].scene];
     float4 o = q.o[i], d = q.d[i];
     HitRec h;
-    bool hit = trace<false, SHORT>(sc, f4v(o), f4v(d), T_MIN, FLT_MAX, h, st);
+    bool hit = trace<false, SHORT>(sc, f4v(o), f4v(d), T_MIN, FLT_MAX, h, st, vc);
     hits[i] = make_float4(h.t, __int_as_float(hit ? h.prim : -1), __int_as_float(h.tri), 0.0f);
+}
+
+template <int BLOCK, int SHORT, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_extend(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
+                                                  PathSoA q, float4* __restrict__ hits, Counters* __restrict__ ctr,
+                                                  int* __restrict__ overflow, unsigned long long* __restrict__ trav) {
+    __shared__ int lds[SHORT * BLOCK];
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int n = ctr->n_active;
+    VisitCount vc;
+    if (i < n) extend_one<BLOCK, SHORT>(scenes, jobs, q, hits, overflow, lds, i, vc);
+    if (COUNT) {
+        unsigned long long a = vc.nodes, b = vc.tris;
+        for (int off = 32; off > 0; off >>= 1) { a += __shfl_down(a, off); b += __shfl_down(b, off); }
+        if (lane_id() == 0) { atomicAdd(&trav[0], a); atomicAdd(&trav[1], b); }
+    }
 }
 
 // Material-sort keys: material type of the hit (0..5), 6 = miss, 7 = dropped.
@@ -516,7 +527,8 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_shade(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
                                                  PathSoA q, const float4* __restrict__ hits,
                                                  const int32_t* __restrict__ perm, PathSoA out, ShadowSoA sh,
-                                                 Counters* __restrict__ ctr, float4* __restrict__ film, int use_perm) {
+                                                 Counters* __restrict__ ctr, float4* __restrict__ film, int use_perm,
+                                                 int* __restrict__ job_done) {
     int k = blockIdx.x * BLOCK + threadIdx.x;
     int n = ctr->n_active;
     bool active = k < n;
@@ -591,6 +603,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const DevScene* __restrict__ sc
             // each NEE request carries the film index of its path
         }
     }
+    if (job_done && active && !cont) atomicAdd(&job_done[q.info[i].z], 1);
     int slot = wave_append(&ctr->n_next, cont);
     if (cont) {
         out.o[slot] = make_float4(no.x, no.y, no.z, 0.0f);
@@ -643,28 +656,49 @@ void wavefront_init(WavefrontBuffers& w, const DevJob* jobs, int n_jobs, int64_t
     hipMemcpyAsync(w.ctr, &c, sizeof(Counters), hipMemcpyHostToDevice, s);
 }
 
-// One wavefront iteration; the queue state lives on the device, so a
-// sequence of iterations needs no host round trip.
-void wavefront_iteration(const DevScene* scenes, const DevJob* jobs, int n_jobs, float4* film, WavefrontBuffers& w,
-                         const WavefrontParams& p, hipStream_t s) {
+template <int B>
+static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, float4* film, WavefrontBuffers& w,
+                        const WavefrontParams& p, const IterProfile* prof, hipStream_t s) {
     const int cap = w.capacity;
-    const int grid = (cap + 255) / 256;
+    const int grid = (cap + B - 1) / B;
+    const int grid256 = (cap + 255) / 256;
     // top up the queue with new camera paths, then commit the counters
-    hipLaunchKernelGGL(k_generate, dim3(grid), dim3(256), 0, s, jobs, n_jobs, w.a, w.ctr, film, cap);
+    hipLaunchKernelGGL(k_generate, dim3(grid256), dim3(256), 0, s, jobs, n_jobs, w.a, w.ctr, film, cap);
     hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, s, w.ctr, cap);
-    hipLaunchKernelGGL((k_extend<256, SHORT_STACK>), dim3(grid), dim3(256), 0, s, scenes, jobs, w.a, w.hit, w.ctr, w.overflow);
+    const bool timing = prof && prof->timing;
+    if (timing) hipEventRecord(prof->ev[0], s);
+    if (prof && prof->trav)
+        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, true>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr,
+                           w.overflow, prof->trav);
+    else
+        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, false>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr,
+                           w.overflow, (unsigned long long*)nullptr);
+    if (timing) hipEventRecord(prof->ev[1], s);
     int use_perm = p.sort_keys > 0;
     if (use_perm) {
-        hipLaunchKernelGGL(k_sort_keys, dim3(grid), dim3(256), 0, s, scenes, jobs, w.a, w.hit, w.keys, w.ctr);
+        hipLaunchKernelGGL(k_sort_keys, dim3(grid256), dim3(256), 0, s, scenes, jobs, w.a, w.hit, w.keys, w.ctr);
         hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1), 0, s, w.ctr);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(256), 0, s, w.keys, w.perm, w.ctr);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(grid256), dim3(256), 0, s, w.keys, w.perm, w.ctr);
     }
-    hipLaunchKernelGGL((k_shade<256>), dim3(grid), dim3(256), 0, s, scenes, jobs, w.a, w.hit, w.perm, w.b, w.sh, w.ctr, film,
-                       use_perm);
+    if (timing) hipEventRecord(prof->ev[2], s);
+    hipLaunchKernelGGL((k_shade<B>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.perm, w.b, w.sh, w.ctr, film,
+                       use_perm, w.job_done);
+    if (timing) hipEventRecord(prof->ev[3], s);
     hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(1), 0, s, w.ctr);
-    hipLaunchKernelGGL((k_shadow<256, SHORT_STACK>), dim3((2 * cap + 255) / 256), dim3(256), 0, s, scenes, jobs, w.sh, w.ctr,
+    if (timing) hipEventRecord(prof->ev[4], s);
+    hipLaunchKernelGGL((k_shadow<B, SHORT_STACK>), dim3((2 * cap + B - 1) / B), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr,
                        film, w.overflow);
+    if (timing) hipEventRecord(prof->ev[5], s);
     std::swap(w.a, w.b);
+}
+
+// One wavefront iteration; the queue state lives on the device, so a
+// sequence of iterations needs no host round trip.  Block sizes 128 and 256
+// are built (the autotuner picks one).
+void wavefront_iteration(const DevScene* scenes, const DevJob* jobs, int n_jobs, float4* film, WavefrontBuffers& w,
+                         const WavefrontParams& p, const IterProfile* prof, hipStream_t s) {
+    if (p.block == 128) iteration_b<128>(scenes, jobs, n_jobs, film, w, p, prof, s);
+    else iteration_b<256>(scenes, jobs, n_jobs, film, w, p, prof, s);
 }
 
 }  // namespace dfr

@@ -170,6 +170,12 @@ class GPURenderer:
     def setSamplesPerPass(self, spp):
         self._seti(abi.KEY_SPP_PER_PASS, spp)
 
+    def setProfile(self, on):
+        self._seti(abi.KEY_PROFILE, int(bool(on)))
+
+    def setCountTraversal(self, on):
+        self._seti(abi.KEY_COUNT_TRAVERSAL, int(bool(on)))
+
     # -- rendering -----------------------------------------------------------
     def setup(self, compile_scene=True):
         self._c(self.lib.dfr_setup(self.ctx, 1 if compile_scene else 0))
@@ -209,8 +215,7 @@ class GPURenderer:
     def stats(self):
         st = abi.Stats()
         self._c(self.lib.dfr_get_stats(self.ctx, ctypes.byref(st)))
-        return {"seconds": st.seconds, "path_samples": st.path_samples, "rays_closest": st.rays_closest,
-                "rays_any": st.rays_any, "bounces": st.bounces}
+        return {name: getattr(st, name) for name, _ in abi.Stats._fields_}
 
     # -- ray queries -----------------------------------------------------------
     def trace_rays(self, rays, any_hit=False):
@@ -248,6 +253,15 @@ class GPURenderer:
     def removeScene(self, sid):
         self._c(self.lib.dfr_scene_remove(self.ctx, int(sid)))
 
+    def autotune(self, jobs):
+        """Pick block size, path-pool capacity and material-sort keys for the
+        packed launch on `jobs`; returns (block, capacity, sort_keys)."""
+        n = len(jobs)
+        arr = (abi.Job * n)(*jobs)
+        out = (ctypes.c_int32 * 3)()
+        self._c(self.lib.dfr_autotune(self.ctx, arr, n, out))
+        return tuple(out)
+
     def batchRender(self, jobs, mode=abi.BATCH_PACKED, readback=True):
         """Render a list of abi.Job records; returns per-job (H, W, 3) films."""
         n = len(jobs)
@@ -271,3 +285,61 @@ def make_job(scene_id, camera, width, height, spp, seed, maximum_bounce=8, minim
     j.camera[17] = float(width)
     j.camera[18] = float(height)
     return j
+
+
+class Server:
+    """Serving loop over a GPURenderer's resident scenes (continuous
+    batching; see dfr_server_* in include/dayflower_hip.h)."""
+
+    def __init__(self, renderer, hbm_budget_bytes=64 << 30, max_active=256, pool_paths=1 << 20, spp_quantum=0):
+        self.r = renderer
+        self.lib = renderer.lib
+        o = abi.ServerOpts()
+        o.hbm_budget_bytes, o.max_active, o.pool_paths, o.spp_quantum = int(hbm_budget_bytes), int(max_active), \
+            int(pool_paths), int(spp_quantum)
+        self.h = self.lib.dfr_server_create(renderer.ctx, ctypes.byref(o))
+        if not self.h:
+            msg = self.lib.dfr_last_error(renderer.ctx)
+            raise DeviceException("dfr_server_create failed: " + (msg.decode() if msg else ""))
+        self._jobs = {}
+
+    def submit(self, job):
+        rid = self.lib.dfr_server_submit(self.h, ctypes.byref(job))
+        if rid < 0:
+            raise IllegalArgumentException(f"submit failed ({rid})")
+        self._jobs[rid] = job
+        return rid
+
+    def poll(self, rid):
+        return self.lib.dfr_server_poll(self.h, rid)
+
+    def wait(self, rid):
+        j = self._jobs[rid]
+        film = np.zeros((j.height, j.width, 3), np.float32)
+        rc = self.lib.dfr_server_wait(self.h, rid, film.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        if rc != 0:
+            raise DeviceException(f"request {rid} failed ({rc})")
+        return film
+
+    def latency(self, rid):
+        out = ctypes.c_double()
+        rc = self.lib.dfr_server_latency(self.h, rid, ctypes.byref(out))
+        if rc != 0:
+            raise IllegalStateException(f"request {rid} not done ({rc})")
+        return out.value
+
+    def stats(self):
+        st = abi.ServerStats()
+        self.lib.dfr_server_get_stats(self.h, ctypes.byref(st))
+        return {name: getattr(st, name) for name, _ in abi.ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            self.lib.dfr_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

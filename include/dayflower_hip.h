@@ -193,6 +193,8 @@ typedef struct dfr_ctx dfr_ctx;
 #define DFR_KEY_KERNEL                  9  /* 0 wavefront (default), 1 plain   */
 #define DFR_KEY_LIGHT_PDF_MODE         10  /* 0 reference (default), 1 area    */
 #define DFR_KEY_SPP_PER_PASS           11  /* samples per pixel per pass, 1    */
+#define DFR_KEY_PROFILE                12  /* 1: time each kernel launch       */
+#define DFR_KEY_COUNT_TRAVERSAL        13  /* 1: count BVH node / triangle visits */
 #define DFR_KEY_MAXIMUM_DISTANCE      101  /* float, default 20                */
 
 /* new GPURenderer(observer): M/renderer/gpu/GPURenderer.java:55-57 with the
@@ -258,6 +260,15 @@ typedef struct dfr_stats {
     int64_t  rays_closest;       /* closest-hit queries                     */
     int64_t  rays_any;           /* any-hit (shadow / AO) queries           */
     int64_t  bounces;            /* path vertices shaded                    */
+    /* with DFR_KEY_PROFILE: summed HIP-event durations per kernel kind     */
+    double   extend_seconds;     /* closest-hit traversal (wavefront extend) */
+    double   shade_seconds;
+    double   shadow_seconds;
+    int64_t  extend_launches;
+    int64_t  extend_rays;
+    /* with DFR_KEY_COUNT_TRAVERSAL: visits made by the extend kernel       */
+    int64_t  extend_nodes;       /* two-wide nodes fetched                  */
+    int64_t  extend_tris;        /* triangle tests                          */
 } dfr_stats;
 int32_t dfr_get_stats(const dfr_ctx* ctx, dfr_stats* stats);
 

@@ -47,7 +47,7 @@ def test_soup_primary_hits_match_fp64(n_tris, res):
     prim_o, tri_o, t_o = o.trace_f64(rays)
     frac, rel, nhit = _hit_parity(prim, tri, t, prim_o, tri_o, t_o)
     print(f"soup {n_tris}: id match {frac:.6f}, max rel t {rel:.2e}, hits {nhit}")
-    assert nhit > 0.2 * res * res
+    assert nhit > 0.1 * res * res
     assert frac >= 0.9999
     assert rel <= 1e-5
     # the GPU generates the same camera rays as the oracle's restatement
@@ -194,3 +194,33 @@ def test_batch_alone_equals_in_mix():
     np.testing.assert_allclose(films[1], alone, rtol=1e-4, atol=1e-5)
     close = np.isclose(plain, alone, rtol=1e-3, atol=1e-4).all(-1).mean()
     assert close > 0.9
+
+
+def test_server_alone_equals_in_mix():
+    """Requests submitted while others are on the GPU are admitted as memory
+    frees up; every result equals the same job rendered alone (packed batch of
+    one), within float-atomic reordering tolerance."""
+    import time
+    from dayflower_amd.renderer import Server, make_job
+    r = _renderer()
+    sid = r.addScene(S.icosphere_scene(20000, with_perez=True, width=64, height=64))
+    cam = S.icosphere_scene(20, with_perez=False, width=64, height=64).camera
+    jobs = [make_job(sid, cam, 32 + 16 * (i % 3), 32, [1, 4, 8][i % 3], seed=100 + i) for i in range(12)]
+    alone = [r.batchRender([j])[0] for j in jobs]
+    # budget admits only a few films at a time, so admission waits for frees
+    pool = 1 << 16
+    pool_bytes = pool * (9 * 16 + 8 + 2 * 4 * 16) + 4096  # path SoA, queues and counters
+    srv = Server(r, hbm_budget_bytes=pool_bytes + 3 * 64 * 32 * 16, max_active=4, pool_paths=pool)
+    ids = []
+    for j in jobs:
+        ids.append(srv.submit(j))
+        time.sleep(0.002)
+    films = [srv.wait(i) for i in ids]
+    lat = [srv.latency(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    r.dispose()
+    for f, a in zip(films, alone):
+        np.testing.assert_allclose(f, a, rtol=1e-4, atol=1e-5)
+    assert st["completed"] == len(jobs) and st["peak_active"] <= 4
+    assert all(x > 0 for x in lat)
