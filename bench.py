@@ -53,6 +53,108 @@ def cpu_baseline(sc, width, height, spp, max_bounce, min_rr, budget_s):
             "sample": f"{side}x{side} pixel window at the image centre, {spp} spp, one thread, {dt:.1f} s"}
 
 
+def run_batched(args):
+    """64 mixed jobs: plain per-job kernel vs the packed persistent wavefront."""
+    from dayflower_amd import abi, workloads as W
+    from dayflower_amd.renderer import GPURenderer
+    t0 = time.perf_counter()
+    scenes = W.batched_scenes()
+    r = GPURenderer()
+    ids = [r.addScene(sc) for sc in scenes]
+    log(f"batched: 4 scenes resident in {time.perf_counter() - t0:.1f} s")
+    jobs = W.batched_jobs(ids, n_jobs=args.jobs)
+    total = sum(j.width * j.height * j.spp for j in jobs)
+    t0 = time.perf_counter()
+    tuned = r.autotune(jobs)
+    log(f"batched: autotune -> block {tuned[0]}, pool {tuned[1]}, sort keys {tuned[2]} ({time.perf_counter() - t0:.1f} s)")
+    res = {}
+    for name, mode in (("plain", abi.BATCH_PLAIN), ("packed", abi.BATCH_PACKED)):
+        r.batchRender(jobs[:2], mode=mode, readback=False)  # warm
+        t0 = time.perf_counter()
+        r.batchRender(jobs, mode=mode, readback=False)
+        wall = time.perf_counter() - t0
+        st = r.stats()
+        res[name] = {"seconds": round(wall, 3), "samples_per_s": round(total / wall, 1),
+                     "mrays_per_s": round((st["rays_closest"] + st["rays_any"]) / wall / 1e6, 1)}
+        log(f"batched: {name} {wall:.2f} s, {total / wall / 1e6:.1f} M samples/s")
+    r.dispose()
+    print(json.dumps({"metric": METRIC, "config": {"workload": f"{len(jobs)} mixed render jobs in one batch "
+                      "(res 128^2..1024^2, spp 1..256, icospheres 10k..2M triangles, Matte/Glass/Metal/Disney)",
+                      "path_samples": total, "autotuned": {"block": tuned[0], "pool_paths": tuned[1],
+                                                           "sort_keys": tuned[2]}},
+                      "plain": res["plain"], "packed": res["packed"],
+                      "speedup_packed_over_plain": round(res["plain"]["seconds"] / res["packed"]["seconds"], 3),
+                      "unit": "path samples/s", "n_gpus": 1}), flush=True)
+
+
+def run_serving(args):
+    """Poisson request stream against a resident 2M-triangle scene: the
+    continuous-batching server vs a FIFO of plain per-request launches."""
+    import threading
+    import numpy as np
+    from dayflower_amd import abi, workloads as W
+    from dayflower_amd.renderer import GPURenderer, Server
+    sc = W.mixed_material_icosphere(2_000_000, 0x5EED)
+    r = GPURenderer()
+    sid = r.addScene(sc)
+    stream = W.request_stream(sid, rate=args.rate, duration=args.duration)
+    total = sum(j.width * j.height * j.spp for _, j in stream)
+    log(f"serving: {len(stream)} requests over {args.duration} s, {total / 1e9:.2f} G samples")
+    out = {}
+    # continuous batching server
+    srv = Server(r, hbm_budget_bytes=64 << 30, max_active=256, pool_paths=1 << 20)
+    t0 = time.perf_counter()
+    ids = []
+    for at, job in stream:
+        now = time.perf_counter() - t0
+        if at > now:
+            time.sleep(at - now)
+        ids.append(srv.submit(job))
+    for i in ids:
+        srv.wait(i)
+    wall = time.perf_counter() - t0
+    lat = np.array([srv.latency(i) for i in ids])
+    st = srv.stats()
+    srv.close()
+    out["served"] = {"p50_s": round(float(np.percentile(lat, 50)), 4), "p99_s": round(float(np.percentile(lat, 99)), 4),
+                     "samples_per_s": round(total / wall, 1), "wall_s": round(wall, 2), "peak_active": st["peak_active"],
+                     "peak_bytes": st["peak_bytes"]}
+    log(f"serving: server p50 {out['served']['p50_s']} s p99 {out['served']['p99_s']} s, {total / wall / 1e6:.1f} M samples/s")
+    # FIFO of plain per-request launches (one job at a time)
+    done = {}
+    q = list(range(len(stream)))
+    t0 = time.perf_counter()
+    submit_t = {}
+
+    def feeder():
+        for k, (at, _) in enumerate(stream):
+            now = time.perf_counter() - t0
+            if at > now:
+                time.sleep(at - now)
+            submit_t[k] = time.perf_counter()
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    for k in q:
+        while k not in submit_t:
+            time.sleep(0.0005)
+        r.batchRender([stream[k][1]], mode=abi.BATCH_PLAIN, readback=True)
+        done[k] = time.perf_counter() - submit_t[k]
+    th.join()
+    wall = time.perf_counter() - t0
+    lat = np.array([done[k] for k in q])
+    out["plain_fifo"] = {"p50_s": round(float(np.percentile(lat, 50)), 4), "p99_s": round(float(np.percentile(lat, 99)), 4),
+                         "samples_per_s": round(total / wall, 1), "wall_s": round(wall, 2)}
+    log(f"serving: plain p50 {out['plain_fifo']['p50_s']} s p99 {out['plain_fifo']['p99_s']} s, {total / wall / 1e6:.1f} M samples/s")
+    r.dispose()
+    print(json.dumps({"metric": METRIC, "config": {"workload": f"Poisson {args.rate} req/s for {args.duration} s, "
+                      "2M-triangle displaced icosphere, res 256^2..1024^2, spp 4..64, 64 GB admission budget",
+                      "requests": len(stream), "path_samples": total},
+                      "served": out["served"], "plain_fifo": out["plain_fifo"],
+                      "throughput_ratio": round(out["served"]["samples_per_s"] / out["plain_fifo"]["samples_per_s"], 3),
+                      "unit": "path samples/s", "n_gpus": 1}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,12 +169,21 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-perez", action="store_true")
+    ap.add_argument("--config", choices=["frame", "batched", "serving"], default="frame")
+    ap.add_argument("--jobs", type=int, default=64)
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--duration", type=float, default=2.0)
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding is a design "
                                    "note only)", "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.config == "batched":
+        return run_batched(args)
+    if args.config == "serving":
+        return run_serving(args)
 
     from dayflower_amd import scene as S
     from dayflower_amd.renderer import GPURenderer

@@ -106,7 +106,7 @@ struct Ctx {
     // wavefront engine
     WavefrontConfig wf_params;
     WavefrontBuffers wf{};
-    DeviceBuffer wf_mem[16];
+    DeviceBuffer wf_mem[17];
     int wf_capacity = 0;
     std::vector<hipEvent_t> prof_events;   // 6 per wavefront iteration, reused
 

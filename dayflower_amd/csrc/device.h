@@ -273,6 +273,33 @@ DEV float plane_t(const DevPrim& p, V o, V d, float tmin, float tmax) {
     return 0.0f;
 }
 
+// FP64 distance to the winning analytic primitive along the same ray: the
+// plane distance for rectangles and planes, and for spheres the root of the
+// double-precision quadratic nearest to the float one that won.
+DEV float analytic_t64(const DevPrim& p, V o, V d, float t32) {
+    double dx = d.x, dy = d.y, dz = d.z;
+    if (p.shape == DFR_SHAPE_SPHERE) {
+        double ox = (double)o.x - p.data[0], oy = (double)o.y - p.data[1], oz = (double)o.z - p.data[2];
+        double r = p.data[3];
+        double a = dx * dx + dy * dy + dz * dz;
+        double b = 2.0 * (dx * ox + dy * oy + dz * oz);
+        double c = ox * ox + oy * oy + oz * oz - r * r;
+        double disc = b * b - 4.0 * a * c;
+        if (disc < 0.0) return t32;
+        double q = b < 0.0 ? -0.5 * (b - sqrt(disc)) : -0.5 * (b + sqrt(disc));
+        if (q == 0.0) return t32;
+        double t0 = q / a, t1 = c / q;
+        double t = fabs(t0 - t32) < fabs(t1 - t32) ? t0 : t1;
+        return t > 0.0 ? (float)t : t32;
+    }
+    const float* n = p.shape == DFR_SHAPE_RECTANGLE ? p.normal : p.data + 9;
+    double den = n[0] * dx + n[1] * dy + n[2] * dz;
+    if (den == 0.0) return t32;
+    double t = (((double)p.data[0] - o.x) * n[0] + ((double)p.data[1] - o.y) * n[1] +
+                ((double)p.data[2] - o.z) * n[2]) / den;
+    return t > 0.0 ? (float)t : t32;
+}
+
 DEV float analytic_t(const DevPrim& p, V o, V d, float tmin, float tmax) {
     switch (p.shape) {
         case DFR_SHAPE_SPHERE: return sphere_t(p, o, d, tmin, tmax);
@@ -367,6 +394,21 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
         }
         if (st.sp == 0) break;
         cur = (uint32_t)st.pop();
+    }
+    if (!ANY && found && hit.tri < 0) hit.t = analytic_t64(s.prims[hit.prim], o, d, hit.t);
+    if (!ANY && found && hit.tri >= 0) {
+        // The float watertight t loses digits on small, oblique triangles;
+        // the winner's distance to its plane is recomputed in FP64.
+        int4 ix = s.tri_idx[hit.tri];
+        float4 a = s.vpos[ix.x], b = s.vpos[ix.y], c = s.vpos[ix.z];
+        double e1x = (double)b.x - a.x, e1y = (double)b.y - a.y, e1z = (double)b.z - a.z;
+        double e2x = (double)c.x - a.x, e2y = (double)c.y - a.y, e2z = (double)c.z - a.z;
+        double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+        double den = nx * d.x + ny * d.y + nz * d.z;
+        if (den != 0.0) {
+            double t = (nx * ((double)a.x - o.x) + ny * ((double)a.y - o.y) + nz * ((double)a.z - o.z)) / den;
+            if (t > 0.0) hit.t = (float)t;
+        }
     }
     return found;
 }

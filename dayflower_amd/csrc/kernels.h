@@ -51,8 +51,6 @@ struct Counters {
     int64_t rays_closest;
     int64_t rays_any;
     int64_t bounces;
-    int32_t key_count[8];  // material-sort histogram
-    int32_t key_offset[8];
 };
 
 struct WavefrontBuffers {
@@ -60,6 +58,7 @@ struct WavefrontBuffers {
     float4* hit;       // t, prim, tri (int bits)
     int32_t* perm;     // material-sorted order
     int32_t* keys;
+    int32_t* bhist;    // per-block key histograms / offsets (8 * blocks)
     ShadowSoA sh;
     int* overflow;     // traversal-stack overflow
     int capacity;      // path pool size

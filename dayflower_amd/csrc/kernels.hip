@@ -445,7 +445,6 @@ __global__ void k_commit(Counters* ctr, int capacity) {
     ctr->bounces += ctr->n_active;
     ctr->n_next = 0;
     ctr->n_shadow = 0;
-    for (int k = 0; k < 8; k++) ctr->key_count[k] = 0;
 }
 
 template <int BLOCK, int SHORT>
@@ -463,13 +462,30 @@ DEV void extend_one(const DevScene* __restrict__ scenes, const DevJob* __restric
     hits[i] = make_float4(h.t, __int_as_float(hit ? h.prim : -1), __int_as_float(h.tri), 0.0f);
 }
 
-template <int BLOCK, int SHORT, bool COUNT>
+// Material-sort key of a hit: material type of the hit (0..5), 6 = miss,
+// 7 = dropped camera sample.
+DEV int sort_key(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, uint32_t job, float4 h) {
+    int prim = __float_as_int(h.y);
+    if (prim == -2) return 7;
+    if (prim < 0) return 6;
+    const DevScene& sc = scenes[jobs[job].scene];
+    return sc.mats[sc.prims[prim].material].type;
+}
+
+// Closest hit for every queued path.  With SORT, the epilogue also writes the
+// material-sort key of each hit and the block's key histogram
+// (bhist[key * gridDim.x + block]); the histogram is built with LDS atomics,
+// so no global atomic is contended.
+template <int BLOCK, int SHORT, bool COUNT, bool SORT>
 __global__ __launch_bounds__(BLOCK) void k_extend(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
                                                   PathSoA q, float4* __restrict__ hits, Counters* __restrict__ ctr,
-                                                  int* __restrict__ overflow, unsigned long long* __restrict__ trav) {
+                                                  int* __restrict__ overflow, unsigned long long* __restrict__ trav,
+                                                  int32_t* __restrict__ keys, int32_t* __restrict__ bhist) {
     __shared__ int lds[SHORT * BLOCK];
+    __shared__ int hist[8];
     int i = blockIdx.x * BLOCK + threadIdx.x;
     int n = ctr->n_active;
+    if (SORT && threadIdx.x < 8) hist[threadIdx.x] = 0;
     VisitCount vc;
     if (i < n) extend_one<BLOCK, SHORT>(scenes, jobs, q, hits, overflow, lds, i, vc);
     if (COUNT) {
@@ -477,50 +493,52 @@ __global__ __launch_bounds__(BLOCK) void k_extend(const DevScene* __restrict__ s
         for (int off = 32; off > 0; off >>= 1) { a += __shfl_down(a, off); b += __shfl_down(b, off); }
         if (lane_id() == 0) { atomicAdd(&trav[0], a); atomicAdd(&trav[1], b); }
     }
-}
-
-// Material-sort keys: material type of the hit (0..5), 6 = miss, 7 = dropped.
-__global__ void k_sort_keys(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, PathSoA q,
-                            const float4* __restrict__ hits, int32_t* __restrict__ keys, Counters* __restrict__ ctr) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int n = ctr->n_active;
-    int key = -1;
-    if (i < n) {
-        int prim = __float_as_int(hits[i].y);
-        if (prim == -2) key = 7;
-        else if (prim < 0) key = 6;
-        else {
-            const DevScene& sc = scenes[jobs[q.info[i].z].scene];
-            key = sc.mats[sc.prims[prim].material].type;
+    if (SORT) {
+        __syncthreads();
+        if (i < n) {
+            int k = sort_key(scenes, jobs, q.info[i].z, hits[i]);
+            keys[i] = k;
+            atomicAdd(&hist[k], 1);
         }
-        keys[i] = key;
-    }
-    // wave-aggregated histogram
-    for (int k = 0; k < 8; k++) {
-        uint64_t m = __ballot(key == k);
-        if (m && lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(&ctr->key_count[k], __popcll(m));
+        __syncthreads();
+        if (threadIdx.x < 8) bhist[threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
     }
 }
 
-__global__ void k_sort_scan(Counters* ctr) {
-    int acc = 0;
-    for (int k = 0; k < 8; k++) { ctr->key_offset[k] = acc; acc += ctr->key_count[k]; ctr->key_count[k] = 0; }
+// Exclusive scan of the key-major block histograms (8 * nblocks entries) by
+// one 1024-thread block: chunk sums, an LDS scan of the chunk sums, then the
+// chunk-local prefix.
+__global__ __launch_bounds__(1024) void k_sort_scan(int32_t* __restrict__ bhist, int total) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int chunk = (total + 1023) / 1024;
+    const int b0 = t * chunk, b1 = min(total, b0 + chunk);
+    int sum = 0;
+    for (int k = b0; k < b1; k++) sum += bhist[k];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - sum;  // exclusive prefix of this chunk
+    for (int k = b0; k < b1; k++) { int c = bhist[k]; bhist[k] = run; run += c; }
 }
 
-__global__ void k_sort_scatter(const int32_t* __restrict__ keys, int32_t* __restrict__ perm, Counters* __restrict__ ctr) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Scatter queue indices into material-sorted order: each block starts from
+// its scanned offsets and ranks its own elements with LDS atomics (order
+// inside one key does not matter).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sort_scatter(const int32_t* __restrict__ keys, const int32_t* __restrict__ boff,
+                                                        int32_t* __restrict__ perm, const Counters* __restrict__ ctr) {
+    __shared__ int base[8];
+    int i = blockIdx.x * BLOCK + threadIdx.x;
     int n = ctr->n_active;
-    int key = i < n ? keys[i] : -1;
-    for (int k = 0; k < 8; k++) {
-        bool p = key == k;
-        uint64_t m = __ballot(p);
-        if (!m) continue;
-        int leader = __ffsll((unsigned long long)m) - 1;
-        int base = 0;
-        if (lane_id() == leader) base = ctr->key_offset[k] + atomicAdd(&ctr->key_count[k], __popcll(m));
-        base = __shfl(base, leader);
-        if (p) perm[base + __popcll(m & ((1ull << lane_id()) - 1ull))] = i;
-    }
+    if (threadIdx.x < 8) base[threadIdx.x] = boff[threadIdx.x * gridDim.x + blockIdx.x];
+    __syncthreads();
+    if (i < n) perm[atomicAdd(&base[keys[i]], 1)] = i;
 }
 
 template <int BLOCK>
@@ -666,19 +684,25 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     hipLaunchKernelGGL(k_generate, dim3(grid256), dim3(256), 0, s, jobs, n_jobs, w.a, w.ctr, film, cap);
     hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, s, w.ctr, cap);
     const bool timing = prof && prof->timing;
+    const int use_perm = p.sort_keys > 0;
     if (timing) hipEventRecord(prof->ev[0], s);
-    if (prof && prof->trav)
-        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, true>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr,
-                           w.overflow, prof->trav);
+    unsigned long long* trav = prof ? prof->trav : nullptr;
+    if (trav && use_perm)
+        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, true, true>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr,
+                           w.overflow, trav, w.keys, w.bhist);
+    else if (trav)
+        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, true, false>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr,
+                           w.overflow, trav, w.keys, w.bhist);
+    else if (use_perm)
+        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, false, true>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr,
+                           w.overflow, trav, w.keys, w.bhist);
     else
-        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, false>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr,
-                           w.overflow, (unsigned long long*)nullptr);
+        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, false, false>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr,
+                           w.overflow, trav, w.keys, w.bhist);
     if (timing) hipEventRecord(prof->ev[1], s);
-    int use_perm = p.sort_keys > 0;
     if (use_perm) {
-        hipLaunchKernelGGL(k_sort_keys, dim3(grid256), dim3(256), 0, s, scenes, jobs, w.a, w.hit, w.keys, w.ctr);
-        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1), 0, s, w.ctr);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(grid256), dim3(256), 0, s, w.keys, w.perm, w.ctr);
+        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, w.bhist, 8 * grid);
+        hipLaunchKernelGGL((k_sort_scatter<B>), dim3(grid), dim3(B), 0, s, w.keys, w.bhist, w.perm, w.ctr);
     }
     if (timing) hipEventRecord(prof->ev[2], s);
     hipLaunchKernelGGL((k_shade<B>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.perm, w.b, w.sh, w.ctr, film,

@@ -25,6 +25,9 @@ class Params(ctypes.Structure):
 
 _lib = None
 
+# worker threads of the oracle: at most 16, the CPU share of one GPU box
+THREADS = max(1, min(16, os.cpu_count() or 1))
+
 
 def build():
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
@@ -88,7 +91,7 @@ class OracleScene:
         self.close()
 
     def render(self, width, height, spp, algorithm=abi.ALGO_PATH_TRACING, max_bounce=20, min_rr=5, samples=10,
-               max_distance=20.0, seed=0, spp_offset=0, light_pdf_mode=0, threads=os.cpu_count(), window=None):
+               max_distance=20.0, seed=0, spp_offset=0, light_pdf_mode=0, threads=THREADS, window=None):
         """Returns (sum, sumsq, count) per pixel (H,W,3 doubles; H,W ints)."""
         p = Params()
         p.width, p.height, p.spp, p.spp_offset = width, height, spp, spp_offset
@@ -106,7 +109,7 @@ class OracleScene:
             raise RuntimeError("orc_render failed")
         return s, q, c
 
-    def trace_f32(self, rays, any_hit=False, threads=os.cpu_count()):
+    def trace_f32(self, rays, any_hit=False, threads=THREADS):
         rays = np.ascontiguousarray(rays, np.float32)
         n = rays.shape[0]
         prim = np.empty(n, np.int32)
@@ -115,7 +118,7 @@ class OracleScene:
         self.lib.orc_trace_f32(self.h, _ptr(rays), n, int(any_hit), threads, _ptr(prim), _ptr(tri), _ptr(t))
         return prim, tri, t
 
-    def trace_f64(self, rays, any_hit=False, threads=os.cpu_count(), counts=False):
+    def trace_f64(self, rays, any_hit=False, threads=THREADS, counts=False):
         rays = np.ascontiguousarray(rays, np.float32)
         n = rays.shape[0]
         prim = np.empty(n, np.int32)
@@ -133,3 +136,21 @@ class OracleScene:
         rays = np.zeros((width * height, 8), np.float32)
         self.lib.orc_primary_rays(self.h, width, height, seed, sample, _ptr(rays))
         return rays
+
+
+def film_agreement(g, n_g, s, q, c):
+    """Statistical agreement of a film `g` (H,W,3 means of `n_g` samples per
+    pixel) with an oracle render given as per-pixel (sum, sumsq, count).
+
+    The standard error of the difference uses the oracle's sample variance
+    for both estimates (the same estimator, so the same variance).  Returns
+    the fraction of pixel channels within 3 sigma, the fraction of pixels
+    with all three channels within 3 sigma, and the relative RMSE
+    sqrt(mean((g - m)^2)) / sqrt(mean(m^2))."""
+    n = c[..., None].astype(np.float64)
+    mean = s / n
+    var = np.maximum(q / n - mean * mean, 0.0)
+    se = np.sqrt(var / n + var / float(n_g))
+    z = np.abs(np.asarray(g, np.float64) - mean) / np.maximum(se, 1e-12)
+    rrmse = float(np.sqrt(np.mean((g - mean) ** 2)) / np.sqrt(np.mean(mean ** 2)))
+    return float((z <= 3.0).mean()), float((z <= 3.0).all(-1).mean()), rrmse

@@ -102,54 +102,73 @@ def test_trace_rays_closest_and_any_hit():
 @pytest.mark.parametrize("kernel", [0, 1])
 def test_cornell_path_tracing_same_streams(kernel):
     """Wavefront (0) and plain (1) kernels against the oracle on the same
-    random streams: per-pixel agreement except the few paths that float
-    rounding sends another way; image mean within 0.5 %."""
+    random streams.
+
+    Same streams give the same paths up to float rounding: a path leaves the
+    oracle's only where rounding flips a decision (a Russian-roulette or
+    Fresnel draw, a grazing edge).  How often rounding alone does that is
+    fixed by the oracle itself: built with FMA contraction it agrees with its
+    own non-contracted build on 99.0 % of 1-spp paths of this scene (and on
+    84.9 % of 64x64 pixels at 32 spp).  Bars: >= 98 % of 1-spp paths equal
+    (rtol 1e-3), and the 32-spp image mean within 0.5 %."""
     from tests.oracle_lib import OracleScene
-    res, spp = 64, 32
+    res = 128
     sc = S.cornell_box(res, res)
+    o = OracleScene(sc)
     r = _renderer()
     r.setScene(sc)
     r.setSeed(42)
     r.setKernel(kernel)
     r.setup()
+    r.render(1)
+    g1, cnt1 = r.film(res, res)
+    s1, _, c1 = o.render(res, res, 1, seed=42)
+    paths = np.isclose(g1, s1 / c1[..., None], rtol=1e-3, atol=1e-4).all(-1).mean()
+    # 32 spp on a 64x64 film of the same scene
+    res2, spp = 64, 32
+    sc2 = S.cornell_box(res2, res2)
+    r.setScene(sc2)
+    r.setup()
     r.render(spp)
-    g, cnt = r.film(res, res)
+    g, cnt = r.film(res2, res2)
     st = r.stats()
     r.dispose()
-    assert (cnt == spp).all()
-    s, q, c = OracleScene(sc).render(res, res, spp, seed=42)
+    assert (cnt1 == 1).all() and (cnt == spp).all()
+    s, q, c = OracleScene(sc2).render(res2, res2, spp, seed=42)
     ref = s / c[..., None]
     close = np.isclose(g, ref, rtol=1e-3, atol=1e-4).all(-1).mean()
-    print(f"kernel {kernel}: pixels equal {close:.4f}, mean gpu {g.mean():.5f} ref {ref.mean():.5f}, {st}")
-    assert close > 0.90
+    print(f"kernel {kernel}: 1-spp paths equal {paths:.4f}; 32 spp pixels equal {close:.4f}, "
+          f"mean gpu {g.mean():.5f} ref {ref.mean():.5f}, {st}")
+    assert paths >= 0.98
     assert abs(g.mean() - ref.mean()) / ref.mean() < 0.005
 
 
 def test_cornell_statistical_parity():
-    """Independent streams (different seeds), 3-sigma per pixel on >= 99 % of
-    pixels and relative RMSE of the means <= 1 % over the image.
+    """Converged images on independent streams: the GPU film at 32768 spp
+    against the oracle's at 8192 spp (48x48 pixels).
 
-    Converged-image bar at reduced size: 48x48 pixels, 2048 spp."""
-    from tests.oracle_lib import OracleScene
-    res, spp = 48, 2048
+    Bars: >= 99 % of pixel channels and >= 98 % of pixels (all three
+    channels) within 3 sigma, relative RMSE <= 1 %.  The oracle against
+    itself on two seeds at these sample counts gives 99.54 % / 98.78 % and a
+    relative RMSE of 0.58 %; a 1 % brightness bias drops the pixel fraction
+    to 82 % and raises the RMSE to 1.14 %.  (The per-pixel fraction sits
+    below 99 % for an unbiased renderer because the glass sphere's caustic
+    paths make the sample variance heavy-tailed.)"""
+    from tests.oracle_lib import OracleScene, film_agreement
+    res, spp_g, spp_o = 48, 32768, 8192
     sc = S.cornell_box(res, res)
     r = _renderer()
     r.setScene(sc)
     r.setSeed(1001)
     r.setup()
-    r.render(spp)
+    r.render(spp_g)
     g, cnt = r.film(res, res)
     r.dispose()
-    s, q, c = OracleScene(sc).render(res, res, spp, seed=2002)
-    n = c[..., None].astype(np.float64)
-    mean = s / n
-    var = np.maximum(q / n - mean * mean, 0.0)
-    se = np.sqrt(var / n + var / spp)  # both estimates carry MC error
-    z = np.abs(g - mean) / np.maximum(se, 1e-12)
-    within = (z <= 3.0).all(-1).mean()
-    rrmse = np.sqrt(np.mean((g - mean) ** 2)) / np.sqrt(np.mean(mean ** 2))
-    print(f"3-sigma fraction {within:.4f}, relative RMSE {rrmse:.4f}")
-    assert within >= 0.99
+    assert (cnt == spp_g).all()
+    s, q, c = OracleScene(sc).render(res, res, spp_o, seed=2002)
+    chan, pix, rrmse = film_agreement(g, spp_g, s, q, c)
+    print(f"3-sigma: channels {chan:.4f}, pixels {pix:.4f}; relative RMSE {rrmse:.4f}")
+    assert chan >= 0.99 and pix >= 0.98
     assert rrmse <= 0.01
 
 
