@@ -30,6 +30,41 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_icosphere1m_1080p.txt")
+DEFAULT_FRAME = {"triangles": 1_000_000, "width": 1920, "height": 1080, "spp": 64, "max_bounce": 8, "no_perez": False}
+
+
+def hbm_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` on the default frame workload, as
+    measured by rocprofv3 FETCH_SIZE + WRITE_SIZE and scaled by the same
+    counters on a copy of known size (lines `bytes_per_dispatch.<kernel> = N`
+    of the committed traffic summary); None for other workloads."""
+    if any(getattr(args, k) != v for k, v in DEFAULT_FRAME.items()) or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            key, _, val = line.partition("=")
+            if key.strip() == f"bytes_per_dispatch.{kernel}":
+                return int(float(val))
+    return None
+
+
+def copy_calibration(mib):
+    """A fill and a device-to-device copy of `mib` MiB before the render, so a
+    counter run of this process also records kernels of known traffic (the
+    fill writes mib MiB; the copy reads and writes mib MiB each)."""
+    import torch
+    n = int(mib) << 20
+    a = torch.empty(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(7)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"bench: copy calibration: fill {mib} MiB, copy {mib} MiB")
+    del a, b
+
+
 def cpu_baseline(sc, width, height, spp, max_bounce, min_rr, budget_s):
     """Oracle (float32 restatement of CPURenderer), one thread, on a centred
     pixel window sized to take about `budget_s` seconds."""
@@ -173,6 +208,8 @@ def main():
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=2.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="fill and copy MIB MiB on the device first (reference kernels for HBM counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -180,6 +217,8 @@ def main():
                                    "note only)", "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
     if args.config == "batched":
         return run_batched(args)
     if args.config == "serving":
@@ -209,14 +248,18 @@ def main():
     r.setCountTraversal(False)
     nodes_per_ray = cst["extend_nodes"] / max(1, cst["extend_rays"])
     tris_per_ray = cst["extend_tris"] / max(1, cst["extend_rays"])
-    log(f"bench: extend visits per ray: {nodes_per_ray:.2f} nodes, {tris_per_ray:.2f} triangles")
+    sh_nodes_per_ray = cst["shadow_nodes"] / max(1, cst["shadow_rays"])
+    sh_tris_per_ray = cst["shadow_tris"] / max(1, cst["shadow_rays"])
+    log(f"bench: extend visits per ray: {nodes_per_ray:.2f} nodes, {tris_per_ray:.2f} triangles; "
+        f"shadow: {sh_nodes_per_ray:.2f} nodes, {sh_tris_per_ray:.2f} triangles")
 
     for i in range(args.warmup):
         r.render(args.spp)
         log(f"bench: warmup {i + 1}/{args.warmup} {r.stats()['seconds']:.3f} s")
     r.clear()
     r.setProfile(args.kernel == "wavefront")
-    tot = {"samples": 0, "closest": 0, "any": 0, "ext_s": 0.0, "ext_launch": 0, "ext_rays": 0, "dev_s": 0.0}
+    tot = {"samples": 0, "closest": 0, "any": 0, "ext_s": 0.0, "ext_launch": 0, "ext_rays": 0, "dev_s": 0.0,
+           "sh_s": 0.0, "sh_rays": 0, "shade_s": 0.0}
     t0 = time.perf_counter()
     for i in range(args.steps):
         r.render(args.spp)
@@ -227,6 +270,9 @@ def main():
         tot["ext_s"] += st["extend_seconds"]
         tot["ext_launch"] += st["extend_launches"]
         tot["ext_rays"] += st["extend_rays"]
+        tot["sh_s"] += st["shadow_seconds"]
+        tot["sh_rays"] += st["shadow_rays"]
+        tot["shade_s"] += st["shade_seconds"]
         tot["dev_s"] += st["seconds"]
         log(f"bench: step {i + 1}/{args.steps} {st['seconds']:.3f} s")
     wall = time.perf_counter() - t0
@@ -235,17 +281,37 @@ def main():
 
     value = tot["samples"] / wall
     roofline = None
+    stages = None
     if tot["ext_s"] > 0:
-        # algorithmic bytes per closest-hit ray: ray in (32) + hit out (16) +
-        # 64 B per two-wide node fetched + 36 B of positions per triangle test
-        bytes_per_ray = 32 + 16 + 64 * nodes_per_ray + 36 * tris_per_ray
-        traffic = bytes_per_ray * tot["ext_rays"] / max(1, tot["ext_launch"])
-        achieved = bytes_per_ray * tot["ext_rays"] / tot["ext_s"] / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "kernel": "k_extend (BVH closest hit)", "algorithmic_bytes_per_launch": round(traffic),
-                    "avg_launch_ms": round(tot["ext_s"] / max(1, tot["ext_launch"]) * 1e3, 3),
-                    "bytes_per_ray": round(bytes_per_ray, 1)}
+        # Algorithmic bytes per traversal query: ray in (32) + hit out (16) +
+        # 32 B per bounding box tested (a two-wide node fetch tests two) + 36 B
+        # of vertex positions per triangle test.
+        def bytes_per(nodes, tris):
+            return 32 + 16 + 64 * nodes + 36 * tris
+
+        launches = max(1, tot["ext_launch"])
+        kern = {
+            "k_extend": {"seconds": tot["ext_s"], "rays": tot["ext_rays"], "bytes_per_ray": bytes_per(nodes_per_ray, tris_per_ray),
+                         "nodes_per_ray": nodes_per_ray, "tris_per_ray": tris_per_ray, "what": "BVH closest hit"},
+            "k_shadow": {"seconds": tot["sh_s"], "rays": tot["sh_rays"], "bytes_per_ray": bytes_per(sh_nodes_per_ray, sh_tris_per_ray),
+                         "nodes_per_ray": sh_nodes_per_ray, "tris_per_ray": sh_tris_per_ray,
+                         "what": "NEE shadow / MIS rays (any and closest hit)"},
+        }
+        for k in kern.values():
+            k["achieved"] = k["bytes_per_ray"] * k["rays"] / max(k["seconds"], 1e-12) / 1e9
+        stages = {name: {"ms_per_launch": round(k["seconds"] / launches * 1e3, 3), "rays": k["rays"],
+                         "nodes_per_ray": round(k["nodes_per_ray"], 2), "tris_per_ray": round(k["tris_per_ray"], 2),
+                         "bytes_per_ray": round(k["bytes_per_ray"], 1), "achieved_GBs": round(k["achieved"], 1)}
+                  for name, k in kern.items()}
+        stages["k_shade"] = {"ms_per_launch": round(tot["shade_s"] / launches * 1e3, 3)}
+        dom = max(kern, key=lambda n: kern[n]["seconds"])
+        k = kern[dom]
+        traffic = hbm_traffic(dom, args)
+        roofline = {"bound": "hbm", "achieved": round(k["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(k["achieved"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "kernel": f"{dom} ({k['what']})",
+                    "algorithmic_bytes_per_launch": round(k["bytes_per_ray"] * k["rays"] / launches),
+                    "avg_launch_ms": round(k["seconds"] / launches * 1e3, 3), "bytes_per_ray": round(k["bytes_per_ray"], 1)}
     cpu = None
     if not args.no_cpu:
         try:
@@ -275,6 +341,7 @@ def main():
         "mrays_any_per_s": round(tot["any"] / wall / 1e6, 1),
         "device_seconds": round(tot["dev_s"], 3),
         "roofline": roofline,
+        "stages": stages,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -81,7 +81,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("seconds", c_double), ("path_samples", c_int64), ("rays_closest", c_int64),
                 ("rays_any", c_int64), ("bounces", c_int64), ("extend_seconds", c_double),
                 ("shade_seconds", c_double), ("shadow_seconds", c_double), ("extend_launches", c_int64),
-                ("extend_rays", c_int64), ("extend_nodes", c_int64), ("extend_tris", c_int64)]
+                ("extend_rays", c_int64), ("extend_nodes", c_int64), ("extend_tris", c_int64),
+                ("shadow_rays", c_int64), ("shadow_nodes", c_int64), ("shadow_tris", c_int64)]
 
 
 class Job(ctypes.Structure):

@@ -445,7 +445,7 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
     if (count_traversal) {
         if (counters.alloc(64) != hipSuccess) return fail(DFR_E_MEMORY, "counters");
         trav = (unsigned long long*)counters.ptr + 4;
-        hipMemsetAsync(trav, 0, 16, stream);
+        hipMemsetAsync(trav, 0, 32, stream);
     }
     int iterations = 0;
     for (int it = 0;; it++) {
@@ -488,12 +488,15 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
         prof_stats.extend_launches = iterations;
     }
     if (trav) {
-        unsigned long long v[2];
-        hipMemcpy(v, trav, 16, hipMemcpyDeviceToHost);
+        unsigned long long v[4];
+        hipMemcpy(v, trav, 32, hipMemcpyDeviceToHost);
         prof_stats.extend_nodes = (int64_t)v[0];
         prof_stats.extend_tris = (int64_t)v[1];
+        prof_stats.shadow_nodes = (int64_t)v[2];
+        prof_stats.shadow_tris = (int64_t)v[3];
     }
-    prof_stats.extend_rays = host.rays_closest;
+    prof_stats.extend_rays = host.bounces;
+    prof_stats.shadow_rays = host.rays_closest + host.rays_any - host.bounces;
     if (out_ctr) *out_ctr = host;
     return DFR_OK;
 }
@@ -747,7 +750,8 @@ int32_t dfr_render(dfr_ctx* h, int32_t passes) {
         st.extend_seconds = c->prof_stats.extend_seconds; st.shade_seconds = c->prof_stats.shade_seconds;
         st.shadow_seconds = c->prof_stats.shadow_seconds; st.extend_launches = c->prof_stats.extend_launches;
         st.extend_rays = c->prof_stats.extend_rays; st.extend_nodes = c->prof_stats.extend_nodes;
-        st.extend_tris = c->prof_stats.extend_tris;
+        st.extend_tris = c->prof_stats.extend_tris; st.shadow_rays = c->prof_stats.shadow_rays;
+        st.shadow_nodes = c->prof_stats.shadow_nodes; st.shadow_tris = c->prof_stats.shadow_tris;
     }
     float ms = 0.0f;
     hipEventElapsedTime(&ms, c->ev0, c->ev1);

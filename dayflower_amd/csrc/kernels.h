@@ -51,6 +51,8 @@ struct Counters {
     int64_t rays_closest;
     int64_t rays_any;
     int64_t bounces;
+    int32_t n_shadow_closest;  // shadow requests of this iteration that need the closest hit
+    int32_t pad;
 };
 
 struct WavefrontBuffers {
@@ -87,7 +89,7 @@ struct WavefrontParams {
 struct IterProfile {
     hipEvent_t ev[6];
     bool timing;
-    unsigned long long* trav;  // [nodes, triangles] or null
+    unsigned long long* trav;  // [extend nodes, extend triangles, shadow nodes, shadow triangles] or null
 };
 
 void wavefront_init(WavefrontBuffers& w, const DevJob* jobs, int n_jobs, int64_t work_total, hipStream_t s);

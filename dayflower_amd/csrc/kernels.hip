@@ -108,13 +108,13 @@ DEV int nee_requests(const DevScene& sc, const Bsdf& b, const Surf& si, int li, 
 }
 
 template <int SHORT>
-DEV V resolve_shadow(const DevScene& sc, const ShadowReq& r, TraversalStack<SHORT>& st) {
+DEV V resolve_shadow(const DevScene& sc, const ShadowReq& r, TraversalStack<SHORT>& st, VisitCount& vc) {
     HitRec h;
     if (r.mode == 0) {
-        if (trace<true, SHORT>(sc, r.o, r.d, T_MIN, r.tmax, h, st)) return mk(0, 0, 0);
+        if (trace<true, SHORT>(sc, r.o, r.d, T_MIN, r.tmax, h, st, vc)) return mk(0, 0, 0);
         return r.c;
     }
-    bool hit = trace<false, SHORT>(sc, r.o, r.d, T_MIN, r.tmax, h, st);
+    bool hit = trace<false, SHORT>(sc, r.o, r.d, T_MIN, r.tmax, h, st, vc);
     if (r.mode == 1) return (hit && sc.prims[h.prim].area_light == r.light) ? r.c : mk(0, 0, 0);
     if (hit) {
         if (sc.prims[h.prim].area_light != r.light) return mk(0, 0, 0);
@@ -160,7 +160,8 @@ DEV V radiance_pt(const DevScene& sc, V o, V d, const Rng& rng, int max_bounce, 
                 int nr = nee_requests(sc, b, si, li, rng, dim, beta * (float)n, req);
                 for (int k = 0; k < nr; k++) {
                     if (req[k].mode == 0) n_any++; else n_closest++;
-                    L = L + resolve_shadow<SHORT>(sc, req[k], st);
+                    VisitCount vc;
+                    L = L + resolve_shadow<SHORT>(sc, req[k], st, vc);
                 }
             }
             LSample bs;
@@ -445,6 +446,7 @@ __global__ void k_commit(Counters* ctr, int capacity) {
     ctr->bounces += ctr->n_active;
     ctr->n_next = 0;
     ctr->n_shadow = 0;
+    ctr->n_shadow_closest = 0;
 }
 
 template <int BLOCK, int SHORT>
@@ -632,6 +634,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const DevScene* __restrict__ sc
     for (int r = 0; r < 2; r++) {
         bool has = r < nr;
         int s = wave_append(&ctr->n_shadow, has);
+        wave_append(&ctr->n_shadow_closest, has && req[r].mode != 0);
         if (has) {
             const ShadowReq& R = req[r];
             sh.o[s] = make_float4(R.o.x, R.o.y, R.o.z, R.tmax);
@@ -642,14 +645,10 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const DevScene* __restrict__ sc
     }
 }
 
+// Resolves one queued shadow request and adds its contribution to the film.
 template <int BLOCK, int SHORT>
-__global__ __launch_bounds__(BLOCK) void k_shadow(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
-                                                  ShadowSoA sh, Counters* __restrict__ ctr, float4* __restrict__ film,
-                                                  int* __restrict__ overflow) {
-    __shared__ int lds[SHORT * BLOCK];
-    int i = blockIdx.x * BLOCK + threadIdx.x;
-    int n = ctr->n_shadow;
-    if (i >= n) return;
+DEV void shadow_one(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, ShadowSoA sh,
+                    float4* __restrict__ film, int* __restrict__ overflow, int* lds, int i, VisitCount& vc) {
     TraversalStack<SHORT> st;
     st.lds = lds + threadIdx.x; st.bdim = BLOCK;
     st.glob = overflow + (size_t)blockIdx.x * BLOCK + threadIdx.x; st.gstride = gridDim.x * BLOCK;
@@ -658,13 +657,29 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(const DevScene* __restrict__ s
     const DevScene& sc = scenes[jobs[aux.y].scene];
     ShadowReq r;
     r.o = f4v(o); r.tmax = o.w; r.d = f4v(d); r.mode = __float_as_int(d.w); r.light = aux.x; r.c = f4v(c);
-    V v = resolve_shadow<SHORT>(sc, r, st);
+    V v = resolve_shadow<SHORT>(sc, r, st, vc);
     if (!black(v)) film_add(film, (int64_t)__float_as_int(c.w), v);
 }
 
+template <int BLOCK, int SHORT, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_shadow(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
+                                                  ShadowSoA sh, Counters* __restrict__ ctr, float4* __restrict__ film,
+                                                  int* __restrict__ overflow, unsigned long long* __restrict__ trav) {
+    __shared__ int lds[SHORT * BLOCK];
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    int n = ctr->n_shadow;
+    VisitCount vc;
+    if (i < n) shadow_one<BLOCK, SHORT>(scenes, jobs, sh, film, overflow, lds, i, vc);
+    if (COUNT) {
+        unsigned long long a = vc.nodes, b = vc.tris;
+        for (int off = 32; off > 0; off >>= 1) { a += __shfl_down(a, off); b += __shfl_down(b, off); }
+        if (lane_id() == 0) { atomicAdd(&trav[2], a); atomicAdd(&trav[3], b); }
+    }
+}
+
 __global__ void k_count_rays(Counters* ctr) {
-    ctr->rays_closest += ctr->n_active;
-    ctr->rays_any += ctr->n_shadow;
+    ctr->rays_closest += ctr->n_active + ctr->n_shadow_closest;
+    ctr->rays_any += ctr->n_shadow - ctr->n_shadow_closest;
 }
 
 void wavefront_init(WavefrontBuffers& w, const DevJob* jobs, int n_jobs, int64_t work_total, hipStream_t s) {
@@ -710,8 +725,12 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     if (timing) hipEventRecord(prof->ev[3], s);
     hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(1), 0, s, w.ctr);
     if (timing) hipEventRecord(prof->ev[4], s);
-    hipLaunchKernelGGL((k_shadow<B, SHORT_STACK>), dim3((2 * cap + B - 1) / B), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr,
-                       film, w.overflow);
+    if (trav)
+        hipLaunchKernelGGL((k_shadow<B, SHORT_STACK, true>), dim3((2 * cap + B - 1) / B), dim3(B), 0, s, scenes, jobs, w.sh,
+                           w.ctr, film, w.overflow, trav);
+    else
+        hipLaunchKernelGGL((k_shadow<B, SHORT_STACK, false>), dim3((2 * cap + B - 1) / B), dim3(B), 0, s, scenes, jobs, w.sh,
+                           w.ctr, film, w.overflow, trav);
     if (timing) hipEventRecord(prof->ev[5], s);
     std::swap(w.a, w.b);
 }

@@ -103,7 +103,8 @@ int batch_render_impl(Ctx* c, const dfr_job* jobs, int n, int mode, float* const
         st.extend_seconds = c->prof_stats.extend_seconds; st.shade_seconds = c->prof_stats.shade_seconds;
         st.shadow_seconds = c->prof_stats.shadow_seconds; st.extend_launches = c->prof_stats.extend_launches;
         st.extend_rays = c->prof_stats.extend_rays; st.extend_nodes = c->prof_stats.extend_nodes;
-        st.extend_tris = c->prof_stats.extend_tris;
+        st.extend_tris = c->prof_stats.extend_tris; st.shadow_rays = c->prof_stats.shadow_rays;
+        st.shadow_nodes = c->prof_stats.shadow_nodes; st.shadow_tris = c->prof_stats.shadow_tris;
     }
     float ms = 0;
     hipEventElapsedTime(&ms, c->ev0, c->ev1);

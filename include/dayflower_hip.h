@@ -269,6 +269,9 @@ typedef struct dfr_stats {
     /* with DFR_KEY_COUNT_TRAVERSAL: visits made by the extend kernel       */
     int64_t  extend_nodes;       /* two-wide nodes fetched                  */
     int64_t  extend_tris;        /* triangle tests                          */
+    int64_t  shadow_rays;        /* shadow-stage queries (any and closest)  */
+    int64_t  shadow_nodes;       /* with DFR_KEY_COUNT_TRAVERSAL            */
+    int64_t  shadow_tris;
 } dfr_stats;
 int32_t dfr_get_stats(const dfr_ctx* ctx, dfr_stats* stats);
 

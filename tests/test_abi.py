@@ -46,7 +46,8 @@ def c_layout(tmp_path_factory):
         "dfr_scene_desc": ["n_vertices", "positions", "normals", "uvs", "n_triangles", "indices", "n_primitives",
                            "primitives", "n_materials", "materials", "n_lights", "lights", "camera"],
         "dfr_stats": ["seconds", "path_samples", "rays_closest", "rays_any", "bounces", "extend_seconds",
-                      "shadow_seconds", "extend_launches", "extend_rays", "extend_nodes", "extend_tris"],
+                      "shadow_seconds", "extend_launches", "extend_rays", "extend_nodes", "extend_tris",
+                      "shadow_rays", "shadow_nodes", "shadow_tris"],
         "dfr_job": ["scene", "width", "height", "spp", "maximum_bounce", "minimum_bounce_rr", "seed", "camera"],
         "dfr_server_opts": ["hbm_budget_bytes", "max_active", "pool_paths", "spp_quantum"],
         "dfr_server_stats": ["completed", "rejected", "path_samples", "busy_seconds", "peak_bytes", "peak_active"],
