@@ -31,7 +31,8 @@ def log(*a):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_icosphere1m_1080p.txt")
-DEFAULT_FRAME = {"triangles": 1_000_000, "width": 1920, "height": 1080, "spp": 64, "max_bounce": 8, "no_perez": False}
+DEFAULT_FRAME = {"triangles": 1_000_000, "width": 1920, "height": 1080, "spp": 64, "max_bounce": 8, "no_perez": False,
+                 "scene": "icosphere", "kernel": "wavefront"}
 
 
 def hbm_traffic(kernel, args):
@@ -204,6 +205,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-perez", action="store_true")
+    ap.add_argument("--scene", choices=["icosphere", "mixed"], default="icosphere",
+                    help="icosphere: Plastic mesh (default frame); mixed: the Matte/Glass/Metal/Disney icosphere "
+                         "of the batched and serving workloads")
     ap.add_argument("--config", choices=["frame", "batched", "serving"], default="frame")
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--rate", type=float, default=200.0)
@@ -228,8 +232,13 @@ def main():
     from dayflower_amd.renderer import GPURenderer
 
     t0 = time.perf_counter()
-    sc = S.icosphere_scene(args.triangles, seed=0x5EED, width=args.width, height=args.height,
-                           with_perez=not args.no_perez)
+    if args.scene == "mixed":
+        from dayflower_amd import workloads as W
+        sc = W.mixed_material_icosphere(args.triangles, 0x5EED, with_perez=not args.no_perez, width=args.width,
+                                        height=args.height)
+    else:
+        sc = S.icosphere_scene(args.triangles, seed=0x5EED, width=args.width, height=args.height,
+                               with_perez=not args.no_perez)
     log(f"bench: scene {sc.n_triangles} triangles generated in {time.perf_counter() - t0:.1f} s")
     r = GPURenderer()
     r.setScene(sc)
@@ -331,7 +340,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded scene generator, no dataset)",
-        "config": {"workload": f"displaced icosphere path tracing ({sc.n_triangles} triangles), "
+        "config": {"workload": f"{'mixed-material ' if args.scene == 'mixed' else ''}displaced icosphere path tracing ({sc.n_triangles} triangles), "
                                f"{args.width}x{args.height}, {args.spp} spp per step, maxBounce {args.max_bounce}, "
                                f"RR from 5, {'Perez sky + ' if not args.no_perez else ''}point light",
                    "kernel": args.kernel, "triangles": sc.n_triangles, "width": args.width, "height": args.height,

@@ -433,7 +433,7 @@ size_t wavefront_bytes(int capacity) { return (size_t)capacity * (9 * 16 + 8 + 2
 int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counters* out_ctr) {
     int rc = ensure_wavefront(wf_params.capacity);
     if (rc) return rc;
-    int64_t threads = std::max<int64_t>(2 * (int64_t)wf_params.capacity, 1);
+    int64_t threads = wavefront_stack_threads(wf_params.capacity);
     if ((rc = ensure_overflow(threads))) return rc;
     wf.overflow = (int*)overflow.ptr;
     wavefront_init(wf, (const DevJob*)job_buf.ptr, n_jobs, work_total, stream);

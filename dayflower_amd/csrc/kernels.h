@@ -44,16 +44,26 @@ struct ShadowSoA {
 struct Counters {
     int32_t n_active;      // paths in the current queue
     int32_t n_next;        // continuing paths appended by shade
-    int32_t n_shadow;      // shadow rays appended by shade
     int32_t taken;         // new paths appended by generate
+    int32_t pad;
+    // shadow requests appended by shade: any-hit requests fill the shadow
+    // queue from the front (count in the low 32 bits), closest-hit requests
+    // from the back (count in the high 32 bits); one 64-bit atomic reserves
+    // both.
+    uint64_t n_shadow2;
     int64_t work_next;     // next work item
     int64_t work_total;
     int64_t rays_closest;
     int64_t rays_any;
     int64_t bounces;
-    int32_t n_shadow_closest;  // shadow requests of this iteration that need the closest hit
-    int32_t pad;
 };
+static_assert(sizeof(Counters) == 64, "Counters is one 64-byte line");
+
+// Threads of the widest traversal launch (the shadow stage over a pool of
+// `capacity` paths: 2 * capacity requests plus one wave of padding between
+// the any-hit and closest-hit partitions, rounded up to a 256-thread block);
+// the HBM overflow stacks are sized for it.
+inline int64_t wavefront_stack_threads(int capacity) { return 2 * (int64_t)capacity + 64 + 256; }
 
 struct WavefrontBuffers {
     PathSoA a, b;

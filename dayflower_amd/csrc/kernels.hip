@@ -6,7 +6,8 @@
 //    :272-326, 451-718).
 //  * wavefront: generate -> extend (closest hit) -> material sort -> shade ->
 //    shadow (NEE visibility) -> commit, over a path pool in HBM with
-//    wave-aggregated queue appends (ballot + popcount prefix).
+//    block-aggregated queue appends (wave ballot + popcount prefix, one
+//    atomic per block and queue).
 //  * ray queries: closest/any hit for a ray array (hit-parity checks and the
 //    traversal benchmark).
 #include "device.h"
@@ -19,17 +20,58 @@ namespace dfr {
 // ---------------------------------------------------------------------------
 DEV int lane_id() { return __lane_id(); }
 
-// Appends the predicate-true lanes of the wave to a queue with one atomic per
-// wave; returns each lane's slot (or -1).
-DEV int wave_append(int* counter, bool pred) {
-    uint64_t mask = __ballot(pred);
-    if (mask == 0) return -1;
-    int leader = __ffsll((unsigned long long)mask) - 1;
-    int base = 0;
-    if (lane_id() == leader) base = atomicAdd(counter, __popcll(mask));
-    base = __shfl(base, leader);
-    int rank = __popcll(mask & ((1ull << lane_id()) - 1ull));
-    return pred ? base + rank : -1;
+// Block-wide queue reservation.  Each thread asks for `a` continuing-path
+// slots (0/1), `b` any-hit shadow slots (0/1) and `c` closest-hit shadow
+// slots (0..2); the block makes one atomic on n_next and one 64-bit atomic on
+// n_shadow2 (both shadow partitions at once).  Returns the thread's first slot
+// of each kind.  Every thread of the block must call it.
+template <int BLOCK>
+DEV void block_reserve(Counters* ctr, bool a, bool b, int c, int& sa, int& sb, int& sc) {
+    constexpr int NW = BLOCK / 64;
+    __shared__ int wsum[3][NW];
+    __shared__ int wbase[3][NW];
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const uint64_t ma = __ballot(a), mb = __ballot(b), mc0 = __ballot(c & 1), mc1 = __ballot(c >> 1);
+    const int pa = __popcll(ma & lt), pb = __popcll(mb & lt);
+    const int pc = __popcll(mc0 & lt) + 2 * __popcll(mc1 & lt);
+    if (lane == 0) {
+        wsum[0][w] = __popcll(ma);
+        wsum[1][w] = __popcll(mb);
+        wsum[2][w] = __popcll(mc0) + 2 * __popcll(mc1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ta = 0, tb = 0, tc = 0;
+        for (int k = 0; k < NW; k++) {
+            wbase[0][k] = ta; ta += wsum[0][k];
+            wbase[1][k] = tb; tb += wsum[1][k];
+            wbase[2][k] = tc; tc += wsum[2][k];
+        }
+        int ga = ta ? atomicAdd(&ctr->n_next, ta) : 0;
+        unsigned long long g = 0;
+        if (tb | tc)
+            g = atomicAdd((unsigned long long*)&ctr->n_shadow2, (unsigned long long)tb | ((unsigned long long)tc << 32));
+        const int gb = (int)(g & 0xffffffffull), gc = (int)(g >> 32);
+        for (int k = 0; k < NW; k++) { wbase[0][k] += ga; wbase[1][k] += gb; wbase[2][k] += gc; }
+    }
+    __syncthreads();
+    sa = wbase[0][w] + pa;
+    sb = wbase[1][w] + pb;
+    sc = wbase[2][w] + pc;
+}
+
+// Adds `n` to job_done[job] for every lane with pred, one atomic per distinct
+// job in the wave.
+DEV void wave_count_jobs(int* job_done, bool pred, uint32_t job) {
+    uint64_t left = __ballot(pred);
+    while (left) {
+        const int leader = __ffsll((unsigned long long)left) - 1;
+        const uint32_t j = __shfl(job, leader);
+        const uint64_t same = __ballot(pred && job == j) & left;
+        if (lane_id() == leader) atomicAdd(&job_done[j], __popcll(same));
+        left &= ~same;
+    }
 }
 
 DEV void film_add(float4* film, int64_t idx, V c) {
@@ -50,16 +92,20 @@ DEV void film_add(float4* film, int64_t idx, V c) {
 struct ShadowReq {
     V o, d;
     float tmax;
-    int mode;
+    int mode;   // 0, 1, 2 as above; -1: no request
     int light;
     V c;
 };
 
-DEV int nee_requests(const DevScene& sc, const Bsdf& b, const Surf& si, int li, const Rng& rng, uint32_t dim,
-                     V scale, ShadowReq* out) {
+// Fills `ls_req` with the light-sample request (mode 0 or 1) and `bs_req`
+// with the BSDF-sample request (mode 2); a slot without a request keeps
+// mode -1.  Two named slots (not an array) keep both in registers.
+DEV void nee_requests(const DevScene& sc, const Bsdf& b, const Surf& si, int li, const Rng& rng, uint32_t dim,
+                      V scale, ShadowReq& ls_req, ShadowReq& bs_req) {
+    ls_req.mode = -1;
+    bs_req.mode = -1;
     const DevLight& L = sc.lights[li];
     const uint32_t T = BX_ALL_EXCEPT_SPECULAR;
-    int n = 0;
     LightSmp ls;
     bool have = sample_light(sc, L, si, rng.get(dim + 1), rng.get(dim + 2), rng.get(dim + 5), ls);
     V ns = si.s.w;
@@ -67,19 +113,19 @@ DEV int nee_requests(const DevScene& sc, const Bsdf& b, const Surf& si, int li, 
         if (have && !black(ls.L) && ls.pdf > 0.0f) {
             V f = bsdf_f(b, T, ls.wi) * fabsf(dot(ls.wi, ns));
             if (!black(f)) {
-                ShadowReq& r = out[n++];
+                ShadowReq& r = ls_req;
                 r.o = si.p; r.d = nrm(ls.p - si.p); r.tmax = fabsf(len(si.p - ls.p)) + 0.001f;
                 r.mode = 0; r.light = li; r.c = scale * f * ls.L / ls.pdf;
             }
         }
-        return n;
+        return;
     }
     if (have && !black(ls.L) && ls.pdf > 0.0f) {
         V f = bsdf_f(b, T, ls.wi) * fabsf(dot(ls.wi, ns));
         float sp = bsdf_pdf(b, T, ls.wi);
         if (!black(f)) {
             float w = power_heuristic(ls.pdf, sp);
-            ShadowReq& r = out[n++];
+            ShadowReq& r = ls_req;
             r.o = si.p; r.d = nrm(ls.p - si.p); r.tmax = fabsf(len(si.p - ls.p)) + 0.001f;
             r.mode = L.type == DFR_LIGHT_DIFFUSE_AREA ? 1 : 0; r.light = li;
             r.c = scale * f * ls.L * w / ls.pdf;
@@ -98,13 +144,12 @@ DEV int nee_requests(const DevScene& sc, const Bsdf& b, const Surf& si, int li, 
                 else w = power_heuristic(bs.pdf, lp);
             }
             if (ok) {
-                ShadowReq& r = out[n++];
+                ShadowReq& r = bs_req;
                 r.o = si.p; r.d = nrm(bs.wi); r.tmax = FLT_MAX;
                 r.mode = 2; r.light = li; r.c = scale * f * w / bs.pdf;
             }
         }
     }
-    return n;
 }
 
 template <int SHORT>
@@ -156,12 +201,16 @@ DEV V radiance_pt(const DevScene& sc, V o, V d, const Rng& rng, int max_bounce, 
             if (sc.n_lights > 0 && count_nonspecular(b) > 0) {
                 int n = sc.n_lights;
                 int li = min((int)(rng.get(dim) * n), n - 1);
-                ShadowReq req[2];
-                int nr = nee_requests(sc, b, si, li, rng, dim, beta * (float)n, req);
-                for (int k = 0; k < nr; k++) {
-                    if (req[k].mode == 0) n_any++; else n_closest++;
-                    VisitCount vc;
-                    L = L + resolve_shadow<SHORT>(sc, req[k], st, vc);
+                ShadowReq r0, r1;
+                nee_requests(sc, b, si, li, rng, dim, beta * (float)n, r0, r1);
+                VisitCount vc;
+                if (r0.mode >= 0) {
+                    if (r0.mode == 0) n_any++; else n_closest++;
+                    L = L + resolve_shadow<SHORT>(sc, r0, st, vc);
+                }
+                if (r1.mode >= 0) {
+                    n_closest++;
+                    L = L + resolve_shadow<SHORT>(sc, r1, st, vc);
                 }
             }
             LSample bs;
@@ -445,8 +494,7 @@ __global__ void k_commit(Counters* ctr, int capacity) {
     ctr->n_active = base + (int)take;
     ctr->bounces += ctr->n_active;
     ctr->n_next = 0;
-    ctr->n_shadow = 0;
-    ctr->n_shadow_closest = 0;
+    ctr->n_shadow2 = 0;
 }
 
 template <int BLOCK, int SHORT>
@@ -543,12 +591,19 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scatter(const int32_t* __restric
     if (i < n) perm[atomicAdd(&base[keys[i]], 1)] = i;
 }
 
+DEV void put_shadow(ShadowSoA sh, int s, const ShadowReq& R, int64_t film_idx, int job) {
+    sh.o[s] = make_float4(R.o.x, R.o.y, R.o.z, R.tmax);
+    sh.d[s] = make_float4(R.d.x, R.d.y, R.d.z, __int_as_float(R.mode));
+    sh.c[s] = make_float4(R.c.x, R.c.y, R.c.z, __uint_as_float((uint32_t)film_idx));
+    sh.aux[s] = make_int4(R.light, job, (int)(film_idx >> 32), 0);
+}
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_shade(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
                                                  PathSoA q, const float4* __restrict__ hits,
                                                  const int32_t* __restrict__ perm, PathSoA out, ShadowSoA sh,
                                                  Counters* __restrict__ ctr, float4* __restrict__ film, int use_perm,
-                                                 int* __restrict__ job_done) {
+                                                 int* __restrict__ job_done, int capacity) {
     int k = blockIdx.x * BLOCK + threadIdx.x;
     int n = ctr->n_active;
     bool active = k < n;
@@ -557,8 +612,9 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const DevScene* __restrict__ sc
     V no = mk(0, 0, 0), nd = mk(0, 0, 1), nbeta = mk(0, 0, 0);
     float neta = 1.0f;
     uint4 ninfo = make_uint4(0, 0, 0, 0);
-    ShadowReq req[2];
-    int nr = 0;
+    ShadowReq r0, r1;
+    r0.mode = -1;
+    r1.mode = -1;
     int64_t film_idx = 0;
     if (active) {
         uint4 info = q.info[i];
@@ -594,7 +650,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const DevScene* __restrict__ sc
                         if (sc.n_lights > 0 && count_nonspecular(b) > 0) {
                             int nl = sc.n_lights;
                             int li = min((int)(rng.get(dim) * nl), nl - 1);
-                            nr = nee_requests(sc, b, si, li, rng, dim, beta * (float)nl, req);
+                            nee_requests(sc, b, si, li, rng, dim, beta * (float)nl, r0, r1);
                         }
                         LSample bs;
                         if (bsdf_sample(b, BX_ALL, rng.get(dim + 6), rng.get(dim + 7), bs) && !black(bs.f) && bs.pdf != 0.0f) {
@@ -623,26 +679,23 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const DevScene* __restrict__ sc
             // each NEE request carries the film index of its path
         }
     }
-    if (job_done && active && !cont) atomicAdd(&job_done[q.info[i].z], 1);
-    int slot = wave_append(&ctr->n_next, cont);
+    if (job_done) wave_count_jobs(job_done, active && !cont, active ? q.info[i].z : 0u);
+    // queue appends: one reservation per block
+    const bool has0 = r0.mode >= 0, has1 = r1.mode >= 0;
+    const bool any0 = has0 && r0.mode == 0;
+    const int ncl = (has0 && !any0 ? 1 : 0) + (has1 ? 1 : 0);
+    int s_path, s_any, s_cl;
+    block_reserve<BLOCK>(ctr, cont, any0, ncl, s_path, s_any, s_cl);
     if (cont) {
-        out.o[slot] = make_float4(no.x, no.y, no.z, 0.0f);
-        out.d[slot] = make_float4(nd.x, nd.y, nd.z, neta);
-        out.beta[slot] = make_float4(nbeta.x, nbeta.y, nbeta.z, 0.0f);
-        out.info[slot] = ninfo;
+        out.o[s_path] = make_float4(no.x, no.y, no.z, 0.0f);
+        out.d[s_path] = make_float4(nd.x, nd.y, nd.z, neta);
+        out.beta[s_path] = make_float4(nbeta.x, nbeta.y, nbeta.z, 0.0f);
+        out.info[s_path] = ninfo;
     }
-    for (int r = 0; r < 2; r++) {
-        bool has = r < nr;
-        int s = wave_append(&ctr->n_shadow, has);
-        wave_append(&ctr->n_shadow_closest, has && req[r].mode != 0);
-        if (has) {
-            const ShadowReq& R = req[r];
-            sh.o[s] = make_float4(R.o.x, R.o.y, R.o.z, R.tmax);
-            sh.d[s] = make_float4(R.d.x, R.d.y, R.d.z, __int_as_float(R.mode));
-            sh.c[s] = make_float4(R.c.x, R.c.y, R.c.z, __int_as_float((int)film_idx));
-            sh.aux[s] = make_int4(R.light, (int)q.info[i].z, 0, 0);
-        }
-    }
+    const int last = 2 * capacity - 1;  // closest-hit requests fill the shadow queue from here down
+    const int job = active ? (int)q.info[i].z : 0;
+    if (has0) put_shadow(sh, any0 ? s_any : last - s_cl, r0, film_idx, job);
+    if (has1) put_shadow(sh, last - (s_cl + (has0 && !any0 ? 1 : 0)), r1, film_idx, job);
 }
 
 // Resolves one queued shadow request and adds its contribution to the film.
@@ -658,18 +711,26 @@ DEV void shadow_one(const DevScene* __restrict__ scenes, const DevJob* __restric
     ShadowReq r;
     r.o = f4v(o); r.tmax = o.w; r.d = f4v(d); r.mode = __float_as_int(d.w); r.light = aux.x; r.c = f4v(c);
     V v = resolve_shadow<SHORT>(sc, r, st, vc);
-    if (!black(v)) film_add(film, (int64_t)__float_as_int(c.w), v);
+    if (!black(v)) film_add(film, ((int64_t)aux.z << 32) | __float_as_uint(c.w), v);
 }
 
 template <int BLOCK, int SHORT, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_shadow(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
                                                   ShadowSoA sh, Counters* __restrict__ ctr, float4* __restrict__ film,
-                                                  int* __restrict__ overflow, unsigned long long* __restrict__ trav) {
+                                                  int* __restrict__ overflow, unsigned long long* __restrict__ trav,
+                                                  int capacity) {
     __shared__ int lds[SHORT * BLOCK];
-    int i = blockIdx.x * BLOCK + threadIdx.x;
-    int n = ctr->n_shadow;
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const unsigned long long n2 = ctr->n_shadow2;
+    const int n_any = (int)(n2 & 0xffffffffull), n_cl = (int)(n2 >> 32);
+    // any-hit requests at [0, n_any); closest-hit ones from the queue's end,
+    // started on a fresh wave so the two kinds do not share waves
+    const int cl0 = (n_any + 63) & ~63;
+    int slot = -1;
+    if (i < n_any) slot = i;
+    else if (i >= cl0 && i - cl0 < n_cl) slot = 2 * capacity - 1 - (i - cl0);
     VisitCount vc;
-    if (i < n) shadow_one<BLOCK, SHORT>(scenes, jobs, sh, film, overflow, lds, i, vc);
+    if (slot >= 0) shadow_one<BLOCK, SHORT>(scenes, jobs, sh, film, overflow, lds, slot, vc);
     if (COUNT) {
         unsigned long long a = vc.nodes, b = vc.tris;
         for (int off = 32; off > 0; off >>= 1) { a += __shfl_down(a, off); b += __shfl_down(b, off); }
@@ -678,8 +739,9 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(const DevScene* __restrict__ s
 }
 
 __global__ void k_count_rays(Counters* ctr) {
-    ctr->rays_closest += ctr->n_active + ctr->n_shadow_closest;
-    ctr->rays_any += ctr->n_shadow - ctr->n_shadow_closest;
+    const unsigned long long n2 = ctr->n_shadow2;
+    ctr->rays_closest += ctr->n_active + (int64_t)(n2 >> 32);
+    ctr->rays_any += (int64_t)(n2 & 0xffffffffull);
 }
 
 void wavefront_init(WavefrontBuffers& w, const DevJob* jobs, int n_jobs, int64_t work_total, hipStream_t s) {
@@ -721,16 +783,17 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     }
     if (timing) hipEventRecord(prof->ev[2], s);
     hipLaunchKernelGGL((k_shade<B>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.perm, w.b, w.sh, w.ctr, film,
-                       use_perm, w.job_done);
+                       use_perm, w.job_done, cap);
     if (timing) hipEventRecord(prof->ev[3], s);
     hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(1), 0, s, w.ctr);
     if (timing) hipEventRecord(prof->ev[4], s);
+    const int sgrid = (int)((2 * (int64_t)cap + 64 + B - 1) / B);  // within wavefront_stack_threads(cap)
     if (trav)
-        hipLaunchKernelGGL((k_shadow<B, SHORT_STACK, true>), dim3((2 * cap + B - 1) / B), dim3(B), 0, s, scenes, jobs, w.sh,
-                           w.ctr, film, w.overflow, trav);
+        hipLaunchKernelGGL((k_shadow<B, SHORT_STACK, true>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr, film,
+                           w.overflow, trav, cap);
     else
-        hipLaunchKernelGGL((k_shadow<B, SHORT_STACK, false>), dim3((2 * cap + B - 1) / B), dim3(B), 0, s, scenes, jobs, w.sh,
-                           w.ctr, film, w.overflow, trav);
+        hipLaunchKernelGGL((k_shadow<B, SHORT_STACK, false>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr, film,
+                           w.overflow, trav, cap);
     if (timing) hipEventRecord(prof->ev[5], s);
     std::swap(w.a, w.b);
 }

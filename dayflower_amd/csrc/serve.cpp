@@ -233,7 +233,7 @@ static void server_loop(Server* S) {
     Ctx* c = S->ctx;
     std::lock_guard<std::mutex> ctx_lock(c->mutex);  // the server owns the context while it runs
     hipSetDevice(c->device);
-    if (c->ensure_wavefront(S->opts.pool_paths) || c->ensure_overflow(2 * (int64_t)S->opts.pool_paths)) {
+    if (c->ensure_wavefront(S->opts.pool_paths) || c->ensure_overflow(wavefront_stack_threads(S->opts.pool_paths))) {
         std::lock_guard<std::mutex> l(S->m);
         S->error = c->error;
         S->stopping = true;
