@@ -176,6 +176,8 @@ def run_serving(args):
             time.sleep(0.0005)
         r.batchRender([stream[k][1]], mode=abi.BATCH_PLAIN, readback=True)
         done[k] = time.perf_counter() - submit_t[k]
+        if (k + 1) % 50 == 0:
+            log(f"serving: plain {k + 1}/{len(q)} requests done")
     th.join()
     wall = time.perf_counter() - t0
     lat = np.array([done[k] for k in q])

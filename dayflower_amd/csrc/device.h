@@ -344,7 +344,10 @@ struct VisitCount {
     int nodes = 0, tris = 0;
 };
 
-template <bool ANY, int SHORT>
+// With REFINE (closest hit only), the winner's distance is recomputed in
+// FP64 for the ray-query outputs; the path tracer keeps the float distance of
+// the reference intersection routines, from which its next rays start.
+template <bool ANY, int SHORT, bool REFINE = false>
 DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit, TraversalStack<SHORT>& st,
                VisitCount& vc) {
     bool found = false;
@@ -395,8 +398,8 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
         if (st.sp == 0) break;
         cur = (uint32_t)st.pop();
     }
-    if (!ANY && found && hit.tri < 0) hit.t = analytic_t64(s.prims[hit.prim], o, d, hit.t);
-    if (!ANY && found && hit.tri >= 0) {
+    if (REFINE && !ANY && found && hit.tri < 0) hit.t = analytic_t64(s.prims[hit.prim], o, d, hit.t);
+    if (REFINE && !ANY && found && hit.tri >= 0) {
         // The float watertight t loses digits on small, oblique triangles;
         // the winner's distance to its plane is recomputed in FP64.
         int4 ix = s.tri_idx[hit.tri];
@@ -413,10 +416,10 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
     return found;
 }
 
-template <bool ANY, int SHORT>
+template <bool ANY, int SHORT, bool REFINE = false>
 DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit, TraversalStack<SHORT>& st) {
     VisitCount vc;
-    return trace<ANY, SHORT>(s, o, d, tmin, tmax, hit, st, vc);
+    return trace<ANY, SHORT, REFINE>(s, o, d, tmin, tmax, hit, st, vc);
 }
 
 // ---------------------------------------------------------------------------

@@ -368,7 +368,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_rays(const DevScene* __restrict
         float4 a = rays[2 * i], b = rays[2 * i + 1];
         HitRec h;
         bool hit = any_hit ? trace<true, SHORT>(sc, f4v(a), f4v(b), a.w, b.w, h, st)
-                           : trace<false, SHORT>(sc, f4v(a), f4v(b), a.w, b.w, h, st);
+                           : trace<false, SHORT, true>(sc, f4v(a), f4v(b), a.w, b.w, h, st);
         prim[i] = hit ? h.prim : -1;
         tri[i] = hit ? h.tri : -1;
         tout[i] = hit ? h.t : INFINITY;
@@ -405,7 +405,7 @@ __global__ __launch_bounds__(BLOCK) void k_cast_primary(const DevScene* __restri
     }
     r[0] = o.x; r[1] = o.y; r[2] = o.z; r[3] = T_MIN; r[4] = d.x; r[5] = d.y; r[6] = d.z; r[7] = FLT_MAX;
     HitRec h;
-    bool hit = trace<false, SHORT>(*scene, o, d, T_MIN, FLT_MAX, h, st);
+    bool hit = trace<false, SHORT, true>(*scene, o, d, T_MIN, FLT_MAX, h, st);
     prim[pix] = hit ? h.prim : -1;
     tri[pix] = hit ? h.tri : -1;
     tout[pix] = hit ? h.t : INFINITY;

@@ -341,7 +341,52 @@ float analyticT(const Prim& p, const Ray& ray, float tMin, float tMax) {
 // Scene.intersection (M/scene/Scene.java:874-900) closest hit: every
 // primitive tested with the shrinking tMax; meshes through a BVH
 // (LeafBVHNode3F/TreeBVHNode3F visit order does not change the closest hit).
+// Test switch: recompute the winning closest hit's distance in FP64 (plane
+// distance for triangles, rectangles and planes; nearest double root for
+// spheres).  Off by default: the reference keeps the float distance.
+bool g_refine_hits = false;
+
+static float refineT(const Scene& sc, const Ray& ray, const Hit& h) {
+    const double ox = ray.o.x, oy = ray.o.y, oz = ray.o.z, dx = ray.d.x, dy = ray.d.y, dz = ray.d.z;
+    double ax, ay, az, nx, ny, nz;
+    if (h.tri >= 0) {
+        const Tri& t = sc.tris[h.tri];
+        ax = t.a.x; ay = t.a.y; az = t.a.z;
+        double e1x = (double)t.b.x - ax, e1y = (double)t.b.y - ay, e1z = (double)t.b.z - az;
+        double e2x = (double)t.c.x - ax, e2y = (double)t.c.y - ay, e2z = (double)t.c.z - az;
+        nx = e1y * e2z - e1z * e2y; ny = e1z * e2x - e1x * e2z; nz = e1x * e2y - e1y * e2x;
+    } else {
+        const Prim& p = sc.prims[h.prim];
+        if (p.shape == DFR_SHAPE_SPHERE) {
+            double qx = ox - p.data[0], qy = oy - p.data[1], qz = oz - p.data[2], r = p.data[3];
+            double a = dx * dx + dy * dy + dz * dz, b = 2.0 * (dx * qx + dy * qy + dz * qz);
+            double c = qx * qx + qy * qy + qz * qz - r * r, disc = b * b - 4.0 * a * c;
+            if (disc < 0.0) return h.t;
+            double q = b < 0.0 ? -0.5 * (b - std::sqrt(disc)) : -0.5 * (b + std::sqrt(disc));
+            if (q == 0.0) return h.t;
+            double t0 = q / a, t1 = c / q;
+            double t = std::fabs(t0 - h.t) < std::fabs(t1 - h.t) ? t0 : t1;
+            return t > 0.0 ? (float)t : h.t;
+        }
+        ax = p.data[0]; ay = p.data[1]; az = p.data[2];
+        if (p.shape == DFR_SHAPE_RECTANGLE) { nx = p.rectNormal.x; ny = p.rectNormal.y; nz = p.rectNormal.z; }
+        else { nx = p.data[9]; ny = p.data[10]; nz = p.data[11]; }
+    }
+    double den = nx * dx + ny * dy + nz * dz;
+    if (den == 0.0) return h.t;
+    double t = (nx * (ax - ox) + ny * (ay - oy) + nz * (az - oz)) / den;
+    return t > 0.0 ? (float)t : h.t;
+}
+
+static bool sceneClosestF(const Scene& sc, const Ray& ray, float tMin, float tMax, Hit* hit, bool anyHit);
+
 bool sceneClosest(const Scene& sc, const Ray& ray, float tMin, float tMax, Hit* hit, bool anyHit) {
+    bool found = sceneClosestF(sc, ray, tMin, tMax, hit, anyHit);
+    if (found && !anyHit && g_refine_hits) hit->t = refineT(sc, ray, *hit);
+    return found;
+}
+
+static bool sceneClosestF(const Scene& sc, const Ray& ray, float tMin, float tMax, Hit* hit, bool anyHit) {
     bool found = false;
     float best = tMax;
     for (int pi : sc.analytic) {
@@ -1536,6 +1581,7 @@ int32_t orc_camera_ray(const float* cam, float ix, float iy, float px, float py,
 }
 
 void orc_set_literal_tmin(int32_t on) { g_literal_tmin = on != 0; }
+void orc_set_refine_hits(int32_t on) { g_refine_hits = on != 0; }
 
 float orc_rng(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t dim) {
     return makeRng(seed, pixel, sample).get(dim);

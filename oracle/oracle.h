@@ -69,6 +69,8 @@ int32_t orc_camera_ray(const float* camera19, float ix, float iy, float px, floa
 /* Tests only: 1 makes the float triangle test compare t * det with tmin
  * literally, as the reference does (Triangle3F.java:570-574). */
 void orc_set_literal_tmin(int32_t on);
+/* test switch: FP64 recomputation of the winning closest-hit distance */
+void orc_set_refine_hits(int32_t on);
 float orc_rng(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t dim);
 
 float orc_triangle_t(const float* a, const float* b, const float* c, const float* ray, float tmin, float tmax);

@@ -50,6 +50,7 @@ def load():
     lib.orc_primary_rays.argtypes = [vp, i32, i32, ctypes.c_uint64, i32, vp]
     lib.orc_camera_ray.argtypes = [vp, f, f, f, f, vp]
     lib.orc_set_literal_tmin.argtypes = [i32]
+    lib.orc_set_refine_hits.argtypes = [i32]
     lib.orc_rng.restype = f
     lib.orc_rng.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
     for name in ("orc_triangle_t",):

@@ -151,7 +151,10 @@ def test_cornell_statistical_parity():
     channels) within 3 sigma, relative RMSE <= 1 %.  The oracle against
     itself on two seeds at these sample counts gives 99.54 % / 98.78 % and a
     relative RMSE of 0.58 %; a 1 % brightness bias drops the pixel fraction
-    to 82 % and raises the RMSE to 1.14 %.  (The per-pixel fraction sits
+    to 82 % and raises the RMSE to 1.14 %.  The test is sensitive enough to
+    see a 0.06 % shift: recomputing every hit distance in FP64 before
+    spawning the next ray (instead of the reference's float distance) fails
+    it, which is why only the ray-query outputs are refined.  (The per-pixel fraction sits
     below 99 % for an unbiased renderer because the glass sphere's caustic
     paths make the sample variance heavy-tailed.)"""
     from tests.oracle_lib import OracleScene, film_agreement
