@@ -214,6 +214,9 @@ def main():
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=2.0)
+    ap.add_argument("--pool", type=int, default=0, help="path pool capacity (0: library default)")
+    ap.add_argument("--traversal", type=int, default=-1, help="0: thread per ray, 1: persistent (default: library)")
+    ap.add_argument("--sort-keys", type=int, default=-1, help="0 or 8 (default: library)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="fill and copy MIB MiB on the device first (reference kernels for HBM counters)")
     args = ap.parse_args()
@@ -248,6 +251,9 @@ def main():
     r.setMaximumBounce(args.max_bounce)
     r.setMinimumBounceRussianRoulette(5)
     r.setKernel(0 if args.kernel == "wavefront" else 1)
+    r.setWavefront(pool_paths=args.pool or None, traversal=None if args.traversal < 0 else args.traversal,
+                   sort_keys=None if args.sort_keys < 0 else args.sort_keys)
+    wf = r.getWavefront()
     t0 = time.perf_counter()
     r.setup()
     log(f"bench: setup (BVH build + upload) {time.perf_counter() - t0:.1f} s")
@@ -346,7 +352,8 @@ def main():
                                f"{args.width}x{args.height}, {args.spp} spp per step, maxBounce {args.max_bounce}, "
                                f"RR from 5, {'Perez sky + ' if not args.no_perez else ''}point light",
                    "kernel": args.kernel, "triangles": sc.n_triangles, "width": args.width, "height": args.height,
-                   "spp": args.spp, "max_bounce": args.max_bounce},
+                   "spp": args.spp, "max_bounce": args.max_bounce,
+                   "wavefront": wf if args.kernel == "wavefront" else None},
         "mrays_per_s": round((tot["closest"] + tot["any"]) / wall / 1e6, 1),
         "mrays_closest_per_s": round(tot["closest"] / wall / 1e6, 1),
         "mrays_any_per_s": round(tot["any"] / wall / 1e6, 1),

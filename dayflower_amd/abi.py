@@ -30,6 +30,10 @@ KEY_LIGHT_PDF_MODE = 10
 KEY_SPP_PER_PASS = 11
 KEY_PROFILE = 12
 KEY_COUNT_TRAVERSAL = 13
+KEY_WF_BLOCK = 14
+KEY_WF_POOL_PATHS = 15
+KEY_WF_SORT_KEYS = 16
+KEY_WF_TRAVERSAL = 17
 KEY_MAXIMUM_DISTANCE = 101
 
 E_OK, E_NULL, E_ARGUMENT, E_STATE, E_DEVICE, E_MEMORY, E_STOPPED, E_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6, -7
@@ -132,6 +136,7 @@ ENTRY_POINTS = [
     ("dfr_batch_render", c_int32, [_c_void_p, P(Job), c_int32, c_int32, P(P(c_float))]),
     ("dfr_autotune", c_int32, [_c_void_p, P(Job), c_int32, P(c_int32)]),
     ("dfr_server_create", _c_void_p, [_c_void_p, P(ServerOpts)]),
+    ("dfr_wavefront_bytes", c_int64, [c_int32]),
     ("dfr_server_destroy", None, [_c_void_p]),
     ("dfr_server_submit", c_int64, [_c_void_p, P(Job)]),
     ("dfr_server_poll", c_int32, [_c_void_p, c_int64]),

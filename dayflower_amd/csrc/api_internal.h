@@ -70,6 +70,7 @@ struct WavefrontConfig {
     int block = 256;
     int capacity = 1 << 20;  // path pool size
     int sort_keys = 8;       // material-sort keys (0 = unsorted)
+    int traversal = 1;       // 0: thread per ray, 1: persistent with dynamic fetch
 };
 
 struct Ctx {
@@ -106,7 +107,7 @@ struct Ctx {
     // wavefront engine
     WavefrontConfig wf_params;
     WavefrontBuffers wf{};
-    DeviceBuffer wf_mem[17];
+    DeviceBuffer wf_mem[19];
     int wf_capacity = 0;
     std::vector<hipEvent_t> prof_events;   // 6 per wavefront iteration, reused
 

@@ -245,7 +245,7 @@ static void server_loop(Server* S) {
     Counters zero{};
     hipMemcpyAsync(c->wf.ctr, &zero, sizeof(Counters), hipMemcpyHostToDevice, c->stream);
     hipMemsetAsync(S->done_mem.ptr, 0, (size_t)S->job_capacity * sizeof(int), c->stream);
-    WavefrontParams p{c->wf_params.block, c->wf_params.sort_keys, 0};
+    WavefrontParams p{c->wf_params.block, c->wf_params.sort_keys, 0, c->wf_params.traversal};
     std::vector<int> done_host;
     Counters host{};
     for (;;) {
@@ -479,6 +479,10 @@ int32_t dfr_autotune(dfr_ctx* h, const dfr_job* jobs, int32_t n, int32_t* out) {
     if (!jobs) return c->fail(DFR_E_NULL, "jobs == null");
     if (n < 1) return c->fail(DFR_E_ARGUMENT, "n_jobs must be >= 1");
     return autotune_impl(c, jobs, n, out);
+}
+
+int64_t dfr_wavefront_bytes(int32_t pool_paths) {
+    return pool_paths > 0 ? (int64_t)wavefront_bytes(pool_paths) : DFR_E_ARGUMENT;
 }
 
 dfr_server* dfr_server_create(dfr_ctx* h, const dfr_server_opts* opts) {

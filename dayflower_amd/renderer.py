@@ -176,6 +176,17 @@ class GPURenderer:
     def setCountTraversal(self, on):
         self._seti(abi.KEY_COUNT_TRAVERSAL, int(bool(on)))
 
+    def setWavefront(self, block=None, pool_paths=None, sort_keys=None, traversal=None):
+        """Wavefront engine configuration (the values dfr_autotune picks)."""
+        for key, v in ((abi.KEY_WF_BLOCK, block), (abi.KEY_WF_POOL_PATHS, pool_paths),
+                       (abi.KEY_WF_SORT_KEYS, sort_keys), (abi.KEY_WF_TRAVERSAL, traversal)):
+            if v is not None:
+                self._seti(key, v)
+
+    def getWavefront(self):
+        return {"block": self._geti(abi.KEY_WF_BLOCK), "pool_paths": self._geti(abi.KEY_WF_POOL_PATHS),
+                "sort_keys": self._geti(abi.KEY_WF_SORT_KEYS), "traversal": self._geti(abi.KEY_WF_TRAVERSAL)}
+
     # -- rendering -----------------------------------------------------------
     def setup(self, compile_scene=True):
         self._c(self.lib.dfr_setup(self.ctx, 1 if compile_scene else 0))
@@ -302,6 +313,11 @@ class Server:
             msg = self.lib.dfr_last_error(renderer.ctx)
             raise DeviceException("dfr_server_create failed: " + (msg.decode() if msg else ""))
         self._jobs = {}
+
+    @staticmethod
+    def pool_bytes(pool_paths):
+        """Device bytes of the wavefront engine for `pool_paths` paths."""
+        return int(abi.load().dfr_wavefront_bytes(int(pool_paths)))
 
     def submit(self, job):
         rid = self.lib.dfr_server_submit(self.h, ctypes.byref(job))

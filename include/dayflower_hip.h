@@ -195,6 +195,11 @@ typedef struct dfr_ctx dfr_ctx;
 #define DFR_KEY_SPP_PER_PASS           11  /* samples per pixel per pass, 1    */
 #define DFR_KEY_PROFILE                12  /* 1: time each kernel launch       */
 #define DFR_KEY_COUNT_TRAVERSAL        13  /* 1: count BVH node / triangle visits */
+/* wavefront engine configuration (what dfr_autotune sets)                   */
+#define DFR_KEY_WF_BLOCK               14  /* threads per block: 128 or 256     */
+#define DFR_KEY_WF_POOL_PATHS          15  /* path pool capacity, >= 1024       */
+#define DFR_KEY_WF_SORT_KEYS           16  /* 0: no material sort, 8: sort      */
+#define DFR_KEY_WF_TRAVERSAL           17  /* 0: thread per ray, 1: persistent waves with dynamic ray fetch */
 #define DFR_KEY_MAXIMUM_DISTANCE      101  /* float, default 20                */
 
 /* new GPURenderer(observer): M/renderer/gpu/GPURenderer.java:55-57 with the
@@ -343,6 +348,9 @@ typedef struct dfr_server_opts {
 } dfr_server_opts;
 
 dfr_server* dfr_server_create(dfr_ctx* ctx, const dfr_server_opts* opts);
+/* Device bytes of the wavefront engine for a path pool of `pool_paths`
+ * (counted against hbm_budget_bytes before any film). */
+int64_t dfr_wavefront_bytes(int32_t pool_paths);
 void dfr_server_destroy(dfr_server* server);
 /* Submit a request (non-blocking); returns its id (>= 0) or an error. */
 int64_t dfr_server_submit(dfr_server* server, const dfr_job* job);

@@ -99,10 +99,10 @@ def test_trace_rays_closest_and_any_hit():
     assert ((pa >= 0) == (p >= 0)).mean() >= 0.9999
 
 
-@pytest.mark.parametrize("kernel", [0, 1])
-def test_cornell_path_tracing_same_streams(kernel):
-    """Wavefront (0) and plain (1) kernels against the oracle on the same
-    random streams.
+@pytest.mark.parametrize("kernel,traversal", [(0, 1), (0, 0), (1, 0)])
+def test_cornell_path_tracing_same_streams(kernel, traversal):
+    """Wavefront (0; persistent or thread-per-ray traversal) and plain (1)
+    kernels against the oracle on the same random streams.
 
     Same streams give the same paths up to float rounding: a path leaves the
     oracle's only where rounding flips a decision (a Russian-roulette or
@@ -119,6 +119,7 @@ def test_cornell_path_tracing_same_streams(kernel):
     r.setScene(sc)
     r.setSeed(42)
     r.setKernel(kernel)
+    r.setWavefront(traversal=traversal)
     r.setup()
     r.render(1)
     g1, cnt1 = r.film(res, res)
@@ -231,7 +232,7 @@ def test_server_alone_equals_in_mix():
     alone = [r.batchRender([j])[0] for j in jobs]
     # budget admits only a few films at a time, so admission waits for frees
     pool = 1 << 16
-    pool_bytes = pool * (9 * 16 + 8 + 2 * 4 * 16) + 4096  # path SoA, queues and counters
+    pool_bytes = Server.pool_bytes(pool)  # path SoA, queues and counters
     srv = Server(r, hbm_budget_bytes=pool_bytes + 3 * 64 * 32 * 16, max_active=4, pool_paths=pool)
     ids = []
     for j in jobs:
