@@ -215,7 +215,6 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=2.0)
     ap.add_argument("--pool", type=int, default=0, help="path pool capacity (0: library default)")
-    ap.add_argument("--traversal", type=int, default=-1, help="0: thread per ray, 1: persistent (default: library)")
     ap.add_argument("--sort-keys", type=int, default=-1, help="0 or 8 (default: library)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="fill and copy MIB MiB on the device first (reference kernels for HBM counters)")
@@ -251,8 +250,7 @@ def main():
     r.setMaximumBounce(args.max_bounce)
     r.setMinimumBounceRussianRoulette(5)
     r.setKernel(0 if args.kernel == "wavefront" else 1)
-    r.setWavefront(pool_paths=args.pool or None, traversal=None if args.traversal < 0 else args.traversal,
-                   sort_keys=None if args.sort_keys < 0 else args.sort_keys)
+    r.setWavefront(pool_paths=args.pool or None, sort_keys=None if args.sort_keys < 0 else args.sort_keys)
     wf = r.getWavefront()
     t0 = time.perf_counter()
     r.setup()

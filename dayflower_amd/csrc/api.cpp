@@ -416,12 +416,11 @@ int Ctx::ensure_wavefront(int capacity) {
     wf.hit = (float4*)A(wf_mem[8], c * 16);
     wf.perm = (int32_t*)A(wf_mem[9], c * 4);
     wf.keys = (int32_t*)A(wf_mem[10], c * 4);
-    wf.bhist = (int32_t*)A(wf_mem[16], 8 * (c / 128 + 1) * 4);
+    wf.bhist = (int32_t*)A(wf_mem[16], 8 * (c / 4096 + 1) * 4);
     wf.sh.o = (float4*)A(wf_mem[11], 2 * c * 16); wf.sh.d = (float4*)A(wf_mem[12], 2 * c * 16);
     wf.sh.c = (float4*)A(wf_mem[13], 2 * c * 16); wf.sh.aux = (int4*)A(wf_mem[14], 2 * c * 16);
     wf.ctr = (Counters*)A(wf_mem[15], sizeof(Counters));
-    wf.sh_hit = (float4*)A(wf_mem[17], 2 * c * 16);
-    wf.fetch = (int*)A(wf_mem[18], 2 * NFETCH * FETCH_STRIDE * sizeof(int));
+
     if (!ok) { wf.ctr = nullptr; return fail(DFR_E_MEMORY, "wavefront buffers"); }
     wf.capacity = capacity;
     wf_capacity = capacity;
@@ -429,8 +428,7 @@ int Ctx::ensure_wavefront(int capacity) {
 }
 
 size_t wavefront_bytes(int capacity) {
-    return (size_t)capacity * (9 * 16 + 8 + 2 * 5 * 16) + 8 * ((size_t)capacity / 128 + 1) * 4 + sizeof(Counters) +
-           2 * NFETCH * FETCH_STRIDE * sizeof(int);
+    return (size_t)capacity * (9 * 16 + 8 + 2 * 4 * 16) + 8 * ((size_t)capacity / 4096 + 1) * 4 + sizeof(Counters);
 }
 
 // Runs `jobs` (already on the device in job_buf) to completion through the
@@ -442,7 +440,7 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
     if ((rc = ensure_overflow(threads))) return rc;
     wf.overflow = (int*)overflow.ptr;
     wavefront_init(wf, (const DevJob*)job_buf.ptr, n_jobs, work_total, stream);
-    WavefrontParams p{wf_params.block, wf_params.sort_keys, (wf_params.capacity + 255) / 256, wf_params.traversal};
+    WavefrontParams p{wf_params.block, wf_params.sort_keys, (wf_params.capacity + 255) / 256};
     Counters host{};
     const int poll = 4;
     const int max_iterations = 1 << 20;  // far above max_bounce * (work / capacity + 1)
@@ -632,9 +630,6 @@ int32_t dfr_set_i32(dfr_ctx* h, int32_t key, int32_t v) {
         case DFR_KEY_WF_SORT_KEYS:
             if (v != 0 && v != 8) return c->fail(DFR_E_ARGUMENT, "sort keys is 0 or 8");
             c->wf_params.sort_keys = v; break;
-        case DFR_KEY_WF_TRAVERSAL:
-            if (v != 0 && v != 1) return c->fail(DFR_E_ARGUMENT, "traversal is 0 or 1");
-            c->wf_params.traversal = v; break;
         default: return c->fail(DFR_E_ARGUMENT, "unknown integer key");
     }
     return DFR_OK;
@@ -660,7 +655,6 @@ int32_t dfr_get_i32(const dfr_ctx* h, int32_t key, int32_t* v) {
         case DFR_KEY_WF_BLOCK: *v = c->wf_params.block; break;
         case DFR_KEY_WF_POOL_PATHS: *v = c->wf_params.capacity; break;
         case DFR_KEY_WF_SORT_KEYS: *v = c->wf_params.sort_keys; break;
-        case DFR_KEY_WF_TRAVERSAL: *v = c->wf_params.traversal; break;
         default: return DFR_E_ARGUMENT;
     }
     return DFR_OK;

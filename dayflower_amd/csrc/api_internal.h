@@ -70,7 +70,6 @@ struct WavefrontConfig {
     int block = 256;
     int capacity = 1 << 20;  // path pool size
     int sort_keys = 8;       // material-sort keys (0 = unsorted)
-    int traversal = 1;       // 0: thread per ray, 1: persistent with dynamic fetch
 };
 
 struct Ctx {
@@ -107,7 +106,7 @@ struct Ctx {
     // wavefront engine
     WavefrontConfig wf_params;
     WavefrontBuffers wf{};
-    DeviceBuffer wf_mem[19];
+    DeviceBuffer wf_mem[17];
     int wf_capacity = 0;
     std::vector<hipEvent_t> prof_events;   // 6 per wavefront iteration, reused
 

@@ -70,10 +70,8 @@ struct WavefrontBuffers {
     float4* hit;       // t, prim, tri (int bits)
     int32_t* perm;     // material-sorted order
     int32_t* keys;
-    int32_t* bhist;    // per-block key histograms / offsets (8 * blocks)
+    int32_t* bhist;    // per-tile key histograms / offsets (8 * tiles of SORT_TILE entries)
     ShadowSoA sh;
-    float4* sh_hit;    // shadow-ray hits of the persistent traversal: t, prim, tri
-    int* fetch;        // work-fetch counters of the persistent traversal (2 * NFETCH lines)
     int* overflow;     // traversal-stack overflow
     int capacity;      // path pool size
     Counters* ctr;
@@ -94,16 +92,7 @@ struct WavefrontParams {
     int block;          // threads per block
     int sort_keys;      // 0: no material sort, else number of sort keys
     int grid_cap;       // blocks of the queue kernels (capacity / block)
-    int traversal;      // 0: one thread per ray; 1: persistent waves with dynamic ray fetch
 };
-
-// Persistent traversal: work is fetched in chunks of FETCH_CHUNK queue
-// entries from NFETCH counters (each on its own 128-byte line, owning one
-// contiguous NFETCH-th of the queue), so no single address takes every
-// wave's atomic.
-constexpr int NFETCH = 32;
-constexpr int FETCH_STRIDE = 32;  // ints between two counters
-constexpr int FETCH_CHUNK = 64;
 
 // Optional instrumentation of one wavefront iteration: HIP events around the
 // extend, shade and shadow launches, and BVH visit counters for extend.

@@ -10,10 +10,6 @@
 //    atomic per block and queue).
 //  * ray queries: closest/any hit for a ray array (hit-parity checks and the
 //    traversal benchmark).
-#include <algorithm>
-#include <map>
-#include <mutex>
-
 #include "device.h"
 #include "kernels.h"
 
@@ -491,8 +487,7 @@ __global__ void k_generate(const DevJob* __restrict__ jobs, int n_jobs, PathSoA 
     out.info[slot] = make_uint4((uint32_t)pix, sample, (uint32_t)j, flags);
 }
 
-__global__ void k_commit(Counters* ctr, int capacity, int* fetch) {
-    for (int k = 0; k < 2 * NFETCH; k++) fetch[k * FETCH_STRIDE] = 0;
+__global__ void k_commit(Counters* ctr, int capacity) {
     int base = ctr->n_next;
     int64_t room = (int64_t)capacity - base;
     int64_t avail = ctr->work_total - ctr->work_next;
@@ -506,18 +501,24 @@ __global__ void k_commit(Counters* ctr, int capacity, int* fetch) {
 }
 
 template <int BLOCK, int SHORT>
-DEV void extend_one(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, PathSoA q,
-                    float4* __restrict__ hits, int* __restrict__ overflow, int* lds, int i, VisitCount& vc) {
+DEV float4 extend_one(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, PathSoA q,
+                      float4* __restrict__ hits, int* __restrict__ overflow, int* lds, int i, VisitCount& vc) {
     TraversalStack<SHORT> st;
     st.lds = lds + threadIdx.x; st.bdim = BLOCK;
     st.glob = overflow + (size_t)blockIdx.x * BLOCK + threadIdx.x; st.gstride = gridDim.x * BLOCK;
     uint4 info = q.info[i];
-    if (info.w & 0x80000000u) { hits[i] = make_float4(0.0f, __int_as_float(-2), __int_as_float(-1), 0.0f); return; }
-    const DevScene& sc = scenes[jobs[info.z].scene];
-    float4 o = q.o[i], d = q.d[i];
-    HitRec h;
-    bool hit = trace<false, SHORT>(sc, f4v(o), f4v(d), T_MIN, FLT_MAX, h, st, vc);
-    hits[i] = make_float4(h.t, __int_as_float(hit ? h.prim : -1), __int_as_float(h.tri), 0.0f);
+    float4 out;
+    if (info.w & 0x80000000u) {
+        out = make_float4(0.0f, __int_as_float(-2), __int_as_float(-1), 0.0f);
+    } else {
+        const DevScene& sc = scenes[jobs[info.z].scene];
+        float4 o = q.o[i], d = q.d[i];
+        HitRec h;
+        bool hit = trace<false, SHORT>(sc, f4v(o), f4v(d), T_MIN, FLT_MAX, h, st, vc);
+        out = make_float4(h.t, __int_as_float(hit ? h.prim : -1), __int_as_float(h.tri), 0.0f);
+    }
+    hits[i] = out;
+    return out;
 }
 
 // Material-sort key of a hit: material type of the hit (0..5), 6 = miss,
@@ -534,69 +535,116 @@ DEV int sort_key(const DevScene* __restrict__ scenes, const DevJob* __restrict__
 // material-sort key of each hit and the block's key histogram
 // (bhist[key * gridDim.x + block]); the histogram is built with LDS atomics,
 // so no global atomic is contended.
+// Closest hit for every queued path; with SORT the material-sort key of
+// each hit is written too.
 template <int BLOCK, int SHORT, bool COUNT, bool SORT>
 __global__ __launch_bounds__(BLOCK) void k_extend(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
                                                   PathSoA q, float4* __restrict__ hits, Counters* __restrict__ ctr,
                                                   int* __restrict__ overflow, unsigned long long* __restrict__ trav,
-                                                  int32_t* __restrict__ keys, int32_t* __restrict__ bhist) {
+                                                  int32_t* __restrict__ keys) {
     __shared__ int lds[SHORT * BLOCK];
-    __shared__ int hist[8];
     int i = blockIdx.x * BLOCK + threadIdx.x;
     int n = ctr->n_active;
-    if (SORT && threadIdx.x < 8) hist[threadIdx.x] = 0;
     VisitCount vc;
-    if (i < n) extend_one<BLOCK, SHORT>(scenes, jobs, q, hits, overflow, lds, i, vc);
+    if (i < n) {
+        float4 h = extend_one<BLOCK, SHORT>(scenes, jobs, q, hits, overflow, lds, i, vc);
+        if (SORT) keys[i] = sort_key(scenes, jobs, q.info[i].z, h);
+    }
     if (COUNT) {
         unsigned long long a = vc.nodes, b = vc.tris;
         for (int off = 32; off > 0; off >>= 1) { a += __shfl_down(a, off); b += __shfl_down(b, off); }
         if (lane_id() == 0) { atomicAdd(&trav[0], a); atomicAdd(&trav[1], b); }
     }
-    if (SORT) {
-        __syncthreads();
-        if (i < n) {
-            int k = sort_key(scenes, jobs, q.info[i].z, hits[i]);
-            keys[i] = k;
-            atomicAdd(&hist[k], 1);
-        }
-        __syncthreads();
-        if (threadIdx.x < 8) bhist[threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
-    }
 }
 
-// Exclusive scan of the key-major block histograms (8 * nblocks entries) by
-// one 1024-thread block: chunk sums, an LDS scan of the chunk sums, then the
-// chunk-local prefix.
+// Counting sort of the path queue by material key, over tiles of SORT_TILE
+// entries: per-tile key histograms (wave ballots, no atomics on HBM), one
+// exclusive scan of the key-major histogram array, then a scatter in which
+// every wave ranks its entries per key with ballots and takes its base with
+// one LDS atomic per key.
+constexpr int SORT_TILE = 4096;
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sort_hist(const int32_t* __restrict__ keys, int32_t* __restrict__ bhist,
+                                                     const Counters* __restrict__ ctr) {
+    __shared__ int hist[8];
+    const int n = ctr->n_active;
+    const int t0 = blockIdx.x * SORT_TILE;
+    if (threadIdx.x < 8) hist[threadIdx.x] = 0;
+    __syncthreads();
+    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = threadIdx.x; k < SORT_TILE; k += BLOCK) {
+        const int i = t0 + k;
+        const int key = i < n ? keys[i] : -1;
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) cnt[kk] += __popcll(__ballot(key == kk));
+    }
+    if (lane_id() == 0) {
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) if (cnt[kk]) atomicAdd(&hist[kk], cnt[kk]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) bhist[threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
+}
+
+// Exclusive scan of the key-major tile histograms (8 * tiles entries) by one
+// 1024-thread block: each thread sums a contiguous chunk, the chunk sums are
+// scanned with wave shuffles and one LDS pass over the 16 wave totals, then
+// each thread writes its chunk's prefix.
 __global__ __launch_bounds__(1024) void k_sort_scan(int32_t* __restrict__ bhist, int total) {
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
+    __shared__ int wsum[16];
+    const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
     const int chunk = (total + 1023) / 1024;
-    const int b0 = t * chunk, b1 = min(total, b0 + chunk);
+    const int b0 = min(total, t * chunk), b1 = min(total, b0 + chunk);
     int sum = 0;
     for (int k = b0; k < b1; k++) sum += bhist[k];
-    part[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    int incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off);
+        if (lane >= off) incl += v;
     }
-    int run = part[t] - sum;  // exclusive prefix of this chunk
-    for (int k = b0; k < b1; k++) { int c = bhist[k]; bhist[k] = run; run += c; }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    if (t < 16) {
+        int v = wsum[t], x = v;
+        for (int off = 1; off < 16; off <<= 1) {
+            const int u = __shfl_up(x, off, 16);
+            if (t >= off) x += u;
+        }
+        wsum[t] = x - v;  // exclusive prefix of the wave totals
+    }
+    __syncthreads();
+    int run = wsum[w] + incl - sum;
+    for (int k = b0; k < b1; k++) { const int c = bhist[k]; bhist[k] = run; run += c; }
 }
 
-// Scatter queue indices into material-sorted order: each block starts from
-// its scanned offsets and ranks its own elements with LDS atomics (order
-// inside one key does not matter).
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sort_scatter(const int32_t* __restrict__ keys, const int32_t* __restrict__ boff,
                                                         int32_t* __restrict__ perm, const Counters* __restrict__ ctr) {
     __shared__ int base[8];
-    int i = blockIdx.x * BLOCK + threadIdx.x;
-    int n = ctr->n_active;
+    const int n = ctr->n_active;
+    const int t0 = blockIdx.x * SORT_TILE;
+    const int lane = lane_id();
+    const uint64_t lt = (1ull << lane) - 1ull;
     if (threadIdx.x < 8) base[threadIdx.x] = boff[threadIdx.x * gridDim.x + blockIdx.x];
     __syncthreads();
-    if (i < n) perm[atomicAdd(&base[keys[i]], 1)] = i;
+    for (int k = threadIdx.x; k < SORT_TILE; k += BLOCK) {
+        const int i = t0 + k;
+        const int key = i < n ? keys[i] : -1;
+        int slot = 0;
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) {
+            const uint64_t m = __ballot(key == kk);
+            if (m) {
+                const int leader = __ffsll((unsigned long long)m) - 1;
+                int b = 0;
+                if (lane == leader) b = atomicAdd(&base[kk], __popcll(m));
+                b = __shfl(b, leader);
+                if (key == kk) slot = b + __popcll(m & lt);
+            }
+        }
+        if (key >= 0) perm[slot] = i;
+    }
 }
 
 DEV void put_shadow(ShadowSoA sh, int s, const ShadowReq& R, int64_t film_idx, int job) {
@@ -746,218 +794,6 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(const DevScene* __restrict__ s
     }
 }
 
-// ---------------------------------------------------------------------------
-// Persistent traversal with dynamic ray fetch.  A grid sized to the resident
-// capacity of the device runs waves that each keep 64 rays in flight: a lane
-// whose ray has finished takes the next queue entry from the wave's current
-// chunk (FETCH_CHUNK entries reserved with one atomic on one of NFETCH
-// counters), so lanes do not idle while the slowest ray of their wave
-// finishes.  One loop step processes one node or one leaf per lane.
-//   KIND 0: extension rays of the path queue, closest hit, written to hits[i].
-//   KIND 1: the shadow queue (any-hit entries at [0, n_any), closest-hit
-//           entries from the end), hits written to sh_hit[slot].
-// ---------------------------------------------------------------------------
-constexpr uint32_t POP_REF = 0xfffffffeu;  // "take the next node from the stack"
-
-DEV int fetch_chunk(int* counter, int lo, int hi) {
-    int seen = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (lo + seen >= hi) return -1;
-    int got = atomicAdd(counter, FETCH_CHUNK);
-    return lo + got < hi ? lo + got : -1;
-}
-
-template <int BLOCK, int SHORT, int KIND, bool COUNT>
-__global__ __launch_bounds__(BLOCK) void k_trace_p(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
-                                                   const float4* __restrict__ ro, const float4* __restrict__ rd,
-                                                   const int4* __restrict__ raux, float4* __restrict__ hits,
-                                                   const Counters* __restrict__ ctr, int* __restrict__ fetch,
-                                                   int* __restrict__ overflow, unsigned long long* __restrict__ trav,
-                                                   int capacity) {
-    __shared__ int lds[SHORT * BLOCK];
-    const int lane = lane_id();
-    const uint64_t lt = (1ull << lane) - 1ull;
-    int n, n_any = 0;
-    if (KIND == 0) {
-        n = ctr->n_active;
-    } else {
-        const unsigned long long n2 = ctr->n_shadow2;
-        n_any = (int)(n2 & 0xffffffffull);
-        n = n_any + (int)(n2 >> 32);
-    }
-    // waves beyond one per 64 entries have nothing to do
-    const int gw = (int)((blockIdx.x * BLOCK + threadIdx.x) >> 6);
-    const int nw = (int)min((int64_t)gridDim.x * (BLOCK / 64), ((int64_t)n + 63) / 64);
-    VisitCount vc;
-    if (gw < nw) {
-        const int nf = min(NFETCH, nw);
-        int c = gw % nf, tried = 0;
-        int cnext = 0, cend = 0;
-        bool exhausted = false;
-        TraversalStack<SHORT> st;
-        st.lds = lds + threadIdx.x; st.bdim = BLOCK;
-        st.glob = overflow + (size_t)blockIdx.x * BLOCK + threadIdx.x; st.gstride = gridDim.x * BLOCK;
-        st.sp = 0;
-        int slot = -1;              // buffer slot of the lane's ray (-1: idle)
-        PRay r;
-        float tbest = 0.0f;
-        int prim = -1, tri = -1;
-        bool any = false;
-        uint32_t cur = POP_REF;
-        const float4* nodes = nullptr;
-        const float4* tris = nullptr;
-        for (;;) {
-            const uint64_t idle = __ballot(slot < 0);
-            const int nidle = __popcll(idle);
-            if (nidle >= 16) {
-                if (cnext >= cend && !exhausted) {
-                    // wave-uniform: lane 0 reserves a chunk, moving on to the
-                    // next counter when one is drained
-                    for (;;) {
-                        const int lo = (int)((int64_t)c * n / nf), hi = (int)((int64_t)(c + 1) * n / nf);
-                        int got = 0;
-                        if (lane == 0) got = fetch_chunk(fetch + c * FETCH_STRIDE, lo, hi);
-                        got = __shfl(got, 0);
-                        if (got >= 0) { cnext = got; cend = min(hi, got + FETCH_CHUNK); break; }
-                        if (++tried >= nf) { exhausted = true; break; }
-                        c = c + 1 == nf ? 0 : c + 1;
-                    }
-                }
-                if (cnext < cend) {
-                    const int q = cnext + __popcll(idle & lt);
-                    if (slot < 0 && q < cend) {
-                        slot = KIND == 0 ? q : (q < n_any ? q : 2 * capacity - 1 - (q - n_any));
-                        const float4 o4 = ro[slot], d4 = rd[slot];
-                        const int4 ax = raux[slot];
-                        const int job = KIND == 0 ? ax.z : ax.y;
-                        const DevScene& sc = scenes[jobs[job].scene];
-                        const V o = f4v(o4), d = f4v(d4);
-                        r = prep_ray(o, d);
-                        tbest = KIND == 0 ? FLT_MAX : o4.w;
-                        any = KIND == 1 && __float_as_int(d4.w) == 0;
-                        prim = -1; tri = -1;
-                        st.sp = 0;
-                        nodes = sc.nodes; tris = sc.tris;
-                        cur = sc.has_bvh ? 0u : POP_REF;
-                        bool done = false;
-                        if (KIND == 0 && ((uint32_t)ax.w & 0x80000000u)) {
-                            prim = -2; done = true;  // dropped camera sample
-                        } else {
-                            for (int k = 0; k < sc.n_analytic; k++) {
-                                const int pi = sc.analytic[k];
-                                const float t = analytic_t(sc.prims[pi], o, d, T_MIN, tbest);
-                                if (t != 0.0f && t < tbest) {
-                                    tbest = t; prim = pi; tri = -1;
-                                    if (any) { done = true; break; }
-                                }
-                            }
-                            if (!sc.has_bvh) done = true;
-                        }
-                        if (done) {
-                            hits[slot] = make_float4(tbest, __int_as_float(prim), __int_as_float(tri), 0.0f);
-                            slot = -1;
-                        }
-                    }
-                    cnext = min(cend, cnext + nidle);
-                }
-            }
-            if (__ballot(slot >= 0) == 0) {
-                if (exhausted && cnext >= cend) break;
-                continue;
-            }
-            if (slot < 0) continue;
-            bool done = false;
-            if (!(cur & LEAF_FLAG_DEV)) {
-                const float4* np = nodes + 4 * (size_t)cur;
-                const float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
-                if (COUNT) vc.nodes++;
-                float t0, t1;
-                bool h0, h1;
-                slab2(n0, n1, n2, r, T_MIN, tbest, t0, t1, h0, h1);
-                const uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
-                if (c1 == 0xffffffffu) h1 = false;
-                if (h0 && h1) {
-                    const bool swap = t1 < t0;
-                    st.push((int)(swap ? c0 : c1));
-                    cur = swap ? c1 : c0;
-                } else {
-                    cur = h0 ? c0 : (h1 ? c1 : POP_REF);
-                }
-            } else if (cur != POP_REF) {
-                const uint32_t first = (cur & 0x7fffffffu) >> 3, cnt = (cur & 7u) + 1;
-                for (uint32_t k = 0; k < cnt; k++) {
-                    if (COUNT) vc.tris++;
-                    const float4* tp = tris + 3 * (size_t)(first + k);
-                    const float4 a = tp[0], b = tp[1], cc = tp[2];
-                    const float t = tri_t(f4v(a), f4v(b), f4v(cc), r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, T_MIN, tbest);
-                    if (t != 0.0f && t < tbest) {
-                        tbest = t; tri = __float_as_int(a.w); prim = __float_as_int(b.w);
-                        if (any) { done = true; break; }
-                    }
-                }
-                cur = POP_REF;
-            }
-            if (!done && cur == POP_REF) {
-                if (st.sp == 0) done = true;
-                else cur = (uint32_t)st.pop();
-            }
-            if (done) {
-                hits[slot] = make_float4(tbest, __int_as_float(prim), __int_as_float(tri), 0.0f);
-                slot = -1;
-            }
-        }
-    }
-    if (COUNT) {
-        unsigned long long a = vc.nodes, b = vc.tris;
-        for (int off = 32; off > 0; off >>= 1) { a += __shfl_down(a, off); b += __shfl_down(b, off); }
-        if (lane == 0) { atomicAdd(&trav[KIND == 0 ? 0 : 2], a); atomicAdd(&trav[KIND == 0 ? 1 : 3], b); }
-    }
-}
-
-// Material-sort keys and per-block key histograms of the extension hits (the
-// epilogue of the thread-per-ray extend kernel, as its own pass).
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_sort_keys(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
-                                                     PathSoA q, const float4* __restrict__ hits,
-                                                     const Counters* __restrict__ ctr, int32_t* __restrict__ keys,
-                                                     int32_t* __restrict__ bhist) {
-    __shared__ int hist[8];
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    const int n = ctr->n_active;
-    if (threadIdx.x < 8) hist[threadIdx.x] = 0;
-    __syncthreads();
-    if (i < n) {
-        const int k = sort_key(scenes, jobs, q.info[i].z, hits[i]);
-        keys[i] = k;
-        atomicAdd(&hist[k], 1);
-    }
-    __syncthreads();
-    if (threadIdx.x < 8) bhist[threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
-}
-
-// Contributions of the traced shadow queue (hits from the persistent
-// traversal), one thread per queue entry in the same order.
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_shadow_resolve(const DevScene* __restrict__ scenes,
-                                                          const DevJob* __restrict__ jobs, ShadowSoA sh,
-                                                          const float4* __restrict__ sh_hit,
-                                                          const Counters* __restrict__ ctr, float4* __restrict__ film,
-                                                          int capacity) {
-    const int q = blockIdx.x * BLOCK + threadIdx.x;
-    const unsigned long long n2 = ctr->n_shadow2;
-    const int n_any = (int)(n2 & 0xffffffffull), n = n_any + (int)(n2 >> 32);
-    if (q >= n) return;
-    const int slot = q < n_any ? q : 2 * capacity - 1 - (q - n_any);
-    const float4 o = sh.o[slot], d = sh.d[slot], c = sh.c[slot], hv = sh_hit[slot];
-    const int4 aux = sh.aux[slot];
-    const DevScene& sc = scenes[jobs[aux.y].scene];
-    ShadowReq r;
-    r.o = f4v(o); r.tmax = o.w; r.d = f4v(d); r.mode = __float_as_int(d.w); r.light = aux.x; r.c = f4v(c);
-    HitRec h;
-    h.t = hv.x; h.prim = __float_as_int(hv.y); h.tri = __float_as_int(hv.z);
-    const V v = resolve_hit(sc, r, h.prim >= 0, h);
-    if (!black(v)) film_add(film, ((int64_t)aux.z << 32) | __float_as_uint(c.w), v);
-}
-
 __global__ void k_count_rays(Counters* ctr) {
     const unsigned long long n2 = ctr->n_shadow2;
     ctr->rays_closest += ctr->n_active + (int64_t)(n2 >> 32);
@@ -971,26 +807,6 @@ void wavefront_init(WavefrontBuffers& w, const DevJob* jobs, int n_jobs, int64_t
     hipMemcpyAsync(w.ctr, &c, sizeof(Counters), hipMemcpyHostToDevice, s);
 }
 
-// Blocks of `kernel` the whole device holds at once (occupancy per CU times
-// the CU count), cached per kernel and device.
-template <typename K>
-static int resident_blocks(K kernel, int block) {
-    static std::mutex m;
-    static std::map<std::pair<const void*, int>, int> cache;
-    int dev = 0;
-    hipGetDevice(&dev);
-    const auto key = std::make_pair((const void*)kernel, dev);
-    std::lock_guard<std::mutex> lock(m);
-    auto it = cache.find(key);
-    if (it != cache.end()) return it->second;
-    int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1) per_cu = 1;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
-    const int n = per_cu * cus;
-    cache[key] = n;
-    return n;
-}
-
 template <int B>
 static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, float4* film, WavefrontBuffers& w,
                         const WavefrontParams& p, const IterProfile* prof, hipStream_t s) {
@@ -999,39 +815,29 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     const int grid256 = (cap + 255) / 256;
     // top up the queue with new camera paths, then commit the counters
     hipLaunchKernelGGL(k_generate, dim3(grid256), dim3(256), 0, s, jobs, n_jobs, w.a, w.ctr, film, cap);
-    hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, s, w.ctr, cap, w.fetch);
+    hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, s, w.ctr, cap);
     const bool timing = prof && prof->timing;
     const int use_perm = p.sort_keys > 0;
     if (timing) hipEventRecord(prof->ev[0], s);
     unsigned long long* trav = prof ? prof->trav : nullptr;
-    const bool persistent = p.traversal == 1;
-    if (persistent) {
-        if (trav)
-            hipLaunchKernelGGL((k_trace_p<B, SHORT_STACK, 0, true>),
-                               dim3(std::min(grid, resident_blocks(k_trace_p<B, SHORT_STACK, 0, true>, B))), dim3(B), 0, s,
-                               scenes, jobs, w.a.o, w.a.d, (const int4*)w.a.info, w.hit, w.ctr, w.fetch, w.overflow, trav, cap);
-        else
-            hipLaunchKernelGGL((k_trace_p<B, SHORT_STACK, 0, false>),
-                               dim3(std::min(grid, resident_blocks(k_trace_p<B, SHORT_STACK, 0, false>, B))), dim3(B), 0, s,
-                               scenes, jobs, w.a.o, w.a.d, (const int4*)w.a.info, w.hit, w.ctr, w.fetch, w.overflow, trav, cap);
-    } else if (trav && use_perm)
+    if (trav && use_perm)
         hipLaunchKernelGGL((k_extend<B, SHORT_STACK, true, true>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr,
-                           w.overflow, trav, w.keys, w.bhist);
+                           w.overflow, trav, w.keys);
     else if (trav)
         hipLaunchKernelGGL((k_extend<B, SHORT_STACK, true, false>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr,
-                           w.overflow, trav, w.keys, w.bhist);
+                           w.overflow, trav, w.keys);
     else if (use_perm)
         hipLaunchKernelGGL((k_extend<B, SHORT_STACK, false, true>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr,
-                           w.overflow, trav, w.keys, w.bhist);
+                           w.overflow, trav, w.keys);
     else
         hipLaunchKernelGGL((k_extend<B, SHORT_STACK, false, false>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr,
-                           w.overflow, trav, w.keys, w.bhist);
+                           w.overflow, trav, w.keys);
     if (timing) hipEventRecord(prof->ev[1], s);
-    if (use_perm && persistent)
-        hipLaunchKernelGGL((k_sort_keys<B>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr, w.keys, w.bhist);
     if (use_perm) {
-        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, w.bhist, 8 * grid);
-        hipLaunchKernelGGL((k_sort_scatter<B>), dim3(grid), dim3(B), 0, s, w.keys, w.bhist, w.perm, w.ctr);
+        const int tiles = (cap + SORT_TILE - 1) / SORT_TILE;
+        hipLaunchKernelGGL((k_sort_hist<256>), dim3(tiles), dim3(256), 0, s, w.keys, w.bhist, w.ctr);
+        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, w.bhist, 8 * tiles);
+        hipLaunchKernelGGL((k_sort_scatter<256>), dim3(tiles), dim3(256), 0, s, w.keys, w.bhist, w.perm, w.ctr);
     }
     if (timing) hipEventRecord(prof->ev[2], s);
     hipLaunchKernelGGL((k_shade<B>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.perm, w.b, w.sh, w.ctr, film,
@@ -1040,19 +846,7 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(1), 0, s, w.ctr);
     if (timing) hipEventRecord(prof->ev[4], s);
     const int sgrid = (int)((2 * (int64_t)cap + 64 + B - 1) / B);  // within wavefront_stack_threads(cap)
-    if (persistent) {
-        int* fetch_sh = w.fetch + NFETCH * FETCH_STRIDE;
-        if (trav)
-            hipLaunchKernelGGL((k_trace_p<B, SHORT_STACK, 1, true>),
-                               dim3(std::min(sgrid, resident_blocks(k_trace_p<B, SHORT_STACK, 1, true>, B))), dim3(B), 0, s,
-                               scenes, jobs, w.sh.o, w.sh.d, w.sh.aux, w.sh_hit, w.ctr, fetch_sh, w.overflow, trav, cap);
-        else
-            hipLaunchKernelGGL((k_trace_p<B, SHORT_STACK, 1, false>),
-                               dim3(std::min(sgrid, resident_blocks(k_trace_p<B, SHORT_STACK, 1, false>, B))), dim3(B), 0, s,
-                               scenes, jobs, w.sh.o, w.sh.d, w.sh.aux, w.sh_hit, w.ctr, fetch_sh, w.overflow, trav, cap);
-        hipLaunchKernelGGL((k_shadow_resolve<B>), dim3((2 * cap + B - 1) / B), dim3(B), 0, s, scenes, jobs, w.sh, w.sh_hit,
-                           w.ctr, film, cap);
-    } else if (trav)
+    if (trav)
         hipLaunchKernelGGL((k_shadow<B, SHORT_STACK, true>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr, film,
                            w.overflow, trav, cap);
     else

@@ -176,16 +176,16 @@ class GPURenderer:
     def setCountTraversal(self, on):
         self._seti(abi.KEY_COUNT_TRAVERSAL, int(bool(on)))
 
-    def setWavefront(self, block=None, pool_paths=None, sort_keys=None, traversal=None):
+    def setWavefront(self, block=None, pool_paths=None, sort_keys=None):
         """Wavefront engine configuration (the values dfr_autotune picks)."""
         for key, v in ((abi.KEY_WF_BLOCK, block), (abi.KEY_WF_POOL_PATHS, pool_paths),
-                       (abi.KEY_WF_SORT_KEYS, sort_keys), (abi.KEY_WF_TRAVERSAL, traversal)):
+                       (abi.KEY_WF_SORT_KEYS, sort_keys)):
             if v is not None:
                 self._seti(key, v)
 
     def getWavefront(self):
         return {"block": self._geti(abi.KEY_WF_BLOCK), "pool_paths": self._geti(abi.KEY_WF_POOL_PATHS),
-                "sort_keys": self._geti(abi.KEY_WF_SORT_KEYS), "traversal": self._geti(abi.KEY_WF_TRAVERSAL)}
+                "sort_keys": self._geti(abi.KEY_WF_SORT_KEYS)}
 
     # -- rendering -----------------------------------------------------------
     def setup(self, compile_scene=True):

@@ -199,7 +199,6 @@ typedef struct dfr_ctx dfr_ctx;
 #define DFR_KEY_WF_BLOCK               14  /* threads per block: 128 or 256     */
 #define DFR_KEY_WF_POOL_PATHS          15  /* path pool capacity, >= 1024       */
 #define DFR_KEY_WF_SORT_KEYS           16  /* 0: no material sort, 8: sort      */
-#define DFR_KEY_WF_TRAVERSAL           17  /* 0: thread per ray, 1: persistent waves with dynamic ray fetch */
 #define DFR_KEY_MAXIMUM_DISTANCE      101  /* float, default 20                */
 
 /* new GPURenderer(observer): M/renderer/gpu/GPURenderer.java:55-57 with the

@@ -99,10 +99,10 @@ def test_trace_rays_closest_and_any_hit():
     assert ((pa >= 0) == (p >= 0)).mean() >= 0.9999
 
 
-@pytest.mark.parametrize("kernel,traversal", [(0, 1), (0, 0), (1, 0)])
-def test_cornell_path_tracing_same_streams(kernel, traversal):
-    """Wavefront (0; persistent or thread-per-ray traversal) and plain (1)
-    kernels against the oracle on the same random streams.
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_cornell_path_tracing_same_streams(kernel):
+    """Wavefront (0) and plain (1) kernels against the oracle on the same
+    random streams.
 
     Same streams give the same paths up to float rounding: a path leaves the
     oracle's only where rounding flips a decision (a Russian-roulette or
@@ -119,7 +119,6 @@ def test_cornell_path_tracing_same_streams(kernel, traversal):
     r.setScene(sc)
     r.setSeed(42)
     r.setKernel(kernel)
-    r.setWavefront(traversal=traversal)
     r.setup()
     r.render(1)
     g1, cnt1 = r.film(res, res)
