@@ -405,7 +405,8 @@ int Ctx::ensure_film(int64_t pixels) {
 }
 
 int Ctx::ensure_wavefront(int capacity) {
-    if (wf_capacity == capacity && wf.ctr) return DFR_OK;
+    // buffers only grow; a run may use fewer slots than are allocated
+    if (wf_capacity >= capacity && wf.ctr) { wf.capacity = capacity; return DFR_OK; }
     const size_t c = (size_t)capacity;
     bool ok = true;
     auto A = [&](DeviceBuffer& b, size_t bytes) { ok = ok && b.alloc(bytes) == hipSuccess; return b.ptr; };
@@ -434,13 +435,18 @@ size_t wavefront_bytes(int capacity) {
 // Runs `jobs` (already on the device in job_buf) to completion through the
 // wavefront engine.  Counters are polled every `poll` iterations.
 int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counters* out_ctr) {
-    int rc = ensure_wavefront(wf_params.capacity);
+    // a run never needs more slots than it has work items: smaller runs get a
+    // smaller pool (and smaller launch grids)
+    int64_t want = 65536;
+    while (want < work_total && want < wf_params.capacity) want <<= 1;
+    const int capacity = (int)std::min<int64_t>(want, wf_params.capacity);
+    int rc = ensure_wavefront(capacity);
     if (rc) return rc;
-    int64_t threads = wavefront_stack_threads(wf_params.capacity);
+    int64_t threads = wavefront_stack_threads(capacity);
     if ((rc = ensure_overflow(threads))) return rc;
     wf.overflow = (int*)overflow.ptr;
     wavefront_init(wf, (const DevJob*)job_buf.ptr, n_jobs, work_total, stream);
-    WavefrontParams p{wf_params.block, wf_params.sort_keys, (wf_params.capacity + 255) / 256};
+    WavefrontParams p{wf_params.block, wf_params.sort_keys, (capacity + 255) / 256};
     Counters host{};
     const int poll = 4;
     const int max_iterations = 1 << 20;  // far above max_bounce * (work / capacity + 1)
