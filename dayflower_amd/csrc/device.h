@@ -16,6 +16,19 @@ namespace dfr {
 
 #define DEV __device__ __forceinline__
 
+// Address-space qualified pointers.  Scene buffers are reached through
+// pointers stored in DevScene records, which the compiler cannot place in an
+// address space by itself: without these it emits flat accesses (which wait
+// on both the vector-memory and the LDS counters).
+#define AS_GLOBAL __attribute__((address_space(1)))
+#define AS_LDS __attribute__((address_space(3)))
+typedef float native_float4 __attribute__((ext_vector_type(4)));
+// p[i] of a float4 array in HBM, loaded as a global (not flat) access
+DEV float4 ld4g(const float4* p, size_t i) {
+    const native_float4 v = ((const AS_GLOBAL native_float4*)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // ---------------------------------------------------------------------------
 // small vector type
 // ---------------------------------------------------------------------------
@@ -106,18 +119,29 @@ constexpr uint32_t LEAF_FLAG_DEV = 0x80000000u;  // leaf reference bit (see bvh.
 // (entry k at glob[(k - SHORT) * gstride]).
 template <int SHORT>
 struct TraversalStack {
-    int* lds;
-    int* glob;
+    AS_LDS int* lds;
+    AS_GLOBAL int* glob;
     int gstride;
     int bdim;
     int sp;
+    // `lds_block`: the block's SHORT * bdim ints of LDS; `overflow`: the HBM
+    // overflow buffer, one column per thread of the grid
+    DEV void init(int* lds_block, int* overflow, int block_dim) {
+        lds = (AS_LDS int*)(lds_block + threadIdx.x);
+        bdim = block_dim;
+        glob = (AS_GLOBAL int*)(overflow + (size_t)blockIdx.x * block_dim + threadIdx.x);
+        gstride = gridDim.x * block_dim;
+        sp = 0;
+    }
     DEV void push(int v) {
-        if (sp < SHORT) lds[sp * bdim] = v; else glob[(sp - SHORT) * gstride] = v;
+        if (sp < SHORT) lds[sp * bdim] = v;
+        else glob[(sp - SHORT) * gstride] = v;
         sp++;
     }
     DEV int pop() {
         sp--;
-        return sp < SHORT ? lds[sp * bdim] : glob[(sp - SHORT) * gstride];
+        if (sp < SHORT) return lds[sp * bdim];
+        return glob[(sp - SHORT) * gstride];
     }
 };
 
@@ -367,8 +391,8 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
     uint32_t cur = 0;  // root node
     for (;;) {
         if (!(cur & LEAF_FLAG_DEV)) {
-            const float4* np = s.nodes + 4 * (size_t)cur;
-            float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+            const size_t nb = 4 * (size_t)cur;
+            float4 n0 = ld4g(s.nodes, nb), n1 = ld4g(s.nodes, nb + 1), n2 = ld4g(s.nodes, nb + 2), n3 = ld4g(s.nodes, nb + 3);
             vc.nodes++;
             float t0, t1; bool h0, h1;
             slab2(n0, n1, n2, r, tmin, hit.t, t0, t1, h0, h1);
@@ -386,8 +410,8 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
             uint32_t first = (cur & 0x7fffffffu) >> 3, cnt = (cur & 7u) + 1;
             for (uint32_t k = 0; k < cnt; k++) {
                 vc.tris++;
-                const float4* tp = s.tris + 3 * (size_t)(first + k);
-                float4 a = tp[0], b = tp[1], c = tp[2];
+                const size_t tb = 3 * (size_t)(first + k);
+                float4 a = ld4g(s.tris, tb), b = ld4g(s.tris, tb + 1), c = ld4g(s.tris, tb + 2);
                 float t = tri_t(f4v(a), f4v(b), f4v(c), r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, tmin, hit.t);
                 if (t != 0.0f && t < hit.t) {
                     hit.t = t; hit.tri = __float_as_int(a.w); hit.prim = __float_as_int(b.w); found = true;

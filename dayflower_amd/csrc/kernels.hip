@@ -301,8 +301,7 @@ __global__ __launch_bounds__(BLOCK) void k_megakernel(const DevScene* __restrict
     int64_t cnt[2] = {0, 0};
     if (pix < n_pixels) {
         TraversalStack<SHORT> st;
-        st.lds = lds + threadIdx.x; st.bdim = BLOCK;
-        st.glob = overflow + (size_t)blockIdx.x * BLOCK + threadIdx.x; st.gstride = gridDim.x * BLOCK;
+        st.init(lds, overflow, BLOCK);
         int x = pix % job.width, y = pix / job.width;
         V sum = mk(0, 0, 0);
         int count = 0;
@@ -364,8 +363,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_rays(const DevScene* __restrict
                                                       int* __restrict__ overflow) {
     __shared__ int lds[SHORT * BLOCK];
     TraversalStack<SHORT> st;
-    st.lds = lds + threadIdx.x; st.bdim = BLOCK;
-    st.glob = overflow + (size_t)blockIdx.x * BLOCK + threadIdx.x; st.gstride = gridDim.x * BLOCK;
+    st.init(lds, overflow, BLOCK);
     const DevScene& sc = *scene;
     for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
         float4 a = rays[2 * i], b = rays[2 * i + 1];
@@ -395,8 +393,7 @@ __global__ __launch_bounds__(BLOCK) void k_cast_primary(const DevScene* __restri
     int pix = blockIdx.x * BLOCK + threadIdx.x;
     if (pix >= n_pixels) return;
     TraversalStack<SHORT> st;
-    st.lds = lds + threadIdx.x; st.bdim = BLOCK;
-    st.glob = overflow + (size_t)blockIdx.x * BLOCK + threadIdx.x; st.gstride = gridDim.x * BLOCK;
+    st.init(lds, overflow, BLOCK);
     int x = pix % job.width, y = pix / job.width;
     Rng rng = make_rng(job.seed, (uint32_t)pix, (uint32_t)job.sample_base);
     V o, d;
@@ -504,8 +501,7 @@ template <int BLOCK, int SHORT>
 DEV float4 extend_one(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, PathSoA q,
                       float4* __restrict__ hits, int* __restrict__ overflow, int* lds, int i, VisitCount& vc) {
     TraversalStack<SHORT> st;
-    st.lds = lds + threadIdx.x; st.bdim = BLOCK;
-    st.glob = overflow + (size_t)blockIdx.x * BLOCK + threadIdx.x; st.gstride = gridDim.x * BLOCK;
+    st.init(lds, overflow, BLOCK);
     uint4 info = q.info[i];
     float4 out;
     if (info.w & 0x80000000u) {
@@ -759,8 +755,7 @@ template <int BLOCK, int SHORT>
 DEV void shadow_one(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, ShadowSoA sh,
                     float4* __restrict__ film, int* __restrict__ overflow, int* lds, int i, VisitCount& vc) {
     TraversalStack<SHORT> st;
-    st.lds = lds + threadIdx.x; st.bdim = BLOCK;
-    st.glob = overflow + (size_t)blockIdx.x * BLOCK + threadIdx.x; st.gstride = gridDim.x * BLOCK;
+    st.init(lds, overflow, BLOCK);
     float4 o = sh.o[i], d = sh.d[i], c = sh.c[i];
     int4 aux = sh.aux[i];
     const DevScene& sc = scenes[jobs[aux.y].scene];
