@@ -21,23 +21,25 @@ namespace dfr {
 DEV int lane_id() { return __lane_id(); }
 
 // Block-wide queue reservation.  Each thread asks for `a` continuing-path
-// slots (0/1), `b` any-hit shadow slots (0/1) and `c` closest-hit shadow
+// slots (0/1), `b` any-hit shadow slots (0..2) and `c` closest-hit shadow
 // slots (0..2); the block makes one atomic on n_next and one 64-bit atomic on
 // n_shadow2 (both shadow partitions at once).  Returns the thread's first slot
 // of each kind.  Every thread of the block must call it.
 template <int BLOCK>
-DEV void block_reserve(Counters* ctr, bool a, bool b, int c, int& sa, int& sb, int& sc) {
+DEV void block_reserve(Counters* ctr, bool a, int b, int c, int& sa, int& sb, int& sc) {
     constexpr int NW = BLOCK / 64;
     __shared__ int wsum[3][NW];
     __shared__ int wbase[3][NW];
     const int lane = lane_id(), w = threadIdx.x >> 6;
     const uint64_t lt = (1ull << lane) - 1ull;
-    const uint64_t ma = __ballot(a), mb = __ballot(b), mc0 = __ballot(c & 1), mc1 = __ballot(c >> 1);
-    const int pa = __popcll(ma & lt), pb = __popcll(mb & lt);
+    const uint64_t ma = __ballot(a), mb0 = __ballot(b & 1), mb1 = __ballot(b >> 1);
+    const uint64_t mc0 = __ballot(c & 1), mc1 = __ballot(c >> 1);
+    const int pa = __popcll(ma & lt);
+    const int pb = __popcll(mb0 & lt) + 2 * __popcll(mb1 & lt);
     const int pc = __popcll(mc0 & lt) + 2 * __popcll(mc1 & lt);
     if (lane == 0) {
         wsum[0][w] = __popcll(ma);
-        wsum[1][w] = __popcll(mb);
+        wsum[1][w] = __popcll(mb0) + 2 * __popcll(mb1);
         wsum[2][w] = __popcll(mc0) + 2 * __popcll(mc1);
     }
     __syncthreads();
@@ -86,8 +88,10 @@ DEV void film_add(float4* film, int64_t idx, V c) {
 // M/scene/Scene.java:1530-1633, isSpecular = false).
 //   mode 0: contribute c if nothing is hit in (T_MIN, tmax)          (delta / sky lights)
 //   mode 1: contribute c if the closest hit in (T_MIN, tmax) is the light    (area lights)
-//   mode 2: BSDF-sampled ray: closest hit in (T_MIN, inf); the light's
-//           emitted radiance there (or the sky on a miss) times c
+//   mode 2: BSDF-sampled ray toward an area light: closest hit in
+//           (T_MIN, inf); the light's emitted radiance there times c.  (A
+//           BSDF-sampled ray toward an infinite light is a mode-0 request
+//           whose c already holds the light's radiance along the ray.)
 // ---------------------------------------------------------------------------
 struct ShadowReq {
     V o, d;
@@ -146,7 +150,15 @@ DEV void nee_requests(const DevScene& sc, const Bsdf& b, const Surf& si, int li,
             if (ok) {
                 ShadowReq& r = bs_req;
                 r.o = si.p; r.d = nrm(bs.wi); r.tmax = FLT_MAX;
-                r.mode = 2; r.light = li; r.c = scale * f * w / bs.pdf;
+                r.light = li; r.c = scale * f * w / bs.pdf;
+                if (L.type == DFR_LIGHT_DIFFUSE_AREA) {
+                    r.mode = 2;
+                } else {
+                    // an infinite light is seen exactly when nothing is hit:
+                    // its radiance folds into c and the query is any-hit
+                    r.c = r.c * emitted_ray(sc, L, r.d);
+                    r.mode = black(r.c) ? -1 : 0;
+                }
             }
         }
     }
@@ -212,7 +224,7 @@ DEV V radiance_pt(const DevScene& sc, V o, V d, const Rng& rng, int max_bounce, 
                     L = L + resolve_shadow<SHORT>(sc, r0, st, vc);
                 }
                 if (r1.mode >= 0) {
-                    n_closest++;
+                    if (r1.mode == 0) n_any++; else n_closest++;
                     L = L + resolve_shadow<SHORT>(sc, r1, st, vc);
                 }
             }
@@ -734,10 +746,11 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const DevScene* __restrict__ sc
     if (job_done) wave_count_jobs(job_done, active && !cont, active ? q.info[i].z : 0u);
     // queue appends: one reservation per block
     const bool has0 = r0.mode >= 0, has1 = r1.mode >= 0;
-    const bool any0 = has0 && r0.mode == 0;
-    const int ncl = (has0 && !any0 ? 1 : 0) + (has1 ? 1 : 0);
+    const bool any0 = has0 && r0.mode == 0, any1 = has1 && r1.mode == 0;
+    const int nany = (any0 ? 1 : 0) + (any1 ? 1 : 0);
+    const int ncl = (has0 && !any0 ? 1 : 0) + (has1 && !any1 ? 1 : 0);
     int s_path, s_any, s_cl;
-    block_reserve<BLOCK>(ctr, cont, any0, ncl, s_path, s_any, s_cl);
+    block_reserve<BLOCK>(ctr, cont, nany, ncl, s_path, s_any, s_cl);
     if (cont) {
         out.o[s_path] = make_float4(no.x, no.y, no.z, 0.0f);
         out.d[s_path] = make_float4(nd.x, nd.y, nd.z, neta);
@@ -747,7 +760,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const DevScene* __restrict__ sc
     const int last = 2 * capacity - 1;  // closest-hit requests fill the shadow queue from here down
     const int job = active ? (int)q.info[i].z : 0;
     if (has0) put_shadow(sh, any0 ? s_any : last - s_cl, r0, film_idx, job);
-    if (has1) put_shadow(sh, last - (s_cl + (has0 && !any0 ? 1 : 0)), r1, film_idx, job);
+    if (has1) put_shadow(sh, any1 ? s_any + (any0 ? 1 : 0) : last - (s_cl + (has0 && !any0 ? 1 : 0)), r1, film_idx, job);
 }
 
 // Resolves one queued shadow request and adds its contribution to the film.
