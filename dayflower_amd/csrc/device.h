@@ -388,9 +388,15 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
     if (!s.has_bvh) return found;
     PRay r = prep_ray(o, d);
     st.sp = 0;
-    uint32_t cur = 0;  // root node
-    for (;;) {
-        if (!(cur & LEAF_FLAG_DEV)) {
+    // Speculative while-while traversal: a lane that reaches a leaf parks it
+    // and keeps descending inner nodes until every lane of the wave has a
+    // parked leaf (or is done); then the wave tests leaves together.  This
+    // keeps lanes busy while others are still in inner nodes.
+    constexpr uint32_t DONE = 0x7fffffffu;  // no node left (bit 31 clear, never a node index)
+    uint32_t cur = 0;                       // root node
+    uint32_t leaf = 0;                      // parked leaf reference (0: none)
+    while (cur != DONE) {
+        while (cur < DONE) {  // inner node
             const size_t nb = 4 * (size_t)cur;
             float4 n0 = ld4g(s.nodes, nb), n1 = ld4g(s.nodes, nb + 1), n2 = ld4g(s.nodes, nb + 2), n3 = ld4g(s.nodes, nb + 3);
             vc.nodes++;
@@ -400,14 +406,21 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
             if (c1 == 0xffffffffu) h1 = false;
             if (h0 && h1) {
                 bool swap = t1 < t0;
-                uint32_t near = swap ? c1 : c0, far = swap ? c0 : c1;
-                st.push((int)far);
-                cur = near;
-                continue;
-            } else if (h0) { cur = c0; continue; }
-            else if (h1) { cur = c1; continue; }
-        } else {
-            uint32_t first = (cur & 0x7fffffffu) >> 3, cnt = (cur & 7u) + 1;
+                st.push((int)(swap ? c0 : c1));
+                cur = swap ? c1 : c0;
+            } else if (h0 || h1) {
+                cur = h0 ? c0 : c1;
+            } else {
+                cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
+            }
+            if ((cur & LEAF_FLAG_DEV) && leaf == 0) {
+                leaf = cur;
+                cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
+            }
+            if (__ballot(leaf == 0) == 0) break;
+        }
+        while (leaf != 0) {
+            uint32_t first = (leaf & 0x7fffffffu) >> 3, cnt = (leaf & 7u) + 1;
             for (uint32_t k = 0; k < cnt; k++) {
                 vc.tris++;
                 const size_t tb = 3 * (size_t)(first + k);
@@ -418,9 +431,12 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
                     if (ANY) return true;
                 }
             }
+            leaf = 0;
+            if (cur & LEAF_FLAG_DEV) {
+                leaf = cur;
+                cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
+            }
         }
-        if (st.sp == 0) break;
-        cur = (uint32_t)st.pop();
     }
     if (REFINE && !ANY && found && hit.tri < 0) hit.t = analytic_t64(s.prims[hit.prim], o, d, hit.t);
     if (REFINE && !ANY && found && hit.tri >= 0) {
