@@ -763,38 +763,63 @@ __global__ __launch_bounds__(BLOCK) void k_shade(const DevScene* __restrict__ sc
     if (has1) put_shadow(sh, any1 ? s_any + (any0 ? 1 : 0) : last - (s_cl + (has0 && !any0 ? 1 : 0)), r1, film_idx, job);
 }
 
-// Resolves one queued shadow request and adds its contribution to the film.
-template <int BLOCK, int SHORT>
-DEV void shadow_one(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, ShadowSoA sh,
-                    float4* __restrict__ film, int* __restrict__ overflow, int* lds, int i, VisitCount& vc) {
-    TraversalStack<SHORT> st;
-    st.init(lds, overflow, BLOCK);
-    float4 o = sh.o[i], d = sh.d[i], c = sh.c[i];
-    int4 aux = sh.aux[i];
-    const DevScene& sc = scenes[jobs[aux.y].scene];
-    ShadowReq r;
-    r.o = f4v(o); r.tmax = o.w; r.d = f4v(d); r.mode = __float_as_int(d.w); r.light = aux.x; r.c = f4v(c);
-    V v = resolve_shadow<SHORT>(sc, r, st, vc);
-    if (!black(v)) film_add(film, ((int64_t)aux.z << 32) | __float_as_uint(c.w), v);
-}
-
+// Shadow queue, any-hit partition (entries [0, n_any)): a delta-light or
+// infinite-light visibility ray contributes c when nothing is hit.
 template <int BLOCK, int SHORT, bool COUNT>
-__global__ __launch_bounds__(BLOCK) void k_shadow(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
-                                                  ShadowSoA sh, Counters* __restrict__ ctr, float4* __restrict__ film,
-                                                  int* __restrict__ overflow, unsigned long long* __restrict__ trav,
-                                                  int capacity) {
+__global__ __launch_bounds__(BLOCK) void k_shadow_any(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
+                                                      ShadowSoA sh, const Counters* __restrict__ ctr,
+                                                      float4* __restrict__ film, int* __restrict__ overflow,
+                                                      unsigned long long* __restrict__ trav) {
     __shared__ int lds[SHORT * BLOCK];
     const int i = blockIdx.x * BLOCK + threadIdx.x;
-    const unsigned long long n2 = ctr->n_shadow2;
-    const int n_any = (int)(n2 & 0xffffffffull), n_cl = (int)(n2 >> 32);
-    // any-hit requests at [0, n_any); closest-hit ones from the queue's end,
-    // started on a fresh wave so the two kinds do not share waves
-    const int cl0 = (n_any + 63) & ~63;
-    int slot = -1;
-    if (i < n_any) slot = i;
-    else if (i >= cl0 && i - cl0 < n_cl) slot = 2 * capacity - 1 - (i - cl0);
+    const int n_any = (int)(ctr->n_shadow2 & 0xffffffffull);
     VisitCount vc;
-    if (slot >= 0) shadow_one<BLOCK, SHORT>(scenes, jobs, sh, film, overflow, lds, slot, vc);
+    if (i < n_any) {
+        TraversalStack<SHORT> st;
+        st.init(lds, overflow, BLOCK);
+        const float4 o = sh.o[i], d = sh.d[i];
+        const int4 aux = sh.aux[i];
+        const DevScene& sc = scenes[jobs[aux.y].scene];
+        HitRec h;
+        if (!trace<true, SHORT>(sc, f4v(o), f4v(d), T_MIN, o.w, h, st, vc)) {
+            const float4 c = sh.c[i];
+            film_add(film, ((int64_t)aux.z << 32) | __float_as_uint(c.w), f4v(c));
+        }
+    }
+    if (COUNT) {
+        unsigned long long a = vc.nodes, b = vc.tris;
+        for (int off = 32; off > 0; off >>= 1) { a += __shfl_down(a, off); b += __shfl_down(b, off); }
+        if (lane_id() == 0) { atomicAdd(&trav[2], a); atomicAdd(&trav[3], b); }
+    }
+}
+
+// Shadow queue, closest-hit partition (n_cl entries from the queue's end):
+// area-light visibility (mode 1) and BSDF-sampled rays toward area lights
+// (mode 2).
+template <int BLOCK, int SHORT, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_shadow_closest(const DevScene* __restrict__ scenes,
+                                                          const DevJob* __restrict__ jobs, ShadowSoA sh,
+                                                          const Counters* __restrict__ ctr, float4* __restrict__ film,
+                                                          int* __restrict__ overflow,
+                                                          unsigned long long* __restrict__ trav, int capacity) {
+    __shared__ int lds[SHORT * BLOCK];
+    const int j = blockIdx.x * BLOCK + threadIdx.x;
+    const int n_cl = (int)(ctr->n_shadow2 >> 32);
+    VisitCount vc;
+    if (j < n_cl) {
+        const int i = 2 * capacity - 1 - j;
+        TraversalStack<SHORT> st;
+        st.init(lds, overflow, BLOCK);
+        const float4 o = sh.o[i], d = sh.d[i], c = sh.c[i];
+        const int4 aux = sh.aux[i];
+        const DevScene& sc = scenes[jobs[aux.y].scene];
+        ShadowReq r;
+        r.o = f4v(o); r.tmax = o.w; r.d = f4v(d); r.mode = __float_as_int(d.w); r.light = aux.x; r.c = f4v(c);
+        HitRec h;
+        const bool hit = trace<false, SHORT>(sc, r.o, r.d, T_MIN, r.tmax, h, st, vc);
+        const V v = resolve_hit(sc, r, hit, h);
+        if (!black(v)) film_add(film, ((int64_t)aux.z << 32) | __float_as_uint(c.w), v);
+    }
     if (COUNT) {
         unsigned long long a = vc.nodes, b = vc.tris;
         for (int off = 32; off > 0; off >>= 1) { a += __shfl_down(a, off); b += __shfl_down(b, off); }
@@ -853,13 +878,18 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     if (timing) hipEventRecord(prof->ev[3], s);
     hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(1), 0, s, w.ctr);
     if (timing) hipEventRecord(prof->ev[4], s);
-    const int sgrid = (int)((2 * (int64_t)cap + 64 + B - 1) / B);  // within wavefront_stack_threads(cap)
-    if (trav)
-        hipLaunchKernelGGL((k_shadow<B, SHORT_STACK, true>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr, film,
-                           w.overflow, trav, cap);
-    else
-        hipLaunchKernelGGL((k_shadow<B, SHORT_STACK, false>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr, film,
-                           w.overflow, trav, cap);
+    const int sgrid = (int)((2 * (int64_t)cap + B - 1) / B);  // shadow requests <= 2 per path
+    if (trav) {
+        hipLaunchKernelGGL((k_shadow_any<B, SHORT_STACK, true>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr, film,
+                           w.overflow, trav);
+        hipLaunchKernelGGL((k_shadow_closest<B, SHORT_STACK, true>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr,
+                           film, w.overflow, trav, cap);
+    } else {
+        hipLaunchKernelGGL((k_shadow_any<B, SHORT_STACK, false>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr, film,
+                           w.overflow, trav);
+        hipLaunchKernelGGL((k_shadow_closest<B, SHORT_STACK, false>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr,
+                           film, w.overflow, trav, cap);
+    }
     if (timing) hipEventRecord(prof->ev[5], s);
     std::swap(w.a, w.b);
 }
