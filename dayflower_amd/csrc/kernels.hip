@@ -662,8 +662,10 @@ DEV void put_shadow(ShadowSoA sh, int s, const ShadowReq& R, int64_t film_idx, i
     sh.aux[s] = make_int4(R.light, job, (int)(film_idx >> 32), 0);
 }
 
+// At most 128 VGPRs (4 waves per SIMD): the few spills this costs are
+// cheaper than running at 3 waves per SIMD.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_shade(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
+__global__ __launch_bounds__(BLOCK, 4) void k_shade(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
                                                  PathSoA q, const float4* __restrict__ hits,
                                                  const int32_t* __restrict__ perm, PathSoA out, ShadowSoA sh,
                                                  Counters* __restrict__ ctr, float4* __restrict__ film, int use_perm,
