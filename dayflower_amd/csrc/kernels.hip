@@ -509,26 +509,6 @@ __global__ void k_commit(Counters* ctr, int capacity) {
     ctr->n_shadow2 = 0;
 }
 
-template <int BLOCK, int SHORT>
-DEV float4 extend_one(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, PathSoA q,
-                      float4* __restrict__ hits, int* __restrict__ overflow, int* lds, int i, VisitCount& vc) {
-    TraversalStack<SHORT> st;
-    st.init(lds, overflow, BLOCK);
-    uint4 info = q.info[i];
-    float4 out;
-    if (info.w & 0x80000000u) {
-        out = make_float4(0.0f, __int_as_float(-2), __int_as_float(-1), 0.0f);
-    } else {
-        const DevScene& sc = scenes[jobs[info.z].scene];
-        float4 o = q.o[i], d = q.d[i];
-        HitRec h;
-        bool hit = trace<false, SHORT>(sc, f4v(o), f4v(d), T_MIN, FLT_MAX, h, st, vc);
-        out = make_float4(h.t, __int_as_float(hit ? h.prim : -1), __int_as_float(h.tri), 0.0f);
-    }
-    hits[i] = out;
-    return out;
-}
-
 // Material-sort key of a hit: material type of the hit (0..5), 6 = miss,
 // 7 = dropped camera sample.
 DEV int sort_key(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, uint32_t job, float4 h) {
@@ -543,25 +523,130 @@ DEV int sort_key(const DevScene* __restrict__ scenes, const DevJob* __restrict__
 // material-sort key of each hit and the block's key histogram
 // (bhist[key * gridDim.x + block]); the histogram is built with LDS atomics,
 // so no global atomic is contended.
-// Closest hit for every queued path; with SORT the material-sort key of
-// each hit is written too.
-template <int BLOCK, int SHORT, bool COUNT, bool SORT>
-__global__ __launch_bounds__(BLOCK) void k_extend(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
-                                                  PathSoA q, float4* __restrict__ hits, Counters* __restrict__ ctr,
-                                                  int* __restrict__ overflow, unsigned long long* __restrict__ trav,
-                                                  int32_t* __restrict__ keys) {
+// Extension with ray replacement: block b owns RPL * BLOCK consecutive queue
+// entries; a lane whose ray is finished takes the next unclaimed entry of the
+// block's range (one LDS atomic per wave), so the lanes of a wave keep
+// working while its slowest ray finishes.  Traversal is the speculative
+// while-while walk of trace().
+constexpr int EXTEND_RPL = 4;  // queue entries per thread of an extension block
+
+template <int BLOCK, int SHORT, int RPL, bool COUNT, bool SORT>
+__global__ __launch_bounds__(BLOCK) void k_extend(const DevScene* __restrict__ scenes,
+                                                    const DevJob* __restrict__ jobs, PathSoA q,
+                                                    float4* __restrict__ hits, const Counters* __restrict__ ctr,
+                                                    int* __restrict__ overflow, unsigned long long* __restrict__ trav,
+                                                    int32_t* __restrict__ keys) {
     __shared__ int lds[SHORT * BLOCK];
-    int i = blockIdx.x * BLOCK + threadIdx.x;
-    int n = ctr->n_active;
+    __shared__ int next;
+    constexpr uint32_t DONE = 0x7fffffffu;
+    const int n = ctr->n_active;
+    const int base = blockIdx.x * RPL * BLOCK;
+    const int limit = min(n - base, RPL * BLOCK);
+    if (threadIdx.x == 0) next = BLOCK;
+    __syncthreads();
+    if (limit <= 0) return;
+    const int lane = lane_id();
+    const uint64_t lt = (1ull << lane) - 1ull;
+    TraversalStack<SHORT> st;
+    st.init(lds, overflow, BLOCK);
     VisitCount vc;
-    if (i < n) {
-        float4 h = extend_one<BLOCK, SHORT>(scenes, jobs, q, hits, overflow, lds, i, vc);
-        if (SORT) keys[i] = sort_key(scenes, jobs, q.info[i].z, h);
+    int e = threadIdx.x;  // entry of the block's range this lane traces
+    bool active = e < limit;
+    PRay r;
+    float tbest = FLT_MAX;
+    int prim = -1, tri = -1;
+    uint32_t cur = DONE, leaf = 0;
+    const float4* nodes = nullptr;
+    const float4* tris = nullptr;
+    bool fresh = active;  // a ray to set up at the top of the loop
+    for (;;) {
+        if (fresh) {
+            const int i = base + e;
+            const uint4 info = q.info[i];
+            fresh = false;
+            prim = -1; tri = -1; tbest = FLT_MAX; leaf = 0; st.sp = 0; cur = DONE;
+            if (info.w & 0x80000000u) {
+                prim = -2;  // dropped camera sample
+            } else {
+                const DevScene& sc = scenes[jobs[info.z].scene];
+                const V o = f4v(q.o[i]), d = f4v(q.d[i]);
+                r = prep_ray(o, d);
+                for (int k = 0; k < sc.n_analytic; k++) {
+                    const int pi = sc.analytic[k];
+                    const float t = analytic_t(sc.prims[pi], o, d, T_MIN, tbest);
+                    if (t != 0.0f && t < tbest) { tbest = t; prim = pi; tri = -1; }
+                }
+                nodes = sc.nodes; tris = sc.tris;
+                cur = sc.has_bvh ? 0u : DONE;
+            }
+        }
+        if (active && (cur != DONE || leaf != 0)) {
+            while (cur < DONE) {
+                const size_t nb = 4 * (size_t)cur;
+                const float4 n0 = ld4g(nodes, nb), n1 = ld4g(nodes, nb + 1), n2 = ld4g(nodes, nb + 2), n3 = ld4g(nodes, nb + 3);
+                if (COUNT) vc.nodes++;
+                float t0, t1;
+                bool h0, h1;
+                slab2(n0, n1, n2, r, T_MIN, tbest, t0, t1, h0, h1);
+                const uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
+                if (c1 == 0xffffffffu) h1 = false;
+                if (h0 && h1) {
+                    const bool swap = t1 < t0;
+                    st.push((int)(swap ? c0 : c1));
+                    cur = swap ? c1 : c0;
+                } else if (h0 || h1) {
+                    cur = h0 ? c0 : c1;
+                } else {
+                    cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
+                }
+                if ((cur & LEAF_FLAG_DEV) && leaf == 0) {
+                    leaf = cur;
+                    cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
+                }
+                if (__ballot(leaf == 0) == 0) break;
+            }
+            while (leaf != 0) {
+                const uint32_t first = (leaf & 0x7fffffffu) >> 3, cnt = (leaf & 7u) + 1;
+                for (uint32_t k = 0; k < cnt; k++) {
+                    if (COUNT) vc.tris++;
+                    const size_t tb = 3 * (size_t)(first + k);
+                    const float4 a = ld4g(tris, tb), b = ld4g(tris, tb + 1), c = ld4g(tris, tb + 2);
+                    const float t = tri_t(f4v(a), f4v(b), f4v(c), r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, T_MIN, tbest);
+                    if (t != 0.0f && t < tbest) { tbest = t; tri = __float_as_int(a.w); prim = __float_as_int(b.w); }
+                }
+                leaf = 0;
+                if (cur & LEAF_FLAG_DEV) {
+                    leaf = cur;
+                    cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
+                }
+            }
+        }
+        // finished rays: write the hit, then claim the next entry of the range
+        const bool finished = active && cur == DONE && leaf == 0;
+        if (finished) {
+            const int i = base + e;
+            const float4 h = make_float4(tbest, __int_as_float(prim), __int_as_float(tri), 0.0f);
+            hits[i] = h;
+            if (SORT) keys[i] = sort_key(scenes, jobs, q.info[i].z, h);
+        }
+        const uint64_t m = __ballot(finished);
+        if (m) {
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            int got = 0;
+            if (lane == leader) got = atomicAdd(&next, __popcll(m));
+            got = __shfl(got, leader);
+            if (finished) {
+                e = got + __popcll(m & lt);
+                active = e < limit;
+                fresh = active;
+            }
+        }
+        if (__ballot(active) == 0) break;
     }
     if (COUNT) {
         unsigned long long a = vc.nodes, b = vc.tris;
         for (int off = 32; off > 0; off >>= 1) { a += __shfl_down(a, off); b += __shfl_down(b, off); }
-        if (lane_id() == 0) { atomicAdd(&trav[0], a); atomicAdd(&trav[1], b); }
+        if (lane == 0) { atomicAdd(&trav[0], a); atomicAdd(&trav[1], b); }
     }
 }
 
@@ -855,18 +940,19 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     const int use_perm = p.sort_keys > 0;
     if (timing) hipEventRecord(prof->ev[0], s);
     unsigned long long* trav = prof ? prof->trav : nullptr;
+    const int egrid = (cap + EXTEND_RPL * B - 1) / (EXTEND_RPL * B);
     if (trav && use_perm)
-        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, true, true>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr,
-                           w.overflow, trav, w.keys);
+        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, EXTEND_RPL, true, true>), dim3(egrid), dim3(B), 0, s, scenes, jobs, w.a, w.hit,
+                           w.ctr, w.overflow, trav, w.keys);
     else if (trav)
-        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, true, false>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr,
-                           w.overflow, trav, w.keys);
+        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, EXTEND_RPL, true, false>), dim3(egrid), dim3(B), 0, s, scenes, jobs, w.a, w.hit,
+                           w.ctr, w.overflow, trav, w.keys);
     else if (use_perm)
-        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, false, true>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr,
-                           w.overflow, trav, w.keys);
+        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, EXTEND_RPL, false, true>), dim3(egrid), dim3(B), 0, s, scenes, jobs, w.a, w.hit,
+                           w.ctr, w.overflow, trav, w.keys);
     else
-        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, false, false>), dim3(grid), dim3(B), 0, s, scenes, jobs, w.a, w.hit, w.ctr,
-                           w.overflow, trav, w.keys);
+        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, EXTEND_RPL, false, false>), dim3(egrid), dim3(B), 0, s, scenes, jobs, w.a, w.hit,
+                           w.ctr, w.overflow, trav, w.keys);
     if (timing) hipEventRecord(prof->ev[1], s);
     if (use_perm) {
         const int tiles = (cap + SORT_TILE - 1) / SORT_TILE;
