@@ -519,15 +519,12 @@ DEV int sort_key(const DevScene* __restrict__ scenes, const DevJob* __restrict__
     return sc.mats[sc.prims[prim].material].type;
 }
 
-// Closest hit for every queued path.  With SORT, the epilogue also writes the
-// material-sort key of each hit and the block's key histogram
-// (bhist[key * gridDim.x + block]); the histogram is built with LDS atomics,
-// so no global atomic is contended.
-// Extension with ray replacement: block b owns RPL * BLOCK consecutive queue
-// entries; a lane whose ray is finished takes the next unclaimed entry of the
-// block's range (one LDS atomic per wave), so the lanes of a wave keep
-// working while its slowest ray finishes.  Traversal is the speculative
-// while-while walk of trace().
+// Closest hit for every queued path (with SORT, also its material-sort key).
+// Ray replacement: block b owns RPL * BLOCK consecutive queue entries; a lane
+// whose ray is finished takes the next unclaimed entry of the block's range
+// (one LDS atomic per wave), so the lanes of a wave keep working while its
+// slowest ray finishes.  Traversal is the speculative while-while walk of
+// trace().
 constexpr int EXTEND_RPL = 4;  // queue entries per thread of an extension block
 
 template <int BLOCK, int SHORT, int RPL, bool COUNT, bool SORT>
