@@ -188,10 +188,11 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
 
     auto t0 = std::chrono::steady_clock::now();
     int threads = (int)std::max(1u, std::thread::hardware_concurrency());
-    BvhResult bvh = build_bvh(in.positions.data(), in.indices.data(), nt, threads);
+    BvhResult bvh = build_bvh(in.positions.data(), in.indices.data(), nt, threads, BVH_WIDTH);
     out.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    out.max_depth = bvh.max_depth + 1;
-    out.n_nodes = (int64_t)bvh.nodes.size();
+    // deepest traversal stack (entries) the kernels can need
+    out.max_depth = BVH_WIDTH == 4 ? bvh.max_stack4 + 1 : bvh.max_depth + 1;
+    out.n_nodes = BVH_WIDTH == 4 ? (int64_t)bvh.nodes4.size() : (int64_t)bvh.nodes.size();
     out.n_tris = nt;
     out.sah_cost = bvh.sah_cost;
 
@@ -304,7 +305,7 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
         }
     }
     int rc;
-    if ((rc = upload(out.nodes, bvh.nodes, s))) return rc;
+    if ((rc = BVH_WIDTH == 4 ? upload(out.nodes, bvh.nodes4, s) : upload(out.nodes, bvh.nodes, s))) return rc;
     if ((rc = upload(out.tris, tris, s))) return rc;
     if ((rc = upload(out.tri_idx, tri_idx, s))) return rc;
     if ((rc = upload(out.vpos, vpos, s))) return rc;

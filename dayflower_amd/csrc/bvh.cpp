@@ -157,7 +157,68 @@ struct Builder {
 
 }  // namespace
 
-BvhResult build_bvh(const float* positions, const int32_t* indices, int64_t n, int threads) {
+// Collapse of the binary tree into four-wide nodes: each four-wide node takes
+// a binary node's two children and keeps opening the largest-area internal
+// child until it has four children or only leaves.  Breadth-first numbering.
+static void emit_four_wide(const std::vector<BinNode>& bn, BvhResult& r) {
+    auto leaf_ref = [&](const BinNode& x) { return make_leaf((uint32_t)x.first, (uint32_t)x.count); };
+    std::vector<std::vector<int64_t>> kids;  // binary children of each four-wide node
+    std::vector<int64_t> queue;              // binary node of each four-wide node
+    auto open = [&](int64_t root) {
+        std::vector<int64_t> c;
+        if (bn[root].left < 0) { c.push_back(root); return c; }
+        c.push_back(bn[root].left);
+        c.push_back(bn[root].left + 1);
+        while (c.size() < 4) {
+            int best = -1;
+            float best_area = -1.0f;
+            for (int k = 0; k < (int)c.size(); k++)
+                if (bn[c[k]].left >= 0 && bn[c[k]].box.area() > best_area) { best = k; best_area = bn[c[k]].box.area(); }
+            if (best < 0) break;
+            int64_t x = c[best];
+            c[best] = bn[x].left;
+            c.push_back(bn[x].left + 1);
+        }
+        return c;
+    };
+    queue.push_back(0);
+    for (size_t q = 0; q < queue.size(); q++) {
+        kids.push_back(open(queue[q]));
+        for (int64_t ch : kids.back())
+            if (bn[ch].left >= 0) queue.push_back(ch);
+    }
+    const int64_t n4 = (int64_t)queue.size();
+    r.nodes4.resize(n4);
+    // four-wide index of every queued binary node
+    std::vector<int64_t> index_of(bn.size(), -1);
+    for (int64_t q = 0; q < n4; q++) index_of[queue[q]] = q;
+    std::vector<int> stack_need(n4, 0);
+    for (int64_t q = n4 - 1; q >= 0; q--) {
+        Node4& w = r.nodes4[q];
+        std::memset(&w, 0, sizeof(w));
+        const std::vector<int64_t>& c = kids[q];
+        int deepest = 0;
+        for (int k = 0; k < 4; k++) {
+            uint32_t ref = EMPTY_REF;
+            Box box;  // inverted: never entered (the reference is EMPTY_REF as well)
+            if (k < (int)c.size()) {
+                const BinNode& ch = bn[c[k]];
+                box = ch.box;
+                ref = ch.left >= 0 ? (uint32_t)index_of[c[k]] : leaf_ref(ch);
+                if (ch.left >= 0) deepest = std::max(deepest, stack_need[index_of[c[k]]]);
+            }
+            w.f[0 + k] = box.lo[0]; w.f[4 + k] = box.hi[0];
+            w.f[8 + k] = box.lo[1]; w.f[12 + k] = box.hi[1];
+            w.f[16 + k] = box.lo[2]; w.f[20 + k] = box.hi[2];
+            std::memcpy(&w.f[24 + k], &ref, 4);
+        }
+        // entering this node pushes all but one child
+        stack_need[q] = (int)c.size() - 1 + deepest;
+    }
+    r.max_stack4 = stack_need.empty() ? 0 : stack_need[0];
+}
+
+BvhResult build_bvh(const float* positions, const int32_t* indices, int64_t n, int threads, int width) {
     BvhResult r;
     if (n <= 0) return r;
     if (n >= (int64_t(1) << 28)) throw std::runtime_error("too many triangles for the leaf encoding");
@@ -179,55 +240,59 @@ BvhResult build_bvh(const float* positions, const int32_t* indices, int64_t n, i
     b.build(0, 0);
     const int64_t nb = b.next_node.load();
 
-    // Emit the two-wide layout: one wide node per internal binary node.
-    std::vector<int64_t> wide_of(nb, -1);
-    int64_t nw = 0;
-    std::vector<int64_t> stack{0};
-    // breadth-first numbering keeps siblings close in memory
-    std::vector<int64_t> queue{0};
-    if (b.nodes[0].left < 0) {
-        wide_of[0] = nw++;
-    } else {
-        for (size_t q = 0; q < queue.size(); q++) {
-            int64_t i = queue[q];
-            wide_of[i] = nw++;
-            for (int c = 0; c < 2; c++) {
-                int64_t ch = b.nodes[i].left + c;
-                if (b.nodes[ch].left >= 0) queue.push_back(ch);
-            }
-        }
-    }
-    r.nodes.resize(nw);
-    auto leaf_ref = [&](const BinNode& x) { return make_leaf((uint32_t)x.first, (uint32_t)x.count); };
-    auto put = [&](WideNode& w, int c, const BinNode& ch, uint32_t ref) {
-        w.f[c * 4 + 0] = ch.box.lo[0]; w.f[c * 4 + 1] = ch.box.hi[0];
-        w.f[c * 4 + 2] = ch.box.lo[1]; w.f[c * 4 + 3] = ch.box.hi[1];
-        w.f[8 + c * 2 + 0] = ch.box.lo[2]; w.f[8 + c * 2 + 1] = ch.box.hi[2];
-        std::memcpy(&w.f[12 + c], &ref, 4);
-    };
     double sah = 0.0;
-    float root_area = b.nodes[0].box.area();
-    if (b.nodes[0].left < 0) {
-        WideNode w; std::memset(&w, 0, sizeof(w));
-        put(w, 0, b.nodes[0], leaf_ref(b.nodes[0]));
-        BinNode empty;  // inverted box never hit
-        put(w, 1, empty, EMPTY_REF);
-        r.nodes[0] = w;
+    if (width == 4) {
+        emit_four_wide(b.nodes, r);
     } else {
-        for (int64_t i = 0; i < nb; i++) {
-            if (wide_of[i] < 0) continue;
-            const BinNode& n = b.nodes[i];
-            WideNode w; std::memset(&w, 0, sizeof(w));
-            for (int c = 0; c < 2; c++) {
-                const BinNode& ch = b.nodes[n.left + c];
-                uint32_t ref = ch.left >= 0 ? (uint32_t)wide_of[n.left + c] : leaf_ref(ch);
-                put(w, c, ch, ref);
-                if (root_area > 0.0f) {
-                    float a = ch.box.area() / root_area;
-                    sah += ch.left >= 0 ? COST_TRAVERSE * a : COST_INTERSECT * a * (double)ch.count;
+        // Emit the two-wide layout: one wide node per internal binary node.
+        std::vector<int64_t> wide_of(nb, -1);
+        int64_t nw = 0;
+        std::vector<int64_t> stack{0};
+        // breadth-first numbering keeps siblings close in memory
+        std::vector<int64_t> queue{0};
+        if (b.nodes[0].left < 0) {
+            wide_of[0] = nw++;
+        } else {
+            for (size_t q = 0; q < queue.size(); q++) {
+                int64_t i = queue[q];
+                wide_of[i] = nw++;
+                for (int c = 0; c < 2; c++) {
+                    int64_t ch = b.nodes[i].left + c;
+                    if (b.nodes[ch].left >= 0) queue.push_back(ch);
                 }
             }
-            r.nodes[wide_of[i]] = w;
+        }
+        r.nodes.resize(nw);
+        auto leaf_ref = [&](const BinNode& x) { return make_leaf((uint32_t)x.first, (uint32_t)x.count); };
+        auto put = [&](WideNode& w, int c, const BinNode& ch, uint32_t ref) {
+            w.f[c * 4 + 0] = ch.box.lo[0]; w.f[c * 4 + 1] = ch.box.hi[0];
+            w.f[c * 4 + 2] = ch.box.lo[1]; w.f[c * 4 + 3] = ch.box.hi[1];
+            w.f[8 + c * 2 + 0] = ch.box.lo[2]; w.f[8 + c * 2 + 1] = ch.box.hi[2];
+            std::memcpy(&w.f[12 + c], &ref, 4);
+        };
+        float root_area = b.nodes[0].box.area();
+        if (b.nodes[0].left < 0) {
+            WideNode w; std::memset(&w, 0, sizeof(w));
+            put(w, 0, b.nodes[0], leaf_ref(b.nodes[0]));
+            BinNode empty;  // inverted box never hit
+            put(w, 1, empty, EMPTY_REF);
+            r.nodes[0] = w;
+        } else {
+            for (int64_t i = 0; i < nb; i++) {
+                if (wide_of[i] < 0) continue;
+                const BinNode& n = b.nodes[i];
+                WideNode w; std::memset(&w, 0, sizeof(w));
+                for (int c = 0; c < 2; c++) {
+                    const BinNode& ch = b.nodes[n.left + c];
+                    uint32_t ref = ch.left >= 0 ? (uint32_t)wide_of[n.left + c] : leaf_ref(ch);
+                    put(w, c, ch, ref);
+                    if (root_area > 0.0f) {
+                        float a = ch.box.area() / root_area;
+                        sah += ch.left >= 0 ? COST_TRAVERSE * a : COST_INTERSECT * a * (double)ch.count;
+                    }
+                }
+                r.nodes[wide_of[i]] = w;
+            }
         }
     }
     r.tri_order = std::move(b.order);

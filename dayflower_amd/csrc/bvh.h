@@ -30,15 +30,27 @@ struct WideNode {
 };
 static_assert(sizeof(WideNode) == 64, "node is one 64-byte record");
 
+// Four-wide node (128 bytes, one L2 line): child boxes as structure of
+// arrays, then the child references.  An unused child slot has EMPTY_REF.
+struct Node4 {
+    // f[0..3] lo.x, f[4..7] hi.x, f[8..11] lo.y, f[12..15] hi.y,
+    // f[16..19] lo.z, f[20..23] hi.z, f[24..27] refs (uint32 bits), f[28..31] 0
+    float f[32];
+};
+static_assert(sizeof(Node4) == 128, "node is one 128-byte record");
+
 struct BvhResult {
-    std::vector<WideNode> nodes;        // nodes[0] is the root
+    std::vector<WideNode> nodes;        // two-wide layout; nodes[0] is the root
+    std::vector<Node4> nodes4;          // four-wide layout (when requested); nodes4[0] is the root
     std::vector<uint32_t> tri_order;    // BVH position -> original triangle
-    int max_depth = 0;
+    int max_depth = 0;                  // of the binary tree
+    int max_stack4 = 0;                 // deepest traversal stack of the four-wide tree
     int64_t leaf_count = 0;
     double sah_cost = 0.0;
 };
 
-// positions: 3 floats per vertex; indices: 3 per triangle.
-BvhResult build_bvh(const float* positions, const int32_t* indices, int64_t n_triangles, int threads);
+// positions: 3 floats per vertex; indices: 3 per triangle.  width 2 emits
+// `nodes`, width 4 collapses the binary tree into `nodes4`.
+BvhResult build_bvh(const float* positions, const int32_t* indices, int64_t n_triangles, int threads, int width = 2);
 
 }  // namespace dfr

@@ -368,6 +368,77 @@ struct VisitCount {
     int nodes = 0, tris = 0;
 };
 
+constexpr uint32_t TRAVERSAL_DONE = 0x7fffffffu;  // no node left (bit 31 clear, never a node index)
+
+// One inner-node step of an ordered traversal: tests the children's boxes
+// against (tmin, tmax), continues with the nearest entered child and pushes
+// the other entered children (farthest first, so the nearer pop first); with
+// none entered, pops.  Returns the next reference or TRAVERSAL_DONE.
+template <int SHORT>
+DEV uint32_t visit2(const float4* nodes, uint32_t cur, const PRay& r, float tmin, float tmax, TraversalStack<SHORT>& st) {
+    const size_t nb = 4 * (size_t)cur;
+    const float4 n0 = ld4g(nodes, nb), n1 = ld4g(nodes, nb + 1), n2 = ld4g(nodes, nb + 2), n3 = ld4g(nodes, nb + 3);
+    float t0, t1;
+    bool h0, h1;
+    slab2(n0, n1, n2, r, tmin, tmax, t0, t1, h0, h1);
+    const uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
+    if (c1 == 0xffffffffu) h1 = false;
+    if (h0 && h1) {
+        const bool swap = t1 < t0;
+        st.push((int)(swap ? c0 : c1));
+        return swap ? c1 : c0;
+    }
+    if (h0 || h1) return h0 ? c0 : c1;
+    return st.sp == 0 ? TRAVERSAL_DONE : (uint32_t)st.pop();
+}
+
+template <int SHORT>
+DEV uint32_t visit4(const float4* nodes, uint32_t cur, const PRay& r, float tmin, float tmax, TraversalStack<SHORT>& st) {
+    const size_t nb = 8 * (size_t)cur;
+    const float4 lx = ld4g(nodes, nb), hx = ld4g(nodes, nb + 1), ly = ld4g(nodes, nb + 2), hy = ld4g(nodes, nb + 3);
+    const float4 lz = ld4g(nodes, nb + 4), hz = ld4g(nodes, nb + 5), rf = ld4g(nodes, nb + 6);
+    const float L[3][4] = {{lx.x, lx.y, lx.z, lx.w}, {ly.x, ly.y, ly.z, ly.w}, {lz.x, lz.y, lz.z, lz.w}};
+    const float H[3][4] = {{hx.x, hx.y, hx.z, hx.w}, {hy.x, hy.y, hy.z, hy.w}, {hz.x, hz.y, hz.z, hz.w}};
+    const uint32_t R[4] = {__float_as_uint(rf.x), __float_as_uint(rf.y), __float_as_uint(rf.z), __float_as_uint(rf.w)};
+    float key[4];
+    uint32_t ref[4];
+    int hits = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const float ax = L[0][c] * r.inv.x - r.oinv.x, bx = H[0][c] * r.inv.x - r.oinv.x;
+        const float ay = L[1][c] * r.inv.y - r.oinv.y, by = H[1][c] * r.inv.y - r.oinv.y;
+        const float az = L[2][c] * r.inv.z - r.oinv.z, bz = H[2][c] * r.inv.z - r.oinv.z;
+        const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
+        const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
+        // far end widened by 1 + 2*gamma(3), as in the two-wide test
+        const bool h = tn <= tf * 1.00000048f && R[c] != 0xffffffffu;
+        key[c] = h ? tn : INFINITY;
+        ref[c] = R[c];
+        hits += h ? 1 : 0;
+    }
+    // sort the four (key, ref) pairs by key: misses (infinite keys) go last
+    auto ce = [&](int a, int b) {
+        const bool sw = key[b] < key[a];
+        const float ka = key[a], kb = key[b];
+        const uint32_t ra = ref[a], rb = ref[b];
+        key[a] = sw ? kb : ka; key[b] = sw ? ka : kb;
+        ref[a] = sw ? rb : ra; ref[b] = sw ? ra : rb;
+    };
+    ce(0, 1); ce(2, 3); ce(0, 2); ce(1, 3); ce(1, 2);
+    if (hits == 0) return st.sp == 0 ? TRAVERSAL_DONE : (uint32_t)st.pop();
+    if (hits >= 4) st.push((int)ref[3]);
+    if (hits >= 3) st.push((int)ref[2]);
+    if (hits >= 2) st.push((int)ref[1]);
+    return ref[0];
+}
+
+template <int SHORT>
+DEV uint32_t visit_node(const float4* nodes, uint32_t cur, const PRay& r, float tmin, float tmax,
+                        TraversalStack<SHORT>& st) {
+    if (BVH_WIDTH == 4) return visit4<SHORT>(nodes, cur, r, tmin, tmax, st);
+    return visit2<SHORT>(nodes, cur, r, tmin, tmax, st);
+}
+
 // With REFINE (closest hit only), the winner's distance is recomputed in
 // FP64 for the ray-query outputs; the path tracer keeps the float distance of
 // the reference intersection routines, from which its next rays start.
@@ -392,27 +463,13 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
     // and keeps descending inner nodes until every lane of the wave has a
     // parked leaf (or is done); then the wave tests leaves together.  This
     // keeps lanes busy while others are still in inner nodes.
-    constexpr uint32_t DONE = 0x7fffffffu;  // no node left (bit 31 clear, never a node index)
+    constexpr uint32_t DONE = TRAVERSAL_DONE;
     uint32_t cur = 0;                       // root node
     uint32_t leaf = 0;                      // parked leaf reference (0: none)
     while (cur != DONE) {
         while (cur < DONE) {  // inner node
-            const size_t nb = 4 * (size_t)cur;
-            float4 n0 = ld4g(s.nodes, nb), n1 = ld4g(s.nodes, nb + 1), n2 = ld4g(s.nodes, nb + 2), n3 = ld4g(s.nodes, nb + 3);
             vc.nodes++;
-            float t0, t1; bool h0, h1;
-            slab2(n0, n1, n2, r, tmin, hit.t, t0, t1, h0, h1);
-            uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
-            if (c1 == 0xffffffffu) h1 = false;
-            if (h0 && h1) {
-                bool swap = t1 < t0;
-                st.push((int)(swap ? c0 : c1));
-                cur = swap ? c1 : c0;
-            } else if (h0 || h1) {
-                cur = h0 ? c0 : c1;
-            } else {
-                cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
-            }
+            cur = visit_node<SHORT>(s.nodes, cur, r, tmin, hit.t, st);
             if ((cur & LEAF_FLAG_DEV) && leaf == 0) {
                 leaf = cur;
                 cur = st.sp == 0 ? DONE : (uint32_t)st.pop();

@@ -5,6 +5,10 @@
 
 namespace dfr {
 
+// Branching factor of the BVH the scene compiler emits and the kernels walk:
+// 2 (64-byte child-pair nodes) or 4 (128-byte four-child nodes).
+constexpr int BVH_WIDTH = 4;
+
 // Lobe kinds of the compiled BSDF (M/scene/bxdf/*).
 enum LobeKind : int32_t {
     LOBE_LAMBERT = 0,          // LambertianBRDF

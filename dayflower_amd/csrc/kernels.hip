@@ -535,7 +535,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend(const DevScene* __restrict__ s
                                                     int32_t* __restrict__ keys) {
     __shared__ int lds[SHORT * BLOCK];
     __shared__ int next;
-    constexpr uint32_t DONE = 0x7fffffffu;
+    constexpr uint32_t DONE = TRAVERSAL_DONE;
     const int n = ctr->n_active;
     const int base = blockIdx.x * RPL * BLOCK;
     const int limit = min(n - base, RPL * BLOCK);
@@ -579,23 +579,8 @@ __global__ __launch_bounds__(BLOCK) void k_extend(const DevScene* __restrict__ s
         }
         if (active && (cur != DONE || leaf != 0)) {
             while (cur < DONE) {
-                const size_t nb = 4 * (size_t)cur;
-                const float4 n0 = ld4g(nodes, nb), n1 = ld4g(nodes, nb + 1), n2 = ld4g(nodes, nb + 2), n3 = ld4g(nodes, nb + 3);
                 if (COUNT) vc.nodes++;
-                float t0, t1;
-                bool h0, h1;
-                slab2(n0, n1, n2, r, T_MIN, tbest, t0, t1, h0, h1);
-                const uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
-                if (c1 == 0xffffffffu) h1 = false;
-                if (h0 && h1) {
-                    const bool swap = t1 < t0;
-                    st.push((int)(swap ? c0 : c1));
-                    cur = swap ? c1 : c0;
-                } else if (h0 || h1) {
-                    cur = h0 ? c0 : c1;
-                } else {
-                    cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
-                }
+                cur = visit_node<SHORT>(nodes, cur, r, T_MIN, tbest, st);
                 if ((cur & LEAF_FLAG_DEV) && leaf == 0) {
                     leaf = cur;
                     cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
