@@ -527,8 +527,10 @@ DEV int sort_key(const DevScene* __restrict__ scenes, const DevJob* __restrict__
 // trace().
 constexpr int EXTEND_RPL = 4;  // queue entries per thread of an extension block
 
+// At most 72 VGPRs (7 waves per SIMD): measured faster than 6 waves with no
+// register limit and than 8 waves with spills.
 template <int BLOCK, int SHORT, int RPL, bool COUNT, bool SORT>
-__global__ __launch_bounds__(BLOCK) void k_extend(const DevScene* __restrict__ scenes,
+__global__ __launch_bounds__(BLOCK, 7) void k_extend(const DevScene* __restrict__ scenes,
                                                     const DevJob* __restrict__ jobs, PathSoA q,
                                                     float4* __restrict__ hits, const Counters* __restrict__ ctr,
                                                     int* __restrict__ overflow, unsigned long long* __restrict__ trav,
