@@ -252,6 +252,7 @@ def main():
     r.setKernel(0 if args.kernel == "wavefront" else 1)
     r.setWavefront(pool_paths=args.pool or None, sort_keys=None if args.sort_keys < 0 else args.sort_keys)
     wf = r.getWavefront()
+    bvh_width = r.getBvhWidth()
     t0 = time.perf_counter()
     r.setup()
     log(f"bench: setup (BVH build + upload) {time.perf_counter() - t0:.1f} s")
@@ -299,10 +300,13 @@ def main():
     stages = None
     if tot["ext_s"] > 0:
         # Algorithmic bytes per traversal query: ray in (32) + hit out (16) +
-        # 32 B per bounding box tested (a two-wide node fetch tests two) + 36 B
-        # of vertex positions per triangle test.
+        # the node record per node visit (two-wide: 4 float4 = 64 B; four-wide:
+        # 6 box planes x 4 children + 4 child references = 7 float4 = 112 B) +
+        # 36 B of vertex positions per triangle test.
+        node_bytes = 112 if bvh_width == 4 else 64
+
         def bytes_per(nodes, tris):
-            return 32 + 16 + 64 * nodes + 36 * tris
+            return 32 + 16 + node_bytes * nodes + 36 * tris
 
         launches = max(1, tot["ext_launch"])
         kern = {
@@ -351,7 +355,7 @@ def main():
                                f"RR from 5, {'Perez sky + ' if not args.no_perez else ''}point light",
                    "kernel": args.kernel, "triangles": sc.n_triangles, "width": args.width, "height": args.height,
                    "spp": args.spp, "max_bounce": args.max_bounce,
-                   "wavefront": wf if args.kernel == "wavefront" else None},
+                   "wavefront": wf if args.kernel == "wavefront" else None, "bvh_width": bvh_width},
         "mrays_per_s": round((tot["closest"] + tot["any"]) / wall / 1e6, 1),
         "mrays_closest_per_s": round(tot["closest"] / wall / 1e6, 1),
         "mrays_any_per_s": round(tot["any"] / wall / 1e6, 1),

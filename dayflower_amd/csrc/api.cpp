@@ -332,6 +332,7 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
     ds.perez = (const DevPerez*)out.perez.ptr;
     ds.n_analytic = (int32_t)analytic.size();
     ds.n_lights = (int32_t)lights.size();
+    for (const DevLight& L : lights) out.area_lights = out.area_lights || L.type == DFR_LIGHT_DIFFUSE_AREA;
     ds.has_bvh = nt > 0 ? 1 : 0;
     ds.light_pdf_mode = light_pdf_mode;
     std::memcpy(out.camera, in.camera, sizeof(out.camera));
@@ -384,6 +385,11 @@ int Ctx::sync_scene_table() {
     e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "scene table upload");
     return DFR_OK;
+}
+
+bool Ctx::any_area_lights() const {
+    for (auto& s : scenes) if (s && s->area_lights) return true;
+    return false;
 }
 
 int Ctx::max_depth() const {
@@ -447,7 +453,7 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
     if ((rc = ensure_overflow(threads))) return rc;
     wf.overflow = (int*)overflow.ptr;
     wavefront_init(wf, (const DevJob*)job_buf.ptr, n_jobs, work_total, stream);
-    WavefrontParams p{wf_params.block, wf_params.sort_keys, (capacity + 255) / 256};
+    WavefrontParams p{wf_params.block, wf_params.sort_keys, (capacity + 255) / 256, any_area_lights()};
     Counters host{};
     const int poll = 4;
     const int max_iterations = 1 << 20;  // far above max_bounce * (work / capacity + 1)
@@ -662,6 +668,7 @@ int32_t dfr_get_i32(const dfr_ctx* h, int32_t key, int32_t* v) {
         case DFR_KEY_WF_BLOCK: *v = c->wf_params.block; break;
         case DFR_KEY_WF_POOL_PATHS: *v = c->wf_params.capacity; break;
         case DFR_KEY_WF_SORT_KEYS: *v = c->wf_params.sort_keys; break;
+        case DFR_KEY_BVH_WIDTH: *v = BVH_WIDTH; break;
         default: return DFR_E_ARGUMENT;
     }
     return DFR_OK;

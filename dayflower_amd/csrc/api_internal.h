@@ -56,6 +56,7 @@ struct CompiledScene {
     DeviceBuffer nodes, tris, tri_idx, vpos, vnrm, tri_ng, prims, analytic, mats, lobes, lights, perez;
     DevScene dev{};
     int max_depth = 1;
+    bool area_lights = false;  // any DiffuseAreaLight (the only lights whose NEE needs closest hits)
     int64_t n_nodes = 0, n_tris = 0;
     double sah_cost = 0.0, build_seconds = 0.0;
     size_t bytes = 0;
@@ -117,6 +118,7 @@ struct Ctx {
     int hip_fail(hipError_t e, const char* what);
     int sync_scene_table();
     int max_depth() const;
+    bool any_area_lights() const;
     int ensure_overflow(int64_t threads);
     int ensure_film(int64_t pixels);
     int ensure_wavefront(int capacity);

@@ -92,6 +92,7 @@ struct WavefrontParams {
     int block;          // threads per block
     int sort_keys;      // 0: no material sort, else number of sort keys
     int grid_cap;       // blocks of the queue kernels (capacity / block)
+    bool closest_shadow;  // some scene has area lights: launch the closest-hit shadow kernel
 };
 
 // Optional instrumentation of one wavefront iteration: HIP events around the

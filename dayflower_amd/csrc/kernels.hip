@@ -76,6 +76,21 @@ DEV void wave_count_jobs(int* job_done, bool pred, uint32_t job) {
     }
 }
 
+// Adds the block's traversal visit counts to counter[0] (nodes) and
+// counter[1] (triangles): wave sums, then one atomic pair per block.  Every
+// thread of the block must call it.
+template <int BLOCK>
+DEV void add_visit_counts(unsigned long long* counter, const VisitCount& vc) {
+    __shared__ unsigned long long part[2];
+    unsigned long long a = vc.nodes, b = vc.tris;
+    for (int off = 32; off > 0; off >>= 1) { a += __shfl_down(a, off); b += __shfl_down(b, off); }
+    if (threadIdx.x == 0) { part[0] = 0; part[1] = 0; }
+    __syncthreads();
+    if (lane_id() == 0 && (a | b)) { atomicAdd(&part[0], a); atomicAdd(&part[1], b); }
+    __syncthreads();
+    if (threadIdx.x == 0 && (part[0] | part[1])) { atomicAdd(&counter[0], part[0]); atomicAdd(&counter[1], part[1]); }
+}
+
 DEV void film_add(float4* film, int64_t idx, V c) {
     if (!(isfinite(c.x) && isfinite(c.y) && isfinite(c.z))) return;
     if (c.x != 0.0f) unsafeAtomicAdd(&film[idx].x, c.x);
@@ -627,11 +642,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_extend(const DevScene* __restrict_
         }
         if (__ballot(active) == 0) break;
     }
-    if (COUNT) {
-        unsigned long long a = vc.nodes, b = vc.tris;
-        for (int off = 32; off > 0; off >>= 1) { a += __shfl_down(a, off); b += __shfl_down(b, off); }
-        if (lane == 0) { atomicAdd(&trav[0], a); atomicAdd(&trav[1], b); }
-    }
+    if (COUNT) add_visit_counts<BLOCK>(trav + 0, vc);
 }
 
 // Counting sort of the path queue by material key, over tiles of SORT_TILE
@@ -857,11 +868,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow_any(const DevScene* __restrict
             film_add(film, ((int64_t)aux.z << 32) | __float_as_uint(c.w), f4v(c));
         }
     }
-    if (COUNT) {
-        unsigned long long a = vc.nodes, b = vc.tris;
-        for (int off = 32; off > 0; off >>= 1) { a += __shfl_down(a, off); b += __shfl_down(b, off); }
-        if (lane_id() == 0) { atomicAdd(&trav[2], a); atomicAdd(&trav[3], b); }
-    }
+    if (COUNT) add_visit_counts<BLOCK>(trav + 2, vc);
 }
 
 // Shadow queue, closest-hit partition (n_cl entries from the queue's end):
@@ -891,11 +898,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow_closest(const DevScene* __rest
         const V v = resolve_hit(sc, r, hit, h);
         if (!black(v)) film_add(film, ((int64_t)aux.z << 32) | __float_as_uint(c.w), v);
     }
-    if (COUNT) {
-        unsigned long long a = vc.nodes, b = vc.tris;
-        for (int off = 32; off > 0; off >>= 1) { a += __shfl_down(a, off); b += __shfl_down(b, off); }
-        if (lane_id() == 0) { atomicAdd(&trav[2], a); atomicAdd(&trav[3], b); }
-    }
+    if (COUNT) add_visit_counts<BLOCK>(trav + 2, vc);
 }
 
 __global__ void k_count_rays(Counters* ctr) {
@@ -954,13 +957,15 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     if (trav) {
         hipLaunchKernelGGL((k_shadow_any<B, SHORT_STACK, true>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr, film,
                            w.overflow, trav);
-        hipLaunchKernelGGL((k_shadow_closest<B, SHORT_STACK, true>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr,
-                           film, w.overflow, trav, cap);
+        if (p.closest_shadow)
+            hipLaunchKernelGGL((k_shadow_closest<B, SHORT_STACK, true>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh,
+                               w.ctr, film, w.overflow, trav, cap);
     } else {
         hipLaunchKernelGGL((k_shadow_any<B, SHORT_STACK, false>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr, film,
                            w.overflow, trav);
-        hipLaunchKernelGGL((k_shadow_closest<B, SHORT_STACK, false>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr,
-                           film, w.overflow, trav, cap);
+        if (p.closest_shadow)
+            hipLaunchKernelGGL((k_shadow_closest<B, SHORT_STACK, false>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh,
+                               w.ctr, film, w.overflow, trav, cap);
     }
     if (timing) hipEventRecord(prof->ev[5], s);
     std::swap(w.a, w.b);

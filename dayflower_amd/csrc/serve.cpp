@@ -245,7 +245,7 @@ static void server_loop(Server* S) {
     Counters zero{};
     hipMemcpyAsync(c->wf.ctr, &zero, sizeof(Counters), hipMemcpyHostToDevice, c->stream);
     hipMemsetAsync(S->done_mem.ptr, 0, (size_t)S->job_capacity * sizeof(int), c->stream);
-    WavefrontParams p{c->wf_params.block, c->wf_params.sort_keys, 0};
+    WavefrontParams p{c->wf_params.block, c->wf_params.sort_keys, 0, c->any_area_lights()};
     std::vector<int> done_host;
     Counters host{};
     for (;;) {

@@ -183,6 +183,10 @@ class GPURenderer:
             if v is not None:
                 self._seti(key, v)
 
+    def getBvhWidth(self):
+        """Children per node of the BVH the library builds (2 or 4)."""
+        return self._geti(abi.KEY_BVH_WIDTH)
+
     def getWavefront(self):
         return {"block": self._geti(abi.KEY_WF_BLOCK), "pool_paths": self._geti(abi.KEY_WF_POOL_PATHS),
                 "sort_keys": self._geti(abi.KEY_WF_SORT_KEYS)}
