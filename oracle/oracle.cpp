@@ -1639,6 +1639,34 @@ int32_t orc_bsdf_sample(const dfr_material* m, const float* wo, float u, float v
     *pdf = s.pdf; *type = (int32_t)s.type;
     return 1;
 }
+// Perez light probes over arrays.  pdf_many: pdf of each direction.
+// sample_many: uvw[3*i] = (u, v, uSun), surface normal nrm[3*i] (geometric
+// and shading), out[8*i] = (L.rgb, wi.xyz, pdf, ok).
+int32_t orc_perez_pdf_many(float turbidity, const float* sun, int64_t n, const float* dirs, float* pdf) {
+    Perez p;
+    perezInit(p, turbidity, V3(sun[0], sun[1], sun[2]));
+    for (int64_t i = 0; i < n; i++) pdf[i] = perezPdf(p, V3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+    return 0;
+}
+
+int32_t orc_perez_sample_many(float turbidity, const float* sun, int64_t n, const float* uvw, const float* nrm,
+                              float* out) {
+    Perez p;
+    perezInit(p, turbidity, V3(sun[0], sun[1], sun[2]));
+    for (int64_t i = 0; i < n; i++) {
+        Surface si{};
+        V3 w(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+        si.g = onbFromW(w);
+        si.s = si.g;
+        LightSample ls{};
+        bool ok = perezSample(p, si, uvw[3 * i], uvw[3 * i + 1], uvw[3 * i + 2], &ls);
+        float* o = out + 8 * i;
+        o[0] = ls.L.r; o[1] = ls.L.g; o[2] = ls.L.b;
+        o[3] = ls.wi.x; o[4] = ls.wi.y; o[5] = ls.wi.z; o[6] = ls.pdf; o[7] = ok ? 1.0f : 0.0f;
+    }
+    return 0;
+}
+
 int32_t orc_perez_sky(float turbidity, const float* sun, const float* dir, float* rgb) {
     Perez p;
     perezInit(p, turbidity, V3(sun[0], sun[1], sun[2]));

@@ -82,6 +82,9 @@ int32_t orc_bsdf_eval(const dfr_material* m, const float* wo, const float* wi, f
 int32_t orc_bsdf_sample(const dfr_material* m, const float* wo, float u, float v, float* wi, float* f,
                         float* pdf, int32_t* type);
 int32_t orc_perez_sky(float turbidity, const float* sun, const float* dir, float* rgb);
+int32_t orc_perez_pdf_many(float turbidity, const float* sun, int64_t n, const float* dirs, float* pdf);
+int32_t orc_perez_sample_many(float turbidity, const float* sun, int64_t n, const float* uvw, const float* nrm,
+                              float* out);
 
 #ifdef __cplusplus
 }

@@ -453,3 +453,110 @@ def test_refined_distances_match_fp64_on_analytic_and_mesh_hits():
     rel_plain = np.abs(t_plain[same] - t64[same]) / t64[same]
     assert rel.max() <= 1e-6
     assert rel_plain.max() > rel.max()
+
+
+# ---------------------------------------------------------------------------
+# Perez sun/sky (M/scene/light/PerezLight.java).  No reference test covers the
+# light; these checks pin the restatement by what the model itself fixes.
+# ---------------------------------------------------------------------------
+def _srgb_rgb_to_xyz():
+    """RGB -> XYZ of the sRGB primaries (0.64, 0.33), (0.30, 0.60),
+    (0.15, 0.06) and white (0.31271, 0.32902) (ColorSpaceF.S_R_G_B), derived
+    by the standard construction: columns are the primaries' XYZ scaled so
+    that RGB (1, 1, 1) maps to the white point with Y = 1."""
+    prim = np.array([[0.64, 0.33], [0.30, 0.60], [0.15, 0.06]])
+    xyz = np.array([[x / y, 1.0, (1 - x - y) / y] for x, y in prim]).T
+    wx, wy = 0.31271, 0.32902
+    white = np.array([wx / wy, 1.0, (1 - wx - wy) / wy])
+    return xyz * np.linalg.solve(xyz, white)
+
+
+def _perez_sky(oracle, turbidity, sun, d):
+    rgb = np.zeros(3, np.float32)
+    oracle.orc_perez_sky(turbidity, oracle_lib._ptr(np.asarray(sun, np.float32)),
+                         oracle_lib._ptr(np.asarray(d, np.float32)), oracle_lib._ptr(rgb))
+    return rgb.astype(np.float64)
+
+
+@pytest.mark.parametrize("turbidity", [2.0, 4.0, 8.0])
+@pytest.mark.parametrize("elev_deg", [20.0, 45.0, 70.0])
+def test_perez_zenith_luminance_and_chromaticity(oracle, turbidity, elev_deg):
+    """Looking straight up (theta = 0) the Perez ratio F(0, gamma)/F(0, theta_s)
+    is 1, so the sky's Y is Preetham's zenith luminance (kcd/m^2, times
+    1000 * 0.0001 in the reference) and its chromaticity is Preetham's zenith
+    (x, y) up to the CIE daylight-basis reconstruction of the spectrum."""
+    e = math.radians(elev_deg)
+    sun = np.array([math.cos(e), math.sin(e), 0.0])  # world y is up
+    ts = math.pi / 2 - e
+    T = turbidity
+    chi = (4.0 / 9.0 - T / 120.0) * (math.pi - 2.0 * ts)
+    Yz = ((4.0453 * T - 4.9710) * math.tan(chi) - 0.2155 * T + 2.4192) * 0.1
+    th = np.array([ts ** 3, ts ** 2, ts, 1.0])
+    xz = np.array([T * T, T, 1.0]) @ np.array([[0.00165, -0.00374, 0.00208, 0.0],
+                                               [-0.02902, 0.06377, -0.03202, 0.00394],
+                                               [0.11693, -0.21196, 0.06052, 0.25885]]) @ th
+    yz = np.array([T * T, T, 1.0]) @ np.array([[0.00275, -0.00610, 0.00316, 0.0],
+                                               [-0.04212, 0.08970, -0.04153, 0.00515],
+                                               [0.15346, -0.26756, 0.06669, 0.26688]]) @ th
+    X, Y, Z = _srgb_rgb_to_xyz() @ _perez_sky(oracle, T, sun, (0.0, 1.0, 0.0))
+    assert abs(Y - Yz) <= 2e-4 * Yz
+    s = X + Y + Z
+    assert abs(X / s - xz) < 0.01 and abs(Y / s - yz) < 0.01
+
+
+def test_perez_symmetry_and_horizon(oracle):
+    """The sky depends on the zenith angle and the angle to the sun only: it
+    is mirror-symmetric about the vertical plane through the sun.  Below the
+    horizon it is black."""
+    sun = np.array([1.0, 1.0, -1.0]) / math.sqrt(3.0)
+    up = np.array([0.0, 1.0, 0.0])
+    nrm = np.cross(up, sun)
+    nrm /= np.linalg.norm(nrm)
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        d = rng.normal(size=3)
+        d[1] = abs(d[1]) + 0.05
+        d /= np.linalg.norm(d)
+        m = d - 2.0 * np.dot(d, nrm) * nrm
+        a, b = _perez_sky(oracle, 2.0, sun, d), _perez_sky(oracle, 2.0, sun, m)
+        np.testing.assert_allclose(a, b, rtol=2e-5, atol=1e-7)
+        low = d.copy()
+        low[1] = -low[1]
+        assert not _perez_sky(oracle, 2.0, sun, low).any()
+
+
+def test_perez_pdf_normalises_and_matches_samples(oracle):
+    """The light's solid-angle pdf integrates to 1 over the sphere (Monte Carlo
+    over uniform directions); sampled directions carry the pdf that
+    evaluating the pdf at them gives, and their histogram over zenith-angle
+    bands agrees with the pdf (chi-square)."""
+    sun = np.array([1.0, 1.0, -1.0], np.float32) / np.float32(math.sqrt(3.0))
+    rng = np.random.default_rng(11)
+    n = 400_000
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    pdf = np.zeros(n, np.float32)
+    oracle.orc_perez_pdf_many(2.0, oracle_lib._ptr(sun), n, oracle_lib._ptr(d), oracle_lib._ptr(pdf))
+    integral = 4.0 * math.pi * pdf.astype(np.float64).mean()
+    assert abs(integral - 1.0) < 0.02, integral
+    # sampling with the sun branch disabled (uSun = 1 is never exceeded)
+    m = 200_000
+    uvw = np.column_stack([rng.random(m), rng.random(m), np.ones(m)]).astype(np.float32)
+    nrm = np.tile(np.array([0.0, 1.0, 0.0], np.float32), (m, 1))
+    out = np.zeros((m, 8), np.float32)
+    oracle.orc_perez_sample_many(2.0, oracle_lib._ptr(sun), m, oracle_lib._ptr(uvw), oracle_lib._ptr(nrm),
+                                 oracle_lib._ptr(out))
+    ok = out[:, 7] > 0
+    assert ok.mean() > 0.99
+    wi = np.ascontiguousarray(out[ok, 3:6])
+    pe = np.zeros(wi.shape[0], np.float32)
+    oracle.orc_perez_pdf_many(2.0, oracle_lib._ptr(sun), wi.shape[0], oracle_lib._ptr(wi), oracle_lib._ptr(pe))
+    np.testing.assert_allclose(out[ok, 6], pe, rtol=2e-3)
+    # bands of cos(zenith angle) (world y): expected mass from the uniform-direction estimate
+    edges = np.linspace(-1.0, 1.0, 11)
+    counts = np.histogram(wi[:, 1], edges)[0].astype(np.float64)
+    band = np.digitize(d[:, 1], edges) - 1
+    expect = np.array([4.0 * math.pi * np.sum(pdf[band == k]) / n for k in range(10)]) * wi.shape[0]
+    keep = expect > 50
+    chi2 = np.sum((counts[keep] - expect[keep]) ** 2 / expect[keep])
+    assert chi2 < 3.0 * keep.sum() + 30.0, (chi2, counts, expect)
