@@ -193,6 +193,40 @@ def run_serving(args):
                       "unit": "path samples/s", "n_gpus": 1}), flush=True)
 
 
+def run_soup_ao(args):
+    """BVH stress: 5M-triangle soup, 2048 x 2048 primary rays plus 16
+    ambient-occlusion rays of radius 0.05 per hit, per step."""
+    from dayflower_amd import abi, scene as S
+    from dayflower_amd.renderer import GPURenderer
+    res = 2048
+    t0 = time.perf_counter()
+    sc = S.soup_scene(5_000_000, seed=7, width=res, height=res)
+    log(f"soup_ao: scene generated in {time.perf_counter() - t0:.1f} s")
+    r = GPURenderer()
+    r.setScene(sc)
+    r.setSeed(8)
+    r.setRenderingAlgorithm(abi.ALGO_AMBIENT_OCCLUSION)
+    r.setSamples(16)
+    r.setMaximumDistance(0.05)
+    r.setup()
+    for _ in range(args.warmup):
+        r.render(1)
+    closest = anyhit = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.render(1)
+        st = r.stats()
+        closest += st["rays_closest"]
+        anyhit += st["rays_any"]
+    wall = time.perf_counter() - t0
+    r.dispose()
+    print(json.dumps({"metric": METRIC, "config": {"workload": "5M-triangle soup, 2048x2048 primary rays + 16 AO rays "
+                      "(radius 0.05) per hit per step", "triangles": sc.n_triangles, "steps": args.steps},
+                      "mrays_per_s": round((closest + anyhit) / wall / 1e6, 1),
+                      "mrays_closest_per_s": round(closest / wall / 1e6, 1), "mrays_any_per_s": round(anyhit / wall / 1e6, 1),
+                      "ms_per_step": round(wall / args.steps * 1e3, 2), "unit": "Mrays/s", "n_gpus": 1}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,7 +244,7 @@ def main():
     ap.add_argument("--scene", choices=["icosphere", "mixed"], default="icosphere",
                     help="icosphere: Plastic mesh (default frame); mixed: the Matte/Glass/Metal/Disney icosphere "
                          "of the batched and serving workloads")
-    ap.add_argument("--config", choices=["frame", "batched", "serving"], default="frame")
+    ap.add_argument("--config", choices=["frame", "batched", "serving", "soup_ao"], default="frame")
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=2.0)
@@ -231,6 +265,8 @@ def main():
         return run_batched(args)
     if args.config == "serving":
         return run_serving(args)
+    if args.config == "soup_ao":
+        return run_soup_ao(args)
 
     from dayflower_amd import scene as S
     from dayflower_amd.renderer import GPURenderer

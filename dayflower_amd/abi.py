@@ -34,6 +34,7 @@ KEY_WF_BLOCK = 14
 KEY_WF_POOL_PATHS = 15
 KEY_WF_SORT_KEYS = 16
 KEY_BVH_WIDTH = 17
+KEY_USE_GRAPHS = 18
 KEY_MAXIMUM_DISTANCE = 101
 
 E_OK, E_NULL, E_ARGUMENT, E_STATE, E_DEVICE, E_MEMORY, E_STOPPED, E_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6, -7

@@ -441,6 +441,29 @@ size_t wavefront_bytes(int capacity) {
 
 // Runs `jobs` (already on the device in job_buf) to completion through the
 // wavefront engine.  Counters are polled every `poll` iterations.
+// Runs `iterations` (an even number, so the ping-pong queues end where they
+// started) as one graph launch, capturing the graph first when `key` differs
+// from the cached one.
+int Ctx::replay_iterations(const std::vector<uintptr_t>& key, const DevScene* scenes, const DevJob* jobs, int n_jobs,
+                           float4* film, const WavefrontParams& p, int iterations) {
+    if (!graph_exec || key != graph_key) {
+        if (graph_exec) { hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return hip_fail(e, "graph capture");
+        for (int k = 0; k < iterations; k++) wavefront_iteration(scenes, jobs, n_jobs, film, wf, p, nullptr, stream);
+        e = hipStreamEndCapture(stream, &g);
+        if (e != hipSuccess) return hip_fail(e, "graph capture");
+        e = hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) { graph_exec = nullptr; return hip_fail(e, "graph instantiate"); }
+        graph_key = key;
+    }
+    hipError_t e = hipGraphLaunch(graph_exec, stream);
+    if (e != hipSuccess) return hip_fail(e, "graph launch");
+    return DFR_OK;
+}
+
 int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counters* out_ctr) {
     // a run never needs more slots than it has work items: smaller runs get a
     // smaller pool (and smaller launch grids)
@@ -464,7 +487,29 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
         hipMemsetAsync(trav, 0, 32, stream);
     }
     int iterations = 0;
-    for (int it = 0;; it++) {
+    const DevScene* scenes = (const DevScene*)scene_table.ptr;
+    const DevJob* jobs = (const DevJob*)job_buf.ptr;
+    if (use_graphs && !profile && !trav) {
+        // everything the captured launches bake in
+        const std::vector<uintptr_t> key = {
+            (uintptr_t)scenes, (uintptr_t)jobs, (uintptr_t)n_jobs, (uintptr_t)film_ptr, (uintptr_t)wf.a.o,
+            (uintptr_t)wf.a.d, (uintptr_t)wf.a.beta, (uintptr_t)wf.a.info, (uintptr_t)wf.b.o, (uintptr_t)wf.b.d,
+            (uintptr_t)wf.b.beta, (uintptr_t)wf.b.info, (uintptr_t)wf.hit, (uintptr_t)wf.perm, (uintptr_t)wf.keys,
+            (uintptr_t)wf.bhist, (uintptr_t)wf.sh.o, (uintptr_t)wf.sh.d, (uintptr_t)wf.sh.c, (uintptr_t)wf.sh.aux,
+            (uintptr_t)wf.overflow, (uintptr_t)wf.ctr, (uintptr_t)wf.job_done, (uintptr_t)wf.capacity,
+            (uintptr_t)p.block, (uintptr_t)p.sort_keys, (uintptr_t)p.closest_shadow};
+        for (int it = 0;; it += poll) {
+            if (it >= max_iterations) return fail(DFR_E_DEVICE, "wavefront did not drain");
+            if ((rc = replay_iterations(key, scenes, jobs, n_jobs, film_ptr, p, poll))) return rc;
+            iterations = it + poll;
+            hipError_t e = hipMemcpyAsync(&host, wf.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) return hip_fail(e, "wavefront counters");
+            if (host.n_next == 0 && host.work_next >= host.work_total) break;
+            if (stop.load()) return fail(DFR_E_STOPPED, "render stopped");
+        }
+    }
+    for (int it = 0; iterations == 0; it++) {
         if (it >= max_iterations) return fail(DFR_E_DEVICE, "wavefront did not drain");
         IterProfile prof{};
         prof.timing = profile != 0;
@@ -477,15 +522,13 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
             }
             for (int k = 0; k < 6; k++) prof.ev[k] = prof_events[6 * it + k];
         }
-        wavefront_iteration((const DevScene*)scene_table.ptr, (const DevJob*)job_buf.ptr, n_jobs, film_ptr, wf, p,
-                            (prof.timing || trav) ? &prof : nullptr, stream);
-        iterations = it + 1;
+        wavefront_iteration(scenes, jobs, n_jobs, film_ptr, wf, p, (prof.timing || trav) ? &prof : nullptr, stream);
         if ((it + 1) % poll == 0) {
             hipError_t e = hipMemcpyAsync(&host, wf.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream);
             if (e == hipSuccess) e = hipStreamSynchronize(stream);
             if (e != hipSuccess) return hip_fail(e, "wavefront counters");
             // after an iteration, the paths of the next queue wait in n_next
-            if (host.n_next == 0 && host.work_next >= host.work_total) break;
+            if (host.n_next == 0 && host.work_next >= host.work_total) { iterations = it + 1; break; }
             if (stop.load()) return fail(DFR_E_STOPPED, "render stopped");
         }
     }
@@ -579,6 +622,7 @@ void dfr_destroy(dfr_ctx* h) {
     hipSetDevice(c->device);
     c->scenes.clear();
     for (hipEvent_t e : c->prof_events) hipEventDestroy(e);
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     hipEventDestroy(c->ev0);
     hipEventDestroy(c->ev1);
     hipStreamDestroy(c->stream);
@@ -643,6 +687,7 @@ int32_t dfr_set_i32(dfr_ctx* h, int32_t key, int32_t v) {
         case DFR_KEY_WF_SORT_KEYS:
             if (v != 0 && v != 8) return c->fail(DFR_E_ARGUMENT, "sort keys is 0 or 8");
             c->wf_params.sort_keys = v; break;
+        case DFR_KEY_USE_GRAPHS: c->use_graphs = v != 0; break;
         default: return c->fail(DFR_E_ARGUMENT, "unknown integer key");
     }
     return DFR_OK;
@@ -669,6 +714,7 @@ int32_t dfr_get_i32(const dfr_ctx* h, int32_t key, int32_t* v) {
         case DFR_KEY_WF_POOL_PATHS: *v = c->wf_params.capacity; break;
         case DFR_KEY_WF_SORT_KEYS: *v = c->wf_params.sort_keys; break;
         case DFR_KEY_BVH_WIDTH: *v = BVH_WIDTH; break;
+        case DFR_KEY_USE_GRAPHS: *v = c->use_graphs; break;
         default: return DFR_E_ARGUMENT;
     }
     return DFR_OK;

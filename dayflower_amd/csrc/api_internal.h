@@ -110,6 +110,13 @@ struct Ctx {
     DeviceBuffer wf_mem[17];
     int wf_capacity = 0;
     std::vector<hipEvent_t> prof_events;   // 6 per wavefront iteration, reused
+    // A captured run of WAVEFRONT_POLL iterations, replayed between counter
+    // polls.  `graph_key` holds every pointer and size its launches bake in.
+    bool use_graphs = true;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uintptr_t> graph_key;
+    int replay_iterations(const std::vector<uintptr_t>& key, const DevScene* scenes, const DevJob* jobs, int n_jobs,
+                          float4* film, const WavefrontParams& p, int iterations);
 
     dfr_stats stats{};
     dfr_stats prof_stats{};   // instrumentation of the last wavefront run

@@ -183,6 +183,10 @@ class GPURenderer:
             if v is not None:
                 self._seti(key, v)
 
+    def setUseGraphs(self, on):
+        """Replay captured HIP graphs of the wavefront iterations (default on)."""
+        self._seti(abi.KEY_USE_GRAPHS, int(bool(on)))
+
     def getBvhWidth(self):
         """Children per node of the BVH the library builds (2 or 4)."""
         return self._geti(abi.KEY_BVH_WIDTH)
