@@ -249,7 +249,7 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=2.0)
     ap.add_argument("--pool", type=int, default=0, help="path pool capacity (0: library default)")
-    ap.add_argument("--sort-keys", type=int, default=-1, help="0 or 8 (default: library)")
+    ap.add_argument("--sort-keys", type=int, default=-1, help="0 (off) or 1 (on) (default: library)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="fill and copy MIB MiB on the device first (reference kernels for HBM counters)")
     args = ap.parse_args()

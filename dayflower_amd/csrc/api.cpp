@@ -424,7 +424,7 @@ int Ctx::ensure_wavefront(int capacity) {
     wf.hit = (float4*)A(wf_mem[8], c * 16);
     wf.perm = (int32_t*)A(wf_mem[9], c * 4);
     wf.keys = (int32_t*)A(wf_mem[10], c * 4);
-    wf.bhist = (int32_t*)A(wf_mem[16], 8 * (c / 4096 + 1) * 4);
+    wf.bhist = (int32_t*)A(wf_mem[16], 32 * (c / 4096 + 1) * 4);
     wf.sh.o = (float4*)A(wf_mem[11], 2 * c * 16); wf.sh.d = (float4*)A(wf_mem[12], 2 * c * 16);
     wf.sh.c = (float4*)A(wf_mem[13], 2 * c * 16); wf.sh.aux = (int4*)A(wf_mem[14], 2 * c * 16);
     wf.ctr = (Counters*)A(wf_mem[15], sizeof(Counters));
@@ -436,7 +436,7 @@ int Ctx::ensure_wavefront(int capacity) {
 }
 
 size_t wavefront_bytes(int capacity) {
-    return (size_t)capacity * (9 * 16 + 8 + 2 * 4 * 16) + 8 * ((size_t)capacity / 4096 + 1) * 4 + sizeof(Counters);
+    return (size_t)capacity * (9 * 16 + 8 + 2 * 4 * 16) + 32 * ((size_t)capacity / 4096 + 1) * 4 + sizeof(Counters);
 }
 
 // Runs `jobs` (already on the device in job_buf) to completion through the
@@ -685,7 +685,7 @@ int32_t dfr_set_i32(dfr_ctx* h, int32_t key, int32_t v) {
             if (v < 1024 || v > (1 << 24)) return c->fail(DFR_E_ARGUMENT, "path pool must hold 1024 .. 2^24 paths");
             c->wf_params.capacity = v; break;
         case DFR_KEY_WF_SORT_KEYS:
-            if (v != 0 && v != 8) return c->fail(DFR_E_ARGUMENT, "sort keys is 0 or 8");
+            if (v != 0 && v != 1) return c->fail(DFR_E_ARGUMENT, "sort is 0 (off) or 1 (on)");
             c->wf_params.sort_keys = v; break;
         case DFR_KEY_USE_GRAPHS: c->use_graphs = v != 0; break;
         default: return c->fail(DFR_E_ARGUMENT, "unknown integer key");

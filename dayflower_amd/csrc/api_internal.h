@@ -70,7 +70,7 @@ size_t wavefront_bytes(int capacity);
 struct WavefrontConfig {
     int block = 256;
     int capacity = 1 << 23;  // path pool size (8M paths, about 2.5 GB of HBM)
-    int sort_keys = 8;       // material-sort keys (0 = unsorted)
+    int sort_keys = 1;       // 1: shade in (material, light type) order; 0: queue order
 };
 
 struct Ctx {

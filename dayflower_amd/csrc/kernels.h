@@ -70,7 +70,7 @@ struct WavefrontBuffers {
     float4* hit;       // t, prim, tri (int bits)
     int32_t* perm;     // material-sorted order
     int32_t* keys;
-    int32_t* bhist;    // per-tile key histograms / offsets (8 * tiles of SORT_TILE entries)
+    int32_t* bhist;    // per-tile key histograms / offsets (32 keys * tiles of 4096 entries)
     ShadowSoA sh;
     int* overflow;     // traversal-stack overflow
     int capacity;      // path pool size
@@ -90,7 +90,7 @@ void launch_rgba8(const float4* film, uint8_t* out, int n, int tone_mapper, hipS
 
 struct WavefrontParams {
     int block;          // threads per block
-    int sort_keys;      // 0: no material sort, else number of sort keys
+    int sort_keys;      // 0: unsorted; 1: shade in (material, light type) order
     int grid_cap;       // blocks of the queue kernels (capacity / block)
     bool closest_shadow;  // some scene has area lights: launch the closest-hit shadow kernel
 };

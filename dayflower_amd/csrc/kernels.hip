@@ -526,15 +526,28 @@ __global__ void k_commit(Counters* ctr, int capacity) {
 
 // Material-sort key of a hit: material type of the hit (0..5), 6 = miss,
 // 7 = dropped camera sample.
-DEV int sort_key(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, uint32_t job, float4 h) {
-    int prim = __float_as_int(h.y);
-    if (prim == -2) return 7;
-    if (prim < 0) return 6;
-    const DevScene& sc = scenes[jobs[job].scene];
-    return sc.mats[sc.prims[prim].material].type;
+// Sort key of a hit: material type (0..5) x the type of the light its NEE
+// will sample (0..2; 3 with no lights), so that a wave of the shade kernel
+// runs one material and one light-sampling routine.  The light is chosen by
+// the same stateless random number the shade kernel draws.  30 = miss,
+// 31 = dropped camera sample.
+constexpr int NSORT_KEYS = 32;
+
+DEV int sort_key(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, uint4 info, float4 h) {
+    const int prim = __float_as_int(h.y);
+    if (prim == -2) return 31;
+    if (prim < 0) return 30;
+    const DevJob& job = jobs[info.z];
+    const DevScene& sc = scenes[job.scene];
+    const int m = sc.mats[sc.prims[prim].material].type;
+    if (sc.n_lights == 0) return 4 * m + 3;
+    const Rng rng = make_rng(job.seed, info.x, info.y);
+    const uint32_t dim = DIM_BOUNCE + DIMS_PER_BOUNCE * (info.w & 0xffu);
+    const int li = min((int)(rng.get(dim) * sc.n_lights), sc.n_lights - 1);
+    return 4 * m + sc.lights[li].type;
 }
 
-// Closest hit for every queued path (with SORT, also its material-sort key).
+// Closest hit for every queued path.
 // Ray replacement: block b owns RPL * BLOCK consecutive queue entries; a lane
 // whose ray is finished takes the next unclaimed entry of the block's range
 // (one LDS atomic per wave), so the lanes of a wave keep working while its
@@ -544,12 +557,11 @@ constexpr int EXTEND_RPL = 4;  // queue entries per thread of an extension block
 
 // At most 72 VGPRs (7 waves per SIMD): measured faster than 6 waves with no
 // register limit and than 8 waves with spills.
-template <int BLOCK, int SHORT, int RPL, bool COUNT, bool SORT>
+template <int BLOCK, int SHORT, int RPL, bool COUNT>
 __global__ __launch_bounds__(BLOCK, 7) void k_extend(const DevScene* __restrict__ scenes,
                                                     const DevJob* __restrict__ jobs, PathSoA q,
                                                     float4* __restrict__ hits, const Counters* __restrict__ ctr,
-                                                    int* __restrict__ overflow, unsigned long long* __restrict__ trav,
-                                                    int32_t* __restrict__ keys) {
+                                                    int* __restrict__ overflow, unsigned long long* __restrict__ trav) {
     __shared__ int lds[SHORT * BLOCK];
     __shared__ int next;
     constexpr uint32_t DONE = TRAVERSAL_DONE;
@@ -626,8 +638,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_extend(const DevScene* __restrict_
             const int i = base + e;
             const float4 h = make_float4(tbest, __int_as_float(prim), __int_as_float(tri), 0.0f);
             hits[i] = h;
-            if (SORT) keys[i] = sort_key(scenes, jobs, q.info[i].z, h);
-        }
+            }
         const uint64_t m = __ballot(finished);
         if (m) {
             const int leader = __ffsll((unsigned long long)m) - 1;
@@ -645,7 +656,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_extend(const DevScene* __restrict_
     if (COUNT) add_visit_counts<BLOCK>(trav + 0, vc);
 }
 
-// Counting sort of the path queue by material key, over tiles of SORT_TILE
+// Counting sort of the path queue by sort key, over tiles of SORT_TILE
 // entries: per-tile key histograms (wave ballots, no atomics on HBM), one
 // exclusive scan of the key-major histogram array, then a scatter in which
 // every wave ranks its entries per key with ballots and takes its base with
@@ -653,29 +664,36 @@ __global__ __launch_bounds__(BLOCK, 7) void k_extend(const DevScene* __restrict_
 constexpr int SORT_TILE = 4096;
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_sort_hist(const int32_t* __restrict__ keys, int32_t* __restrict__ bhist,
+__global__ __launch_bounds__(BLOCK) void k_sort_hist(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
+                                                     PathSoA q, const float4* __restrict__ hits,
+                                                     int32_t* __restrict__ keys, int32_t* __restrict__ bhist,
                                                      const Counters* __restrict__ ctr) {
-    __shared__ int hist[8];
+    __shared__ int hist[NSORT_KEYS];
     const int n = ctr->n_active;
     const int t0 = blockIdx.x * SORT_TILE;
-    if (threadIdx.x < 8) hist[threadIdx.x] = 0;
+    if (threadIdx.x < NSORT_KEYS) hist[threadIdx.x] = 0;
     __syncthreads();
-    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // each lane counts one key over the tile; lane kk of a wave owns key kk
+    int mine = 0;
     for (int k = threadIdx.x; k < SORT_TILE; k += BLOCK) {
         const int i = t0 + k;
-        const int key = i < n ? keys[i] : -1;
+        int key = -1;
+        if (i < n) {
+            key = sort_key(scenes, jobs, q.info[i], hits[i]);
+            keys[i] = key;
+        }
 #pragma unroll
-        for (int kk = 0; kk < 8; kk++) cnt[kk] += __popcll(__ballot(key == kk));
+        for (int kk = 0; kk < NSORT_KEYS; kk++) {
+            const int c = __popcll(__ballot(key == kk));
+            if (lane_id() == kk) mine += c;
+        }
     }
-    if (lane_id() == 0) {
-#pragma unroll
-        for (int kk = 0; kk < 8; kk++) if (cnt[kk]) atomicAdd(&hist[kk], cnt[kk]);
-    }
+    if (lane_id() < NSORT_KEYS && mine) atomicAdd(&hist[lane_id()], mine);
     __syncthreads();
-    if (threadIdx.x < 8) bhist[threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
+    if (threadIdx.x < NSORT_KEYS) bhist[threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
 }
 
-// Exclusive scan of the key-major tile histograms (8 * tiles entries) by one
+// Exclusive scan of the key-major tile histograms (NSORT_KEYS * tiles entries) by one
 // 1024-thread block: each thread sums a contiguous chunk, the chunk sums are
 // scanned with wave shuffles and one LDS pass over the 16 wave totals, then
 // each thread writes its chunk's prefix.
@@ -709,19 +727,19 @@ __global__ __launch_bounds__(1024) void k_sort_scan(int32_t* __restrict__ bhist,
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sort_scatter(const int32_t* __restrict__ keys, const int32_t* __restrict__ boff,
                                                         int32_t* __restrict__ perm, const Counters* __restrict__ ctr) {
-    __shared__ int base[8];
+    __shared__ int base[NSORT_KEYS];
     const int n = ctr->n_active;
     const int t0 = blockIdx.x * SORT_TILE;
     const int lane = lane_id();
     const uint64_t lt = (1ull << lane) - 1ull;
-    if (threadIdx.x < 8) base[threadIdx.x] = boff[threadIdx.x * gridDim.x + blockIdx.x];
+    if (threadIdx.x < NSORT_KEYS) base[threadIdx.x] = boff[threadIdx.x * gridDim.x + blockIdx.x];
     __syncthreads();
     for (int k = threadIdx.x; k < SORT_TILE; k += BLOCK) {
         const int i = t0 + k;
         const int key = i < n ? keys[i] : -1;
         int slot = 0;
 #pragma unroll
-        for (int kk = 0; kk < 8; kk++) {
+        for (int kk = 0; kk < NSORT_KEYS; kk++) {
             const uint64_t m = __ballot(key == kk);
             if (m) {
                 const int leader = __ffsll((unsigned long long)m) - 1;
@@ -928,23 +946,18 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     if (timing) hipEventRecord(prof->ev[0], s);
     unsigned long long* trav = prof ? prof->trav : nullptr;
     const int egrid = (cap + EXTEND_RPL * B - 1) / (EXTEND_RPL * B);
-    if (trav && use_perm)
-        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, EXTEND_RPL, true, true>), dim3(egrid), dim3(B), 0, s, scenes, jobs, w.a, w.hit,
-                           w.ctr, w.overflow, trav, w.keys);
-    else if (trav)
-        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, EXTEND_RPL, true, false>), dim3(egrid), dim3(B), 0, s, scenes, jobs, w.a, w.hit,
-                           w.ctr, w.overflow, trav, w.keys);
-    else if (use_perm)
-        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, EXTEND_RPL, false, true>), dim3(egrid), dim3(B), 0, s, scenes, jobs, w.a, w.hit,
-                           w.ctr, w.overflow, trav, w.keys);
+    if (trav)
+        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, EXTEND_RPL, true>), dim3(egrid), dim3(B), 0, s, scenes, jobs, w.a, w.hit,
+                           w.ctr, w.overflow, trav);
     else
-        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, EXTEND_RPL, false, false>), dim3(egrid), dim3(B), 0, s, scenes, jobs, w.a, w.hit,
-                           w.ctr, w.overflow, trav, w.keys);
+        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, EXTEND_RPL, false>), dim3(egrid), dim3(B), 0, s, scenes, jobs, w.a, w.hit,
+                           w.ctr, w.overflow, trav);
     if (timing) hipEventRecord(prof->ev[1], s);
     if (use_perm) {
         const int tiles = (cap + SORT_TILE - 1) / SORT_TILE;
-        hipLaunchKernelGGL((k_sort_hist<256>), dim3(tiles), dim3(256), 0, s, w.keys, w.bhist, w.ctr);
-        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, w.bhist, 8 * tiles);
+        hipLaunchKernelGGL((k_sort_hist<256>), dim3(tiles), dim3(256), 0, s, scenes, jobs, w.a, w.hit, w.keys, w.bhist,
+                           w.ctr);
+        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, w.bhist, NSORT_KEYS * tiles);
         hipLaunchKernelGGL((k_sort_scatter<256>), dim3(tiles), dim3(256), 0, s, w.keys, w.bhist, w.perm, w.ctr);
     }
     if (timing) hipEventRecord(prof->ev[2], s);

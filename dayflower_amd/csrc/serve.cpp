@@ -142,7 +142,7 @@ int autotune_impl(Ctx* c, const dfr_job* jobs, int n, int32_t* out) {
     for (auto& j : probe) { j.spp = std::min(j.spp, 4); work += (int64_t)j.width * j.height * j.spp; }
     const int blocks[] = {128, 256};
     const int caps[] = {1 << 20, 1 << 21, 1 << 22, 1 << 23};
-    const int sorts[] = {0, 8};
+    const int sorts[] = {0, 1};
     WavefrontConfig best = c->wf_params;
     double best_t = 1e30;
     for (int b : blocks)
