@@ -65,6 +65,10 @@ static_assert(sizeof(Counters) == 64, "Counters is one 64-byte line");
 // the HBM overflow stacks are sized for it.
 inline int64_t wavefront_stack_threads(int capacity) { return 2 * (int64_t)capacity + 64 + 256; }
 
+// Largest path pool: the material sort's one-block scan holds 64 histogram
+// entries per thread (32 keys x capacity / 4096 tiles <= 1024 x 64).
+constexpr int MAX_POOL_PATHS = 1 << 23;
+
 struct WavefrontBuffers {
     PathSoA a, b;
     float4* hit;       // t, prim, tri (int bits)

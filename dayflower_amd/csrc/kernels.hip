@@ -673,7 +673,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_hist(const DevScene* __restrict_
     const int t0 = blockIdx.x * SORT_TILE;
     if (threadIdx.x < NSORT_KEYS) hist[threadIdx.x] = 0;
     __syncthreads();
-    // each lane counts one key over the tile; lane kk of a wave owns key kk
+    // per wave, one pass per distinct key present: the leader lane's key,
+    // its lanes counted by one ballot; lane kk of the wave accumulates key kk
     int mine = 0;
     for (int k = threadIdx.x; k < SORT_TILE; k += BLOCK) {
         const int i = t0 + k;
@@ -682,10 +683,12 @@ __global__ __launch_bounds__(BLOCK) void k_sort_hist(const DevScene* __restrict_
             key = sort_key(scenes, jobs, q.info[i], hits[i]);
             keys[i] = key;
         }
-#pragma unroll
-        for (int kk = 0; kk < NSORT_KEYS; kk++) {
-            const int c = __popcll(__ballot(key == kk));
-            if (lane_id() == kk) mine += c;
+        uint64_t left = __ballot(key >= 0);
+        while (left) {
+            const int kk = __shfl(key, __ffsll((unsigned long long)left) - 1);
+            const uint64_t m = __ballot(key == kk);
+            if (lane_id() == kk) mine += __popcll(m);
+            left &= ~m;
         }
     }
     if (lane_id() < NSORT_KEYS && mine) atomicAdd(&hist[lane_id()], mine);
@@ -693,35 +696,54 @@ __global__ __launch_bounds__(BLOCK) void k_sort_hist(const DevScene* __restrict_
     if (threadIdx.x < NSORT_KEYS) bhist[threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
 }
 
-// Exclusive scan of the key-major tile histograms (NSORT_KEYS * tiles entries) by one
-// 1024-thread block: each thread sums a contiguous chunk, the chunk sums are
-// scanned with wave shuffles and one LDS pass over the 16 wave totals, then
-// each thread writes its chunk's prefix.
+// Exclusive scan of the key-major tile histograms (NSORT_KEYS * tiles entries;
+// a multiple of 4, at most 1024 * SCAN_CHUNK) by one 1024-thread block: each
+// thread loads a contiguous chunk with int4 loads into registers, the chunk
+// sums are scanned with wave shuffles and one pass over the 16 wave totals,
+// then each thread writes its chunk's prefixes.
+constexpr int SCAN_CHUNK = 64;
+
 __global__ __launch_bounds__(1024) void k_sort_scan(int32_t* __restrict__ bhist, int total) {
     __shared__ int wsum[16];
     const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
-    const int chunk = (total + 1023) / 1024;
+    const int chunk = ((total + 1023) / 1024 + 3) & ~3;  // entries per thread, whole int4s
     const int b0 = min(total, t * chunk), b1 = min(total, b0 + chunk);
+    int4 v[SCAN_CHUNK / 4];
     int sum = 0;
-    for (int k = b0; k < b1; k++) sum += bhist[k];
+#pragma unroll
+    for (int k = 0; k < SCAN_CHUNK / 4; k++) {
+        const int e = b0 + 4 * k;
+        v[k] = e < b1 ? *(const int4*)(bhist + e) : make_int4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int k = 0; k < SCAN_CHUNK / 4; k++) sum += v[k].x + v[k].y + v[k].z + v[k].w;
     int incl = sum;
     for (int off = 1; off < 64; off <<= 1) {
-        const int v = __shfl_up(incl, off);
-        if (lane >= off) incl += v;
+        const int u = __shfl_up(incl, off);
+        if (lane >= off) incl += u;
     }
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
     if (t < 16) {
-        int v = wsum[t], x = v;
+        int x = wsum[t], y = x;
         for (int off = 1; off < 16; off <<= 1) {
-            const int u = __shfl_up(x, off, 16);
-            if (t >= off) x += u;
+            const int u = __shfl_up(y, off, 16);
+            if (t >= off) y += u;
         }
-        wsum[t] = x - v;  // exclusive prefix of the wave totals
+        wsum[t] = y - x;  // exclusive prefix of the wave totals
     }
     __syncthreads();
     int run = wsum[w] + incl - sum;
-    for (int k = b0; k < b1; k++) { const int c = bhist[k]; bhist[k] = run; run += c; }
+#pragma unroll
+    for (int k = 0; k < SCAN_CHUNK / 4; k++) {
+        const int e = b0 + 4 * k;
+        int4 o;
+        o.x = run; run += v[k].x;
+        o.y = run; run += v[k].y;
+        o.z = run; run += v[k].z;
+        o.w = run; run += v[k].w;
+        if (e < b1) *(int4*)(bhist + e) = o;
+    }
 }
 
 template <int BLOCK>
@@ -738,16 +760,16 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scatter(const int32_t* __restric
         const int i = t0 + k;
         const int key = i < n ? keys[i] : -1;
         int slot = 0;
-#pragma unroll
-        for (int kk = 0; kk < NSORT_KEYS; kk++) {
+        uint64_t left = __ballot(key >= 0);
+        while (left) {
+            const int leader = __ffsll((unsigned long long)left) - 1;
+            const int kk = __shfl(key, leader);
             const uint64_t m = __ballot(key == kk);
-            if (m) {
-                const int leader = __ffsll((unsigned long long)m) - 1;
-                int b = 0;
-                if (lane == leader) b = atomicAdd(&base[kk], __popcll(m));
-                b = __shfl(b, leader);
-                if (key == kk) slot = b + __popcll(m & lt);
-            }
+            int b = 0;
+            if (lane == leader) b = atomicAdd(&base[kk], __popcll(m));
+            b = __shfl(b, leader);
+            if (key == kk) slot = b + __popcll(m & lt);
+            left &= ~m;
         }
         if (key >= 0) perm[slot] = i;
     }
@@ -954,7 +976,7 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
                            w.ctr, w.overflow, trav);
     if (timing) hipEventRecord(prof->ev[1], s);
     if (use_perm) {
-        const int tiles = (cap + SORT_TILE - 1) / SORT_TILE;
+        const int tiles = (cap + SORT_TILE - 1) / SORT_TILE;  // NSORT_KEYS * tiles <= 1024 * SCAN_CHUNK
         hipLaunchKernelGGL((k_sort_hist<256>), dim3(tiles), dim3(256), 0, s, scenes, jobs, w.a, w.hit, w.keys, w.bhist,
                            w.ctr);
         hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, w.bhist, NSORT_KEYS * tiles);

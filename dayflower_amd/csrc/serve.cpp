@@ -344,6 +344,7 @@ Server* server_create(Ctx* c, const dfr_server_opts& o) {
     S->opts = o;
     if (S->opts.max_active <= 0) S->opts.max_active = 256;
     if (S->opts.pool_paths <= 0) S->opts.pool_paths = c->wf_params.capacity;
+    if (S->opts.pool_paths > MAX_POOL_PATHS) { c->error = "pool_paths above 2^23"; return nullptr; }
     std::lock_guard<std::mutex> lock(c->mutex);
     hipSetDevice(c->device);
     if (c->scenes.empty()) { c->error = "no resident scene"; return nullptr; }
