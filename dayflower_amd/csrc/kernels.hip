@@ -547,25 +547,28 @@ DEV int sort_key(const DevScene* __restrict__ scenes, const DevJob* __restrict__
     return 4 * m + sc.lights[li].type;
 }
 
-// Closest hit for every queued path.
-// Ray replacement: block b owns RPL * BLOCK consecutive queue entries; a lane
-// whose ray is finished takes the next unclaimed entry of the block's range
-// (one LDS atomic per wave), so the lanes of a wave keep working while its
-// slowest ray finishes.  Traversal is the speculative while-while walk of
-// trace().
-constexpr int EXTEND_RPL = 4;  // queue entries per thread of an extension block
+// Traversal of a ray queue with ray replacement: block b owns RPL * BLOCK
+// consecutive queue entries; a lane whose ray is finished takes the next
+// unclaimed entry of the block's range (one LDS atomic per wave), so the
+// lanes of a wave keep working while its slowest ray finishes.  Traversal is
+// the speculative while-while walk of trace().
+//   SHADOW false: closest hit of every queued path, written to hits[i].
+//   SHADOW true:  the any-hit part of the shadow queue (entries [0, n_any)):
+//                 a request that hits nothing adds its contribution to the film.
+constexpr int EXTEND_RPL = 4;  // queue entries per thread of a traversal block
 
 // At most 72 VGPRs (7 waves per SIMD): measured faster than 6 waves with no
 // register limit and than 8 waves with spills.
-template <int BLOCK, int SHORT, int RPL, bool COUNT>
-__global__ __launch_bounds__(BLOCK, 7) void k_extend(const DevScene* __restrict__ scenes,
-                                                    const DevJob* __restrict__ jobs, PathSoA q,
-                                                    float4* __restrict__ hits, const Counters* __restrict__ ctr,
-                                                    int* __restrict__ overflow, unsigned long long* __restrict__ trav) {
+template <int BLOCK, int SHORT, int RPL, bool COUNT, bool SHADOW>
+__global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __restrict__ scenes,
+                                                         const DevJob* __restrict__ jobs, PathSoA q, ShadowSoA sh,
+                                                         float4* __restrict__ hits, float4* __restrict__ film,
+                                                         const Counters* __restrict__ ctr, int* __restrict__ overflow,
+                                                         unsigned long long* __restrict__ trav) {
     __shared__ int lds[SHORT * BLOCK];
     __shared__ int next;
     constexpr uint32_t DONE = TRAVERSAL_DONE;
-    const int n = ctr->n_active;
+    const int n = SHADOW ? (int)(ctr->n_shadow2 & 0xffffffffull) : ctr->n_active;
     const int base = blockIdx.x * RPL * BLOCK;
     const int limit = min(n - base, RPL * BLOCK);
     if (threadIdx.x == 0) next = BLOCK;
@@ -588,14 +591,12 @@ __global__ __launch_bounds__(BLOCK, 7) void k_extend(const DevScene* __restrict_
     for (;;) {
         if (fresh) {
             const int i = base + e;
-            const uint4 info = q.info[i];
             fresh = false;
-            prim = -1; tri = -1; tbest = FLT_MAX; leaf = 0; st.sp = 0; cur = DONE;
-            if (info.w & 0x80000000u) {
-                prim = -2;  // dropped camera sample
-            } else {
-                const DevScene& sc = scenes[jobs[info.z].scene];
-                const V o = f4v(q.o[i]), d = f4v(q.d[i]);
+            prim = -1; tri = -1; leaf = 0; st.sp = 0; cur = DONE;
+            // the ray, the analytic primitives, then the BVH root
+            auto setup = [&](const float4 o4, const float4 d4, uint32_t job) {
+                const DevScene& sc = scenes[jobs[job].scene];
+                const V o = f4v(o4), d = f4v(d4);
                 r = prep_ray(o, d);
                 for (int k = 0; k < sc.n_analytic; k++) {
                     const int pi = sc.analytic[k];
@@ -603,7 +604,17 @@ __global__ __launch_bounds__(BLOCK, 7) void k_extend(const DevScene* __restrict_
                     if (t != 0.0f && t < tbest) { tbest = t; prim = pi; tri = -1; }
                 }
                 nodes = sc.nodes; tris = sc.tris;
-                cur = sc.has_bvh ? 0u : DONE;
+                cur = sc.has_bvh && !(SHADOW && prim >= 0) ? 0u : DONE;
+            };
+            if (SHADOW) {
+                const float4 o4 = sh.o[i];
+                tbest = o4.w;
+                setup(o4, sh.d[i], (uint32_t)sh.aux[i].y);
+            } else {
+                const uint4 info = q.info[i];
+                tbest = FLT_MAX;
+                if (info.w & 0x80000000u) prim = -2;  // dropped camera sample
+                else setup(q.o[i], q.d[i], info.z);
             }
         }
         if (active && (cur != DONE || leaf != 0)) {
@@ -618,27 +629,38 @@ __global__ __launch_bounds__(BLOCK, 7) void k_extend(const DevScene* __restrict_
             }
             while (leaf != 0) {
                 const uint32_t first = (leaf & 0x7fffffffu) >> 3, cnt = (leaf & 7u) + 1;
+                bool occluded = false;
                 for (uint32_t k = 0; k < cnt; k++) {
                     if (COUNT) vc.tris++;
                     const size_t tb = 3 * (size_t)(first + k);
                     const float4 a = ld4g(tris, tb), b = ld4g(tris, tb + 1), c = ld4g(tris, tb + 2);
                     const float t = tri_t(f4v(a), f4v(b), f4v(c), r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, T_MIN, tbest);
-                    if (t != 0.0f && t < tbest) { tbest = t; tri = __float_as_int(a.w); prim = __float_as_int(b.w); }
+                    if (t != 0.0f && t < tbest) {
+                        tbest = t; tri = __float_as_int(a.w); prim = __float_as_int(b.w);
+                        if (SHADOW) { occluded = true; break; }
+                    }
                 }
                 leaf = 0;
+                if (SHADOW && occluded) { cur = DONE; break; }
                 if (cur & LEAF_FLAG_DEV) {
                     leaf = cur;
                     cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
                 }
             }
         }
-        // finished rays: write the hit, then claim the next entry of the range
+        // finished rays: record the result, then claim the next entry of the range
         const bool finished = active && cur == DONE && leaf == 0;
         if (finished) {
             const int i = base + e;
-            const float4 h = make_float4(tbest, __int_as_float(prim), __int_as_float(tri), 0.0f);
-            hits[i] = h;
+            if (SHADOW) {
+                if (prim < 0) {
+                    const float4 c = sh.c[i];
+                    film_add(film, ((int64_t)sh.aux[i].z << 32) | __float_as_uint(c.w), f4v(c));
+                }
+            } else {
+                hits[i] = make_float4(tbest, __int_as_float(prim), __int_as_float(tri), 0.0f);
             }
+        }
         const uint64_t m = __ballot(finished);
         if (m) {
             const int leader = __ffsll((unsigned long long)m) - 1;
@@ -653,7 +675,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_extend(const DevScene* __restrict_
         }
         if (__ballot(active) == 0) break;
     }
-    if (COUNT) add_visit_counts<BLOCK>(trav + 0, vc);
+    if (COUNT) add_visit_counts<BLOCK>(trav + (SHADOW ? 2 : 0), vc);
 }
 
 // Counting sort of the path queue by sort key, over tiles of SORT_TILE
@@ -885,32 +907,6 @@ __global__ __launch_bounds__(BLOCK, 4) void k_shade(const DevScene* __restrict__
     if (has1) put_shadow(sh, any1 ? s_any + (any0 ? 1 : 0) : last - (s_cl + (has0 && !any0 ? 1 : 0)), r1, film_idx, job);
 }
 
-// Shadow queue, any-hit partition (entries [0, n_any)): a delta-light or
-// infinite-light visibility ray contributes c when nothing is hit.
-template <int BLOCK, int SHORT, bool COUNT>
-__global__ __launch_bounds__(BLOCK) void k_shadow_any(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
-                                                      ShadowSoA sh, const Counters* __restrict__ ctr,
-                                                      float4* __restrict__ film, int* __restrict__ overflow,
-                                                      unsigned long long* __restrict__ trav) {
-    __shared__ int lds[SHORT * BLOCK];
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    const int n_any = (int)(ctr->n_shadow2 & 0xffffffffull);
-    VisitCount vc;
-    if (i < n_any) {
-        TraversalStack<SHORT> st;
-        st.init(lds, overflow, BLOCK);
-        const float4 o = sh.o[i], d = sh.d[i];
-        const int4 aux = sh.aux[i];
-        const DevScene& sc = scenes[jobs[aux.y].scene];
-        HitRec h;
-        if (!trace<true, SHORT>(sc, f4v(o), f4v(d), T_MIN, o.w, h, st, vc)) {
-            const float4 c = sh.c[i];
-            film_add(film, ((int64_t)aux.z << 32) | __float_as_uint(c.w), f4v(c));
-        }
-    }
-    if (COUNT) add_visit_counts<BLOCK>(trav + 2, vc);
-}
-
 // Shadow queue, closest-hit partition (n_cl entries from the queue's end):
 // area-light visibility (mode 1) and BSDF-sampled rays toward area lights
 // (mode 2).
@@ -969,11 +965,11 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     unsigned long long* trav = prof ? prof->trav : nullptr;
     const int egrid = (cap + EXTEND_RPL * B - 1) / (EXTEND_RPL * B);
     if (trav)
-        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, EXTEND_RPL, true>), dim3(egrid), dim3(B), 0, s, scenes, jobs, w.a, w.hit,
-                           w.ctr, w.overflow, trav);
+        hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, EXTEND_RPL, true, false>), dim3(egrid), dim3(B), 0, s, scenes, jobs,
+                           w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav);
     else
-        hipLaunchKernelGGL((k_extend<B, SHORT_STACK, EXTEND_RPL, false>), dim3(egrid), dim3(B), 0, s, scenes, jobs, w.a, w.hit,
-                           w.ctr, w.overflow, trav);
+        hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, EXTEND_RPL, false, false>), dim3(egrid), dim3(B), 0, s, scenes,
+                           jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav);
     if (timing) hipEventRecord(prof->ev[1], s);
     if (use_perm) {
         const int tiles = (cap + SORT_TILE - 1) / SORT_TILE;  // NSORT_KEYS * tiles <= 1024 * SCAN_CHUNK
@@ -989,15 +985,16 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(1), 0, s, w.ctr);
     if (timing) hipEventRecord(prof->ev[4], s);
     const int sgrid = (int)((2 * (int64_t)cap + B - 1) / B);  // shadow requests <= 2 per path
+    const int agrid = (int)((2 * (int64_t)cap + EXTEND_RPL * B - 1) / (EXTEND_RPL * B));
     if (trav) {
-        hipLaunchKernelGGL((k_shadow_any<B, SHORT_STACK, true>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr, film,
-                           w.overflow, trav);
+        hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, EXTEND_RPL, true, true>), dim3(agrid), dim3(B), 0, s, scenes,
+                           jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav);
         if (p.closest_shadow)
             hipLaunchKernelGGL((k_shadow_closest<B, SHORT_STACK, true>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh,
                                w.ctr, film, w.overflow, trav, cap);
     } else {
-        hipLaunchKernelGGL((k_shadow_any<B, SHORT_STACK, false>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh, w.ctr, film,
-                           w.overflow, trav);
+        hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, EXTEND_RPL, false, true>), dim3(agrid), dim3(B), 0, s, scenes,
+                           jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav);
         if (p.closest_shadow)
             hipLaunchKernelGGL((k_shadow_closest<B, SHORT_STACK, false>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh,
                                w.ctr, film, w.overflow, trav, cap);
