@@ -392,7 +392,9 @@ DEV uint32_t visit2(const float4* nodes, uint32_t cur, const PRay& r, float tmin
     return st.sp == 0 ? TRAVERSAL_DONE : (uint32_t)st.pop();
 }
 
-template <int SHORT>
+// ORDERED false (any-hit queries): the entered children are taken in slot
+// order, without the sorting network.
+template <int SHORT, bool ORDERED = true>
 DEV uint32_t visit4(const float4* nodes, uint32_t cur, const PRay& r, float tmin, float tmax, TraversalStack<SHORT>& st) {
     const size_t nb = 8 * (size_t)cur;
     const float4 lx = ld4g(nodes, nb), hx = ld4g(nodes, nb + 1), ly = ld4g(nodes, nb + 2), hy = ld4g(nodes, nb + 3);
@@ -403,6 +405,7 @@ DEV uint32_t visit4(const float4* nodes, uint32_t cur, const PRay& r, float tmin
     float key[4];
     uint32_t ref[4];
     int hits = 0;
+    bool entered[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         const float ax = L[0][c] * r.inv.x - r.oinv.x, bx = H[0][c] * r.inv.x - r.oinv.x;
@@ -414,6 +417,7 @@ DEV uint32_t visit4(const float4* nodes, uint32_t cur, const PRay& r, float tmin
         const bool h = tn <= tf * 1.00000048f && R[c] != 0xffffffffu;
         key[c] = h ? tn : INFINITY;
         ref[c] = R[c];
+        entered[c] = h;
         hits += h ? 1 : 0;
     }
     // sort the four (key, ref) pairs by key: misses (infinite keys) go last
@@ -424,6 +428,18 @@ DEV uint32_t visit4(const float4* nodes, uint32_t cur, const PRay& r, float tmin
         key[a] = sw ? kb : ka; key[b] = sw ? ka : kb;
         ref[a] = sw ? rb : ra; ref[b] = sw ? ra : rb;
     };
+    if (!ORDERED) {
+        // any-hit: first entered child next, the others pushed as they come
+        uint32_t next = TRAVERSAL_DONE;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            if (!entered[c]) continue;
+            if (next == TRAVERSAL_DONE) next = ref[c];
+            else st.push((int)ref[c]);
+        }
+        if (next != TRAVERSAL_DONE) return next;
+        return st.sp == 0 ? TRAVERSAL_DONE : (uint32_t)st.pop();
+    }
     ce(0, 1); ce(2, 3); ce(0, 2); ce(1, 3); ce(1, 2);
     if (hits == 0) return st.sp == 0 ? TRAVERSAL_DONE : (uint32_t)st.pop();
     if (hits >= 4) st.push((int)ref[3]);
@@ -432,10 +448,10 @@ DEV uint32_t visit4(const float4* nodes, uint32_t cur, const PRay& r, float tmin
     return ref[0];
 }
 
-template <int SHORT>
+template <int SHORT, bool ORDERED = true>
 DEV uint32_t visit_node(const float4* nodes, uint32_t cur, const PRay& r, float tmin, float tmax,
                         TraversalStack<SHORT>& st) {
-    if (BVH_WIDTH == 4) return visit4<SHORT>(nodes, cur, r, tmin, tmax, st);
+    if (BVH_WIDTH == 4) return visit4<SHORT, ORDERED>(nodes, cur, r, tmin, tmax, st);
     return visit2<SHORT>(nodes, cur, r, tmin, tmax, st);
 }
 
@@ -469,7 +485,7 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
     while (cur != DONE) {
         while (cur < DONE) {  // inner node
             vc.nodes++;
-            cur = visit_node<SHORT>(s.nodes, cur, r, tmin, hit.t, st);
+            cur = visit_node<SHORT, !ANY>(s.nodes, cur, r, tmin, hit.t, st);
             if ((cur & LEAF_FLAG_DEV) && leaf == 0) {
                 leaf = cur;
                 cur = st.sp == 0 ? DONE : (uint32_t)st.pop();

@@ -620,7 +620,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
         if (active && (cur != DONE || leaf != 0)) {
             while (cur < DONE) {
                 if (COUNT) vc.nodes++;
-                cur = visit_node<SHORT>(nodes, cur, r, T_MIN, tbest, st);
+                cur = visit_node<SHORT, !SHADOW>(nodes, cur, r, T_MIN, tbest, st);
                 if ((cur & LEAF_FLAG_DEV) && leaf == 0) {
                     leaf = cur;
                     cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
