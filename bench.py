@@ -347,10 +347,11 @@ def main():
         launches = max(1, tot["ext_launch"])
         kern = {
             "k_extend": {"seconds": tot["ext_s"], "rays": tot["ext_rays"], "bytes_per_ray": bytes_per(nodes_per_ray, tris_per_ray),
-                         "nodes_per_ray": nodes_per_ray, "tris_per_ray": tris_per_ray, "what": "BVH closest hit"},
+                         "nodes_per_ray": nodes_per_ray, "tris_per_ray": tris_per_ray,
+                         "what": "k_trace_queue on the extension rays: BVH closest hit"},
             "k_shadow": {"seconds": tot["sh_s"], "rays": tot["sh_rays"], "bytes_per_ray": bytes_per(sh_nodes_per_ray, sh_tris_per_ray),
                          "nodes_per_ray": sh_nodes_per_ray, "tris_per_ray": sh_tris_per_ray,
-                         "what": "NEE shadow / MIS rays (any and closest hit)"},
+                         "what": "k_trace_queue on the any-hit shadow queue + k_shadow_closest"},
         }
         for k in kern.values():
             k["achieved"] = k["bytes_per_ray"] * k["rays"] / max(k["seconds"], 1e-12) / 1e9
@@ -361,7 +362,7 @@ def main():
         stages["k_shade"] = {"ms_per_launch": round(tot["shade_s"] / launches * 1e3, 3)}
         dom = max(kern, key=lambda n: kern[n]["seconds"])
         k = kern[dom]
-        traffic = hbm_traffic(dom, args)
+        traffic = hbm_traffic({"k_extend": "k_extend", "k_shadow": "k_shadow_any"}[dom], args)
         roofline = {"bound": "hbm", "achieved": round(k["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(k["achieved"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel": f"{dom} ({k['what']})",
