@@ -83,6 +83,11 @@ struct DevPerez {
     float sun_color[3];
     float theta;                 // sun zenith angle
     float radius;
+    // per coefficient set (luminance, x, y) in single precision: the five
+    // Perez coefficients and the denominator F(0, theta_s), which depends
+    // on the sun only
+    float coef_lum[5], coef_x[5], coef_y[5];
+    float den_lum, den_x, den_y;
     float s0[3], s1[3], s2[3];   // daylight basis in XYZ
     float xyz_to_rgb[9];         // sRGB, row major
     float world_to_object[9];    // rotation, row major

@@ -181,6 +181,16 @@ void perez_build(DevPerez& p, float turbidity, const float sun_direction_world[3
     std::memcpy(p.perez_lum, lum, sizeof(lum));
     std::memcpy(p.perez_x, px, sizeof(px));
     std::memcpy(p.perez_y, py, sizeof(py));
+    // single-precision copies for the device and the sun-only denominator
+    // (1 + A e^B)(1 + C e^(D theta_s) + E cos^2 theta_s) of each set
+    auto den = [&](const double* l, float* c) {
+        for (int k = 0; k < 5; k++) c[k] = (float)l[k];
+        const double cs = std::cos((double)p.theta);
+        return (float)((1.0 + l[0] * std::exp(l[1])) * (1.0 + l[2] * std::exp(l[3] * (double)p.theta) + l[4] * cs * cs));
+    };
+    p.den_lum = den(lum, p.coef_lum);
+    p.den_x = den(px, p.coef_x);
+    p.den_y = den(py, p.coef_y);
     to_xyz([&](float lam) { return regular_sample(kS0, 54, 300.0f, 830.0f, lam); }, p.s0);
     to_xyz([&](float lam) { return regular_sample(kS1, 54, 300.0f, 830.0f, lam); }, p.s1);
     to_xyz([&](float lam) { return regular_sample(kS2, 54, 300.0f, 830.0f, lam); }, p.s2);
