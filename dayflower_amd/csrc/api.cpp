@@ -238,6 +238,20 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
             vnrm[v] = f4((float)(acc[3 * v] / l), (float)(acc[3 * v + 1] / l), (float)(acc[3 * v + 2] / l), 0.0f);
         }
     }
+    // shading records (one 128-byte line per triangle, original order)
+    std::vector<float4> tshade(8 * nt, f4(0.0f, 0.0f, 0.0f, 0.0f));
+    for (int64_t t = 0; t < nt; t++) {
+        const int4 ix = tri_idx[t];
+        const float4 a = vpos[ix.x], b = vpos[ix.y], c = vpos[ix.z];
+        const float4 na = vnrm[ix.x], nb = vnrm[ix.y], nc = vnrm[ix.z], ng = tri_ng[t];
+        float4* r = &tshade[8 * t];
+        r[0] = f4(a.x, a.y, a.z, na.x);
+        r[1] = f4(b.x, b.y, b.z, na.y);
+        r[2] = f4(c.x, c.y, c.z, na.z);
+        r[3] = f4(nb.x, nb.y, nb.z, ng.x);
+        r[4] = f4(nc.x, nc.y, nc.z, ng.y);
+        r[5] = f4(ng.z, 0.0f, 0.0f, 0.0f);
+    }
     for (int64_t i = 0; i < nt; i++) {
         uint32_t t = bvh.tri_order[i];
         const int32_t* ix = &in.indices[3 * (int64_t)t];
@@ -307,10 +321,7 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
     int rc;
     if ((rc = BVH_WIDTH == 4 ? upload(out.nodes, bvh.nodes4, s) : upload(out.nodes, bvh.nodes, s))) return rc;
     if ((rc = upload(out.tris, tris, s))) return rc;
-    if ((rc = upload(out.tri_idx, tri_idx, s))) return rc;
-    if ((rc = upload(out.vpos, vpos, s))) return rc;
-    if ((rc = upload(out.vnrm, vnrm, s))) return rc;
-    if ((rc = upload(out.tri_ng, tri_ng, s))) return rc;
+    if ((rc = upload(out.tshade, tshade, s))) return rc;
     if ((rc = upload(out.prims, prims, s))) return rc;
     if ((rc = upload(out.analytic, analytic, s))) return rc;
     if ((rc = upload(out.mats, mats, s))) return rc;
@@ -320,10 +331,7 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
     DevScene& ds = out.dev;
     ds.nodes = (const float4*)out.nodes.ptr;
     ds.tris = (const float4*)out.tris.ptr;
-    ds.tri_idx = (const int4*)out.tri_idx.ptr;
-    ds.vpos = (const float4*)out.vpos.ptr;
-    ds.vnrm = (const float4*)out.vnrm.ptr;
-    ds.tri_ng = (const float4*)out.tri_ng.ptr;
+    ds.tshade = (const float4*)out.tshade.ptr;
     ds.prims = (const DevPrim*)out.prims.ptr;
     ds.analytic = (const int32_t*)out.analytic.ptr;
     ds.mats = (const DevMaterial*)out.mats.ptr;
@@ -336,8 +344,8 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
     ds.has_bvh = nt > 0 ? 1 : 0;
     ds.light_pdf_mode = light_pdf_mode;
     std::memcpy(out.camera, in.camera, sizeof(out.camera));
-    out.bytes = out.nodes.bytes + out.tris.bytes + out.tri_idx.bytes + out.vpos.bytes + out.vnrm.bytes +
-                out.tri_ng.bytes + out.prims.bytes + out.analytic.bytes + out.mats.bytes + out.lobes.bytes +
+    out.bytes = out.nodes.bytes + out.tris.bytes +
+                out.tshade.bytes + out.prims.bytes + out.analytic.bytes + out.mats.bytes + out.lobes.bytes +
                 out.lights.bytes + out.perez.bytes;
     if (hipStreamSynchronize(s) != hipSuccess) { err = "upload failed"; return DFR_E_DEVICE; }
     return DFR_OK;

@@ -96,10 +96,11 @@ constexpr uint32_t DIM_BOUNCE = 2, DIMS_PER_BOUNCE = 12;
 struct DevScene {
     const float4* nodes;      // 4 float4 per two-wide node
     const float4* tris;       // 3 float4 per triangle (BVH order); tris[3i].w = original id (int bits)
-    const int4* tri_idx;      // original order: vertices a, b, c and primitive
-    const float4* vpos;       // per vertex
-    const float4* vnrm;       // per vertex shading normal
-    const float4* tri_ng;     // per triangle geometric normal
+    // per triangle (original order), the shading record of one 128-byte line:
+    // (a, nA.x) (b, nA.y) (c, nA.z) (nB, ng.x) (nC, ng.y) (ng.z, -, -, -), two
+    // float4 of padding; a, b, c positions, nA..nC vertex normals, ng the
+    // geometric normal
+    const float4* tshade;
     const DevPrim* prims;
     const int32_t* analytic;  // indices of non-mesh primitives
     const DevMaterial* mats;
@@ -515,8 +516,8 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
     if (REFINE && !ANY && found && hit.tri >= 0) {
         // The float watertight t loses digits on small, oblique triangles;
         // the winner's distance to its plane is recomputed in FP64.
-        int4 ix = s.tri_idx[hit.tri];
-        float4 a = s.vpos[ix.x], b = s.vpos[ix.y], c = s.vpos[ix.z];
+        const size_t rb = 8 * (size_t)hit.tri;
+        float4 a = ld4g(s.tshade, rb), b = ld4g(s.tshade, rb + 1), c = ld4g(s.tshade, rb + 2);
         double e1x = (double)b.x - a.x, e1y = (double)b.y - a.y, e1z = (double)b.z - a.z;
         double e2x = (double)c.x - a.x, e2y = (double)c.y - a.y, e2z = (double)c.z - a.z;
         double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
@@ -567,15 +568,17 @@ DEV Surf make_surface(const DevScene& sc, V o, V d, const HitRec& h) {
     s.p = o + d * h.t;
     const DevPrim& p = sc.prims[h.prim];
     if (h.tri >= 0) {
-        int4 ix = sc.tri_idx[h.tri];
-        V a = f4v(sc.vpos[ix.x]), b = f4v(sc.vpos[ix.y]), c = f4v(sc.vpos[ix.z]);
+        const size_t rb = 8 * (size_t)h.tri;
+        const float4 r0 = ld4g(sc.tshade, rb), r1 = ld4g(sc.tshade, rb + 1), r2 = ld4g(sc.tshade, rb + 2);
+        const float4 r3 = ld4g(sc.tshade, rb + 3), r4 = ld4g(sc.tshade, rb + 4), r5 = ld4g(sc.tshade, rb + 5);
+        V a = f4v(r0), b = f4v(r1), c = f4v(r2);
         V eab = b - a, eca = a - c;
         V d1 = cross(eab, eca);
         float rdet = 1.0f / dot(d, d1);
         V d3 = cross(a - o, d);
         float u = dot(d3, eca) * rdet, v = dot(d3, eab) * rdet, w = 1.0f - u - v;
-        s.g = onb_w(f4v(sc.tri_ng[h.tri]));
-        V ns = nrm(f4v(sc.vnrm[ix.x]) * w + f4v(sc.vnrm[ix.y]) * u + f4v(sc.vnrm[ix.z]) * v);
+        s.g = onb_w(mk(r3.w, r4.w, r5.x));
+        V ns = nrm(mk(r0.w, r1.w, r2.w) * w + f4v(r3) * u + f4v(r4) * v);
         s.s = onb_w(ns);  // texture coordinates are absent: UV determinant 0
     } else if (p.shape == DFR_SHAPE_SPHERE) {
         V c = ld3(p.data);
@@ -1055,12 +1058,14 @@ DEV bool sample_light(const DevScene& sc, const DevLight& L, const Surf& si, flo
     }
     if (L.type == DFR_LIGHT_PEREZ) return perez_sample(sc.perez[L.perez], si, u, v, usun, out);
     if (L.tri < 0) return false;
-    int4 ix = sc.tri_idx[L.tri];
-    V a = f4v(sc.vpos[ix.x]), b = f4v(sc.vpos[ix.y]), c = f4v(sc.vpos[ix.z]);
+    const size_t rb = 8 * (size_t)L.tri;
+    const float4 r0 = ld4g(sc.tshade, rb), r1 = ld4g(sc.tshade, rb + 1), r2 = ld4g(sc.tshade, rb + 2);
+    const float4 r3 = ld4g(sc.tshade, rb + 3), r4 = ld4g(sc.tshade, rb + 4);
+    V a = f4v(r0), b = f4v(r1), c = f4v(r2);
     float su = sqrtf(u);
     float b0 = 1.0f - su, b1 = v * su, b2 = 1.0f - b0 - b1;
     V p = a * b0 + b * b1 + c * b2;
-    V n = nrm(f4v(sc.vnrm[ix.x]) * b0 + f4v(sc.vnrm[ix.y]) * b1 + f4v(sc.vnrm[ix.z]) * b2);
+    V n = nrm(mk(r0.w, r1.w, r2.w) * b0 + f4v(r3) * b1 + f4v(r4) * b2);
     V inc = p - si.p;
     if (dot(inc, inc) == 0.0f) return false;
     V incn = nrm(inc);
@@ -1079,8 +1084,8 @@ DEV bool sample_light(const DevScene& sc, const DevLight& L, const Surf& si, flo
 DEV float light_pdf(const DevScene& sc, const DevLight& L, const Surf& si, V wi) {
     if (L.type == DFR_LIGHT_PEREZ) return perez_pdf(sc.perez[L.perez], wi);
     if (L.type != DFR_LIGHT_DIFFUSE_AREA || L.tri < 0) return 0.0f;
-    int4 ix = sc.tri_idx[L.tri];
-    V a = f4v(sc.vpos[ix.x]), b = f4v(sc.vpos[ix.y]), c = f4v(sc.vpos[ix.z]);
+    const size_t rb = 8 * (size_t)L.tri;
+    V a = f4v(ld4g(sc.tshade, rb)), b = f4v(ld4g(sc.tshade, rb + 1)), c = f4v(ld4g(sc.tshade, rb + 2));
     V d = nrm(wi);
     PRay r = prep_ray(si.p, d);
     float t = tri_t(a, b, c, r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, T_MIN, FLT_MAX);
