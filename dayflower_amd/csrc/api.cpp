@@ -395,6 +395,15 @@ int Ctx::sync_scene_table() {
     return DFR_OK;
 }
 
+// The device record of the scene all uploaded jobs share, when its node and
+// triangle arrays are addressable with 32-bit byte offsets; else null.
+const DevScene* Ctx::uniform_scene() const {
+    if (jobs_scene < 0 || jobs_scene >= (int)scenes.size() || !scenes[jobs_scene]) return nullptr;
+    const CompiledScene& s = *scenes[jobs_scene];
+    if (s.nodes.bytes >= (size_t(1) << 32) || s.tris.bytes >= (size_t(1) << 32)) return nullptr;
+    return (const DevScene*)scene_table.ptr + jobs_scene;
+}
+
 bool Ctx::any_area_lights() const {
     for (auto& s : scenes) if (s && s->area_lights) return true;
     return false;
@@ -484,7 +493,7 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
     if ((rc = ensure_overflow(threads))) return rc;
     wf.overflow = (int*)overflow.ptr;
     wavefront_init(wf, (const DevJob*)job_buf.ptr, n_jobs, work_total, stream);
-    WavefrontParams p{wf_params.block, wf_params.sort_keys, (capacity + 255) / 256, any_area_lights()};
+    WavefrontParams p{wf_params.block, wf_params.sort_keys, (capacity + 255) / 256, any_area_lights(), uniform_scene()};
     Counters host{};
     const int poll = 4;
     const int max_iterations = 1 << 20;  // far above max_bounce * (work / capacity + 1)
@@ -505,7 +514,7 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
             (uintptr_t)wf.b.beta, (uintptr_t)wf.b.info, (uintptr_t)wf.hit, (uintptr_t)wf.perm, (uintptr_t)wf.keys,
             (uintptr_t)wf.bhist, (uintptr_t)wf.sh.o, (uintptr_t)wf.sh.d, (uintptr_t)wf.sh.c, (uintptr_t)wf.sh.aux,
             (uintptr_t)wf.overflow, (uintptr_t)wf.ctr, (uintptr_t)wf.job_done, (uintptr_t)wf.capacity,
-            (uintptr_t)p.block, (uintptr_t)p.sort_keys, (uintptr_t)p.closest_shadow};
+            (uintptr_t)p.block, (uintptr_t)p.sort_keys, (uintptr_t)p.closest_shadow, (uintptr_t)p.uscene};
         for (int it = 0;; it += poll) {
             if (it >= max_iterations) return fail(DFR_E_DEVICE, "wavefront did not drain");
             if ((rc = replay_iterations(key, scenes, jobs, n_jobs, film_ptr, p, poll))) return rc;
@@ -569,6 +578,9 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
 }
 
 int Ctx::upload_jobs(const std::vector<DevJob>& jobs) {
+    jobs_scene = jobs.empty() ? -1 : jobs[0].scene;
+    for (const DevJob& j : jobs)
+        if (j.scene != jobs_scene) jobs_scene = -1;
     if (job_buf.alloc(std::max<size_t>(1, jobs.size()) * sizeof(DevJob)) != hipSuccess) return fail(DFR_E_MEMORY, "job table");
     hipError_t e = hipMemcpyAsync(job_buf.ptr, jobs.data(), jobs.size() * sizeof(DevJob), hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return hip_fail(e, "job upload");

@@ -130,6 +130,8 @@ struct Ctx {
     int ensure_film(int64_t pixels);
     int ensure_wavefront(int capacity);
     int upload_jobs(const std::vector<DevJob>& jobs);
+    int jobs_scene = -1;  // the scene every job of the last upload_jobs uses, -1 if they differ
+    const DevScene* uniform_scene() const;
     int run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counters* out);
     DevJob main_job(int spp, int sample_base) const;
 };

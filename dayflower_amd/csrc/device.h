@@ -393,15 +393,13 @@ DEV uint32_t visit2(const float4* nodes, uint32_t cur, const PRay& r, float tmin
     return st.sp == 0 ? TRAVERSAL_DONE : (uint32_t)st.pop();
 }
 
-// ORDERED false (any-hit queries): the entered children are taken in slot
-// order, without the sorting network.
-template <int SHORT, bool ORDERED = true>
-DEV uint32_t visit4(const float4* nodes, uint32_t cur, const PRay& r, float tmin, float tmax, TraversalStack<SHORT>& st) {
-    const size_t nb = 8 * (size_t)cur;
-    const float4 lx = ld4g(nodes, nb), hx = ld4g(nodes, nb + 1), ly = ld4g(nodes, nb + 2), hy = ld4g(nodes, nb + 3);
-    const float4 lz = ld4g(nodes, nb + 4), hz = ld4g(nodes, nb + 5), rf = ld4g(nodes, nb + 6);
-    const float L[3][4] = {{lx.x, lx.y, lx.z, lx.w}, {ly.x, ly.y, ly.z, ly.w}, {lz.x, lz.y, lz.z, lz.w}};
-    const float H[3][4] = {{hx.x, hx.y, hx.z, hx.w}, {hy.x, hy.y, hy.z, hy.w}, {hz.x, hz.y, hz.z, hz.w}};
+// Entry distances tn and widened exit distances tf of the four children of a
+// four-wide node -> the next reference (ORDERED: the nearest entered child,
+// the other entered children pushed farthest first, through a five-exchange
+// sorting network; otherwise, for any-hit queries, the entered children in
+// slot order); with none entered, the top of the stack.
+template <int SHORT, bool ORDERED>
+DEV uint32_t visit4_next(const float (&tn)[4], const float (&tf)[4], const float4 rf, TraversalStack<SHORT>& st) {
     const uint32_t R[4] = {__float_as_uint(rf.x), __float_as_uint(rf.y), __float_as_uint(rf.z), __float_as_uint(rf.w)};
     float key[4];
     uint32_t ref[4];
@@ -409,14 +407,9 @@ DEV uint32_t visit4(const float4* nodes, uint32_t cur, const PRay& r, float tmin
     bool entered[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-        const float ax = L[0][c] * r.inv.x - r.oinv.x, bx = H[0][c] * r.inv.x - r.oinv.x;
-        const float ay = L[1][c] * r.inv.y - r.oinv.y, by = H[1][c] * r.inv.y - r.oinv.y;
-        const float az = L[2][c] * r.inv.z - r.oinv.z, bz = H[2][c] * r.inv.z - r.oinv.z;
-        const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
-        const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
         // far end widened by 1 + 2*gamma(3), as in the two-wide test
-        const bool h = tn <= tf * 1.00000048f && R[c] != 0xffffffffu;
-        key[c] = h ? tn : INFINITY;
+        const bool h = tn[c] <= tf[c] * 1.00000048f && R[c] != 0xffffffffu;
+        key[c] = h ? tn[c] : INFINITY;
         ref[c] = R[c];
         entered[c] = h;
         hits += h ? 1 : 0;
@@ -447,6 +440,73 @@ DEV uint32_t visit4(const float4* nodes, uint32_t cur, const PRay& r, float tmin
     if (hits >= 3) st.push((int)ref[2]);
     if (hits >= 2) st.push((int)ref[1]);
     return ref[0];
+}
+
+// One step of a four-wide node: both planes of every slab, ordered by min/max.
+template <int SHORT, bool ORDERED = true>
+DEV uint32_t visit4(const float4* nodes, uint32_t cur, const PRay& r, float tmin, float tmax, TraversalStack<SHORT>& st) {
+    const size_t nb = 8 * (size_t)cur;
+    const float4 lx = ld4g(nodes, nb), hx = ld4g(nodes, nb + 1), ly = ld4g(nodes, nb + 2), hy = ld4g(nodes, nb + 3);
+    const float4 lz = ld4g(nodes, nb + 4), hz = ld4g(nodes, nb + 5), rf = ld4g(nodes, nb + 6);
+    const float L[3][4] = {{lx.x, lx.y, lx.z, lx.w}, {ly.x, ly.y, ly.z, ly.w}, {lz.x, lz.y, lz.z, lz.w}};
+    const float H[3][4] = {{hx.x, hx.y, hx.z, hx.w}, {hy.x, hy.y, hy.z, hy.w}, {hz.x, hz.y, hz.z, hz.w}};
+    float tn[4], tf[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const float ax = L[0][c] * r.inv.x - r.oinv.x, bx = H[0][c] * r.inv.x - r.oinv.x;
+        const float ay = L[1][c] * r.inv.y - r.oinv.y, by = H[1][c] * r.inv.y - r.oinv.y;
+        const float az = L[2][c] * r.inv.z - r.oinv.z, bz = H[2][c] * r.inv.z - r.oinv.z;
+        tn[c] = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
+        tf[c] = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
+    }
+    return visit4_next<SHORT, ORDERED>(tn, tf, rf, st);
+}
+
+// Near/far plane selection of a ray for visit4_signed: per axis, the byte
+// offset (0 or 16) of the entry plane within the axis' (lo, hi) pair of a
+// node.  A ray enters a slab through lo where its inverse direction is
+// positive (prep_ray never leaves a zero component), so the entry distance is
+// the lo plane's and the exit distance the hi plane's, the same values
+// min/max would pick: fma is monotonic in its first operand for a fixed
+// positive factor.
+struct PlaneSel { uint32_t x, y, z; };
+DEV PlaneSel plane_sel(const PRay& r) {
+    PlaneSel p;
+    p.x = r.inv.x < 0.0f ? 16u : 0u;
+    p.y = r.inv.y < 0.0f ? 16u : 0u;
+    p.z = r.inv.z < 0.0f ? 16u : 0u;
+    return p;
+}
+// a float4 at a byte offset of a buffer whose base is the same for the whole
+// wave: the load takes the base from scalar registers and a 32-bit offset
+DEV float4 ld4off(const float4* base, uint32_t byte_off, uint32_t imm) {
+    const native_float4 v = *(const AS_GLOBAL native_float4*)((const AS_GLOBAL char*)base + byte_off + imm);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// visit4 for a node array shared by the whole wave (byte offsets below 4 GiB):
+// the entry and exit planes are loaded directly (no min/max per slab).
+template <int SHORT, bool ORDERED = true>
+DEV uint32_t visit4_signed(const float4* nodes, uint32_t cur, const PRay& r, const PlaneSel& ps, float tmin, float tmax,
+                           TraversalStack<SHORT>& st) {
+    const uint32_t nb = cur << 7;  // 128 bytes per node
+    const uint32_t ox = nb | ps.x, oy = nb | ps.y, oz = nb | ps.z;
+    const float4 nx = ld4off(nodes, ox, 0), fx = ld4off(nodes, ox ^ 16u, 0);
+    const float4 ny = ld4off(nodes, oy, 32), fy = ld4off(nodes, oy ^ 16u, 32);
+    const float4 nz = ld4off(nodes, oz, 64), fz = ld4off(nodes, oz ^ 16u, 64);
+    const float4 rf = ld4off(nodes, nb, 96);
+    const float N[3][4] = {{nx.x, nx.y, nx.z, nx.w}, {ny.x, ny.y, ny.z, ny.w}, {nz.x, nz.y, nz.z, nz.w}};
+    const float F[3][4] = {{fx.x, fx.y, fx.z, fx.w}, {fy.x, fy.y, fy.z, fy.w}, {fz.x, fz.y, fz.z, fz.w}};
+    float tn[4], tf[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const float ax = N[0][c] * r.inv.x - r.oinv.x, bx = F[0][c] * r.inv.x - r.oinv.x;
+        const float ay = N[1][c] * r.inv.y - r.oinv.y, by = F[1][c] * r.inv.y - r.oinv.y;
+        const float az = N[2][c] * r.inv.z - r.oinv.z, bz = F[2][c] * r.inv.z - r.oinv.z;
+        tn[c] = fmaxf(fmaxf(ax, ay), fmaxf(az, tmin));
+        tf[c] = fminf(fminf(bx, by), fminf(bz, tmax));
+    }
+    return visit4_next<SHORT, ORDERED>(tn, tf, rf, st);
 }
 
 template <int SHORT, bool ORDERED = true>

@@ -97,6 +97,9 @@ struct WavefrontParams {
     int sort_keys;      // 0: unsorted; 1: shade in (material, light type) order
     int grid_cap;       // blocks of the queue kernels (capacity / block)
     bool closest_shadow;  // some scene has area lights: launch the closest-hit shadow kernel
+    // the scene record every job of the run uses (node and triangle arrays
+    // below 4 GiB), or null: selects the scene-uniform traversal kernels
+    const DevScene* uscene = nullptr;
 };
 
 // Optional instrumentation of one wavefront iteration: HIP events around the

@@ -559,12 +559,18 @@ constexpr int EXTEND_RPL = 4;  // queue entries per thread of a traversal block
 
 // At most 72 VGPRs (7 waves per SIMD): measured faster than 6 waves with no
 // register limit and than 8 waves with spills.
-template <int BLOCK, int SHORT, int RPL, bool COUNT, bool SHADOW>
+// UNI: every job of the launch uses the scene `uscene` (node and triangle
+// arrays below 4 GiB): the arrays' bases stay in scalar registers, and the
+// nodes are tested with the entry/exit planes picked by the ray's direction
+// signs (visit4_signed).  Otherwise each ray reaches its arrays through its
+// job's scene.
+template <int BLOCK, int SHORT, int RPL, bool COUNT, bool SHADOW, bool UNI>
 __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __restrict__ scenes,
                                                          const DevJob* __restrict__ jobs, PathSoA q, ShadowSoA sh,
                                                          float4* __restrict__ hits, float4* __restrict__ film,
                                                          const Counters* __restrict__ ctr, int* __restrict__ overflow,
-                                                         unsigned long long* __restrict__ trav) {
+                                                         unsigned long long* __restrict__ trav,
+                                                         const DevScene* __restrict__ uscene) {
     __shared__ int lds[SHORT * BLOCK];
     __shared__ int next;
     constexpr uint32_t DONE = TRAVERSAL_DONE;
@@ -585,8 +591,9 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
     float tbest = FLT_MAX;
     int prim = -1, tri = -1;
     uint32_t cur = DONE, leaf = 0;
-    const float4* nodes = nullptr;
-    const float4* tris = nullptr;
+    const float4* nodes = UNI ? uscene->nodes : nullptr;
+    const float4* tris = UNI ? uscene->tris : nullptr;
+    PlaneSel ps{0u, 0u, 0u};
     bool fresh = active;  // a ray to set up at the top of the loop
     for (;;) {
         if (fresh) {
@@ -603,7 +610,8 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                     const float t = analytic_t(sc.prims[pi], o, d, T_MIN, tbest);
                     if (t != 0.0f && t < tbest) { tbest = t; prim = pi; tri = -1; }
                 }
-                nodes = sc.nodes; tris = sc.tris;
+                if (!UNI) { nodes = sc.nodes; tris = sc.tris; }
+                if (UNI) ps = plane_sel(r);
                 cur = sc.has_bvh && !(SHADOW && prim >= 0) ? 0u : DONE;
             };
             if (SHADOW) {
@@ -620,7 +628,8 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
         if (active && (cur != DONE || leaf != 0)) {
             while (cur < DONE) {
                 if (COUNT) vc.nodes++;
-                cur = visit_node<SHORT, !SHADOW>(nodes, cur, r, T_MIN, tbest, st);
+                cur = UNI ? visit4_signed<SHORT, !SHADOW>(nodes, cur, r, ps, T_MIN, tbest, st)
+                          : visit_node<SHORT, !SHADOW>(nodes, cur, r, T_MIN, tbest, st);
                 if ((cur & LEAF_FLAG_DEV) && leaf == 0) {
                     leaf = cur;
                     cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
@@ -632,8 +641,14 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                 bool occluded = false;
                 for (uint32_t k = 0; k < cnt; k++) {
                     if (COUNT) vc.tris++;
-                    const size_t tb = 3 * (size_t)(first + k);
-                    const float4 a = ld4g(tris, tb), b = ld4g(tris, tb + 1), c = ld4g(tris, tb + 2);
+                    float4 a, b, c;
+                    if (UNI) {
+                        const uint32_t tb = (first + k) * 48u;
+                        a = ld4off(tris, tb, 0); b = ld4off(tris, tb, 16); c = ld4off(tris, tb, 32);
+                    } else {
+                        const size_t tb = 3 * (size_t)(first + k);
+                        a = ld4g(tris, tb); b = ld4g(tris, tb + 1); c = ld4g(tris, tb + 2);
+                    }
                     const float t = tri_t(f4v(a), f4v(b), f4v(c), r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, T_MIN, tbest);
                     if (t != 0.0f && t < tbest) {
                         tbest = t; tri = __float_as_int(a.w); prim = __float_as_int(b.w);
@@ -950,6 +965,19 @@ void wavefront_init(WavefrontBuffers& w, const DevJob* jobs, int n_jobs, int64_t
     hipMemcpyAsync(w.ctr, &c, sizeof(Counters), hipMemcpyHostToDevice, s);
 }
 
+// One traversal launch over the path queue (SHADOW false) or the any-hit
+// shadow queue (SHADOW true); the scene-uniform variant when p.uscene is set.
+template <int B, bool COUNT, bool SHADOW>
+static void launch_trace_queue(int grid, const DevScene* scenes, const DevJob* jobs, float4* film, WavefrontBuffers& w,
+                               const WavefrontParams& p, unsigned long long* trav, hipStream_t s) {
+    if (BVH_WIDTH == 4 && p.uscene)
+        hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, EXTEND_RPL, COUNT, SHADOW, true>), dim3(grid), dim3(B), 0, s,
+                           scenes, jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav, p.uscene);
+    else
+        hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, EXTEND_RPL, COUNT, SHADOW, false>), dim3(grid), dim3(B), 0, s,
+                           scenes, jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav, p.uscene);
+}
+
 template <int B>
 static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, float4* film, WavefrontBuffers& w,
                         const WavefrontParams& p, const IterProfile* prof, hipStream_t s) {
@@ -964,12 +992,8 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     if (timing) hipEventRecord(prof->ev[0], s);
     unsigned long long* trav = prof ? prof->trav : nullptr;
     const int egrid = (cap + EXTEND_RPL * B - 1) / (EXTEND_RPL * B);
-    if (trav)
-        hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, EXTEND_RPL, true, false>), dim3(egrid), dim3(B), 0, s, scenes, jobs,
-                           w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav);
-    else
-        hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, EXTEND_RPL, false, false>), dim3(egrid), dim3(B), 0, s, scenes,
-                           jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav);
+    if (trav) launch_trace_queue<B, true, false>(egrid, scenes, jobs, film, w, p, trav, s);
+    else launch_trace_queue<B, false, false>(egrid, scenes, jobs, film, w, p, trav, s);
     if (timing) hipEventRecord(prof->ev[1], s);
     if (use_perm) {
         const int tiles = (cap + SORT_TILE - 1) / SORT_TILE;  // NSORT_KEYS * tiles <= 1024 * SCAN_CHUNK
@@ -987,14 +1011,12 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     const int sgrid = (int)((2 * (int64_t)cap + B - 1) / B);  // shadow requests <= 2 per path
     const int agrid = (int)((2 * (int64_t)cap + EXTEND_RPL * B - 1) / (EXTEND_RPL * B));
     if (trav) {
-        hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, EXTEND_RPL, true, true>), dim3(agrid), dim3(B), 0, s, scenes,
-                           jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav);
+        launch_trace_queue<B, true, true>(agrid, scenes, jobs, film, w, p, trav, s);
         if (p.closest_shadow)
             hipLaunchKernelGGL((k_shadow_closest<B, SHORT_STACK, true>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh,
                                w.ctr, film, w.overflow, trav, cap);
     } else {
-        hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, EXTEND_RPL, false, true>), dim3(agrid), dim3(B), 0, s, scenes,
-                           jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav);
+        launch_trace_queue<B, false, true>(agrid, scenes, jobs, film, w, p, trav, s);
         if (p.closest_shadow)
             hipLaunchKernelGGL((k_shadow_closest<B, SHORT_STACK, false>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh,
                                w.ctr, film, w.overflow, trav, cap);
