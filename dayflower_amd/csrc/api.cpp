@@ -318,6 +318,12 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
             perez.push_back(p);
         }
     }
+    // traversal addresses nodes and triangles with 32-bit byte offsets
+    const size_t node_bytes = BVH_WIDTH == 4 ? bvh.nodes4.size() * sizeof(bvh.nodes4[0]) : bvh.nodes.size() * sizeof(bvh.nodes[0]);
+    if (node_bytes >= (size_t(1) << 32) || tris.size() * sizeof(float4) >= (size_t(1) << 32)) {
+        err = "scene too large: BVH nodes and triangles must each stay below 4 GiB";
+        return DFR_E_UNSUPPORTED;
+    }
     int rc;
     if ((rc = BVH_WIDTH == 4 ? upload(out.nodes, bvh.nodes4, s) : upload(out.nodes, bvh.nodes, s))) return rc;
     if ((rc = upload(out.tris, tris, s))) return rc;
@@ -395,12 +401,9 @@ int Ctx::sync_scene_table() {
     return DFR_OK;
 }
 
-// The device record of the scene all uploaded jobs share, when its node and
-// triangle arrays are addressable with 32-bit byte offsets; else null.
+// The device record of the scene all uploaded jobs share, else null.
 const DevScene* Ctx::uniform_scene() const {
     if (jobs_scene < 0 || jobs_scene >= (int)scenes.size() || !scenes[jobs_scene]) return nullptr;
-    const CompiledScene& s = *scenes[jobs_scene];
-    if (s.nodes.bytes >= (size_t(1) << 32) || s.tris.bytes >= (size_t(1) << 32)) return nullptr;
     return (const DevScene*)scene_table.ptr + jobs_scene;
 }
 

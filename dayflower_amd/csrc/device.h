@@ -442,26 +442,6 @@ DEV uint32_t visit4_next(const float (&tn)[4], const float (&tf)[4], const float
     return ref[0];
 }
 
-// One step of a four-wide node: both planes of every slab, ordered by min/max.
-template <int SHORT, bool ORDERED = true>
-DEV uint32_t visit4(const float4* nodes, uint32_t cur, const PRay& r, float tmin, float tmax, TraversalStack<SHORT>& st) {
-    const size_t nb = 8 * (size_t)cur;
-    const float4 lx = ld4g(nodes, nb), hx = ld4g(nodes, nb + 1), ly = ld4g(nodes, nb + 2), hy = ld4g(nodes, nb + 3);
-    const float4 lz = ld4g(nodes, nb + 4), hz = ld4g(nodes, nb + 5), rf = ld4g(nodes, nb + 6);
-    const float L[3][4] = {{lx.x, lx.y, lx.z, lx.w}, {ly.x, ly.y, ly.z, ly.w}, {lz.x, lz.y, lz.z, lz.w}};
-    const float H[3][4] = {{hx.x, hx.y, hx.z, hx.w}, {hy.x, hy.y, hy.z, hy.w}, {hz.x, hz.y, hz.z, hz.w}};
-    float tn[4], tf[4];
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const float ax = L[0][c] * r.inv.x - r.oinv.x, bx = H[0][c] * r.inv.x - r.oinv.x;
-        const float ay = L[1][c] * r.inv.y - r.oinv.y, by = H[1][c] * r.inv.y - r.oinv.y;
-        const float az = L[2][c] * r.inv.z - r.oinv.z, bz = H[2][c] * r.inv.z - r.oinv.z;
-        tn[c] = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
-        tf[c] = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
-    }
-    return visit4_next<SHORT, ORDERED>(tn, tf, rf, st);
-}
-
 // Near/far plane selection of a ray for visit4_signed: per axis, the byte
 // offset (0 or 16) of the entry plane within the axis' (lo, hi) pair of a
 // node.  A ray enters a slab through lo where its inverse direction is
@@ -484,8 +464,10 @@ DEV float4 ld4off(const float4* base, uint32_t byte_off, uint32_t imm) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
-// visit4 for a node array shared by the whole wave (byte offsets below 4 GiB):
-// the entry and exit planes are loaded directly (no min/max per slab).
+// One step of a four-wide node (byte offsets below 4 GiB, which compile_scene
+// checks): the entry and exit planes of each slab are loaded directly, so no
+// min/max orders them.  With a node array shared by the wave, the loads take
+// its base from scalar registers.
 template <int SHORT, bool ORDERED = true>
 DEV uint32_t visit4_signed(const float4* nodes, uint32_t cur, const PRay& r, const PlaneSel& ps, float tmin, float tmax,
                            TraversalStack<SHORT>& st) {
@@ -510,10 +492,16 @@ DEV uint32_t visit4_signed(const float4* nodes, uint32_t cur, const PRay& r, con
 }
 
 template <int SHORT, bool ORDERED = true>
-DEV uint32_t visit_node(const float4* nodes, uint32_t cur, const PRay& r, float tmin, float tmax,
+DEV uint32_t visit_node(const float4* nodes, uint32_t cur, const PRay& r, const PlaneSel& ps, float tmin, float tmax,
                         TraversalStack<SHORT>& st) {
-    if (BVH_WIDTH == 4) return visit4<SHORT, ORDERED>(nodes, cur, r, tmin, tmax, st);
+    if (BVH_WIDTH == 4) return visit4_signed<SHORT, ORDERED>(nodes, cur, r, ps, tmin, tmax, st);
     return visit2<SHORT>(nodes, cur, r, tmin, tmax, st);
+}
+
+// Vertices of triangle i in BVH order (48 bytes each)
+DEV void load_tri(const float4* tris, uint32_t i, float4& a, float4& b, float4& c) {
+    const uint32_t tb = i * 48u;
+    a = ld4off(tris, tb, 0); b = ld4off(tris, tb, 16); c = ld4off(tris, tb, 32);
 }
 
 // With REFINE (closest hit only), the winner's distance is recomputed in
@@ -535,6 +523,7 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
     }
     if (!s.has_bvh) return found;
     PRay r = prep_ray(o, d);
+    const PlaneSel ps = plane_sel(r);
     st.sp = 0;
     // Speculative while-while traversal: a lane that reaches a leaf parks it
     // and keeps descending inner nodes until every lane of the wave has a
@@ -546,7 +535,7 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
     while (cur != DONE) {
         while (cur < DONE) {  // inner node
             vc.nodes++;
-            cur = visit_node<SHORT, !ANY>(s.nodes, cur, r, tmin, hit.t, st);
+            cur = visit_node<SHORT, !ANY>(s.nodes, cur, r, ps, tmin, hit.t, st);
             if ((cur & LEAF_FLAG_DEV) && leaf == 0) {
                 leaf = cur;
                 cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
@@ -557,8 +546,8 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
             uint32_t first = (leaf & 0x7fffffffu) >> 3, cnt = (leaf & 7u) + 1;
             for (uint32_t k = 0; k < cnt; k++) {
                 vc.tris++;
-                const size_t tb = 3 * (size_t)(first + k);
-                float4 a = ld4g(s.tris, tb), b = ld4g(s.tris, tb + 1), c = ld4g(s.tris, tb + 2);
+                float4 a, b, c;
+                load_tri(s.tris, first + k, a, b, c);
                 float t = tri_t(f4v(a), f4v(b), f4v(c), r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, tmin, hit.t);
                 if (t != 0.0f && t < hit.t) {
                     hit.t = t; hit.tri = __float_as_int(a.w); hit.prim = __float_as_int(b.w); found = true;

@@ -559,11 +559,9 @@ constexpr int EXTEND_RPL = 4;  // queue entries per thread of a traversal block
 
 // At most 72 VGPRs (7 waves per SIMD): measured faster than 6 waves with no
 // register limit and than 8 waves with spills.
-// UNI: every job of the launch uses the scene `uscene` (node and triangle
-// arrays below 4 GiB): the arrays' bases stay in scalar registers, and the
-// nodes are tested with the entry/exit planes picked by the ray's direction
-// signs (visit4_signed).  Otherwise each ray reaches its arrays through its
-// job's scene.
+// UNI: every job of the launch uses the scene `uscene`, whose node and
+// triangle arrays' bases then stay in scalar registers; otherwise each ray
+// reaches its arrays through its job's scene.
 template <int BLOCK, int SHORT, int RPL, bool COUNT, bool SHADOW, bool UNI>
 __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __restrict__ scenes,
                                                          const DevJob* __restrict__ jobs, PathSoA q, ShadowSoA sh,
@@ -611,7 +609,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                     if (t != 0.0f && t < tbest) { tbest = t; prim = pi; tri = -1; }
                 }
                 if (!UNI) { nodes = sc.nodes; tris = sc.tris; }
-                if (UNI) ps = plane_sel(r);
+                ps = plane_sel(r);
                 cur = sc.has_bvh && !(SHADOW && prim >= 0) ? 0u : DONE;
             };
             if (SHADOW) {
@@ -628,8 +626,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
         if (active && (cur != DONE || leaf != 0)) {
             while (cur < DONE) {
                 if (COUNT) vc.nodes++;
-                cur = UNI ? visit4_signed<SHORT, !SHADOW>(nodes, cur, r, ps, T_MIN, tbest, st)
-                          : visit_node<SHORT, !SHADOW>(nodes, cur, r, T_MIN, tbest, st);
+                cur = visit_node<SHORT, !SHADOW>(nodes, cur, r, ps, T_MIN, tbest, st);
                 if ((cur & LEAF_FLAG_DEV) && leaf == 0) {
                     leaf = cur;
                     cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
@@ -642,13 +639,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                 for (uint32_t k = 0; k < cnt; k++) {
                     if (COUNT) vc.tris++;
                     float4 a, b, c;
-                    if (UNI) {
-                        const uint32_t tb = (first + k) * 48u;
-                        a = ld4off(tris, tb, 0); b = ld4off(tris, tb, 16); c = ld4off(tris, tb, 32);
-                    } else {
-                        const size_t tb = 3 * (size_t)(first + k);
-                        a = ld4g(tris, tb); b = ld4g(tris, tb + 1); c = ld4g(tris, tb + 2);
-                    }
+                    load_tri(tris, first + k, a, b, c);
                     const float t = tri_t(f4v(a), f4v(b), f4v(c), r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, T_MIN, tbest);
                     if (t != 0.0f && t < tbest) {
                         tbest = t; tri = __float_as_int(a.w); prim = __float_as_int(b.w);
