@@ -219,6 +219,7 @@ struct Server {
     Arena arena;
     int job_capacity = 1 << 16;
     int n_slots = 0;
+    int slots_scene = -2;  // the scene of every admitted request (-2: none yet, -1: more than one)
     int64_t work_total = 0;
     int64_t arena_pixels = 0;
     dfr_server_stats stats{};
@@ -262,6 +263,7 @@ static void server_loop(Server* S) {
                 r.film_off = off;
                 r.slot = S->n_slots++;
                 r.state = 1;
+                S->slots_scene = S->slots_scene == -2 || S->slots_scene == r.job.scene ? r.job.scene : -1;
                 DevJob dj = to_dev(r.job, off, S->work_total);
                 S->work_total += r.work;
                 hipMemcpyAsync((DevJob*)S->jobs_mem.ptr + r.slot, &dj, sizeof(DevJob), hipMemcpyHostToDevice, c->stream);
@@ -283,8 +285,10 @@ static void server_loop(Server* S) {
             }
             if (!S->busy) { S->busy = true; S->t_busy0 = std::chrono::steady_clock::now(); }
         }
-        // publish the new work total, run iterations
+        // publish the new work total, run iterations (scene-uniform traversal
+        // while every admitted request uses one scene)
         hipMemcpyAsync(&c->wf.ctr->work_total, &S->work_total, sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
+        p.uscene = S->slots_scene >= 0 ? (const DevScene*)c->scene_table.ptr + S->slots_scene : nullptr;
         for (int k = 0; k < 2; k++)
             wavefront_iteration((const DevScene*)c->scene_table.ptr, (const DevJob*)S->jobs_mem.ptr, S->n_slots,
                                 (float4*)S->arena_mem.ptr, c->wf, p, nullptr, c->stream);
