@@ -65,9 +65,9 @@ static_assert(sizeof(Counters) == 64, "Counters is one 64-byte line");
 // the HBM overflow stacks are sized for it.
 inline int64_t wavefront_stack_threads(int capacity) { return 2 * (int64_t)capacity + 64 + 256; }
 
-// Largest path pool: the material sort's one-block scan holds 64 histogram
-// entries per thread (32 keys x capacity / 4096 tiles <= 1024 x 64).
-constexpr int MAX_POOL_PATHS = 1 << 23;
+// Largest path pool: the material sort's one-block scan holds 128 histogram
+// entries per thread (32 keys x capacity / 8192-entry tiles <= 1024 x 128).
+constexpr int MAX_POOL_PATHS = 1 << 25;
 
 struct WavefrontBuffers {
     PathSoA a, b;

@@ -689,7 +689,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
 // exclusive scan of the key-major histogram array, then a scatter in which
 // every wave ranks its entries per key with ballots and takes its base with
 // one LDS atomic per key.
-constexpr int SORT_TILE = 4096;
+constexpr int SORT_TILE = 8192;
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sort_hist(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
@@ -729,7 +729,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_hist(const DevScene* __restrict_
 // thread loads a contiguous chunk with int4 loads into registers, the chunk
 // sums are scanned with wave shuffles and one pass over the 16 wave totals,
 // then each thread writes its chunk's prefixes.
-constexpr int SCAN_CHUNK = 64;
+constexpr int SCAN_CHUNK = 128;
 
 __global__ __launch_bounds__(1024) void k_sort_scan(int32_t* __restrict__ bhist, int total) {
     __shared__ int wsum[16];
