@@ -138,7 +138,7 @@ def run_serving(args):
     log(f"serving: {len(stream)} requests over {args.duration} s, {total / 1e9:.2f} G samples")
     out = {}
     # continuous batching server
-    srv = Server(r, hbm_budget_bytes=64 << 30, max_active=256, pool_paths=1 << 22)
+    srv = Server(r, hbm_budget_bytes=64 << 30, max_active=256, pool_paths=args.pool or (1 << 24))
     t0 = time.perf_counter()
     ids = []
     for at, job in stream:

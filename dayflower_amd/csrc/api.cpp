@@ -705,7 +705,7 @@ int32_t dfr_set_i32(dfr_ctx* h, int32_t key, int32_t v) {
             if (v != 128 && v != 256) return c->fail(DFR_E_ARGUMENT, "wavefront block is 128 or 256");
             c->wf_params.block = v; break;
         case DFR_KEY_WF_POOL_PATHS:
-            if (v < 1024 || v > MAX_POOL_PATHS) return c->fail(DFR_E_ARGUMENT, "path pool must hold 1024 .. 2^25 paths");
+            if (v < 1024 || v > MAX_POOL_PATHS) return c->fail(DFR_E_ARGUMENT, "path pool must hold 1024 .. 2^26 paths");
             c->wf_params.capacity = v; break;
         case DFR_KEY_WF_SORT_KEYS:
             if (v != 0 && v != 1) return c->fail(DFR_E_ARGUMENT, "sort is 0 (off) or 1 (on)");

@@ -69,7 +69,7 @@ size_t wavefront_bytes(int capacity);
 
 struct WavefrontConfig {
     int block = 256;
-    int capacity = 1 << 25;  // path pool size (32M paths, about 9.4 GB of HBM)
+    int capacity = 1 << 26;  // path pool size (64M paths, about 19 GB of HBM)
     int sort_keys = 1;       // 1: shade in (material, light type) order; 0: queue order
 };
 

@@ -66,8 +66,8 @@ static_assert(sizeof(Counters) == 64, "Counters is one 64-byte line");
 inline int64_t wavefront_stack_threads(int capacity) { return 2 * (int64_t)capacity + 64 + 256; }
 
 // Largest path pool: the material sort's one-block scan holds 128 histogram
-// entries per thread (32 keys x capacity / 8192-entry tiles <= 1024 x 128).
-constexpr int MAX_POOL_PATHS = 1 << 25;
+// entries per thread (32 keys x capacity / 16384-entry tiles <= 1024 x 128).
+constexpr int MAX_POOL_PATHS = 1 << 26;
 
 struct WavefrontBuffers {
     PathSoA a, b;

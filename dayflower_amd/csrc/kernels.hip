@@ -689,7 +689,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
 // exclusive scan of the key-major histogram array, then a scatter in which
 // every wave ranks its entries per key with ballots and takes its base with
 // one LDS atomic per key.
-constexpr int SORT_TILE = 8192;
+constexpr int SORT_TILE = 16384;
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sort_hist(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,

@@ -141,7 +141,7 @@ int autotune_impl(Ctx* c, const dfr_job* jobs, int n, int32_t* out) {
     int64_t work = 0;
     for (auto& j : probe) { j.spp = std::min(j.spp, 4); work += (int64_t)j.width * j.height * j.spp; }
     const int blocks[] = {128, 256};
-    const int caps[] = {1 << 20, 1 << 22, 1 << 23, 1 << 24, 1 << 25};
+    const int caps[] = {1 << 20, 1 << 22, 1 << 24, 1 << 25, 1 << 26};
     const int sorts[] = {0, 1};
     WavefrontConfig best = c->wf_params;
     double best_t = 1e30;
@@ -348,7 +348,7 @@ Server* server_create(Ctx* c, const dfr_server_opts& o) {
     S->opts = o;
     if (S->opts.max_active <= 0) S->opts.max_active = 256;
     if (S->opts.pool_paths <= 0) S->opts.pool_paths = c->wf_params.capacity;
-    if (S->opts.pool_paths > MAX_POOL_PATHS) { c->error = "pool_paths above 2^25"; return nullptr; }
+    if (S->opts.pool_paths > MAX_POOL_PATHS) { c->error = "pool_paths above 2^26"; return nullptr; }
     std::lock_guard<std::mutex> lock(c->mutex);
     hipSetDevice(c->device);
     if (c->scenes.empty()) { c->error = "no resident scene"; return nullptr; }

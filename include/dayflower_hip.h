@@ -197,7 +197,7 @@ typedef struct dfr_ctx dfr_ctx;
 #define DFR_KEY_COUNT_TRAVERSAL        13  /* 1: count BVH node / triangle visits */
 /* wavefront engine configuration (what dfr_autotune sets)                   */
 #define DFR_KEY_WF_BLOCK               14  /* threads per block: 128 or 256     */
-#define DFR_KEY_WF_POOL_PATHS          15  /* path pool capacity, 1024 .. 2^25  */
+#define DFR_KEY_WF_POOL_PATHS          15  /* path pool capacity, 1024 .. 2^26  */
 #define DFR_KEY_WF_SORT_KEYS           16  /* 1: shade in (material, light type) order, 0: queue order */
 #define DFR_KEY_BVH_WIDTH              17  /* read-only: children per BVH node (2 or 4) */
 #define DFR_KEY_USE_GRAPHS             18  /* 1 (default): replay captured HIP graphs of 4 wavefront iterations */
