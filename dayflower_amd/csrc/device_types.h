@@ -88,6 +88,7 @@ struct DevPerez {
     // on the sun only
     float coef_lum[5], coef_x[5], coef_y[5];
     float den_lum, den_x, den_y;
+    float zd_lum, zd_x, zd_y;    // zenith value / denominator, per set
     float s0[3], s1[3], s2[3];   // daylight basis in XYZ
     float xyz_to_rgb[9];         // sRGB, row major
     float world_to_object[9];    // rotation, row major

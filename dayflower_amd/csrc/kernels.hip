@@ -157,21 +157,26 @@ DEV void nee_requests(const DevScene& sc, const Bsdf& b, const Surf& si, int li,
         if (!black(f) && bs.pdf > 0.0f) {
             float w = 1.0f;
             bool ok = true;
+            // the sky's pdf and radiance along the sampled direction share
+            // its object-space form
+            const bool sky = L.type == DFR_LIGHT_PEREZ;
+            const V wd = nrm(bs.wi);
+            const V io = sky ? perez_to_object(sc.perez[L.perez], wd) : wd;
             if (!spec) {
-                float lp = light_pdf(sc, L, si, bs.wi);
+                float lp = sky ? perez_pdf_object(sc.perez[L.perez], io) : light_pdf(sc, L, si, bs.wi);
                 if (lp == 0.0f) ok = false;
                 else w = power_heuristic(bs.pdf, lp);
             }
             if (ok) {
                 ShadowReq& r = bs_req;
-                r.o = si.p; r.d = nrm(bs.wi); r.tmax = FLT_MAX;
+                r.o = si.p; r.d = wd; r.tmax = FLT_MAX;
                 r.light = li; r.c = scale * f * w / bs.pdf;
                 if (L.type == DFR_LIGHT_DIFFUSE_AREA) {
                     r.mode = 2;
                 } else {
                     // an infinite light is seen exactly when nothing is hit:
                     // its radiance folds into c and the query is any-hit
-                    r.c = r.c * emitted_ray(sc, L, r.d);
+                    r.c = r.c * (sky ? perez_sky_object(sc.perez[L.perez], io) : emitted_ray(sc, L, r.d));
                     r.mode = black(r.c) ? -1 : 0;
                 }
             }

@@ -191,6 +191,9 @@ void perez_build(DevPerez& p, float turbidity, const float sun_direction_world[3
     p.den_lum = den(lum, p.coef_lum);
     p.den_x = den(px, p.coef_x);
     p.den_y = den(py, p.coef_y);
+    p.zd_lum = (float)(p.zenith[0] / (double)p.den_lum);
+    p.zd_x = (float)(p.zenith[1] / (double)p.den_x);
+    p.zd_y = (float)(p.zenith[2] / (double)p.den_y);
     to_xyz([&](float lam) { return regular_sample(kS0, 54, 300.0f, 830.0f, lam); }, p.s0);
     to_xyz([&](float lam) { return regular_sample(kS1, 54, 300.0f, 830.0f, lam); }, p.s1);
     to_xyz([&](float lam) { return regular_sample(kS2, 54, 300.0f, 830.0f, lam); }, p.s2);
