@@ -41,17 +41,21 @@ DEV V operator-(V a, V b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
 DEV V operator-(V a) { return mk(-a.x, -a.y, -a.z); }
 DEV V operator*(V a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
 DEV V operator*(V a, V b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
-DEV V operator/(V a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+// one correctly rounded reciprocal, three products (within an ulp of three
+// quotients)
+DEV V operator/(V a, float s) { const float r = 1.0f / s; return mk(a.x * r, a.y * r, a.z * r); }
 DEV float dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 DEV V cross(V a, V b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 DEV float comp(V v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
 DEV float len(V v) { return sqrtf(dot(v, v)); }
 // Vector3F.normalize (M/geometry/Vector3F.java:1000-1011): unchanged when the
-// length is already within [nextDown^3(1), nextUp(1)].
+// length is already within [nextDown^3(1), nextUp(1)] (tested on the squared
+// length); otherwise scaled by the hardware reciprocal square root (about an
+// ulp), which replaces a square root and three divisions.
 DEV V nrm(V v) {
-    float l = len(v);
-    if (l >= 0.99999982f && l <= 1.00000012f) return v;
-    return v / l;
+    const float d = dot(v, v);
+    if (d >= 0.99999964f && d <= 1.00000024f) return v;
+    return v * __builtin_amdgcn_rsqf(d);
 }
 DEV V ld3(const float* p) { return mk(p[0], p[1], p[2]); }
 DEV V f4v(float4 a) { return mk(a.x, a.y, a.z); }
