@@ -43,7 +43,7 @@ DEV V operator*(V a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
 DEV V operator*(V a, V b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
 // one correctly rounded reciprocal, three products (within an ulp of three
 // quotients)
-DEV V operator/(V a, float s) { const float r = 1.0f / s; return mk(a.x * r, a.y * r, a.z * r); }
+DEV V operator/(V a, float s) { const float r = __builtin_amdgcn_rcpf(s); return mk(a.x * r, a.y * r, a.z * r); }
 DEV float dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 DEV V cross(V a, V b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 DEV float comp(V v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
@@ -627,7 +627,7 @@ DEV Surf make_surface(const DevScene& sc, V o, V d, const HitRec& h) {
         V a = f4v(r0), b = f4v(r1), c = f4v(r2);
         V eab = b - a, eca = a - c;
         V d1 = cross(eab, eca);
-        float rdet = 1.0f / dot(d, d1);
+        float rdet = __builtin_amdgcn_rcpf(dot(d, d1));
         V d3 = cross(a - o, d);
         float u = dot(d3, eca) * rdet, v = dot(d3, eab) * rdet, w = 1.0f - u - v;
         s.g = onb_w(mk(r3.w, r4.w, r5.x));
@@ -786,22 +786,23 @@ DEV float tr_G(const DevLobe& l, V wo, V wi) {
 DEV float tr_pdf(const DevLobe& l, V wo, V h) { return tr_D(l, h) * tr_G1(l, wo) * fabsf(dot(wo, h)) * __builtin_amdgcn_rcpf(cos_abs(wo)); }
 DEV void tr_slope(float ct, float u, float v, float& sx, float& sy) {
     if (ct > 0.9999f) {
-        float r = sqrtf(u / (1.0f - u)), phi = 2.0f * PI_F * v;
-        sx = r * cosf(phi); sy = r * sinf(phi); return;
+        float r = __builtin_amdgcn_sqrtf(u * __builtin_amdgcn_rcpf(1.0f - u)), phi = 2.0f * PI_F * v;
+        sx = r * __cosf(phi); sy = r * __sinf(phi); return;
     }
-    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
-    float tt = st / ct;
-    float a = 2.0f / (1.0f + sqrtf(1.0f + 1.0f / ((1.0f / tt) * (1.0f / tt))));
-    float b = 2.0f * u / a - 1.0f;
-    float c = fminf(1.0f / (b * b - 1.0f), 1.0e10f);
+    float st = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    float tt = st * __builtin_amdgcn_rcpf(ct);
+    const float itt = __builtin_amdgcn_rcpf(tt);
+    float a = 2.0f * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(1.0f + __builtin_amdgcn_rcpf(itt * itt)));
+    float b = 2.0f * u * __builtin_amdgcn_rcpf(a) - 1.0f;
+    float c = fminf(__builtin_amdgcn_rcpf(b * b - 1.0f), 1.0e10f);
     float dd = tt;
-    float e = sqrtf(fmaxf(dd * dd * c * c - (b * b - dd * dd) * c, 0.0f));
+    float e = __builtin_amdgcn_sqrtf(fmaxf(dd * dd * c * c - (b * b - dd * dd) * c, 0.0f));
     float f = dd * c - e, g = dd * c + e;
     float hs = v > 0.5f ? 1.0f : -1.0f;
     float i = v > 0.5f ? 2.0f * (v - 0.5f) : 2.0f * (0.5f - v);
-    float j = (i * (i * (i * 0.27385f - 0.73369f) + 0.46341f)) / (i * (i * (i * 0.093073f + 0.309420f) - 1.0f) + 0.597999f);
-    float x = (b < 0.0f || g > 1.0f / tt) ? f : g;
-    sx = x; sy = hs * j * sqrtf(1.0f + x * x);
+    float j = (i * (i * (i * 0.27385f - 0.73369f) + 0.46341f)) * __builtin_amdgcn_rcpf(i * (i * (i * 0.093073f + 0.309420f) - 1.0f) + 0.597999f);
+    float x = (b < 0.0f || g > itt) ? f : g;
+    sx = x; sy = hs * j * __builtin_amdgcn_sqrtf(1.0f + x * x);
 }
 DEV V tr_sample_vis(const DevLobe& l, V wi, float u, float v) {
     V s = nrm(mk(wi.x * l.alphaX, wi.y * l.alphaY, wi.z));
@@ -932,17 +933,17 @@ DEV void disk_concentric(float u, float v, float& x, float& y) {
     if (u == 0.0f && v == 0.0f) { x = 0.0f; y = 0.0f; return; }
     float a = u * 2.0f - 1.0f, b = v * 2.0f - 1.0f;
     if (a * a > b * b) {
-        float phi = (PI_F / 4.0f) * (b / a);
-        x = a * cosf(phi); y = a * sinf(phi);
+        float phi = (PI_F / 4.0f) * (b * __builtin_amdgcn_rcpf(a));
+        x = a * __cosf(phi); y = a * __sinf(phi);
     } else {
-        float phi = (PI_F / 2.0f) - (PI_F / 4.0f) * (a / b);
-        x = b * cosf(phi); y = b * sinf(phi);
+        float phi = (PI_F / 2.0f) - (PI_F / 4.0f) * (a * __builtin_amdgcn_rcpf(b));
+        x = b * __cosf(phi); y = b * __sinf(phi);
     }
 }
 DEV V hemi_cos(float u, float v) {
     float x, y;
     disk_concentric(u, v, x, y);
-    return mk(x, y, sqrtf(fmaxf(0.0f, 1.0f - x * x - y * y)));
+    return mk(x, y, __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - x * x - y * y)));
 }
 
 struct LSample { V f; V wi; float pdf; uint32_t type; };
@@ -1169,6 +1170,6 @@ DEV V emitted_ray(const DevScene& sc, const DevLight& L, V d) {
     return mk(0, 0, 0);
 }
 
-DEV float power_heuristic(float a, float b) { return a * a / (a * a + b * b); }
+DEV float power_heuristic(float a, float b) { return a * a * __builtin_amdgcn_rcpf(a * a + b * b); }
 
 }  // namespace dfr
