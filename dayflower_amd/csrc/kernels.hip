@@ -604,8 +604,9 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
             fresh = false;
             prim = -1; tri = -1; leaf = 0; st.sp = 0; cur = DONE;
             // the ray, the analytic primitives, then the BVH root
+            // (the scene-uniform kernel skips the job -> scene hops)
             auto setup = [&](const float4 o4, const float4 d4, uint32_t job) {
-                const DevScene& sc = scenes[jobs[job].scene];
+                const DevScene& sc = UNI ? *uscene : scenes[jobs[job].scene];
                 const V o = f4v(o4), d = f4v(d4);
                 r = prep_ray(o, d);
                 for (int k = 0; k < sc.n_analytic; k++) {
@@ -620,12 +621,15 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
             if (SHADOW) {
                 const float4 o4 = sh.o[i];
                 tbest = o4.w;
-                setup(o4, sh.d[i], (uint32_t)sh.aux[i].y);
+                setup(o4, sh.d[i], UNI ? 0u : (uint32_t)sh.aux[i].y);
             } else {
+                // the three loads issued together (the ray's are wasted on a
+                // dropped camera sample)
                 const uint4 info = q.info[i];
+                const float4 o4 = q.o[i], d4 = q.d[i];
                 tbest = FLT_MAX;
                 if (info.w & 0x80000000u) prim = -2;  // dropped camera sample
-                else setup(q.o[i], q.d[i], info.z);
+                else setup(o4, d4, info.z);
             }
         }
         if (active && (cur != DONE || leaf != 0)) {
