@@ -374,6 +374,12 @@ struct VisitCount {
 };
 
 constexpr uint32_t TRAVERSAL_DONE = 0x7fffffffu;  // no node left (bit 31 clear, never a node index)
+// Phase switches of the while-while traversal on 64-wide waves: the wave
+// leaves the inner-node loop once at most this many of its lanes still have
+// no leaf parked, and leaves the leaf loop once at most this many lanes still
+// hold one (the others' work carries over to the next round).  Waiting for
+// every lane, as on 32-wide warps, idles most of a 64-wide wave.
+constexpr int TRAVERSAL_STRAGGLERS = 8;
 
 // One inner-node step of an ordered traversal: tests the children's boxes
 // against (tmin, tmax), continues with the nearest entered child and pushes
@@ -530,13 +536,14 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
     const PlaneSel ps = plane_sel(r);
     st.sp = 0;
     // Speculative while-while traversal: a lane that reaches a leaf parks it
-    // and keeps descending inner nodes until every lane of the wave has a
-    // parked leaf (or is done); then the wave tests leaves together.  This
-    // keeps lanes busy while others are still in inner nodes.
+    // and keeps descending inner nodes until all but TRAVERSAL_STRAGGLERS
+    // lanes of the wave have a parked leaf (or are done); then the wave tests
+    // leaves together, until all but that many lanes have run out of leaves.
+    // A lane resumes where it stopped in the next round.
     constexpr uint32_t DONE = TRAVERSAL_DONE;
     uint32_t cur = 0;                       // root node
     uint32_t leaf = 0;                      // parked leaf reference (0: none)
-    while (cur != DONE) {
+    while (cur != DONE || leaf != 0) {
         while (cur < DONE) {  // inner node
             vc.nodes++;
             cur = visit_node<SHORT, !ANY>(s.nodes, cur, r, ps, tmin, hit.t, st);
@@ -544,7 +551,7 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
                 leaf = cur;
                 cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
             }
-            if (__ballot(leaf == 0) == 0) break;
+            if (__popcll(__ballot(leaf == 0)) <= TRAVERSAL_STRAGGLERS) break;
         }
         while (leaf != 0) {
             uint32_t first = (leaf & 0x7fffffffu) >> 3, cnt = (leaf & 7u) + 1;
@@ -563,6 +570,7 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
                 leaf = cur;
                 cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
             }
+            if (__popcll(__ballot(leaf != 0)) <= TRAVERSAL_STRAGGLERS) break;
         }
     }
     if (REFINE && !ANY && found && hit.tri < 0) hit.t = analytic_t64(s.prims[hit.prim], o, d, hit.t);

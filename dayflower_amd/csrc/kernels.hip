@@ -640,7 +640,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                     leaf = cur;
                     cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
                 }
-                if (__ballot(leaf == 0) == 0) break;
+                if (__popcll(__ballot(leaf == 0)) <= TRAVERSAL_STRAGGLERS) break;
             }
             while (leaf != 0) {
                 const uint32_t first = (leaf & 0x7fffffffu) >> 3, cnt = (leaf & 7u) + 1;
@@ -661,6 +661,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                     leaf = cur;
                     cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
                 }
+                if (__popcll(__ballot(leaf != 0)) <= TRAVERSAL_STRAGGLERS) break;
             }
         }
         // finished rays: record the result, then claim the next entry of the range
