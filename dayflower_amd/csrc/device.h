@@ -554,19 +554,18 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
             if (__popcll(__ballot(leaf == 0)) <= TRAVERSAL_STRAGGLERS) break;
         }
         while (leaf != 0) {
-            uint32_t first = (leaf & 0x7fffffffu) >> 3, cnt = (leaf & 7u) + 1;
-            for (uint32_t k = 0; k < cnt; k++) {
-                vc.tris++;
-                float4 a, b, c;
-                load_tri(s.tris, first + k, a, b, c);
-                float t = tri_t(f4v(a), f4v(b), f4v(c), r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, tmin, hit.t);
-                if (t != 0.0f && t < hit.t) {
-                    hit.t = t; hit.tri = __float_as_int(a.w); hit.prim = __float_as_int(b.w); found = true;
-                    if (ANY) return true;
-                }
+            // one triangle per round (the reference steps by 7, as in k_trace_queue)
+            const uint32_t first = (leaf & 0x7fffffffu) >> 3;
+            vc.tris++;
+            float4 a, b, c;
+            load_tri(s.tris, first, a, b, c);
+            float t = tri_t(f4v(a), f4v(b), f4v(c), r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, tmin, hit.t);
+            if (t != 0.0f && t < hit.t) {
+                hit.t = t; hit.tri = __float_as_int(a.w); hit.prim = __float_as_int(b.w); found = true;
+                if (ANY) return true;
             }
-            leaf = 0;
-            if (cur & LEAF_FLAG_DEV) {
+            leaf = (leaf & 7u) ? leaf + 7u : 0u;
+            if (leaf == 0 && (cur & LEAF_FLAG_DEV)) {
                 leaf = cur;
                 cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
             }

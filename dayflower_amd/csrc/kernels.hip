@@ -643,21 +643,21 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                 if (__popcll(__ballot(leaf == 0)) <= TRAVERSAL_STRAGGLERS) break;
             }
             while (leaf != 0) {
-                const uint32_t first = (leaf & 0x7fffffffu) >> 3, cnt = (leaf & 7u) + 1;
+                // one triangle per round: the leaf reference steps to its next
+                // triangle ((first + 1) << 3 | (count - 2) = reference + 7)
+                const uint32_t first = (leaf & 0x7fffffffu) >> 3;
+                if (COUNT) vc.tris++;
+                float4 a, b, c;
+                load_tri(tris, first, a, b, c);
+                const float t = tri_t(f4v(a), f4v(b), f4v(c), r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, T_MIN, tbest);
                 bool occluded = false;
-                for (uint32_t k = 0; k < cnt; k++) {
-                    if (COUNT) vc.tris++;
-                    float4 a, b, c;
-                    load_tri(tris, first + k, a, b, c);
-                    const float t = tri_t(f4v(a), f4v(b), f4v(c), r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, T_MIN, tbest);
-                    if (t != 0.0f && t < tbest) {
-                        tbest = t; tri = __float_as_int(a.w); prim = __float_as_int(b.w);
-                        if (SHADOW) { occluded = true; break; }
-                    }
+                if (t != 0.0f && t < tbest) {
+                    tbest = t; tri = __float_as_int(a.w); prim = __float_as_int(b.w);
+                    occluded = SHADOW;
                 }
-                leaf = 0;
-                if (SHADOW && occluded) { cur = DONE; break; }
-                if (cur & LEAF_FLAG_DEV) {
+                leaf = (leaf & 7u) ? leaf + 7u : 0u;
+                if (SHADOW && occluded) { leaf = 0; cur = DONE; break; }
+                if (leaf == 0 && (cur & LEAF_FLAG_DEV)) {
                     leaf = cur;
                     cur = st.sp == 0 ? DONE : (uint32_t)st.pop();
                 }
