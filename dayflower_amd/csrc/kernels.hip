@@ -560,7 +560,10 @@ DEV int sort_key(const DevScene* __restrict__ scenes, const DevJob* __restrict__
 //   SHADOW false: closest hit of every queued path, written to hits[i].
 //   SHADOW true:  the any-hit part of the shadow queue (entries [0, n_any)):
 //                 a request that hits nothing adds its contribution to the film.
-constexpr int EXTEND_RPL = 4;  // queue entries per thread of a traversal block
+// queue entries per thread of a traversal block: 8 for the extension rays,
+// 4 for the shorter-lived shadow requests (whose queue is shorter too)
+constexpr int EXTEND_RPL = 8;
+constexpr int SHADOW_RPL = 4;
 
 // At most 72 VGPRs (7 waves per SIMD): measured faster than 6 waves with no
 // register limit and than 8 waves with spills.
@@ -972,10 +975,10 @@ template <int B, bool COUNT, bool SHADOW>
 static void launch_trace_queue(int grid, const DevScene* scenes, const DevJob* jobs, float4* film, WavefrontBuffers& w,
                                const WavefrontParams& p, unsigned long long* trav, hipStream_t s) {
     if (BVH_WIDTH == 4 && p.uscene)
-        hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, EXTEND_RPL, COUNT, SHADOW, true>), dim3(grid), dim3(B), 0, s,
+        hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, SHADOW ? SHADOW_RPL : EXTEND_RPL, COUNT, SHADOW, true>), dim3(grid), dim3(B), 0, s,
                            scenes, jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav, p.uscene);
     else
-        hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, EXTEND_RPL, COUNT, SHADOW, false>), dim3(grid), dim3(B), 0, s,
+        hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, SHADOW ? SHADOW_RPL : EXTEND_RPL, COUNT, SHADOW, false>), dim3(grid), dim3(B), 0, s,
                            scenes, jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav, p.uscene);
 }
 
@@ -1010,7 +1013,7 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(1), 0, s, w.ctr);
     if (timing) hipEventRecord(prof->ev[4], s);
     const int sgrid = (int)((2 * (int64_t)cap + B - 1) / B);  // shadow requests <= 2 per path
-    const int agrid = (int)((2 * (int64_t)cap + EXTEND_RPL * B - 1) / (EXTEND_RPL * B));
+    const int agrid = (int)((2 * (int64_t)cap + SHADOW_RPL * B - 1) / (SHADOW_RPL * B));
     if (trav) {
         launch_trace_queue<B, true, true>(agrid, scenes, jobs, film, w, p, trav, s);
         if (p.closest_shadow)
