@@ -483,8 +483,19 @@ DEV int find_job(const DevJob* jobs, int n_jobs, int64_t item) {
 }
 
 // New camera paths appended after the continuing ones (up to capacity).
-__global__ void k_generate(const DevJob* __restrict__ jobs, int n_jobs, PathSoA out, Counters* __restrict__ ctr,
-                           float4* __restrict__ film, int capacity) {
+// Type (0..2) of the light a path's NEE at `bounce` will sample, 3 without
+// lights: the light choice is the stateless random number of that bounce.
+// Paths carry it in bits 9-10 of info.w, so the sort key of a hit needs only
+// the hit's material.
+constexpr uint32_t INFO_LT_SHIFT = 9;
+DEV uint32_t nee_light_type(const DevScene& sc, const Rng& rng, uint32_t bounce) {
+    if (sc.n_lights == 0) return 3u;
+    const int li = min((int)(rng.get(DIM_BOUNCE + DIMS_PER_BOUNCE * bounce) * sc.n_lights), sc.n_lights - 1);
+    return (uint32_t)sc.lights[li].type;
+}
+
+__global__ void k_generate(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, int n_jobs,
+                           PathSoA out, Counters* __restrict__ ctr, float4* __restrict__ film, int capacity) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int base = ctr->n_next;
     int64_t next = ctr->work_next, total = ctr->work_total;
@@ -510,6 +521,7 @@ __global__ void k_generate(const DevJob* __restrict__ jobs, int n_jobs, PathSoA 
     } else {
         unsafeAtomicAdd(&film[job.film_offset + pix].w, 1.0f);
     }
+    flags |= nee_light_type(scenes[job.scene], rng, 0u) << INFO_LT_SHIFT;
     out.o[slot] = make_float4(o.x, o.y, o.z, 0.0f);
     out.d[slot] = make_float4(d.x, d.y, d.z, 1.0f);
     out.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
@@ -538,18 +550,14 @@ __global__ void k_commit(Counters* ctr, int capacity) {
 // 31 = dropped camera sample.
 constexpr int NSORT_KEYS = 32;
 
-DEV int sort_key(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, uint4 info, float4 h) {
-    const int prim = __float_as_int(h.y);
+DEV int sort_key_in(const DevScene& sc, int prim, uint32_t info_w) {
     if (prim == -2) return 31;
     if (prim < 0) return 30;
-    const DevJob& job = jobs[info.z];
-    const DevScene& sc = scenes[job.scene];
-    const int m = sc.mats[sc.prims[prim].material].type;
-    if (sc.n_lights == 0) return 4 * m + 3;
-    const Rng rng = make_rng(job.seed, info.x, info.y);
-    const uint32_t dim = DIM_BOUNCE + DIMS_PER_BOUNCE * (info.w & 0xffu);
-    const int li = min((int)(rng.get(dim) * sc.n_lights), sc.n_lights - 1);
-    return 4 * m + sc.lights[li].type;
+    return 4 * sc.mats[sc.prims[prim].material].type + (int)((info_w >> INFO_LT_SHIFT) & 3u);
+}
+DEV int sort_key(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, uint4 info, float4 h) {
+    const int prim = __float_as_int(h.y);
+    return prim < 0 ? sort_key_in(scenes[0], prim, info.w) : sort_key_in(scenes[jobs[info.z].scene], prim, info.w);
 }
 
 // Traversal of a ray queue with ray replacement: block b owns RPL * BLOCK
@@ -576,9 +584,12 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                                                          float4* __restrict__ hits, float4* __restrict__ film,
                                                          const Counters* __restrict__ ctr, int* __restrict__ overflow,
                                                          unsigned long long* __restrict__ trav,
-                                                         const DevScene* __restrict__ uscene) {
+                                                         const DevScene* __restrict__ uscene, int32_t* __restrict__ keys) {
     __shared__ int lds[SHORT * BLOCK];
     __shared__ int next;
+    // scene-uniform extension with sorted shading: the finished ray writes its
+    // sort key (the material of the hit, the NEE light type from info.w)
+    uint32_t info_w = 0;
     constexpr uint32_t DONE = TRAVERSAL_DONE;
     const int n = SHADOW ? (int)(ctr->n_shadow2 & 0xffffffffull) : ctr->n_active;
     const int base = blockIdx.x * RPL * BLOCK;
@@ -630,6 +641,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                 // dropped camera sample)
                 const uint4 info = q.info[i];
                 const float4 o4 = q.o[i], d4 = q.d[i];
+                info_w = info.w;
                 tbest = FLT_MAX;
                 if (info.w & 0x80000000u) prim = -2;  // dropped camera sample
                 else setup(o4, d4, info.z);
@@ -678,6 +690,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                 }
             } else {
                 hits[i] = make_float4(tbest, __int_as_float(prim), __int_as_float(tri), 0.0f);
+                if (UNI && keys) keys[i] = sort_key_in(*uscene, prim, info_w);
             }
         }
         const uint64_t m = __ballot(finished);
@@ -708,7 +721,7 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sort_hist(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
                                                      PathSoA q, const float4* __restrict__ hits,
                                                      int32_t* __restrict__ keys, int32_t* __restrict__ bhist,
-                                                     const Counters* __restrict__ ctr) {
+                                                     const Counters* __restrict__ ctr, int keys_ready) {
     __shared__ int hist[NSORT_KEYS];
     const int n = ctr->n_active;
     const int t0 = blockIdx.x * SORT_TILE;
@@ -721,8 +734,13 @@ __global__ __launch_bounds__(BLOCK) void k_sort_hist(const DevScene* __restrict_
         const int i = t0 + k;
         int key = -1;
         if (i < n) {
-            key = sort_key(scenes, jobs, q.info[i], hits[i]);
-            keys[i] = key;
+            // the scene-uniform extension kernel has written the keys
+            if (keys_ready) {
+                key = keys[i];
+            } else {
+                key = sort_key(scenes, jobs, q.info[i], hits[i]);
+                keys[i] = key;
+            }
         }
         uint64_t left = __ballot(key >= 0);
         while (left) {
@@ -869,7 +887,8 @@ __global__ __launch_bounds__(BLOCK, 4) void k_shade(const DevScene* __restrict__
                     if (!m.has_bsdf) {
                         if (pass + 1 <= 256) {
                             cont = true; no = si.p; nd = d; nbeta = beta; neta = eta_scale;
-                            ninfo = make_uint4(info.x, info.y, info.z, (uint32_t)bounce | ((uint32_t)spec << 8) | ((uint32_t)(pass + 1) << 16));
+                            ninfo = make_uint4(info.x, info.y, info.z, (uint32_t)bounce | ((uint32_t)spec << 8) | ((uint32_t)(pass + 1) << 16) |
+                                                                        (info.w & (3u << INFO_LT_SHIFT)));
                         }
                     } else {
                         Bsdf b = make_bsdf(sc, m, si, d);
@@ -894,7 +913,8 @@ __global__ __launch_bounds__(BLOCK, 4) void k_shade(const DevScene* __restrict__
                             }
                             if (alive) {
                                 cont = true; no = si.p; nd = nrm(bs.wi); nbeta = nb; neta = ne;
-                                ninfo = make_uint4(info.x, info.y, info.z, (uint32_t)(bounce + 1) | ((uint32_t)nspec << 8) | ((uint32_t)pass << 16));
+                                ninfo = make_uint4(info.x, info.y, info.z, (uint32_t)(bounce + 1) | ((uint32_t)nspec << 8) | ((uint32_t)pass << 16) |
+                                                                            (nee_light_type(sc, rng, (uint32_t)bounce + 1u) << INFO_LT_SHIFT));
                             }
                         }
                     }
@@ -976,10 +996,12 @@ static void launch_trace_queue(int grid, const DevScene* scenes, const DevJob* j
                                const WavefrontParams& p, unsigned long long* trav, hipStream_t s) {
     if (BVH_WIDTH == 4 && p.uscene)
         hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, SHADOW ? SHADOW_RPL : EXTEND_RPL, COUNT, SHADOW, true>), dim3(grid), dim3(B), 0, s,
-                           scenes, jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav, p.uscene);
+                           scenes, jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav, p.uscene,
+                           SHADOW || !p.sort_keys ? nullptr : w.keys);
     else
         hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, SHADOW ? SHADOW_RPL : EXTEND_RPL, COUNT, SHADOW, false>), dim3(grid), dim3(B), 0, s,
-                           scenes, jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav, p.uscene);
+                           scenes, jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav, p.uscene,
+                           SHADOW || !p.sort_keys ? nullptr : w.keys);
 }
 
 template <int B>
@@ -989,7 +1011,7 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     const int grid = (cap + B - 1) / B;
     const int grid256 = (cap + 255) / 256;
     // top up the queue with new camera paths, then commit the counters
-    hipLaunchKernelGGL(k_generate, dim3(grid256), dim3(256), 0, s, jobs, n_jobs, w.a, w.ctr, film, cap);
+    hipLaunchKernelGGL(k_generate, dim3(grid256), dim3(256), 0, s, scenes, jobs, n_jobs, w.a, w.ctr, film, cap);
     hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, s, w.ctr, cap);
     const bool timing = prof && prof->timing;
     const int use_perm = p.sort_keys > 0;
@@ -1002,7 +1024,7 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     if (use_perm) {
         const int tiles = (cap + SORT_TILE - 1) / SORT_TILE;  // NSORT_KEYS * tiles <= 1024 * SCAN_CHUNK
         hipLaunchKernelGGL((k_sort_hist<256>), dim3(tiles), dim3(256), 0, s, scenes, jobs, w.a, w.hit, w.keys, w.bhist,
-                           w.ctr);
+                           w.ctr, BVH_WIDTH == 4 && p.uscene ? 1 : 0);
         hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, w.bhist, NSORT_KEYS * tiles);
         hipLaunchKernelGGL((k_sort_scatter<256>), dim3(tiles), dim3(256), 0, s, w.keys, w.bhist, w.perm, w.ctr);
     }
