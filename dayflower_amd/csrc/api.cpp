@@ -911,10 +911,11 @@ int32_t dfr_read_rgba8(dfr_ctx* h, uint8_t* rgba) {
     if (!rgba) return c->fail(DFR_E_NULL, "rgba == null");
     if (!c->is_setup) return c->fail(DFR_E_STATE, "setup before read");
     const int64_t n = (int64_t)c->width * c->height;
-    DeviceBuffer tmp;
-    if (tmp.alloc(n * 4) != hipSuccess) return c->fail(DFR_E_MEMORY, "rgba8");
-    launch_rgba8((const float4*)c->film.ptr, (uint8_t*)tmp.ptr, (int)n, c->tone_mapper, c->stream);
-    hipError_t e = hipMemcpyAsync(rgba, tmp.ptr, n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (c->image.alloc(n * 4) != hipSuccess) return c->fail(DFR_E_MEMORY, "rgba8");
+    // GPURenderer.java:312-335: only doRunPathTracing tone-maps
+    const int tone = c->algorithm == DFR_ALGO_PATH_TRACING ? c->tone_mapper : DFR_TONE_MAPPER_NONE;
+    launch_rgba8((const float4*)c->film.ptr, (uint8_t*)c->image.ptr, (int)n, tone, c->stream);
+    hipError_t e = hipMemcpyAsync(rgba, c->image.ptr, n * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "rgba8 readback");
     return DFR_OK;

@@ -87,7 +87,7 @@ struct Ctx {
     int algorithm = DFR_ALGO_PATH_TRACING;
     int ao_samples = 10;
     float max_distance = 20.0f;
-    int tone_mapper = 0, preview = 0;
+    int tone_mapper = DFR_TONE_MAPPER_NONE, preview = 0;
     uint64_t seed = 0;
     int kernel = 0;            // 0 wavefront, 1 plain megakernel
     int light_pdf_mode = 0;
@@ -102,7 +102,7 @@ struct Ctx {
     int render_pass = 0;
     std::vector<std::unique_ptr<CompiledScene>> scenes;
     int main_scene = -1;
-    DeviceBuffer scene_table, job_buf, film, overflow, counters;
+    DeviceBuffer scene_table, job_buf, film, overflow, counters, image;
 
     // wavefront engine
     WavefrontConfig wf_params;

@@ -449,25 +449,43 @@ void launch_cast_primary(const DevScene* scene, const DevJob* job, float* rays, 
                        prim, tri, t, overflow, n_pixels);
 }
 
-// imageBegin..imageEnd without tone mapping (AbstractImageKernel.java
-// :1104-1168, sRGB constants :39-47).
-__global__ void k_rgba8(const float4* __restrict__ film, uint8_t* __restrict__ out, int n) {
+// Tone map operators of AbstractImageKernel.java:1189-1324 at exposure 1,
+// the only exposure GPURenderer.java:312-335 passes.  Ordinals as in
+// AbstractGPURenderer.java:50-79; any other value leaves the colour as is.
+__device__ __forceinline__ float tone_map(int op, float a) {
+    switch (op) {
+        case 0: {  // filmic curve, ACES modified: subtract 0, minimum -FLT_MAX
+            const float b = fmaxf(a, -3.40282347e38f);
+            return fminf(fmaxf((b * (2.51f * b + 0.03f)) / (b * (2.43f * b + 0.59f) + 0.14f), 0.0f), 1.0f);
+        }
+        case 2: return a / (1.0f + a);                          // Reinhard
+        case 3: return a * (1.0f + a / 16.0f) / (1.0f + a);     // Reinhard, white point 4
+        case 4: return 1.0f - expf(-a);                         // 1 - exp(-exposure^2 a)
+        case 5: return a / (a + 0.155f) * 1.019f;               // Unreal 3
+        default: return a;
+    }
+}
+
+// imageBegin, tone map, imageRedoGammaCorrection, imageEnd
+// (AbstractImageKernel.java:1104-1168, sRGB constants :39-47) over the
+// (sum, count) film: one pixel per thread, 16 B in, 4 B out.
+__global__ void k_rgba8(const float4* __restrict__ film, uint8_t* __restrict__ out, int n, int tone) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 f = film[i];
     float inv = f.w > 0.0f ? 1.0f / f.w : 0.0f;
     float c[3] = {f.x * inv, f.y * inv, f.z * inv};
+    uint32_t px = 0xff000000u;
     for (int k = 0; k < 3; k++) {
-        float v = c[k];
+        float v = tone_map(tone, c[k]);
         v = v <= 0.00304f ? v * 12.9230566f : 1.05500054f * powf(v, 0.41666666f) - 0.05500052f;
-        out[4 * i + k] = (uint8_t)(int)(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f + 0.5f);
+        px |= (uint32_t)(uint8_t)(int)(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f + 0.5f) << (8 * k);
     }
-    out[4 * i + 3] = 255;
+    reinterpret_cast<uint32_t*>(out)[i] = px;
 }
 
 void launch_rgba8(const float4* film, uint8_t* out, int n, int tone_mapper, hipStream_t s) {
-    (void)tone_mapper;
-    hipLaunchKernelGGL(k_rgba8, dim3((n + 255) / 256), dim3(256), 0, s, film, out, n);
+    hipLaunchKernelGGL(k_rgba8, dim3((n + 255) / 256), dim3(256), 0, s, film, out, n, tone_mapper);
 }
 
 // ---------------------------------------------------------------------------

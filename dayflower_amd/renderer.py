@@ -41,6 +41,16 @@ class RenderingAlgorithm(enum.IntEnum):
     RAY_TRACING = abi.ALGO_RAY_TRACING
 
 
+class ToneMapper(enum.IntEnum):
+    """AbstractGPURenderer.java:50-79 (TONE_MAPPER_* ordinals; NONE is the default)."""
+    FILMIC_CURVE_ACES_MODIFIED_VERSION_1 = abi.TONE_MAPPER_FILMIC_CURVE_ACES_MODIFIED_VERSION_1
+    NONE = abi.TONE_MAPPER_NONE
+    REINHARD = abi.TONE_MAPPER_REINHARD
+    REINHARD_MODIFIED_VERSION_1 = abi.TONE_MAPPER_REINHARD_MODIFIED_VERSION_1
+    REINHARD_MODIFIED_VERSION_2 = abi.TONE_MAPPER_REINHARD_MODIFIED_VERSION_2
+    UNREAL_3 = abi.TONE_MAPPER_UNREAL_3
+
+
 def _check(lib, ctx, rc):
     if rc >= 0:
         return rc
@@ -149,7 +159,10 @@ class GPURenderer:
         return out.value
 
     def setToneMapper(self, v):
-        self._seti(abi.KEY_TONE_MAPPER, v)
+        self._seti(abi.KEY_TONE_MAPPER, int(v))
+
+    def getToneMapper(self):
+        return self._geti(abi.KEY_TONE_MAPPER)
 
     def setPreviewMode(self, v):
         self._seti(abi.KEY_PREVIEW_MODE, int(bool(v)))

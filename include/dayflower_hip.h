@@ -47,6 +47,16 @@ extern "C" {
 #define DFR_ALGO_RAY_CASTING       3
 #define DFR_ALGO_RAY_TRACING       4
 
+/* Tone map operators (AbstractGPURenderer.java:50-79; ordinals as there).
+ * Only path tracing applies them, with exposure 1 (GPURenderer.java:312-335);
+ * any other value, and every other algorithm, sRGB-encodes the film as is. */
+#define DFR_TONE_MAPPER_FILMIC_CURVE_ACES_MODIFIED_VERSION_1 0
+#define DFR_TONE_MAPPER_NONE                                 1
+#define DFR_TONE_MAPPER_REINHARD                             2
+#define DFR_TONE_MAPPER_REINHARD_MODIFIED_VERSION_1          3
+#define DFR_TONE_MAPPER_REINHARD_MODIFIED_VERSION_2          4
+#define DFR_TONE_MAPPER_UNREAL_3                             5
+
 /* Shapes: M/geometry/shape/{TriangleMesh3F,Sphere3F,Rectangle3F,Plane3F}.java */
 #define DFR_SHAPE_TRIANGLE_MESH 0
 #define DFR_SHAPE_SPHERE        1
@@ -186,7 +196,7 @@ typedef struct dfr_ctx dfr_ctx;
 #define DFR_KEY_MINIMUM_BOUNCE_RR       2  /* default 5                        */
 #define DFR_KEY_RENDERING_ALGORITHM     3  /* default DFR_ALGO_PATH_TRACING    */
 #define DFR_KEY_SAMPLES                 4  /* AO samples, default 10           */
-#define DFR_KEY_TONE_MAPPER             5  /* default 0 (none)                 */
+#define DFR_KEY_TONE_MAPPER             5  /* default DFR_TONE_MAPPER_NONE      */
 #define DFR_KEY_PREVIEW_MODE            6  /* default 0                        */
 #define DFR_KEY_SEED                    7  /* 64-bit seed, low 32 bits         */
 #define DFR_KEY_SEED_HI                 8  /* 64-bit seed, high 32 bits        */
@@ -255,8 +265,11 @@ int32_t dfr_update_camera(dfr_ctx* ctx, const float camera[DFR_CAMERA_LENGTH]);
  * per pixel.  Either pointer may be NULL. */
 int32_t dfr_read_film(dfr_ctx* ctx, float* rgb, int32_t* count);
 
-/* RGBA8 image (imageBegin..imageEnd, AbstractImageKernel.java:1104-1168):
- * tone map (if any), sRGB encode, (int)(clamp*255+0.5). */
+/* RGBA8 image (imageBegin..imageEnd, AbstractImageKernel.java:1104-1324;
+ * GPURenderer.java:312-335): the tone map selected by DFR_KEY_TONE_MAPPER
+ * when the algorithm is path tracing, sRGB encode (break point 0.00304,
+ * slope 12.923, gamma 2.4), then (int)(clamp(v, 0, 1) * 255 + 0.5); alpha
+ * 255.  rgba: 4 bytes per pixel, row-major, y down. */
 int32_t dfr_read_rgba8(dfr_ctx* ctx, uint8_t* rgba);
 
 /* Counters of the last render call. */

@@ -1667,6 +1667,59 @@ int32_t orc_perez_sample_many(float turbidity, const float* sun, int64_t n, cons
     return 0;
 }
 
+// Tone map operators at exposure 1 (AbstractImageKernel.java:1189-1324).
+// Aparapi's Java-mode exp and pow evaluate in double and round to float.
+static float toneMap(int op, float a) {
+    switch (op) {
+        case DFR_TONE_MAPPER_FILMIC_CURVE_ACES_MODIFIED_VERSION_1: {  // :1189-1216
+            const float b = std::max(a * 1.0f - 0.0f, -FLT_MAX);
+            const float c = (b * (2.51f * b + 0.03f)) / (b * (2.43f * b + 0.59f) + 0.14f);
+            return std::min(std::max(c, 0.0f), 1.0f);
+        }
+        case DFR_TONE_MAPPER_REINHARD: {  // :1223-1241
+            const float b = a * 1.0f;
+            return b / (1.0f + b);
+        }
+        case DFR_TONE_MAPPER_REINHARD_MODIFIED_VERSION_1: {  // :1248-1268, lWhite = 4
+            const float lWhite = 4.0f;
+            const float b = a * 1.0f;
+            return b * (1.0f + b / (lWhite * lWhite)) / (1.0f + b);
+        }
+        case DFR_TONE_MAPPER_REINHARD_MODIFIED_VERSION_2: {  // :1275-1293
+            const float b = a * 1.0f;
+            return 1.0f - (float)std::exp(-(double)(b * 1.0f));
+        }
+        case DFR_TONE_MAPPER_UNREAL_3: {  // :1300-1318
+            const float b = a * 1.0f;
+            return b / (b + 0.155f) * 1.019f;
+        }
+        default: return a;
+    }
+}
+
+// imageRedoGammaCorrection (:1146-1160) then imageEnd's quantisation (:1122-1139).
+static uint8_t imageByte(float v) {
+    const float g = v <= 0.00304f ? v * 12.9230566f
+                                  : 1.05500054f * (float)std::pow((double)v, (double)0.41666666f) - 0.05500052f;
+    return (uint8_t)(int)(std::min(std::max(g, 0.0f), 1.0f) * 255.0f + 0.5f);
+}
+
+int32_t orc_tone_map(int32_t tone_mapper, int64_t n, const float* in, float* out) {
+    for (int64_t i = 0; i < n; i++) out[i] = toneMap(tone_mapper, in[i]);
+    return 0;
+}
+
+int32_t orc_image_rgba8(int32_t algorithm, int32_t tone_mapper, int64_t n, const float* rgb, uint8_t* rgba) {
+    // GPURenderer.java:312-335: only doRunPathTracing applies the tone map;
+    // the other algorithms go imageBegin, imageRedoGammaCorrection, imageEnd.
+    const int op = algorithm == DFR_ALGO_PATH_TRACING ? tone_mapper : DFR_TONE_MAPPER_NONE;
+    for (int64_t i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++) rgba[4 * i + k] = imageByte(toneMap(op, rgb[3 * i + k]));
+        rgba[4 * i + 3] = 255;
+    }
+    return 0;
+}
+
 int32_t orc_perez_sky(float turbidity, const float* sun, const float* dir, float* rgb) {
     Perez p;
     perezInit(p, turbidity, V3(sun[0], sun[1], sun[2]));

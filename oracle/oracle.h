@@ -22,6 +22,10 @@
  *   integrators     analytic furnace scene (Le/(1-a)), AO of an open plane
  *                   and of a box interior in closed form
  *   Perez sky       parity unpinned (no reference test, no runnable JVM)
+ *   image stage     textbook sRGB bytes (linear 0.5 -> 188, 0.18 -> 118),
+ *                   tone curves at closed-form points (Reinhard 1 -> 1/2,
+ *                   white point 4 -> 1, 1 - e^-ln2 = 1/2, ACES fit at 1),
+ *                   composition order (tone map before the sRGB encode)
  *
  * M/ = src/main/java/org/dayflower/ and T/ = src/test/java/org/dayflower/ of
  * the reference tree.
@@ -85,6 +89,12 @@ int32_t orc_perez_sky(float turbidity, const float* sun, const float* dir, float
 int32_t orc_perez_pdf_many(float turbidity, const float* sun, int64_t n, const float* dirs, float* pdf);
 int32_t orc_perez_sample_many(float turbidity, const float* sun, int64_t n, const float* uvw, const float* nrm,
                               float* out);
+/* Image stage (AbstractImageKernel.java:1104-1324; GPURenderer.java:312-335):
+ * orc_tone_map applies one operator (DFR_TONE_MAPPER_*) at exposure 1 to n
+ * floats; orc_image_rgba8 turns n linear RGB means into RGBA8 as the
+ * renderer does for `algorithm` (a DFR_ALGO_* ordinal). */
+int32_t orc_tone_map(int32_t tone_mapper, int64_t n, const float* in, float* out);
+int32_t orc_image_rgba8(int32_t algorithm, int32_t tone_mapper, int64_t n, const float* rgb, uint8_t* rgba);
 
 #ifdef __cplusplus
 }

@@ -69,6 +69,8 @@ def load():
     lib.orc_perez_sky.argtypes = [f, vp, vp, vp]
     lib.orc_perez_pdf_many.argtypes = [f, vp, i64, vp, vp]
     lib.orc_perez_sample_many.argtypes = [f, vp, i64, vp, vp, vp]
+    lib.orc_tone_map.argtypes = [i32, i64, vp, vp]
+    lib.orc_image_rgba8.argtypes = [i32, i32, i64, vp, vp]
     _lib = lib
     return lib
 
@@ -157,3 +159,20 @@ def film_agreement(g, n_g, s, q, c):
     z = np.abs(np.asarray(g, np.float64) - mean) / np.maximum(se, 1e-12)
     rrmse = float(np.sqrt(np.mean((g - mean) ** 2)) / np.sqrt(np.mean(mean ** 2)))
     return float((z <= 3.0).mean()), float((z <= 3.0).all(-1).mean()), rrmse
+
+
+def tone_map(tone_mapper, values):
+    """The oracle's tone map operator (exposure 1) over a float array."""
+    x = np.ascontiguousarray(values, np.float32)
+    out = np.empty_like(x)
+    load().orc_tone_map(int(tone_mapper), x.size, _ptr(x), _ptr(out))
+    return out
+
+
+def image_rgba8(rgb, tone_mapper, algorithm=abi.ALGO_PATH_TRACING):
+    """The oracle's RGBA8 image of linear RGB means (..., 3) -> (..., 4) uint8."""
+    x = np.ascontiguousarray(rgb, np.float32)
+    n = x.size // 3
+    out = np.empty(x.shape[:-1] + (4,), np.uint8)
+    load().orc_image_rgba8(int(algorithm), int(tone_mapper), n, _ptr(x), _ptr(out))
+    return out

@@ -32,6 +32,16 @@ def test_abi_table_matches_header():
     assert sorted(n for n, _, _ in abi.ENTRY_POINTS) == _declared()
 
 
+def test_ordinals_match_header():
+    """Rendering-algorithm and tone-mapper ordinals of the header (those of
+    AbstractGPURenderer.java:50-87) are the ones the Python mirror passes."""
+    text = open(HEADER).read()
+    defs = dict((m.group(1), int(m.group(2))) for m in re.finditer(r"#define\s+DFR_((?:ALGO|TONE_MAPPER)_\w+)\s+(\d+)", text))
+    assert len(defs) == 11
+    for name, value in defs.items():
+        assert getattr(abi, name) == value, name
+
+
 @pytest.fixture(scope="module")
 def c_layout(tmp_path_factory):
     d = tmp_path_factory.mktemp("layout")

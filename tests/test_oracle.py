@@ -560,3 +560,56 @@ def test_perez_pdf_normalises_and_matches_samples(oracle):
     keep = expect > 50
     chi2 = np.sum((counts[keep] - expect[keep]) ** 2 / expect[keep])
     assert chi2 < 3.0 * keep.sum() + 30.0, (chi2, counts, expect)
+
+
+# --- image stage (AbstractImageKernel.java:1104-1324, GPURenderer.java:312-335)
+
+def test_srgb_encode_round_trips_every_byte():
+    """Linear values from the IEC 61966-2-1 inverse of each 8-bit code come
+    back as that code through the reference's encode and quantisation."""
+    k = np.arange(256, dtype=np.float64)
+    c = k / 255.0
+    lin = np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4).astype(np.float32)
+    rgb = np.repeat(lin[:, None], 3, axis=1)
+    out = oracle_lib.image_rgba8(rgb, abi.TONE_MAPPER_NONE)
+    assert (out[:, :3] == k[:, None].astype(np.uint8)).all()
+    assert (out[:, 3] == 255).all()
+    # textbook points (18 % grey, one half) and the clamps
+    rgb = np.repeat(fa(0.18, 0.5, 1.0, 7.0, -2.0, 0.0)[:, None], 3, axis=1)
+    assert oracle_lib.image_rgba8(rgb, abi.TONE_MAPPER_NONE)[:, 0].tolist() == [118, 188, 255, 255, 0, 0]
+
+
+def test_tone_curves_at_closed_form_points():
+    T = oracle_lib.tone_map
+    # Reinhard L / (1 + L)
+    assert T(abi.TONE_MAPPER_REINHARD, fa(0.0, 1.0, 3.0)).tolist() == [0.0, 0.5, 0.75]
+    # extended Reinhard maps its white point (4) to one
+    assert T(abi.TONE_MAPPER_REINHARD_MODIFIED_VERSION_1, fa(0.0, 2.0, 4.0)).tolist() == [0.0, 0.75, 1.0]
+    # 1 - exp(-L): one half at ln 2, zero at zero, one far out
+    v = T(abi.TONE_MAPPER_REINHARD_MODIFIED_VERSION_2, fa(0.0, math.log(2.0), 50.0))
+    assert v[0] == 0.0 and abs(v[1] - 0.5) < 1e-7 and v[2] == 1.0
+    # Unreal 3 fit L / (L + 0.155) * 1.019: half of 1.019 at 0.155
+    v = T(abi.TONE_MAPPER_UNREAL_3, fa(0.0, 0.155, 1e30))
+    assert v[0] == 0.0 and abs(v[1] - 0.5095) < 1e-6 and abs(v[2] - 1.019) < 1e-6
+    # ACES fit x (2.51 x + 0.03) / (x (2.43 x + 0.59) + 0.14): 2.54 / 3.16 at
+    # one, zero at zero, clamped to one far out; the curve is even about
+    # neither axis, so -1 gives 2.48 / 1.98 > 1 and clamps to one
+    v = T(abi.TONE_MAPPER_FILMIC_CURVE_ACES_MODIFIED_VERSION_1, fa(0.0, 1.0, 1e6, -1.0))
+    assert v[0] == 0.0 and abs(v[1] - 2.54 / 3.16) < 1e-6 and v[2] == 1.0 and v[3] == 1.0
+    # none, and ordinals outside the table, leave the value as is
+    x = fa(0.0, 0.25, 3.5, 1e4)
+    for op in (abi.TONE_MAPPER_NONE, 6, -1):
+        assert (T(op, x) == x).all()
+
+
+def test_image_tone_maps_before_encoding_and_only_when_path_tracing():
+    rng = np.random.default_rng(7)
+    rgb = (rng.exponential(0.6, size=(4096, 3))).astype(np.float32)
+    for op in range(6):
+        pt = oracle_lib.image_rgba8(rgb, op)
+        assert (pt == oracle_lib.image_rgba8(oracle_lib.tone_map(op, rgb), abi.TONE_MAPPER_NONE)).all()
+        for algo in (abi.ALGO_AMBIENT_OCCLUSION, abi.ALGO_RAY_CASTING, abi.ALGO_DEPTH_CAMERA):
+            assert (oracle_lib.image_rgba8(rgb, op, algo) == oracle_lib.image_rgba8(rgb, abi.TONE_MAPPER_NONE)).all()
+    # distinct operators give distinct images on an HDR film
+    imgs = [oracle_lib.image_rgba8(rgb, op).tobytes() for op in range(6)]
+    assert len(set(imgs)) == 6
