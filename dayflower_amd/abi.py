@@ -15,7 +15,8 @@ ALGO_AMBIENT_OCCLUSION, ALGO_DEPTH_CAMERA, ALGO_PATH_TRACING, ALGO_RAY_CASTING, 
 (TONE_MAPPER_FILMIC_CURVE_ACES_MODIFIED_VERSION_1, TONE_MAPPER_NONE, TONE_MAPPER_REINHARD,
  TONE_MAPPER_REINHARD_MODIFIED_VERSION_1, TONE_MAPPER_REINHARD_MODIFIED_VERSION_2, TONE_MAPPER_UNREAL_3) = range(6)
 SHAPE_TRIANGLE_MESH, SHAPE_SPHERE, SHAPE_RECTANGLE, SHAPE_PLANE = range(4)
-MAT_MATTE, MAT_MIRROR, MAT_GLASS, MAT_METAL, MAT_PLASTIC, MAT_DISNEY = range(6)
+(MAT_MATTE, MAT_MIRROR, MAT_GLASS, MAT_METAL, MAT_PLASTIC, MAT_DISNEY, MAT_GLOSSY, MAT_SUBSTRATE, MAT_TRANSLUCENT,
+ MAT_CLEAR_COAT, MAT_UBER) = range(11)
 LIGHT_DIFFUSE_AREA, LIGHT_POINT, LIGHT_PEREZ = range(3)
 LENS_FISHEYE, LENS_THIN = 0, 1
 

@@ -147,6 +147,104 @@ void compile_material(const dfr_material& m, DevMaterial& out, std::vector<DevLo
             }
             break;
         }
+        case DFR_MAT_GLOSSY: {
+            // TorranceSparrowBRDF(KR, ConductorFresnel(etaI = KR, etaT = 1,
+            // k = 1)): the conductor term only sees etaT / etaI and k / etaI.
+            DevLobe l{}; l.kind = LOBE_TS_R; l.type = BX_GLOSSY | BX_REFL; std::memcpy(l.R, m.kr, 12);
+            l.fresnel = FR_CONDUCTOR;
+            for (int k = 0; k < 3; k++) { l.condEta[k] = 1.0f / m.kr[k]; l.condK[k] = 1.0f / m.kr[k]; }
+            tr(l, m.roughness_u, m.roughness_u, false);
+            add(l);
+            break;
+        }
+        case DFR_MAT_SUBSTRATE: {
+            float kd[3], ks[3]; sat3(kd, m.kd); sat3(ks, m.ks);
+            if (black3(kd) && black3(ks)) { out.has_bsdf = 0; break; }
+            float ru = m.remap_roughness ? roughness_to_alpha(m.roughness_u) : m.roughness_u;
+            float rv = m.remap_roughness ? roughness_to_alpha(m.roughness_v) : m.roughness_v;
+            DevLobe l{}; l.kind = LOBE_FRESNEL_BLEND; l.type = BX_GLOSSY | BX_REFL;
+            std::memcpy(l.R, kd, 12); std::memcpy(l.T, ks, 12); tr(l, ru, rv, false);
+            add(l);
+            break;
+        }
+        case DFR_MAT_TRANSLUCENT: {  // kr: reflectance, kt: transmittance
+            float refl[3], trans[3], kd[3], ks[3]; sat3(refl, m.kr); sat3(trans, m.kt); sat3(kd, m.kd); sat3(ks, m.ks);
+            if (black3(refl) && black3(trans)) { out.has_bsdf = 0; break; }
+            if (!black3(kd)) {
+                if (!black3(refl)) {
+                    DevLobe l{}; l.kind = LOBE_LAMBERT; l.type = BX_DIFF | BX_REFL;
+                    for (int k = 0; k < 3; k++) l.R[k] = kd[k] * refl[k];
+                    add(l);
+                }
+                if (!black3(trans)) {
+                    DevLobe l{}; l.kind = LOBE_LAMBERT_T; l.type = BX_DIFF | BX_TRANS;
+                    for (int k = 0; k < 3; k++) l.T[k] = kd[k] * trans[k];
+                    add(l);
+                }
+            }
+            if (!black3(ks)) {
+                float r = m.remap_roughness ? roughness_to_alpha(m.roughness_u) : m.roughness_u;
+                if (!black3(refl)) {
+                    DevLobe l{}; l.kind = LOBE_TS_R; l.type = BX_GLOSSY | BX_REFL;
+                    for (int k = 0; k < 3; k++) l.R[k] = ks[k] * refl[k];
+                    tr(l, r, r, false); l.fresnel = FR_DIELECTRIC; l.frEtaI = 1.0f; l.frEtaT = 1.5f;
+                    add(l);
+                }
+                if (!black3(trans)) {
+                    DevLobe l{}; l.kind = LOBE_TS_T; l.type = BX_GLOSSY | BX_TRANS;
+                    for (int k = 0; k < 3; k++) l.T[k] = ks[k] * trans[k];
+                    tr(l, r, r, false); l.etaA = 1.0f; l.etaB = 1.5f;
+                    add(l);
+                }
+            }
+            if (out.n_lobes == 0) out.has_bsdf = 0;
+            break;
+        }
+        case DFR_MAT_CLEAR_COAT: {
+            // Both layers; the shade kernel keeps one per hit (clear_coat_layer).
+            DevLobe s{}; s.kind = LOBE_SPECULAR_R; s.type = BX_SPEC | BX_REFL; std::memcpy(s.R, m.ks, 12); s.fresnel = FR_CONST;
+            add(s);
+            DevLobe d{}; d.kind = LOBE_LAMBERT; d.type = BX_DIFF | BX_REFL; std::memcpy(d.R, m.kd, 12);
+            add(d);
+            break;
+        }
+        case DFR_MAT_UBER: {  // k: opacity
+            float op[3], c[3];
+            sat3(op, m.k);
+            float kd[3], kr[3], ks[3], kt[3], ts[3];
+            sat3(c, m.kd); for (int k = 0; k < 3; k++) kd[k] = op[k] * c[k];
+            sat3(c, m.kr); for (int k = 0; k < 3; k++) kr[k] = op[k] * c[k];
+            sat3(c, m.ks); for (int k = 0; k < 3; k++) ks[k] = op[k] * c[k];
+            sat3(c, m.kt); for (int k = 0; k < 3; k++) kt[k] = op[k] * c[k];
+            for (int k = 0; k < 3; k++) ts[k] = std::max(1.0f - op[k], 0.0f);
+            const float eta = m.eta[0];
+            float ru = m.remap_roughness ? roughness_to_alpha(m.roughness_u) : m.roughness_u;
+            float rv = m.remap_roughness ? roughness_to_alpha(m.roughness_v) : m.roughness_v;
+            if (!black3(ts)) {
+                DevLobe l{}; l.kind = LOBE_SPECULAR_T; l.type = BX_SPEC | BX_TRANS; std::memcpy(l.T, ts, 12);
+                l.etaA = 1.0f; l.etaB = 1.0f;
+                add(l);
+            }
+            if (!black3(kd)) { DevLobe l{}; l.kind = LOBE_LAMBERT; l.type = BX_DIFF | BX_REFL; std::memcpy(l.R, kd, 12); add(l); }
+            if (!black3(ks)) {
+                DevLobe l{}; l.kind = LOBE_TS_R; l.type = BX_GLOSSY | BX_REFL; std::memcpy(l.R, ks, 12);
+                tr(l, ru, rv, false); l.fresnel = FR_DIELECTRIC; l.frEtaI = 1.0f; l.frEtaT = eta;
+                add(l);
+            }
+            if (!black3(kr)) {
+                DevLobe l{}; l.kind = LOBE_SPECULAR_R; l.type = BX_SPEC | BX_REFL; std::memcpy(l.R, kr, 12);
+                l.fresnel = FR_DIELECTRIC; l.frEtaI = 1.0f; l.frEtaT = eta;
+                add(l);
+            }
+            if (!black3(kt)) {
+                DevLobe l{}; l.kind = LOBE_SPECULAR_T; l.type = BX_SPEC | BX_TRANS; std::memcpy(l.T, kt, 12);
+                l.etaA = 1.0f; l.etaB = eta;
+                add(l);
+            }
+            // a BSDF with no lobe still ends the path (no pass-through)
+            out.eta = black3(ts) ? eta : 1.0f;
+            break;
+        }
         default: out.has_bsdf = 0; break;
     }
 }
@@ -291,6 +389,7 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
     std::vector<DevMaterial> mats(in.mats.size());
     std::vector<DevLobe> lobes;
     for (size_t m = 0; m < in.mats.size(); m++) compile_material(in.mats[m], mats[m], lobes);
+    for (const dfr_material& m : in.mats) out.ext_materials = out.ext_materials || m.type >= DFR_MAT_GLOSSY;
     std::vector<DevLight> lights(in.lights.size());
     std::vector<DevPerez> perez;
     for (size_t l = 0; l < in.lights.size(); l++) {
@@ -366,6 +465,10 @@ int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err) {
         (d->n_materials && !d->materials) || (d->n_lights && !d->lights)) {
         err = "null array"; return DFR_E_NULL;
     }
+    for (int32_t i = 0; i < d->n_materials; i++)
+        if (d->materials[i].type < DFR_MAT_MATTE || d->materials[i].type > DFR_MAT_UBER) {
+            err = "material " + std::to_string(i) + ": type out of range"; return DFR_E_ARGUMENT;
+        }
     h.positions.assign(d->positions, d->positions + 3 * d->n_vertices);
     if (d->normals) h.normals.assign(d->normals, d->normals + 3 * d->n_vertices); else h.normals.clear();
     h.indices.assign(d->indices, d->indices + 3 * d->n_triangles);
@@ -409,6 +512,11 @@ const DevScene* Ctx::uniform_scene() const {
 
 bool Ctx::any_area_lights() const {
     for (auto& s : scenes) if (s && s->area_lights) return true;
+    return false;
+}
+
+bool Ctx::any_ext_materials() const {
+    for (auto& s : scenes) if (s && s->ext_materials) return true;
     return false;
 }
 
@@ -496,7 +604,8 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
     if ((rc = ensure_overflow(threads))) return rc;
     wf.overflow = (int*)overflow.ptr;
     wavefront_init(wf, (const DevJob*)job_buf.ptr, n_jobs, work_total, stream);
-    WavefrontParams p{wf_params.block, wf_params.sort_keys, (capacity + 255) / 256, any_area_lights(), uniform_scene()};
+    WavefrontParams p{wf_params.block, wf_params.sort_keys, (capacity + 255) / 256, any_area_lights(), uniform_scene(),
+                      any_ext_materials()};
     Counters host{};
     const int poll = 4;
     const int max_iterations = 1 << 20;  // far above max_bounce * (work / capacity + 1)
@@ -517,7 +626,8 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
             (uintptr_t)wf.b.beta, (uintptr_t)wf.b.info, (uintptr_t)wf.hit, (uintptr_t)wf.perm, (uintptr_t)wf.keys,
             (uintptr_t)wf.bhist, (uintptr_t)wf.sh.o, (uintptr_t)wf.sh.d, (uintptr_t)wf.sh.c, (uintptr_t)wf.sh.aux,
             (uintptr_t)wf.overflow, (uintptr_t)wf.ctr, (uintptr_t)wf.job_done, (uintptr_t)wf.capacity,
-            (uintptr_t)p.block, (uintptr_t)p.sort_keys, (uintptr_t)p.closest_shadow, (uintptr_t)p.uscene};
+            (uintptr_t)p.block, (uintptr_t)p.sort_keys, (uintptr_t)p.closest_shadow, (uintptr_t)p.uscene,
+            (uintptr_t)p.ext_materials};
         for (int it = 0;; it += poll) {
             if (it >= max_iterations) return fail(DFR_E_DEVICE, "wavefront did not drain");
             if ((rc = replay_iterations(key, scenes, jobs, n_jobs, film_ptr, p, poll))) return rc;
@@ -828,7 +938,8 @@ int32_t dfr_render(dfr_ctx* h, int32_t passes) {
             e = hipMemcpyAsync(c->job_buf.ptr, &jp, sizeof(DevJob), hipMemcpyHostToDevice, c->stream);
             if (e != hipSuccess) return c->hip_fail(e, "job upload");
             launch_megakernel((const DevScene*)c->scene_table.ptr, (const DevJob*)c->job_buf.ptr, (float4*)c->film.ptr,
-                              (int*)c->overflow.ptr, (int)npix, 256, (int64_t*)c->counters.ptr, c->stream);
+                              (int*)c->overflow.ptr, (int)npix, 256, c->any_ext_materials(), (int64_t*)c->counters.ptr,
+                              c->stream);
             if (c->stop.load()) break;
         }
         hipEventRecord(c->ev1, c->stream);

@@ -71,7 +71,8 @@ constexpr float T_MIN = 0.001f;  // Scene.T_MINIMUM, M/scene/Scene.java:70
 // integer hash is Wellons' "lowbias32".  Dimension layout: 0,1 pixel offset;
 // path-tracing bounce b uses 2 + 12*b + {0 light choice, 1-2 light sample,
 // 3-4 NEE BSDF sample, 5 Perez sun choice, 6-7 BSDF continuation, 8 Russian
-// roulette}; ambient-occlusion sample i uses 2 + 2*i and 3 + 2*i.
+// roulette, 9 clear coat layer choice}; ambient-occlusion sample i uses
+// 2 + 2*i and 3 + 2*i.
 // ---------------------------------------------------------------------------
 DEV uint32_t lowbias32(uint32_t x) {
     x ^= x >> 16; x *= 0x7feb352du;
@@ -746,6 +747,7 @@ DEV float schlick_w(float c) {
     return (m * m) * (m * m) * m;
 }
 DEV float lerpf(float a, float b, float t) { return (1.0f - t) * a + t * b; }
+DEV float pow5(float x) { return x * x * x * x * x; }
 
 DEV V fresnel(const DevLobe& l, float cosi) {
     switch (l.fresnel) {
@@ -833,7 +835,48 @@ DEV float smith_ggx(float c, float alpha) {
     return 1.0f / (c + sqrtf(a + b - a * b));
 }
 
+// Lobes of the glossy, substrate, translucent, clear coat and uber
+// materials; reached only from kernels instantiated with EXT = true, so the
+// kernels of the other materials carry none of their code or registers.
+DEV V lobe_f_ext(const DevLobe& l, V wo, V wi) {
+    switch (l.kind) {
+        case LOBE_FRESNEL_BLEND: {  // diffuse under a Schlick-Fresnel glossy coat
+            V h = wi + wo;
+            if (h.x == 0.0f && h.y == 0.0f && h.z == 0.0f) return mk(0, 0, 0);
+            h = nrm(h);
+            const float ih = dot(wi, h), ci = cos_abs(wi), co = cos_abs(wo);
+            // scalar factors: diffuse 28/(23 pi) (1 - (1 - ci/2)^5) (1 - (1 - co/2)^5),
+            // glossy D / (4 |wi.h| max(ci, co)), Schlick weight (1 - wi.h)^5
+            const float kd = (28.0f / (23.0f * PI_F)) * (1.0f - pow5(1.0f - 0.5f * ci)) * (1.0f - pow5(1.0f - 0.5f * co));
+            const float kg = tr_D(l, h) / (4.0f * fabsf(ih) * fmaxf(ci, co));
+            const float w = pow5(1.0f - ih);
+            V r;
+            r.x = l.R[0] * (1.0f - l.T[0]) * kd + (l.T[0] + (1.0f - l.T[0]) * w) * kg;
+            r.y = l.R[1] * (1.0f - l.T[1]) * kd + (l.T[1] + (1.0f - l.T[1]) * w) * kg;
+            r.z = l.R[2] * (1.0f - l.T[2]) * kd + (l.T[2] + (1.0f - l.T[2]) * w) * kg;
+            return r;
+        }
+        case LOBE_LAMBERT_T: return ld3(l.T) * INV_PI;
+        default: return mk(0, 0, 0);
+    }
+}
+DEV float lobe_pdf_ext(const DevLobe& l, V wo, V wi) {
+    switch (l.kind) {
+        case LOBE_FRESNEL_BLEND: {  // half cosine-weighted, half microfacet
+            if (!same_hemi(wo, wi)) return 0.0f;
+            V h = nrm(wo + wi);
+            return 0.5f * (cos_abs(wi) * INV_PI + tr_pdf(l, wo, h) / (4.0f * dot(wo, h)));
+        }
+        case LOBE_LAMBERT_T: return same_hemi(wo, wi) ? 0.0f : cos_abs(wi) * INV_PI;
+        default: return 0.0f;
+    }
+}
+
+template <bool EXT>
 DEV V lobe_f(const DevLobe& l, V wo, V wi) {
+    if constexpr (EXT) {
+        if (l.kind >= LOBE_FRESNEL_BLEND) return lobe_f_ext(l, wo, wi);
+    }
     switch (l.kind) {
         case LOBE_LAMBERT: return ld3(l.R) * INV_PI;
         case LOBE_OREN_NAYAR: {
@@ -904,7 +947,11 @@ DEV V lobe_f(const DevLobe& l, V wo, V wi) {
     }
 }
 
+template <bool EXT>
 DEV float lobe_pdf(const DevLobe& l, V wo, V wi) {
+    if constexpr (EXT) {
+        if (l.kind >= LOBE_FRESNEL_BLEND) return lobe_pdf_ext(l, wo, wi);
+    }
     switch (l.kind) {
         case LOBE_LAMBERT: case LOBE_OREN_NAYAR: case LOBE_DISNEY_DIFFUSE: case LOBE_DISNEY_RETRO:
         case LOBE_DISNEY_SHEEN:
@@ -955,13 +1002,51 @@ DEV V hemi_cos(float u, float v) {
 
 struct LSample { V f; V wi; float pdf; uint32_t type; };
 
+DEV bool lobe_sample_ext(const DevLobe& l, V wo, float u, float v, LSample& out) {
+    out.type = l.type;
+    switch (l.kind) {
+        case LOBE_FRESNEL_BLEND: {
+            V wi;
+            if (u < 0.5f) {
+                wi = face_fwd_z(wo, hemi_cos(fminf(2.0f * u, 0.99999994f), v));
+            } else {
+                V h = tr_sample_h(l, wo, fminf(2.0f * (u - 0.5f), 0.99999994f), v);
+                wi = reflect_pbrt(wo, h);
+                if (!same_hemi(wo, wi)) return false;
+            }
+            out.wi = wi; out.f = lobe_f_ext(l, wo, wi); out.pdf = lobe_pdf_ext(l, wo, wi);
+            return true;
+        }
+        case LOBE_LAMBERT_T: {  // cosine sample on the side opposite wo
+            V s = hemi_cos(u, v);
+            V wi = wo.z > 0.0f ? mk(s.x, s.y, -s.z) : s;
+            out.wi = wi; out.f = lobe_f_ext(l, wo, wi); out.pdf = lobe_pdf_ext(l, wo, wi);
+            return true;
+        }
+        case LOBE_SPECULAR_T: {
+            bool entering = cos_t(wo) > 0.0f;
+            float ei = entering ? l.etaA : l.etaB, et = entering ? l.etaB : l.etaA;
+            V wi;
+            if (!refract(wo, face_fwd(mk(0, 0, 1), wo), ei / et, wi)) return false;
+            float F = fr_dielectric(cos_t(wi), l.etaA, l.etaB);
+            out.wi = wi; out.f = ld3(l.T) * (1.0f - F) * ((ei * ei) / (et * et)) / cos_abs(wi); out.pdf = 1.0f;
+            return true;
+        }
+        default: return false;
+    }
+}
+
+template <bool EXT>
 DEV bool lobe_sample(const DevLobe& l, V wo, float u, float v, LSample& out) {
+    if constexpr (EXT) {
+        if (l.kind >= LOBE_FRESNEL_BLEND) return lobe_sample_ext(l, wo, u, v, out);
+    }
     out.type = l.type;
     switch (l.kind) {
         case LOBE_LAMBERT: case LOBE_OREN_NAYAR: case LOBE_DISNEY_DIFFUSE: case LOBE_DISNEY_RETRO:
         case LOBE_DISNEY_SHEEN: {
             V wi = face_fwd_z(wo, hemi_cos(u, v));
-            out.wi = wi; out.f = lobe_f(l, wo, wi); out.pdf = lobe_pdf(l, wo, wi);
+            out.wi = wi; out.f = lobe_f<false>(l, wo, wi); out.pdf = lobe_pdf<false>(l, wo, wi);
             return true;
         }
         case LOBE_SPECULAR_R: {
@@ -993,7 +1078,7 @@ DEV bool lobe_sample(const DevLobe& l, V wo, float u, float v, LSample& out) {
             if (dot(wo, h) < 0.0f) return false;
             V wi = reflect_pbrt(wo, h);
             if (!same_hemi(wo, wi)) return false;
-            out.wi = wi; out.f = lobe_f(l, wo, wi); out.pdf = tr_pdf(l, wo, h) / (4.0f * dot(wo, h));
+            out.wi = wi; out.f = lobe_f<false>(l, wo, wi); out.pdf = tr_pdf(l, wo, h) / (4.0f * dot(wo, h));
             return true;
         }
         case LOBE_TS_T: {
@@ -1003,7 +1088,7 @@ DEV bool lobe_sample(const DevLobe& l, V wo, float u, float v, LSample& out) {
             float eta = cos_t(wo) > 0.0f ? l.etaA / l.etaB : l.etaB / l.etaA;
             V wi;
             if (!refract(wo, h, eta, wi)) return false;
-            out.wi = wi; out.f = lobe_f(l, wo, wi); out.pdf = lobe_pdf(l, wo, wi);
+            out.wi = wi; out.f = lobe_f<false>(l, wo, wi); out.pdf = lobe_pdf<false>(l, wo, wi);
             return true;
         }
         case LOBE_DISNEY_CLEARCOAT: {
@@ -1016,7 +1101,7 @@ DEV bool lobe_sample(const DevLobe& l, V wo, float u, float v, LSample& out) {
             V h = same_hemi(wo, hs) ? hs : -hs;
             V wi = reflect_pbrt(wo, h);
             if (!same_hemi(wo, wi)) return false;
-            out.wi = wi; out.f = lobe_f(l, wo, wi); out.pdf = lobe_pdf(l, wo, wi);
+            out.wi = wi; out.f = lobe_f<false>(l, wo, wi); out.pdf = lobe_pdf<false>(l, wo, wi);
             return true;
         }
         default: return false;
@@ -1031,7 +1116,33 @@ struct Bsdf {
     ONB s;
     V ng, wo_w, wo;
 };
-DEV Bsdf make_bsdf(const DevScene& sc, const DevMaterial& m, const Surf& si, V ray_d) {
+// ClearCoatMaterial.java:292-333: the material picks one layer per hit by
+// Russian roulette on the dielectric (1 -> 1.5) reflectance of the shading
+// normal; its compiled lobes are [specular KS, Lambertian KD].  Total
+// internal reflection leaves the specular layer at weight 1.  Returns the
+// layer's weight, which scales every value of f; the callers fold it into
+// the path throughput, which multiplies both NEE and the continuation.
+DEV float clear_coat_layer(Bsdf& b, const Surf& si, V d, float u) {
+    const V n = si.s.w;
+    const V nco = dot(n, d) > 0.0f ? -n : n;
+    const bool entering = dot(n, nco) > 0.0f;
+    const float eta = entering ? 1.0f / 1.5f : 1.5f;
+    const float ci = dot(d, nco);
+    const float s2t = eta * eta * fmaxf(0.0f, 1.0f - ci * ci);
+    b.n = 1;
+    if (s2t >= 1.0f) return 1.0f;
+    const float ct = sqrtf(1.0f - s2t);
+    const V refr = (-d) * eta + nco * (eta * ci - ct);
+    const float refl = fr_dielectric(entering ? -ci : dot(refr, n), 1.0f, 1.5f);
+    const float prr = 0.25f + 0.5f * refl;
+    if (u < prr) return refl / prr;
+    b.lobes += 1;
+    return (1.0f - refl) / (1.0f - prr);
+}
+// `beta` is the path throughput at the hit (after its emission is added).
+template <bool EXT>
+DEV Bsdf make_bsdf(const DevScene& sc, const DevMaterial& m, const Surf& si, V ray_d, const Rng& rng, uint32_t dim,
+                   V& beta) {
     Bsdf b;
     b.lobes = sc.lobes + m.first_lobe;
     b.n = m.n_lobes;
@@ -1039,6 +1150,9 @@ DEV Bsdf make_bsdf(const DevScene& sc, const DevMaterial& m, const Surf& si, V r
     b.s = si.s; b.ng = si.g.w;
     b.wo_w = -ray_d;
     b.wo = to_local(b.wo_w, si.s);
+    if constexpr (EXT) {
+        if (m.type == DFR_MAT_CLEAR_COAT) beta = beta * clear_coat_layer(b, si, ray_d, rng.get(dim + 9));
+    }
     return b;
 }
 DEV bool matches(uint32_t lt, uint32_t t) { return (lt & t) == lt; }
@@ -1047,6 +1161,7 @@ DEV int count_nonspecular(const Bsdf& b) {
     for (int i = 0; i < b.n; i++) if (!(b.lobes[i].type & BX_SPEC)) c++;
     return c;
 }
+template <bool EXT>
 DEV V bsdf_f(const Bsdf& b, uint32_t t, V wi_w) {
     V wi = to_local(wi_w, b.s);
     bool refl = dot(wi_w, b.ng) * dot(b.wo_w, b.ng) > 0.0f;
@@ -1054,10 +1169,11 @@ DEV V bsdf_f(const Bsdf& b, uint32_t t, V wi_w) {
     for (int i = 0; i < b.n; i++) {
         const DevLobe& l = b.lobes[i];
         if (matches(l.type, t) && ((refl && (l.type & BX_REFL)) || (!refl && (l.type & BX_TRANS))))
-            r = r + lobe_f(l, b.wo, wi);
+            r = r + lobe_f<EXT>(l, b.wo, wi);
     }
     return r;
 }
+template <bool EXT>
 DEV float bsdf_pdf(const Bsdf& b, uint32_t t, V wi_w) {
     if (b.n == 0) return 0.0f;
     V wi = to_local(wi_w, b.s);
@@ -1065,11 +1181,12 @@ DEV float bsdf_pdf(const Bsdf& b, uint32_t t, V wi_w) {
     float p = 0.0f; int m = 0;
     for (int i = 0; i < b.n; i++) {
         const DevLobe& l = b.lobes[i];
-        if (matches(l.type, t)) { m++; p += lobe_pdf(l, b.wo, wi); }
+        if (matches(l.type, t)) { m++; p += lobe_pdf<EXT>(l, b.wo, wi); }
     }
     if (m > 1) p /= m;
     return p;
 }
+template <bool EXT>
 DEV bool bsdf_sample(const Bsdf& b, uint32_t t, float u, float v, LSample& out) {
     if (b.wo.z == 0.0f) return false;
     int m = 0;
@@ -1082,7 +1199,7 @@ DEV bool bsdf_sample(const Bsdf& b, uint32_t t, float u, float v, LSample& out) 
     const DevLobe& L = b.lobes[sel];
     float ur = fminf(u * m - k, 0.99999994f);
     LSample ls;
-    if (!lobe_sample(L, b.wo, ur, v, ls)) return false;
+    if (!lobe_sample<EXT>(L, b.wo, ur, v, ls)) return false;
     V wi_w = to_world(ls.wi, b.s);
     V f = ls.f;
     float pdf = ls.pdf;
@@ -1092,8 +1209,8 @@ DEV bool bsdf_sample(const Bsdf& b, uint32_t t, float u, float v, LSample& out) 
         for (int i = 0; i < b.n; i++) {
             const DevLobe& B = b.lobes[i];
             if (!matches(B.type, t)) continue;
-            if (i != sel) pdf += lobe_pdf(B, b.wo, ls.wi);
-            if ((refl && (B.type & BX_REFL)) || (!refl && (B.type & BX_TRANS))) f = f + lobe_f(B, b.wo, ls.wi);
+            if (i != sel) pdf += lobe_pdf<EXT>(B, b.wo, ls.wi);
+            if ((refl && (B.type & BX_REFL)) || (!refl && (B.type & BX_TRANS))) f = f + lobe_f<EXT>(B, b.wo, ls.wi);
         }
     }
     if (m > 1) pdf /= m;

@@ -21,6 +21,9 @@ enum LobeKind : int32_t {
     LOBE_DISNEY_RETRO = 7,
     LOBE_DISNEY_SHEEN = 8,
     LOBE_DISNEY_CLEARCOAT = 9,
+    LOBE_FRESNEL_BLEND = 10,   // FresnelBlendBRDF (R = diffuse, T = specular scale)
+    LOBE_LAMBERT_T = 11,       // LambertianBTDF
+    LOBE_SPECULAR_T = 12,      // SpecularBTDF (dielectric etaA -> etaB)
 };
 
 // BXDFType bit flags (M/scene/BXDFType.java).

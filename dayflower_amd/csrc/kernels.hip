@@ -119,6 +119,7 @@ struct ShadowReq {
 // Fills `ls_req` with the light-sample request (mode 0 or 1) and `bs_req`
 // with the BSDF-sample request (mode 2); a slot without a request keeps
 // mode -1.  Two named slots (not an array) keep both in registers.
+template <bool EXT>
 DEV void nee_requests(const DevScene& sc, const Bsdf& b, const Surf& si, int li, const Rng& rng, uint32_t dim,
                       V scale, ShadowReq& ls_req, ShadowReq& bs_req) {
     ls_req.mode = -1;
@@ -130,7 +131,7 @@ DEV void nee_requests(const DevScene& sc, const Bsdf& b, const Surf& si, int li,
     V ns = si.s.w;
     if (L.type == DFR_LIGHT_POINT) {
         if (have && !black(ls.L) && ls.pdf > 0.0f) {
-            V f = bsdf_f(b, T, ls.wi) * fabsf(dot(ls.wi, ns));
+            V f = bsdf_f<EXT>(b, T, ls.wi) * fabsf(dot(ls.wi, ns));
             if (!black(f)) {
                 ShadowReq& r = ls_req;
                 r.o = si.p; r.d = nrm(ls.p - si.p); r.tmax = fabsf(len(si.p - ls.p)) + 0.001f;
@@ -140,8 +141,8 @@ DEV void nee_requests(const DevScene& sc, const Bsdf& b, const Surf& si, int li,
         return;
     }
     if (have && !black(ls.L) && ls.pdf > 0.0f) {
-        V f = bsdf_f(b, T, ls.wi) * fabsf(dot(ls.wi, ns));
-        float sp = bsdf_pdf(b, T, ls.wi);
+        V f = bsdf_f<EXT>(b, T, ls.wi) * fabsf(dot(ls.wi, ns));
+        float sp = bsdf_pdf<EXT>(b, T, ls.wi);
         if (!black(f)) {
             float w = power_heuristic(ls.pdf, sp);
             ShadowReq& r = ls_req;
@@ -151,7 +152,7 @@ DEV void nee_requests(const DevScene& sc, const Bsdf& b, const Surf& si, int li,
         }
     }
     LSample bs;
-    if (bsdf_sample(b, T, rng.get(dim + 3), rng.get(dim + 4), bs)) {
+    if (bsdf_sample<EXT>(b, T, rng.get(dim + 3), rng.get(dim + 4), bs)) {
         V f = bs.f * fabsf(dot(bs.wi, ns));
         bool spec = (bs.type & BX_SPEC) != 0;
         if (!black(f) && bs.pdf > 0.0f) {
@@ -208,7 +209,7 @@ DEV V resolve_shadow(const DevScene& sc, const ShadowReq& r, TraversalStack<SHOR
 // ---------------------------------------------------------------------------
 // Plain megakernel (one work-item per pixel)
 // ---------------------------------------------------------------------------
-template <int SHORT>
+template <bool EXT, int SHORT>
 DEV V radiance_pt(const DevScene& sc, V o, V d, const Rng& rng, int max_bounce, int min_rr,
                   TraversalStack<SHORT>& st, int64_t* cnt) {
     V L = mk(0, 0, 0), beta = mk(1, 1, 1);
@@ -231,13 +232,13 @@ DEV V radiance_pt(const DevScene& sc, V o, V d, const Rng& rng, int max_bounce, 
                 if (++pass > 256) break;
                 continue;
             }
-            Bsdf b = make_bsdf(sc, m, si, d);
             const uint32_t dim = DIM_BOUNCE + DIMS_PER_BOUNCE * (uint32_t)bounce;
+            Bsdf b = make_bsdf<EXT>(sc, m, si, d, rng, dim, beta);
             if (sc.n_lights > 0 && count_nonspecular(b) > 0) {
                 int n = sc.n_lights;
                 int li = min((int)(rng.get(dim) * n), n - 1);
                 ShadowReq r0, r1;
-                nee_requests(sc, b, si, li, rng, dim, beta * (float)n, r0, r1);
+                nee_requests<EXT>(sc, b, si, li, rng, dim, beta * (float)n, r0, r1);
                 VisitCount vc;
                 if (r0.mode >= 0) {
                     if (r0.mode == 0) n_any++; else n_closest++;
@@ -249,7 +250,7 @@ DEV V radiance_pt(const DevScene& sc, V o, V d, const Rng& rng, int max_bounce, 
                 }
             }
             LSample bs;
-            if (!bsdf_sample(b, BX_ALL, rng.get(dim + 6), rng.get(dim + 7), bs)) break;
+            if (!bsdf_sample<EXT>(b, BX_ALL, rng.get(dim + 6), rng.get(dim + 7), bs)) break;
             if (black(bs.f) || bs.pdf == 0.0f) break;
             beta = beta * (bs.f * fabsf(dot(bs.wi, si.s.w)) / bs.pdf);
             spec = (bs.type & BX_SPEC) != 0;
@@ -322,7 +323,7 @@ DEV V radiance_rc(const DevScene& sc, V o, V d, TraversalStack<SHORT>& st, int64
     return L;
 }
 
-template <int BLOCK, int SHORT>
+template <int BLOCK, int SHORT, bool EXT>
 __global__ __launch_bounds__(BLOCK) void k_megakernel(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobp,
                                                       float4* __restrict__ film, int* __restrict__ overflow, int n_pixels,
                                                       unsigned long long* __restrict__ counters) {
@@ -343,7 +344,7 @@ __global__ __launch_bounds__(BLOCK) void k_megakernel(const DevScene* __restrict
             V o, d;
             if (!primary_ray(job.cam, (float)x, (float)y, rng.get(0), rng.get(1), o, d)) continue;
             V L;
-            if (job.algorithm == DFR_ALGO_PATH_TRACING) L = radiance_pt<SHORT>(sc, o, d, rng, job.max_bounce, job.min_rr, st, cnt);
+            if (job.algorithm == DFR_ALGO_PATH_TRACING) L = radiance_pt<EXT, SHORT>(sc, o, d, rng, job.max_bounce, job.min_rr, st, cnt);
             else if (job.algorithm == DFR_ALGO_AMBIENT_OCCLUSION) L = radiance_ao<SHORT>(sc, o, d, rng, job.max_distance, job.ao_samples, st, cnt);
             else if (job.algorithm == DFR_ALGO_RAY_CASTING) L = radiance_rc<SHORT>(sc, o, d, st, cnt);
             else L = mk(0, 0, 0);
@@ -374,15 +375,21 @@ int overflow_entries_per_thread(int max_depth) {
     return need > 1 ? need : 1;
 }
 
-void launch_megakernel(const DevScene* scenes, const DevJob* job, float4* film, int* overflow, int n_pixels, int block,
-                       int64_t* counters, hipStream_t s) {
+template <bool EXT>
+static void launch_megakernel_t(const DevScene* scenes, const DevJob* job, float4* film, int* overflow, int n_pixels,
+                                int block, unsigned long long* c, hipStream_t s) {
     int grid = (n_pixels + block - 1) / block;
-    auto c = reinterpret_cast<unsigned long long*>(counters);
     switch (block) {
-        case 64: hipLaunchKernelGGL((k_megakernel<64, SHORT_STACK>), dim3(grid), dim3(64), 0, s, scenes, job, film, overflow, n_pixels, c); break;
-        case 128: hipLaunchKernelGGL((k_megakernel<128, SHORT_STACK>), dim3(grid), dim3(128), 0, s, scenes, job, film, overflow, n_pixels, c); break;
-        default: hipLaunchKernelGGL((k_megakernel<256, SHORT_STACK>), dim3((n_pixels + 255) / 256), dim3(256), 0, s, scenes, job, film, overflow, n_pixels, c); break;
+        case 64: hipLaunchKernelGGL((k_megakernel<64, SHORT_STACK, EXT>), dim3(grid), dim3(64), 0, s, scenes, job, film, overflow, n_pixels, c); break;
+        case 128: hipLaunchKernelGGL((k_megakernel<128, SHORT_STACK, EXT>), dim3(grid), dim3(128), 0, s, scenes, job, film, overflow, n_pixels, c); break;
+        default: hipLaunchKernelGGL((k_megakernel<256, SHORT_STACK, EXT>), dim3((n_pixels + 255) / 256), dim3(256), 0, s, scenes, job, film, overflow, n_pixels, c); break;
     }
+}
+void launch_megakernel(const DevScene* scenes, const DevJob* job, float4* film, int* overflow, int n_pixels, int block,
+                       bool ext_materials, int64_t* counters, hipStream_t s) {
+    auto c = reinterpret_cast<unsigned long long*>(counters);
+    if (ext_materials) launch_megakernel_t<true>(scenes, job, film, overflow, n_pixels, block, c, s);
+    else launch_megakernel_t<false>(scenes, job, film, overflow, n_pixels, block, c, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -559,11 +566,11 @@ __global__ void k_commit(Counters* ctr, int capacity) {
     ctr->n_shadow2 = 0;
 }
 
-// Material-sort key of a hit: material type of the hit (0..5), 6 = miss,
-// 7 = dropped camera sample.
-// Sort key of a hit: material type (0..5) x the type of the light its NEE
-// will sample (0..2; 3 with no lights), so that a wave of the shade kernel
-// runs one material and one light-sampling routine.  The light is chosen by
+// Sort key of a hit: material class x the type of the light its NEE will
+// sample (0..2; 3 with no lights), so that a wave of the shade kernel runs
+// one material and one light-sampling routine.  The class is the material
+// type for matte .. Disney (0..5); glossy, substrate, translucent, clear
+// coat and uber share class 6.  The light is chosen by
 // the same stateless random number the shade kernel draws.  30 = miss,
 // 31 = dropped camera sample.
 constexpr int NSORT_KEYS = 32;
@@ -571,7 +578,7 @@ constexpr int NSORT_KEYS = 32;
 DEV int sort_key_in(const DevScene& sc, int prim, uint32_t info_w) {
     if (prim == -2) return 31;
     if (prim < 0) return 30;
-    return 4 * sc.mats[sc.prims[prim].material].type + (int)((info_w >> INFO_LT_SHIFT) & 3u);
+    return 4 * min(sc.mats[sc.prims[prim].material].type, 6) + (int)((info_w >> INFO_LT_SHIFT) & 3u);
 }
 DEV int sort_key(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, uint4 info, float4 h) {
     const int prim = __float_as_int(h.y);
@@ -865,7 +872,7 @@ DEV void put_shadow(ShadowSoA sh, int s, const ShadowReq& R, int64_t film_idx, i
 // device-wide serial points, so fewer, larger blocks shade faster (512:
 // -6 % against 256; 1024 loses more at the block barriers than it saves).
 constexpr int SHADE_BLOCK = 512;
-template <int BLOCK>
+template <int BLOCK, bool EXT>
 __global__ __launch_bounds__(BLOCK, 4) void k_shade(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
                                                  PathSoA q, const float4* __restrict__ hits,
                                                  const int32_t* __restrict__ perm, PathSoA out, ShadowSoA sh,
@@ -913,15 +920,15 @@ __global__ __launch_bounds__(BLOCK, 4) void k_shade(const DevScene* __restrict__
                                                                         (info.w & (3u << INFO_LT_SHIFT)));
                         }
                     } else {
-                        Bsdf b = make_bsdf(sc, m, si, d);
                         const uint32_t dim = DIM_BOUNCE + DIMS_PER_BOUNCE * (uint32_t)bounce;
+                        Bsdf b = make_bsdf<EXT>(sc, m, si, d, rng, dim, beta);
                         if (sc.n_lights > 0 && count_nonspecular(b) > 0) {
                             int nl = sc.n_lights;
                             int li = min((int)(rng.get(dim) * nl), nl - 1);
-                            nee_requests(sc, b, si, li, rng, dim, beta * (float)nl, r0, r1);
+                            nee_requests<EXT>(sc, b, si, li, rng, dim, beta * (float)nl, r0, r1);
                         }
                         LSample bs;
-                        if (bsdf_sample(b, BX_ALL, rng.get(dim + 6), rng.get(dim + 7), bs) && !black(bs.f) && bs.pdf != 0.0f) {
+                        if (bsdf_sample<EXT>(b, BX_ALL, rng.get(dim + 6), rng.get(dim + 7), bs) && !black(bs.f) && bs.pdf != 0.0f) {
                             V nb = beta * (bs.f * fabsf(dot(bs.wi, si.s.w)) / bs.pdf);
                             bool nspec = (bs.type & BX_SPEC) != 0;
                             float ne = eta_scale;
@@ -1052,8 +1059,12 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     if (timing) hipEventRecord(prof->ev[2], s);
     // shading in 512-thread blocks whatever the traversal block: one queue
     // reservation (two same-address atomics) per 512 paths
-    hipLaunchKernelGGL((k_shade<SHADE_BLOCK>), dim3((cap + SHADE_BLOCK - 1) / SHADE_BLOCK), dim3(SHADE_BLOCK), 0, s, scenes,
-                       jobs, w.a, w.hit, w.perm, w.b, w.sh, w.ctr, film, use_perm, w.job_done, cap);
+    if (p.ext_materials)
+        hipLaunchKernelGGL((k_shade<SHADE_BLOCK, true>), dim3((cap + SHADE_BLOCK - 1) / SHADE_BLOCK), dim3(SHADE_BLOCK), 0, s,
+                           scenes, jobs, w.a, w.hit, w.perm, w.b, w.sh, w.ctr, film, use_perm, w.job_done, cap);
+    else
+        hipLaunchKernelGGL((k_shade<SHADE_BLOCK, false>), dim3((cap + SHADE_BLOCK - 1) / SHADE_BLOCK), dim3(SHADE_BLOCK), 0, s,
+                           scenes, jobs, w.a, w.hit, w.perm, w.b, w.sh, w.ctr, film, use_perm, w.job_done, cap);
     if (timing) hipEventRecord(prof->ev[3], s);
     hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(1), 0, s, w.ctr);
     if (timing) hipEventRecord(prof->ev[4], s);

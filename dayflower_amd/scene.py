@@ -112,6 +112,39 @@ def disney(color=(0.8, 0.5, 0.3), metallic=0.0, roughness=0.5, specular_tint=0.0
                 disney_clearcoat_gloss=float(clearcoat_gloss), disney_spec_trans=float(spec_trans))
 
 
+def glossy(kr=(0.5, 0.5, 0.5), roughness=0.2):
+    """GlossyMaterial: TorranceSparrowBRDF(KR) with ConductorFresnel(KR, 1, 1);
+    the roughness is used as the microfacet alpha (no remapping)."""
+    return _mat(abi.MAT_GLOSSY, kr=kr, roughness_u=float(roughness), roughness_v=float(roughness))
+
+
+def substrate(kd=(0.5, 0.5, 0.5), ks=(0.5, 0.5, 0.5), roughness=0.1, remap=True):
+    """SubstrateMaterial: FresnelBlendBRDF(KD, KS)."""
+    return _mat(abi.MAT_SUBSTRATE, kd=kd, ks=ks, roughness_u=float(roughness), roughness_v=float(roughness),
+                remap_roughness=int(remap))
+
+
+def translucent(kd=(0.25, 0.25, 0.25), ks=(0.25, 0.25, 0.25), reflectance=(0.5, 0.5, 0.5),
+                transmittance=(0.5, 0.5, 0.5), roughness=0.1, remap=True):
+    """TranslucentMaterial: Lambertian BRDF/BTDF and Torrance-Sparrow
+    BRDF/BTDF (1 -> 1.5); reflectance and transmittance travel in kr, kt."""
+    return _mat(abi.MAT_TRANSLUCENT, kd=kd, ks=ks, kr=reflectance, kt=transmittance, roughness_u=float(roughness),
+                roughness_v=float(roughness), remap_roughness=int(remap))
+
+
+def clear_coat(kd=(0.5, 0.5, 0.5), ks=(1.0, 1.0, 1.0)):
+    """ClearCoatMaterial: per hit, a specular (KS) or Lambertian (KD) layer
+    chosen by Russian roulette on the 1 -> 1.5 dielectric reflectance."""
+    return _mat(abi.MAT_CLEAR_COAT, kd=kd, ks=ks)
+
+
+def uber(kd=(0.25, 0.25, 0.25), ks=(0.25, 0.25, 0.25), kr=(0.0, 0.0, 0.0), kt=(0.0, 0.0, 0.0),
+         opacity=(1.0, 1.0, 1.0), eta=1.5, roughness=0.1, remap=True):
+    """UberMaterial; the opacity travels in k."""
+    return _mat(abi.MAT_UBER, kd=kd, ks=ks, kr=kr, kt=kt, k=opacity, eta=(eta, 0, 0), roughness_u=float(roughness),
+                roughness_v=float(roughness), remap_roughness=int(remap))
+
+
 class Scene:
     """World-space scene; see the module docstring."""
 

@@ -24,15 +24,19 @@ SERVE_RES = (256, 512, 1024)
 SERVE_SPP = (4, 16, 64)
 
 
-def mixed_material_icosphere(n_triangles, seed, with_perez=False, width=512, height=512):
+def mixed_material_icosphere(n_triangles, seed, with_perez=False, width=512, height=512, materials=None,
+                             ground=None):
     """Displaced icosphere split into four contiguous patches (five
-    icosahedron faces each) of Matte, Glass, Metal and Disney, on a Matte
-    ground plane, lit by a point light (and optionally the Perez sky)."""
+    icosahedron faces each) of Matte, Glass, Metal and Disney (or the four
+    `materials` given), on a Matte ground plane (or `ground`), lit by a point
+    light (and optionally the Perez sky)."""
     sc = S.Scene()
-    mats = [sc.add_material(S.matte((0.7, 0.6, 0.5))), sc.add_material(S.glass(1.5, roughness=0.0)),
-            sc.add_material(S.metal(roughness=0.25)),
-            sc.add_material(S.disney((0.3, 0.5, 0.8), metallic=0.2, roughness=0.4, clearcoat=0.5, sheen=0.3))]
-    ground = sc.add_material(S.matte((0.5, 0.5, 0.5)))
+    if materials is None:
+        materials = [S.matte((0.7, 0.6, 0.5)), S.glass(1.5, roughness=0.0), S.metal(roughness=0.25),
+                     S.disney((0.3, 0.5, 0.8), metallic=0.2, roughness=0.4, clearcoat=0.5, sheen=0.3)]
+    assert len(materials) == 4
+    mats = [sc.add_material(m) for m in materials]
+    ground = sc.add_material(ground if ground is not None else S.matte((0.5, 0.5, 0.5)))
     p, t, nrm = S.displaced_icosphere(n_triangles, seed)
     nt = t.shape[0]
     cuts = [0, nt // 4, nt // 2, 3 * nt // 4, nt]
@@ -101,3 +105,16 @@ def request_stream(scene_id, seed=0x5EED, rate=200.0, duration=60.0, max_bounce=
         out.append((t, make_job(scene_id, cam, res, res, spp, seed=(seed << 16) ^ i, maximum_bounce=max_bounce)))
         i += 1
     return out
+
+
+def more_materials_icosphere(n_triangles, seed, with_perez=False, width=512, height=512):
+    """The four-patch icosphere in Glossy, Substrate, Translucent and Uber
+    (partly transparent) on a ClearCoat ground plane."""
+    return mixed_material_icosphere(
+        n_triangles, seed, with_perez=with_perez, width=width, height=height,
+        materials=[S.glossy((0.8, 0.6, 0.3), roughness=0.15),
+                   S.substrate((0.5, 0.3, 0.2), (0.2, 0.2, 0.2), roughness=0.1),
+                   S.translucent((0.6, 0.6, 0.4), (0.3, 0.3, 0.3), (0.5, 0.5, 0.5), (0.5, 0.5, 0.5), roughness=0.2),
+                   S.uber((0.5, 0.4, 0.3), (0.2, 0.2, 0.2), kr=(0.1, 0.1, 0.1), kt=(0.1, 0.1, 0.1),
+                          opacity=(0.8, 0.7, 0.8), eta=1.5, roughness=0.15)],
+        ground=S.clear_coat((0.4, 0.4, 0.5), (1.0, 1.0, 1.0)))

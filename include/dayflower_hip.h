@@ -63,13 +63,19 @@ extern "C" {
 #define DFR_SHAPE_RECTANGLE     2
 #define DFR_SHAPE_PLANE         3
 
-/* Materials: M/scene/material/{Matte,Mirror,Glass,Metal,Plastic,Disney}Material.java */
-#define DFR_MAT_MATTE   0
-#define DFR_MAT_MIRROR  1
-#define DFR_MAT_GLASS   2
-#define DFR_MAT_METAL   3
-#define DFR_MAT_PLASTIC 4
-#define DFR_MAT_DISNEY  5
+/* Materials: M/scene/material/{Matte,Mirror,Glass,Metal,Plastic,Disney,
+ * Glossy,Substrate,Translucent,ClearCoat,Uber}Material.java */
+#define DFR_MAT_MATTE       0
+#define DFR_MAT_MIRROR      1
+#define DFR_MAT_GLASS       2
+#define DFR_MAT_METAL       3
+#define DFR_MAT_PLASTIC     4
+#define DFR_MAT_DISNEY      5
+#define DFR_MAT_GLOSSY      6
+#define DFR_MAT_SUBSTRATE   7
+#define DFR_MAT_TRANSLUCENT 8
+#define DFR_MAT_CLEAR_COAT  9
+#define DFR_MAT_UBER       10
 
 /* Lights: M/scene/light/{DiffuseAreaLight,PointLight,PerezLight}.java */
 #define DFR_LIGHT_DIFFUSE_AREA 0
@@ -106,6 +112,13 @@ extern "C" {
  *   METAL   eta[0..2], k[0..2], roughness_u/v, remap_roughness, emission
  *   PLASTIC kd, ks, roughness_u, remap_roughness, emission
  *   DISNEY  kd (base color), the disney_* scalars, eta[0], thin, emission
+ *   GLOSSY      kr, roughness_u (used as is), emission
+ *   SUBSTRATE   kd, ks, roughness_u/v, remap_roughness, emission
+ *   TRANSLUCENT kd, ks, kr (reflectance), kt (transmittance), roughness_u,
+ *               remap_roughness, emission
+ *   CLEAR_COAT  kd, ks, emission
+ *   UBER        kd, ks, kr, kt, k (opacity), eta[0], roughness_u/v,
+ *               remap_roughness, emission
  * Record size is fixed at 128 bytes. */
 typedef struct dfr_material {
     int32_t type;

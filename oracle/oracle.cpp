@@ -46,7 +46,8 @@ inline Rng makeRng(uint64_t seed, uint32_t pixel, uint32_t sample) {
 }
 // Dimension layout: 0,1 pixel offset; path-tracing bounce b uses
 // 2 + 12*b + {0 light choice, 1-2 light sample, 3-4 NEE BSDF sample,
-// 5 Perez sun choice, 6-7 BSDF continuation, 8 Russian roulette}.
+// 5 Perez sun choice, 6-7 BSDF continuation, 8 Russian roulette,
+// 9 clear coat layer choice}.
 // Ambient occlusion sample i uses 2 + 2*i, 3 + 2*i.
 const uint32_t DIM_BOUNCE = 2, DIMS_PER_BOUNCE = 12;
 
@@ -628,7 +629,8 @@ float roughnessToAlpha(float r) {
 enum : uint32_t { F_REFL = 1, F_TRANS = 2, F_DIFF = 4, F_GLOSSY = 8, F_SPEC = 16 };
 const uint32_t ALL = 31, ALL_EXCEPT_SPECULAR = 15;
 enum LobeKind { L_LAMBERT, L_OREN, L_SPEC_R, L_FRESNEL_SPEC, L_TS_R, L_TS_T,
-                L_DISNEY_DIFFUSE, L_DISNEY_RETRO, L_DISNEY_SHEEN, L_DISNEY_CLEARCOAT };
+                L_DISNEY_DIFFUSE, L_DISNEY_RETRO, L_DISNEY_SHEEN, L_DISNEY_CLEARCOAT,
+                L_FRESNEL_BLEND, L_LAMBERT_T, L_SPEC_T };
 enum FresnelKind { FR_CONST, FR_DIELECTRIC, FR_CONDUCTOR, FR_DISNEY };
 
 struct Lobe {
@@ -639,7 +641,7 @@ struct Lobe {
     float etaA, etaB;    // dielectric pairs
     int fresnel;
     float frEtaI, frEtaT;        // dielectric fresnel
-    C3 condEta, condK;           // conductor
+    C3 condEtaI, condEta, condK; // conductor (ConductorFresnel(etaI, etaT, k))
     C3 disneyR0; float disneyEta, disneyMetallic;
     TR md;
     float roughness, gloss, weight;
@@ -652,8 +654,8 @@ C3 fresnelEval(const Lobe& l, float cosI) {
         case FR_DIELECTRIC: return C3(fresnelDielectric(cosI, l.frEtaI, l.frEtaT));
         case FR_CONDUCTOR: {
             float c = std::fabs(cosI);
-            return C3(conductor1(c, 1.0f, l.condEta.r, l.condK.r), conductor1(c, 1.0f, l.condEta.g, l.condK.g),
-                      conductor1(c, 1.0f, l.condEta.b, l.condK.b));
+            return C3(conductor1(c, l.condEtaI.r, l.condEta.r, l.condK.r), conductor1(c, l.condEtaI.g, l.condEta.g, l.condK.g),
+                      conductor1(c, l.condEtaI.b, l.condEta.b, l.condK.b));
         }
         default: {  // DisneyFresnel.java:67-69
             C3 die(fresnelDielectric(cosI, 1.0f, l.disneyEta));
@@ -661,6 +663,8 @@ C3 fresnelEval(const Lobe& l, float cosI) {
         }
     }
 }
+
+float pow5f(float x) { return x * x * x * x * x; }  // Floats.pow5
 
 float gtr1(float c, float alpha) {  // DisneyClearCoatBRDF.doGTR1
     float a = alpha * alpha;
@@ -737,6 +741,23 @@ C3 lobeF(const Lobe& l, V3 wo, V3 wi) {
             float g = smithGGX(cosThetaAbs(wo), 0.25f) * smithGGX(cosThetaAbs(wi), 0.25f);
             return C3(l.weight * g * f * d / 4.0f);
         }
+        case L_FRESNEL_BLEND: {  // FresnelBlendBRDF.java:194-221 (R = KD, T = KS)
+            V3 h = wo + wi;
+            if (isZero(h.x) && isZero(h.y) && isZero(h.z)) return C3(0.0f);
+            h = normalize(h);
+            const float a = 28.0f / (23.0f * PI);
+            const float b = 1.0f - pow5f(1.0f - 0.5f * cosThetaAbs(wi));
+            const float c = 1.0f - pow5f(1.0f - 0.5f * cosThetaAbs(wo));
+            const float d = l.md.D(h);
+            const float e = 4.0f * std::fabs(dot(wi, h)) * std::max(cosThetaAbs(wi), cosThetaAbs(wo));
+            const float f = d / e;
+            // Schlick.fresnelDielectric(cos, r0) = r0 + (1 - r0) pow5(1 - cos)
+            const float w = pow5f(1.0f - dot(wi, h));
+            const C3 fr(l.T.r + (1.0f - l.T.r) * w, l.T.g + (1.0f - l.T.g) * w, l.T.b + (1.0f - l.T.b) * w);
+            const C3 diffuse = (((l.R * a) * (C3(1.0f) - l.T)) * b) * c;
+            return diffuse + fr * f;
+        }
+        case L_LAMBERT_T: return l.T * PI_RECIPROCAL;  // LambertianBTDF.java
         default: return C3(0.0f);  // specular lobes: delta
     }
 }
@@ -768,6 +789,13 @@ float lobePdf(const Lobe& l, V3 wo, V3 wi) {
             float ch = cosThetaAbs(h);
             return gtr1(ch, l.gloss) * ch / (4.0f * dot(wo, h));
         }
+        case L_FRESNEL_BLEND: {  // FresnelBlendBRDF.java:327-339
+            if (!sameHemisphereZ(wo, wi)) return 0.0f;
+            V3 h = normalize(wo + wi);
+            return 0.5f * (cosThetaAbs(wi) * PI_RECIPROCAL + l.md.pdf(wo, h) / (4.0f * dot(wo, h)));
+        }
+        case L_LAMBERT_T:  // LambertianBTDF.java:211-217
+            return sameHemisphereZ(wo, wi) ? 0.0f : cosThetaAbs(wi) * PI_RECIPROCAL;
         default: return 0.0f;
     }
 }
@@ -840,6 +868,35 @@ bool lobeSample(const Lobe& l, V3 wo, float u, float v, LobeSample* out) {
             out->wi = wi; out->f = lobeF(l, wo, wi); out->pdf = lobePdf(l, wo, wi);
             return true;
         }
+        case L_FRESNEL_BLEND: {  // FresnelBlendBRDF.java:237-276
+            V3 wi;
+            if (u < 0.5f) {
+                wi = faceForwardZ(wo, hemisphereCosine(std::min(2.0f * u, 0.99999994f), v));
+            } else {
+                V3 h = l.md.sampleH(wo, std::min(2.0f * (u - 0.5f), 0.99999994f), v);
+                wi = reflectionPBRT(wo, h);
+                if (!sameHemisphereZ(wo, wi)) return false;
+            }
+            out->wi = wi; out->f = lobeF(l, wo, wi); out->pdf = lobePdf(l, wo, wi);
+            return true;
+        }
+        case L_LAMBERT_T: {  // LambertianBTDF.java: faceForwardZNegated
+            V3 s = hemisphereCosine(u, v);
+            V3 wi = wo.z > 0.0f ? negateZ(s) : s;
+            out->wi = wi; out->f = lobeF(l, wo, wi); out->pdf = lobePdf(l, wo, wi);
+            return true;
+        }
+        case L_SPEC_T: {  // SpecularBTDF.java:214-240, 313-322 (transport RADIANCE)
+            bool entering = cosTheta(wo) > 0.0f;
+            float ei = entering ? l.etaA : l.etaB, et = entering ? l.etaB : l.etaA;
+            V3 wi;
+            if (!refraction(wo, faceForward(V3(0, 0, 1), wo), ei / et, &wi)) return false;
+            const float fr = fresnelDielectric(cosTheta(wi), l.etaA, l.etaB);
+            out->wi = wi;
+            out->f = ((l.T * (1.0f - fr)) * ((ei * ei) / (et * et))) / cosThetaAbs(wi);
+            out->pdf = 1.0f;
+            return true;
+        }
         default: return false;
     }
 }
@@ -910,7 +967,9 @@ struct BSDF {
 C3 col(const float* c) { return C3(c[0], c[1], c[2]); }
 
 // Material.computeScatteringFunctions (transport RADIANCE, multiple lobes on).
-bool buildBSDF(const dfr_material& m, const Surface& si, BSDF* b) {
+// `uMaterial` is the random number ClearCoatMaterial draws while building
+// its BSDF (ClearCoatMaterial.java:318); no other material reads it.
+bool buildBSDF(const dfr_material& m, const Surface& si, float uMaterial, BSDF* b) {
     b->n = 0; b->eta = 1.0f;
     b->s = si.s; b->ng = si.g.w;
     b->woWorld = -si.rayDir;
@@ -964,7 +1023,7 @@ bool buildBSDF(const dfr_material& m, const Surface& si, BSDF* b) {
             float ru = m.remap_roughness ? roughnessToAlpha(m.roughness_u) : m.roughness_u;
             float rv = m.remap_roughness ? roughnessToAlpha(m.roughness_v) : m.roughness_v;
             Lobe l{}; l.kind = L_TS_R; l.type = F_GLOSSY | F_REFL; l.R = C3(1.0f);
-            l.fresnel = FR_CONDUCTOR; l.condEta = col(m.eta); l.condK = col(m.k); l.md = TR(ru, rv, false);
+            l.fresnel = FR_CONDUCTOR; l.condEtaI = C3(1.0f); l.condEta = col(m.eta); l.condK = col(m.k); l.md = TR(ru, rv, false);
             b->add(l);
             return true;
         }
@@ -1010,6 +1069,101 @@ bool buildBSDF(const dfr_material& m, const Surface& si, BSDF* b) {
                 Lobe t{}; t.kind = L_TS_T; t.type = F_GLOSSY | F_TRANS; t.T = ts; t.md = md; t.etaA = 1.0f; t.etaB = eta;
                 b->add(t);
             }
+            return true;
+        }
+        case DFR_MAT_GLOSSY: {  // GlossyMaterial.java:323-337
+            C3 kr = col(m.kr);
+            Lobe l{}; l.kind = L_TS_R; l.type = F_GLOSSY | F_REFL; l.R = kr;
+            l.fresnel = FR_CONDUCTOR; l.condEtaI = kr; l.condEta = C3(1.0f); l.condK = C3(1.0f);
+            l.md = TR(m.roughness_u, m.roughness_u, false);
+            b->add(l);
+            return true;
+        }
+        case DFR_MAT_SUBSTRATE: {  // SubstrateMaterial.java:437-455
+            C3 kd = saturate0(col(m.kd)), ks = saturate0(col(m.ks));
+            if (isBlack(kd) && isBlack(ks)) return false;
+            float ru = m.remap_roughness ? roughnessToAlpha(m.roughness_u) : m.roughness_u;
+            float rv = m.remap_roughness ? roughnessToAlpha(m.roughness_v) : m.roughness_v;
+            Lobe l{}; l.kind = L_FRESNEL_BLEND; l.type = F_GLOSSY | F_REFL; l.R = kd; l.T = ks; l.md = TR(ru, rv, false);
+            b->add(l);
+            return true;
+        }
+        case DFR_MAT_TRANSLUCENT: {  // TranslucentMaterial.java:344-390 (kr: reflectance, kt: transmittance)
+            C3 refl = saturate0(col(m.kr)), trans = saturate0(col(m.kt));
+            if (isBlack(refl) && isBlack(trans)) return false;
+            C3 kd = saturate0(col(m.kd)), ks = saturate0(col(m.ks));
+            if (!isBlack(kd)) {
+                if (!isBlack(refl)) { Lobe l{}; l.kind = L_LAMBERT; l.type = F_DIFF | F_REFL; l.R = kd * refl; b->add(l); }
+                if (!isBlack(trans)) { Lobe l{}; l.kind = L_LAMBERT_T; l.type = F_DIFF | F_TRANS; l.T = kd * trans; b->add(l); }
+            }
+            if (!isBlack(ks)) {
+                float r = m.remap_roughness ? roughnessToAlpha(m.roughness_u) : m.roughness_u;
+                TR md(r, r, false);
+                if (!isBlack(refl)) {
+                    Lobe l{}; l.kind = L_TS_R; l.type = F_GLOSSY | F_REFL; l.R = ks * refl; l.md = md;
+                    l.fresnel = FR_DIELECTRIC; l.frEtaI = 1.0f; l.frEtaT = 1.5f;
+                    b->add(l);
+                }
+                if (!isBlack(trans)) {
+                    Lobe l{}; l.kind = L_TS_T; l.type = F_GLOSSY | F_TRANS; l.T = ks * trans; l.md = md; l.etaA = 1.0f; l.etaB = 1.5f;
+                    b->add(l);
+                }
+            }
+            return b->n > 0;
+        }
+        case DFR_MAT_CLEAR_COAT: {  // ClearCoatMaterial.java:292-333
+            C3 kd = col(m.kd), ks = col(m.ks);
+            const V3 d = si.rayDir, n = si.s.w;
+            const V3 nco = dot(n, d) > 0.0f ? -n : n;  // getSurfaceNormalSCorrectlyOriented
+            const bool entering = dot(n, nco) > 0.0f;
+            const float etaA = 1.0f, etaB = 1.5f;
+            const float etaI = entering ? etaA : etaB, etaT = entering ? etaB : etaA;
+            const float eta = etaI / etaT;
+            // Vector3F.refraction(direction, normal, eta)
+            const float cosI = dot(d, nco);
+            const float sin2I = std::max(0.0f, 1.0f - cosI * cosI);
+            const float sin2T = eta * eta * sin2I;
+            Lobe l{};
+            if (sin2T >= 1.0f) {
+                l.kind = L_SPEC_R; l.type = F_SPEC | F_REFL; l.R = ks; l.fresnel = FR_CONST;
+                b->add(l);
+                return true;
+            }
+            const float cosT = fsqrt(1.0f - sin2T);
+            const V3 refr = (-d) * eta + nco * (eta * cosI - cosT);
+            const float cosCO = entering ? -cosI : dot(refr, n);
+            const float reflectance = fresnelDielectric(cosCO, etaA, etaB);
+            const float transmittance = 1.0f - reflectance;
+            const float pRR = 0.25f + 0.5f * reflectance;
+            if (uMaterial < pRR) {
+                l.kind = L_SPEC_R; l.type = F_SPEC | F_REFL; l.R = ks * (reflectance / pRR); l.fresnel = FR_CONST;
+            } else {
+                l.kind = L_LAMBERT; l.type = F_DIFF | F_REFL; l.R = kd * (transmittance / (1.0f - pRR));
+            }
+            b->add(l);
+            return true;
+        }
+        case DFR_MAT_UBER: {  // UberMaterial.java:684-723 (k: opacity)
+            C3 op = saturate0(col(m.k));
+            C3 kd = op * saturate0(col(m.kd)), kr = op * saturate0(col(m.kr));
+            C3 ks = op * saturate0(col(m.ks)), kt = op * saturate0(col(m.kt));
+            C3 ts = saturate0(C3(1.0f) - op);
+            float eta = m.eta[0];
+            float ru = m.remap_roughness ? roughnessToAlpha(m.roughness_u) : m.roughness_u;
+            float rv = m.remap_roughness ? roughnessToAlpha(m.roughness_v) : m.roughness_v;
+            if (!isBlack(ts)) { Lobe l{}; l.kind = L_SPEC_T; l.type = F_SPEC | F_TRANS; l.T = ts; l.etaA = 1.0f; l.etaB = 1.0f; b->add(l); }
+            if (!isBlack(kd)) { Lobe l{}; l.kind = L_LAMBERT; l.type = F_DIFF | F_REFL; l.R = kd; b->add(l); }
+            if (!isBlack(ks)) {
+                Lobe l{}; l.kind = L_TS_R; l.type = F_GLOSSY | F_REFL; l.R = ks; l.md = TR(ru, rv, false);
+                l.fresnel = FR_DIELECTRIC; l.frEtaI = 1.0f; l.frEtaT = eta;
+                b->add(l);
+            }
+            if (!isBlack(kr)) {
+                Lobe l{}; l.kind = L_SPEC_R; l.type = F_SPEC | F_REFL; l.R = kr; l.fresnel = FR_DIELECTRIC; l.frEtaI = 1.0f; l.frEtaT = eta;
+                b->add(l);
+            }
+            if (!isBlack(kt)) { Lobe l{}; l.kind = L_SPEC_T; l.type = F_SPEC | F_TRANS; l.T = kt; l.etaA = 1.0f; l.etaB = eta; b->add(l); }
+            b->eta = isBlack(ts) ? eta : 1.0f;
             return true;
         }
         default: return false;
@@ -1204,13 +1358,13 @@ C3 radiancePT(const Scene& sc, Ray ray, const Rng& rng, int maxBounce, int minRR
             if (bounce == 0 || specBounce) L = L + beta * emitted(sc, si, wo);
             if (bounce >= maxBounce) break;
             BSDF bsdf;
-            if (!buildBSDF(sc.mats[sc.prims[h.prim].material], si, &bsdf)) {
+            const uint32_t dim = DIM_BOUNCE + DIMS_PER_BOUNCE * (uint32_t)bounce;
+            if (!buildBSDF(sc.mats[sc.prims[h.prim].material], si, rng.get(dim + 9), &bsdf)) {
                 ray = Ray{si.p, ray.d};
                 bounce--;
                 if (++passThrough > 256) break;  // guard against an endless chain of null surfaces
                 continue;
             }
-            const uint32_t dim = DIM_BOUNCE + DIMS_PER_BOUNCE * (uint32_t)bounce;
             if (bsdf.countSpecular(false) > 0 && !sc.lights.empty()) {
                 int n = (int)sc.lights.size();
                 int li = std::min((int)(rng.get(dim + 0) * n), n - 1);
@@ -1610,14 +1764,17 @@ float orc_fresnel_dielectric(float cosThetaI, float etaI, float etaT) { return f
 float orc_fresnel_conductor(float cosThetaI, float eta, float k) { return conductor1(std::fabs(cosThetaI), 1.0f, eta, k); }
 
 // BSDF probes: evaluate f and pdf of a material for local outgoing/incoming
-// directions with the shading frame equal to the world axes.
+// directions with the shading frame equal to the world axes.  The random
+// number a clear coat draws while building its BSDF is g_material_u.
+static float g_material_u = 0.5f;
+void orc_set_material_u(float u) { g_material_u = u; }
 int32_t orc_bsdf_eval(const dfr_material* m, const float* wo, const float* wi, float* f, float* pdf) {
     Surface si{};
     si.s.u = V3(1, 0, 0); si.s.v = V3(0, 1, 0); si.s.w = V3(0, 0, 1);
     si.g = si.s;
     si.rayDir = -V3(wo[0], wo[1], wo[2]);
     BSDF b;
-    if (!buildBSDF(*m, si, &b)) return 0;
+    if (!buildBSDF(*m, si, g_material_u, &b)) return 0;
     V3 w(wi[0], wi[1], wi[2]);
     C3 c = b.f(ALL, w);
     f[0] = c.r; f[1] = c.g; f[2] = c.b;
@@ -1631,7 +1788,7 @@ int32_t orc_bsdf_sample(const dfr_material* m, const float* wo, float u, float v
     si.g = si.s;
     si.rayDir = -V3(wo[0], wo[1], wo[2]);
     BSDF b;
-    if (!buildBSDF(*m, si, &b)) return 0;
+    if (!buildBSDF(*m, si, g_material_u, &b)) return 0;
     LobeSample s;
     if (!b.sample(ALL, u, v, &s)) return 0;
     wi[0] = s.wi.x; wi[1] = s.wi.y; wi[2] = s.wi.z;

@@ -82,6 +82,7 @@ float orc_sphere_t(const float* center, float radius, const float* ray, float tm
 float orc_plane_t(const float* origin, const float* normal, const float* ray, float tmin, float tmax);
 float orc_fresnel_dielectric(float cos_theta_i, float eta_i, float eta_t);
 float orc_fresnel_conductor(float cos_theta_i, float eta, float k);
+void orc_set_material_u(float u);  /* clear coat's layer draw in the BSDF probes (default 0.5) */
 int32_t orc_bsdf_eval(const dfr_material* m, const float* wo, const float* wi, float* f, float* pdf);
 int32_t orc_bsdf_sample(const dfr_material* m, const float* wo, float u, float v, float* wi, float* f,
                         float* pdf, int32_t* type);

@@ -70,6 +70,8 @@ def load():
     lib.orc_perez_pdf_many.argtypes = [f, vp, i64, vp, vp]
     lib.orc_perez_sample_many.argtypes = [f, vp, i64, vp, vp, vp]
     lib.orc_tone_map.argtypes = [i32, i64, vp, vp]
+    lib.orc_set_material_u.argtypes = [f]
+    lib.orc_set_material_u.restype = None
     lib.orc_image_rgba8.argtypes = [i32, i32, i64, vp, vp]
     _lib = lib
     return lib

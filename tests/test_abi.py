@@ -33,11 +33,12 @@ def test_abi_table_matches_header():
 
 
 def test_ordinals_match_header():
-    """Rendering-algorithm and tone-mapper ordinals of the header (those of
-    AbstractGPURenderer.java:50-87) are the ones the Python mirror passes."""
+    """Rendering-algorithm, tone-mapper and material ordinals of the header
+    (the first two those of AbstractGPURenderer.java:50-87) are the ones the
+    Python mirror passes."""
     text = open(HEADER).read()
-    defs = dict((m.group(1), int(m.group(2))) for m in re.finditer(r"#define\s+DFR_((?:ALGO|TONE_MAPPER)_\w+)\s+(\d+)", text))
-    assert len(defs) == 11
+    defs = dict((m.group(1), int(m.group(2))) for m in re.finditer(r"#define\s+DFR_((?:ALGO|TONE_MAPPER|MAT)_\w+)\s+(\d+)", text))
+    assert len(defs) == 22
     for name, value in defs.items():
         assert getattr(abi, name) == value, name
 

@@ -189,20 +189,23 @@ def test_cornell_path_tracing_same_streams(kernel):
     assert abs(g.mean() - ref.mean()) / ref.mean() < 0.005
 
 
-@pytest.mark.parametrize("name", ["plastic_perez_point", "matte_glass_metal_disney"])
+@pytest.mark.parametrize("name", ["plastic_perez_point", "matte_glass_metal_disney", "glossy_substrate_translucent_uber"])
 def test_materials_and_lights_same_streams(name):
     """Same-stream parity on the other BSDFs and lights: a Plastic icosphere
-    on a Matte ground under a point light and the Perez sun/sky, and the
-    four-patch Matte/Glass/Metal/Disney icosphere under a point light; both
-    kernels.  Bars as for the Cornell box: >= 98 % of 1-spp paths equal
-    (rtol 1e-3) and the 16-spp image mean within 0.5 %."""
+    on a Matte ground under a point light and the Perez sun/sky, the
+    four-patch Matte/Glass/Metal/Disney icosphere under a point light, and
+    the four-patch Glossy/Substrate/Translucent/Uber icosphere on a ClearCoat
+    ground; both kernels.  Bars as for the Cornell box: >= 98 % of 1-spp
+    paths equal (rtol 1e-3) and the 16-spp image mean within 0.5 %."""
     from tests.oracle_lib import OracleScene
     from dayflower_amd import workloads as W
     res = 96
     if name == "plastic_perez_point":
         sc = S.icosphere_scene(5000, seed=0x5EED, width=res, height=res, with_perez=True)
-    else:
+    elif name == "matte_glass_metal_disney":
         sc = W.mixed_material_icosphere(5000, 0x5EED, width=res, height=res)
+    else:
+        sc = W.more_materials_icosphere(5000, 0x5EED, width=res, height=res)
     o = OracleScene(sc)
     s1, _, c1 = o.render(res, res, 1, seed=21, max_bounce=8)
     s16, _, c16 = o.render(res, res, 16, seed=21, max_bounce=8)

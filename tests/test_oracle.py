@@ -188,7 +188,9 @@ def test_lambertian_value(oracle):
 
 
 @pytest.mark.parametrize("mat", [S.matte((0.7, 0.7, 0.7), angle=20.0), S.metal(roughness=0.3),
-                                 S.plastic(roughness=0.3), S.disney(roughness=0.6)])
+                                 S.plastic(roughness=0.3), S.disney(roughness=0.6), S.glossy((0.6, 0.5, 0.9), 0.3),
+                                 S.substrate((0.6, 0.4, 0.2), (0.1, 0.2, 0.05), 0.3),
+                                 S.translucent(roughness=0.3), S.uber(kr=(0, 0, 0), opacity=(1, 1, 1))])
 def test_reciprocity(oracle, mat):
     rng = np.random.default_rng(4)
     a, b = _dirs(200, rng), _dirs(200, rng)
@@ -232,11 +234,12 @@ def test_white_furnace_energy(oracle, mat, lo, hi):
         assert lo <= alb.mean() <= hi
 
 
-def test_sampled_pdf_matches_evaluated_pdf(oracle):
+@pytest.mark.parametrize("mat", [S.metal(roughness=0.4), S.substrate((0.5, 0.5, 0.5), (0.3, 0.3, 0.3), 0.3),
+                                 S.glossy((0.7, 0.7, 0.7), 0.25)])
+def test_sampled_pdf_matches_evaluated_pdf(oracle, mat):
     """Sample_f's pdf equals pdf() for the same direction, and the sampled
     direction histogram follows it (chi-square over theta bins)."""
     rng = np.random.default_rng(6)
-    mat = S.metal(roughness=0.4)
     wo = fa(0.5, 0.0, math.sqrt(0.75))
     wi = np.zeros(3, np.float32)
     f = np.zeros(3, np.float32)
@@ -267,6 +270,106 @@ def test_sampled_pdf_matches_evaluated_pdf(oracle):
     keep = expect > 20
     chi2 = (((obs - expect) ** 2) / expect)[keep].sum()
     assert chi2 < 3 * keep.sum() + 30, (obs, expect)
+
+
+def test_glossy_conductor_at_normal_incidence(oracle):
+    """GlossyMaterial's ConductorFresnel(etaI = KR, etaT = 1, k = 1) sees
+    eta = k = 1/KR: F(0) = ((eta-1)^2 + k^2) / ((eta+1)^2 + k^2); with
+    wi = wo = n, D = 1/(pi alpha^2) and G = 1, so f = KR D F / 4."""
+    kr, alpha = np.array([0.5, 0.25, 1.0]), 0.3
+    f, _ = _eval(oracle, S.glossy(tuple(kr), alpha), (0, 0, 1), (0, 0, 1))
+    eta = 1.0 / kr
+    F = ((eta - 1) ** 2 + eta ** 2) / ((eta + 1) ** 2 + eta ** 2)
+    np.testing.assert_allclose(F, [5 / 13, 25 / 41, 1 / 5])
+    np.testing.assert_allclose(f, kr * F / (math.pi * alpha ** 2) / 4.0, rtol=1e-5)
+
+
+def test_fresnel_blend_at_normal_incidence(oracle):
+    """Ashikhmin-Shirley (FresnelBlendBRDF) with wi = wo = n: the diffuse term
+    is Rd 28/(23 pi) (1 - Rs) (1 - (1/2)^5)^2 and the glossy term D Rs / 4
+    (Schlick's F(cos 0) = Rs, D = 1/(pi alpha^2)); pdf = (1/pi + D/4) / 2."""
+    rd, rs, alpha = np.array([0.6, 0.4, 0.2]), np.array([0.1, 0.2, 0.05]), 0.3
+    f, pdf = _eval(oracle, S.substrate(tuple(rd), tuple(rs), alpha, remap=False), (0, 0, 1), (0, 0, 1))
+    D = 1.0 / (math.pi * alpha ** 2)
+    expect = rd * 28.0 / (23.0 * math.pi) * (1 - rs) * (31.0 / 32.0) ** 2 + D * rs / 4.0
+    np.testing.assert_allclose(f, expect, rtol=1e-5)
+    assert pdf == pytest.approx(0.5 * (1.0 / math.pi + D / 4.0), rel=1e-5)
+
+
+def _albedo_material_u(oracle, mat, wo, n, rng):
+    """Directional albedo with the clear coat's layer draw taken uniformly."""
+    wi = np.zeros(3, np.float32)
+    f = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    ty = np.zeros(1, np.int32)
+    acc = np.zeros(3)
+    wo = fa(wo)
+    try:
+        for um, u, v in rng.random((n, 3)):
+            oracle.orc_set_material_u(float(um))
+            if oracle.orc_bsdf_sample(ctypes.byref(mat), p(wo), float(u), float(v), p(wi), p(f), p(pdf), p(ty)):
+                if pdf[0] > 0:
+                    acc += f * abs(wi[2]) / pdf[0]
+    finally:
+        oracle.orc_set_material_u(0.5)
+    return acc / n
+
+
+def test_clear_coat_layer_weights(oracle):
+    """The clear coat's Russian roulette is unbiased: averaged over its draw,
+    the albedo is R KS + (1 - R) KD with R the 1 -> 1.5 dielectric
+    reflectance (each layer alone has albedo KS or KD: a mirror, a
+    Lambertian); under total internal reflection the coat is KS."""
+    kd, ks = np.array([0.5, 0.3, 0.8]), np.array([1.0, 0.9, 0.7])
+    mat = S.clear_coat(tuple(kd), tuple(ks))
+    rng = np.random.default_rng(12)
+    for cz in (0.95, 0.5, 0.15):
+        wo = (math.sqrt(1 - cz * cz), 0.0, cz)
+        R = oracle.orc_fresnel_dielectric(cz, 1.0, 1.5)
+        alb = _albedo_material_u(oracle, mat, wo, 40000, rng)
+        np.testing.assert_allclose(alb, R * ks + (1 - R) * kd, rtol=0.02)
+    # from inside at |cos| = 0.6 < sqrt(1 - 1/1.5^2): total internal reflection
+    alb = _albedo_material_u(oracle, mat, (0.8, 0.0, -0.6), 2000, rng)
+    np.testing.assert_allclose(alb, ks, rtol=1e-5)
+
+
+def test_translucent_and_uber_closed_forms(oracle):
+    """Lambertian reflection + transmission: every sample carries exactly
+    kd (refl + trans) / 1 and half of them leave through the back; Uber with
+    only opacity passes the ray straight through with weight 1 - opacity."""
+    rng = np.random.default_rng(13)
+    mat = S.translucent(kd=(1.0, 0.5, 0.25), ks=(0, 0, 0), reflectance=(0.5, 0.5, 0.5),
+                        transmittance=(0.5, 0.5, 0.5))
+    wo = fa(0.3, 0.2, math.sqrt(1 - 0.13))
+    wi = np.zeros(3, np.float32)
+    f = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    ty = np.zeros(1, np.int32)
+    below = 0
+    for u, v in rng.random((400, 2)):
+        assert oracle.orc_bsdf_sample(ctypes.byref(mat), p(wo), float(u), float(v), p(wi), p(f), p(pdf), p(ty))
+        np.testing.assert_allclose(f * abs(wi[2]) / pdf[0], [1.0, 0.5, 0.25], rtol=1e-5)
+        below += wi[2] < 0
+        assert bool(ty[0] & 2) == bool(wi[2] < 0)  # transmission flag on the back side
+    assert 150 < below < 250
+    mat = S.uber(kd=(0, 0, 0), ks=(0, 0, 0), opacity=(0.3, 0.6, 1.0))
+    assert oracle.orc_bsdf_sample(ctypes.byref(mat), p(wo), 0.4, 0.7, p(wi), p(f), p(pdf), p(ty))
+    np.testing.assert_allclose(wi, -wo, atol=1e-6)
+    np.testing.assert_allclose(f * abs(wi[2]) / pdf[0], [0.7, 0.4, 0.0], atol=1e-6)
+
+
+@pytest.mark.parametrize("mat,hi", [(S.glossy((1, 1, 1), 0.1), 1.0), (S.substrate((1, 1, 1), (0.04, 0.04, 0.04), 0.2), 1.02),
+                                    (S.translucent((0.25, 0.25, 0.25), (0.25, 0.25, 0.25), (1, 1, 1), (1, 1, 1), 0.2), 1.0),
+                                    (S.uber((0.5, 0.5, 0.5), (0.3, 0.3, 0.3), (0.1, 0.1, 0.1), (0.1, 0.1, 0.1),
+                                            (0.8, 0.8, 0.8)), 1.0)])
+def test_new_materials_create_no_energy(oracle, mat, hi):
+    """Directional albedo at most one where the weights sum to at most one
+    (Translucent weighs reflection and transmission separately, so kd + ks
+    <= 1/2 with both at one; FresnelBlend's Schlick coat allows a few per
+    cent over)."""
+    rng = np.random.default_rng(14)
+    alb = _albedo(oracle, mat, (0.3, 0.1, math.sqrt(1 - 0.1)), 20000, rng)
+    assert 0.0 < alb.max() <= hi
 
 
 # ---------------------------------------------------------------- integrators
