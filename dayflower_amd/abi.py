@@ -17,7 +17,7 @@ ALGO_AMBIENT_OCCLUSION, ALGO_DEPTH_CAMERA, ALGO_PATH_TRACING, ALGO_RAY_CASTING, 
 SHAPE_TRIANGLE_MESH, SHAPE_SPHERE, SHAPE_RECTANGLE, SHAPE_PLANE = range(4)
 (MAT_MATTE, MAT_MIRROR, MAT_GLASS, MAT_METAL, MAT_PLASTIC, MAT_DISNEY, MAT_GLOSSY, MAT_SUBSTRATE, MAT_TRANSLUCENT,
  MAT_CLEAR_COAT, MAT_UBER) = range(11)
-LIGHT_DIFFUSE_AREA, LIGHT_POINT, LIGHT_PEREZ = range(3)
+LIGHT_DIFFUSE_AREA, LIGHT_POINT, LIGHT_PEREZ, LIGHT_SPOT, LIGHT_DIRECTIONAL = range(5)
 LENS_FISHEYE, LENS_THIN = 0, 1
 
 KEY_MAXIMUM_BOUNCE = 1
@@ -70,7 +70,8 @@ class Light(ctypes.Structure):
         ("type", c_int32), ("primitive", c_int32), ("triangle", c_int64),
         ("two_sided", c_int32), ("reserved0", c_int32),
         ("radiance", c_float * 3), ("position", c_float * 3), ("direction", c_float * 3),
-        ("turbidity", c_float),
+        ("turbidity", c_float), ("cone_angle", c_float), ("cone_delta", c_float), ("radius", c_float),
+        ("reserved1", c_float),
     ]
 
 

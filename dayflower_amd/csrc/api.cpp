@@ -389,7 +389,7 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
     std::vector<DevMaterial> mats(in.mats.size());
     std::vector<DevLobe> lobes;
     for (size_t m = 0; m < in.mats.size(); m++) compile_material(in.mats[m], mats[m], lobes);
-    for (const dfr_material& m : in.mats) out.ext_materials = out.ext_materials || m.type >= DFR_MAT_GLOSSY;
+    for (const dfr_material& m : in.mats) out.ext_features = out.ext_features || m.type >= DFR_MAT_GLOSSY;
     std::vector<DevLight> lights(in.lights.size());
     std::vector<DevPerez> perez;
     for (size_t l = 0; l < in.lights.size(); l++) {
@@ -415,6 +415,19 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
             perez_build(p, L.turbidity, L.direction);
             d.perez = (int32_t)perez.size();
             perez.push_back(p);
+        } else if (L.type == DFR_LIGHT_SPOT || L.type == DFR_LIGHT_DIRECTIONAL) {
+            const double x = L.direction[0], y = L.direction[1], z = L.direction[2], n = std::sqrt(x * x + y * y + z * z);
+            if (!(n > 0.0)) { err = "light " + std::to_string(l) + ": zero direction"; return DFR_E_ARGUMENT; }
+            for (int k = 0; k < 3; k++) d.dir[k] = (float)(L.direction[k] / n);
+            if (L.type == DFR_LIGHT_SPOT) {  // SpotLight.java:205-206
+                const double rad = 3.14159265358979323846 / 180.0;
+                d.area = (float)std::cos((double)(float)(L.cone_angle * rad));
+                d.param = (float)std::cos((double)(float)((L.cone_angle - L.cone_delta) * rad));
+            } else {
+                d.param = L.radius;
+            }
+        } else if (L.type != DFR_LIGHT_POINT) {
+            err = "light " + std::to_string(l) + ": type out of range"; return DFR_E_ARGUMENT;
         }
     }
     // traversal addresses nodes and triangles with 32-bit byte offsets
@@ -445,7 +458,10 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
     ds.perez = (const DevPerez*)out.perez.ptr;
     ds.n_analytic = (int32_t)analytic.size();
     ds.n_lights = (int32_t)lights.size();
-    for (const DevLight& L : lights) out.area_lights = out.area_lights || L.type == DFR_LIGHT_DIFFUSE_AREA;
+    for (const DevLight& L : lights) {
+        out.area_lights = out.area_lights || L.type == DFR_LIGHT_DIFFUSE_AREA;
+        out.ext_features = out.ext_features || L.type == DFR_LIGHT_SPOT || L.type == DFR_LIGHT_DIRECTIONAL;
+    }
     ds.has_bvh = nt > 0 ? 1 : 0;
     ds.light_pdf_mode = light_pdf_mode;
     std::memcpy(out.camera, in.camera, sizeof(out.camera));
@@ -515,8 +531,8 @@ bool Ctx::any_area_lights() const {
     return false;
 }
 
-bool Ctx::any_ext_materials() const {
-    for (auto& s : scenes) if (s && s->ext_materials) return true;
+bool Ctx::any_ext_features() const {
+    for (auto& s : scenes) if (s && s->ext_features) return true;
     return false;
 }
 
@@ -605,7 +621,7 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
     wf.overflow = (int*)overflow.ptr;
     wavefront_init(wf, (const DevJob*)job_buf.ptr, n_jobs, work_total, stream);
     WavefrontParams p{wf_params.block, wf_params.sort_keys, (capacity + 255) / 256, any_area_lights(), uniform_scene(),
-                      any_ext_materials()};
+                      any_ext_features()};
     Counters host{};
     const int poll = 4;
     const int max_iterations = 1 << 20;  // far above max_bounce * (work / capacity + 1)
@@ -627,7 +643,7 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
             (uintptr_t)wf.bhist, (uintptr_t)wf.sh.o, (uintptr_t)wf.sh.d, (uintptr_t)wf.sh.c, (uintptr_t)wf.sh.aux,
             (uintptr_t)wf.overflow, (uintptr_t)wf.ctr, (uintptr_t)wf.job_done, (uintptr_t)wf.capacity,
             (uintptr_t)p.block, (uintptr_t)p.sort_keys, (uintptr_t)p.closest_shadow, (uintptr_t)p.uscene,
-            (uintptr_t)p.ext_materials};
+            (uintptr_t)p.ext_features};
         for (int it = 0;; it += poll) {
             if (it >= max_iterations) return fail(DFR_E_DEVICE, "wavefront did not drain");
             if ((rc = replay_iterations(key, scenes, jobs, n_jobs, film_ptr, p, poll))) return rc;
@@ -938,7 +954,7 @@ int32_t dfr_render(dfr_ctx* h, int32_t passes) {
             e = hipMemcpyAsync(c->job_buf.ptr, &jp, sizeof(DevJob), hipMemcpyHostToDevice, c->stream);
             if (e != hipSuccess) return c->hip_fail(e, "job upload");
             launch_megakernel((const DevScene*)c->scene_table.ptr, (const DevJob*)c->job_buf.ptr, (float4*)c->film.ptr,
-                              (int*)c->overflow.ptr, (int)npix, 256, c->any_ext_materials(), (int64_t*)c->counters.ptr,
+                              (int*)c->overflow.ptr, (int)npix, 256, c->any_ext_features(), (int64_t*)c->counters.ptr,
                               c->stream);
             if (c->stop.load()) break;
         }

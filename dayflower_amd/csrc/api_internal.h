@@ -57,7 +57,7 @@ struct CompiledScene {
     DevScene dev{};
     int max_depth = 1;
     bool area_lights = false;  // any DiffuseAreaLight (the only lights whose NEE needs closest hits)
-    bool ext_materials = false;  // any glossy, substrate, translucent, clear coat or uber material
+    bool ext_features = false;  // any glossy, substrate, translucent, clear coat or uber material, or spot or directional light
     int64_t n_nodes = 0, n_tris = 0;
     double sah_cost = 0.0, build_seconds = 0.0;
     size_t bytes = 0;
@@ -127,7 +127,7 @@ struct Ctx {
     int sync_scene_table();
     int max_depth() const;
     bool any_area_lights() const;
-    bool any_ext_materials() const;
+    bool any_ext_features() const;
     int ensure_overflow(int64_t threads);
     int ensure_film(int64_t pixels);
     int ensure_wavefront(int capacity);

@@ -836,8 +836,9 @@ DEV float smith_ggx(float c, float alpha) {
 }
 
 // Lobes of the glossy, substrate, translucent, clear coat and uber
-// materials; reached only from kernels instantiated with EXT = true, so the
-// kernels of the other materials carry none of their code or registers.
+// materials; reached only from kernels instantiated with EXT = true (scenes
+// with these materials or with spot or directional lights), so the kernels
+// of the other scenes carry none of their code or registers.
 DEV V lobe_f_ext(const DevLobe& l, V wo, V wi) {
     switch (l.kind) {
         case LOBE_FRESNEL_BLEND: {  // diffuse under a Schlick-Fresnel glossy coat
@@ -1225,7 +1226,36 @@ struct LightSmp { V L; V p; V wi; float pdf; };
 
 #include "perez_device.inc"
 
-// DiffuseAreaLight over Triangle3F, PointLight, PerezLight sampling.
+// SpotLight and DirectionalLight (delta distributions); reached only from
+// kernels instantiated with EXT = true.
+DEV bool sample_delta_light_ext(const DevLight& L, const Surf& si, LightSmp& out) {
+    if (L.type == DFR_LIGHT_SPOT) {  // SpotLight.sampleRadianceIncoming, doComputeFalloff
+        V pos = ld3(L.position);
+        V wi = nrm(pos - si.p);
+        const float ct = dot(-wi, ld3(L.dir));
+        float fall = 1.0f;
+        if (ct < L.area) fall = 0.0f;
+        else if (ct < L.param) { const float t = (ct - L.area) / (L.param - L.area); fall = (t * t) * (t * t); }
+        V dd = si.p - pos;
+        out.L = ld3(L.radiance) * fall / dot(dd, dd);
+        out.p = pos;
+        out.wi = wi;
+        out.pdf = 1.0f;
+        return true;
+    }
+    {  // DirectionalLight.sampleRadianceIncoming
+        V d = ld3(L.dir);
+        out.L = ld3(L.radiance);
+        out.p = si.p + d * (2.0f * L.param);
+        out.wi = d;
+        out.pdf = 1.0f;
+        return true;
+    }
+}
+
+// DiffuseAreaLight over Triangle3F, PointLight, PerezLight sampling (and the
+// spot and directional lights with EXT).
+template <bool EXT>
 DEV bool sample_light(const DevScene& sc, const DevLight& L, const Surf& si, float u, float v, float usun,
                       LightSmp& out) {
     if (L.type == DFR_LIGHT_POINT) {
@@ -1236,6 +1266,9 @@ DEV bool sample_light(const DevScene& sc, const DevLight& L, const Surf& si, flo
         out.wi = nrm(pos - si.p);
         out.pdf = 1.0f;
         return true;
+    }
+    if constexpr (EXT) {
+        if (L.type == DFR_LIGHT_SPOT || L.type == DFR_LIGHT_DIRECTIONAL) return sample_delta_light_ext(L, si, out);
     }
     if (L.type == DFR_LIGHT_PEREZ) return perez_sample(sc.perez[L.perez], si, u, v, usun, out);
     if (L.tri < 0) return false;
@@ -1292,6 +1325,12 @@ DEV V emitted(const DevScene& sc, const Surf& si, V dir) {
 DEV V emitted_ray(const DevScene& sc, const DevLight& L, V d) {
     if (L.type == DFR_LIGHT_PEREZ) return perez_sky(sc.perez[L.perez], d);
     return mk(0, 0, 0);
+}
+
+// lights sampled by a delta distribution: no MIS, no BSDF-sampled request
+template <bool EXT>
+DEV bool delta_light(int type) {
+    return type == DFR_LIGHT_POINT || (EXT && (type == DFR_LIGHT_SPOT || type == DFR_LIGHT_DIRECTIONAL));
 }
 
 DEV float power_heuristic(float a, float b) { return a * a * __builtin_amdgcn_rcpf(a * a + b * b); }

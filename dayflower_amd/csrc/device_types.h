@@ -73,7 +73,11 @@ struct DevLight {          // 64 bytes
     float area;
     float position[3];
     int32_t perez;         // offset of the Perez record in the perez table
-    float pad[4];
+    // spot: cone axis, cos(cone angle) in `area`, cos(cone angle - delta) in
+    // `param`; directional: unit direction towards the light, scene radius
+    // in `param`
+    float dir[3];
+    float param;
 };
 
 // Perez sky record: parameters and the 32x32 importance distribution.

@@ -85,7 +85,7 @@ struct WavefrontBuffers {
 // Launch helpers (kernels.hip).  `scenes` points to a device array of
 // DevScene records (one per resident scene); jobs to a device array.
 void launch_megakernel(const DevScene* scenes, const DevJob* job, float4* film, int* overflow, int n_pixels,
-                       int block, bool ext_materials, int64_t* counters, hipStream_t s);
+                       int block, bool ext_features, int64_t* counters, hipStream_t s);
 void launch_trace_rays(const DevScene* scene, const float4* rays, int64_t n, int any_hit, int32_t* prim,
                        int64_t* tri, float* t, int* overflow, int block, int grid, hipStream_t s);
 void launch_cast_primary(const DevScene* scene, const DevJob* job, float* rays, int32_t* prim, int64_t* tri,
@@ -101,8 +101,9 @@ struct WavefrontParams {
     // below 4 GiB), or null: selects the scene-uniform traversal kernels
     const DevScene* uscene = nullptr;
     // some scene uses a glossy, substrate, translucent, clear coat or uber
-    // material: launch the shade kernel built with their lobes
-    bool ext_materials = false;
+    // material or a spot or directional light: launch the shade kernel
+    // built with their code
+    bool ext_features = false;
 };
 
 // Optional instrumentation of one wavefront iteration: HIP events around the

@@ -127,9 +127,9 @@ DEV void nee_requests(const DevScene& sc, const Bsdf& b, const Surf& si, int li,
     const DevLight& L = sc.lights[li];
     const uint32_t T = BX_ALL_EXCEPT_SPECULAR;
     LightSmp ls;
-    bool have = sample_light(sc, L, si, rng.get(dim + 1), rng.get(dim + 2), rng.get(dim + 5), ls);
+    bool have = sample_light<EXT>(sc, L, si, rng.get(dim + 1), rng.get(dim + 2), rng.get(dim + 5), ls);
     V ns = si.s.w;
-    if (L.type == DFR_LIGHT_POINT) {
+    if (delta_light<EXT>(L.type)) {
         if (have && !black(ls.L) && ls.pdf > 0.0f) {
             V f = bsdf_f<EXT>(b, T, ls.wi) * fabsf(dot(ls.wi, ns));
             if (!black(f)) {
@@ -386,9 +386,9 @@ static void launch_megakernel_t(const DevScene* scenes, const DevJob* job, float
     }
 }
 void launch_megakernel(const DevScene* scenes, const DevJob* job, float4* film, int* overflow, int n_pixels, int block,
-                       bool ext_materials, int64_t* counters, hipStream_t s) {
+                       bool ext_features, int64_t* counters, hipStream_t s) {
     auto c = reinterpret_cast<unsigned long long*>(counters);
-    if (ext_materials) launch_megakernel_t<true>(scenes, job, film, overflow, n_pixels, block, c, s);
+    if (ext_features) launch_megakernel_t<true>(scenes, job, film, overflow, n_pixels, block, c, s);
     else launch_megakernel_t<false>(scenes, job, film, overflow, n_pixels, block, c, s);
 }
 
@@ -516,7 +516,8 @@ constexpr uint32_t INFO_LT_SHIFT = 9;
 DEV uint32_t nee_light_type(const DevScene& sc, const Rng& rng, uint32_t bounce) {
     if (sc.n_lights == 0) return 3u;
     const int li = min((int)(rng.get(DIM_BOUNCE + DIMS_PER_BOUNCE * bounce) * sc.n_lights), sc.n_lights - 1);
-    return (uint32_t)sc.lights[li].type;
+    const int t = sc.lights[li].type;
+    return t == DFR_LIGHT_PEREZ ? 2u : (t == DFR_LIGHT_DIFFUSE_AREA ? 0u : 1u);  // delta lights share class 1
 }
 
 __global__ void k_generate(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, int n_jobs,
@@ -1059,7 +1060,7 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     if (timing) hipEventRecord(prof->ev[2], s);
     // shading in 512-thread blocks whatever the traversal block: one queue
     // reservation (two same-address atomics) per 512 paths
-    if (p.ext_materials)
+    if (p.ext_features)
         hipLaunchKernelGGL((k_shade<SHADE_BLOCK, true>), dim3((cap + SHADE_BLOCK - 1) / SHADE_BLOCK), dim3(SHADE_BLOCK), 0, s,
                            scenes, jobs, w.a, w.hit, w.perm, w.b, w.sh, w.ctr, film, use_perm, w.job_done, cap);
     else

@@ -85,7 +85,7 @@ int batch_render_impl(Ctx* c, const dfr_job* jobs, int n, int mode, float* const
             for (int s = 0; s < jobs[j].spp; s++, k++)
                 launch_megakernel((const DevScene*)c->scene_table.ptr, (const DevJob*)c->job_buf.ptr + k,
                                   (float4*)film.ptr, (int*)c->overflow.ptr, jobs[j].width * jobs[j].height, 256,
-                                  c->any_ext_materials(), (int64_t*)c->counters.ptr, c->stream);
+                                  c->any_ext_features(), (int64_t*)c->counters.ptr, c->stream);
         hipEventRecord(c->ev1, c->stream);
         hipError_t e = hipEventSynchronize(c->ev1);
         if (e != hipSuccess) return c->hip_fail(e, "plain batch");
@@ -247,7 +247,7 @@ static void server_loop(Server* S) {
     hipMemcpyAsync(c->wf.ctr, &zero, sizeof(Counters), hipMemcpyHostToDevice, c->stream);
     hipMemsetAsync(S->done_mem.ptr, 0, (size_t)S->job_capacity * sizeof(int), c->stream);
     WavefrontParams p{c->wf_params.block, c->wf_params.sort_keys, 0, c->any_area_lights()};
-    p.ext_materials = c->any_ext_materials();
+    p.ext_features = c->any_ext_features();
     std::vector<int> done_host;
     Counters host{};
     for (;;) {

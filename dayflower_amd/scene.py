@@ -233,6 +233,37 @@ class Scene:
         self.lights.append(L)
         return len(self.lights) - 1
 
+    def add_spot_light(self, position, direction, intensity, cone_angle=30.0, cone_delta=5.0):
+        """SpotLight(coneAngle, coneAngleDelta, intensity, position, direction)
+        (M/scene/light/SpotLight.java:184); angles in degrees."""
+        L = abi.Light()
+        L.type = abi.LIGHT_SPOT
+        L.primitive = -1
+        L.triangle = -1
+        L.cone_angle = float(cone_angle)
+        L.cone_delta = float(cone_delta)
+        for i in range(3):
+            L.radiance[i] = float(intensity[i])
+            L.position[i] = float(position[i])
+            L.direction[i] = float(direction[i])
+        self.lights.append(L)
+        return len(self.lights) - 1
+
+    def add_directional_light(self, direction, radiance, radius=10.0):
+        """DirectionalLight(transform, radiance, direction, radius)
+        (M/scene/light/DirectionalLight.java:150); `direction` points towards
+        the light, `radius` bounds the scene."""
+        L = abi.Light()
+        L.type = abi.LIGHT_DIRECTIONAL
+        L.primitive = -1
+        L.triangle = -1
+        L.radius = float(radius)
+        for i in range(3):
+            L.radiance[i] = float(radiance[i])
+            L.direction[i] = float(direction[i])
+        self.lights.append(L)
+        return len(self.lights) - 1
+
     def add_perez_light(self, turbidity=2.0, sun_direction=(1.0, 1.0, -1.0)):
         """PerezLight(turbidity, sunDirectionWorldSpace) (M/scene/light/PerezLight.java)."""
         L = abi.Light()

@@ -77,10 +77,13 @@ extern "C" {
 #define DFR_MAT_CLEAR_COAT  9
 #define DFR_MAT_UBER       10
 
-/* Lights: M/scene/light/{DiffuseAreaLight,PointLight,PerezLight}.java */
+/* Lights: M/scene/light/{DiffuseAreaLight,PointLight,PerezLight,SpotLight,
+ * DirectionalLight}.java */
 #define DFR_LIGHT_DIFFUSE_AREA 0
 #define DFR_LIGHT_POINT        1
 #define DFR_LIGHT_PEREZ        2
+#define DFR_LIGHT_SPOT         3
+#define DFR_LIGHT_DIRECTIONAL  4
 
 /* Camera lens ordinals: M/scene/Lens.java:31-36 */
 #define DFR_LENS_FISHEYE 0
@@ -170,7 +173,12 @@ typedef struct dfr_primitive {
  *                 (global triangle index when the shape is one triangle of a
  *                 mesh primitive, else -1)
  *   POINT         radiance = intensity, position
- *   PEREZ         turbidity, direction = sun direction (world space) */
+ *   PEREZ         turbidity, direction = sun direction (world space)
+ *   SPOT          radiance = intensity, position, direction = cone axis
+ *                 (world space), cone_angle and cone_delta in degrees
+ *   DIRECTIONAL   radiance, direction = towards the light (world space,
+ *                 normalised by the compiler), radius = scene radius (the
+ *                 shadow ray's length is twice it) */
 typedef struct dfr_light {
     int32_t type;
     int32_t primitive;
@@ -181,6 +189,10 @@ typedef struct dfr_light {
     float position[3];
     float direction[3];
     float turbidity;
+    float cone_angle;
+    float cone_delta;
+    float radius;
+    float reserved1;
 } dfr_light;
 
 typedef struct dfr_scene_desc {

@@ -78,6 +78,10 @@ struct Light {
     V3 position;
     // Perez sky (filled by perezInit)
     struct Perez* perez;
+    // SpotLight: cone axis, cos(cone angle), cos(cone angle - delta);
+    // DirectionalLight: unit direction towards the light, scene radius
+    V3 axis;
+    float cosCone = 0.0f, cosConeMinusDelta = 0.0f, radius = 0.0f;
 };
 
 struct Ray { V3 o, d; };
@@ -1200,6 +1204,30 @@ bool sampleLight(const Scene& sc, const Light& L, const Surface& si, float u, fl
         out->pdf = 1.0f;
         return true;
     }
+    if (L.type == DFR_LIGHT_SPOT) {  // SpotLight.java:261-276, 369-385
+        V3 incoming = normalize(L.position - si.p);
+        V3 dir = -incoming;
+        float cosTheta = dot(dir, L.axis);
+        float falloff;
+        if (cosTheta < L.cosCone) falloff = 0.0f;
+        else if (cosTheta >= L.cosConeMinusDelta) falloff = 1.0f;
+        else {
+            float delta = (cosTheta - L.cosCone) / (L.cosConeMinusDelta - L.cosCone);
+            falloff = (delta * delta) * (delta * delta);
+        }
+        out->L = (L.radiance * falloff) / lengthSquared(si.p - L.position);
+        out->p = L.position;
+        out->wi = incoming;
+        out->pdf = 1.0f;
+        return true;
+    }
+    if (L.type == DFR_LIGHT_DIRECTIONAL) {  // DirectionalLight.java:192-208
+        out->L = L.radiance;
+        out->p = si.p + L.axis * (2.0f * L.radius);
+        out->wi = L.axis;
+        out->pdf = 1.0f;
+        return true;
+    }
     if (L.type == DFR_LIGHT_PEREZ) return perezSample(*L.perez, si, u, v, uSun, out);
     if (L.tri < 0) return false;
     const Tri& tr = sc.tris[L.tri];
@@ -1279,7 +1307,8 @@ C3 estimateDirect(const Scene& sc, const BSDF& bsdf, const Surface& si, int li, 
     LightSample ls;
     bool have = sampleLight(sc, L, si, ua, va, uSun, pdfMode, &ls);
     V3 n = si.s.w;
-    if (L.type == DFR_LIGHT_POINT) {
+    // Light.isUsingDeltaDistribution: point, spot and directional lights
+    if (L.type == DFR_LIGHT_POINT || L.type == DFR_LIGHT_SPOT || L.type == DFR_LIGHT_DIRECTIONAL) {
         if (have && !isBlack(ls.L) && ls.pdf > 0.0f) {
             C3 f = bsdf.f(T, ls.wi) * dotAbs(ls.wi, n);
             if (!isBlack(f) && visible(sc, si, li, ls)) Ld = Ld + f * ls.L / ls.pdf;
@@ -1618,6 +1647,16 @@ orc_scene* orc_scene_create(const dfr_scene_desc* d) {
         if (l.type == DFR_LIGHT_PEREZ) {
             L.perez = new Perez();
             perezInit(*L.perez, l.turbidity, V3(l.direction[0], l.direction[1], l.direction[2]));
+        }
+        if (l.type == DFR_LIGHT_SPOT || l.type == DFR_LIGHT_DIRECTIONAL) {
+            const double x = l.direction[0], y = l.direction[1], z = l.direction[2], n = std::sqrt(x * x + y * y + z * z);
+            L.axis = V3((float)(x / n), (float)(y / n), (float)(z / n));
+            // SpotLight.java:205-206: cos of the cone angle and of the cone
+            // angle minus its delta (AngleF in degrees, cos in double)
+            const double rad = 3.14159265358979323846 / 180.0;
+            L.cosCone = (float)std::cos((double)(float)(l.cone_angle * rad));
+            L.cosConeMinusDelta = (float)std::cos((double)(float)((l.cone_angle - l.cone_delta) * rad));
+            L.radius = l.radius;
         }
         sc.lights.push_back(L);
     }

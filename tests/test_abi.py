@@ -53,7 +53,7 @@ def c_layout(tmp_path_factory):
                          "disney_flatness"],
         "dfr_primitive": ["shape", "material", "area_light", "first_triangle", "triangle_count", "data"],
         "dfr_light": ["type", "primitive", "triangle", "two_sided", "radiance", "position", "direction",
-                      "turbidity"],
+                      "turbidity", "cone_angle", "cone_delta", "radius"],
         "dfr_scene_desc": ["n_vertices", "positions", "normals", "uvs", "n_triangles", "indices", "n_primitives",
                            "primitives", "n_materials", "materials", "n_lights", "lights", "camera"],
         "dfr_stats": ["seconds", "path_samples", "rays_closest", "rays_any", "bounces", "extend_seconds",

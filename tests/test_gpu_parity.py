@@ -189,13 +189,15 @@ def test_cornell_path_tracing_same_streams(kernel):
     assert abs(g.mean() - ref.mean()) / ref.mean() < 0.005
 
 
-@pytest.mark.parametrize("name", ["plastic_perez_point", "matte_glass_metal_disney", "glossy_substrate_translucent_uber"])
+@pytest.mark.parametrize("name", ["plastic_perez_point", "matte_glass_metal_disney", "glossy_substrate_translucent_uber",
+                                  "spot_directional"])
 def test_materials_and_lights_same_streams(name):
     """Same-stream parity on the other BSDFs and lights: a Plastic icosphere
     on a Matte ground under a point light and the Perez sun/sky, the
-    four-patch Matte/Glass/Metal/Disney icosphere under a point light, and
-    the four-patch Glossy/Substrate/Translucent/Uber icosphere on a ClearCoat
-    ground; both kernels.  Bars as for the Cornell box: >= 98 % of 1-spp
+    four-patch Matte/Glass/Metal/Disney icosphere under a point light or
+    under a spot and a directional light, and the four-patch
+    Glossy/Substrate/Translucent/Uber icosphere on a ClearCoat ground; both
+    kernels.  Bars as for the Cornell box: >= 98 % of 1-spp
     paths equal (rtol 1e-3) and the 16-spp image mean within 0.5 %."""
     from tests.oracle_lib import OracleScene
     from dayflower_amd import workloads as W
@@ -204,6 +206,11 @@ def test_materials_and_lights_same_streams(name):
         sc = S.icosphere_scene(5000, seed=0x5EED, width=res, height=res, with_perez=True)
     elif name == "matte_glass_metal_disney":
         sc = W.mixed_material_icosphere(5000, 0x5EED, width=res, height=res)
+    elif name == "spot_directional":
+        sc = W.mixed_material_icosphere(5000, 0x5EED, width=res, height=res)
+        sc.lights = []
+        sc.add_spot_light((0.5, 3.0, -1.0), (-0.15, -1.0, 0.3), (40.0, 40.0, 40.0), cone_angle=35.0, cone_delta=10.0)
+        sc.add_directional_light((-1.0, 1.5, -0.7), (1.5, 1.4, 1.2), radius=10.0)
     else:
         sc = W.more_materials_icosphere(5000, 0x5EED, width=res, height=res)
     o = OracleScene(sc)

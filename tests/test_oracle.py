@@ -396,6 +396,52 @@ def test_point_light_plane_closed_form():
     np.testing.assert_allclose(got, expect, rtol=1e-4)
 
 
+def _plane_hits(o, res, seed):
+    rays = o.primary_rays(res, res, seed, 0)
+    org, d = rays[:, 0:3].astype(np.float64), rays[:, 4:7].astype(np.float64)
+    return org + d * (-org[:, 1] / d[:, 1])[:, None]
+
+
+def test_spot_light_plane_closed_form():
+    """A Lambertian plane under a spot light pointing down: radiance is
+    (albedo / pi) I falloff cos / d^2 with the quartic cone falloff (one
+    inside the inner cone, ((cos - cos_outer) / (cos_inner - cos_outer))^4
+    across the rim, zero outside), and every region is seen."""
+    sc = S.Scene()
+    m = sc.add_material(S.matte((0.5, 0.5, 0.5)))
+    sc.add_plane((0, 0, 0), (1, 0, 0), (0, 0, -1), (0, 1, 0), m)
+    P = np.array([0.3, 2.0, 0.1])
+    sc.add_spot_light(tuple(P), (0.0, -1.0, 0.0), (10.0, 10.0, 10.0), cone_angle=30.0, cone_delta=10.0)
+    sc.set_camera(S.camera_record((0.0, 3.0, -3.0), (0.0, 0.0, 0.0), 48, 48, fov_x_deg=60.0))
+    o = OracleScene(sc)
+    s, _, _ = o.render(48, 48, 1, seed=4, max_bounce=5)
+    hit = _plane_hits(o, 48, 4)
+    v = P - hit
+    dist2 = (v * v).sum(1)
+    cos = v[:, 1] / np.sqrt(dist2)  # both the surface cosine and the cone cosine
+    co, ci = math.cos(math.radians(30.0)), math.cos(math.radians(20.0))
+    fall = np.where(cos < co, 0.0, np.where(cos >= ci, 1.0, ((cos - co) / (ci - co)) ** 4))
+    expect = 0.5 / math.pi * 10.0 * fall * cos / dist2
+    np.testing.assert_allclose(s[..., 0].reshape(-1), expect, rtol=1e-4, atol=1e-7)
+    rim = (cos > co) & (cos < ci)
+    assert (cos < co).sum() > 50 and (cos >= ci).sum() > 50 and rim.sum() > 50
+
+
+def test_directional_light_plane_closed_form():
+    """A Lambertian plane under a directional light: radiance is
+    (albedo / pi) E cos(theta) everywhere, with the direction normalised."""
+    sc = S.Scene()
+    m = sc.add_material(S.matte((0.6, 0.4, 0.2)))
+    sc.add_plane((0, 0, 0), (1, 0, 0), (0, 0, -1), (0, 1, 0), m)
+    sc.add_directional_light((1.0, 2.0, 0.5), (3.0, 3.0, 3.0), radius=10.0)
+    sc.set_camera(S.camera_record((0.0, 3.0, -3.0), (0.0, 0.0, 0.0), 32, 32, fov_x_deg=50.0))
+    o = OracleScene(sc)
+    s, _, _ = o.render(32, 32, 1, seed=4, max_bounce=5)
+    cos = 2.0 / math.sqrt(1.0 + 4.0 + 0.25)
+    np.testing.assert_allclose(s.reshape(-1, 3), np.tile(np.array([0.6, 0.4, 0.2]) / math.pi * 3.0 * cos, (32 * 32, 1)),
+                               rtol=1e-5)
+
+
 def _polygon_irradiance(x, n, verts):
     """Irradiance at x with normal n from a unit-radiance Lambertian polygon
     (Lambert's formula: 1/2 sum_i angle_i * (n . unit normal of edge plane))."""
