@@ -71,6 +71,42 @@ def test_cornell_primary_hits_match_fp64():
     assert rel <= 1e-5
 
 
+def test_obj_meshes_primary_hits_match_fp64():
+    """Meshes read from OBJ text (a fan-triangulated, group-split grid of
+    quads under a tetrahedron) through the scene compiler: primary hits
+    against the FP64 intersector, same bars as above."""
+    from tests.oracle_lib import OracleScene
+    from dayflower_amd import obj
+    n = 24
+    lines = [f"v {x / n * 4 - 2} {0.1 * ((x * 7 + y * 3) % 5) / 5} {y / n * 4 - 2}" for y in range(n + 1) for x in range(n + 1)]
+    lines.append("g ground")
+    for y in range(n):
+        for x in range(n):
+            a = y * (n + 1) + x + 1
+            lines.append(f"f {a} {a + n + 1} {a + n + 2} {a + 1}")
+    base = (n + 1) * (n + 1)
+    lines += ["g tetra", "usemtl red", "v 0.6 1.5 0.6", "v 0.6 0.5 -0.6", "v -0.6 1.5 -0.6", "v -0.6 0.5 0.6"]
+    lines += [f"f {base + a} {base + b} {base + c}" for a, b, c in ((1, 2, 3), (1, 4, 2), (1, 3, 4), (2, 4, 3))]
+    meshes = obj.parse_wavefront_object("\n".join(lines))
+    assert [len(m.indices) for m in meshes] == [2 * n * n, 4]
+    res = 128
+    sc = S.Scene()
+    obj.add_to_scene(sc, meshes, {"": sc.add_material(S.matte((0.5, 0.5, 0.5))), "red": sc.add_material(S.plastic())})
+    sc.add_point_light((1.0, 4.0, -1.0), (20.0, 20.0, 20.0))
+    sc.set_camera(S.camera_record((0.0, 2.5, -4.0), (0.0, 0.5, 0.0), res, res, fov_x_deg=50.0))
+    r = _renderer()
+    r.setScene(sc)
+    r.setup()
+    rays, prim, tri, t = r.cast_primary(res, res, sample_index=1)
+    r.dispose()
+    prim_o, tri_o, t_o = OracleScene(sc).trace_f64(rays)
+    frac, rel, nhit = _hit_parity(prim, tri, t, prim_o, tri_o, t_o)
+    print(f"obj: id match {frac:.6f}, max rel t {rel:.2e}, hits {nhit}")
+    assert nhit > 0.5 * res * res and (prim == 1).sum() > 100
+    assert frac >= 0.9999
+    assert rel <= 1e-5
+
+
 def test_soup_5m_full_size_ray_casting_and_ao():
     """The BVH stress configuration at full size: a 5M-triangle soup
     (centroids uniform in the unit cube, lognormal edge lengths, random
