@@ -441,6 +441,14 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
     if ((rc = upload(out.tris, tris, s))) return rc;
     if ((rc = upload(out.tshade, tshade, s))) return rc;
     if ((rc = upload(out.prims, prims, s))) return rc;
+    // 4 x material sort class per primitive (see sort_key_in)
+    std::vector<int32_t> prim_class(std::max<size_t>(1, prims.size()), 0);
+    for (size_t i = 0; i < prims.size(); i++) {
+        const int m = prims[i].material;
+        const int t = (m >= 0 && m < (int)in.mats.size()) ? in.mats[m].type : 0;
+        prim_class[i] = 4 * std::min(t, 6);
+    }
+    if ((rc = upload(out.prim_class, prim_class, s))) return rc;
     if ((rc = upload(out.analytic, analytic, s))) return rc;
     if ((rc = upload(out.mats, mats, s))) return rc;
     if ((rc = upload(out.lobes, lobes, s))) return rc;
@@ -451,6 +459,7 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
     ds.tris = (const float4*)out.tris.ptr;
     ds.tshade = (const float4*)out.tshade.ptr;
     ds.prims = (const DevPrim*)out.prims.ptr;
+    ds.prim_class = (const int32_t*)out.prim_class.ptr;
     ds.analytic = (const int32_t*)out.analytic.ptr;
     ds.mats = (const DevMaterial*)out.mats.ptr;
     ds.lobes = (const DevLobe*)out.lobes.ptr;

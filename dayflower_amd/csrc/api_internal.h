@@ -53,7 +53,7 @@ struct SceneHost {
 
 // A scene compiled into SoA device buffers, resident in HBM.
 struct CompiledScene {
-    DeviceBuffer nodes, tris, tshade, prims, analytic, mats, lobes, lights, perez;
+    DeviceBuffer nodes, tris, tshade, prims, prim_class, analytic, mats, lobes, lights, perez;
     DevScene dev{};
     int max_depth = 1;
     bool area_lights = false;  // any DiffuseAreaLight (the only lights whose NEE needs closest hits)

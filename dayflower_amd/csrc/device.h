@@ -107,6 +107,7 @@ struct DevScene {
     // geometric normal
     const float4* tshade;
     const DevPrim* prims;
+    const int32_t* prim_class;  // per primitive: 4 x the material's sort class
     const int32_t* analytic;  // indices of non-mesh primitives
     const DevMaterial* mats;
     const DevLobe* lobes;

@@ -579,7 +579,7 @@ constexpr int NSORT_KEYS = 32;
 DEV int sort_key_in(const DevScene& sc, int prim, uint32_t info_w) {
     if (prim == -2) return 31;
     if (prim < 0) return 30;
-    return 4 * min(sc.mats[sc.prims[prim].material].type, 6) + (int)((info_w >> INFO_LT_SHIFT) & 3u);
+    return sc.prim_class[prim] + (int)((info_w >> INFO_LT_SHIFT) & 3u);
 }
 DEV int sort_key(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, uint4 info, float4 h) {
     const int prim = __float_as_int(h.y);
