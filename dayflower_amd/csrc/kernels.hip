@@ -873,8 +873,12 @@ DEV void put_shadow(ShadowSoA sh, int s, const ShadowReq& R, int64_t film_idx, i
 // device-wide serial points, so fewer, larger blocks shade faster (512:
 // -6 % against 256; 1024 loses more at the block barriers than it saves).
 constexpr int SHADE_BLOCK = 512;
-template <int BLOCK, bool EXT>
-__global__ __launch_bounds__(BLOCK, 4) void k_shade(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
+// UNI: every job of the launch uses `uscene`, read through the kernel
+// argument so that the scene record's pointers stay in scalar registers
+// (−3 % shade time against the per-job lookup, which puts them in VGPRs).
+template <int BLOCK, bool EXT, bool UNI>
+__global__ __launch_bounds__(BLOCK, 4) void k_shade(const DevScene* __restrict__ scenes, const DevScene* __restrict__ uscene,
+                                                 const DevJob* __restrict__ jobs,
                                                  PathSoA q, const float4* __restrict__ hits,
                                                  const int32_t* __restrict__ perm, PathSoA out, ShadowSoA sh,
                                                  Counters* __restrict__ ctr, float4* __restrict__ film, int use_perm,
@@ -897,7 +901,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_shade(const DevScene* __restrict__
         int prim = __float_as_int(hv.y);
         if (prim != -2) {
             const DevJob& job = jobs[info.z];
-            const DevScene& sc = scenes[job.scene];
+            const DevScene& sc = UNI ? *uscene : scenes[job.scene];
             film_idx = job.film_offset + info.x;
             float4 o4 = q.o[i], d4 = q.d[i], b4 = q.beta[i];
             V o = f4v(o4), d = f4v(d4), beta = f4v(b4);
@@ -1060,12 +1064,22 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     if (timing) hipEventRecord(prof->ev[2], s);
     // shading in 512-thread blocks whatever the traversal block: one queue
     // reservation (two same-address atomics) per 512 paths
-    if (p.ext_features)
-        hipLaunchKernelGGL((k_shade<SHADE_BLOCK, true>), dim3((cap + SHADE_BLOCK - 1) / SHADE_BLOCK), dim3(SHADE_BLOCK), 0, s,
-                           scenes, jobs, w.a, w.hit, w.perm, w.b, w.sh, w.ctr, film, use_perm, w.job_done, cap);
-    else
-        hipLaunchKernelGGL((k_shade<SHADE_BLOCK, false>), dim3((cap + SHADE_BLOCK - 1) / SHADE_BLOCK), dim3(SHADE_BLOCK), 0, s,
-                           scenes, jobs, w.a, w.hit, w.perm, w.b, w.sh, w.ctr, film, use_perm, w.job_done, cap);
+    {
+        const dim3 grid((cap + SHADE_BLOCK - 1) / SHADE_BLOCK), block(SHADE_BLOCK);
+        const DevScene* us = p.uscene;
+        if (p.ext_features && us)
+            hipLaunchKernelGGL((k_shade<SHADE_BLOCK, true, true>), grid, block, 0, s, scenes, us, jobs, w.a, w.hit, w.perm, w.b,
+                               w.sh, w.ctr, film, use_perm, w.job_done, cap);
+        else if (p.ext_features)
+            hipLaunchKernelGGL((k_shade<SHADE_BLOCK, true, false>), grid, block, 0, s, scenes, us, jobs, w.a, w.hit, w.perm, w.b,
+                               w.sh, w.ctr, film, use_perm, w.job_done, cap);
+        else if (us)
+            hipLaunchKernelGGL((k_shade<SHADE_BLOCK, false, true>), grid, block, 0, s, scenes, us, jobs, w.a, w.hit, w.perm, w.b,
+                               w.sh, w.ctr, film, use_perm, w.job_done, cap);
+        else
+            hipLaunchKernelGGL((k_shade<SHADE_BLOCK, false, false>), grid, block, 0, s, scenes, us, jobs, w.a, w.hit, w.perm,
+                               w.b, w.sh, w.ctr, film, use_perm, w.job_done, cap);
+    }
     if (timing) hipEventRecord(prof->ev[3], s);
     hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(1), 0, s, w.ctr);
     if (timing) hipEventRecord(prof->ev[4], s);
