@@ -577,7 +577,7 @@ int Ctx::ensure_wavefront(int capacity) {
     wf.hit = (float4*)A(wf_mem[8], c * 16);
     wf.perm = (int32_t*)A(wf_mem[9], c * 4);
     wf.keys = (int32_t*)A(wf_mem[10], c * 4);
-    wf.bhist = (int32_t*)A(wf_mem[16], 32 * (c / 4096 + 1) * 4);
+    wf.bhist = (int32_t*)A(wf_mem[16], 32 * (c / 4096 + 2) * 4);  // tile histograms + key totals
     wf.sh.o = (float4*)A(wf_mem[11], 2 * c * 16); wf.sh.d = (float4*)A(wf_mem[12], 2 * c * 16);
     wf.sh.c = (float4*)A(wf_mem[13], 2 * c * 16); wf.sh.aux = (int4*)A(wf_mem[14], 2 * c * 16);
     wf.ctr = (Counters*)A(wf_mem[15], sizeof(Counters));
@@ -589,7 +589,7 @@ int Ctx::ensure_wavefront(int capacity) {
 }
 
 size_t wavefront_bytes(int capacity) {
-    return (size_t)capacity * (9 * 16 + 8 + 2 * 4 * 16) + 32 * ((size_t)capacity / 4096 + 1) * 4 + sizeof(Counters);
+    return (size_t)capacity * (9 * 16 + 8 + 2 * 4 * 16) + 32 * ((size_t)capacity / 4096 + 2) * 4 + sizeof(Counters);
 }
 
 // Runs `jobs` (already on the device in job_buf) to completion through the

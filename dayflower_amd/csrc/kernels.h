@@ -74,7 +74,7 @@ struct WavefrontBuffers {
     float4* hit;       // t, prim, tri (int bits)
     int32_t* perm;     // material-sorted order
     int32_t* keys;
-    int32_t* bhist;    // per-tile key histograms / offsets (32 keys * tiles of 4096 entries)
+    int32_t* bhist;    // per-tile key histograms / offsets (32 keys x tiles of 16384 entries), then 32 key totals
     ShadowSoA sh;
     int* overflow;     // traversal-stack overflow
     int capacity;      // path pool size

@@ -781,27 +781,22 @@ __global__ __launch_bounds__(BLOCK) void k_sort_hist(const DevScene* __restrict_
     if (threadIdx.x < NSORT_KEYS) bhist[threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
 }
 
-// Exclusive scan of the key-major tile histograms (NSORT_KEYS * tiles entries;
-// a multiple of 4, at most 1024 * SCAN_CHUNK) by one 1024-thread block: each
-// thread loads a contiguous chunk with int4 loads into registers, the chunk
-// sums are scanned with wave shuffles and one pass over the 16 wave totals,
-// then each thread writes its chunk's prefixes.
-constexpr int SCAN_CHUNK = 128;
-
-__global__ __launch_bounds__(1024) void k_sort_scan(int32_t* __restrict__ bhist, int total) {
-    __shared__ int wsum[16];
-    const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
-    const int chunk = ((total + 1023) / 1024 + 3) & ~3;  // entries per thread, whole int4s
-    const int b0 = min(total, t * chunk), b1 = min(total, b0 + chunk);
-    int4 v[SCAN_CHUNK / 4];
+// Per-key exclusive scan of the tile histograms, one block per key (tiles <=
+// 4096, so at most 16 entries per thread); the key totals go to
+// bhist[NSORT_KEYS * tiles + key] and k_sort_scatter adds their prefix.
+__global__ __launch_bounds__(256) void k_sort_scan_rows(int32_t* __restrict__ bhist, int tiles) {
+    __shared__ int wsum[4];
+    const int key = blockIdx.x, t = threadIdx.x, lane = lane_id(), w = t >> 6;
+    int32_t* row = bhist + key * tiles;
+    const int per = (tiles + 255) / 256;
+    const int b0 = min(tiles, t * per), b1 = min(tiles, b0 + per);
+    int v[16];
     int sum = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_CHUNK / 4; k++) {
-        const int e = b0 + 4 * k;
-        v[k] = e < b1 ? *(const int4*)(bhist + e) : make_int4(0, 0, 0, 0);
+    for (int e = 0; e < 16; e++) {
+        v[e] = b0 + e < b1 ? row[b0 + e] : 0;
+        sum += v[e];
     }
-#pragma unroll
-    for (int k = 0; k < SCAN_CHUNK / 4; k++) sum += v[k].x + v[k].y + v[k].z + v[k].w;
     int incl = sum;
     for (int off = 1; off < 64; off <<= 1) {
         const int u = __shfl_up(incl, off);
@@ -809,26 +804,15 @@ __global__ __launch_bounds__(1024) void k_sort_scan(int32_t* __restrict__ bhist,
     }
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
-    if (t < 16) {
-        int x = wsum[t], y = x;
-        for (int off = 1; off < 16; off <<= 1) {
-            const int u = __shfl_up(y, off, 16);
-            if (t >= off) y += u;
-        }
-        wsum[t] = y - x;  // exclusive prefix of the wave totals
-    }
-    __syncthreads();
-    int run = wsum[w] + incl - sum;
+    int pre = 0;
+    for (int k = 0; k < w; k++) pre += wsum[k];
+    int run = pre + incl - sum;
 #pragma unroll
-    for (int k = 0; k < SCAN_CHUNK / 4; k++) {
-        const int e = b0 + 4 * k;
-        int4 o;
-        o.x = run; run += v[k].x;
-        o.y = run; run += v[k].y;
-        o.z = run; run += v[k].z;
-        o.w = run; run += v[k].w;
-        if (e < b1) *(int4*)(bhist + e) = o;
+    for (int e = 0; e < 16; e++) {
+        if (b0 + e < b1) row[b0 + e] = run;
+        run += v[e];
     }
+    if (t == 255) bhist[NSORT_KEYS * tiles + key] = pre + incl;
 }
 
 template <int BLOCK>
@@ -839,7 +823,16 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scatter(const int32_t* __restric
     const int t0 = blockIdx.x * SORT_TILE;
     const int lane = lane_id();
     const uint64_t lt = (1ull << lane) - 1ull;
-    if (threadIdx.x < NSORT_KEYS) base[threadIdx.x] = boff[threadIdx.x * gridDim.x + blockIdx.x];
+    if (threadIdx.x < NSORT_KEYS) {
+        // exclusive prefix of the key totals (lanes 0..31 of wave 0)
+        const int tot = boff[NSORT_KEYS * gridDim.x + threadIdx.x];
+        int incl = tot;
+        for (int off = 1; off < NSORT_KEYS; off <<= 1) {
+            const int u = __shfl_up(incl, off);
+            if ((int)threadIdx.x >= off) incl += u;
+        }
+        base[threadIdx.x] = boff[threadIdx.x * gridDim.x + blockIdx.x] + incl - tot;
+    }
     __syncthreads();
     for (int k = threadIdx.x; k < SORT_TILE; k += BLOCK) {
         const int i = t0 + k;
@@ -1057,10 +1050,10 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     else launch_trace_queue<B, false, false>(egrid, scenes, jobs, film, w, p, trav, s);
     if (timing) hipEventRecord(prof->ev[1], s);
     if (use_perm) {
-        const int tiles = (cap + SORT_TILE - 1) / SORT_TILE;  // NSORT_KEYS * tiles <= 1024 * SCAN_CHUNK
+        const int tiles = (cap + SORT_TILE - 1) / SORT_TILE;  // <= 4096 (k_sort_scan_rows)
         hipLaunchKernelGGL((k_sort_hist<256>), dim3(tiles), dim3(256), 0, s, scenes, jobs, w.a, w.hit, w.keys, w.bhist,
                            w.ctr, BVH_WIDTH == 4 && p.uscene ? 1 : 0);
-        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, w.bhist, NSORT_KEYS * tiles);
+        hipLaunchKernelGGL(k_sort_scan_rows, dim3(NSORT_KEYS), dim3(256), 0, s, w.bhist, tiles);
         hipLaunchKernelGGL((k_sort_scatter<256>), dim3(tiles), dim3(256), 0, s, w.keys, w.bhist, w.perm, w.ctr);
     }
     if (timing) hipEventRecord(prof->ev[2], s);

@@ -441,3 +441,26 @@ def test_rgba8_image_matches_oracle_for_every_tone_mapper():
     ref = oracle_lib.image_rgba8(g, abi.TONE_MAPPER_REINHARD, abi.ALGO_AMBIENT_OCCLUSION)
     d = np.abs(img.astype(np.int32) - ref)
     assert (d == 0).mean() >= 0.9999 and d.max() <= 1
+
+
+def test_path_pool_size_does_not_change_the_image():
+    """A path's result depends only on its random stream, so pools of 1024
+    and 5000 paths (many refills, partial sort tiles) and the default pool
+    give the same film up to the order of the film's float additions."""
+    res = 64
+    sc = S.cornell_box(res, res)
+    films = []
+    for pool in (1024, 5000, None):
+        r = _renderer()
+        r.setScene(sc)
+        r.setSeed(17)
+        if pool:
+            r.setWavefront(pool_paths=pool)
+        r.setup()
+        r.render(4)
+        g, cnt = r.film(res, res)
+        r.dispose()
+        assert (cnt == 4).all()
+        films.append(g)
+    for g in films[:2]:
+        np.testing.assert_allclose(g, films[2], rtol=1e-4, atol=1e-6)
