@@ -575,6 +575,7 @@ __global__ void k_commit(Counters* ctr, int capacity) {
 // the same stateless random number the shade kernel draws.  30 = miss,
 // 31 = dropped camera sample.
 constexpr int NSORT_KEYS = 32;
+constexpr int SORT_TILE = 16384;
 
 DEV int sort_key_in(const DevScene& sc, int prim, uint32_t info_w) {
     if (prim == -2) return 31;
@@ -610,9 +611,11 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                                                          float4* __restrict__ hits, float4* __restrict__ film,
                                                          const Counters* __restrict__ ctr, int* __restrict__ overflow,
                                                          unsigned long long* __restrict__ trav,
-                                                         const DevScene* __restrict__ uscene, int32_t* __restrict__ keys) {
+                                                         const DevScene* __restrict__ uscene, int32_t* __restrict__ keys,
+                                                         int32_t* __restrict__ bhist, int tiles) {
     __shared__ int lds[SHORT * BLOCK];
     __shared__ int next;
+    __shared__ int s_hist[NSORT_KEYS];
     // scene-uniform extension with sorted shading: the finished ray writes its
     // sort key (the material of the hit, the NEE light type from info.w)
     uint32_t info_w = 0;
@@ -621,6 +624,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
     const int base = blockIdx.x * RPL * BLOCK;
     const int limit = min(n - base, RPL * BLOCK);
     if (threadIdx.x == 0) next = BLOCK;
+    if (UNI && !SHADOW && threadIdx.x < NSORT_KEYS) s_hist[threadIdx.x] = 0;
     __syncthreads();
     if (limit <= 0) return;
     const int lane = lane_id();
@@ -716,7 +720,11 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                 }
             } else {
                 hits[i] = make_float4(tbest, __int_as_float(prim), __int_as_float(tri), 0.0f);
-                if (UNI && keys) keys[i] = sort_key_in(*uscene, prim, info_w);
+                if (UNI && keys) {
+                    const int key = sort_key_in(*uscene, prim, info_w);
+                    keys[i] = key;
+                    atomicAdd(&s_hist[key], 1);
+                }
             }
         }
         const uint64_t m = __ballot(finished);
@@ -733,21 +741,26 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
         }
         if (__ballot(active) == 0) break;
     }
+    if (UNI && !SHADOW && keys) {
+        // the block's range lies in one sort tile: add its key counts
+        __syncthreads();
+        if (threadIdx.x < NSORT_KEYS && s_hist[threadIdx.x])
+            atomicAdd(&bhist[threadIdx.x * tiles + base / SORT_TILE], s_hist[threadIdx.x]);
+    }
     if (COUNT) add_visit_counts<BLOCK>(trav + (SHADOW ? 2 : 0), vc);
 }
 
 // Counting sort of the path queue by sort key, over tiles of SORT_TILE
-// entries: per-tile key histograms (wave ballots, no atomics on HBM), one
-// exclusive scan of the key-major histogram array, then a scatter in which
-// every wave ranks its entries per key with ballots and takes its base with
-// one LDS atomic per key.
-constexpr int SORT_TILE = 16384;
+// entries: per-tile key histograms (here with wave ballots, or added by the
+// scene-uniform extension kernel), per-key scans of the key-major histogram
+// array, then a scatter in which every wave ranks its entries per key with
+// ballots and takes its base with one LDS atomic per key.
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sort_hist(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs,
                                                      PathSoA q, const float4* __restrict__ hits,
                                                      int32_t* __restrict__ keys, int32_t* __restrict__ bhist,
-                                                     const Counters* __restrict__ ctr, int keys_ready) {
+                                                     const Counters* __restrict__ ctr) {
     __shared__ int hist[NSORT_KEYS];
     const int n = ctr->n_active;
     const int t0 = blockIdx.x * SORT_TILE;
@@ -760,13 +773,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_hist(const DevScene* __restrict_
         const int i = t0 + k;
         int key = -1;
         if (i < n) {
-            // the scene-uniform extension kernel has written the keys
-            if (keys_ready) {
-                key = keys[i];
-            } else {
-                key = sort_key(scenes, jobs, q.info[i], hits[i]);
-                keys[i] = key;
-            }
+            key = sort_key(scenes, jobs, q.info[i], hits[i]);
+            keys[i] = key;
         }
         uint64_t left = __ballot(key >= 0);
         while (left) {
@@ -1026,11 +1034,16 @@ static void launch_trace_queue(int grid, const DevScene* scenes, const DevJob* j
     if (BVH_WIDTH == 4 && p.uscene)
         hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, SHADOW ? SHADOW_RPL : EXTEND_RPL, COUNT, SHADOW, true>), dim3(grid), dim3(B), 0, s,
                            scenes, jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav, p.uscene,
-                           SHADOW || !p.sort_keys ? nullptr : w.keys);
+                           SHADOW || !p.sort_keys ? nullptr : w.keys, w.bhist, (w.capacity + SORT_TILE - 1) / SORT_TILE);
     else
         hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, SHADOW ? SHADOW_RPL : EXTEND_RPL, COUNT, SHADOW, false>), dim3(grid), dim3(B), 0, s,
                            scenes, jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav, p.uscene,
-                           SHADOW || !p.sort_keys ? nullptr : w.keys);
+                           SHADOW || !p.sort_keys ? nullptr : w.keys, w.bhist, (w.capacity + SORT_TILE - 1) / SORT_TILE);
+}
+
+__global__ void k_zero_i32(int32_t* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 template <int B>
@@ -1043,6 +1056,12 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, s, w.ctr, cap);
     const bool timing = prof && prof->timing;
     const int use_perm = p.sort_keys > 0;
+    const int tiles = (cap + SORT_TILE - 1) / SORT_TILE;  // <= 4096 (k_sort_scan_rows)
+    // the scene-uniform extension kernel writes the sort keys and adds their
+    // per-tile counts: clear the counts first
+    const bool fused_hist = use_perm && BVH_WIDTH == 4 && p.uscene;
+    if (fused_hist)
+        hipLaunchKernelGGL(k_zero_i32, dim3((NSORT_KEYS * tiles + 255) / 256), dim3(256), 0, s, w.bhist, NSORT_KEYS * tiles);
     if (timing) hipEventRecord(prof->ev[0], s);
     unsigned long long* trav = prof ? prof->trav : nullptr;
     const int egrid = (cap + EXTEND_RPL * B - 1) / (EXTEND_RPL * B);
@@ -1050,9 +1069,9 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     else launch_trace_queue<B, false, false>(egrid, scenes, jobs, film, w, p, trav, s);
     if (timing) hipEventRecord(prof->ev[1], s);
     if (use_perm) {
-        const int tiles = (cap + SORT_TILE - 1) / SORT_TILE;  // <= 4096 (k_sort_scan_rows)
-        hipLaunchKernelGGL((k_sort_hist<256>), dim3(tiles), dim3(256), 0, s, scenes, jobs, w.a, w.hit, w.keys, w.bhist,
-                           w.ctr, BVH_WIDTH == 4 && p.uscene ? 1 : 0);
+        if (!fused_hist)
+            hipLaunchKernelGGL((k_sort_hist<256>), dim3(tiles), dim3(256), 0, s, scenes, jobs, w.a, w.hit, w.keys, w.bhist,
+                               w.ctr);
         hipLaunchKernelGGL(k_sort_scan_rows, dim3(NSORT_KEYS), dim3(256), 0, s, w.bhist, tiles);
         hipLaunchKernelGGL((k_sort_scatter<256>), dim3(tiles), dim3(256), 0, s, w.keys, w.bhist, w.perm, w.ctr);
     }
