@@ -1096,14 +1096,18 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(1), 0, s, w.ctr);
     if (timing) hipEventRecord(prof->ev[4], s);
     const int sgrid = (int)((2 * (int64_t)cap + B - 1) / B);  // shadow requests <= 2 per path
-    const int agrid = (int)((2 * (int64_t)cap + SHADOW_RPL * B - 1) / (SHADOW_RPL * B));
+    // the any-hit queue runs in blocks of at most 128 threads (−0.8 % against
+    // 256; the extension rays prefer 256); the thread count, and so the
+    // overflow stack, is the same
+    constexpr int SB = B < 128 ? B : 128;
+    const int agrid = (int)((2 * (int64_t)cap + SHADOW_RPL * SB - 1) / (SHADOW_RPL * SB));
     if (trav) {
-        launch_trace_queue<B, true, true>(agrid, scenes, jobs, film, w, p, trav, s);
+        launch_trace_queue<SB, true, true>(agrid, scenes, jobs, film, w, p, trav, s);
         if (p.closest_shadow)
             hipLaunchKernelGGL((k_shadow_closest<B, SHORT_STACK, true>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh,
                                w.ctr, film, w.overflow, trav, cap);
     } else {
-        launch_trace_queue<B, false, true>(agrid, scenes, jobs, film, w, p, trav, s);
+        launch_trace_queue<SB, false, true>(agrid, scenes, jobs, film, w, p, trav, s);
         if (p.closest_shadow)
             hipLaunchKernelGGL((k_shadow_closest<B, SHORT_STACK, false>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh,
                                w.ctr, film, w.overflow, trav, cap);
