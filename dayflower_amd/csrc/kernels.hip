@@ -513,6 +513,8 @@ DEV int find_job(const DevJob* jobs, int n_jobs, int64_t item) {
 // Paths carry it in bits 9-10 of info.w, so the sort key of a hit needs only
 // the hit's material.
 constexpr uint32_t INFO_LT_SHIFT = 9;
+// bits of info.w that path records also keep in the origin's w
+constexpr uint32_t ORIGIN_W_BITS = 0x80000000u | (3u << INFO_LT_SHIFT);
 DEV uint32_t nee_light_type(const DevScene& sc, const Rng& rng, uint32_t bounce) {
     if (sc.n_lights == 0) return 3u;
     const int li = min((int)(rng.get(DIM_BOUNCE + DIMS_PER_BOUNCE * bounce) * sc.n_lights), sc.n_lights - 1);
@@ -548,7 +550,7 @@ __global__ void k_generate(const DevScene* __restrict__ scenes, const DevJob* __
         unsafeAtomicAdd(&film[job.film_offset + pix].w, 1.0f);
     }
     flags |= nee_light_type(scenes[job.scene], rng, 0u) << INFO_LT_SHIFT;
-    out.o[slot] = make_float4(o.x, o.y, o.z, 0.0f);
+    out.o[slot] = make_float4(o.x, o.y, o.z, __uint_as_float(flags & ORIGIN_W_BITS));
     out.d[slot] = make_float4(d.x, d.y, d.z, 1.0f);
     out.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     out.info[slot] = make_uint4((uint32_t)pix, sample, (uint32_t)j, flags);
@@ -666,6 +668,14 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                 const float4 o4 = sh.o[i];
                 tbest = o4.w;
                 setup(o4, sh.d[i], UNI ? 0u : (uint32_t)sh.aux[i].y);
+            } else if (UNI) {
+                // the origin's w carries the dropped flag and the NEE light
+                // type: no info load
+                const float4 o4 = q.o[i], d4 = q.d[i];
+                info_w = __float_as_uint(o4.w);
+                tbest = FLT_MAX;
+                if (info_w & 0x80000000u) prim = -2;  // dropped camera sample
+                else setup(o4, d4, 0u);
             } else {
                 // the three loads issued together (the ray's are wasted on a
                 // dropped camera sample)
@@ -972,7 +982,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_shade(const DevScene* __restrict__
     int s_path, s_any, s_cl;
     block_reserve<BLOCK>(ctr, cont, nany, ncl, s_path, s_any, s_cl);
     if (cont) {
-        out.o[s_path] = make_float4(no.x, no.y, no.z, 0.0f);
+        out.o[s_path] = make_float4(no.x, no.y, no.z, __uint_as_float(ninfo.w & ORIGIN_W_BITS));
         out.d[s_path] = make_float4(nd.x, nd.y, nd.z, neta);
         out.beta[s_path] = make_float4(nbeta.x, nbeta.y, nbeta.z, 0.0f);
         out.info[s_path] = ninfo;
