@@ -1916,6 +1916,21 @@ int32_t orc_image_rgba8(int32_t algorithm, int32_t tone_mapper, int64_t n, const
     return 0;
 }
 
+// The sun of a Perez light: its zenith angle in the light's object frame,
+// the attenuated solar spectrum (91 samples, 350..800 nm step 5; zeros when
+// the sun is below the horizon) and the linear sRGB sun colour.
+int32_t orc_perez_sun(float turbidity, const float* sun, float* theta, float* spectrum, float* rgb) {
+    Perez p;
+    perezInit(p, turbidity, V3(sun[0], sun[1], sun[2]));
+    if (theta) *theta = p.theta;
+    if (spectrum) {
+        for (int i = 0; i < 91; i++) spectrum[i] = 0.0f;
+        if (p.sunObject.z > 0.0f) sunSpectrum(p.theta, turbidity, spectrum);
+    }
+    if (rgb) { rgb[0] = p.sunColor.r; rgb[1] = p.sunColor.g; rgb[2] = p.sunColor.b; }
+    return 0;
+}
+
 int32_t orc_perez_sky(float turbidity, const float* sun, const float* dir, float* rgb) {
     Perez p;
     perezInit(p, turbidity, V3(sun[0], sun[1], sun[2]));

@@ -21,7 +21,11 @@
  *                   to one, sampled-direction histograms against the pdf
  *   integrators     analytic furnace scene (Le/(1-a)), AO of an open plane
  *                   and of a box interior in closed form
- *   Perez sky       parity unpinned (no reference test, no runnable JVM)
+ *   Perez sky       zenith luminance and chromaticity in closed form,
+ *                   symmetry, horizon, pdf normalisation
+ *   Perez sun       attenuated spectrum against Preetham et al.'s
+ *                   transmittance formulas in float64, colour against an
+ *                   independent CIE integration and sRGB matrix
  *   image stage     textbook sRGB bytes (linear 0.5 -> 188, 0.18 -> 118),
  *                   tone curves at closed-form points (Reinhard 1 -> 1/2,
  *                   white point 4 -> 1, 1 - e^-ln2 = 1/2, ACES fit at 1),
@@ -87,6 +91,7 @@ int32_t orc_bsdf_eval(const dfr_material* m, const float* wo, const float* wi, f
 int32_t orc_bsdf_sample(const dfr_material* m, const float* wo, float u, float v, float* wi, float* f,
                         float* pdf, int32_t* type);
 int32_t orc_perez_sky(float turbidity, const float* sun, const float* dir, float* rgb);
+int32_t orc_perez_sun(float turbidity, const float* sun, float* theta, float* spectrum, float* rgb);
 int32_t orc_perez_pdf_many(float turbidity, const float* sun, int64_t n, const float* dirs, float* pdf);
 int32_t orc_perez_sample_many(float turbidity, const float* sun, int64_t n, const float* uvw, const float* nrm,
                               float* out);

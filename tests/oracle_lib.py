@@ -67,6 +67,7 @@ def load():
     lib.orc_bsdf_eval.argtypes = [P(abi.Material), vp, vp, vp, vp]
     lib.orc_bsdf_sample.argtypes = [P(abi.Material), vp, f, f, vp, vp, vp, vp]
     lib.orc_perez_sky.argtypes = [f, vp, vp, vp]
+    lib.orc_perez_sun.argtypes = [f, vp, vp, vp, vp]
     lib.orc_perez_pdf_many.argtypes = [f, vp, i64, vp, vp]
     lib.orc_perez_sample_many.argtypes = [f, vp, i64, vp, vp, vp]
     lib.orc_tone_map.argtypes = [i32, i64, vp, vp]

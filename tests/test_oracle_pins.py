@@ -1,0 +1,263 @@
+"""Closed-form pins of oracle functions that reciprocity and furnace tests
+leave open: the Perez sun (attenuated solar spectrum and its colour), the
+Disney BSDF, Oren-Nayar's A and B, and the specular Mirror and smooth Glass
+lobes.  Every expected value is computed here in float64 from the published
+formulas; a dropped factor, a wrong sign or a swapped operand in the oracle
+fails one of them.
+
+Reference files (M/ = src/main/java/org/dayflower/): PerezLight.java:691-697,
+728-752; curve/SpectralCurveF.java:62-82; scene/material/DisneyMaterial.java
+:1121-1200 with scene/bxdf/Disney*BRDF.java and fresnel/DisneyFresnel.java;
+scene/bxdf/OrenNayarBRDF.java:73-74, 190-210; scene/material/MirrorMaterial.java
+:235; scene/bxdf/FresnelSpecularBXDF.java.
+"""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from dayflower_amd import scene as S
+from tests import oracle_lib
+
+TABLES = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "dayflower_amd", "csrc",
+                      "spectral_tables.inc")
+
+
+def _table(name):
+    """A published table of the spectral data file (CIE 1931 observer,
+    Preetham et al.'s solar spectrum and absorption coefficients)."""
+    text = open(TABLES).read()
+    m = re.search(r"\b%s\[\d+\]\s*=\s*\{([^}]*)\}" % re.escape(name), text)
+    return np.array([float(x.rstrip("f")) for x in m.group(1).replace("\n", " ").split(",") if x.strip()])
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _sun(oracle, turbidity, sun_dir):
+    theta = np.zeros(1, np.float32)
+    spec = np.zeros(91, np.float32)
+    rgb = np.zeros(3, np.float32)
+    oracle.orc_perez_sun(float(turbidity), _p(np.asarray(sun_dir, np.float32)), _p(theta), _p(spec), _p(rgb))
+    return float(theta[0]), spec.astype(np.float64), rgb.astype(np.float64)
+
+
+def _preetham_sun_spectrum(zenith, turbidity):
+    """Spectral radiance of the sun after the atmosphere (Preetham, Shirley and
+    Smits 1999, appendix A.2) at 350, 355, .., 800 nm, in float64:
+    relative optical mass m (Kasten's formula, zenith angle in degrees),
+    Rayleigh tau_r = exp(-0.008735 m lambda^-4.08), Angstrom aerosols
+    tau_a = exp(-beta m lambda^-1.3) with beta = 0.04608365822050 T -
+    0.04586025928522, ozone tau_o = exp(-k_o l m), l = 0.35 cm, mixed gases
+    tau_g = exp(-1.41 k_g m / (1 + 118.93 k_g m)^0.45), water vapour
+    tau_wa = exp(-0.2385 k_wa w m / (1 + 20.07 k_wa w m)^0.45), w = 2 cm;
+    lambda in micrometres, tabulated data interpolated linearly."""
+    deg = math.degrees(zenith)
+    m = 1.0 / (math.cos(zenith) + 0.15 * (93.885 - deg) ** -1.253)
+    lam = np.arange(350, 801, 5, dtype=np.float64)
+    mu = lam / 1000.0
+    beta = 0.04608365822050 * turbidity - 0.04586025928522
+    sol = np.interp(lam, np.arange(380, 751, 10), _table("kSol"), left=0.0, right=0.0)
+    ko = np.interp(lam, _table("kOzoneWl"), _table("kOzoneAmp"))
+    kg = np.interp(lam, _table("kGasWl"), _table("kGasAmp"))
+    kw = np.interp(lam, _table("kWaterWl"), _table("kWaterAmp"))
+    tr = np.exp(-m * 0.008735 * mu ** -4.08)
+    ta = np.exp(-m * beta * mu ** -1.3)
+    to = np.exp(-m * ko * 0.35)
+    tg = np.exp(-1.41 * kg * m / (1.0 + 118.93 * kg * m) ** 0.45)
+    tw = np.exp(-0.2385 * kw * 2.0 * m / (1.0 + 20.07 * kw * 2.0 * m) ** 0.45)
+    return sol * tr * ta * to * tg * tw
+
+
+def _sun_dir(zenith_deg):
+    """World direction of a sun at `zenith_deg` from the world's up axis (+y)."""
+    z = math.radians(zenith_deg)
+    return np.array([math.sin(z), math.cos(z), 0.0])
+
+
+@pytest.mark.parametrize("turbidity", [2.0, 6.0])
+@pytest.mark.parametrize("zenith_deg", [15.0, 40.0, 75.0])
+def test_perez_sun_spectrum_matches_preetham_transmittance(oracle, turbidity, zenith_deg):
+    theta, spec, _ = _sun(oracle, turbidity, _sun_dir(zenith_deg))
+    assert theta == pytest.approx(math.radians(zenith_deg), abs=2e-6)
+    expect = _preetham_sun_spectrum(theta, turbidity)
+    # every one of the 91 samples (the solar table is zero outside 380..750 nm);
+    # the reference writes Kasten's air mass in radians with the constant
+    # 0.000940 (0.15 * (180/pi)^-1.253 = 0.00093975), which moves a low sun's
+    # spectrum by a few parts per million
+    assert np.count_nonzero(expect) >= 75
+    np.testing.assert_allclose(spec, expect, rtol=2e-5, atol=1e-30)
+
+
+def _xyz_to_rgb():
+    """Inverse of the sRGB RGB -> XYZ matrix built from the primaries (0.64,
+    0.33), (0.30, 0.60), (0.15, 0.06) and white (0.31271, 0.32902) so that
+    RGB (1, 1, 1) is the white point with Y = 1."""
+    prim = np.array([[0.64, 0.33], [0.30, 0.60], [0.15, 0.06]])
+    xyz = np.array([[x / y, 1.0, (1 - x - y) / y] for x, y in prim]).T
+    wx, wy = 0.31271, 0.32902
+    white = np.array([wx / wy, 1.0, (1 - wx - wy) / wy])
+    return np.linalg.inv(xyz * np.linalg.solve(xyz, white))
+
+
+@pytest.mark.parametrize("turbidity", [2.0, 6.0])
+@pytest.mark.parametrize("zenith_deg", [15.0, 40.0, 75.0])
+def test_perez_sun_colour_from_its_spectrum(oracle, turbidity, zenith_deg):
+    """Sun colour = max(0, sRGB(XYZ of the attenuated spectrum) * 1e-4).  XYZ
+    integrates the spectrum (linear between its 5 nm samples, zero outside
+    350..800 nm) against the CIE 1931 observer at 360..830 nm, normalised as
+    SpectralCurveF.toColorXYZ does: (830 - 360) / (CIE Y integral * 471)."""
+    theta, _, rgb = _sun(oracle, turbidity, _sun_dir(zenith_deg))
+    spec = _preetham_sun_spectrum(theta, turbidity)
+    lam = _table("kCieLambda")
+    s = np.interp(lam, np.arange(350, 801, 5), spec, left=0.0, right=0.0)
+    scale = (lam[-1] - lam[0]) / (106.856895 * lam.size)
+    xyz = scale * np.array([np.sum(s * _table("kCieX")), np.sum(s * _table("kCieY")), np.sum(s * _table("kCieZ"))])
+    expect = np.maximum(_xyz_to_rgb() @ (xyz * 1e-4), 0.0)
+    np.testing.assert_allclose(rgb, expect, rtol=2e-4, atol=1e-6)
+    assert rgb.min() > 0.0
+
+
+def test_perez_sun_reddens_with_lower_sun_and_higher_turbidity(oracle):
+    """Rayleigh and aerosol extinction grow toward short wavelengths and with
+    the air mass: the blue-to-red ratio of the sun falls as the sun sinks and
+    as the turbidity rises; below the horizon the sun is black."""
+    def br(t, z):
+        _, _, c = _sun(oracle, t, _sun_dir(z))
+        return c[2] / c[0]
+    for t in (2.0, 4.0, 8.0):
+        ratios = [br(t, z) for z in (10.0, 35.0, 60.0, 80.0)]
+        assert all(a > b for a, b in zip(ratios, ratios[1:])), (t, ratios)
+    for z in (20.0, 60.0):
+        ratios = [br(t, z) for t in (2.0, 4.0, 8.0)]
+        assert all(a > b for a, b in zip(ratios, ratios[1:])), (z, ratios)
+    _, spec, c = _sun(oracle, 3.0, _sun_dir(100.0))
+    assert not spec.any() and not c.any()
+
+
+# ---------------------------------------------------------------- BSDF lobes
+def _eval(oracle, m, wo, wi):
+    f = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    ok = oracle.orc_bsdf_eval(ctypes.byref(m), _p(np.asarray(wo, np.float32)), _p(np.asarray(wi, np.float32)),
+                              _p(f), _p(pdf))
+    assert ok
+    return f.astype(np.float64), float(pdf[0])
+
+
+def _sample(oracle, m, wo, u, v):
+    wi = np.zeros(3, np.float32)
+    f = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    ty = np.zeros(1, np.int32)
+    ok = oracle.orc_bsdf_sample(ctypes.byref(m), _p(np.asarray(wo, np.float32)), float(u), float(v), _p(wi), _p(f),
+                                _p(pdf), _p(ty))
+    return bool(ok), wi.astype(np.float64), f.astype(np.float64), float(pdf[0]), int(ty[0])
+
+
+def test_disney_at_normal_incidence(oracle):
+    """wo = wi = n: the Schlick weights vanish, so the diffuse lobe is
+    color (1 - metallic)(1 - specTrans) / pi and the retro-reflection and
+    sheen lobes are 0; the half vector is n, so the Trowbridge-Reitz D is
+    1 / (pi ax ay), G = 1 and the specular lobe is F / (4 pi ax ay) with the
+    Disney Fresnel F = (1 - metallic) ((eta-1)/(eta+1))^2 + metallic r0;
+    the clear coat is weight * G * 0.04 * GTR1(1, gloss) / 4 with
+    G = (1 / (1 + 1))^2.  The pdf averages five lobes: three cosine lobes
+    (1/pi), the microfacet lobe (D/4) and the clear coat (GTR1/4)."""
+    color = np.array([0.8, 0.5, 0.3])
+    metal, rough, aniso, sheen, cc, gloss, tint, eta = 0.3, 0.5, 0.5, 0.5, 0.7, 0.6, 0.4, 1.5
+    m = S.disney(tuple(color), metallic=metal, roughness=rough, specular_tint=tint, anisotropic=aniso, sheen=sheen,
+                 clearcoat=cc, clearcoat_gloss=gloss, eta=eta)
+    f, pdf = _eval(oracle, m, (0, 0, 1), (0, 0, 1))
+    aspect = math.sqrt(1.0 - 0.9 * aniso)
+    ax, ay = rough * rough / aspect, rough * rough * aspect
+    f0 = ((eta - 1) / (eta + 1)) ** 2
+    lum = color @ np.array([0.212671, 0.715160, 0.072169])
+    r0 = (1 - metal) * (((1 - tint) + tint * color / lum) * f0) + metal * color
+    F = (1 - metal) * f0 + metal * r0
+    D = 1.0 / (math.pi * ax * ay)
+    a2 = (0.1 + (0.001 - 0.1) * gloss) ** 2
+    gtr1 = (a2 - 1.0) / (math.pi * math.log(a2) * a2)
+    expect = color * (1 - metal) / math.pi + F * D / 4.0 + cc * 0.25 * 0.04 * gtr1 / 4.0
+    np.testing.assert_allclose(f, expect, rtol=2e-5)
+    assert pdf == pytest.approx((3.0 / math.pi + D / 4.0 + gtr1 / 4.0) / 5.0, rel=2e-5)
+
+
+def test_oren_nayar_a_and_b(oracle):
+    """f = R/pi (A + B max(0, cos(phi_i - phi_o)) sin(alpha) tan(beta)) with
+    sigma the angle in radians, A = 1 - sigma^2 / (2 (sigma^2 + 0.33)),
+    B = 0.45 sigma^2 / (sigma^2 + 0.09), alpha = max(theta_i, theta_o),
+    beta = min(theta_i, theta_o)."""
+    R = np.array([0.9, 0.6, 0.3])
+    deg = 25.0
+    s2 = math.radians(deg) ** 2
+    A = 1.0 - s2 / (2.0 * (s2 + 0.33))
+    B = 0.45 * s2 / (s2 + 0.09)
+    m = S.matte(tuple(R), angle=deg)
+
+    def d(theta, phi):
+        t, p_ = math.radians(theta), math.radians(phi)
+        return (math.sin(t) * math.cos(p_), math.sin(t) * math.sin(p_), math.cos(t))
+
+    for ti, to, dphi in ((60.0, 30.0, 0.0), (30.0, 60.0, 0.0), (50.0, 20.0, 45.0), (70.0, 10.0, 90.0),
+                         (40.0, 40.0, 180.0)):
+        f, pdf = _eval(oracle, m, d(to, 10.0), d(ti, 10.0 + dphi))
+        a, b = math.radians(max(ti, to)), math.radians(min(ti, to))
+        k = A + B * max(0.0, math.cos(math.radians(dphi))) * math.sin(a) * math.tan(b)
+        np.testing.assert_allclose(f, R / math.pi * k, rtol=2e-5)
+        assert pdf == pytest.approx(math.cos(math.radians(ti)) / math.pi, rel=1e-5)
+    # A and B move the value off Lambert's in both directions
+    f_back, _ = _eval(oracle, m, d(60.0, 0.0), d(60.0, 180.0))
+    f_fwd, _ = _eval(oracle, m, d(60.0, 0.0), d(60.0, 0.0))
+    assert f_back[0] < R[0] / math.pi < f_fwd[0]
+
+
+def test_mirror_reflects_its_colour(oracle):
+    """MirrorMaterial: a specular reflection with Fresnel 1: the sampled
+    direction is the mirror image of wo and f |cos| / pdf = KR; evaluating
+    f or the pdf for any direction gives 0 (a delta lobe)."""
+    kr = np.array([0.9, 0.7, 0.4])
+    m = S.mirror(tuple(kr))
+    for wo in ((0.0, 0.0, 1.0), (0.6, 0.0, 0.8), (-0.3, 0.4, math.sqrt(0.75))):
+        ok, wi, f, pdf, _ = _sample(oracle, m, wo, 0.3, 0.7)
+        assert ok
+        np.testing.assert_allclose(wi, (-wo[0], -wo[1], wo[2]), atol=1e-6)
+        np.testing.assert_allclose(f * abs(wi[2]) / pdf, kr, rtol=1e-6)
+        fe, pe = _eval(oracle, m, wo, wi)
+        assert not fe.any() and pe == 0.0
+
+
+def _fresnel_dielectric(ci, eta):
+    """Unpolarised Fresnel reflectance from air into a medium of index eta
+    (Snell's law and the two polarisations)."""
+    st = math.sqrt(max(0.0, 1 - ci * ci)) / eta
+    ct = math.sqrt(1 - st * st)
+    rs = (ci - eta * ct) / (ci + eta * ct)
+    rp = (eta * ci - ct) / (eta * ci + ct)
+    return 0.5 * (rs * rs + rp * rp)
+
+
+@pytest.mark.parametrize("theta_deg", [0.0, 40.0, 75.0])
+def test_smooth_glass_reflects_and_refracts_by_fresnel(oracle, theta_deg):
+    """Smooth GlassMaterial (one Fresnel-specular lobe, eta 1.5): u below the
+    Fresnel reflectance F reflects with pdf F and f |cos| / pdf = KR; above
+    it refracts along Snell's direction with pdf 1 - F and f |cos| / pdf =
+    KT / eta^2 (radiance compressed entering the denser medium)."""
+    eta, kr, kt = 1.5, np.array([0.9, 0.8, 0.7]), np.array([0.6, 0.9, 1.0])
+    m = S.glass(eta, kr=tuple(kr), kt=tuple(kt))
+    t = math.radians(theta_deg)
+    wo = (math.sin(t), 0.0, math.cos(t))
+    F = _fresnel_dielectric(math.cos(t), eta)
+    ok, wi, f, pdf, _ = _sample(oracle, m, wo, F * 0.5, 0.5)
+    assert ok and pdf == pytest.approx(F, rel=1e-5)
+    np.testing.assert_allclose(wi, (-wo[0], 0.0, wo[2]), atol=1e-6)
+    np.testing.assert_allclose(f * abs(wi[2]) / pdf, kr, rtol=1e-5)
+    ok, wi, f, pdf, _ = _sample(oracle, m, wo, F + (1 - F) * 0.5, 0.5)
+    assert ok and pdf == pytest.approx(1 - F, rel=1e-5)
+    st = math.sin(t) / eta
+    np.testing.assert_allclose(wi, (-st, 0.0, -math.sqrt(1 - st * st)), atol=2e-6)
+    np.testing.assert_allclose(f * abs(wi[2]) / pdf, kt / eta ** 2, rtol=1e-5)
