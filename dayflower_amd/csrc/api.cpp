@@ -400,8 +400,24 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
         std::memcpy(d.radiance, L.radiance, 12);
         std::memcpy(d.position, L.position, 12);
         d.perez = -1;
-        if (L.type == DFR_LIGHT_DIFFUSE_AREA) {
-            if (L.triangle < 0 || L.triangle >= nt) { err = "area light needs a triangle"; return DFR_E_UNSUPPORTED; }
+        if (L.type == DFR_LIGHT_DIFFUSE_AREA && L.triangle < 0) {
+            // a sphere primitive (Sphere3F.sample, Sphere3F.java:156-226)
+            const int sp = L.primitive;
+            if (sp < 0 || sp >= (int)in.prims.size() || in.prims[sp].shape != DFR_SHAPE_SPHERE) {
+                err = "light " + std::to_string(l) + ": an area light is a triangle of a mesh or a sphere primitive";
+                return DFR_E_UNSUPPORTED;
+            }
+            if (in.prims[sp].area_light != (int32_t)l) {
+                err = "light " + std::to_string(l) + ": its sphere primitive must name it as its area light";
+                return DFR_E_ARGUMENT;
+            }
+            if (!(in.prims[sp].data[3] > 0.0f)) { err = "light " + std::to_string(l) + ": sphere radius must be positive"; return DFR_E_ARGUMENT; }
+            const double r = in.prims[sp].data[3];
+            d.area = (float)(4.0 * 3.14159265358979323846 * r * r);
+            d.prim = sp;
+            d.tri = -1;
+        } else if (L.type == DFR_LIGHT_DIFFUSE_AREA) {
+            if (L.triangle >= nt) { err = "area light triangle out of range"; return DFR_E_ARGUMENT; }
             const int32_t* ix = &in.indices[3 * L.triangle];
             const float* a = &in.positions[3 * (int64_t)ix[0]];
             const float* b = &in.positions[3 * (int64_t)ix[1]];
@@ -490,10 +506,16 @@ int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err) {
         (d->n_materials && !d->materials) || (d->n_lights && !d->lights)) {
         err = "null array"; return DFR_E_NULL;
     }
-    for (int32_t i = 0; i < d->n_materials; i++)
+    for (int32_t i = 0; i < d->n_materials; i++) {
         if (d->materials[i].type < DFR_MAT_MATTE || d->materials[i].type > DFR_MAT_UBER) {
             err = "material " + std::to_string(i) + ": type out of range"; return DFR_E_ARGUMENT;
         }
+        // DisneyMaterial.java:1140-1146, 1185-1197: the thin form's fake
+        // subsurface and diffuse transmission lobes are not compiled
+        if (d->materials[i].type == DFR_MAT_DISNEY && d->materials[i].thin) {
+            err = "material " + std::to_string(i) + ": thin Disney materials are not supported"; return DFR_E_UNSUPPORTED;
+        }
+    }
     h.positions.assign(d->positions, d->positions + 3 * d->n_vertices);
     if (d->normals) h.normals.assign(d->normals, d->normals + 3 * d->n_vertices); else h.normals.clear();
     h.indices.assign(d->indices, d->indices + 3 * d->n_triangles);
@@ -733,6 +755,7 @@ DevJob Ctx::main_job(int spp, int sample_base) const {
     j.max_bounce = max_bounce; j.min_rr = min_rr;
     j.algorithm = algorithm; j.ao_samples = ao_samples;
     j.max_distance = max_distance;
+    j.preview = preview;
     j.film_offset = 0;
     j.work_begin = 0;
     j.work_count = (int64_t)width * height * spp;

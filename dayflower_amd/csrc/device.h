@@ -670,9 +670,14 @@ DEV Surf make_surface(const DevScene& sc, V o, V d, const HitRec& h) {
 
 // ---------------------------------------------------------------------------
 // Camera.createPrimaryRay (M/scene/Camera.java:243-291) over the 19-float
-// camera record; pinhole origin.
+// camera record.  The thin lens (aperture radius above 0.00001) moves the
+// origin to a uniform point of the aperture disk (SampleGeneratorF
+// .sampleDiskUniformDistribution, M/geometry/SampleGeneratorF.java:68-76)
+// drawn from dimensions DIM_LENS and DIM_LENS + 1; every ray of a pixel
+// position then passes through the same point of the focal plane.
 // ---------------------------------------------------------------------------
-DEV bool primary_ray(const float* cam, float ix, float iy, float px, float py, V& o, V& d) {
+constexpr uint32_t DIM_LENS = 0xfffffff0u;
+DEV bool primary_ray(const float* cam, float ix, float iy, float px, float py, const Rng& rng, V& o, V& d) {
     float cx = 2.0f * ((ix + px) / (cam[DFR_CAMERA_RES_X] - 1.0f)) - 1.0f;
     float cy = 2.0f * ((iy + py) / (cam[DFR_CAMERA_RES_Y] - 1.0f)) - 1.0f;
     float wf = 1.0f;
@@ -683,9 +688,16 @@ DEV bool primary_ray(const float* cam, float ix, float iy, float px, float py, V
     }
     V U = ld3(cam + 3), Vv = ld3(cam + 6), W = ld3(cam + 9), eye = ld3(cam + 12);
     V on_plane = (U * (cam[0] * cx) + Vv * (cam[1] * cy)) + (eye + W * wf);
-    V on_image = eye + nrm(on_plane - eye) * cam[DFR_CAMERA_FOCAL];
+    // Point3F.add(eye, Vector3F.direction(eye, onPlane), focalDistance)
+    V on_image = eye + (on_plane - eye) * cam[DFR_CAMERA_FOCAL];
     o = eye;
-    d = nrm(on_image - eye);
+    const float aperture = cam[DFR_CAMERA_APERTURE];
+    if (aperture > 0.00001f) {
+        const float r = sqrtf(rng.get(DIM_LENS)), theta = 2.0f * PI_F * rng.get(DIM_LENS + 1);
+        const float dx = r * cosf(theta), dy = r * sinf(theta);
+        o = eye + (U * (dx * aperture) + Vv * (dy * aperture));
+    }
+    d = nrm(on_image - o);
     return true;
 }
 
@@ -1254,6 +1266,80 @@ DEV bool sample_delta_light_ext(const DevLight& L, const Surf& si, LightSmp& out
     }
 }
 
+// DiffuseAreaLight over a Sphere3F (M/geometry/shape/Sphere3F.java:156-226):
+// the reference samples its unit sphere in object space, where a world point
+// x of a sphere with centre c and radius r is (x - c) / r.  Inside the sphere
+// a uniform point on it, pdf d^2 / |cos| in object units (light_pdf_mode 1:
+// d^2 / (|cos| 4 pi r^2) in world units); outside a direction of the cone the
+// sphere subtends, pdf 1 / (2 pi (1 - cos theta_max)).
+DEV bool sample_sphere_light(const DevScene& sc, const DevLight& L, const Surf& si, float u, float v, LightSmp& out) {
+    const DevPrim& sp = sc.prims[L.prim];
+    const V c = ld3(sp.data);
+    const float r = sp.data[3];
+    const V po = (si.p - c) * (1.0f / r);
+    V pt, n;
+    float pdf;
+    if (dot(po, po) <= 1.0f) {
+        const float ct = 1.0f - 2.0f * u, st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct)), phi = v * 2.0f * PI_F;
+        n = nrm(mk(st * cosf(phi), st * sinf(phi), ct));
+        pt = n;
+        const V inc = pt - po;
+        if (dot(inc, inc) == 0.0f) return false;
+        pdf = dot(inc, inc) / fabsf(dot(n, -nrm(inc)));
+        if (sc.light_pdf_mode == 1) pdf = pdf / (4.0f * PI_F);
+        if (isinf(pdf) || isnan(pdf)) return false;
+    } else {
+        const float dist = len(po), dr = 1.0f / dist;
+        const ONB b = onb_w(-po * dr);
+        const float s2 = dr * dr, sr = 1.0f / dr;
+        const float cmax = sqrtf(fmaxf(0.0f, 1.0f - s2));
+        const bool small = s2 < 0.00068523f;
+        const float ctheta = small ? sqrtf(1.0f - s2 * u) : (cmax - 1.0f) * u + 1.0f;
+        const float st2 = small ? s2 * u : 1.0f - ctheta * ctheta;
+        const float ca = st2 * sr + ctheta * sqrtf(fmaxf(0.0f, 1.0f - st2 * sr * sr));
+        const float sa = sqrtf(fmaxf(0.0f, 1.0f - ca * ca));
+        const float phi = v * 2.0f * PI_F;
+        const V dir = (-b.u) * (sa * cosf(phi)) + (-b.v) * (sa * sinf(phi)) + (-b.w) * ca;
+        pt = dir;
+        n = nrm(dir);
+        pdf = 1.0f / (2.0f * PI_F * (1.0f - cmax));
+    }
+    const V pw = c + pt * r;
+    const V wi = nrm(pw - si.p);
+    if (pdf > 0.0f && (L.two_sided || dot(n, -wi) > 0.0f)) {
+        out.L = ld3(L.radiance); out.p = pw; out.wi = wi; out.pdf = pdf;
+        return true;
+    }
+    return false;
+}
+
+// Sphere3F.evaluateProbabilityDensityFunction (Sphere3F.java:460-479):
+// d^2 / |cos| * 4 pi in object units (the reference multiplies by the area);
+// light_pdf_mode 1: the cone pdf outside the sphere, d^2 / (|cos| 4 pi r^2)
+// inside.
+DEV float sphere_light_pdf(const DevScene& sc, const DevLight& L, const Surf& si, V wi) {
+    const DevPrim& sp = sc.prims[L.prim];
+    const V c = ld3(sp.data);
+    const float r = sp.data[3];
+    const V d = nrm(wi);
+    const V po = (si.p - c) * (1.0f / r);
+    if (sc.light_pdf_mode == 1 && dot(po, po) > 1.0f) {
+        const float dist = len(po);
+        const float cmax = sqrtf(fmaxf(0.0f, 1.0f - 1.0f / (dist * dist)));
+        if (dot(d, -po * (1.0f / dist)) < cmax) return 0.0f;
+        return 1.0f / (2.0f * PI_F * (1.0f - cmax));
+    }
+    const float t = sphere_t(sp, si.p, d, T_MIN, FLT_MAX);
+    if (t == 0.0f) return 0.0f;
+    const V n = nrm((si.p + d * t) - c);
+    const float ca = fabsf(dot(n, -d));
+    const float to = t / r;
+    float pdf = to * to / ca * (4.0f * PI_F);
+    if (sc.light_pdf_mode == 1) pdf = (t * t) / (ca * (4.0f * PI_F * r * r));
+    if (isinf(pdf) || isnan(pdf)) return 0.0f;
+    return pdf;
+}
+
 // DiffuseAreaLight over Triangle3F, PointLight, PerezLight sampling (and the
 // spot and directional lights with EXT).
 template <bool EXT>
@@ -1272,7 +1358,7 @@ DEV bool sample_light(const DevScene& sc, const DevLight& L, const Surf& si, flo
         if (L.type == DFR_LIGHT_SPOT || L.type == DFR_LIGHT_DIRECTIONAL) return sample_delta_light_ext(L, si, out);
     }
     if (L.type == DFR_LIGHT_PEREZ) return perez_sample(sc.perez[L.perez], si, u, v, usun, out);
-    if (L.tri < 0) return false;
+    if (L.tri < 0) return sample_sphere_light(sc, L, si, u, v, out);
     const size_t rb = 8 * (size_t)L.tri;
     const float4 r0 = ld4g(sc.tshade, rb), r1 = ld4g(sc.tshade, rb + 1), r2 = ld4g(sc.tshade, rb + 2);
     const float4 r3 = ld4g(sc.tshade, rb + 3), r4 = ld4g(sc.tshade, rb + 4);
@@ -1295,10 +1381,11 @@ DEV bool sample_light(const DevScene& sc, const DevLight& L, const Surf& si, flo
     return false;
 }
 
-// Shape3F.evaluateProbabilityDensityFunction for a triangle light
+// Shape3F.evaluateProbabilityDensityFunction for a triangle or sphere light
 DEV float light_pdf(const DevScene& sc, const DevLight& L, const Surf& si, V wi) {
     if (L.type == DFR_LIGHT_PEREZ) return perez_pdf(sc.perez[L.perez], wi);
-    if (L.type != DFR_LIGHT_DIFFUSE_AREA || L.tri < 0) return 0.0f;
+    if (L.type != DFR_LIGHT_DIFFUSE_AREA) return 0.0f;
+    if (L.tri < 0) return sphere_light_pdf(sc, L, si, wi);
     const size_t rb = 8 * (size_t)L.tri;
     V a = f4v(ld4g(sc.tshade, rb)), b = f4v(ld4g(sc.tshade, rb + 1)), c = f4v(ld4g(sc.tshade, rb + 2));
     V d = nrm(wi);

@@ -18,7 +18,7 @@ struct DevJob {
     int32_t max_bounce, min_rr;
     int32_t algorithm, ao_samples;
     float max_distance;
-    int32_t pad0;
+    int32_t preview;          // setPreviewMode: a primary ray that misses is white
     int64_t film_offset;      // first pixel of this job in the film array
     int64_t work_begin;       // first work item of this job (prefix sum)
     int64_t work_count;       // width * height * spp

@@ -210,7 +210,7 @@ DEV V resolve_shadow(const DevScene& sc, const ShadowReq& r, TraversalStack<SHOR
 // Plain megakernel (one work-item per pixel)
 // ---------------------------------------------------------------------------
 template <bool EXT, int SHORT>
-DEV V radiance_pt(const DevScene& sc, V o, V d, const Rng& rng, int max_bounce, int min_rr,
+DEV V radiance_pt(const DevScene& sc, V o, V d, const Rng& rng, int max_bounce, int min_rr, bool preview,
                   TraversalStack<SHORT>& st, int64_t* cnt) {
     V L = mk(0, 0, 0), beta = mk(1, 1, 1);
     bool spec = false;
@@ -264,7 +264,10 @@ DEV V radiance_pt(const DevScene& sc, V o, V d, const Rng& rng, int max_bounce, 
                 beta = beta / (1.0f - q);
             }
         } else {
-            if (bounce == 0 || spec)
+            // Scene.java:603-606: in preview mode a primary ray that leaves
+            // the scene is white
+            if (bounce == 0 && preview) L = mk(1.0f, 1.0f, 1.0f);
+            else if (bounce == 0 || spec)
                 for (int i = 0; i < sc.n_lights; i++) L = L + beta * emitted_ray(sc, sc.lights[i], d);
             break;
         }
@@ -275,11 +278,14 @@ DEV V radiance_pt(const DevScene& sc, V o, V d, const Rng& rng, int max_bounce, 
 }
 
 template <int SHORT>
-DEV V radiance_ao(const DevScene& sc, V o, V d, const Rng& rng, float max_dist, int samples,
+DEV V radiance_ao(const DevScene& sc, V o, V d, const Rng& rng, float max_dist, int samples, bool preview,
                   TraversalStack<SHORT>& st, int64_t* cnt) {
     HitRec h;
     cnt[0]++;
-    if (!trace<false, SHORT>(sc, o, d, T_MIN, FLT_MAX, h, st)) return mk(0, 0, 0);
+    if (!trace<false, SHORT>(sc, o, d, T_MIN, FLT_MAX, h, st)) {
+        const float w = preview ? 1.0f : 0.0f;  // Scene.java:308-309
+        return mk(w, w, w);
+    }
     Surf si = make_surface(sc, o, d, h);
     float acc = 0.0f;
     for (int i = 0; i < samples; i++) {
@@ -310,7 +316,7 @@ DEV V radiance_ao(const DevScene& sc, V o, V d, const Rng& rng, float max_dist, 
 }
 
 template <int SHORT>
-DEV V radiance_rc(const DevScene& sc, V o, V d, TraversalStack<SHORT>& st, int64_t* cnt) {
+DEV V radiance_rc(const DevScene& sc, V o, V d, bool preview, TraversalStack<SHORT>& st, int64_t* cnt) {
     HitRec h;
     cnt[0]++;
     if (trace<false, SHORT>(sc, o, d, T_MIN, FLT_MAX, h, st)) {
@@ -318,6 +324,7 @@ DEV V radiance_rc(const DevScene& sc, V o, V d, TraversalStack<SHORT>& st, int64
         float g = 0.5f * fabsf(dot(si.s.w, d));
         return mk(g, g, g);
     }
+    if (preview) return mk(1.0f, 1.0f, 1.0f);  // Scene.java:692-693
     V L = mk(0, 0, 0);
     for (int i = 0; i < sc.n_lights; i++) L = L + emitted_ray(sc, sc.lights[i], d);
     return L;
@@ -342,11 +349,12 @@ __global__ __launch_bounds__(BLOCK) void k_megakernel(const DevScene* __restrict
             uint32_t sample = (uint32_t)(job.sample_base + k);
             Rng rng = make_rng(job.seed, (uint32_t)pix, sample);
             V o, d;
-            if (!primary_ray(job.cam, (float)x, (float)y, rng.get(0), rng.get(1), o, d)) continue;
+            if (!primary_ray(job.cam, (float)x, (float)y, rng.get(0), rng.get(1), rng, o, d)) continue;
             V L;
-            if (job.algorithm == DFR_ALGO_PATH_TRACING) L = radiance_pt<EXT, SHORT>(sc, o, d, rng, job.max_bounce, job.min_rr, st, cnt);
-            else if (job.algorithm == DFR_ALGO_AMBIENT_OCCLUSION) L = radiance_ao<SHORT>(sc, o, d, rng, job.max_distance, job.ao_samples, st, cnt);
-            else if (job.algorithm == DFR_ALGO_RAY_CASTING) L = radiance_rc<SHORT>(sc, o, d, st, cnt);
+            const bool pv = job.preview != 0;
+            if (job.algorithm == DFR_ALGO_PATH_TRACING) L = radiance_pt<EXT, SHORT>(sc, o, d, rng, job.max_bounce, job.min_rr, pv, st, cnt);
+            else if (job.algorithm == DFR_ALGO_AMBIENT_OCCLUSION) L = radiance_ao<SHORT>(sc, o, d, rng, job.max_distance, job.ao_samples, pv, st, cnt);
+            else if (job.algorithm == DFR_ALGO_RAY_CASTING) L = radiance_rc<SHORT>(sc, o, d, pv, st, cnt);
             else L = mk(0, 0, 0);
             // AbstractCPURenderer.java:229: drop NaN/Inf and negative-luminance samples
             float lum = 0.212671f * L.x + 0.715160f * L.y + 0.072169f * L.z;
@@ -437,7 +445,7 @@ __global__ __launch_bounds__(BLOCK) void k_cast_primary(const DevScene* __restri
     Rng rng = make_rng(job.seed, (uint32_t)pix, (uint32_t)job.sample_base);
     V o, d;
     float* r = rays + 8 * (size_t)pix;
-    if (!primary_ray(job.cam, (float)x, (float)y, rng.get(0), rng.get(1), o, d)) {
+    if (!primary_ray(job.cam, (float)x, (float)y, rng.get(0), rng.get(1), rng, o, d)) {
         for (int k = 0; k < 8; k++) r[k] = 0.0f;
         prim[pix] = -1; tri[pix] = -1; tout[pix] = INFINITY;
         return;
@@ -543,7 +551,7 @@ __global__ void k_generate(const DevScene* __restrict__ scenes, const DevJob* __
     V o, d;
     int slot = base + i;
     uint32_t flags = 0;
-    if (!primary_ray(job.cam, (float)x, (float)y, rng.get(0), rng.get(1), o, d)) {
+    if (!primary_ray(job.cam, (float)x, (float)y, rng.get(0), rng.get(1), rng, o, d)) {
         flags = 0x80000000u;  // rejected camera sample: the path is dropped at shade
         o = mk(0, 0, 0); d = mk(0, 0, 1);
     } else {
@@ -966,6 +974,8 @@ __global__ __launch_bounds__(BLOCK, 4) void k_shade(const DevScene* __restrict__
                         }
                     }
                 }
+            } else if (bounce == 0 && job.preview) {
+                L = mk(1.0f, 1.0f, 1.0f);  // Scene.java:603-606
             } else if (bounce == 0 || spec) {
                 for (int l = 0; l < sc.n_lights; l++) L = L + beta * emitted_ray(sc, sc.lights[l], d);
             }

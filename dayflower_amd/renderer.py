@@ -32,6 +32,10 @@ class DeviceException(RuntimeError):
     pass
 
 
+class UnsupportedOperationException(NotImplementedError):
+    """A scene feature the compiled path does not carry (DFR_E_UNSUPPORTED)."""
+
+
 class RenderingAlgorithm(enum.IntEnum):
     """M/renderer/RenderingAlgorithm.java:33-53 (ordinals as on the GPU)."""
     AMBIENT_OCCLUSION = abi.ALGO_AMBIENT_OCCLUSION
@@ -64,6 +68,8 @@ def _check(lib, ctx, rc):
         raise IllegalStateException(msg)
     if rc == abi.E_STOPPED:
         return rc
+    if rc == abi.E_UNSUPPORTED:
+        raise UnsupportedOperationException(msg)
     raise DeviceException(f"{msg} (code {rc})")
 
 

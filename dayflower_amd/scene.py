@@ -221,6 +221,23 @@ class Scene:
         self.lights.append(L)
         return li
 
+    def add_sphere_light(self, center, radius, radiance, material, two_sided=False):
+        """DiffuseAreaLight over a Sphere3F (M/scene/light/DiffuseAreaLight.java
+        :190; sampled by M/geometry/shape/Sphere3F.java:156-226).  Returns the
+        light index; the sphere primitive names it as its area light."""
+        li = len(self.lights)
+        prim = self.add_sphere(center, radius, material)
+        self.primitives[prim].area_light = li
+        L = abi.Light()
+        L.type = abi.LIGHT_DIFFUSE_AREA
+        L.primitive = prim
+        L.triangle = -1
+        L.two_sided = int(two_sided)
+        for i in range(3):
+            L.radiance[i] = float(radiance[i])
+        self.lights.append(L)
+        return li
+
     def add_point_light(self, position, intensity):
         """PointLight(intensity, position) (M/scene/light/PointLight.java:100)."""
         L = abi.Light()

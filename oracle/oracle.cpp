@@ -48,8 +48,10 @@ inline Rng makeRng(uint64_t seed, uint32_t pixel, uint32_t sample) {
 // 2 + 12*b + {0 light choice, 1-2 light sample, 3-4 NEE BSDF sample,
 // 5 Perez sun choice, 6-7 BSDF continuation, 8 Russian roulette,
 // 9 clear coat layer choice}.
-// Ambient occlusion sample i uses 2 + 2*i, 3 + 2*i.
+// Ambient occlusion sample i uses 2 + 2*i, 3 + 2*i.  The thin lens's disk
+// sample uses dimensions 0xfffffff0 and 0xfffffff1.
 const uint32_t DIM_BOUNCE = 2, DIMS_PER_BOUNCE = 12;
+const uint32_t DIM_LENS = 0xfffffff0u;
 
 // ---------------------------------------------------------------------------
 // Scene (world space).  Transforms are folded in by the caller.
@@ -1189,6 +1191,90 @@ Scene::~Scene() { for (auto& l : lights) delete l.perez; }
 
 float triArea(const Tri& t) { return 0.5f * length(cross(t.b - t.a, t.c - t.a)); }
 
+// A DiffuseAreaLight over a Sphere3F (M/geometry/shape/Sphere3F.java:156-226).
+// The reference samples its unit sphere in object space; a sphere of centre c
+// and radius r maps a world point x to (x - c) / r.  Inside the sphere: a
+// uniform point on it with Shape3F's pdf d^2 / |cos| in object units
+// (reference mode; area mode: d^2 / (|cos| 4 pi r^2) in world units).
+// Outside: a direction in the cone the sphere subtends, pdf
+// 1 / (2 pi (1 - cos theta_max)) (SampleGeneratorF.java:491-493).
+bool sampleSphereLight(const Prim& sp, const Light& L, const Surface& si, float u, float v, int pdfMode,
+                       LightSample* out) {
+    const V3 c(sp.data[0], sp.data[1], sp.data[2]);
+    const float r = sp.data[3];
+    const V3 po = (si.p - c) / r;  // object space
+    V3 pointO, normal;
+    float pdf;
+    if (lengthSquared(po) <= 1.0f) {
+        // Sphere3F.sample(sample): SampleGeneratorF.sampleSphereUniformDistribution
+        const float cosT = 1.0f - 2.0f * u;
+        const float sinT = fsqrt(std::max(0.0f, 1.0f - cosT * cosT));
+        const float phi = v * 2.0f * PI;
+        normal = normalize(V3(sinT * fcos(phi), sinT * fsin(phi), cosT));
+        pointO = normal;
+        const V3 inc = pointO - po;
+        if (isZero(lengthSquared(inc))) return false;
+        const V3 incN = normalize(inc);
+        pdf = lengthSquared(pointO - po) / dotAbs(normal, -incN);
+        if (pdfMode == 1) pdf = pdf / (4.0f * PI);  // (r d)^2 / (|cos| 4 pi r^2) in world units
+        if (std::isinf(pdf) || std::isnan(pdf)) return false;
+    } else {
+        const float dist = length(po);
+        const float distR = 1.0f / dist;
+        const V3 toCenter = (V3(0.0f, 0.0f, 0.0f) - po) * distR;
+        const ONB b = onbFromW(toCenter);
+        const V3 x = -b.u, y = -b.v, z = -b.w;
+        const float sinMax = distR;
+        const float sinMax2 = sinMax * sinMax;
+        const float sinMaxR = 1.0f / sinMax;
+        const float cosMax = fsqrt(std::max(0.0f, 1.0f - sinMax2));
+        const bool small = sinMax2 < 0.00068523f;
+        const float cosT = small ? fsqrt(1.0f - sinMax2 * u) : (cosMax - 1.0f) * u + 1.0f;
+        const float sinT2 = small ? sinMax2 * u : 1.0f - cosT * cosT;
+        const float cosA = sinT2 * sinMaxR + cosT * fsqrt(std::max(0.0f, 1.0f - sinT2 * sinMaxR * sinMaxR));
+        const float sinA = fsqrt(std::max(0.0f, 1.0f - cosA * cosA));
+        const float phi = v * 2.0f * PI;
+        const V3 dir = x * (sinA * fcos(phi)) + y * (sinA * fsin(phi)) + z * cosA;  // Vector3F.directionSpherical
+        pointO = dir * 1.0f;
+        normal = normalize(dir);
+        pdf = 1.0f / (2.0f * PI * (1.0f - cosMax));
+    }
+    const V3 pw = c + pointO * r;  // object -> world
+    const V3 wi = normalize(pw - si.p);
+    if (pdf > 0.0f && (L.twoSided || dot(normal, -wi) > 0.0f)) {
+        out->L = L.radiance; out->p = pw; out->wi = wi; out->pdf = pdf;
+        return true;
+    }
+    return false;
+}
+
+// Sphere3F.evaluateProbabilityDensityFunction (Sphere3F.java:460-479) in object
+// space: reference mode d^2 / |cos| * 4 pi (the reference multiplies by the
+// area); area mode: the cone pdf outside the sphere (the direction must
+// enter the cone), d^2 / (|cos| 4 pi r^2) inside.
+float sphereLightPdf(const Prim& sp, const Surface& si, V3 wi, int pdfMode) {
+    const V3 c(sp.data[0], sp.data[1], sp.data[2]);
+    const float r = sp.data[3];
+    const V3 d = normalize(wi);
+    const V3 po = (si.p - c) / r;
+    if (pdfMode == 1 && lengthSquared(po) > 1.0f) {
+        const float dist = length(po);
+        const float cosMax = fsqrt(std::max(0.0f, 1.0f - 1.0f / (dist * dist)));
+        if (dot(d, (V3(0.0f, 0.0f, 0.0f) - po) / dist) < cosMax) return 0.0f;
+        return 1.0f / (2.0f * PI * (1.0f - cosMax));
+    }
+    Ray ray{si.p, d};
+    const float t = sphereT(sp, ray, 0.001f, FLT_MAX);
+    if (std::isnan(t)) return 0.0f;
+    const V3 hit = si.p + d * t;
+    const V3 n = normalize(hit - c);
+    const float dO = t / r;  // object-space distance
+    float pdf = dO * dO / dotAbs(n, -d) * (4.0f * PI);
+    if (pdfMode == 1) pdf = (t * t) / (dotAbs(n, -d) * (4.0f * PI * r * r));
+    if (std::isinf(pdf) || std::isnan(pdf)) return 0.0f;
+    return pdf;
+}
+
 // DiffuseAreaLight.sampleRadianceIncoming (M/scene/light/DiffuseAreaLight.java
 // :252-282) over Triangle3F.sample (M/geometry/shape/Triangle3F.java:174-199)
 // and Shape3F.sample(sample, surfaceIntersection) (M/geometry/Shape3F.java
@@ -1229,7 +1315,10 @@ bool sampleLight(const Scene& sc, const Light& L, const Surface& si, float u, fl
         return true;
     }
     if (L.type == DFR_LIGHT_PEREZ) return perezSample(*L.perez, si, u, v, uSun, out);
-    if (L.tri < 0) return false;
+    if (L.tri < 0) {
+        if (L.prim < 0 || sc.prims[L.prim].shape != DFR_SHAPE_SPHERE) return false;
+        return sampleSphereLight(sc.prims[L.prim], L, si, u, v, pdfMode, out);
+    }
     const Tri& tr = sc.tris[L.tri];
     float su = fsqrt(u);
     float b0 = 1.0f - su, b1 = v * su, b2 = 1.0f - b0 - b1;
@@ -1251,9 +1340,13 @@ bool sampleLight(const Scene& sc, const Light& L, const Surface& si, float u, fl
 
 // DiffuseAreaLight.evaluateProbabilityDensityFunctionRadianceIncoming ->
 // Shape3F.evaluateProbabilityDensityFunction (M/geometry/Shape3F.java:199-217).
-float lightPdf(const Scene& sc, const Light& L, const Surface& si, V3 wi) {
+float lightPdf(const Scene& sc, const Light& L, const Surface& si, V3 wi, int pdfMode) {
     if (L.type == DFR_LIGHT_PEREZ) return perezPdf(*L.perez, wi);
-    if (L.type != DFR_LIGHT_DIFFUSE_AREA || L.tri < 0) return 0.0f;
+    if (L.type != DFR_LIGHT_DIFFUSE_AREA) return 0.0f;
+    if (L.tri < 0) {
+        if (L.prim < 0 || sc.prims[L.prim].shape != DFR_SHAPE_SPHERE) return 0.0f;
+        return sphereLightPdf(sc.prims[L.prim], si, wi, pdfMode);
+    }
     const Tri& tr = sc.tris[L.tri];
     Ray r{si.p, normalize(wi)};
     float t = triangleT(tr, r, 0.001f, FLT_MAX);
@@ -1330,7 +1423,7 @@ C3 estimateDirect(const Scene& sc, const BSDF& bsdf, const Surface& si, int li, 
         if (!isBlack(f) && bs.pdf > 0.0f) {
             float w = 1.0f;
             if (!spec) {
-                float lp = lightPdf(sc, L, si, bs.wi);
+                float lp = lightPdf(sc, L, si, bs.wi, pdfMode);
                 if (isZero(lp)) return Ld;
                 w = powerHeuristic(bs.pdf, lp);
             }
@@ -1352,12 +1445,16 @@ C3 estimateDirect(const Scene& sc, const BSDF& bsdf, const Surface& si, int li, 
 }
 
 // Camera.createPrimaryRay (M/scene/Camera.java:243-291) over the compiled
-// 19-float camera (CompiledCameraCache.java:158-197).
-bool primaryRay(const float* cam, float imageX, float imageY, float px, float py, Ray* out) {
+// 19-float camera (CompiledCameraCache.java:158-197).  (lu, lv) is the disk
+// sample of the thin lens (SampleGeneratorF.sampleDiskUniformDistribution,
+// M/geometry/SampleGeneratorF.java:68-76), used when the aperture radius
+// exceeds 0.00001.
+bool primaryRay(const float* cam, float imageX, float imageY, float px, float py, float lu, float lv, Ray* out) {
     float fovX = cam[DFR_CAMERA_FOV_X], fovY = cam[DFR_CAMERA_FOV_Y];
     float resX = cam[DFR_CAMERA_RES_X], resY = cam[DFR_CAMERA_RES_Y];
     V3 U(cam[3], cam[4], cam[5]), Vv(cam[6], cam[7], cam[8]), W(cam[9], cam[10], cam[11]);
     V3 eye(cam[12], cam[13], cam[14]);
+    const float aperture = cam[DFR_CAMERA_APERTURE];
     float cx = 2.0f * ((imageX + px) / (resX - 1.0f)) - 1.0f;
     float cy = 2.0f * ((imageY + py) / (resY - 1.0f)) - 1.0f;
     float wf = 1.0f;
@@ -1367,14 +1464,21 @@ bool primaryRay(const float* cam, float imageX, float imageY, float px, float py
         wf = fsqrt(1.0f - d);
     }
     V3 onPlane = (U * (fovX * cx) + Vv * (fovY * cy)) + (eye + W * wf);
-    V3 onImage = eye + normalize(onPlane - eye) * cam[DFR_CAMERA_FOCAL];
-    out->o = eye;  // aperture radius 0 (pinhole); thin-lens origin offset not modelled
-    out->d = normalize(onImage - eye);
+    // Point3F.add(eye, Vector3F.direction(eye, onPlane), focalDistance)
+    V3 onImage = eye + (onPlane - eye) * cam[DFR_CAMERA_FOCAL];
+    V3 origin = eye;
+    if (aperture > 0.00001f) {
+        const float r = fsqrt(lu), theta = 2.0f * PI * lv;
+        const float dx = r * fcos(theta), dy = r * fsin(theta);
+        origin = eye + (U * (dx * aperture) + Vv * (dy * aperture));
+    }
+    out->o = origin;
+    out->d = normalize(onImage - origin);
     return true;
 }
 
 // Scene.radiancePathTracer (M/scene/Scene.java:451-636)
-C3 radiancePT(const Scene& sc, Ray ray, const Rng& rng, int maxBounce, int minRR, int pdfMode) {
+C3 radiancePT(const Scene& sc, Ray ray, const Rng& rng, int maxBounce, int minRR, int pdfMode, bool preview) {
     C3 L, beta(1.0f);
     bool specBounce = false;
     float etaScale = 1.0f;
@@ -1416,8 +1520,13 @@ C3 radiancePT(const Scene& sc, Ray ray, const Rng& rng, int maxBounce, int minRR
                 beta = beta / (1.0f - q);
             }
         } else {
-            if (bounce == 0 || specBounce)
+            // Scene.java:603-606: in preview mode a primary ray that leaves
+            // the scene is white
+            if (bounce == 0 && preview) {
+                L = C3(1.0f);
+            } else if (bounce == 0 || specBounce) {
                 for (const Light& Lt : sc.lights) L = L + beta * emittedRay(Lt, ray.d);
+            }
             break;
         }
     }
@@ -1425,9 +1534,9 @@ C3 radiancePT(const Scene& sc, Ray ray, const Rng& rng, int maxBounce, int minRR
 }
 
 // Scene.radianceAmbientOcclusion (M/scene/Scene.java:272-326)
-C3 radianceAO(const Scene& sc, const Ray& ray, const Rng& rng, float maxDist, int samples) {
+C3 radianceAO(const Scene& sc, const Ray& ray, const Rng& rng, float maxDist, int samples, bool preview) {
     Hit h;
-    if (!sceneClosest(sc, ray, 0.001f, FLT_MAX, &h, false)) return C3(0.0f);
+    if (!sceneClosest(sc, ray, 0.001f, FLT_MAX, &h, false)) return C3(preview ? 1.0f : 0.0f);  // :308-309
     Surface si = makeSurface(sc, ray, h);
     float acc = 0.0f;
     for (int i = 0; i < samples; i++) {
@@ -1456,12 +1565,13 @@ C3 radianceAO(const Scene& sc, const Ray& ray, const Rng& rng, float maxDist, in
 }
 
 // Scene.radianceRayCaster (M/scene/Scene.java:675-718)
-C3 radianceRC(const Scene& sc, const Ray& ray) {
+C3 radianceRC(const Scene& sc, const Ray& ray, bool preview) {
     Hit h;
     if (sceneClosest(sc, ray, 0.001f, FLT_MAX, &h, false)) {
         Surface si = makeSurface(sc, ray, h);
         return C3(0.5f) * std::fabs(dot(si.s.w, ray.d));
     }
+    if (preview) return C3(1.0f);  // :692-693
     C3 L;
     for (const Light& Lt : sc.lights) L = L + emittedRay(Lt, ray.d);
     return L;
@@ -1684,12 +1794,17 @@ int32_t orc_render(const orc_scene* h, const orc_params* prm, double* sum, doubl
         for (int k = 0; k < prm->spp; k++) {
             Rng rng = makeRng(prm->seed, pix, (uint32_t)(prm->spp_offset + k));
             Ray ray;
-            if (!primaryRay(cam, (float)x, (float)y, rng.get(0), rng.get(1), &ray)) continue;
+            if (!primaryRay(cam, (float)x, (float)y, rng.get(0), rng.get(1), rng.get(DIM_LENS), rng.get(DIM_LENS + 1), &ray))
+                continue;
             C3 L;
             switch (prm->algorithm) {
-                case DFR_ALGO_PATH_TRACING: L = radiancePT(sc, ray, rng, prm->max_bounce, prm->min_rr, prm->light_pdf_mode); break;
-                case DFR_ALGO_AMBIENT_OCCLUSION: L = radianceAO(sc, ray, rng, prm->max_distance, prm->samples); break;
-                case DFR_ALGO_RAY_CASTING: L = radianceRC(sc, ray); break;
+                case DFR_ALGO_PATH_TRACING:
+                    L = radiancePT(sc, ray, rng, prm->max_bounce, prm->min_rr, prm->light_pdf_mode, prm->preview != 0);
+                    break;
+                case DFR_ALGO_AMBIENT_OCCLUSION:
+                    L = radianceAO(sc, ray, rng, prm->max_distance, prm->samples, prm->preview != 0);
+                    break;
+                case DFR_ALGO_RAY_CASTING: L = radianceRC(sc, ray, prm->preview != 0); break;
                 default: L = C3(0.0f); break;
             }
             // AbstractCPURenderer.java:229: drop NaN/Inf/negative-luminance samples
@@ -1756,7 +1871,7 @@ int32_t orc_primary_rays(const orc_scene* h, int32_t width, int32_t height, uint
             Rng rng = makeRng(seed, pix, (uint32_t)sample);
             Ray r;
             float* o = rays + 8 * (int64_t)pix;
-            if (primaryRay(cam, (float)x, (float)y, rng.get(0), rng.get(1), &r)) {
+            if (primaryRay(cam, (float)x, (float)y, rng.get(0), rng.get(1), rng.get(DIM_LENS), rng.get(DIM_LENS + 1), &r)) {
                 o[0] = r.o.x; o[1] = r.o.y; o[2] = r.o.z; o[3] = 0.001f;
                 o[4] = r.d.x; o[5] = r.d.y; o[6] = r.d.z; o[7] = FLT_MAX;
             } else {
@@ -1767,8 +1882,12 @@ int32_t orc_primary_rays(const orc_scene* h, int32_t width, int32_t height, uint
 }
 
 int32_t orc_camera_ray(const float* cam, float ix, float iy, float px, float py, float* out) {
+    return orc_lens_ray(cam, ix, iy, px, py, 0.5f, 0.5f, out);
+}
+
+int32_t orc_lens_ray(const float* cam, float ix, float iy, float px, float py, float lu, float lv, float* out) {
     Ray r;
-    if (!primaryRay(cam, ix, iy, px, py, &r)) return 0;
+    if (!primaryRay(cam, ix, iy, px, py, lu, lv, &r)) return 0;
     out[0] = r.o.x; out[1] = r.o.y; out[2] = r.o.z; out[3] = r.d.x; out[4] = r.d.y; out[5] = r.d.z;
     return 1;
 }

@@ -52,7 +52,7 @@ typedef struct orc_params {
     int32_t light_pdf_mode, threads;
     int32_t x0, y0, x1, y1;      /* pixel window; x1/y1 <= 0 means full    */
     float max_distance;
-    float reserved;
+    int32_t preview;             /* setPreviewMode: a primary miss is white */
     uint64_t seed;
 } orc_params;
 
@@ -74,6 +74,8 @@ int32_t orc_trace_f64(const orc_scene* scene, const float* rays, int64_t n, int3
 int32_t orc_primary_rays(const orc_scene* scene, int32_t width, int32_t height, uint64_t seed,
                          int32_t sample, float* rays);
 int32_t orc_camera_ray(const float* camera19, float ix, float iy, float px, float py, float* out6);
+/* the same with the thin lens's disk sample (lu, lv) */
+int32_t orc_lens_ray(const float* camera19, float ix, float iy, float px, float py, float lu, float lv, float* out6);
 /* Tests only: 1 makes the float triangle test compare t * det with tmin
  * literally, as the reference does (Triangle3F.java:570-574). */
 void orc_set_literal_tmin(int32_t on);

@@ -20,7 +20,7 @@ class Params(ctypes.Structure):
                 ("min_rr", ctypes.c_int32), ("samples", ctypes.c_int32), ("light_pdf_mode", ctypes.c_int32),
                 ("threads", ctypes.c_int32), ("x0", ctypes.c_int32), ("y0", ctypes.c_int32),
                 ("x1", ctypes.c_int32), ("y1", ctypes.c_int32), ("max_distance", ctypes.c_float),
-                ("reserved", ctypes.c_float), ("seed", ctypes.c_uint64)]
+                ("preview", ctypes.c_int32), ("seed", ctypes.c_uint64)]
 
 
 _lib = None
@@ -49,6 +49,7 @@ def load():
     lib.orc_trace_f64.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp, vp, vp]
     lib.orc_primary_rays.argtypes = [vp, i32, i32, ctypes.c_uint64, i32, vp]
     lib.orc_camera_ray.argtypes = [vp, f, f, f, f, vp]
+    lib.orc_lens_ray.argtypes = [vp, f, f, f, f, f, f, vp]
     lib.orc_set_literal_tmin.argtypes = [i32]
     lib.orc_set_refine_hits.argtypes = [i32]
     lib.orc_rng.restype = f
@@ -99,7 +100,7 @@ class OracleScene:
         self.close()
 
     def render(self, width, height, spp, algorithm=abi.ALGO_PATH_TRACING, max_bounce=20, min_rr=5, samples=10,
-               max_distance=20.0, seed=0, spp_offset=0, light_pdf_mode=0, threads=THREADS, window=None):
+               max_distance=20.0, seed=0, spp_offset=0, light_pdf_mode=0, threads=THREADS, window=None, preview=False):
         """Returns (sum, sumsq, count) per pixel (H,W,3 doubles; H,W ints)."""
         p = Params()
         p.width, p.height, p.spp, p.spp_offset = width, height, spp, spp_offset
@@ -108,6 +109,7 @@ class OracleScene:
         if window is not None:
             p.x0, p.y0, p.x1, p.y1 = window
         p.max_distance = max_distance
+        p.preview = int(bool(preview))
         p.seed = seed
         s = np.zeros((height, width, 3), np.float64)
         q = np.zeros((height, width, 3), np.float64)

@@ -1,0 +1,193 @@
+"""GPU parity of the camera, light and renderer features beyond the basic
+integrators: the thin-lens camera, the fisheye lens (rejected samples),
+preview mode, sphere area lights in both light-pdf modes, and the refusal
+of the thin Disney material.  Each compares the HIP path (through the C-ABI)
+with the oracle on the same counter-based random streams.
+
+Bars: films are compared path by path at 1 spp (rtol 1e-3) and by their
+image mean at 16 spp; the bar for each scene is printed beside the
+measured value.  Sample counts are compared exactly.
+"""
+import numpy as np
+import pytest
+
+from dayflower_amd import abi
+from dayflower_amd import scene as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _renderer():
+    from dayflower_amd.renderer import GPURenderer
+    return GPURenderer()
+
+
+def _same_streams(sc, res, seed, kernels=(0, 1), max_bounce=8, algorithm=abi.ALGO_PATH_TRACING, light_pdf_mode=0,
+                  preview=False, sort_keys=None, path_bar=0.995, mean_bar=0.002, samples=10, max_distance=20.0):
+    """Renders `sc` at 1 and 16 spp on each kernel and compares with the
+    oracle; returns the per-kernel (paths equal, relative mean difference)."""
+    from tests.oracle_lib import OracleScene
+    o = OracleScene(sc)
+    kw = dict(seed=seed, max_bounce=max_bounce, algorithm=algorithm, light_pdf_mode=light_pdf_mode, preview=preview,
+              samples=samples, max_distance=max_distance)
+    s1, _, c1 = o.render(res, res, 1, **kw)
+    s16, _, c16 = o.render(res, res, 16, **kw)
+    o.close()
+    ref1 = s1 / np.maximum(c1, 1)[..., None]
+    ref16 = s16 / np.maximum(c16, 1)[..., None]
+    out = {}
+    for kernel in kernels:
+        r = _renderer()
+        r.setScene(sc)
+        r.setSeed(seed)
+        r.setMaximumBounce(max_bounce)
+        r.setRenderingAlgorithm(algorithm)
+        r.setSamples(samples)
+        r.setMaximumDistance(max_distance)
+        r.setLightPdfMode(light_pdf_mode)
+        r.setPreviewMode(preview)
+        r.setKernel(kernel)
+        if sort_keys is not None:
+            r.setWavefront(sort_keys=sort_keys)
+        r.setup()
+        r.render(1)
+        g1, cnt1 = r.film(res, res)
+        r.clear()
+        r.render(16)
+        g16, cnt16 = r.film(res, res)
+        r.dispose()
+        np.testing.assert_array_equal(cnt1, c1)
+        np.testing.assert_array_equal(cnt16, c16)
+        paths = np.isclose(g1, ref1, rtol=1e-3, atol=1e-4).all(-1).mean()
+        rel = abs(g16.mean() - ref16.mean()) / max(ref16.mean(), 1e-12)
+        print(f"kernel {kernel} sort {sort_keys}: 1-spp paths equal {paths:.5f} (bar {path_bar}); "
+              f"16-spp mean rel diff {rel:.2e} (bar {mean_bar})", flush=True)
+        assert paths >= path_bar
+        assert rel < mean_bar
+        out[kernel] = (paths, rel)
+    return out
+
+
+def _lens_scene(res, lens=abi.LENS_THIN, aperture=0.0, focal=3.5):
+    sc = S.icosphere_scene(5000, seed=0x5EED, width=res, height=res, with_perez=True)
+    sc.set_camera(S.camera_record((0.0, 0.4, -3.6), (0.0, -0.1, 0.0), res, res, fov_x_deg=40.0, lens=lens,
+                                  aperture=aperture, focal_distance=focal))
+    return sc
+
+
+def test_thin_lens_same_streams():
+    """Aperture 0.08, focal distance 3.5 (the sphere's front in focus): every
+    camera ray starts on the aperture disk (Camera.java:272-282)."""
+    _same_streams(_lens_scene(64, aperture=0.08, focal=3.5), 64, seed=31)
+
+
+def test_thin_lens_blurs_off_focus():
+    """Depth of field: with the focal plane far behind the sphere the image is
+    blurred relative to the pinhole image of the same streams, and the same
+    aperture at the sphere's distance blurs it far less."""
+    res = 64
+
+    def render(aperture, focal):
+        r = _renderer()
+        r.setScene(_lens_scene(res, aperture=aperture, focal=focal))
+        r.setSeed(5)
+        r.setMaximumBounce(1)
+        r.setup()
+        r.render(64)
+        g, _ = r.film(res, res)
+        r.dispose()
+        return g.mean(-1)
+
+    def sharpness(img):
+        return float(np.abs(np.diff(img, axis=0)).mean() + np.abs(np.diff(img, axis=1)).mean())
+
+    pin, near, far = render(0.0, 3.5), render(0.15, 3.3), render(0.15, 12.0)
+    print(f"gradient energy: pinhole {sharpness(pin):.4f}, in focus {sharpness(near):.4f}, "
+          f"out of focus {sharpness(far):.4f}")
+    assert sharpness(far) < 0.8 * sharpness(pin)
+    assert sharpness(near) > sharpness(far)
+
+
+@pytest.mark.parametrize("sort_keys", [0, 1])
+def test_fisheye_same_streams_scene_uniform(sort_keys):
+    """Fisheye lens: camera samples outside the unit circle are dropped
+    (Camera.java:263-270).  The scene-uniform wavefront kernels read the
+    dropped flag from the origin's w; the film counts equal the oracle's
+    accepted samples exactly, with the material sort on and off."""
+    _same_streams(_lens_scene(64, lens=abi.LENS_FISHEYE), 64, seed=33, kernels=(0,), sort_keys=sort_keys)
+
+
+def test_fisheye_in_a_two_scene_batch():
+    """The same fisheye job packed with a job on a second resident scene runs
+    the per-job (not scene-uniform) kernels, which read the dropped flag from
+    the path record; both jobs match the oracle on their own streams."""
+    from dayflower_amd.renderer import make_job
+    from tests.oracle_lib import OracleScene
+    res = 48
+    sc_a = _lens_scene(res, lens=abi.LENS_FISHEYE)
+    sc_b = S.cornell_box(res, res)
+    r = _renderer()
+    a, b = r.addScene(sc_a), r.addScene(sc_b)
+    jobs = [make_job(a, sc_a.camera, res, res, 8, seed=41), make_job(b, sc_b.camera, res, res, 8, seed=42)]
+    films = r.batchRender(jobs)
+    r.dispose()
+    for sc, job, g in ((sc_a, jobs[0], films[0]), (sc_b, jobs[1], films[1])):
+        o = OracleScene(sc)
+        s, _, c = o.render(res, res, 8, seed=job.seed, max_bounce=8)
+        o.close()
+        ref = s / np.maximum(c, 1)[..., None]
+        close = np.isclose(g, ref, rtol=1e-3, atol=1e-4).all(-1).mean()
+        rel = abs(g.mean() - ref.mean()) / ref.mean()
+        print(f"batch job seed {job.seed}: pixels equal {close:.5f}, mean rel diff {rel:.2e}, "
+              f"dropped pixels {(c == 0).sum()}", flush=True)
+        assert rel < 0.005 and close >= 0.95
+    assert (OracleScene(sc_a).render(res, res, 1, seed=41)[2] == 0).sum() > 100
+
+
+@pytest.mark.parametrize("algorithm", [abi.ALGO_PATH_TRACING, abi.ALGO_AMBIENT_OCCLUSION, abi.ALGO_RAY_CASTING])
+def test_preview_mode_same_streams(algorithm):
+    """Preview mode paints primary misses white (Scene.java:308, 603, 692):
+    the sky no longer shows behind the sphere."""
+    res = 48
+    sc = _lens_scene(res)
+    kernels = (0, 1) if algorithm == abi.ALGO_PATH_TRACING else (1,)
+    _same_streams(sc, res, seed=35, kernels=kernels, algorithm=algorithm, preview=True, samples=4,
+                  max_distance=0.5)
+
+
+def _sphere_light_scene(res, inside=False):
+    sc = S.Scene()
+    m = sc.add_material(S.matte((0.6, 0.55, 0.5)))
+    black = sc.add_material(S.matte((0.0, 0.0, 0.0)))
+    g = sc.add_material(S.plastic((0.3, 0.4, 0.5), (0.2, 0.2, 0.2), 0.3))
+    sc.add_plane((0, 0, 0), (1, 0, 0), (0, 0, -1), (0, 1, 0), m)
+    p, t, n = S.displaced_icosphere(2000, 0x5EED, radius=0.4, center=(0.3, 0.45, 0.2))
+    sc.add_mesh(p, t, g, normals=n)
+    if inside:
+        sc.add_sphere_light((0.0, 0.3, 0.0), 6.0, (0.8, 0.8, 0.9), black, two_sided=True)
+    else:
+        sc.add_sphere_light((-0.6, 1.4, -0.3), 0.35, (6.0, 5.5, 5.0), black)
+        sc.add_sphere_light((0.9, 0.9, 0.6), 0.8, (1.0, 1.5, 2.0), black)
+    sc.set_camera(S.camera_record((0.0, 1.2, -2.8), (0.0, 0.3, 0.0), res, res, fov_x_deg=45.0))
+    return sc
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("inside", [False, True])
+def test_sphere_area_lights_same_streams(mode, inside):
+    """Sphere area lights sampled by cone (outside) or by area (inside a
+    two-sided enclosing sphere), in the reference light-pdf mode and in the
+    area mode; both kernels."""
+    _same_streams(_sphere_light_scene(48, inside), 48, seed=37 + mode, light_pdf_mode=mode)
+
+
+def test_thin_disney_material_is_refused():
+    """The thin form of the Disney material (its fake-subsurface and
+    diffuse-transmission lobes) is not compiled: setScene refuses it."""
+    from dayflower_amd.renderer import UnsupportedOperationException
+    sc = S.icosphere_scene(500, width=16, height=16, mesh_material=S.disney((0.5, 0.5, 0.5)))
+    sc.materials[0].thin = 1
+    r = _renderer()
+    with pytest.raises(UnsupportedOperationException):
+        r.setScene(sc)
+    r.dispose()

@@ -261,3 +261,149 @@ def test_smooth_glass_reflects_and_refracts_by_fresnel(oracle, theta_deg):
     st = math.sin(t) / eta
     np.testing.assert_allclose(wi, (-st, 0.0, -math.sqrt(1 - st * st)), atol=2e-6)
     np.testing.assert_allclose(f * abs(wi[2]) / pdf, kt / eta ** 2, rtol=1e-5)
+
+
+# ---------------------------------------------------------------- camera
+def _lens_ray(oracle, cam, x, y, px, py, lu, lv):
+    out = np.zeros(6, np.float32)
+    assert oracle.orc_lens_ray(_p(np.asarray(cam, np.float32)), float(x), float(y), float(px), float(py), float(lu),
+                               float(lv), _p(out))
+    return out[:3].astype(np.float64), out[3:].astype(np.float64)
+
+
+def test_thin_lens_rays_meet_on_the_focal_plane(oracle):
+    """Camera.createPrimaryRay with aperture radius a and focal distance f
+    (M/scene/Camera.java:272-282): the origin is eye + a (x U + y V) for the
+    uniform disk point (sqrt(u) cos 2 pi v, sqrt(u) sin 2 pi v), and every
+    ray of one image position passes through eye + f (P1 - eye), P1 being
+    the point one unit in front of the eye that a pinhole ray crosses."""
+    a, f = 0.2, 3.5
+    cam = S.camera_record((0.3, 1.0, -3.0), (0.0, 0.5, 0.0), 64, 48, fov_x_deg=45.0, aperture=a, focal_distance=f)
+    pin = cam.copy()
+    pin[15] = 0.0
+    U, V = cam[3:6].astype(np.float64), cam[6:9].astype(np.float64)
+    eye = cam[12:15].astype(np.float64)
+    rng = np.random.default_rng(21)
+    r2 = []
+    for x, y, px, py in ((0, 0, 0.5, 0.5), (31, 17, 0.2, 0.7), (63, 47, 0.9, 0.1)):
+        o0, d0 = _lens_ray(oracle, pin, x, y, px, py, 0.3, 0.4)
+        np.testing.assert_allclose(o0, eye, atol=1e-6)
+        # the pinhole ray reaches the plane one unit along W at P1; the focal point is eye + f (P1 - eye)
+        W = cam[9:12].astype(np.float64)
+        P1 = o0 + d0 / np.dot(d0, W)
+        focus = eye + f * (P1 - eye)
+        for lu, lv in rng.random((40, 2)):
+            o, d = _lens_ray(oracle, cam, x, y, px, py, lu, lv)
+            disk = np.array([math.sqrt(lu) * math.cos(2 * math.pi * lv), math.sqrt(lu) * math.sin(2 * math.pi * lv)])
+            np.testing.assert_allclose(o, eye + a * (disk[0] * U + disk[1] * V), atol=2e-6)
+            t = np.dot(focus - o, d)
+            np.testing.assert_allclose(o + t * d, focus, atol=2e-5)
+            r2.append(np.sum((o - eye) ** 2))
+    assert np.mean(r2) == pytest.approx(a * a / 2.0, rel=0.15)  # uniform over the disk
+
+
+def test_preview_mode_paints_primary_misses_white():
+    """Scene.java:308, 603, 692: with preview mode a primary ray that leaves
+    the scene is white for path tracing, ambient occlusion and ray casting;
+    hits are unchanged and without preview the same misses are black."""
+    sc = S.Scene()
+    m = sc.add_material(S.matte((0.5, 0.5, 0.5)))
+    sc.add_sphere((0.0, 0.0, 0.0), 0.5, m)
+    sc.add_point_light((1.0, 2.0, -2.0), (5.0, 5.0, 5.0))
+    sc.set_camera(S.camera_record((0.0, 0.0, -3.0), (0.0, 0.0, 0.0), 24, 24, fov_x_deg=40.0))
+    o = oracle_lib.OracleScene(sc)
+    rays = o.primary_rays(24, 24, 5, 0)
+    prim, _, _ = o.trace_f32(rays)
+    miss = (prim < 0).reshape(24, 24)
+    assert 50 < miss.sum() < 24 * 24 - 50
+    for algo in (2, 0, 3):  # path tracing, ambient occlusion, ray casting
+        s1, _, c1 = o.render(24, 24, 1, algorithm=algo, seed=5, preview=True)
+        s0, _, c0 = o.render(24, 24, 1, algorithm=algo, seed=5, preview=False)
+        np.testing.assert_array_equal(s1[miss], 1.0)
+        np.testing.assert_array_equal(s0[miss], 0.0)
+        np.testing.assert_array_equal(s1[~miss], s0[~miss])
+    o.close()
+
+
+# ---------------------------------------------------------------- sphere light
+def test_sphere_light_outside_closed_form():
+    """A Lambertian plane under a sphere light of radius r and radiance Le:
+    the sphere subtends a cone of half angle asin(r / D) seen from a point at
+    distance D from its centre, so the irradiance is pi Le (r / D)^2 cos(theta)
+    (theta between the normal and the centre; the sphere stays above the
+    horizon) and the outgoing radiance (albedo / pi) E.  Area mode (cone pdf
+    for light samples and for MIS) converges to it; the reference mode's MIS
+    pdf (d^2 / |cos| times 4 pi, Sphere3F.java:460-479) does not."""
+    Le, r, C = 3.0, 0.4, np.array([0.2, 1.5, 0.1])
+    sc = S.Scene()
+    m = sc.add_material(S.matte((0.5, 0.5, 0.5)))
+    black = sc.add_material(S.matte((0.0, 0.0, 0.0)))
+    sc.add_plane((0, 0, 0), (1, 0, 0), (0, 0, -1), (0, 1, 0), m)
+    sc.add_sphere_light(tuple(C), r, (Le, Le, Le), black)
+    res = 8
+    sc.set_camera(S.camera_record((0.0, 0.8, -2.5), (0.1, 0.0, 0.0), res, res, fov_x_deg=25.0))
+    o = oracle_lib.OracleScene(sc)
+    spp = 2048
+    s, q, c = o.render(res, res, spp, seed=13, max_bounce=1, light_pdf_mode=1)
+    mean = s[..., 0] / c
+    se = np.sqrt(np.maximum(q[..., 0] / c - mean ** 2, 0) / c)
+    cam = sc.camera
+    n_ok = 0
+    for y in range(res):
+        for x in range(res):
+            vals = []
+            for px, py in ((0.1, 0.1), (0.9, 0.1), (0.1, 0.9), (0.9, 0.9), (0.5, 0.5)):
+                og, dg = _lens_ray(oracle_lib.load(), cam, x, y, px, py, 0.5, 0.5)
+                hp = og + dg * (-og[1] / dg[1])
+                v = C - hp
+                D = np.linalg.norm(v)
+                vals.append(0.5 / math.pi * math.pi * Le * (r / D) ** 2 * v[1] / D)
+            if min(vals) - 4 * se[y, x] <= mean[y, x] <= max(vals) + 4 * se[y, x]:
+                n_ok += 1
+    assert n_ok >= res * res - 2, n_ok
+    o.close()
+
+
+def test_sphere_light_reference_mode_weights_mis_with_the_surface_pdf():
+    """The reference's MIS weight for a BSDF-sampled ray toward a sphere
+    light uses d^2 / |cos| times 4 pi (object units) while its light samples
+    carry the cone pdf: the weights no longer sum to one.  For a small far
+    sphere the surface pdf is huge and the light samples take all the weight
+    (the two modes agree within 1 %); for a large near sphere part of the
+    light is lost (the reference mode renders 5-12 % darker)."""
+    def mean(r, C, mode):
+        sc = S.Scene()
+        m = sc.add_material(S.matte((0.5, 0.5, 0.5)))
+        black = sc.add_material(S.matte((0.0, 0.0, 0.0)))
+        sc.add_plane((0, 0, 0), (1, 0, 0), (0, 0, -1), (0, 1, 0), m)
+        sc.add_sphere_light(C, r, (3.0, 3.0, 3.0), black)
+        sc.set_camera(S.camera_record((0.0, 0.8, -2.5), (0.1, 0.0, 0.0), 8, 8, fov_x_deg=25.0))
+        o = oracle_lib.OracleScene(sc)
+        s, _, c = o.render(8, 8, 1024, seed=13, max_bounce=1, light_pdf_mode=mode)
+        o.close()
+        return (s[..., 0] / c).mean()
+    small = mean(0.4, (0.2, 1.5, 0.1), 0) / mean(0.4, (0.2, 1.5, 0.1), 1)
+    large = mean(2.0, (0.0, 2.2, 0.0), 0) / mean(2.0, (0.0, 2.2, 0.0), 1)
+    assert abs(small - 1.0) < 0.01
+    assert 0.88 < large < 0.95
+
+
+def test_sphere_light_inside_closed_form():
+    """A plane inside a two-sided emitting sphere of radiance Le: every
+    direction of the upper hemisphere sees the sphere, so E = pi Le and the
+    outgoing radiance is albedo * Le everywhere (area mode: uniform points
+    on the sphere with pdf d^2 / (|cos| 4 pi r^2))."""
+    Le, albedo = 2.0, 0.6
+    sc = S.Scene()
+    m = sc.add_material(S.matte((albedo, albedo, albedo)))
+    black = sc.add_material(S.matte((0.0, 0.0, 0.0)))
+    sc.add_plane((0, 0, 0), (1, 0, 0), (0, 0, -1), (0, 1, 0), m)
+    sc.add_sphere_light((0.0, 0.3, 0.0), 4.0, (Le, Le, Le), black, two_sided=True)
+    sc.set_camera(S.camera_record((0.0, 1.0, -1.5), (0.0, 0.0, 0.0), 8, 8, fov_x_deg=30.0))
+    o = oracle_lib.OracleScene(sc)
+    s, q, c = o.render(8, 8, 4096, seed=17, max_bounce=1, light_pdf_mode=1)
+    mean = s[..., 0] / c
+    assert abs(mean.mean() - albedo * Le) < 0.01 * albedo * Le
+    se = np.sqrt(np.maximum(q[..., 0] / c - mean ** 2, 0) / c)
+    assert (np.abs(mean - albedo * Le) <= 4 * se + 1e-6).mean() >= 0.95
+    o.close()
