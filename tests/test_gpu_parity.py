@@ -191,8 +191,9 @@ def test_cornell_path_tracing_same_streams(kernel):
     Fresnel draw, a grazing edge).  How often rounding alone does that is
     fixed by the oracle itself: built with FMA contraction it agrees with its
     own non-contracted build on 99.0 % of 1-spp paths of this scene (and on
-    84.9 % of 64x64 pixels at 32 spp).  Bars: >= 98 % of 1-spp paths equal
-    (rtol 1e-3), and the 32-spp image mean within 0.5 %."""
+    84.9 % of 64x64 pixels at 32 spp).  Measured on MI355X: 98.77 % of paths
+    equal, image means within 0.04 %.  Bars: >= 98.5 % of 1-spp paths equal
+    (rtol 1e-3), and the 32-spp image mean within 0.2 %."""
     from tests.oracle_lib import OracleScene
     res = 128
     sc = S.cornell_box(res, res)
@@ -221,8 +222,8 @@ def test_cornell_path_tracing_same_streams(kernel):
     close = np.isclose(g, ref, rtol=1e-3, atol=1e-4).all(-1).mean()
     print(f"kernel {kernel}: 1-spp paths equal {paths:.4f}; 32 spp pixels equal {close:.4f}, "
           f"mean gpu {g.mean():.5f} ref {ref.mean():.5f}, {st}")
-    assert paths >= 0.98
-    assert abs(g.mean() - ref.mean()) / ref.mean() < 0.005
+    assert paths >= 0.985
+    assert abs(g.mean() - ref.mean()) / ref.mean() < 0.002
 
 
 @pytest.mark.parametrize("name", ["plastic_perez_point", "matte_glass_metal_disney", "glossy_substrate_translucent_uber",
@@ -233,8 +234,11 @@ def test_materials_and_lights_same_streams(name):
     four-patch Matte/Glass/Metal/Disney icosphere under a point light or
     under a spot and a directional light, and the four-patch
     Glossy/Substrate/Translucent/Uber icosphere on a ClearCoat ground; both
-    kernels.  Bars as for the Cornell box: >= 98 % of 1-spp
-    paths equal (rtol 1e-3) and the 16-spp image mean within 0.5 %."""
+    kernels.  Measured on MI355X (paths equal / mean difference): 99.99 % /
+    7e-8, 99.88 % / 2.6e-4, 100 % / 2.2e-5, 99.97 % / 1.1e-5.  Bars just
+    below: >= 99.9 % of 1-spp paths equal (99.8 % for the glass scene, whose
+    Fresnel draws rounding flips most often; rtol 1e-3) and the 16-spp image
+    mean within 0.1 %."""
     from tests.oracle_lib import OracleScene
     from dayflower_amd import workloads as W
     res = 96
@@ -270,8 +274,8 @@ def test_materials_and_lights_same_streams(name):
         paths = np.isclose(g1, ref1, rtol=1e-3, atol=1e-4).all(-1).mean()
         rel = abs(g16.mean() - ref16.mean()) / ref16.mean()
         print(f"{name} kernel {kernel}: 1-spp paths equal {paths:.4f}; 16-spp mean rel diff {rel:.2e}")
-        assert paths >= 0.98
-        assert rel < 0.005
+        assert paths >= (0.998 if name == "matte_glass_metal_disney" else 0.999)
+        assert rel < 0.001
 
 
 def test_cornell_statistical_parity():
