@@ -114,7 +114,8 @@ extern "C" {
  *   GLASS   kr, kt, eta[0], roughness_u/v, remap_roughness, emission
  *   METAL   eta[0..2], k[0..2], roughness_u/v, remap_roughness, emission
  *   PLASTIC kd, ks, roughness_u, remap_roughness, emission
- *   DISNEY  kd (base color), the disney_* scalars, eta[0], thin, emission
+ *   DISNEY  kd (base color), the disney_* scalars, eta[0], emission (thin
+ *           != 0 is refused with DFR_E_UNSUPPORTED)
  *   GLOSSY      kr, roughness_u (used as is), emission
  *   SUBSTRATE   kd, ks, roughness_u/v, remap_roughness, emission
  *   TRANSLUCENT kd, ks, kr (reflectance), kt (transmittance), roughness_u,
@@ -171,7 +172,8 @@ typedef struct dfr_primitive {
 /* One light.
  *   DIFFUSE_AREA  radiance, two_sided, primitive (its shape), triangle
  *                 (global triangle index when the shape is one triangle of a
- *                 mesh primitive, else -1)
+ *                 mesh primitive; -1 when the shape is a sphere primitive,
+ *                 whose area_light must name this light)
  *   POINT         radiance = intensity, position
  *   PEREZ         turbidity, direction = sun direction (world space)
  *   SPOT          radiance = intensity, position, direction = cone axis
