@@ -135,6 +135,7 @@ ENTRY_POINTS = [
     ("dfr_read_rgba8", c_int32, [_c_void_p, P(ctypes.c_uint8)]),
     ("dfr_get_stats", c_int32, [_c_void_p, P(Stats)]),
     ("dfr_trace_rays", c_int32, [_c_void_p, _c_void_p, c_int64, c_int32, _c_void_p, _c_void_p, _c_void_p, c_int32]),
+    ("dfr_trace_rays_noalloc", c_int32, [_c_void_p, _c_void_p, c_int64, c_int32, _c_void_p, _c_void_p, _c_void_p]),
     ("dfr_cast_primary", c_int32, [_c_void_p, c_int32, P(c_float), P(c_int32), P(c_int64), P(c_float)]),
     ("dfr_scene_add", c_int32, [_c_void_p, P(SceneDesc)]),
     ("dfr_scene_remove", c_int32, [_c_void_p, c_int32]),

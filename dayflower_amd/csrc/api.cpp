@@ -787,6 +787,9 @@ dfr_ctx* dfr_create(int32_t device) {
     Ctx* c = new Ctx();
     c->device = dev;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return nullptr; }
+    // the address of this device's copy of the no-allocation scratch array
+    // (dfr_trace_rays_noalloc reports a failure here when it is called)
+    c->scratch_status = noalloc_scratch_init(dev);
     hipEventCreate(&c->ev0);
     hipEventCreate(&c->ev1);
     return reinterpret_cast<dfr_ctx*>(c);
@@ -1121,6 +1124,37 @@ int32_t dfr_trace_rays(dfr_ctx* h, const float* rays, int64_t n, int32_t any_hit
         e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) return c->hip_fail(e, "trace");
     }
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->stats = dfr_stats{};
+    c->stats.seconds = ms * 1e-3;
+    if (any_hit) c->stats.rays_any = n; else c->stats.rays_closest = n;
+    return DFR_OK;
+}
+
+int32_t dfr_trace_rays_noalloc(dfr_ctx* h, const float* rays, int64_t n, int32_t any_hit, int32_t* prim, int64_t* tri,
+                               float* t) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (n < 0) return c->fail(DFR_E_ARGUMENT, "n < 0");
+    if (n == 0) return DFR_OK;
+    if (!rays || !prim || !tri || !t) return c->fail(DFR_E_NULL, "null pointer");
+    if (c->main_scene < 0) return c->fail(DFR_E_STATE, "setup before trace");
+    if (c->scratch_status != (int)hipSuccess) return c->hip_fail((hipError_t)c->scratch_status, "scratch address");
+    int* scratch_ptr = noalloc_scratch(c->device);
+    if (!scratch_ptr) return c->fail(DFR_E_DEVICE, "scratch address unknown on this device");
+    // the deepest traversal stack of the scene must fit the static overflow
+    const int need = overflow_entries_per_thread(c->scenes[c->main_scene]->max_depth);
+    if (need > noalloc_stack_entries())
+        return c->fail(DFR_E_UNSUPPORTED, "BVH too deep for the no-allocation trace: needs " + std::to_string(need) +
+                                              " overflow stack entries per thread, the static scratch holds " +
+                                              std::to_string(noalloc_stack_entries()));
+    const DevScene* sc = (const DevScene*)c->scene_table.ptr + c->main_scene;
+    hipEventRecord(c->ev0, c->stream);
+    launch_trace_rays_noalloc(sc, (const float4*)rays, n, any_hit, prim, tri, t, scratch_ptr, c->stream);
+    hipEventRecord(c->ev1, c->stream);
+    const hipError_t e = hipEventSynchronize(c->ev1);
+    if (e != hipSuccess) return c->hip_fail(e, "trace");
     float ms = 0;
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
     c->stats = dfr_stats{};

@@ -81,6 +81,7 @@ struct Ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string error;
     std::atomic<bool> stop{false};
+    int scratch_status = 0;    // hipError_t of noalloc_scratch_init on this device
 
     // GPURenderer settings (AbstractGPURenderer.java:143-159)
     int width = 800, height = 800;

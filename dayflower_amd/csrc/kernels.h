@@ -92,6 +92,17 @@ void launch_cast_primary(const DevScene* scene, const DevJob* job, float* rays, 
                          float* t, int* overflow, int n_pixels, hipStream_t s);
 void launch_rgba8(const float4* film, uint8_t* out, int n, int tone_mapper, hipStream_t s);
 
+// Ray queries with the traversal overflow in the library's static device
+// array (no allocation).  noalloc_scratch_init takes the array's address on
+// `device`, which must be current, and returns a hipError_t code;
+// noalloc_scratch gives that address (null before the init succeeded);
+// noalloc_stack_entries is the number of overflow entries each thread owns.
+int noalloc_scratch_init(int device);
+int* noalloc_scratch(int device);
+int noalloc_stack_entries();
+void launch_trace_rays_noalloc(const DevScene* scene, const float4* rays, int64_t n, int any_hit, int32_t* prim,
+                               int64_t* tri, float* t, int* scratch_ptr, hipStream_t s);
+
 struct WavefrontParams {
     int block;          // threads per block
     int sort_keys;      // 0: unsorted; 1: shade in (material, light type) order

@@ -10,6 +10,8 @@
 //    atomic per block and queue).
 //  * ray queries: closest/any hit for a ray array (hit-parity checks and the
 //    traversal benchmark).
+#include <mutex>
+
 #include "device.h"
 #include "kernels.h"
 
@@ -428,6 +430,54 @@ void launch_trace_rays(const DevScene* scene, const float4* rays, int64_t n, int
     (void)block;
     hipLaunchKernelGGL((k_trace_rays<256, SHORT_STACK>), dim3(grid), dim3(256), 0, s, scene, rays, n, any_hit, prim, tri, t,
                        overflow);
+}
+
+// ---------------------------------------------------------------------------
+// Ray queries without allocation: the traversal stacks' HBM overflow lives in
+// a file-scope device array (`scratch`) instead of a buffer the library
+// allocates.  The launch has a fixed grid of NOALLOC_BLOCKS blocks of 256
+// threads that stride over the rays, and every thread owns NOALLOC_STACK
+// overflow entries (entry k of thread g at scratch[k * NOALLOC_THREADS + g]),
+// so the array holds NOALLOC_THREADS * NOALLOC_STACK ints (24 MiB).  Every
+// call on one device shares that device's array: calls that overlap on a
+// device race.  Nothing in it needs clearing (entries are written before
+// they are read).
+// ---------------------------------------------------------------------------
+constexpr int NOALLOC_BLOCKS = 1024;
+constexpr int NOALLOC_THREADS = NOALLOC_BLOCKS * 256;
+constexpr int NOALLOC_STACK = 24;
+static __device__ __align__(256) int scratch[(size_t)NOALLOC_THREADS * NOALLOC_STACK];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+int* g_scratch_ptr[MAX_DEVICES] = {};
+}  // namespace
+
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return (int)hipSuccess;
+    // the device is current: its own copy of the array
+    int* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    g_scratch_ptr[device] = ptr;
+    return (int)hipSuccess;
+}
+
+int* noalloc_scratch(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch_ptr[device];
+}
+
+int noalloc_stack_entries() { return NOALLOC_STACK; }
+
+void launch_trace_rays_noalloc(const DevScene* scene, const float4* rays, int64_t n, int any_hit, int32_t* prim,
+                               int64_t* tri, float* t, int* scratch_ptr, hipStream_t s) {
+    hipLaunchKernelGGL((k_trace_rays<256, SHORT_STACK>), dim3(NOALLOC_BLOCKS), dim3(256), 0, s, scene, rays, n, any_hit,
+                       prim, tri, t, scratch_ptr);
 }
 
 template <int BLOCK, int SHORT>

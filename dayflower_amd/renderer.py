@@ -270,6 +270,10 @@ class GPURenderer:
         """Device-resident variant: every pointer is a device address."""
         self._c(self.lib.dfr_trace_rays(self.ctx, rays_ptr, n, int(any_hit), prim_ptr, tri_ptr, t_ptr, 1))
 
+    def trace_rays_device_noalloc(self, rays_ptr, n, prim_ptr, tri_ptr, t_ptr, any_hit=False):
+        """Device-resident variant that allocates nothing (static scratch)."""
+        self._c(self.lib.dfr_trace_rays_noalloc(self.ctx, rays_ptr, n, int(any_hit), prim_ptr, tri_ptr, t_ptr))
+
     def cast_primary(self, width, height, sample_index=0):
         n = width * height
         rays = np.empty((n, 8), np.float32)

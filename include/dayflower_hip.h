@@ -334,6 +334,19 @@ int32_t dfr_trace_rays(dfr_ctx* ctx, const float* rays, int64_t n_rays,
                        int32_t any_hit, int32_t* prim, int64_t* tri, float* t,
                        int32_t device_ptrs);
 
+/* The closest/any-hit query of dfr_trace_rays on device pointers, allocating
+ * nothing: the traversal stacks' overflow lives in a static device array of
+ * 24 MiB that the library holds on each device it runs on (1024 blocks of
+ * 256 threads stride over the rays; 24 overflow entries per thread).  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  rays, prim, tri and t are device pointers.  Returns
+ * DFR_E_UNSUPPORTED when the scene's BVH needs more overflow entries per
+ * thread than the array holds.  Calls that overlap on one device share the
+ * array and race. */
+int32_t dfr_trace_rays_noalloc(dfr_ctx* ctx, const float* rays, int64_t n_rays, int32_t any_hit, int32_t* prim,
+                               int64_t* tri, float* t);
+
 /* Primary rays of the current camera for pass `sample_index` with the
  * uniform sub-pixel offset of ray casting (GPURenderer.java:338-369), traced
  * to the closest hit.  Writes the rays used (8 floats each) and hit records,
