@@ -10,8 +10,12 @@ whole film (64 by default: one converging frame).
 Prints ONE JSON line with the throughput (path samples/s), Mrays/s, the
 roofline of the dominant kernel (BVH closest-hit traversal of the wavefront
 extend stage, timed with HIP events on the library's stream inside the timed
-region) and a CPU baseline (the oracle, one thread, on a bounded pixel
-window of the same frame).
+region; algorithmic bytes priced per ray as 48 + 32 N_node + 36 N_tri with
+N_node and N_tri counted by the oracle's FP64 walk of the same BVH over a
+sample of the frame's own rays), the CPU baseline (the oracle on all host
+cores and on one core, on a bounded pixel window of the same frame, checked
+against the GPU film of the same random streams) and the batched leg (64
+mixed jobs: the plain per-job kernel against the packed wavefront launch).
 """
 import argparse
 import json
@@ -66,31 +70,112 @@ def copy_calibration(mib):
     del a, b
 
 
-def cpu_baseline(sc, width, height, spp, max_bounce, min_rr, budget_s):
-    """Oracle (float32 restatement of CPURenderer), one thread, on a centred
-    pixel window sized to take about `budget_s` seconds."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sc, width, height, spp, max_bounce, min_rr, seed, budget_s, gpu_film=None):
+    """The oracle (float32 restatement of CPURenderer) on a centred pixel
+    window of the same frame and random streams: on all host cores (the
+    reported value, window sized to about `budget_s` seconds) and on one
+    core (about a third of that).  When the GPU film of the same streams is
+    given, the window is compared with it path by path."""
+    import numpy as np
     from tests.oracle_lib import OracleScene
     o = OracleScene(sc)
+    cores = max(1, min(16, os.cpu_count() or 1))
     cx, cy = width // 2, height // 2
-    # probe a small window to estimate the rate
-    w0 = 8
-    t = time.perf_counter()
-    o.render(width, height, spp, seed=99, max_bounce=max_bounce, min_rr=min_rr, threads=1,
-             window=(cx - w0 // 2, cy - w0 // 2, cx + w0 // 2, cy + w0 // 2))
-    dt = max(time.perf_counter() - t, 1e-3)
-    rate = w0 * w0 * spp / dt
-    side = int(max(8, min(256, (rate * budget_s / spp) ** 0.5)))
-    win = (cx - side // 2, cy - side // 2, cx - side // 2 + side, cy - side // 2 + side)
-    t = time.perf_counter()
-    o.render(width, height, spp, seed=99, max_bounce=max_bounce, min_rr=min_rr, threads=1, window=win)
-    dt = time.perf_counter() - t
+
+    def window(side):
+        return (cx - side // 2, cy - side // 2, cx - side // 2 + side, cy - side // 2 + side)
+
+    def timed(side, threads):
+        t = time.perf_counter()
+        out = o.render(width, height, spp, seed=seed, max_bounce=max_bounce, min_rr=min_rr, threads=threads,
+                       window=window(side))
+        return time.perf_counter() - t, out
+
+    # rate probes on small windows, then windows sized to the budget
+    dt1, _ = timed(8, 1)
+    rate1 = 64 * spp / max(dt1, 1e-3)
+    side1 = int(max(8, min(256, (rate1 * budget_s / 3.0 / spp) ** 0.5)))
+    dt1, _ = timed(side1, 1)
+    side = int(max(16, min(1024, (rate1 * cores * budget_s / spp) ** 0.5)))
+    dt, (s, q, c) = timed(side, cores)
     o.close()
-    return {"value": side * side * spp / dt, "unit": "path samples/s", "cores": 1, "kind": "port",
-            "sample": f"{side}x{side} pixel window at the image centre, {spp} spp, one thread, {dt:.1f} s"}
+    res = {"value": side * side * spp / dt, "unit": "path samples/s", "cores": cores, "kind": "port",
+           "sample": f"{side}x{side} pixel window at the image centre, {spp} spp, {cores} threads, {dt:.1f} s",
+           "cpu_model": _cpu_model(),
+           "one_core": {"value": round(side1 * side1 * spp / dt1, 1), "cores": 1,
+                        "sample": f"{side1}x{side1} window, {spp} spp, {dt1:.1f} s"}}
+    if gpu_film is not None:
+        x0, y0, x1, y1 = window(side)
+        ref = (s / np.maximum(c, 1)[..., None])[y0:y1, x0:x1]
+        g = gpu_film[y0:y1, x0:x1]
+        px = np.isclose(g, ref, rtol=1e-3, atol=1e-4).all(-1).mean()
+        rel = abs(float(g.mean()) - float(ref.mean())) / max(float(ref.mean()), 1e-12)
+        res["same_streams_vs_gpu"] = {"pixels_equal": round(float(px), 5), "mean_rel_diff": float(f"{rel:.3g}"),
+                                      "window": f"{side}x{side} at the centre, {spp} spp"}
+    return res
 
 
-def run_batched(args):
-    """64 mixed jobs: plain per-job kernel vs the packed persistent wavefront."""
+def traversal_counts(sample, nodes, tris):
+    """Node visits and triangle tests per ray of the oracle's FP64 walk of the
+    product BVH over the sampled rays, beside the GPU's counts of the same
+    rays (which include the speculative traversal's extra visits), and the
+    agreement of the closest hits."""
+    import numpy as np
+    from tests import oracle_lib
+    rec = np.concatenate([sample["o"], sample["tmax"][:, None], sample["d"],
+                          sample["any"].view(np.float32)[:, None], np.zeros((len(sample["o"]), 4), np.float32)], 1)
+    nv, nt, hit, _ = oracle_lib.bvh4_walk(nodes, tris, rec)
+    out = {}
+    for name, sel in (("extend", sample["any"] == 0), ("shadow_any", sample["any"] == 1)):
+        if not sel.any():
+            continue
+        d = {"rays": int(sel.sum()), "nodes_per_ray": round(float(nv[sel].mean()), 3),
+             "tris_per_ray": round(float(nt[sel].mean()), 3),
+             "gpu_nodes_per_ray": round(float(sample["nodes"][sel].mean()), 3),
+             "gpu_tris_per_ray": round(float(sample["tris"][sel].mean()), 3)}
+        if name == "extend":
+            d["hit_triangle_agreement"] = round(float((hit[sel] == sample["tri"][sel]).mean()), 6)
+        else:
+            d["occlusion_agreement"] = round(float(((hit[sel] >= 0) == (sample["prim"][sel] >= 0)).mean()), 6)
+        out[name] = d
+    return out
+
+
+def valu_profile():
+    """VALU busy / utilisation of the timed traversal and shade kernels from
+    the committed counter summary of the default frame."""
+    path = os.path.join(ROOT, "profiles", "valu_icosphere1m_1080p.txt")
+    if not os.path.exists(path):
+        return None
+    rows = {}
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#") and "|" in line and "k_" in line:
+                cols = [c.strip() for c in line[1:].split("|")]
+                try:
+                    rows[cols[0]] = {"busy": float(cols[4]), "utilization": float(cols[5])}
+                except (ValueError, IndexError):
+                    pass
+    names = {"k_extend": "k_trace_queue<256, 16, 8, false, false, true>",
+             "k_shadow_any": "k_trace_queue<128, 16, 4, false, true, true>", "k_shade": "k_shade<512, false, 2>"}
+    out = {k: rows[v] for k, v in names.items() if v in rows}
+    return out or None
+
+
+def batched_leg(n_jobs=64, autotune=True):
+    """64 mixed jobs: the plain per-job kernel (one launch per job per
+    sample) against the packed persistent wavefront launch."""
     from dayflower_amd import abi, workloads as W
     from dayflower_amd.renderer import GPURenderer
     t0 = time.perf_counter()
@@ -98,10 +183,10 @@ def run_batched(args):
     r = GPURenderer()
     ids = [r.addScene(sc) for sc in scenes]
     log(f"batched: 4 scenes resident in {time.perf_counter() - t0:.1f} s")
-    jobs = W.batched_jobs(ids, n_jobs=args.jobs)
+    jobs = W.batched_jobs(ids, n_jobs=n_jobs)
     total = sum(j.width * j.height * j.spp for j in jobs)
     t0 = time.perf_counter()
-    tuned = r.autotune(jobs)
+    tuned = r.autotune(jobs) if autotune else tuple(r.getWavefront().values())
     log(f"batched: autotune -> block {tuned[0]}, pool {tuned[1]}, sort keys {tuned[2]} ({time.perf_counter() - t0:.1f} s)")
     res = {}
     for name, mode in (("plain", abi.BATCH_PLAIN), ("packed", abi.BATCH_PACKED)):
@@ -114,12 +199,18 @@ def run_batched(args):
                      "mrays_per_s": round((st["rays_closest"] + st["rays_any"]) / wall / 1e6, 1)}
         log(f"batched: {name} {wall:.2f} s, {total / wall / 1e6:.1f} M samples/s")
     r.dispose()
-    print(json.dumps({"metric": METRIC, "config": {"workload": f"{len(jobs)} mixed render jobs in one batch "
-                      "(res 128^2..1024^2, spp 1..256, icospheres 10k..2M triangles, Matte/Glass/Metal/Disney)",
-                      "path_samples": total, "autotuned": {"block": tuned[0], "pool_paths": tuned[1],
-                                                           "sort_keys": tuned[2]}},
-                      "plain": res["plain"], "packed": res["packed"],
-                      "speedup_packed_over_plain": round(res["plain"]["seconds"] / res["packed"]["seconds"], 3),
+    return {"workload": f"{len(jobs)} mixed render jobs in one batch (res 128^2..1024^2, spp 1..256, "
+                        "icospheres 10k..2M triangles, Matte/Glass/Metal/Disney, maxBounce 8)",
+            "path_samples": total, "autotuned": {"block": tuned[0], "pool_paths": tuned[1], "sort_keys": tuned[2]},
+            "plain": res["plain"], "packed": res["packed"],
+            "speedup_packed_over_plain": round(res["plain"]["seconds"] / res["packed"]["seconds"], 3)}
+
+
+def run_batched(args):
+    out = batched_leg(args.jobs)
+    print(json.dumps({"metric": METRIC, "config": {k: out[k] for k in ("workload", "path_samples", "autotuned")},
+                      "plain": out["plain"], "packed": out["packed"],
+                      "speedup_packed_over_plain": out["speedup_packed_over_plain"],
                       "unit": "path samples/s", "n_gpus": 1}), flush=True)
 
 
@@ -238,8 +329,9 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--max-bounce", type=int, default=8)
     ap.add_argument("--kernel", choices=["wavefront", "plain"], default="wavefront")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-job batched leg of the default line")
     ap.add_argument("--no-perez", action="store_true")
     ap.add_argument("--scene", choices=["icosphere", "mixed"], default="icosphere",
                     help="icosphere: Plastic mesh (default frame); mixed: the Matte/Glass/Metal/Disney icosphere "
@@ -250,6 +342,8 @@ def main():
     ap.add_argument("--duration", type=float, default=2.0)
     ap.add_argument("--pool", type=int, default=0, help="path pool capacity (0: library default)")
     ap.add_argument("--sort-keys", type=int, default=-1, help="0 (off) or 1 (on) (default: library)")
+    ap.add_argument("--sample-stride", type=int, default=256,
+                    help="every n-th traced ray of the counting pass is recounted by the oracle")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="fill and copy MIB MiB on the device first (reference kernels for HBM counters)")
     args = ap.parse_args()
@@ -280,9 +374,10 @@ def main():
         sc = S.icosphere_scene(args.triangles, seed=0x5EED, width=args.width, height=args.height,
                                with_perez=not args.no_perez)
     log(f"bench: scene {sc.n_triangles} triangles generated in {time.perf_counter() - t0:.1f} s")
+    seed = 0x5EED
     r = GPURenderer()
     r.setScene(sc)
-    r.setSeed(0x5EED)
+    r.setSeed(seed)
     r.setMaximumBounce(args.max_bounce)
     r.setMinimumBounceRussianRoulette(5)
     r.setKernel(0 if args.kernel == "wavefront" else 1)
@@ -293,17 +388,23 @@ def main():
     r.setup()
     log(f"bench: setup (BVH build + upload) {time.perf_counter() - t0:.1f} s")
 
-    # algorithmic traversal counts of the extend stage (untimed pass)
-    r.setCountTraversal(True)
-    r.render(max(1, args.spp // 8))
-    cst = r.stats()
-    r.setCountTraversal(False)
-    nodes_per_ray = cst["extend_nodes"] / max(1, cst["extend_rays"])
-    tris_per_ray = cst["extend_tris"] / max(1, cst["extend_rays"])
-    sh_nodes_per_ray = cst["shadow_nodes"] / max(1, cst["shadow_rays"])
-    sh_tris_per_ray = cst["shadow_tris"] / max(1, cst["shadow_rays"])
-    log(f"bench: extend visits per ray: {nodes_per_ray:.2f} nodes, {tris_per_ray:.2f} triangles; "
-        f"shadow: {sh_nodes_per_ray:.2f} nodes, {sh_tris_per_ray:.2f} triangles")
+    # untimed counting pass: visit counts of the GPU traversal, and every
+    # n-th traced ray recorded for the oracle's recount
+    sample = bvh = None
+    cst = None
+    if args.kernel == "wavefront":
+        r.setCountTraversal(True, sample_stride=args.sample_stride)
+        r.render(max(1, args.spp // 8))
+        cst = r.stats()
+        if not args.no_cpu:
+            sample = r.ray_sample()
+            bvh = r.bvh()
+        r.setCountTraversal(False)
+        log(f"bench: GPU visits per ray: extend {cst['extend_nodes'] / max(1, cst['extend_rays']):.2f} nodes, "
+            f"{cst['extend_tris'] / max(1, cst['extend_rays']):.2f} triangles; shadow "
+            f"{cst['shadow_nodes'] / max(1, cst['shadow_rays']):.2f} nodes, "
+            f"{cst['shadow_tris'] / max(1, cst['shadow_rays']):.2f} triangles"
+            + (f"; {len(sample['o'])} rays recorded" if sample is not None else ""))
 
     for i in range(args.warmup):
         r.render(args.spp)
@@ -329,51 +430,86 @@ def main():
         log(f"bench: step {i + 1}/{args.steps} {st['seconds']:.3f} s")
     wall = time.perf_counter() - t0
     r.setProfile(False)
+    # untimed: one frame of the first spp samples, for the CPU comparison of the same streams
+    gpu_film = None
+    if not args.no_cpu:
+        r.clear()
+        r.render(args.spp)
+        gpu_film, _ = r.film(args.width, args.height)
     r.dispose()
-
     value = tot["samples"] / wall
-    roofline = None
-    stages = None
+
+    cpu = counts = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(sc, args.width, args.height, args.spp, args.max_bounce, 5, seed, args.cpu_seconds,
+                               gpu_film)
+            if sample is not None and len(sample["o"]):
+                counts = traversal_counts(sample, *bvh)
+        except Exception as e:  # the baseline is informative; never hide the GPU number
+            cpu = {"error": str(e)}
+        log(f"bench: cpu baseline {json.dumps(cpu)}")
+        log(f"bench: traversal counts {json.dumps(counts)}")
+
+    roofline = stages = None
     if tot["ext_s"] > 0:
-        # Algorithmic bytes per traversal query: ray in (32) + hit out (16) +
-        # the node record per node visit (two-wide: 4 float4 = 64 B; four-wide:
-        # 6 box planes x 4 children + 4 child references = 7 float4 = 112 B) +
-        # 36 B of vertex positions per triangle test.
-        node_bytes = 112 if bvh_width == 4 else 64
-
-        def bytes_per(nodes, tris):
-            return 32 + 16 + node_bytes * nodes + 36 * tris
-
         launches = max(1, tot["ext_launch"])
-        kern = {
-            "k_extend": {"seconds": tot["ext_s"], "rays": tot["ext_rays"], "bytes_per_ray": bytes_per(nodes_per_ray, tris_per_ray),
-                         "nodes_per_ray": nodes_per_ray, "tris_per_ray": tris_per_ray,
-                         "what": "k_trace_queue on the extension rays: BVH closest hit"},
-            "k_shadow": {"seconds": tot["sh_s"], "rays": tot["sh_rays"], "bytes_per_ray": bytes_per(sh_nodes_per_ray, sh_tris_per_ray),
-                         "nodes_per_ray": sh_nodes_per_ray, "tris_per_ray": sh_tris_per_ray,
-                         "what": "k_trace_queue on the any-hit shadow queue + k_shadow_closest"},
-        }
-        for k in kern.values():
-            k["achieved"] = k["bytes_per_ray"] * k["rays"] / max(k["seconds"], 1e-12) / 1e9
+        gpu_cnt = {"k_extend": (cst["extend_nodes"] / max(1, cst["extend_rays"]), cst["extend_tris"] / max(1, cst["extend_rays"])),
+                   "k_shadow": (cst["shadow_nodes"] / max(1, cst["shadow_rays"]), cst["shadow_tris"] / max(1, cst["shadow_rays"]))}
+        orc = {}
+        if counts:
+            if "extend" in counts:
+                orc["k_extend"] = (counts["extend"]["nodes_per_ray"], counts["extend"]["tris_per_ray"])
+            if "shadow_any" in counts:
+                orc["k_shadow"] = (counts["shadow_any"]["nodes_per_ray"], counts["shadow_any"]["tris_per_ray"])
+
+        # Algorithmic bytes per traversal query: ray in (32) + hit out (16) +
+        # 32 per node visit + 36 of vertex positions per triangle test, with
+        # the visit counts of the oracle's FP64 walk of the same BVH over the
+        # sampled rays (the GPU's own counts when no CPU leg ran).
+        def bytes_per(nodes, tris):
+            return 32 + 16 + 32 * nodes + 36 * tris
+
+        kern = {}
+        for name, secs, rays, what in (("k_extend", tot["ext_s"], tot["ext_rays"], "k_trace_queue on the extension rays: BVH closest hit"),
+                                       ("k_shadow", tot["sh_s"], tot["sh_rays"], "k_trace_queue on the any-hit shadow queue + k_shadow_closest")):
+            n_node, n_tri = orc.get(name, gpu_cnt[name])
+            bpr = bytes_per(n_node, n_tri)
+            kern[name] = {"seconds": secs, "rays": rays, "nodes_per_ray": n_node, "tris_per_ray": n_tri,
+                          "bytes_per_ray": bpr, "achieved": bpr * rays / max(secs, 1e-12) / 1e9, "what": what,
+                          "gpu_nodes_per_ray": gpu_cnt[name][0], "gpu_tris_per_ray": gpu_cnt[name][1],
+                          # the four-wide node record read per visit (six planes x four children + four references)
+                          "node_record_GBs": (48 + 112 * gpu_cnt[name][0] + 36 * gpu_cnt[name][1]) * rays / max(secs, 1e-12) / 1e9}
         stages = {name: {"ms_per_launch": round(k["seconds"] / launches * 1e3, 3), "rays": k["rays"],
                          "nodes_per_ray": round(k["nodes_per_ray"], 2), "tris_per_ray": round(k["tris_per_ray"], 2),
-                         "bytes_per_ray": round(k["bytes_per_ray"], 1), "achieved_GBs": round(k["achieved"], 1)}
+                         "gpu_nodes_per_ray": round(k["gpu_nodes_per_ray"], 2), "gpu_tris_per_ray": round(k["gpu_tris_per_ray"], 2),
+                         "bytes_per_ray": round(k["bytes_per_ray"], 1), "achieved_GBs": round(k["achieved"], 1),
+                         "node_record_GBs": round(k["node_record_GBs"], 1)}
                   for name, k in kern.items()}
         stages["k_shade"] = {"ms_per_launch": round(tot["shade_s"] / launches * 1e3, 3)}
         dom = max(kern, key=lambda n: kern[n]["seconds"])
         k = kern[dom]
         traffic = hbm_traffic({"k_extend": "k_extend", "k_shadow": "k_shadow_any"}[dom], args)
+        avg_s = k["seconds"] / launches
+        valu = valu_profile()
         roofline = {"bound": "hbm", "achieved": round(k["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(k["achieved"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel": f"{dom} ({k['what']})",
                     "algorithmic_bytes_per_launch": round(k["bytes_per_ray"] * k["rays"] / launches),
-                    "avg_launch_ms": round(k["seconds"] / launches * 1e3, 3), "bytes_per_ray": round(k["bytes_per_ray"], 1)}
-    cpu = None
-    if not args.no_cpu:
+                    "avg_launch_ms": round(avg_s * 1e3, 3), "bytes_per_ray": round(k["bytes_per_ray"], 1),
+                    "counts_from": ("oracle FP64 walk of the same BVH over "
+                                    f"{counts['extend' if dom == 'k_extend' else 'shadow_any']['rays']} sampled rays"
+                                    if dom in orc else "GPU counting pass"),
+                    "dram_frac": round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+                    "valu": (valu or {}).get({"k_extend": "k_extend", "k_shadow": "k_shadow_any"}[dom]),
+                    "limiter": "VALU issue: divergent traversal loops keep under half of each wave's lanes active, "
+                               "with dependent node fetches; the DRAM rate stays below a third of peak"}
+    batched = None
+    if not args.no_batched and args.kernel == "wavefront":
         try:
-            cpu = cpu_baseline(sc, args.width, args.height, args.spp, args.max_bounce, 5, args.cpu_seconds)
-        except Exception as e:  # the baseline is informative; never hide the GPU number
-            cpu = {"error": str(e)}
+            batched = batched_leg(args.jobs)
+        except Exception as e:  # informative; never hide the frame number
+            batched = {"error": str(e)}
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -399,7 +535,9 @@ def main():
         "device_seconds": round(tot["dev_s"], 3),
         "roofline": roofline,
         "stages": stages,
+        "traversal_counts": counts,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

@@ -38,6 +38,7 @@ KEY_WF_POOL_PATHS = 15
 KEY_WF_SORT_KEYS = 16
 KEY_BVH_WIDTH = 17
 KEY_USE_GRAPHS = 18
+KEY_RAY_SAMPLE = 19
 KEY_MAXIMUM_DISTANCE = 101
 
 E_OK, E_NULL, E_ARGUMENT, E_STATE, E_DEVICE, E_MEMORY, E_STOPPED, E_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6, -7
@@ -136,6 +137,8 @@ ENTRY_POINTS = [
     ("dfr_get_stats", c_int32, [_c_void_p, P(Stats)]),
     ("dfr_trace_rays", c_int32, [_c_void_p, _c_void_p, c_int64, c_int32, _c_void_p, _c_void_p, _c_void_p, c_int32]),
     ("dfr_trace_rays_noalloc", c_int32, [_c_void_p, _c_void_p, c_int64, c_int32, _c_void_p, _c_void_p, _c_void_p]),
+    ("dfr_read_ray_sample", c_int32, [_c_void_p, _c_void_p, c_int64, _c_void_p]),
+    ("dfr_read_bvh", c_int32, [_c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
     ("dfr_cast_primary", c_int32, [_c_void_p, c_int32, P(c_float), P(c_int32), P(c_int64), P(c_float)]),
     ("dfr_scene_add", c_int32, [_c_void_p, P(SceneDesc)]),
     ("dfr_scene_remove", c_int32, [_c_void_p, c_int32]),

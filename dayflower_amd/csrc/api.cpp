@@ -657,10 +657,18 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
     const int poll = 4;
     const int max_iterations = 1 << 20;  // far above max_bounce * (work / capacity + 1)
     unsigned long long* trav = nullptr;
+    wf.sample = RaySample{};
     if (count_traversal) {
-        if (counters.alloc(64) != hipSuccess) return fail(DFR_E_MEMORY, "counters");
+        if (counters.alloc(128) != hipSuccess) return fail(DFR_E_MEMORY, "counters");
         trav = (unsigned long long*)counters.ptr + 4;
-        hipMemsetAsync(trav, 0, 32, stream);
+        hipMemsetAsync(trav, 0, 40, stream);
+        if (ray_sample_stride > 0) {
+            constexpr int cap = 1 << 20;  // records of 48 bytes
+            if (sample_buf.alloc((size_t)cap * 3 * sizeof(float4)) != hipSuccess) return fail(DFR_E_MEMORY, "ray sample");
+            wf.sample.rec = (float4*)sample_buf.ptr;
+            wf.sample.cap = cap;
+            wf.sample.stride = ray_sample_stride;
+        }
     }
     int iterations = 0;
     const DevScene* scenes = (const DevScene*)scene_table.ptr;
@@ -724,12 +732,14 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
         prof_stats.extend_launches = iterations;
     }
     if (trav) {
-        unsigned long long v[4];
-        hipMemcpy(v, trav, 32, hipMemcpyDeviceToHost);
+        unsigned long long v[5];
+        hipMemcpy(v, trav, 40, hipMemcpyDeviceToHost);
         prof_stats.extend_nodes = (int64_t)v[0];
         prof_stats.extend_tris = (int64_t)v[1];
         prof_stats.shadow_nodes = (int64_t)v[2];
         prof_stats.shadow_tris = (int64_t)v[3];
+        sample_count = wf.sample.rec ? std::min<int64_t>((int64_t)v[4], wf.sample.cap) : 0;
+        wf.sample = RaySample{};
     }
     prof_stats.extend_rays = host.bounces;
     prof_stats.shadow_rays = host.rays_closest + host.rays_any - host.bounces;
@@ -872,6 +882,9 @@ int32_t dfr_set_i32(dfr_ctx* h, int32_t key, int32_t v) {
             if (v != 0 && v != 1) return c->fail(DFR_E_ARGUMENT, "sort is 0 (off) or 1 (on)");
             c->wf_params.sort_keys = v; break;
         case DFR_KEY_USE_GRAPHS: c->use_graphs = v != 0; break;
+        case DFR_KEY_RAY_SAMPLE:
+            if (v < 0) return c->fail(DFR_E_ARGUMENT, "ray sample stride must be >= 0");
+            c->ray_sample_stride = v; break;
         default: return c->fail(DFR_E_ARGUMENT, "unknown integer key");
     }
     return DFR_OK;
@@ -899,6 +912,7 @@ int32_t dfr_get_i32(const dfr_ctx* h, int32_t key, int32_t* v) {
         case DFR_KEY_WF_SORT_KEYS: *v = c->wf_params.sort_keys; break;
         case DFR_KEY_BVH_WIDTH: *v = BVH_WIDTH; break;
         case DFR_KEY_USE_GRAPHS: *v = c->use_graphs; break;
+        case DFR_KEY_RAY_SAMPLE: *v = c->ray_sample_stride; break;
         default: return DFR_E_ARGUMENT;
     }
     return DFR_OK;
@@ -975,7 +989,7 @@ int32_t dfr_render(dfr_ctx* h, int32_t passes) {
     DevJob job = c->main_job(spp, c->render_pass * c->spp_per_pass);
     int rc = c->upload_jobs({job});
     if (rc) return rc;
-    if (c->counters.alloc(64) != hipSuccess) return c->fail(DFR_E_MEMORY, "counters");
+    if (c->counters.alloc(128) != hipSuccess) return c->fail(DFR_E_MEMORY, "counters");
     hipMemsetAsync(c->counters.ptr, 0, 64, c->stream);
     hipEventRecord(c->ev0, c->stream);
     dfr_stats st{};
@@ -1129,6 +1143,41 @@ int32_t dfr_trace_rays(dfr_ctx* h, const float* rays, int64_t n, int32_t any_hit
     c->stats = dfr_stats{};
     c->stats.seconds = ms * 1e-3;
     if (any_hit) c->stats.rays_any = n; else c->stats.rays_closest = n;
+    return DFR_OK;
+}
+
+int32_t dfr_read_ray_sample(dfr_ctx* h, float* records, int64_t max_records, int64_t* n_records) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (!n_records) return c->fail(DFR_E_NULL, "n_records == null");
+    const int64_t n = records ? std::min(max_records, c->sample_count) : c->sample_count;
+    *n_records = n;
+    if (!records || n <= 0) return DFR_OK;
+    const hipError_t e = hipMemcpy(records, c->sample_buf.ptr, (size_t)n * 3 * sizeof(float4), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return c->hip_fail(e, "ray sample readback");
+    return DFR_OK;
+}
+
+int32_t dfr_read_bvh(dfr_ctx* h, float* nodes, int64_t* n_nodes, float* tris, int64_t* n_tris) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (!n_nodes || !n_tris) return c->fail(DFR_E_NULL, "count pointer == null");
+    if (c->main_scene < 0 || !c->scenes[c->main_scene]) return c->fail(DFR_E_STATE, "setup before reading the BVH");
+    const CompiledScene& cs = *c->scenes[c->main_scene];
+    if (BVH_WIDTH != 4) return c->fail(DFR_E_UNSUPPORTED, "the BVH export describes four-wide nodes");
+    const int64_t want_nodes = *n_nodes, want_tris = *n_tris;
+    *n_nodes = cs.n_nodes;
+    *n_tris = cs.n_tris;
+    hipError_t e = hipSuccess;
+    if (nodes && cs.n_nodes > 0) {
+        if (want_nodes < cs.n_nodes) return c->fail(DFR_E_ARGUMENT, "node buffer too small");
+        e = hipMemcpy(nodes, cs.nodes.ptr, (size_t)cs.n_nodes * 128, hipMemcpyDeviceToHost);
+    }
+    if (e == hipSuccess && tris && cs.n_tris > 0) {
+        if (want_tris < cs.n_tris) return c->fail(DFR_E_ARGUMENT, "triangle buffer too small");
+        e = hipMemcpy(tris, cs.tris.ptr, (size_t)cs.n_tris * 48, hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) return c->hip_fail(e, "BVH readback");
     return DFR_OK;
 }
 

@@ -96,6 +96,9 @@ struct Ctx {
     int spp_per_pass = 1;
     int profile = 0;           // time kernel launches with HIP events
     int count_traversal = 0;   // count BVH visits of the extend kernel
+    int ray_sample_stride = 0; // with count_traversal: record every n-th traced ray (0: none)
+    DeviceBuffer sample_buf;   // the recorded rays (RaySample records)
+    int64_t sample_count = 0;  // records of the last counting run
     float camera[DFR_CAMERA_LENGTH] = {0};
 
     // scene state

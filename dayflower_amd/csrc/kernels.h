@@ -69,6 +69,17 @@ inline int64_t wavefront_stack_threads(int capacity) { return 2 * (int64_t)capac
 // entries per thread (32 keys x capacity / 16384-entry tiles <= 1024 x 128).
 constexpr int MAX_POOL_PATHS = 1 << 26;
 
+// With traversal counting: every `stride`-th traced ray that enters the BVH
+// (extension and any-hit shadow queues) is recorded as three float4 --
+// (origin, far limit at BVH entry), (direction, 1 for any hit else 0 as int
+// bits), (node visits, triangle tests, hit primitive, hit triangle as int
+// bits) -- in up to `cap` records; trav[4] counts the records claimed.
+struct RaySample {
+    float4* rec = nullptr;
+    int cap = 0;
+    int stride = 1;
+};
+
 struct WavefrontBuffers {
     PathSoA a, b;
     float4* hit;       // t, prim, tri (int bits)
@@ -80,6 +91,7 @@ struct WavefrontBuffers {
     int capacity;      // path pool size
     Counters* ctr;
     int* job_done;     // optional: paths finished per job (serving)
+    RaySample sample;  // optional: traced-ray sample of counting runs
 };
 
 // Launch helpers (kernels.hip).  `scenes` points to a device array of
@@ -122,7 +134,7 @@ struct WavefrontParams {
 struct IterProfile {
     hipEvent_t ev[6];
     bool timing;
-    unsigned long long* trav;  // [extend nodes, extend triangles, shadow nodes, shadow triangles] or null
+    unsigned long long* trav;  // [extend nodes, extend triangles, shadow nodes, shadow triangles, ray samples] or null
 };
 
 void wavefront_init(WavefrontBuffers& w, const DevJob* jobs, int n_jobs, int64_t work_total, hipStream_t s);

@@ -672,7 +672,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                                                          const Counters* __restrict__ ctr, int* __restrict__ overflow,
                                                          unsigned long long* __restrict__ trav,
                                                          const DevScene* __restrict__ uscene, int32_t* __restrict__ keys,
-                                                         int32_t* __restrict__ bhist, int tiles) {
+                                                         int32_t* __restrict__ bhist, int tiles, RaySample smp) {
     __shared__ int lds[SHORT * BLOCK];
     __shared__ int next;
     __shared__ int s_hist[NSORT_KEYS];
@@ -698,6 +698,10 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
     float tbest = FLT_MAX;
     int prim = -1, tri = -1;
     uint32_t cur = DONE, leaf = 0;
+    // with COUNT: this ray's visits and the far limit it entered the BVH with
+    int ray_nodes = 0, ray_tris = 0;
+    float t_entry = 0.0f;
+    bool entered = false;
     const float4* nodes = UNI ? uscene->nodes : nullptr;
     const float4* tris = UNI ? uscene->tris : nullptr;
     PlaneSel ps{0u, 0u, 0u};
@@ -707,6 +711,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
             const int i = base + e;
             fresh = false;
             prim = -1; tri = -1; leaf = 0; st.sp = 0; cur = DONE;
+            if (COUNT) { ray_nodes = 0; ray_tris = 0; entered = false; }
             // the ray, the analytic primitives, then the BVH root
             // (the scene-uniform kernel skips the job -> scene hops)
             auto setup = [&](const float4 o4, const float4 d4, uint32_t job) {
@@ -721,6 +726,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                 if (!UNI) { nodes = sc.nodes; tris = sc.tris; }
                 ps = plane_sel(r);
                 cur = sc.has_bvh && !(SHADOW && prim >= 0) ? 0u : DONE;
+                if (COUNT) { t_entry = tbest; entered = cur == 0u; }
             };
             if (SHADOW) {
                 const float4 o4 = sh.o[i];
@@ -747,7 +753,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
         }
         if (active && (cur != DONE || leaf != 0)) {
             while (cur < DONE) {
-                if (COUNT) vc.nodes++;
+                if (COUNT) { vc.nodes++; ray_nodes++; }
                 cur = visit_node<SHORT, !SHADOW>(nodes, cur, r, ps, T_MIN, tbest, st);
                 if ((cur & LEAF_FLAG_DEV) && leaf == 0) {
                     leaf = cur;
@@ -759,7 +765,7 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                 // one triangle per round: the leaf reference steps to its next
                 // triangle ((first + 1) << 3 | (count - 2) = reference + 7)
                 const uint32_t first = (leaf & 0x7fffffffu) >> 3;
-                if (COUNT) vc.tris++;
+                if (COUNT) { vc.tris++; ray_tris++; }
                 float4 a, b, c;
                 load_tri(tris, first, a, b, c);
                 const float t = tri_t(f4v(a), f4v(b), f4v(c), r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, T_MIN, tbest);
@@ -781,6 +787,16 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
         const bool finished = active && cur == DONE && leaf == 0;
         if (finished) {
             const int i = base + e;
+            if (COUNT && smp.rec && entered && i % smp.stride == 0) {
+                // every stride-th ray that entered the BVH, for the count check
+                const unsigned long long k = atomicAdd(&trav[4], 1ull);
+                if (k < (unsigned long long)smp.cap) {
+                    smp.rec[3 * k] = make_float4(r.o.x, r.o.y, r.o.z, t_entry);
+                    smp.rec[3 * k + 1] = make_float4(r.d.x, r.d.y, r.d.z, __int_as_float(SHADOW ? 1 : 0));
+                    smp.rec[3 * k + 2] = make_float4(__int_as_float(ray_nodes), __int_as_float(ray_tris),
+                                                     __int_as_float(prim), __int_as_float(tri));
+                }
+            }
             if (SHADOW) {
                 if (prim < 0) {
                     const float4 c = sh.c[i];
@@ -1101,14 +1117,15 @@ void wavefront_init(WavefrontBuffers& w, const DevJob* jobs, int n_jobs, int64_t
 template <int B, bool COUNT, bool SHADOW>
 static void launch_trace_queue(int grid, const DevScene* scenes, const DevJob* jobs, float4* film, WavefrontBuffers& w,
                                const WavefrontParams& p, unsigned long long* trav, hipStream_t s) {
+    const RaySample smp = COUNT ? w.sample : RaySample{};
     if (BVH_WIDTH == 4 && p.uscene)
         hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, SHADOW ? SHADOW_RPL : EXTEND_RPL, COUNT, SHADOW, true>), dim3(grid), dim3(B), 0, s,
                            scenes, jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav, p.uscene,
-                           SHADOW || !p.sort_keys ? nullptr : w.keys, w.bhist, (w.capacity + SORT_TILE - 1) / SORT_TILE);
+                           SHADOW || !p.sort_keys ? nullptr : w.keys, w.bhist, (w.capacity + SORT_TILE - 1) / SORT_TILE, smp);
     else
         hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, SHADOW ? SHADOW_RPL : EXTEND_RPL, COUNT, SHADOW, false>), dim3(grid), dim3(B), 0, s,
                            scenes, jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav, p.uscene,
-                           SHADOW || !p.sort_keys ? nullptr : w.keys, w.bhist, (w.capacity + SORT_TILE - 1) / SORT_TILE);
+                           SHADOW || !p.sort_keys ? nullptr : w.keys, w.bhist, (w.capacity + SORT_TILE - 1) / SORT_TILE, smp);
 }
 
 __global__ void k_zero_i32(int32_t* __restrict__ p, int n) {

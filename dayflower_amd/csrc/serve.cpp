@@ -64,7 +64,7 @@ int batch_render_impl(Ctx* c, const dfr_job* jobs, int n, int mode, float* const
     hipEventRecord(c->ev0, c->stream);
     if (mode == DFR_BATCH_PLAIN) {
         // the plain per-job kernel: one launch per job per pass, jobs in order
-        if (c->counters.alloc(64) != hipSuccess) return c->fail(DFR_E_MEMORY, "counters");
+        if (c->counters.alloc(128) != hipSuccess) return c->fail(DFR_E_MEMORY, "counters");
         hipMemsetAsync(c->counters.ptr, 0, 64, c->stream);
         int64_t max_threads = 0;
         for (int j = 0; j < n; j++) max_threads = std::max<int64_t>(max_threads, ((int64_t)jobs[j].width * jobs[j].height + 255) / 256 * 256);

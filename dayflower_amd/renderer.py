@@ -192,8 +192,34 @@ class GPURenderer:
     def setProfile(self, on):
         self._seti(abi.KEY_PROFILE, int(bool(on)))
 
-    def setCountTraversal(self, on):
+    def setCountTraversal(self, on, sample_stride=0):
+        """Count BVH node visits and triangle tests of the next renders; with
+        `sample_stride` > 0 also record every n-th traced ray (ray_sample())."""
         self._seti(abi.KEY_COUNT_TRAVERSAL, int(bool(on)))
+        self._seti(abi.KEY_RAY_SAMPLE, int(sample_stride) if on else 0)
+
+    def ray_sample(self):
+        """Rays recorded by the last counting render: (origin (n,3), far limit
+        at BVH entry (n,), direction (n,3), any-hit flag (n,), node visits,
+        triangle tests, hit primitive, hit triangle (n,) each)."""
+        n = ctypes.c_int64()
+        self._c(self.lib.dfr_read_ray_sample(self.ctx, None, 0, ctypes.byref(n)))
+        rec = np.zeros((n.value, 12), np.float32)
+        self._c(self.lib.dfr_read_ray_sample(self.ctx, rec.ctypes.data, n.value, ctypes.byref(n)))
+        ints = rec.view(np.int32)
+        return {"o": rec[:, 0:3], "tmax": rec[:, 3], "d": rec[:, 4:7], "any": ints[:, 7], "nodes": ints[:, 8],
+                "tris": ints[:, 9], "prim": ints[:, 10], "tri": ints[:, 11]}
+
+    def bvh(self):
+        """The compiled four-wide BVH: nodes (n, 32) float32 and triangles in
+        BVH order (m, 12) float32 (layouts in dfr_read_bvh's header comment)."""
+        nn, nt = ctypes.c_int64(0), ctypes.c_int64(0)
+        self._c(self.lib.dfr_read_bvh(self.ctx, None, ctypes.byref(nn), None, ctypes.byref(nt)))
+        nodes = np.zeros((nn.value, 32), np.float32)
+        tris = np.zeros((nt.value, 12), np.float32)
+        self._c(self.lib.dfr_read_bvh(self.ctx, nodes.ctypes.data, ctypes.byref(nn), tris.ctypes.data,
+                                      ctypes.byref(nt)))
+        return nodes, tris
 
     def setWavefront(self, block=None, pool_paths=None, sort_keys=None):
         """Wavefront engine configuration (the values dfr_autotune picks)."""

@@ -238,6 +238,7 @@ typedef struct dfr_ctx dfr_ctx;
 #define DFR_KEY_WF_SORT_KEYS           16  /* 1: shade in (material, light type) order, 0: queue order */
 #define DFR_KEY_BVH_WIDTH              17  /* read-only: children per BVH node (2 or 4) */
 #define DFR_KEY_USE_GRAPHS             18  /* 1 (default): replay captured HIP graphs of 4 wavefront iterations */
+#define DFR_KEY_RAY_SAMPLE             19  /* n > 0 with COUNT_TRAVERSAL: record every n-th traced ray */
 #define DFR_KEY_MAXIMUM_DISTANCE      101  /* float, default 20                */
 
 /* new GPURenderer(observer): M/renderer/gpu/GPURenderer.java:55-57 with the
@@ -320,6 +321,27 @@ typedef struct dfr_stats {
     int64_t  shadow_tris;
 } dfr_stats;
 int32_t dfr_get_stats(const dfr_ctx* ctx, dfr_stats* stats);
+
+/* Measurement hooks (no counterpart in the reference): the rays a counting
+ * render recorded and the compiled BVH, so that an independent traversal can
+ * recount the node visits and triangle tests the roofline prices.
+ *
+ * dfr_read_ray_sample: records of 12 floats -- origin.xyz, far limit at BVH
+ * entry, direction.xyz, kind (int bits: 0 closest hit, 1 any hit), node
+ * visits, triangle tests, hit primitive, hit triangle (int bits) -- of the
+ * last render with DFR_KEY_COUNT_TRAVERSAL and DFR_KEY_RAY_SAMPLE set (at
+ * most 2^20).  With records NULL only *n_records is set.
+ *
+ * dfr_read_bvh: the main scene's four-wide BVH, 32 floats per node (lo.x[4],
+ * hi.x[4], lo.y[4], hi.y[4], lo.z[4], hi.z[4], child references[4] as uint32
+ * bits, 4 unused), and its triangles in BVH order, 12 floats each (a.xyz,
+ * original triangle id as int bits, b.xyz, primitive as int bits, c.xyz, 0).
+ * A child reference is a node index, 0xffffffff (empty slot) or a leaf:
+ * bit 31 set, bits 3..30 the first triangle, bits 0..2 count - 1.  On entry
+ * *n_nodes / *n_tris give the buffers' capacities; on return the counts.
+ * NULL buffers query the counts only. */
+int32_t dfr_read_ray_sample(dfr_ctx* ctx, float* records, int64_t max_records, int64_t* n_records);
+int32_t dfr_read_bvh(dfr_ctx* ctx, float* nodes, int64_t* n_nodes, float* tris, int64_t* n_tris);
 
 /* ------------------------------------------------------------------------ */
 /* Ray queries (ray casting / AO hit parity, BVH bench)                      */

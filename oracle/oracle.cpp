@@ -1860,6 +1860,101 @@ int32_t orc_trace_f64(const orc_scene* h, const float* rays, int64_t n, int32_t 
     return 0;
 }
 
+// An independent FP64 walk of the product's four-wide BVH (the layout of
+// dfr_read_bvh) for the node-visit and triangle-test counts the roofline
+// prices: depth first, closest-hit queries enter the children nearest first
+// (the others pushed farthest first), any-hit queries in slot order and stop
+// at the first hit; every triangle of a reached leaf is tested.  Rays are the
+// 12-float records of dfr_read_ray_sample (origin, far limit, direction,
+// kind); tmin is 0.001.  Outputs per ray: node visits, triangle tests, the
+// hit's original triangle id (-1 on a miss) and its distance.
+int32_t orc_bvh4_walk(const float* nodes, int64_t n_nodes, const float* tris, int64_t n_tris, const float* rays,
+                      int64_t n, int32_t threads, int64_t* node_visits, int64_t* tri_tests, int64_t* hit_tri,
+                      double* hit_t) {
+    if (!nodes || !tris || !rays || n_nodes <= 0) return -1;
+    const uint32_t* refs = reinterpret_cast<const uint32_t*>(nodes);
+    parallelFor(n, threads, [&](int64_t i) {
+        const float* r = rays + 12 * i;
+        const D3 o{r[0], r[1], r[2]}, d{r[4], r[5], r[6]};
+        uint32_t kind;
+        std::memcpy(&kind, &r[7], 4);
+        const bool any = kind == 1;
+        const double oo[3] = {o.x, o.y, o.z};
+        double inv[3];
+        for (int a = 0; a < 3; a++) {
+            double v = a == 0 ? d.x : (a == 1 ? d.y : d.z);
+            if (std::fabs(v) < 1e-20) v = std::copysign(1e-20, v);
+            inv[a] = 1.0 / v;
+        }
+        const double tmin = 0.001;
+        double tmax = r[3];
+        int64_t nv = 0, nt = 0, best = -1;
+        double tbest = INFINITY;
+        std::vector<uint32_t> stack;
+        stack.reserve(64);
+        uint32_t cur = 0;  // the root
+        bool done = false;
+        for (;;) {
+            if (!(cur & 0x80000000u)) {
+                if ((int64_t)cur >= n_nodes) break;  // malformed reference
+                nv++;
+                const float* nd = nodes + 32 * (size_t)cur;
+                double key[4];
+                uint32_t ref[4];
+                int m = 0;
+                for (int c = 0; c < 4; c++) {
+                    const uint32_t rc = refs[32 * (size_t)cur + 24 + c];
+                    if (rc == 0xffffffffu) continue;
+                    double tn = tmin, tf = tmax;
+                    for (int a = 0; a < 3; a++) {
+                        const double t1 = ((double)nd[8 * a + c] - oo[a]) * inv[a];
+                        const double t2 = ((double)nd[8 * a + 4 + c] - oo[a]) * inv[a];
+                        tn = std::max(tn, std::min(t1, t2));
+                        tf = std::min(tf, std::max(t1, t2));
+                    }
+                    if (tn <= tf) { key[m] = tn; ref[m] = rc; m++; }
+                }
+                if (!any) {  // nearest first: insertion sort by entry distance
+                    for (int a = 1; a < m; a++)
+                        for (int b = a; b > 0 && key[b] < key[b - 1]; b--) {
+                            std::swap(key[b], key[b - 1]);
+                            std::swap(ref[b], ref[b - 1]);
+                        }
+                    for (int a = m - 1; a >= 1; a--) stack.push_back(ref[a]);
+                } else {
+                    for (int a = 1; a < m; a++) stack.push_back(ref[a]);
+                }
+                if (m > 0) { cur = ref[0]; continue; }
+            } else {
+                const uint32_t first = (cur & 0x7fffffffu) >> 3, count = (cur & 7u) + 1u;
+                for (uint32_t k = 0; k < count && first + k < (uint64_t)n_tris; k++) {
+                    nt++;
+                    const float* t = tris + 12 * (size_t)(first + k);
+                    Tri tr{};
+                    tr.a = V3(t[0], t[1], t[2]); tr.b = V3(t[4], t[5], t[6]); tr.c = V3(t[8], t[9], t[10]);
+                    const double th = triangleT64(tr, o, d, tmin, std::min(tmax, tbest) * (1.0 + 1e-12));
+                    if (!std::isnan(th) && th < tbest && th <= tmax) {
+                        int32_t id;
+                        std::memcpy(&id, &t[3], 4);
+                        tbest = th; best = id;
+                        if (any) { done = true; break; }
+                        tmax = th;
+                    }
+                }
+                if (done) break;
+            }
+            if (stack.empty()) break;
+            cur = stack.back();
+            stack.pop_back();
+        }
+        node_visits[i] = nv;
+        tri_tests[i] = nt;
+        if (hit_tri) hit_tri[i] = best;
+        if (hit_t) hit_t[i] = tbest;
+    });
+    return 0;
+}
+
 int32_t orc_primary_rays(const orc_scene* h, int32_t width, int32_t height, uint64_t seed, int32_t sample, float* rays) {
     if (!h) return -1;
     float cam[DFR_CAMERA_LENGTH];

@@ -71,6 +71,12 @@ int32_t orc_trace_f32(const orc_scene* scene, const float* rays, int64_t n, int3
 int32_t orc_trace_f64(const orc_scene* scene, const float* rays, int64_t n, int32_t any_hit,
                       int32_t threads, int32_t* prim, int64_t* tri, double* t, int64_t* nodes,
                       int64_t* tris);
+/* FP64 walk of the product's four-wide BVH (dfr_read_bvh layout) over the
+ * 12-float ray records of dfr_read_ray_sample: node visits, triangle tests,
+ * hit triangle id (-1: miss) and distance per ray (the last two may be NULL). */
+int32_t orc_bvh4_walk(const float* nodes, int64_t n_nodes, const float* tris, int64_t n_tris, const float* rays,
+                      int64_t n, int32_t threads, int64_t* node_visits, int64_t* tri_tests, int64_t* hit_tri,
+                      double* hit_t);
 int32_t orc_primary_rays(const orc_scene* scene, int32_t width, int32_t height, uint64_t seed,
                          int32_t sample, float* rays);
 int32_t orc_camera_ray(const float* camera19, float ix, float iy, float px, float py, float* out6);

@@ -48,6 +48,7 @@ def load():
     lib.orc_trace_f32.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
     lib.orc_trace_f64.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp, vp, vp]
     lib.orc_primary_rays.argtypes = [vp, i32, i32, ctypes.c_uint64, i32, vp]
+    lib.orc_bvh4_walk.argtypes = [vp, i64, vp, i64, vp, i64, i32, vp, vp, vp, vp]
     lib.orc_camera_ray.argtypes = [vp, f, f, f, f, vp]
     lib.orc_lens_ray.argtypes = [vp, f, f, f, f, f, f, vp]
     lib.orc_set_literal_tmin.argtypes = [i32]
@@ -146,6 +147,24 @@ class OracleScene:
         rays = np.zeros((width * height, 8), np.float32)
         self.lib.orc_primary_rays(self.h, width, height, seed, sample, _ptr(rays))
         return rays
+
+
+def bvh4_walk(nodes, tris, records, threads=THREADS):
+    """The oracle's FP64 walk of the product BVH over ray-sample records:
+    (node visits, triangle tests, hit triangle id, hit distance) per ray."""
+    nodes = np.ascontiguousarray(nodes, np.float32)
+    tris = np.ascontiguousarray(tris, np.float32)
+    rec = np.ascontiguousarray(records, np.float32).reshape(-1, 12)
+    n = rec.shape[0]
+    nv = np.zeros(n, np.int64)
+    nt = np.zeros(n, np.int64)
+    hit = np.zeros(n, np.int64)
+    t = np.zeros(n, np.float64)
+    rc = load().orc_bvh4_walk(_ptr(nodes), nodes.shape[0], _ptr(tris), tris.shape[0], _ptr(rec), n, threads,
+                              _ptr(nv), _ptr(nt), _ptr(hit), _ptr(t))
+    if rc != 0:
+        raise RuntimeError("orc_bvh4_walk failed")
+    return nv, nt, hit, t
 
 
 def film_agreement(g, n_g, s, q, c):
