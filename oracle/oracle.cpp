@@ -1777,6 +1777,14 @@ orc_scene* orc_scene_create(const dfr_scene_desc* d) {
 
 void orc_scene_destroy(orc_scene* s) { delete s; }
 
+// updateCamera: the 19-float camera record later renders use (resolution
+// entries are replaced by each render's width and height).
+int32_t orc_scene_set_camera(orc_scene* s, const float* camera) {
+    if (!s || !camera) return -1;
+    std::memcpy(s->s.cam, camera, sizeof(s->s.cam));
+    return 0;
+}
+
 int32_t orc_render(const orc_scene* h, const orc_params* prm, double* sum, double* sumsq, int32_t* count) {
     if (!h || !prm) return -1;
     const Scene& sc = h->s;

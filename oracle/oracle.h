@@ -58,6 +58,7 @@ typedef struct orc_params {
 
 orc_scene* orc_scene_create(const dfr_scene_desc* desc);
 void orc_scene_destroy(orc_scene* scene);
+int32_t orc_scene_set_camera(orc_scene* scene, const float* camera19);
 
 /* Per-pixel sum and sum of squares of the accepted samples and their count
  * (unfiltered film, one bin per pixel).  Arrays are width*height*3 doubles

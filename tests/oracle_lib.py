@@ -44,6 +44,7 @@ def load():
     lib.orc_scene_create.restype = vp
     lib.orc_scene_create.argtypes = [P(abi.SceneDesc)]
     lib.orc_scene_destroy.argtypes = [vp]
+    lib.orc_scene_set_camera.argtypes = [vp, vp]
     lib.orc_render.argtypes = [vp, P(Params), vp, vp, vp]
     lib.orc_trace_f32.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
     lib.orc_trace_f64.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp, vp, vp]
@@ -91,6 +92,11 @@ class OracleScene:
         self.h = self.lib.orc_scene_create(ctypes.byref(self.desc))
         if not self.h:
             raise RuntimeError("orc_scene_create failed")
+
+    def set_camera(self, camera):
+        """The camera record later renders use (like updateCamera)."""
+        cam = np.ascontiguousarray(camera, np.float32)
+        self.lib.orc_scene_set_camera(self.h, _ptr(cam))
 
     def close(self):
         if self.h:

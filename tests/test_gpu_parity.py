@@ -515,3 +515,111 @@ def test_traversal_counts_match_the_oracle_walk():
     # the sampled rays' GPU counts are a fair sample of the whole run's
     ext = smp["any"] == 0
     assert abs(smp["nodes"][ext].mean() / (st["extend_nodes"] / st["extend_rays"]) - 1.0) < 0.1
+
+
+def test_benched_frame_full_size_same_streams():
+    """The benched frame itself: the 1M-triangle displaced icosphere at
+    1920x1080 with the default 64M-path pool (4096 sort tiles), 2 spp, against
+    the oracle on a 320x320 window at the image centre on the same streams.
+    Bars: >= 99.5 % of pixels equal (rtol 1e-3; measured 99.85 % at 64 spp on
+    a 745x745 window) and the window mean within 0.1 %; every pixel counts 2
+    samples."""
+    from tests.oracle_lib import OracleScene
+    w, h, spp = 1920, 1080, 2
+    sc = S.icosphere_scene(1_000_000, seed=0x5EED, width=w, height=h, with_perez=True)
+    r = _renderer()
+    r.setScene(sc)
+    r.setSeed(0x5EED)
+    r.setMaximumBounce(8)
+    r.setMinimumBounceRussianRoulette(5)
+    r.setup()
+    assert r.getWavefront()["pool_paths"] == 1 << 26
+    r.render(spp)
+    g, cnt = r.film(w, h)
+    r.dispose()
+    assert (cnt == spp).all()
+    x0, y0, side = w // 2 - 160, h // 2 - 160, 320
+    o = OracleScene(sc)
+    s, _, c = o.render(w, h, spp, seed=0x5EED, max_bounce=8, min_rr=5, window=(x0, y0, x0 + side, y0 + side))
+    o.close()
+    ref = (s / np.maximum(c, 1)[..., None])[y0:y0 + side, x0:x0 + side]
+    gw = g[y0:y0 + side, x0:x0 + side]
+    px = np.isclose(gw, ref, rtol=1e-3, atol=1e-4).all(-1).mean()
+    rel = abs(gw.mean() - ref.mean()) / ref.mean()
+    print(f"full-size frame: window pixels equal {px:.5f}, mean rel diff {rel:.2e}", flush=True)
+    assert px >= 0.995
+    assert rel < 0.001
+
+
+def _jobs_vs_oracle(sc, jobs, films):
+    """(pixels equal, relative mean difference) of each job's film against the
+    oracle's render of the same streams (one oracle scene for all jobs)."""
+    from tests.oracle_lib import OracleScene
+    o = OracleScene(sc)
+    out = []
+    for job, film in zip(jobs, films):
+        o.set_camera(np.array(job.camera[:], np.float32))
+        s, _, c = o.render(job.width, job.height, job.spp, seed=job.seed, max_bounce=job.maximum_bounce,
+                           min_rr=job.minimum_bounce_rr)
+        ref = s / np.maximum(c, 1)[..., None]
+        px = np.isclose(film, ref, rtol=1e-3, atol=1e-4).all(-1).mean()
+        rel = abs(film.mean() - ref.mean()) / max(ref.mean(), 1e-12)
+        print(f"job {job.width}x{job.height} {job.spp} spp (scene {job.scene}): pixels equal {px:.5f}, "
+              f"mean rel diff {rel:.2e}", flush=True)
+        out.append((px, rel))
+    o.close()
+    return out
+
+
+def test_batched_jobs_match_the_oracle():
+    """The 64-job batched workload (4 mixed-material icospheres of 10k, 100k,
+    500k and 2M triangles; Matte, Glass, Metal and Disney patches): the first
+    two jobs of each scene, at their own resolutions with spp capped at 2,
+    rendered together in one packed batch; each job's film against the
+    oracle on the same streams.  Calibration by the oracle against its own
+    FMA-contracted build on the serving workload's jobs (rounding alone):
+    98.3-99.4 % of 2-spp pixels equal, means within 0.17 %; measured on
+    MI355X: 98.7-99.4 % and 2.2e-4..7.7e-4.  Bars: >= 98 % of pixels equal
+    (rtol 1e-3) and the mean within 0.3 %."""
+    from dayflower_amd import workloads as W
+    scenes = W.batched_scenes()
+    r = _renderer()
+    ids = [r.addScene(sc) for sc in scenes]
+    jobs = W.batched_jobs(ids)
+    pick = []
+    for sid in ids:
+        pick += [j for j in jobs if j.scene == sid][:2]
+    for j in pick:
+        j.spp = min(j.spp, 2)
+    films = r.batchRender(pick)
+    r.dispose()
+    assert sorted({j.scene for j in pick}) == sorted(ids)
+    for k, sid in enumerate(ids):
+        sel = [i for i, j in enumerate(pick) if j.scene == sid]
+        for px, rel in _jobs_vs_oracle(scenes[k], [pick[i] for i in sel], [films[i] for i in sel]):
+            assert px >= 0.98 and rel < 0.003
+
+
+def test_served_requests_match_the_oracle():
+    """The serving workload's first 20 requests (2M-triangle mixed-material
+    icosphere, cameras on a radius-3 sphere, resolutions 256^2..1024^2) with
+    spp capped at 2, submitted to the server together: every served film
+    against the oracle on the same streams.  Measured on MI355X: 98.2-99.5 %
+    of pixels equal, means within 1.2e-3.  Bars as for the batch."""
+    from dayflower_amd import workloads as W
+    from dayflower_amd.renderer import Server
+    sc = W.mixed_material_icosphere(2_000_000, 0x5EED)
+    r = _renderer()
+    sid = r.addScene(sc)
+    stream = W.request_stream(sid, duration=0.2)[:20]
+    assert len(stream) == 20
+    jobs = [j for _, j in stream]
+    for j in jobs:
+        j.spp = min(j.spp, 2)
+    srv = Server(r, hbm_budget_bytes=8 << 30, max_active=8, pool_paths=1 << 22)
+    ids = [srv.submit(j) for j in jobs]
+    films = [srv.wait(i) for i in ids]
+    srv.close()
+    r.dispose()
+    for px, rel in _jobs_vs_oracle(sc, jobs, films):
+        assert px >= 0.98 and rel < 0.003
