@@ -103,7 +103,7 @@ class Job(ctypes.Structure):
 
 class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", c_int64), ("max_active", c_int32), ("pool_paths", c_int32),
-                ("spp_quantum", c_int32), ("reserved", c_int32)]
+                ("spp_quantum", c_int32), ("slot_capacity", c_int32)]
 
 
 class ServerStats(ctypes.Structure):

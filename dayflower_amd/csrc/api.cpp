@@ -548,6 +548,7 @@ int Ctx::sync_scene_table() {
     if (e != hipSuccess) return hip_fail(e, "scene table upload");
     e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "scene table upload");
+    scene_gen++;
     return DFR_OK;
 }
 
@@ -586,26 +587,30 @@ int Ctx::ensure_film(int64_t pixels) {
     return DFR_OK;
 }
 
-int Ctx::ensure_wavefront(int capacity) {
-    // buffers only grow; a run may use fewer slots than are allocated
-    if (wf_capacity >= capacity && wf.ctr) { wf.capacity = capacity; return DFR_OK; }
+bool alloc_wavefront_buffers(WavefrontBuffers& wf, DeviceBuffer* mem, int capacity) {
     const size_t c = (size_t)capacity;
     bool ok = true;
     auto A = [&](DeviceBuffer& b, size_t bytes) { ok = ok && b.alloc(bytes) == hipSuccess; return b.ptr; };
-    wf.a.o = (float4*)A(wf_mem[0], c * 16); wf.a.d = (float4*)A(wf_mem[1], c * 16);
-    wf.a.beta = (float4*)A(wf_mem[2], c * 16); wf.a.info = (uint4*)A(wf_mem[3], c * 16);
-    wf.b.o = (float4*)A(wf_mem[4], c * 16); wf.b.d = (float4*)A(wf_mem[5], c * 16);
-    wf.b.beta = (float4*)A(wf_mem[6], c * 16); wf.b.info = (uint4*)A(wf_mem[7], c * 16);
-    wf.hit = (float4*)A(wf_mem[8], c * 16);
-    wf.perm = (int32_t*)A(wf_mem[9], c * 4);
-    wf.keys = (int32_t*)A(wf_mem[10], c * 4);
-    wf.bhist = (int32_t*)A(wf_mem[16], 32 * (c / 4096 + 2) * 4);  // tile histograms + key totals
-    wf.sh.o = (float4*)A(wf_mem[11], 2 * c * 16); wf.sh.d = (float4*)A(wf_mem[12], 2 * c * 16);
-    wf.sh.c = (float4*)A(wf_mem[13], 2 * c * 16); wf.sh.aux = (int4*)A(wf_mem[14], 2 * c * 16);
-    wf.ctr = (Counters*)A(wf_mem[15], sizeof(Counters));
-
-    if (!ok) { wf.ctr = nullptr; return fail(DFR_E_MEMORY, "wavefront buffers"); }
+    wf.a.o = (float4*)A(mem[0], c * 16); wf.a.d = (float4*)A(mem[1], c * 16);
+    wf.a.beta = (float4*)A(mem[2], c * 16); wf.a.info = (uint4*)A(mem[3], c * 16);
+    wf.b.o = (float4*)A(mem[4], c * 16); wf.b.d = (float4*)A(mem[5], c * 16);
+    wf.b.beta = (float4*)A(mem[6], c * 16); wf.b.info = (uint4*)A(mem[7], c * 16);
+    wf.hit = (float4*)A(mem[8], c * 16);
+    wf.perm = (int32_t*)A(mem[9], c * 4);
+    wf.keys = (int32_t*)A(mem[10], c * 4);
+    wf.bhist = (int32_t*)A(mem[16], 32 * (c / 4096 + 2) * 4);  // tile histograms + key totals
+    wf.sh.o = (float4*)A(mem[11], 2 * c * 16); wf.sh.d = (float4*)A(mem[12], 2 * c * 16);
+    wf.sh.c = (float4*)A(mem[13], 2 * c * 16); wf.sh.aux = (int4*)A(mem[14], 2 * c * 16);
+    wf.ctr = (Counters*)A(mem[15], sizeof(Counters));
+    if (!ok) { wf.ctr = nullptr; return false; }
     wf.capacity = capacity;
+    return true;
+}
+
+int Ctx::ensure_wavefront(int capacity) {
+    // buffers only grow; a run may use fewer slots than are allocated
+    if (wf_capacity >= capacity && wf.ctr) { wf.capacity = capacity; return DFR_OK; }
+    if (!alloc_wavefront_buffers(wf, wf_mem, capacity)) return fail(DFR_E_MEMORY, "wavefront buffers");
     wf_capacity = capacity;
     return DFR_OK;
 }

@@ -65,6 +65,10 @@ struct CompiledScene {
 };
 
 int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err);
+struct WavefrontBuffers;
+// Allocates (or grows) the 17 wavefront buffers of a path pool of `capacity`
+// paths into `mem` and points `wf` at them; false when an allocation fails.
+bool alloc_wavefront_buffers(WavefrontBuffers& wf, DeviceBuffer* mem, int capacity);
 int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, hipStream_t s, std::string& err);
 size_t wavefront_bytes(int capacity);
 
@@ -106,6 +110,8 @@ struct Ctx {
     bool has_scene = false, scene_dirty = false, film_dirty = true, is_setup = false, clearing = false;
     int render_pass = 0;
     std::vector<std::unique_ptr<CompiledScene>> scenes;
+    std::vector<int> scene_refs;  // per scene: requests of serving loops that render it
+    uint64_t scene_gen = 0;       // bumped whenever the device scene table changes
     int main_scene = -1;
     DeviceBuffer scene_table, job_buf, film, overflow, counters, image;
 

@@ -115,6 +115,15 @@ int noalloc_stack_entries();
 void launch_trace_rays_noalloc(const DevScene* scene, const float4* rays, int64_t n, int any_hit, int32_t* prim,
                                int64_t* tri, float* t, int* scratch_ptr, hipStream_t s);
 
+// One entry of a generation-order table (the serving loop, whose job slots
+// are reused and so not ordered by their work ranges): the job in `slot` owns
+// the work items from work_begin up to the next entry's work_begin.
+struct GenEntry {
+    int64_t work_begin;
+    int32_t slot;
+    int32_t pad;
+};
+
 struct WavefrontParams {
     int block;          // threads per block
     int sort_keys;      // 0: unsorted; 1: shade in (material, light type) order
@@ -127,6 +136,10 @@ struct WavefrontParams {
     // material or a spot or directional light: launch the shade kernel
     // built with their code
     bool ext_features = false;
+    // generation-order table, or null when the job table itself is ordered
+    // by work_begin (batches)
+    const GenEntry* order = nullptr;
+    int n_order = 0;
 };
 
 // Optional instrumentation of one wavefront iteration: HIP events around the

@@ -580,7 +580,20 @@ DEV uint32_t nee_light_type(const DevScene& sc, const Rng& rng, uint32_t bounce)
     return t == DFR_LIGHT_PEREZ ? 2u : (t == DFR_LIGHT_DIFFUSE_AREA ? 0u : 1u);  // delta lights share class 1
 }
 
+// The job of work item `item` through a generation-order table: entries in
+// increasing work_begin, each naming the job slot that owns the range from its
+// work_begin to the next entry's.
+DEV int find_job_ordered(const GenEntry* order, int n_order, int64_t item) {
+    int lo = 0, hi = n_order - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (order[mid].work_begin <= item) lo = mid; else hi = mid - 1;
+    }
+    return order[lo].slot;
+}
+
 __global__ void k_generate(const DevScene* __restrict__ scenes, const DevJob* __restrict__ jobs, int n_jobs,
+                           const GenEntry* __restrict__ order, int n_order,
                            PathSoA out, Counters* __restrict__ ctr, float4* __restrict__ film, int capacity) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int base = ctr->n_next;
@@ -590,7 +603,7 @@ __global__ void k_generate(const DevScene* __restrict__ scenes, const DevJob* __
     int64_t take = room < avail ? room : avail;
     if (i >= take) return;
     int64_t item = next + i;
-    int j = find_job(jobs, n_jobs, item);
+    int j = order ? find_job_ordered(order, n_order, item) : find_job(jobs, n_jobs, item);
     const DevJob& job = jobs[j];
     int64_t local = item - job.work_begin;
     int npix = job.width * job.height;
@@ -1139,7 +1152,8 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     const int cap = w.capacity;
     const int grid256 = (cap + 255) / 256;
     // top up the queue with new camera paths, then commit the counters
-    hipLaunchKernelGGL(k_generate, dim3(grid256), dim3(256), 0, s, scenes, jobs, n_jobs, w.a, w.ctr, film, cap);
+    hipLaunchKernelGGL(k_generate, dim3(grid256), dim3(256), 0, s, scenes, jobs, n_jobs, p.order, p.n_order, w.a, w.ctr,
+                       film, cap);
     hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, s, w.ctr, cap);
     const bool timing = prof && prof->timing;
     const int use_perm = p.sort_keys > 0;

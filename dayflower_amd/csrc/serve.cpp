@@ -143,30 +143,42 @@ int autotune_impl(Ctx* c, const dfr_job* jobs, int n, int32_t* out) {
     const int blocks[] = {128, 256};
     const int caps[] = {1 << 20, 1 << 22, 1 << 24, 1 << 25, 1 << 26};
     const int sorts[] = {0, 1};
-    WavefrontConfig best = c->wf_params;
+    const WavefrontConfig original = c->wf_params;
+    WavefrontConfig best = original;
     double best_t = 1e30;
     for (int b : blocks)
         for (int cap : caps)
             for (int so : sorts) {
                 c->wf_params.block = b; c->wf_params.capacity = cap; c->wf_params.sort_keys = so;
-                // warm the allocation, then time
-                if ((rc = batch_render_impl(c, probe.data(), n, DFR_BATCH_PACKED, nullptr))) return rc;
-                if ((rc = batch_render_impl(c, probe.data(), n, DFR_BATCH_PACKED, nullptr))) return rc;
+                // warm the allocation, then time; a candidate whose buffers do
+                // not fit is skipped
+                rc = batch_render_impl(c, probe.data(), n, DFR_BATCH_PACKED, nullptr);
+                if (rc == DFR_E_MEMORY) continue;
+                if (!rc) rc = batch_render_impl(c, probe.data(), n, DFR_BATCH_PACKED, nullptr);
+                if (rc) { c->wf_params = best_t < 1e30 ? best : original; return rc; }
                 double t = c->stats.seconds;
                 if (t < best_t) { best_t = t; best = c->wf_params; }
             }
+    if (best_t >= 1e30) { c->wf_params = original; return c->fail(DFR_E_MEMORY, "no autotune candidate fits"); }
     c->wf_params = best;
     if (out) { out[0] = best.block; out[1] = best.capacity; out[2] = best.sort_keys; }
     return DFR_OK;
 }
 
+
 // ---------------------------------------------------------------------------
-// Serving loop with continuous batching.  A worker thread owns the context's
-// wavefront engine: it admits requests while their films fit the HBM budget
-// (and the active cap), appends them to the device job table (the generate
-// kernel pulls work from every admitted job in admission order), runs
-// wavefront iterations, and retires a request once all of its paths have
-// finished (per-job counters on the device), copying its film out.
+// Serving loop with continuous batching.  A worker thread owns a wavefront
+// engine of its own (path pool, queues, traversal overflow, stream).  Each
+// round it admits pending requests while their films fit the HBM budget and
+// a job slot is free (slots of retired requests are reused), publishes the
+// new work, and enqueues ROUND_ITERATIONS wavefront iterations followed by
+// asynchronous copies of the queue counters and the per-slot finished-path
+// counts.  While the device runs that round the worker reads the previous
+// round's counts: a request whose paths have all finished has its film
+// copied out asynchronously, and once that copy is done its film area and
+// slot are released.  The context's mutex is held only while a round is
+// enqueued (the scene table must not change under it), so the context's
+// other calls interleave with serving.
 // ---------------------------------------------------------------------------
 struct Arena {  // first-fit allocator over a film arena (units: float4)
     std::map<int64_t, int64_t> free_;  // offset -> size
@@ -197,12 +209,17 @@ struct Arena {  // first-fit allocator over a film arena (units: float4)
 
 struct Request {
     dfr_job job;
-    int state = 0;  // 0 pending, 1 active, 2 done, -1 rejected
+    int state = 0;  // 0 pending, 1 active, 3 copying its film out, 2 done, -1 rejected
     int slot = -1;
-    int64_t film_off = 0, pixels = 0, work = 0;
+    int64_t film_off = 0, pixels = 0, work = 0, work_begin = 0;
+    int admitted_round = -1;
+    float4* staging = nullptr;  // pinned host copy of the film while state 3
+    hipEvent_t copied = nullptr;
     std::chrono::steady_clock::time_point t_submit, t_done;
     std::vector<float> film;
 };
+
+constexpr int ROUND_ITERATIONS = 4;
 
 struct Server {
     Ctx* ctx = nullptr;
@@ -213,15 +230,30 @@ struct Server {
     bool stopping = false;
     std::deque<int64_t> pending;
     std::vector<std::unique_ptr<Request>> reqs;
-    std::vector<int64_t> active;
-    // device state
-    DeviceBuffer arena_mem, jobs_mem, done_mem;
+    std::vector<int64_t> active;     // admitted and not yet copied out, in admission order
+    std::vector<int64_t> copying;    // film copies in flight
+    // device state, owned by the server
+    int device = 0;
+    hipStream_t stream = nullptr;
+    WavefrontBuffers wf{};
+    DeviceBuffer wf_mem[17], overflow, arena_mem, jobs_mem, done_mem, order_mem, table_mem;
+    int overflow_depth = 0;
+    uint64_t table_gen = ~0ull;
     Arena arena;
-    int job_capacity = 1 << 16;
-    int n_slots = 0;
-    int slots_scene = -2;  // the scene of every admitted request (-2: none yet, -1: more than one)
+    int slot_capacity = 1 << 16;
+    int slot_hw = 0;                 // slots handed out so far (the job table's used length)
+    std::vector<int> free_slots;
     int64_t work_total = 0;
+    int64_t work_next_seen = 0;      // the device's work_next at the last counter read
     int64_t arena_pixels = 0;
+    // pinned host mirrors: two rounds in flight at most
+    int* done_host[2] = {nullptr, nullptr};
+    Counters* ctr_host[2] = {nullptr, nullptr};
+    int64_t* total_host[2] = {nullptr, nullptr};
+    int done_len[2] = {0, 0};
+    hipEvent_t round_ev[2] = {nullptr, nullptr};
+    std::vector<DevJob> job_upload[2];
+    std::vector<GenEntry> order_upload[2];
     dfr_server_stats stats{};
     std::string error;
     std::chrono::steady_clock::time_point t_busy0;
@@ -230,52 +262,97 @@ struct Server {
 
 static int64_t film_pixels(const dfr_job& j) { return (int64_t)j.width * j.height; }
 
+// Marks every unfinished request failed (device error), under S->m.
+static void fail_all(Server* S, const std::string& msg) {
+    S->error = msg;
+    S->stopping = true;
+    for (int64_t id : S->active) S->reqs[id]->state = -1;
+    for (int64_t id : S->copying) S->reqs[id]->state = -1;
+    for (int64_t id : S->pending) S->reqs[id]->state = -1;
+    S->active.clear(); S->copying.clear(); S->pending.clear();
+    S->done_cv.notify_all();
+}
+
+// Films whose copy has landed: means into the request, area and slot freed.
+static void finish_copies(Server* S, bool block) {
+    std::vector<int64_t> still;
+    for (int64_t id : S->copying) {
+        Request& r = *S->reqs[id];
+        hipError_t e = block ? hipEventSynchronize(r.copied) : hipEventQuery(r.copied);
+        if (e == hipErrorNotReady) { still.push_back(id); continue; }
+        r.film.resize(3 * r.pixels);
+        for (int64_t i = 0; i < r.pixels; i++) {
+            const float4 f = r.staging[i];
+            const float inv = f.w > 0.0f ? 1.0f / f.w : 0.0f;
+            r.film[3 * i] = f.x * inv; r.film[3 * i + 1] = f.y * inv; r.film[3 * i + 2] = f.z * inv;
+        }
+        hipHostFree(r.staging);
+        r.staging = nullptr;
+        hipEventDestroy(r.copied);
+        r.copied = nullptr;
+        std::lock_guard<std::mutex> l(S->m);
+        S->arena.release(r.film_off, r.pixels);
+        S->free_slots.push_back(r.slot);
+        r.t_done = std::chrono::steady_clock::now();
+        r.state = e == hipSuccess ? 2 : -1;
+        if (e == hipSuccess) { S->stats.completed++; S->stats.path_samples += r.work; }
+        S->done_cv.notify_all();
+    }
+    S->copying.swap(still);
+}
+
 static void server_loop(Server* S) {
     Ctx* c = S->ctx;
-    std::lock_guard<std::mutex> ctx_lock(c->mutex);  // the server owns the context while it runs
-    hipSetDevice(c->device);
-    if (c->ensure_wavefront(S->opts.pool_paths) || c->ensure_overflow(wavefront_stack_threads(S->opts.pool_paths))) {
-        std::lock_guard<std::mutex> l(S->m);
-        S->error = c->error;
-        S->stopping = true;
-        S->done_cv.notify_all();
-        return;
+    hipSetDevice(S->device);
+    WavefrontParams p{};
+    {
+        std::lock_guard<std::mutex> cl(c->mutex);
+        p.block = c->wf_params.block;
+        p.sort_keys = c->wf_params.sort_keys;
     }
-    c->wf.overflow = (int*)c->overflow.ptr;
-    c->wf.job_done = (int*)S->done_mem.ptr;
+    hipMemsetAsync(S->done_mem.ptr, 0, (size_t)S->slot_capacity * sizeof(int), S->stream);
     Counters zero{};
-    hipMemcpyAsync(c->wf.ctr, &zero, sizeof(Counters), hipMemcpyHostToDevice, c->stream);
-    hipMemsetAsync(S->done_mem.ptr, 0, (size_t)S->job_capacity * sizeof(int), c->stream);
-    WavefrontParams p{c->wf_params.block, c->wf_params.sort_keys, 0, c->any_area_lights()};
-    p.ext_features = c->any_ext_features();
-    std::vector<int> done_host;
-    Counters host{};
+    hipMemcpyAsync(S->wf.ctr, &zero, sizeof(Counters), hipMemcpyHostToDevice, S->stream);
+    hipStreamSynchronize(S->stream);
+    int round = 0;
+    bool prev_pending = false;  // the previous round's counters are on their way
     for (;;) {
+        const int cur = round & 1, prev = cur ^ 1;
+        // ---- admission --------------------------------------------------
+        std::vector<int64_t> admitted;
+        int scene_uniform = -2;  // the scene of every active request (-1: more than one)
         {
             std::unique_lock<std::mutex> l(S->m);
-            // admission: FIFO while the film fits the budget and the active cap
             while (!S->pending.empty()) {
                 Request& r = *S->reqs[S->pending.front()];
                 if ((int)S->active.size() >= S->opts.max_active) break;
-                if (S->n_slots >= S->job_capacity) { r.state = -1; S->stats.rejected++; S->pending.pop_front(); S->done_cv.notify_all(); continue; }
-                if (r.pixels > S->arena_pixels) { r.state = -1; S->stats.rejected++; S->pending.pop_front(); S->done_cv.notify_all(); continue; }
-                int64_t off = S->arena.alloc(r.pixels);
+                if (r.pixels > S->arena_pixels) {
+                    r.state = -1; S->stats.rejected++; S->pending.pop_front(); S->done_cv.notify_all(); continue;
+                }
+                if (S->free_slots.empty() && S->slot_hw >= S->slot_capacity) break;  // wait for a retirement
+                const int64_t off = S->arena.alloc(r.pixels);
                 if (off < 0) break;  // wait for films to be released
+                int slot;
+                if (!S->free_slots.empty()) { slot = S->free_slots.back(); S->free_slots.pop_back(); }
+                else slot = S->slot_hw++;
                 r.film_off = off;
-                r.slot = S->n_slots++;
+                r.slot = slot;
                 r.state = 1;
-                S->slots_scene = S->slots_scene == -2 || S->slots_scene == r.job.scene ? r.job.scene : -1;
-                DevJob dj = to_dev(r.job, off, S->work_total);
+                r.work_begin = S->work_total;
+                r.admitted_round = round;
                 S->work_total += r.work;
-                hipMemcpyAsync((DevJob*)S->jobs_mem.ptr + r.slot, &dj, sizeof(DevJob), hipMemcpyHostToDevice, c->stream);
-                hipMemsetAsync((float4*)S->arena_mem.ptr + off, 0, r.pixels * sizeof(float4), c->stream);
+                admitted.push_back(S->pending.front());
                 S->active.push_back(S->pending.front());
                 S->pending.pop_front();
                 S->stats.peak_active = std::max<int32_t>(S->stats.peak_active, (int32_t)S->active.size());
                 S->stats.peak_bytes = std::max<int64_t>(S->stats.peak_bytes, S->arena.used * (int64_t)sizeof(float4) +
                                                                                  (int64_t)wavefront_bytes(S->opts.pool_paths));
             }
-            if (S->active.empty()) {
+            for (int64_t id : S->active) {
+                const int sc = S->reqs[id]->job.scene;
+                scene_uniform = scene_uniform == -2 || scene_uniform == sc ? sc : -1;
+            }
+            if (S->active.empty() && !prev_pending && S->copying.empty()) {
                 if (S->busy) {
                     S->stats.busy_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - S->t_busy0).count();
                     S->busy = false;
@@ -286,61 +363,113 @@ static void server_loop(Server* S) {
             }
             if (!S->busy) { S->busy = true; S->t_busy0 = std::chrono::steady_clock::now(); }
         }
-        // publish the new work total, run iterations (scene-uniform traversal
-        // while every admitted request uses one scene)
-        hipMemcpyAsync(&c->wf.ctr->work_total, &S->work_total, sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
-        p.uscene = S->slots_scene >= 0 ? (const DevScene*)c->scene_table.ptr + S->slots_scene : nullptr;
-        for (int k = 0; k < 2; k++)
-            wavefront_iteration((const DevScene*)c->scene_table.ptr, (const DevJob*)S->jobs_mem.ptr, S->n_slots,
-                                (float4*)S->arena_mem.ptr, c->wf, p, nullptr, c->stream);
-        // retire finished requests
-        std::vector<int64_t> act;
-        {
-            std::lock_guard<std::mutex> l(S->m);
-            act = S->active;
+        const bool run = !S->active.empty();
+        if (run) {
+            // ---- enqueue one round under the context's mutex --------------
+            std::lock_guard<std::mutex> cl(c->mutex);
+            std::string err;
+            // the server's copy of the scene table (pointers into resident scenes)
+            if (S->table_gen != c->scene_gen) {
+                std::vector<DevScene> table(std::max<size_t>(1, c->scenes.size()));
+                for (size_t i = 0; i < c->scenes.size(); i++) if (c->scenes[i]) table[i] = c->scenes[i]->dev;
+                if (S->table_mem.bytes < table.size() * sizeof(DevScene)) {
+                    hipStreamSynchronize(S->stream);  // kernels in flight still read the old table
+                    if (S->table_mem.alloc(table.size() * sizeof(DevScene)) != hipSuccess) err = "server scene table";
+                }
+                if (err.empty()) {
+                    // pageable source: the copy has completed or been staged when the call returns
+                    hipMemcpy(S->table_mem.ptr, table.data(), table.size() * sizeof(DevScene), hipMemcpyHostToDevice);
+                    S->table_gen = c->scene_gen;
+                }
+            }
+            // traversal overflow for the deepest admitted scene
+            int depth = 1;
+            for (int64_t id : S->active) depth = std::max(depth, c->scenes[S->reqs[id]->job.scene]->max_depth);
+            if (err.empty() && depth > S->overflow_depth) {
+                hipStreamSynchronize(S->stream);
+                const size_t need = (size_t)overflow_entries_per_thread(depth) *
+                                    (size_t)wavefront_stack_threads(S->opts.pool_paths) * sizeof(int);
+                if (S->overflow.alloc(need) != hipSuccess) err = "server traversal overflow";
+                else { S->overflow_depth = depth; S->wf.overflow = (int*)S->overflow.ptr; }
+            }
+            if (!err.empty()) {
+                std::lock_guard<std::mutex> l(S->m);
+                fail_all(S, err);
+                break;
+            }
+            // new jobs: records, cleared films and finished-path counts
+            std::vector<DevJob>& up = S->job_upload[cur];
+            up.clear();
+            for (int64_t id : admitted) {
+                Request& r = *S->reqs[id];
+                up.push_back(to_dev(r.job, r.film_off, r.work_begin));
+            }
+            for (size_t k = 0; k < admitted.size(); k++) {
+                Request& r = *S->reqs[admitted[k]];
+                hipMemcpyAsync((DevJob*)S->jobs_mem.ptr + r.slot, &up[k], sizeof(DevJob), hipMemcpyHostToDevice, S->stream);
+                hipMemsetAsync((float4*)S->arena_mem.ptr + r.film_off, 0, r.pixels * sizeof(float4), S->stream);
+                hipMemsetAsync((int*)S->done_mem.ptr + r.slot, 0, sizeof(int), S->stream);
+            }
+            // generation order: the active requests that may still have work
+            // to generate, in admission (work_begin) order
+            std::vector<GenEntry>& ord = S->order_upload[cur];
+            ord.clear();
+            for (int64_t id : S->active) {
+                const Request& r = *S->reqs[id];
+                if (r.work_begin + r.work > S->work_next_seen) ord.push_back(GenEntry{r.work_begin, r.slot, 0});
+            }
+            if (ord.empty()) ord.push_back(GenEntry{0, 0, 0});  // nothing left to generate
+            hipMemcpyAsync(S->order_mem.ptr, ord.data(), ord.size() * sizeof(GenEntry), hipMemcpyHostToDevice, S->stream);
+            *S->total_host[cur] = S->work_total;
+            hipMemcpyAsync(&S->wf.ctr->work_total, S->total_host[cur], sizeof(int64_t), hipMemcpyHostToDevice, S->stream);
+            p.closest_shadow = c->any_area_lights();
+            p.ext_features = c->any_ext_features();
+            p.uscene = scene_uniform >= 0 ? (const DevScene*)S->table_mem.ptr + scene_uniform : nullptr;
+            p.order = (const GenEntry*)S->order_mem.ptr;
+            p.n_order = (int)ord.size();
+            for (int k = 0; k < ROUND_ITERATIONS; k++)
+                wavefront_iteration((const DevScene*)S->table_mem.ptr, (const DevJob*)S->jobs_mem.ptr, S->slot_hw,
+                                    (float4*)S->arena_mem.ptr, S->wf, p, nullptr, S->stream);
+            S->done_len[cur] = S->slot_hw;
+            hipMemcpyAsync(S->done_host[cur], S->done_mem.ptr, (size_t)S->slot_hw * sizeof(int), hipMemcpyDeviceToHost,
+                           S->stream);
+            hipMemcpyAsync(S->ctr_host[cur], S->wf.ctr, sizeof(Counters), hipMemcpyDeviceToHost, S->stream);
+            hipEventRecord(S->round_ev[cur], S->stream);
         }
-        int lo = S->n_slots, hi = 0;
-        for (int64_t id : act) { lo = std::min(lo, S->reqs[id]->slot); hi = std::max(hi, S->reqs[id]->slot + 1); }
-        done_host.resize(std::max(0, hi - lo));
-        hipMemcpyAsync(done_host.data(), (int*)S->done_mem.ptr + lo, (hi - lo) * sizeof(int), hipMemcpyDeviceToHost, c->stream);
-        hipMemcpyAsync(&host, c->wf.ctr, sizeof(Counters), hipMemcpyDeviceToHost, c->stream);
-        if (hipStreamSynchronize(c->stream) != hipSuccess) {
-            std::lock_guard<std::mutex> l(S->m);
-            S->error = "device error in the serving loop";
-            S->stopping = true;
-            for (int64_t id : S->active) S->reqs[id]->state = -1;
-            S->active.clear();
-            S->done_cv.notify_all();
-            break;
-        }
-        std::vector<int64_t> finished;
-        for (int64_t id : act) {
-            Request& r = *S->reqs[id];
-            if (done_host[r.slot - lo] >= r.work) finished.push_back(id);
-        }
-        for (int64_t id : finished) {
-            Request& r = *S->reqs[id];
-            std::vector<float4> f(r.pixels);
-            hipMemcpy(f.data(), (float4*)S->arena_mem.ptr + r.film_off, r.pixels * sizeof(float4), hipMemcpyDeviceToHost);
-            r.film.resize(3 * r.pixels);
-            for (int64_t i = 0; i < r.pixels; i++) {
-                float inv = f[i].w > 0.0f ? 1.0f / f[i].w : 0.0f;
-                r.film[3 * i] = f[i].x * inv; r.film[3 * i + 1] = f[i].y * inv; r.film[3 * i + 2] = f[i].z * inv;
+        // ---- the previous round's results, while this round runs ---------
+        if (prev_pending) {
+            if (hipEventSynchronize(S->round_ev[prev]) != hipSuccess) {
+                std::lock_guard<std::mutex> l(S->m);
+                fail_all(S, "device error in the serving loop");
+                break;
+            }
+            S->work_next_seen = S->ctr_host[prev]->work_next;
+            std::vector<int64_t> still;
+            for (int64_t id : S->active) {
+                Request& r = *S->reqs[id];
+                const bool seen = r.admitted_round <= round - 1 && r.slot < S->done_len[prev];
+                if (!seen || S->done_host[prev][r.slot] < r.work) { still.push_back(id); continue; }
+                // every path finished: copy the film out behind the work in flight
+                if (hipHostMalloc((void**)&r.staging, r.pixels * sizeof(float4), hipHostMallocDefault) != hipSuccess ||
+                    hipEventCreateWithFlags(&r.copied, hipEventDisableTiming) != hipSuccess) {
+                    std::lock_guard<std::mutex> l(S->m);
+                    fail_all(S, "film staging allocation");
+                    return;
+                }
+                hipMemcpyAsync(r.staging, (float4*)S->arena_mem.ptr + r.film_off, r.pixels * sizeof(float4),
+                               hipMemcpyDeviceToHost, S->stream);
+                hipEventRecord(r.copied, S->stream);
+                r.state = 3;
+                S->copying.push_back(id);
             }
             std::lock_guard<std::mutex> l(S->m);
-            S->arena.release(r.film_off, r.pixels);
-            r.t_done = std::chrono::steady_clock::now();
-            r.state = 2;
-            S->stats.completed++;
-            S->stats.path_samples += r.work;
-            S->active.erase(std::find(S->active.begin(), S->active.end(), id));
+            S->active.swap(still);
         }
-        if (!finished.empty()) {
-            std::lock_guard<std::mutex> l(S->m);
-            S->done_cv.notify_all();
-        }
+        prev_pending = run;
+        finish_copies(S, !run);
+        round++;
     }
-    c->wf.job_done = nullptr;
+    finish_copies(S, true);
 }
 
 Server* server_create(Ctx* c, const dfr_server_opts& o) {
@@ -350,19 +479,39 @@ Server* server_create(Ctx* c, const dfr_server_opts& o) {
     if (S->opts.max_active <= 0) S->opts.max_active = 256;
     if (S->opts.pool_paths <= 0) S->opts.pool_paths = c->wf_params.capacity;
     if (S->opts.pool_paths > MAX_POOL_PATHS) { c->error = "pool_paths above 2^26"; return nullptr; }
+    if (o.slot_capacity < 0) { c->error = "slot_capacity must be >= 0"; return nullptr; }
+    S->slot_capacity = o.slot_capacity > 0 ? o.slot_capacity : 1 << 16;
     std::lock_guard<std::mutex> lock(c->mutex);
     hipSetDevice(c->device);
+    S->device = c->device;
     if (c->scenes.empty()) { c->error = "no resident scene"; return nullptr; }
     int64_t budget = o.hbm_budget_bytes > 0 ? o.hbm_budget_bytes : (int64_t)8 << 30;
     int64_t film_bytes = budget - (int64_t)wavefront_bytes(S->opts.pool_paths);
     if (film_bytes < (int64_t)sizeof(float4)) { c->error = "budget below the wavefront pool"; return nullptr; }
     S->arena_pixels = film_bytes / (int64_t)sizeof(float4);
-    if (S->arena_mem.alloc((size_t)S->arena_pixels * sizeof(float4)) != hipSuccess ||
-        S->jobs_mem.alloc((size_t)S->job_capacity * sizeof(DevJob)) != hipSuccess ||
-        S->done_mem.alloc((size_t)S->job_capacity * sizeof(int)) != hipSuccess) {
-        c->error = "server allocation";
-        return nullptr;
+    bool ok = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && alloc_wavefront_buffers(S->wf, S->wf_mem, S->opts.pool_paths);
+    ok = ok && S->arena_mem.alloc((size_t)S->arena_pixels * sizeof(float4)) == hipSuccess;
+    ok = ok && S->jobs_mem.alloc((size_t)S->slot_capacity * sizeof(DevJob)) == hipSuccess;
+    ok = ok && S->done_mem.alloc((size_t)S->slot_capacity * sizeof(int)) == hipSuccess;
+    ok = ok && S->order_mem.alloc((size_t)(S->opts.max_active + 1) * sizeof(GenEntry)) == hipSuccess;
+    for (int k = 0; k < 2 && ok; k++) {
+        ok = hipHostMalloc((void**)&S->done_host[k], (size_t)S->slot_capacity * sizeof(int), hipHostMallocDefault) == hipSuccess &&
+             hipHostMalloc((void**)&S->ctr_host[k], sizeof(Counters), hipHostMallocDefault) == hipSuccess &&
+             hipHostMalloc((void**)&S->total_host[k], sizeof(int64_t), hipHostMallocDefault) == hipSuccess &&
+             hipEventCreateWithFlags(&S->round_ev[k], hipEventDisableTiming) == hipSuccess;
     }
+    auto cleanup = [&](Server* s) {
+        for (int k = 0; k < 2; k++) {
+            if (s->done_host[k]) hipHostFree(s->done_host[k]);
+            if (s->ctr_host[k]) hipHostFree(s->ctr_host[k]);
+            if (s->total_host[k]) hipHostFree(s->total_host[k]);
+            if (s->round_ev[k]) hipEventDestroy(s->round_ev[k]);
+        }
+        if (s->stream) hipStreamDestroy(s->stream);
+    };
+    if (!ok) { cleanup(S.get()); c->error = "server allocation"; return nullptr; }
+    S->wf.job_done = (int*)S->done_mem.ptr;
     S->arena.init(S->arena_pixels);
     Server* raw = S.release();
     raw->worker = std::thread(server_loop, raw);
@@ -380,26 +529,49 @@ void server_destroy(Server* S) {
     }
     S->cv.notify_all();
     if (S->worker.joinable()) S->worker.join();
+    Ctx* c = S->ctx;
     {
-        std::lock_guard<std::mutex> lock(S->ctx->mutex);
-        hipSetDevice(S->ctx->device);
-        S->arena_mem.release(); S->jobs_mem.release(); S->done_mem.release();
+        std::lock_guard<std::mutex> lock(c->mutex);
+        hipSetDevice(S->device);
+        hipStreamSynchronize(S->stream);
+        // release the scene references of every request that was accepted
+        for (auto& r : S->reqs)
+            if (r->job.scene >= 0 && r->job.scene < (int)c->scene_refs.size()) c->scene_refs[r->job.scene]--;
+        for (auto& r : S->reqs) {
+            if (r->staging) hipHostFree(r->staging);
+            if (r->copied) hipEventDestroy(r->copied);
+        }
+        for (int k = 0; k < 2; k++) {
+            hipHostFree(S->done_host[k]); hipHostFree(S->ctr_host[k]); hipHostFree(S->total_host[k]);
+            hipEventDestroy(S->round_ev[k]);
+        }
+        S->arena_mem.release(); S->jobs_mem.release(); S->done_mem.release(); S->order_mem.release();
+        S->table_mem.release(); S->overflow.release();
+        for (auto& b : S->wf_mem) b.release();
+        hipStreamDestroy(S->stream);
     }
     delete S;
 }
 
 int64_t server_submit(Server* S, const dfr_job& job) {
     if (job.width < 1 || job.height < 1 || job.spp < 1) return DFR_E_ARGUMENT;
-    if (job.scene < 0 || job.scene >= (int)S->ctx->scenes.size() || !S->ctx->scenes[job.scene]) return DFR_E_ARGUMENT;
+    {
+        // the scene must stay resident while a request renders it
+        Ctx* c = S->ctx;
+        std::lock_guard<std::mutex> cl(c->mutex);
+        if (job.scene < 0 || job.scene >= (int)c->scenes.size() || !c->scenes[job.scene]) return DFR_E_ARGUMENT;
+        if ((int)c->scene_refs.size() < (int)c->scenes.size()) c->scene_refs.resize(c->scenes.size(), 0);
+        c->scene_refs[job.scene]++;
+    }
     auto r = std::make_unique<Request>();
     r->job = job;
     r->pixels = film_pixels(job);
     r->work = r->pixels * job.spp;
     r->t_submit = std::chrono::steady_clock::now();
     std::lock_guard<std::mutex> l(S->m);
-    if (S->stopping) return DFR_E_STATE;
     int64_t id = (int64_t)S->reqs.size();
     S->reqs.push_back(std::move(r));
+    if (S->stopping) { S->reqs[id]->state = -1; return DFR_E_STATE; }
     S->pending.push_back(id);
     S->cv.notify_all();
     return id;
@@ -465,6 +637,8 @@ int32_t dfr_scene_remove(dfr_ctx* h, int32_t scene) {
     CTX_GUARD(c);
     if (scene < 0 || scene >= (int)c->scenes.size() || !c->scenes[scene] || scene == c->main_scene)
         return c->fail(DFR_E_ARGUMENT, "no such resident scene");
+    if (scene < (int)c->scene_refs.size() && c->scene_refs[scene] > 0)
+        return c->fail(DFR_E_STATE, "a serving loop still has requests on this scene");
     hipStreamSynchronize(c->stream);
     c->scenes[scene].reset();
     return c->sync_scene_table();

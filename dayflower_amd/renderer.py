@@ -359,12 +359,12 @@ class Server:
     """Serving loop over a GPURenderer's resident scenes (continuous
     batching; see dfr_server_* in include/dayflower_hip.h)."""
 
-    def __init__(self, renderer, hbm_budget_bytes=64 << 30, max_active=256, pool_paths=1 << 20, spp_quantum=0):
+    def __init__(self, renderer, hbm_budget_bytes=64 << 30, max_active=256, pool_paths=1 << 20, slot_capacity=0):
         self.r = renderer
         self.lib = renderer.lib
         o = abi.ServerOpts()
-        o.hbm_budget_bytes, o.max_active, o.pool_paths, o.spp_quantum = int(hbm_budget_bytes), int(max_active), \
-            int(pool_paths), int(spp_quantum)
+        o.hbm_budget_bytes, o.max_active, o.pool_paths, o.slot_capacity = int(hbm_budget_bytes), int(max_active), \
+            int(pool_paths), int(slot_capacity)
         self.h = self.lib.dfr_server_create(renderer.ctx, ctypes.byref(o))
         if not self.h:
             msg = self.lib.dfr_last_error(renderer.ctx)

@@ -419,8 +419,9 @@ typedef struct dfr_server_opts {
     int64_t hbm_budget_bytes;   /* admission budget for films + queues      */
     int32_t max_active;         /* at most this many requests on the device */
     int32_t pool_paths;         /* path pool size of the wavefront engine   */
-    int32_t spp_quantum;        /* samples per pixel per request per round  */
-    int32_t reserved;
+    int32_t spp_quantum;        /* reserved (0)                              */
+    int32_t slot_capacity;      /* job slots (0: 65536); slots of finished
+                                   requests are reused                       */
 } dfr_server_opts;
 
 dfr_server* dfr_server_create(dfr_ctx* ctx, const dfr_server_opts* opts);

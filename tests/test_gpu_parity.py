@@ -623,3 +623,41 @@ def test_served_requests_match_the_oracle():
     r.dispose()
     for px, rel in _jobs_vs_oracle(sc, jobs, films):
         assert px >= 0.98 and rel < 0.003
+
+
+def test_server_reuses_slots_and_leaves_the_context_free():
+    """A server with 3 job slots serves 10 requests (slots of finished
+    requests are reused); every film equals the same job rendered alone.
+    While it serves, the context renders its own frame and compiles another
+    scene (the server holds the context only while it enqueues a round), and
+    a scene with requests on it cannot be removed."""
+    import threading
+    from dayflower_amd.renderer import IllegalStateException, Server, make_job
+    r = _renderer()
+    sc = S.icosphere_scene(20000, with_perez=True, width=48, height=48)
+    sid = r.addScene(sc)
+    cam = sc.camera
+    jobs = [make_job(sid, cam, 32 + 8 * (i % 3), 24 + 8 * (i % 2), [2, 4, 8][i % 3], seed=300 + i) for i in range(10)]
+    alone = [r.batchRender([j])[0] for j in jobs]
+    r.setScene(S.cornell_box(32, 32))
+    r.setSeed(1)
+    r.setup()
+    srv = Server(r, hbm_budget_bytes=Server.pool_bytes(1 << 16) + (64 << 20), max_active=3, pool_paths=1 << 16,
+                 slot_capacity=3)
+    ids = [srv.submit(j) for j in jobs]
+    with pytest.raises(IllegalStateException):
+        r.removeScene(sid)
+    # the context's own work interleaves with the serving rounds
+    r.render(8)
+    own, cnt = r.film(32, 32)
+    extra = r.addScene(S.icosphere_scene(2000, with_perez=False, width=16, height=16))
+    films = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    r.removeScene(sid)
+    r.removeScene(extra)
+    r.dispose()
+    assert (cnt == 8).all() and own.mean() > 0
+    assert st["completed"] == len(jobs) and st["peak_active"] <= 3
+    for f, a in zip(films, alone):
+        np.testing.assert_allclose(f, a, rtol=1e-4, atol=1e-5)
