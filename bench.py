@@ -216,7 +216,8 @@ def run_batched(args):
 
 def run_serving(args):
     """Poisson request stream against a resident 2M-triangle scene: the
-    continuous-batching server vs a FIFO of plain per-request launches."""
+    continuous-batching server on the whole stream, and a FIFO of plain
+    per-request launches on the stream's first --plain-prefix seconds."""
     import threading
     import numpy as np
     from dayflower_amd import abi, workloads as W
@@ -229,32 +230,43 @@ def run_serving(args):
     log(f"serving: {len(stream)} requests over {args.duration} s, {total / 1e9:.2f} G samples")
     out = {}
     # continuous batching server
-    srv = Server(r, hbm_budget_bytes=64 << 30, max_active=256, pool_paths=args.pool or (1 << 24))
+    pool = args.pool or (1 << 24)
+    srv = Server(r, hbm_budget_bytes=64 << 30, max_active=256, pool_paths=pool)
     t0 = time.perf_counter()
     ids = []
+    last_log = t0
     for at, job in stream:
         now = time.perf_counter() - t0
         if at > now:
             time.sleep(at - now)
         ids.append(srv.submit(job))
-    for i in ids:
+        if time.perf_counter() - last_log > 20:
+            last_log = time.perf_counter()
+            log(f"serving: {len(ids)}/{len(stream)} submitted, {srv.stats()['completed']} done at {last_log - t0:.0f} s")
+    for k, i in enumerate(ids):
         srv.wait(i)
+        if time.perf_counter() - last_log > 20:
+            last_log = time.perf_counter()
+            log(f"serving: {k + 1}/{len(ids)} done at {last_log - t0:.0f} s")
     wall = time.perf_counter() - t0
     lat = np.array([srv.latency(i) for i in ids])
     st = srv.stats()
     srv.close()
     out["served"] = {"p50_s": round(float(np.percentile(lat, 50)), 4), "p99_s": round(float(np.percentile(lat, 99)), 4),
                      "samples_per_s": round(total / wall, 1), "wall_s": round(wall, 2), "peak_active": st["peak_active"],
-                     "peak_bytes": st["peak_bytes"]}
+                     "peak_bytes": st["peak_bytes"], "busy_s": round(st["busy_seconds"], 2)}
     log(f"serving: server p50 {out['served']['p50_s']} s p99 {out['served']['p99_s']} s, {total / wall / 1e6:.1f} M samples/s")
-    # FIFO of plain per-request launches (one job at a time)
+    # FIFO of plain per-request launches (one job at a time) on a prefix
+    prefix = [(at, j) for at, j in stream if at < args.plain_prefix]
+    ptotal = sum(j.width * j.height * j.spp for _, j in prefix)
+    # the server's figures on the same prefix, for a like-for-like ratio
+    lat_prefix = lat[:len(prefix)]
     done = {}
-    q = list(range(len(stream)))
     t0 = time.perf_counter()
     submit_t = {}
 
     def feeder():
-        for k, (at, _) in enumerate(stream):
+        for k, (at, _) in enumerate(prefix):
             now = time.perf_counter() - t0
             if at > now:
                 time.sleep(at - now)
@@ -262,24 +274,28 @@ def run_serving(args):
 
     th = threading.Thread(target=feeder)
     th.start()
-    for k in q:
+    for k in range(len(prefix)):
         while k not in submit_t:
             time.sleep(0.0005)
-        r.batchRender([stream[k][1]], mode=abi.BATCH_PLAIN, readback=True)
+        r.batchRender([prefix[k][1]], mode=abi.BATCH_PLAIN, readback=True)
         done[k] = time.perf_counter() - submit_t[k]
-        if (k + 1) % 50 == 0:
-            log(f"serving: plain {k + 1}/{len(q)} requests done")
+        if time.perf_counter() - last_log > 20:
+            last_log = time.perf_counter()
+            log(f"serving: plain {k + 1}/{len(prefix)} requests done")
     th.join()
-    wall = time.perf_counter() - t0
-    lat = np.array([done[k] for k in q])
-    out["plain_fifo"] = {"p50_s": round(float(np.percentile(lat, 50)), 4), "p99_s": round(float(np.percentile(lat, 99)), 4),
-                         "samples_per_s": round(total / wall, 1), "wall_s": round(wall, 2)}
-    log(f"serving: plain p50 {out['plain_fifo']['p50_s']} s p99 {out['plain_fifo']['p99_s']} s, {total / wall / 1e6:.1f} M samples/s")
+    pwall = time.perf_counter() - t0
+    plat = np.array([done[k] for k in range(len(prefix))])
+    out["plain_fifo"] = {"p50_s": round(float(np.percentile(plat, 50)), 4), "p99_s": round(float(np.percentile(plat, 99)), 4),
+                         "samples_per_s": round(ptotal / pwall, 1), "wall_s": round(pwall, 2),
+                         "requests": len(prefix), "stream_prefix_s": args.plain_prefix}
+    log(f"serving: plain p50 {out['plain_fifo']['p50_s']} s p99 {out['plain_fifo']['p99_s']} s, {ptotal / pwall / 1e6:.1f} M samples/s")
     r.dispose()
     print(json.dumps({"metric": METRIC, "config": {"workload": f"Poisson {args.rate} req/s for {args.duration} s, "
                       "2M-triangle displaced icosphere, res 256^2..1024^2, spp 4..64, 64 GB admission budget",
-                      "requests": len(stream), "path_samples": total},
+                      "requests": len(stream), "path_samples": total, "pool_paths": pool},
                       "served": out["served"], "plain_fifo": out["plain_fifo"],
+                      "served_prefix_latency": {"p50_s": round(float(np.percentile(lat_prefix, 50)), 4),
+                                                "p99_s": round(float(np.percentile(lat_prefix, 99)), 4)},
                       "throughput_ratio": round(out["served"]["samples_per_s"] / out["plain_fifo"]["samples_per_s"], 3),
                       "unit": "path samples/s", "n_gpus": 1}), flush=True)
 
@@ -340,6 +356,8 @@ def main():
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=2.0)
+    ap.add_argument("--plain-prefix", type=float, default=2.0,
+                    help="serving: the plain FIFO leg serves the requests arriving in the first this many seconds")
     ap.add_argument("--pool", type=int, default=0, help="path pool capacity (0: library default)")
     ap.add_argument("--sort-keys", type=int, default=-1, help="0 (off) or 1 (on) (default: library)")
     ap.add_argument("--sample-stride", type=int, default=256,
