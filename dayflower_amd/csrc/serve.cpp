@@ -231,7 +231,13 @@ struct Server {
     std::deque<int64_t> pending;
     std::vector<std::unique_ptr<Request>> reqs;
     std::vector<int64_t> active;     // admitted and not yet copied out, in admission order
-    std::vector<int64_t> copying;    // film copies in flight
+    std::deque<int64_t> copying;     // film copies in flight, in the order they were enqueued
+    std::thread finisher;            // completes copied films off the worker thread
+    std::condition_variable copy_cv;
+    bool worker_done = false;
+    uint64_t submits = 0, releases = 0;  // bumped by submissions and by freed films (wake the idle worker)
+    std::multimap<size_t, float4*> staging_free;  // pinned film buffers for reuse, by size
+    std::map<float4*, size_t> staging_sizes;      // every pinned film buffer and its size
     // device state, owned by the server
     int device = 0;
     hipStream_t stream = nullptr;
@@ -269,36 +275,71 @@ static void fail_all(Server* S, const std::string& msg) {
     for (int64_t id : S->active) S->reqs[id]->state = -1;
     for (int64_t id : S->copying) S->reqs[id]->state = -1;
     for (int64_t id : S->pending) S->reqs[id]->state = -1;
-    S->active.clear(); S->copying.clear(); S->pending.clear();
+    S->active.clear(); S->pending.clear();
     S->done_cv.notify_all();
 }
 
-// Films whose copy has landed: means into the request, area and slot freed.
-static void finish_copies(Server* S, bool block) {
-    std::vector<int64_t> still;
-    for (int64_t id : S->copying) {
-        Request& r = *S->reqs[id];
-        hipError_t e = block ? hipEventSynchronize(r.copied) : hipEventQuery(r.copied);
-        if (e == hipErrorNotReady) { still.push_back(id); continue; }
-        r.film.resize(3 * r.pixels);
-        for (int64_t i = 0; i < r.pixels; i++) {
-            const float4 f = r.staging[i];
-            const float inv = f.w > 0.0f ? 1.0f / f.w : 0.0f;
-            r.film[3 * i] = f.x * inv; r.film[3 * i + 1] = f.y * inv; r.film[3 * i + 2] = f.z * inv;
-        }
-        hipHostFree(r.staging);
-        r.staging = nullptr;
-        hipEventDestroy(r.copied);
-        r.copied = nullptr;
+// A pinned staging buffer of at least `n` float4 for a film copy.
+static float4* take_staging(Server* S, size_t n) {
+    {
         std::lock_guard<std::mutex> l(S->m);
+        auto it = S->staging_free.lower_bound(n);
+        if (it != S->staging_free.end()) {
+            float4* p = it->second;
+            S->staging_free.erase(it);
+            return p;
+        }
+    }
+    size_t cap = 1;
+    while (cap < n) cap <<= 1;  // power-of-two sizes keep the pool reusable
+    float4* p = nullptr;
+    if (hipHostMalloc((void**)&p, cap * sizeof(float4), hipHostMallocDefault) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> l(S->m);
+    S->staging_sizes[p] = cap;
+    return p;
+}
+
+// Completes copied films off the worker thread: waits for each copy in
+// order, turns (sum, count) into means, then frees the film area, the slot
+// and the staging buffer.
+static void finisher_loop(Server* S) {
+    hipSetDevice(S->device);
+    for (;;) {
+        int64_t id;
+        {
+            std::unique_lock<std::mutex> l(S->m);
+            S->copy_cv.wait(l, [&] { return !S->copying.empty() || S->worker_done; });
+            if (S->copying.empty()) break;
+            id = S->copying.front();
+            S->copying.pop_front();
+        }
+        Request& r = *S->reqs[id];
+        const hipError_t e = hipEventSynchronize(r.copied);
+        if (e == hipSuccess) {
+            std::vector<float> film(3 * r.pixels);
+            for (int64_t i = 0; i < r.pixels; i++) {
+                const float4 f = r.staging[i];
+                const float inv = f.w > 0.0f ? 1.0f / f.w : 0.0f;
+                film[3 * i] = f.x * inv; film[3 * i + 1] = f.y * inv; film[3 * i + 2] = f.z * inv;
+            }
+            r.film.swap(film);
+        }
+        hipEventDestroy(r.copied);
+        std::lock_guard<std::mutex> l(S->m);
+        r.copied = nullptr;
+        S->staging_free.emplace(S->staging_sizes[r.staging], r.staging);
+        r.staging = nullptr;
         S->arena.release(r.film_off, r.pixels);
         S->free_slots.push_back(r.slot);
+        S->releases++;
+        S->cv.notify_all();
         r.t_done = std::chrono::steady_clock::now();
-        r.state = e == hipSuccess ? 2 : -1;
-        if (e == hipSuccess) { S->stats.completed++; S->stats.path_samples += r.work; }
+        if (r.state == 3) {
+            r.state = e == hipSuccess ? 2 : -1;
+            if (e == hipSuccess) { S->stats.completed++; S->stats.path_samples += r.work; }
+        }
         S->done_cv.notify_all();
     }
-    S->copying.swap(still);
 }
 
 static void server_loop(Server* S) {
@@ -352,13 +393,16 @@ static void server_loop(Server* S) {
                 const int sc = S->reqs[id]->job.scene;
                 scene_uniform = scene_uniform == -2 || scene_uniform == sc ? sc : -1;
             }
-            if (S->active.empty() && !prev_pending && S->copying.empty()) {
+            if (S->active.empty() && !prev_pending) {
                 if (S->busy) {
                     S->stats.busy_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - S->t_busy0).count();
                     S->busy = false;
                 }
                 if (S->stopping) break;
-                S->cv.wait(l, [&] { return S->stopping || !S->pending.empty(); });
+                // nothing runs: wait for a new request, or for a copied-out
+                // film to free the room the pending ones need
+                const uint64_t sub0 = S->submits, rel0 = S->releases;
+                S->cv.wait(l, [&] { return S->stopping || S->submits != sub0 || S->releases != rel0; });
                 continue;
             }
             if (!S->busy) { S->busy = true; S->t_busy0 = std::chrono::steady_clock::now(); }
@@ -450,26 +494,31 @@ static void server_loop(Server* S) {
                 const bool seen = r.admitted_round <= round - 1 && r.slot < S->done_len[prev];
                 if (!seen || S->done_host[prev][r.slot] < r.work) { still.push_back(id); continue; }
                 // every path finished: copy the film out behind the work in flight
-                if (hipHostMalloc((void**)&r.staging, r.pixels * sizeof(float4), hipHostMallocDefault) != hipSuccess ||
-                    hipEventCreateWithFlags(&r.copied, hipEventDisableTiming) != hipSuccess) {
+                r.staging = take_staging(S, (size_t)r.pixels);
+                if (!r.staging || hipEventCreateWithFlags(&r.copied, hipEventDisableTiming) != hipSuccess) {
                     std::lock_guard<std::mutex> l(S->m);
                     fail_all(S, "film staging allocation");
+                    S->worker_done = true;
+                    S->copy_cv.notify_all();
                     return;
                 }
                 hipMemcpyAsync(r.staging, (float4*)S->arena_mem.ptr + r.film_off, r.pixels * sizeof(float4),
                                hipMemcpyDeviceToHost, S->stream);
                 hipEventRecord(r.copied, S->stream);
+                std::lock_guard<std::mutex> l(S->m);
                 r.state = 3;
                 S->copying.push_back(id);
+                S->copy_cv.notify_one();
             }
             std::lock_guard<std::mutex> l(S->m);
             S->active.swap(still);
         }
         prev_pending = run;
-        finish_copies(S, !run);
         round++;
     }
-    finish_copies(S, true);
+    std::lock_guard<std::mutex> l(S->m);
+    S->worker_done = true;
+    S->copy_cv.notify_all();
 }
 
 Server* server_create(Ctx* c, const dfr_server_opts& o) {
@@ -515,6 +564,7 @@ Server* server_create(Ctx* c, const dfr_server_opts& o) {
     S->arena.init(S->arena_pixels);
     Server* raw = S.release();
     raw->worker = std::thread(server_loop, raw);
+    raw->finisher = std::thread(finisher_loop, raw);
     return raw;
 }
 
@@ -529,6 +579,7 @@ void server_destroy(Server* S) {
     }
     S->cv.notify_all();
     if (S->worker.joinable()) S->worker.join();
+    if (S->finisher.joinable()) S->finisher.join();
     Ctx* c = S->ctx;
     {
         std::lock_guard<std::mutex> lock(c->mutex);
@@ -537,10 +588,9 @@ void server_destroy(Server* S) {
         // release the scene references of every request that was accepted
         for (auto& r : S->reqs)
             if (r->job.scene >= 0 && r->job.scene < (int)c->scene_refs.size()) c->scene_refs[r->job.scene]--;
-        for (auto& r : S->reqs) {
-            if (r->staging) hipHostFree(r->staging);
+        for (auto& r : S->reqs)
             if (r->copied) hipEventDestroy(r->copied);
-        }
+        for (auto& kv : S->staging_sizes) hipHostFree(kv.first);
         for (int k = 0; k < 2; k++) {
             hipHostFree(S->done_host[k]); hipHostFree(S->ctr_host[k]); hipHostFree(S->total_host[k]);
             hipEventDestroy(S->round_ev[k]);
@@ -573,6 +623,7 @@ int64_t server_submit(Server* S, const dfr_job& job) {
     S->reqs.push_back(std::move(r));
     if (S->stopping) { S->reqs[id]->state = -1; return DFR_E_STATE; }
     S->pending.push_back(id);
+    S->submits++;
     S->cv.notify_all();
     return id;
 }
@@ -590,7 +641,12 @@ int server_wait(Server* S, int64_t id, float* film) {
     Request& r = *S->reqs[id];
     S->done_cv.wait(l, [&] { return r.state == 2 || r.state < 0; });
     if (r.state < 0) return DFR_E_MEMORY;
-    if (film) std::memcpy(film, r.film.data(), r.film.size() * sizeof(float));
+    if (film) {
+        // the film is handed over once: its host copy is released
+        if (r.film.empty() && r.pixels > 0) return DFR_E_STATE;
+        std::memcpy(film, r.film.data(), r.film.size() * sizeof(float));
+        std::vector<float>().swap(r.film);
+    }
     return DFR_OK;
 }
 

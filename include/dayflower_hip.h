@@ -433,7 +433,9 @@ void dfr_server_destroy(dfr_server* server);
 int64_t dfr_server_submit(dfr_server* server, const dfr_job* job);
 /* 1 done, 0 pending, <0 error (including rejected by admission). */
 int32_t dfr_server_poll(dfr_server* server, int64_t request);
-/* Block until the request is done; then copy its film (3 floats/pixel). */
+/* Block until the request is done; then copy its film (3 floats/pixel).  A
+ * film is handed over once: a second wait with film != NULL returns
+ * DFR_E_STATE (NULL only waits). */
 int32_t dfr_server_wait(dfr_server* server, int64_t request, float* film);
 /* Latency (submit -> film ready) in seconds of request `request`. */
 int32_t dfr_server_latency(dfr_server* server, int64_t request, double* seconds);
