@@ -247,7 +247,7 @@ def run_serving(args):
         srv.wait(i)
         if time.perf_counter() - last_log > 20:
             last_log = time.perf_counter()
-            log(f"serving: {k + 1}/{len(ids)} done at {last_log - t0:.0f} s")
+            log(f"serving: waited for {k + 1}/{len(ids)}, {srv.stats()['completed']} done at {last_log - t0:.0f} s")
     wall = time.perf_counter() - t0
     lat = np.array([srv.latency(i) for i in ids])
     st = srv.stats()
