@@ -121,6 +121,7 @@ ENTRY_POINTS = [
     ("dfr_destroy", None, [_c_void_p]),
     ("dfr_last_error", ctypes.c_char_p, [_c_void_p]),
     ("dfr_set_scene", c_int32, [_c_void_p, P(SceneDesc)]),
+    ("dfr_update_transforms", c_int32, [_c_void_p, _c_void_p, c_int32, c_int32]),
     ("dfr_set_resolution", c_int32, [_c_void_p, c_int32, c_int32]),
     ("dfr_set_i32", c_int32, [_c_void_p, c_int32, c_int32]),
     ("dfr_get_i32", c_int32, [_c_void_p, c_int32, P(c_int32)]),

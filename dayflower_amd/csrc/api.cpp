@@ -263,138 +263,288 @@ static int upload(DeviceBuffer& b, const std::vector<T>& v, hipStream_t s) {
     return DFR_OK;
 }
 
-int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, hipStream_t s, std::string& err) {
+// Object-to-world maps of the primitives (updateMatrix44Fs,
+// AbstractSceneKernel.java:292-300; CompiledPrimitiveCache.java:323-350),
+// validated: affine (last row 0 0 0 1) and invertible.  Spheres and
+// rectangles take similarities (rotation or reflection, uniform scale,
+// translation), which keep them spheres and rectangles; meshes and planes
+// take any affine map.  A mesh that carries an area light takes rigid
+// motions only and a sphere light translations only: the light samplers
+// work in world space, where a scaled light's pdf would differ from the
+// reference's object-space one.
+int make_xforms(const SceneHost& in, const std::vector<float>& m, std::vector<Xform>& out, std::string& err) {
+    out.clear();
+    if (m.empty()) return DFR_OK;
+    if (m.size() != 16 * in.prims.size()) { err = "one 4x4 matrix per primitive"; return DFR_E_ARGUMENT; }
+    out.resize(in.prims.size());
+    for (size_t p = 0; p < in.prims.size(); p++) {
+        const float* M = &m[16 * p];
+        Xform& x = out[p];
+        for (int k = 0; k < 16; k++)
+            if (!std::isfinite(M[k])) { err = "primitive " + std::to_string(p) + ": non-finite matrix"; return DFR_E_ARGUMENT; }
+        if (M[12] != 0.0f || M[13] != 0.0f || M[14] != 0.0f || M[15] != 1.0f) {
+            err = "primitive " + std::to_string(p) + ": projective matrices are not supported"; return DFR_E_UNSUPPORTED;
+        }
+        double A[3][3];
+        for (int r = 0; r < 3; r++) { for (int c = 0; c < 3; c++) A[r][c] = M[4 * r + c]; x.t[r] = M[4 * r + 3]; }
+        const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                           A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+        if (!(std::fabs(det) > 1e-30)) { err = "primitive " + std::to_string(p) + ": singular matrix"; return DFR_E_ARGUMENT; }
+        // normal matrix: the inverse transpose, cofactors / det
+        double C[3][3];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) {
+                const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+                C[r][c] = A[r1][c1] * A[r2][c2] - A[r1][c2] * A[r2][c1];
+            }
+        bool identity = true;
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) {
+                x.a[r][c] = A[r][c];
+                x.n[r][c] = C[r][c] / det;
+                identity = identity && A[r][c] == (r == c ? 1.0 : 0.0);
+            }
+        x.det = det;
+        x.linear_identity = identity;
+        x.identity = identity && x.t[0] == 0.0 && x.t[1] == 0.0 && x.t[2] == 0.0;
+        // columns' lengths and orthogonality: A^T A = s^2 I for a similarity
+        double g[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) g[i][j] = A[0][i] * A[0][j] + A[1][i] * A[1][j] + A[2][i] * A[2][j];
+        const double s2 = (g[0][0] + g[1][1] + g[2][2]) / 3.0;
+        x.scale = std::sqrt(s2);
+        x.similarity = true;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                if (std::fabs(g[i][j] - (i == j ? s2 : 0.0)) > 1e-5 * s2) x.similarity = false;
+        const int shape = in.prims[p].shape;
+        if ((shape == DFR_SHAPE_SPHERE || shape == DFR_SHAPE_RECTANGLE) && !x.similarity) {
+            err = "primitive " + std::to_string(p) + ": spheres and rectangles take rotation, uniform scale and translation only";
+            return DFR_E_UNSUPPORTED;
+        }
+    }
+    // area lights: the primitive that carries each one
+    for (size_t l = 0; l < in.lights.size(); l++) {
+        const dfr_light& L = in.lights[l];
+        if (L.type != DFR_LIGHT_DIFFUSE_AREA) continue;
+        int owner = -1;
+        if (L.triangle < 0) {
+            owner = L.primitive;
+        } else {
+            for (size_t p = 0; p < in.prims.size(); p++)
+                if (in.prims[p].shape == DFR_SHAPE_TRIANGLE_MESH && L.triangle >= in.prims[p].first_triangle &&
+                    L.triangle < in.prims[p].first_triangle + in.prims[p].triangle_count) owner = (int)p;
+        }
+        if (owner < 0 || owner >= (int)out.size()) continue;  // compile reports it
+        const Xform& x = out[owner];
+        const bool rigid = x.similarity && x.det > 0.0 && std::fabs(x.scale - 1.0) < 1e-5;
+        if (L.triangle < 0 && !x.linear_identity) {
+            err = "light " + std::to_string(l) + ": a sphere light takes translations only";
+            return DFR_E_UNSUPPORTED;
+        }
+        if (L.triangle >= 0 && !rigid) {
+            err = "light " + std::to_string(l) + ": a mesh that carries an area light takes rotations and translations only";
+            return DFR_E_UNSUPPORTED;
+        }
+    }
+    return DFR_OK;
+}
+
+namespace {
+struct D3v { double x, y, z; };
+inline D3v xpoint(const Xform* x, const float* p) {
+    if (!x || x->identity) return {p[0], p[1], p[2]};
+    D3v r;
+    r.x = x->a[0][0] * p[0] + x->a[0][1] * p[1] + x->a[0][2] * p[2] + x->t[0];
+    r.y = x->a[1][0] * p[0] + x->a[1][1] * p[1] + x->a[1][2] * p[2] + x->t[1];
+    r.z = x->a[2][0] * p[0] + x->a[2][1] * p[1] + x->a[2][2] * p[2] + x->t[2];
+    return r;
+}
+inline D3v xvec(const double (*m)[3], const float* v) {
+    return {m[0][0] * v[0] + m[0][1] * v[1] + m[0][2] * v[2], m[1][0] * v[0] + m[1][1] * v[1] + m[1][2] * v[2],
+            m[2][0] * v[0] + m[2][1] * v[1] + m[2][2] * v[2]};
+}
+inline std::array<float, 3> fl(D3v v) { return {(float)v.x, (float)v.y, (float)v.z}; }
+inline std::array<float, 3> fl_unit(D3v v) {
+    const double l = std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    if (!(l > 0.0)) return {0.0f, 0.0f, 0.0f};
+    return {(float)(v.x / l), (float)(v.y / l), (float)(v.z / l)};
+}
+}  // namespace
+
+// World-space geometry of a scene under its transforms: the three vertex
+// positions of every triangle (9 floats each), the world-space primitive
+// records and the owning primitive of each triangle.
+static int world_geometry(const SceneHost& in, const std::vector<Xform>& xf, std::vector<int32_t>& tri_prim,
+                          std::vector<float>& wpos, std::vector<float>& wnrm, std::string& err) {
     const int64_t nt = (int64_t)in.indices.size() / 3;
     const int64_t nv = (int64_t)in.positions.size() / 3;
-    for (int64_t i = 0; i < 3 * nt; i++)
-        if (in.indices[i] < 0 || in.indices[i] >= nv) { err = "triangle index out of range"; return DFR_E_ARGUMENT; }
-    std::vector<int32_t> tri_prim(nt, -1);
-    for (size_t p = 0; p < in.prims.size(); p++) {
-        const dfr_primitive& pr = in.prims[p];
-        if (pr.material < 0 || pr.material >= (int)in.mats.size()) { err = "primitive material out of range"; return DFR_E_ARGUMENT; }
-        if (pr.shape == DFR_SHAPE_TRIANGLE_MESH) {
-            if (pr.first_triangle < 0 || pr.triangle_count < 0 || pr.first_triangle + pr.triangle_count > nt) {
-                err = "primitive triangle range out of bounds"; return DFR_E_ARGUMENT;
-            }
-            for (int64_t t = pr.first_triangle; t < pr.first_triangle + pr.triangle_count; t++) tri_prim[t] = (int32_t)p;
-        } else if (pr.shape < DFR_SHAPE_TRIANGLE_MESH || pr.shape > DFR_SHAPE_PLANE) {
-            err = "unknown shape"; return DFR_E_ARGUMENT;
-        }
-    }
-    for (int64_t t = 0; t < nt; t++)
-        if (tri_prim[t] < 0) { err = "triangle not owned by any mesh primitive"; return DFR_E_ARGUMENT; }
-
-    auto t0 = std::chrono::steady_clock::now();
-    int threads = (int)std::max(1u, std::thread::hardware_concurrency());
-    BvhResult bvh = build_bvh(in.positions.data(), in.indices.data(), nt, threads, BVH_WIDTH);
-    out.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    // deepest traversal stack (entries) the kernels can need
-    out.max_depth = BVH_WIDTH == 4 ? bvh.max_stack4 + 1 : bvh.max_depth + 1;
-    out.n_nodes = BVH_WIDTH == 4 ? (int64_t)bvh.nodes4.size() : (int64_t)bvh.nodes.size();
-    out.n_tris = nt;
-    out.sah_cost = bvh.sah_cost;
-
-    std::vector<float4> tris(3 * nt), vpos(nv), vnrm(nv), tri_ng(nt);
-    std::vector<int4> tri_idx(nt);
-    for (int64_t v = 0; v < nv; v++) {
-        const float* p = &in.positions[3 * v];
-        vpos[v] = f4(p[0], p[1], p[2], 0.0f);
-    }
-    for (int64_t t = 0; t < nt; t++) {
-        const int32_t* ix = &in.indices[3 * t];
-        tri_idx[t].x = ix[0]; tri_idx[t].y = ix[1]; tri_idx[t].z = ix[2]; tri_idx[t].w = tri_prim[t];
-        // Triangle3F surface normal: normalize(cross(normalize(b-a), normalize(c-a)))
-        auto P = [&](int k) { const float* p = &in.positions[3 * k]; return std::array<float, 3>{p[0], p[1], p[2]}; };
-        auto a = P(ix[0]), b = P(ix[1]), c = P(ix[2]);
-        auto nz = [](std::array<float, 3> v) {
-            float l = (float)std::sqrt((double)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
-            if (l >= 0.99999982f && l <= 1.00000012f) return v;
-            return std::array<float, 3>{v[0] / l, v[1] / l, v[2] / l};
-        };
-        auto ab = nz({b[0] - a[0], b[1] - a[1], b[2] - a[2]});
-        auto ac = nz({c[0] - a[0], c[1] - a[1], c[2] - a[2]});
-        auto n = nz({ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]});
-        tri_ng[t] = f4(n[0], n[1], n[2], 0.0f);
-        if (in.normals.empty()) {
-            // flat: every vertex of the triangle carries the face normal
-        }
-    }
+    // object-space vertex normals: given, or the normalised sum of the
+    // adjacent faces' normals
+    std::vector<float> vn;
     if (!in.normals.empty()) {
-        for (int64_t v = 0; v < nv; v++) vnrm[v] = f4(in.normals[3 * v], in.normals[3 * v + 1], in.normals[3 * v + 2], 0.0f);
+        vn = in.normals;
     } else {
-        // no vertex normals: per-vertex normal = sum of adjacent face normals (meshes given without normals
-        // are expected to have unshared vertices, which makes this the face normal)
         std::vector<double> acc(3 * nv, 0.0);
-        for (int64_t t = 0; t < nt; t++)
-            for (int k = 0; k < 3; k++) {
-                int v = in.indices[3 * t + k];
-                acc[3 * v] += tri_ng[t].x; acc[3 * v + 1] += tri_ng[t].y; acc[3 * v + 2] += tri_ng[t].z;
-            }
+        for (int64_t t = 0; t < nt; t++) {
+            const int32_t* ix = &in.indices[3 * t];
+            auto P = [&](int k) { const float* p = &in.positions[3 * (int64_t)k]; return std::array<float, 3>{p[0], p[1], p[2]}; };
+            auto nz = [](std::array<float, 3> v) {
+                float l = (float)std::sqrt((double)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+                if (l >= 0.99999982f && l <= 1.00000012f) return v;
+                return std::array<float, 3>{v[0] / l, v[1] / l, v[2] / l};
+            };
+            auto A = P(ix[0]), B = P(ix[1]), C = P(ix[2]);
+            auto ab = nz({B[0] - A[0], B[1] - A[1], B[2] - A[2]});
+            auto ac = nz({C[0] - A[0], C[1] - A[1], C[2] - A[2]});
+            auto n = nz({ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]});
+            for (int k = 0; k < 3; k++) { acc[3 * ix[k]] += n[0]; acc[3 * ix[k] + 1] += n[1]; acc[3 * ix[k] + 2] += n[2]; }
+        }
+        vn.resize(3 * nv);
         for (int64_t v = 0; v < nv; v++) {
             double l = std::sqrt(acc[3 * v] * acc[3 * v] + acc[3 * v + 1] * acc[3 * v + 1] + acc[3 * v + 2] * acc[3 * v + 2]);
             if (l == 0) l = 1;
-            vnrm[v] = f4((float)(acc[3 * v] / l), (float)(acc[3 * v + 1] / l), (float)(acc[3 * v + 2] / l), 0.0f);
+            for (int k = 0; k < 3; k++) vn[3 * v + k] = (float)(acc[3 * v + k] / l);
         }
     }
-    // shading records (one 128-byte line per triangle, original order)
-    std::vector<float4> tshade(8 * nt, f4(0.0f, 0.0f, 0.0f, 0.0f));
+    wpos.resize(9 * nt);
+    wnrm.resize(9 * nt);
     for (int64_t t = 0; t < nt; t++) {
-        const int4 ix = tri_idx[t];
-        const float4 a = vpos[ix.x], b = vpos[ix.y], c = vpos[ix.z];
-        const float4 na = vnrm[ix.x], nb = vnrm[ix.y], nc = vnrm[ix.z], ng = tri_ng[t];
-        float4* r = &tshade[8 * t];
-        r[0] = f4(a.x, a.y, a.z, na.x);
-        r[1] = f4(b.x, b.y, b.z, na.y);
-        r[2] = f4(c.x, c.y, c.z, na.z);
-        r[3] = f4(nb.x, nb.y, nb.z, ng.x);
-        r[4] = f4(nc.x, nc.y, nc.z, ng.y);
-        r[5] = f4(ng.z, 0.0f, 0.0f, 0.0f);
+        const Xform* x = xf.empty() ? nullptr : &xf[tri_prim[t]];
+        for (int k = 0; k < 3; k++) {
+            const int64_t v = in.indices[3 * t + k];
+            const auto p = fl(xpoint(x, &in.positions[3 * v]));
+            std::memcpy(&wpos[9 * t + 3 * k], p.data(), 12);
+            // N n, not normalised: the kernels normalise the barycentric
+            // blend, which then equals the normal matrix applied to the
+            // object-space blend (the reference transforms the interpolated
+            // object-space normal)
+            if (!x || x->linear_identity) std::memcpy(&wnrm[9 * t + 3 * k], &vn[3 * v], 12);
+            else { const auto n = fl(xvec(x->n, &vn[3 * v])); std::memcpy(&wnrm[9 * t + 3 * k], n.data(), 12); }
+        }
     }
+    (void)err;
+    return DFR_OK;
+}
+
+// Bakes the per-triangle and per-primitive device records of a scene from
+// its world-space geometry (`wpos`, `wnrm`: three vertices per triangle) in
+// the BVH's triangle order: traversal triangles, shading lines, primitives,
+// lights.  Fills `out`'s record vectors.
+struct Baked {
+    std::vector<float4> tris, tshade;
+    std::vector<DevPrim> prims;
+    std::vector<int32_t> analytic;
+    std::vector<DevLight> lights;
+    std::vector<DevPerez> perez;
+    std::vector<float4> xforms;   // per primitive: DevScene::xforms
+};
+
+static int bake_scene(const SceneHost& in, const std::vector<Xform>& xf, const std::vector<int32_t>& tri_prim,
+                      const std::vector<float>& wpos, const std::vector<float>& wnrm,
+                      const std::vector<uint32_t>& tri_order, Baked& b, std::string& err) {
+    const int64_t nt = (int64_t)in.indices.size() / 3;
+    // Triangle3F surface normal: normalize(cross(normalize(b-a), normalize(c-a)))
+    auto nz = [](std::array<float, 3> v) {
+        float l = (float)std::sqrt((double)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+        if (l >= 0.99999982f && l <= 1.00000012f) return v;
+        return std::array<float, 3>{v[0] / l, v[1] / l, v[2] / l};
+    };
+    b.tshade.assign(8 * nt, f4(0.0f, 0.0f, 0.0f, 0.0f));
+    for (int64_t t = 0; t < nt; t++) {
+        const float* P = &wpos[9 * t];
+        const float* N = &wnrm[9 * t];
+        auto ab = nz({P[3] - P[0], P[4] - P[1], P[5] - P[2]});
+        auto ac = nz({P[6] - P[0], P[7] - P[1], P[8] - P[2]});
+        auto ng = nz({ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]});
+        // a reflection turns the world-space winding: the normal matrix
+        // keeps the object-space side
+        if (!xf.empty() && xf[tri_prim[t]].det < 0.0) ng = {-ng[0], -ng[1], -ng[2]};
+        float4* r = &b.tshade[8 * t];
+        r[0] = f4(P[0], P[1], P[2], N[0]);
+        r[1] = f4(P[3], P[4], P[5], N[1]);
+        r[2] = f4(P[6], P[7], P[8], N[2]);
+        r[3] = f4(N[3], N[4], N[5], ng[0]);
+        r[4] = f4(N[6], N[7], N[8], ng[1]);
+        r[5] = f4(ng[2], 0.0f, 0.0f, 0.0f);
+    }
+    b.tris.resize(3 * nt);
     for (int64_t i = 0; i < nt; i++) {
-        uint32_t t = bvh.tri_order[i];
-        const int32_t* ix = &in.indices[3 * (int64_t)t];
-        const float* a = &in.positions[3 * (int64_t)ix[0]];
-        const float* b = &in.positions[3 * (int64_t)ix[1]];
-        const float* c = &in.positions[3 * (int64_t)ix[2]];
+        const uint32_t t = tri_order[i];
+        const float* P = &wpos[9 * (int64_t)t];
         float tid, pid;
         int32_t ti = (int32_t)t, pi = tri_prim[t];
         std::memcpy(&tid, &ti, 4); std::memcpy(&pid, &pi, 4);
-        tris[3 * i + 0] = f4(a[0], a[1], a[2], tid);
-        tris[3 * i + 1] = f4(b[0], b[1], b[2], pid);
-        tris[3 * i + 2] = f4(c[0], c[1], c[2], 0.0f);
+        b.tris[3 * i + 0] = f4(P[0], P[1], P[2], tid);
+        b.tris[3 * i + 1] = f4(P[3], P[4], P[5], pid);
+        b.tris[3 * i + 2] = f4(P[6], P[7], P[8], 0.0f);
     }
-    std::vector<DevPrim> prims(in.prims.size());
-    std::vector<int32_t> analytic;
+    b.prims.assign(in.prims.size(), DevPrim{});
+    b.xforms.assign(6 * std::max<size_t>(1, in.prims.size()), f4(0.0f, 0.0f, 0.0f, 0.0f));
+    b.analytic.clear();
     for (size_t p = 0; p < in.prims.size(); p++) {
         const dfr_primitive& pr = in.prims[p];
-        DevPrim& d = prims[p];
+        DevPrim& d = b.prims[p];
         std::memset(&d, 0, sizeof(d));
         d.shape = pr.shape; d.material = pr.material; d.area_light = pr.area_light;
         std::memcpy(d.data, pr.data, sizeof(d.data));
-        if (pr.shape != DFR_SHAPE_TRIANGLE_MESH) analytic.push_back((int32_t)p);
+        const Xform* x = xf.empty() || xf[p].identity ? nullptr : &xf[p];
+        if (pr.shape != DFR_SHAPE_TRIANGLE_MESH) b.analytic.push_back((int32_t)p);
+        if (x && !x->linear_identity && pr.shape != DFR_SHAPE_PLANE) {
+            // surface frames are built in object space and carried to world
+            // space: rows 0-2 map a world normal to its object-space
+            // direction (A^T), rows 3-5 an object-space frame vector to world
+            // space (the normal matrix, OrthonormalBasis33F.transformTranspose
+            // of the inverse); for a sphere both are the rotation itself
+            const double k0 = pr.shape == DFR_SHAPE_SPHERE ? 1.0 / x->scale : 1.0;
+            const double k1 = pr.shape == DFR_SHAPE_SPHERE ? x->scale : 1.0;
+            float4* X = &b.xforms[6 * p];
+            for (int r = 0; r < 3; r++) {
+                X[r] = f4((float)(x->a[0][r] * k0), (float)(x->a[1][r] * k0), (float)(x->a[2][r] * k0), 0.0f);
+                X[3 + r] = f4((float)(x->n[r][0] * k1), (float)(x->n[r][1] * k1), (float)(x->n[r][2] * k1), 0.0f);
+            }
+            d.xformed = 1;
+        }
+        if (pr.shape == DFR_SHAPE_SPHERE && x) {
+            const auto c = fl(xpoint(x, pr.data));
+            d.data[0] = c[0]; d.data[1] = c[1]; d.data[2] = c[2];
+            d.data[3] = (float)(pr.data[3] * x->scale);
+        } else if (pr.shape == DFR_SHAPE_RECTANGLE && x) {
+            for (int k = 0; k < 3; k++) {
+                const auto q = fl(xpoint(x, pr.data + 3 * k));
+                std::memcpy(d.data + 3 * k, q.data(), 12);
+            }
+        } else if (pr.shape == DFR_SHAPE_PLANE && x) {
+            // Plane3F's frame: all three vectors go through the normal
+            // matrix and are normalised, as the reference transforms an
+            // intersection's basis (AbstractSceneKernel.java:1104-1217)
+            const auto o = fl(xpoint(x, pr.data));
+            const auto u = fl_unit(xvec(x->n, pr.data + 3));
+            const auto v = fl_unit(xvec(x->n, pr.data + 6));
+            const auto w = fl_unit(xvec(x->n, pr.data + 9));
+            std::memcpy(d.data, o.data(), 12); std::memcpy(d.data + 3, u.data(), 12);
+            std::memcpy(d.data + 6, v.data(), 12); std::memcpy(d.data + 9, w.data(), 12);
+        }
         if (pr.shape == DFR_SHAPE_RECTANGLE) {
-            auto nz = [](float* v) {
+            auto nz3 = [](float* v) {
                 float l = (float)std::sqrt((double)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
                 if (l >= 0.99999982f && l <= 1.00000012f) return;
                 v[0] /= l; v[1] /= l; v[2] /= l;
             };
-            float ab[3] = {pr.data[3] - pr.data[0], pr.data[4] - pr.data[1], pr.data[5] - pr.data[2]};
-            float ac[3] = {pr.data[6] - pr.data[0], pr.data[7] - pr.data[1], pr.data[8] - pr.data[2]};
-            nz(ab); nz(ac);
+            float ab[3] = {d.data[3] - d.data[0], d.data[4] - d.data[1], d.data[5] - d.data[2]};
+            float ac[3] = {d.data[6] - d.data[0], d.data[7] - d.data[1], d.data[8] - d.data[2]};
+            nz3(ab); nz3(ac);
             float n[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
-            nz(n);
+            nz3(n);
+            if (x && x->det < 0.0) { n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2]; }
             d.normal[0] = n[0]; d.normal[1] = n[1]; d.normal[2] = n[2];
         }
     }
-    std::vector<DevMaterial> mats(in.mats.size());
-    std::vector<DevLobe> lobes;
-    for (size_t m = 0; m < in.mats.size(); m++) compile_material(in.mats[m], mats[m], lobes);
-    for (const dfr_material& m : in.mats) out.ext_features = out.ext_features || m.type >= DFR_MAT_GLOSSY;
-    std::vector<DevLight> lights(in.lights.size());
-    std::vector<DevPerez> perez;
+    b.lights.assign(in.lights.size(), DevLight{});
+    b.perez.clear();
     for (size_t l = 0; l < in.lights.size(); l++) {
         const dfr_light& L = in.lights[l];
-        DevLight& d = lights[l];
+        DevLight& d = b.lights[l];
         std::memset(&d, 0, sizeof(d));
         d.type = L.type; d.prim = L.primitive; d.tri = (int32_t)L.triangle; d.two_sided = L.two_sided;
         std::memcpy(d.radiance, L.radiance, 12);
@@ -412,25 +562,22 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
                 return DFR_E_ARGUMENT;
             }
             if (!(in.prims[sp].data[3] > 0.0f)) { err = "light " + std::to_string(l) + ": sphere radius must be positive"; return DFR_E_ARGUMENT; }
-            const double r = in.prims[sp].data[3];
+            const double r = b.prims[sp].data[3];
             d.area = (float)(4.0 * 3.14159265358979323846 * r * r);
             d.prim = sp;
             d.tri = -1;
         } else if (L.type == DFR_LIGHT_DIFFUSE_AREA) {
             if (L.triangle >= nt) { err = "area light triangle out of range"; return DFR_E_ARGUMENT; }
-            const int32_t* ix = &in.indices[3 * L.triangle];
-            const float* a = &in.positions[3 * (int64_t)ix[0]];
-            const float* b = &in.positions[3 * (int64_t)ix[1]];
-            const float* c = &in.positions[3 * (int64_t)ix[2]];
-            float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+            const float* P = &wpos[9 * L.triangle];
+            float e1[3] = {P[3] - P[0], P[4] - P[1], P[5] - P[2]}, e2[3] = {P[6] - P[0], P[7] - P[1], P[8] - P[2]};
             float x = e1[1] * e2[2] - e1[2] * e2[1], y = e1[2] * e2[0] - e1[0] * e2[2], z = e1[0] * e2[1] - e1[1] * e2[0];
             d.area = 0.5f * (float)std::sqrt((double)(x * x + y * y + z * z));
             d.prim = tri_prim[L.triangle];
         } else if (L.type == DFR_LIGHT_PEREZ) {
-            DevPerez p;
-            perez_build(p, L.turbidity, L.direction);
-            d.perez = (int32_t)perez.size();
-            perez.push_back(p);
+            DevPerez pz;
+            perez_build(pz, L.turbidity, L.direction);
+            d.perez = (int32_t)b.perez.size();
+            b.perez.push_back(pz);
         } else if (L.type == DFR_LIGHT_SPOT || L.type == DFR_LIGHT_DIRECTIONAL) {
             const double x = L.direction[0], y = L.direction[1], z = L.direction[2], n = std::sqrt(x * x + y * y + z * z);
             if (!(n > 0.0)) { err = "light " + std::to_string(l) + ": zero direction"; return DFR_E_ARGUMENT; }
@@ -446,30 +593,92 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
             err = "light " + std::to_string(l) + ": type out of range"; return DFR_E_ARGUMENT;
         }
     }
+    return DFR_OK;
+}
+
+static int upload_baked(const Baked& bk, CompiledScene& out, hipStream_t s) {
+    int rc;
+    if ((rc = upload(out.tris, bk.tris, s))) return rc;
+    if ((rc = upload(out.tshade, bk.tshade, s))) return rc;
+    if ((rc = upload(out.prims, bk.prims, s))) return rc;
+    if ((rc = upload(out.analytic, bk.analytic, s))) return rc;
+    if ((rc = upload(out.lights, bk.lights, s))) return rc;
+    if ((rc = upload(out.perez, bk.perez, s))) return rc;
+    if ((rc = upload(out.xforms, bk.xforms, s))) return rc;
+    return DFR_OK;
+}
+
+static int validate_scene(const SceneHost& in, std::vector<int32_t>& tri_prim, std::string& err) {
+    const int64_t nt = (int64_t)in.indices.size() / 3;
+    const int64_t nv = (int64_t)in.positions.size() / 3;
+    for (int64_t i = 0; i < 3 * nt; i++)
+        if (in.indices[i] < 0 || in.indices[i] >= nv) { err = "triangle index out of range"; return DFR_E_ARGUMENT; }
+    tri_prim.assign(nt, -1);
+    for (size_t p = 0; p < in.prims.size(); p++) {
+        const dfr_primitive& pr = in.prims[p];
+        if (pr.material < 0 || pr.material >= (int)in.mats.size()) { err = "primitive material out of range"; return DFR_E_ARGUMENT; }
+        if (pr.shape == DFR_SHAPE_TRIANGLE_MESH) {
+            if (pr.first_triangle < 0 || pr.triangle_count < 0 || pr.first_triangle + pr.triangle_count > nt) {
+                err = "primitive triangle range out of bounds"; return DFR_E_ARGUMENT;
+            }
+            for (int64_t t = pr.first_triangle; t < pr.first_triangle + pr.triangle_count; t++) tri_prim[t] = (int32_t)p;
+        } else if (pr.shape < DFR_SHAPE_TRIANGLE_MESH || pr.shape > DFR_SHAPE_PLANE) {
+            err = "unknown shape"; return DFR_E_ARGUMENT;
+        }
+    }
+    for (int64_t t = 0; t < nt; t++)
+        if (tri_prim[t] < 0) { err = "triangle not owned by any mesh primitive"; return DFR_E_ARGUMENT; }
+    return DFR_OK;
+}
+
+int compile_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledScene& out, int light_pdf_mode,
+                  hipStream_t s, std::string& err) {
+    const int64_t nt = (int64_t)in.indices.size() / 3;
+    std::vector<int32_t> tri_prim;
+    int rc = validate_scene(in, tri_prim, err);
+    if (rc) return rc;
+    std::vector<float> wpos, wnrm;
+    if ((rc = world_geometry(in, xf, tri_prim, wpos, wnrm, err))) return rc;
+
+    auto t0 = std::chrono::steady_clock::now();
+    int threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::vector<int32_t> flat_idx(3 * nt);
+    for (int64_t i = 0; i < 3 * nt; i++) flat_idx[i] = (int32_t)i;
+    BvhResult bvh = build_bvh(wpos.data(), flat_idx.data(), nt, threads, BVH_WIDTH);
+    out.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    // deepest traversal stack (entries) the kernels can need
+    out.max_depth = BVH_WIDTH == 4 ? bvh.max_stack4 + 1 : bvh.max_depth + 1;
+    out.n_nodes = BVH_WIDTH == 4 ? (int64_t)bvh.nodes4.size() : (int64_t)bvh.nodes.size();
+    out.n_tris = nt;
+    out.sah_cost = bvh.sah_cost;
+
+    Baked bk;
+    if ((rc = bake_scene(in, xf, tri_prim, wpos, wnrm, bvh.tri_order, bk, err))) return rc;
+    std::vector<DevMaterial> mats(in.mats.size());
+    std::vector<DevLobe> lobes;
+    for (size_t m = 0; m < in.mats.size(); m++) compile_material(in.mats[m], mats[m], lobes);
+    for (const dfr_material& m : in.mats) out.ext_features = out.ext_features || m.type >= DFR_MAT_GLOSSY;
     // traversal addresses nodes and triangles with 32-bit byte offsets
     const size_t node_bytes = BVH_WIDTH == 4 ? bvh.nodes4.size() * sizeof(bvh.nodes4[0]) : bvh.nodes.size() * sizeof(bvh.nodes[0]);
-    if (node_bytes >= (size_t(1) << 32) || tris.size() * sizeof(float4) >= (size_t(1) << 32)) {
+    if (node_bytes >= (size_t(1) << 32) || bk.tris.size() * sizeof(float4) >= (size_t(1) << 32)) {
         err = "scene too large: BVH nodes and triangles must each stay below 4 GiB";
         return DFR_E_UNSUPPORTED;
     }
-    int rc;
     if ((rc = BVH_WIDTH == 4 ? upload(out.nodes, bvh.nodes4, s) : upload(out.nodes, bvh.nodes, s))) return rc;
-    if ((rc = upload(out.tris, tris, s))) return rc;
-    if ((rc = upload(out.tshade, tshade, s))) return rc;
-    if ((rc = upload(out.prims, prims, s))) return rc;
+    if ((rc = upload_baked(bk, out, s))) return rc;
     // 4 x material sort class per primitive (see sort_key_in)
-    std::vector<int32_t> prim_class(std::max<size_t>(1, prims.size()), 0);
-    for (size_t i = 0; i < prims.size(); i++) {
-        const int m = prims[i].material;
+    std::vector<int32_t> prim_class(std::max<size_t>(1, bk.prims.size()), 0);
+    for (size_t i = 0; i < bk.prims.size(); i++) {
+        const int m = bk.prims[i].material;
         const int t = (m >= 0 && m < (int)in.mats.size()) ? in.mats[m].type : 0;
         prim_class[i] = 4 * std::min(t, 6);
     }
     if ((rc = upload(out.prim_class, prim_class, s))) return rc;
-    if ((rc = upload(out.analytic, analytic, s))) return rc;
     if ((rc = upload(out.mats, mats, s))) return rc;
     if ((rc = upload(out.lobes, lobes, s))) return rc;
-    if ((rc = upload(out.lights, lights, s))) return rc;
-    if ((rc = upload(out.perez, perez, s))) return rc;
+    // host copies kept for transform updates (the tree's topology stays, its boxes are refitted)
+    out.nodes4_host = std::move(bvh.nodes4);
+    out.tri_order = std::move(bvh.tri_order);
     DevScene& ds = out.dev;
     ds.nodes = (const float4*)out.nodes.ptr;
     ds.tris = (const float4*)out.tris.ptr;
@@ -481,18 +690,72 @@ int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, h
     ds.lobes = (const DevLobe*)out.lobes.ptr;
     ds.lights = (const DevLight*)out.lights.ptr;
     ds.perez = (const DevPerez*)out.perez.ptr;
-    ds.n_analytic = (int32_t)analytic.size();
-    ds.n_lights = (int32_t)lights.size();
-    for (const DevLight& L : lights) {
+    ds.xforms = (const float4*)out.xforms.ptr;
+    ds.n_analytic = (int32_t)bk.analytic.size();
+    ds.n_lights = (int32_t)bk.lights.size();
+    for (const DevLight& L : bk.lights) {
         out.area_lights = out.area_lights || L.type == DFR_LIGHT_DIFFUSE_AREA;
         out.ext_features = out.ext_features || L.type == DFR_LIGHT_SPOT || L.type == DFR_LIGHT_DIRECTIONAL;
     }
     ds.has_bvh = nt > 0 ? 1 : 0;
     ds.light_pdf_mode = light_pdf_mode;
     std::memcpy(out.camera, in.camera, sizeof(out.camera));
-    out.bytes = out.nodes.bytes + out.tris.bytes +
-                out.tshade.bytes + out.prims.bytes + out.analytic.bytes + out.mats.bytes + out.lobes.bytes +
-                out.lights.bytes + out.perez.bytes;
+    out.bytes = out.nodes.bytes + out.tris.bytes + out.tshade.bytes + out.prims.bytes + out.analytic.bytes +
+                out.mats.bytes + out.lobes.bytes + out.lights.bytes + out.perez.bytes + out.xforms.bytes;
+    if (hipStreamSynchronize(s) != hipSuccess) { err = "upload failed"; return DFR_E_DEVICE; }
+    return DFR_OK;
+}
+
+// Moves a compiled scene's primitives to new transforms without a rebuild:
+// world-space geometry is recomputed, the BVH keeps its topology and its
+// boxes are refitted bottom-up (children are numbered after their parents),
+// and every per-triangle and per-primitive record is uploaded again.
+int refit_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledScene& out, hipStream_t s, std::string& err) {
+    if (BVH_WIDTH != 4) { err = "transform updates refit four-wide trees"; return DFR_E_UNSUPPORTED; }
+    const int64_t nt = (int64_t)in.indices.size() / 3;
+    if (nt != out.n_tris || (int64_t)out.tri_order.size() != nt) { err = "scene changed since it was compiled"; return DFR_E_STATE; }
+    std::vector<int32_t> tri_prim;
+    int rc = validate_scene(in, tri_prim, err);
+    if (rc) return rc;
+    std::vector<float> wpos, wnrm;
+    if ((rc = world_geometry(in, xf, tri_prim, wpos, wnrm, err))) return rc;
+    Baked bk;
+    if ((rc = bake_scene(in, xf, tri_prim, wpos, wnrm, out.tri_order, bk, err))) return rc;
+    // refit: each node's bound is the union of its children's
+    std::vector<Node4>& nodes = out.nodes4_host;
+    const int64_t nn = (int64_t)nodes.size();
+    std::vector<std::array<float, 6>> bound(nn);
+    for (int64_t q = nn - 1; q >= 0; q--) {
+        Node4& w = nodes[q];
+        std::array<float, 6> nb = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (int k = 0; k < 4; k++) {
+            uint32_t ref;
+            std::memcpy(&ref, &w.f[24 + k], 4);
+            if (ref == EMPTY_REF) continue;
+            std::array<float, 6> cb = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+            if (ref & LEAF_FLAG) {
+                const uint32_t first = (ref & 0x7fffffffu) >> 3, count = (ref & 7u) + 1u;
+                for (uint32_t i = first; i < first + count; i++)
+                    for (int v = 0; v < 3; v++) {
+                        const float4& P = bk.tris[3 * (size_t)i + v];
+                        const float c[3] = {P.x, P.y, P.z};
+                        for (int a = 0; a < 3; a++) { cb[a] = std::min(cb[a], c[a]); cb[3 + a] = std::max(cb[3 + a], c[a]); }
+                    }
+            } else {
+                if ((int64_t)ref <= q || (int64_t)ref >= nn) { err = "BVH child numbered before its parent"; return DFR_E_STATE; }
+                cb = bound[ref];
+            }
+            w.f[0 + k] = cb[0]; w.f[4 + k] = cb[3];
+            w.f[8 + k] = cb[1]; w.f[12 + k] = cb[4];
+            w.f[16 + k] = cb[2]; w.f[20 + k] = cb[5];
+            for (int a = 0; a < 3; a++) { nb[a] = std::min(nb[a], cb[a]); nb[3 + a] = std::max(nb[3 + a], cb[3 + a]); }
+        }
+        bound[q] = nb;
+    }
+    // same sizes as before: the device buffers are rewritten in place, so the
+    // scene record's pointers stay valid
+    if ((rc = upload(out.nodes, nodes, s))) return rc;
+    if ((rc = upload_baked(bk, out, s))) return rc;
     if (hipStreamSynchronize(s) != hipSuccess) { err = "upload failed"; return DFR_E_DEVICE; }
     return DFR_OK;
 }
@@ -839,9 +1102,42 @@ int32_t dfr_set_scene(dfr_ctx* h, const dfr_scene_desc* scene) {
     if (!scene) return c->fail(DFR_E_NULL, "scene == null");
     int rc = copy_scene_desc(scene, c->scene_host, c->error);
     if (rc) return rc;
+    c->xform_host.clear();
     c->has_scene = true;
     std::memcpy(c->camera, scene->camera, sizeof(c->camera));
     c->scene_dirty = true;
+    return DFR_OK;
+}
+
+int32_t dfr_update_transforms(dfr_ctx* h, const float* object_to_world, int32_t n_primitives, int32_t rebuild) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (!c->has_scene) return c->fail(DFR_E_STATE, "setScene before updateTransforms");
+    if (!object_to_world) return c->fail(DFR_E_NULL, "object_to_world == null");
+    if (n_primitives != (int32_t)c->scene_host.prims.size())
+        return c->fail(DFR_E_ARGUMENT, "one matrix per primitive of the scene: " + std::to_string(c->scene_host.prims.size()));
+    std::vector<float> m(object_to_world, object_to_world + 16 * (size_t)n_primitives);
+    std::vector<Xform> xf;
+    int rc = make_xforms(c->scene_host, m, xf, c->error);
+    if (rc) return rc;
+    const bool compiled = !c->scene_dirty && c->main_scene >= 0 && c->scenes[c->main_scene];
+    if (compiled) {
+        const int ms = c->main_scene;
+        if (ms < (int)c->scene_refs.size() && c->scene_refs[ms] > 0)
+            return c->fail(DFR_E_STATE, "a serving loop still has requests on this scene");
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return c->fail(DFR_E_DEVICE, "stream synchronisation failed");
+        if (rebuild) {
+            auto cs = std::make_unique<CompiledScene>();
+            if ((rc = compile_scene(c->scene_host, xf, *cs, c->light_pdf_mode, c->stream, c->error))) return rc;
+            c->scenes[ms] = std::move(cs);
+            if ((rc = c->sync_scene_table())) return rc;
+        } else if ((rc = refit_scene(c->scene_host, xf, *c->scenes[ms], c->stream, c->error))) {
+            // the device records may be half rewritten: compile afresh at the next setup
+            c->scene_dirty = true;
+            return rc;
+        }
+    }
+    c->xform_host = std::move(m);
     return DFR_OK;
 }
 
@@ -942,8 +1238,11 @@ int32_t dfr_get_f32(const dfr_ctx* h, int32_t key, float* v) {
 static int32_t setup_impl(Ctx* c, int32_t compile_scene_flag) {
     if (compile_scene_flag || c->scene_dirty) {
         if (!c->has_scene) return c->fail(DFR_E_STATE, "setScene before setup");
+        std::vector<Xform> xf;
+        int rc = make_xforms(c->scene_host, c->xform_host, xf, c->error);
+        if (rc) return rc;
         auto cs = std::make_unique<CompiledScene>();
-        int rc = compile_scene(c->scene_host, *cs, c->light_pdf_mode, c->stream, c->error);
+        rc = compile_scene(c->scene_host, xf, *cs, c->light_pdf_mode, c->stream, c->error);
         if (rc) return rc;
         if (c->main_scene < 0) {
             c->main_scene = (int)c->scenes.size();

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/dayflower_hip.h"
+#include "bvh.h"
 #include "device.h"
 #include "kernels.h"
 
@@ -51,9 +52,20 @@ struct SceneHost {
     float camera[DFR_CAMERA_LENGTH];
 };
 
+// A validated object-to-world map of one primitive: the linear part `a`,
+// the translation `t`, the normal matrix `n` (inverse transpose of `a`), its
+// determinant `det`, the mean column length `scale`; `linear_identity` when
+// a is the identity (a translation at most), `similarity` when a is a
+// scaled rotation or reflection.
+struct Xform {
+    double a[3][3], n[3][3], t[3];
+    double det = 1.0, scale = 1.0;
+    bool identity = true, linear_identity = true, similarity = true;
+};
+
 // A scene compiled into SoA device buffers, resident in HBM.
 struct CompiledScene {
-    DeviceBuffer nodes, tris, tshade, prims, prim_class, analytic, mats, lobes, lights, perez;
+    DeviceBuffer nodes, tris, tshade, prims, prim_class, analytic, mats, lobes, lights, perez, xforms;
     DevScene dev{};
     int max_depth = 1;
     bool area_lights = false;  // any DiffuseAreaLight (the only lights whose NEE needs closest hits)
@@ -62,6 +74,8 @@ struct CompiledScene {
     double sah_cost = 0.0, build_seconds = 0.0;
     size_t bytes = 0;
     float camera[DFR_CAMERA_LENGTH];
+    std::vector<Node4> nodes4_host;    // the BVH, kept for refits after transform updates
+    std::vector<uint32_t> tri_order;   // BVH position -> triangle
 };
 
 int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err);
@@ -69,7 +83,10 @@ struct WavefrontBuffers;
 // Allocates (or grows) the 17 wavefront buffers of a path pool of `capacity`
 // paths into `mem` and points `wf` at them; false when an allocation fails.
 bool alloc_wavefront_buffers(WavefrontBuffers& wf, DeviceBuffer* mem, int capacity);
-int compile_scene(const SceneHost& in, CompiledScene& out, int light_pdf_mode, hipStream_t s, std::string& err);
+int make_xforms(const SceneHost& in, const std::vector<float>& m, std::vector<Xform>& out, std::string& err);
+int compile_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledScene& out, int light_pdf_mode,
+                  hipStream_t s, std::string& err);
+int refit_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledScene& out, hipStream_t s, std::string& err);
 size_t wavefront_bytes(int capacity);
 
 struct WavefrontConfig {
@@ -107,6 +124,7 @@ struct Ctx {
 
     // scene state
     SceneHost scene_host;
+    std::vector<float> xform_host;   // 16 floats per primitive of scene_host (object to world), or empty
     bool has_scene = false, scene_dirty = false, film_dirty = true, is_setup = false, clearing = false;
     int render_pass = 0;
     std::vector<std::unique_ptr<CompiledScene>> scenes;

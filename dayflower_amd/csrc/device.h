@@ -113,6 +113,10 @@ struct DevScene {
     const DevLobe* lobes;
     const DevLight* lights;
     const DevPerez* perez;
+    // per primitive, six float4 rows: rows 0-2 map a world-space normal to
+    // its object-space direction, rows 3-5 an object-space frame vector to
+    // world space (read for primitives with `xformed` set)
+    const float4* xforms;
     int32_t n_analytic;
     int32_t n_lights;
     int32_t has_bvh;
@@ -624,11 +628,29 @@ struct Surf {
     int tri;
 };
 
+// A row-vector product: (dot(X[0], v), dot(X[1], v), dot(X[2], v)).
+DEV V mul_rows(const float4* X, V v) {
+    const float4 a = X[0], b = X[1], c = X[2];
+    return mk(a.x * v.x + a.y * v.y + a.z * v.z, b.x * v.x + b.y * v.y + b.z * v.z, c.x * v.x + c.y * v.y + c.z * v.z);
+}
+// The frame of a transformed primitive at world normal `w`: built in object
+// space around the normal's object-space direction and carried to world
+// space vector by vector, each normalised (AbstractSceneKernel.java:1104-1217).
+DEV ONB frame_xformed(const float4* X, V w) {
+    const ONB o = onb_w(nrm(mul_rows(X, w)));
+    ONB r;
+    r.u = nrm(mul_rows(X + 3, o.u));
+    r.v = nrm(mul_rows(X + 3, o.v));
+    r.w = nrm(mul_rows(X + 3, o.w));
+    return r;
+}
+
 DEV Surf make_surface(const DevScene& sc, V o, V d, const HitRec& h) {
     Surf s;
     s.prim = h.prim; s.tri = h.tri;
     s.p = o + d * h.t;
     const DevPrim& p = sc.prims[h.prim];
+    const float4* X = p.xformed ? sc.xforms + 6 * (size_t)h.prim : nullptr;
     if (h.tri >= 0) {
         const size_t rb = 8 * (size_t)h.tri;
         const float4 r0 = ld4g(sc.tshade, rb), r1 = ld4g(sc.tshade, rb + 1), r2 = ld4g(sc.tshade, rb + 2);
@@ -639,14 +661,21 @@ DEV Surf make_surface(const DevScene& sc, V o, V d, const HitRec& h) {
         float rdet = __builtin_amdgcn_rcpf(dot(d, d1));
         V d3 = cross(a - o, d);
         float u = dot(d3, eca) * rdet, v = dot(d3, eab) * rdet, w = 1.0f - u - v;
-        s.g = onb_w(mk(r3.w, r4.w, r5.x));
         V ns = nrm(mk(r0.w, r1.w, r2.w) * w + f4v(r3) * u + f4v(r4) * v);
-        s.s = onb_w(ns);  // texture coordinates are absent: UV determinant 0
+        if (X) {
+            s.g = frame_xformed(X, mk(r3.w, r4.w, r5.x));
+            s.s = frame_xformed(X, ns);
+        } else {
+            s.g = onb_w(mk(r3.w, r4.w, r5.x));
+            s.s = onb_w(ns);  // texture coordinates are absent: UV determinant 0
+        }
     } else if (p.shape == DFR_SHAPE_SPHERE) {
         V c = ld3(p.data);
         float radius = p.data[3];
         V ph = (o - c) + d * h.t;
         ph = ph * (radius / len(ph));
+        const V pw = ph;
+        if (X) ph = mul_rows(X, ph);  // the sphere's own (rotated) coordinates
         if (ph.x == 0.0f && ph.y == 0.0f) ph.x = 1.0e-5f * radius;
         float theta = acosf(fminf(fmaxf(ph.z / radius, -1.0f), 1.0f));
         float zr = sqrtf(ph.x * ph.x + ph.y * ph.y);
@@ -655,10 +684,15 @@ DEV Surf make_surface(const DevScene& sc, V o, V d, const HitRec& h) {
         V dpdv = mk(ph.z * cphi, ph.z * sphi, -radius * sinf(theta)) * (0.0f - PI_F);
         V n = nrm(cross(dpdu, dpdv));
         s.g.w = nrm(n); s.g.v = nrm(dpdv); s.g.u = nrm(dpdu);
+        if (X) {
+            s.g.u = nrm(mul_rows(X + 3, s.g.u)); s.g.v = nrm(mul_rows(X + 3, s.g.v)); s.g.w = nrm(mul_rows(X + 3, s.g.w));
+            s.p = pw + c;
+        } else {
+            s.p = ph + c;
+        }
         s.s = s.g;
-        s.p = ph + c;
     } else if (p.shape == DFR_SHAPE_RECTANGLE) {
-        s.g = onb_w(ld3(p.normal));
+        s.g = X ? frame_xformed(X, ld3(p.normal)) : onb_w(ld3(p.normal));
         s.s = s.g;
     } else {
         s.g.u = nrm(ld3(p.data + 3)); s.g.v = nrm(ld3(p.data + 6)); s.g.w = nrm(ld3(p.data + 9));

@@ -59,7 +59,8 @@ struct DevMaterial {       // 32 bytes
 };
 
 struct DevPrim {           // 80 bytes
-    int32_t shape, material, area_light, pad;
+    int32_t shape, material, area_light;
+    int32_t xformed;       // 1: the surface frame is built in object space (DevScene::xforms)
     float data[12];
     float normal[4];       // rectangle: normalize(cross(normalize(b-a), normalize(c-a)))
 };

@@ -677,7 +677,7 @@ int32_t dfr_scene_add(dfr_ctx* h, const dfr_scene_desc* scene) {
     int rc = copy_scene_desc(scene, host, c->error);
     if (rc) return rc;
     auto cs = std::make_unique<CompiledScene>();
-    rc = compile_scene(host, *cs, c->light_pdf_mode, c->stream, c->error);
+    rc = compile_scene(host, {}, *cs, c->light_pdf_mode, c->stream, c->error);
     if (rc) return rc;
     int id = -1;
     for (size_t i = 0; i < c->scenes.size(); i++)

@@ -113,6 +113,16 @@ class GPURenderer:
         w, h = int(scene.camera[abi.CAMERA_LENGTH - 2]), int(scene.camera[abi.CAMERA_LENGTH - 1])
         self._c(self.lib.dfr_set_resolution(self.ctx, w, h))
 
+    def updateTransforms(self, object_to_world, rebuild=False):
+        """updateMatrix44Fs (AbstractSceneKernel.java:292-300): one 4x4
+        object-to-world matrix per primitive of the scene, shape (n, 4, 4) or
+        (n, 16), row major.  After setup the geometry moves at once (the BVH
+        is refitted, or rebuilt with rebuild=True)."""
+        import numpy as np
+        m = np.ascontiguousarray(np.asarray(object_to_world, dtype=np.float32).reshape(-1, 16))
+        self._xforms = m
+        self._c(self.lib.dfr_update_transforms(self.ctx, m.ctypes.data, m.shape[0], 1 if rebuild else 0))
+
     def setImage(self, width=None, height=None):
         """setImage(): resolution from the scene camera; setImage(w, h) explicit."""
         if width is None:

@@ -256,6 +256,27 @@ const char* dfr_last_error(const dfr_ctx* ctx);
  * Copies the description; nothing is compiled until dfr_setup(). */
 int32_t dfr_set_scene(dfr_ctx* ctx, const dfr_scene_desc* scene);
 
+/* updateMatrix44Fs(): M/renderer/gpu/AbstractSceneKernel.java:292-300, the
+ * primitives' object-to-world matrices of CompiledPrimitiveCache.toMatrix44Fs
+ * (M/scene/compiler/CompiledPrimitiveCache.java:323-350).  `object_to_world`
+ * holds 16 floats per primitive of the scene set by dfr_set_scene, row major
+ * (element (r, c) at 4r + c), in primitive order; `n_primitives` must equal
+ * the scene's primitive count (the reference ignores an array of another
+ * length; this returns DFR_E_ARGUMENT).  A primitive's record (vertices,
+ * sphere centre and radius, rectangle corners, plane frame) is in its object
+ * space; surface frames are built there and carried to world space through
+ * the normal matrix, as the reference does.  Matrices must be affine (last
+ * row 0 0 0 1) and invertible.  Spheres and rectangles take rotations,
+ * reflections, uniform scales and translations; meshes and planes any affine
+ * map.  A mesh carrying an area light takes rotations and translations only
+ * and a sphere light translations only; anything else is DFR_E_UNSUPPORTED.
+ * Before the scene is compiled the matrices are kept for dfr_setup; after,
+ * the world-space geometry is recomputed and uploaded at once, and the BVH
+ * keeps its topology with its boxes refitted (rebuild = 0) or is built
+ * again (rebuild = 1).  Refused (DFR_E_STATE) while a serving loop renders
+ * the scene.  dfr_set_scene resets every primitive to the identity. */
+int32_t dfr_update_transforms(dfr_ctx* ctx, const float* object_to_world, int32_t n_primitives, int32_t rebuild);
+
 /* setImage(): M/renderer/gpu/AbstractGPURenderer.java:417-440.  Sets the
  * film resolution; takes effect at the next setup/render. */
 int32_t dfr_set_resolution(dfr_ctx* ctx, int32_t width, int32_t height);

@@ -112,10 +112,22 @@ struct BvhNode {
     int64_t first, count;
 };
 
+// A primitive's object-to-world matrix and its inverse (Transform.java:
+// objectToWorld, worldToObject), row major.
+struct Xf {
+    float m[16], inv[16];
+    bool identity;
+};
+
 struct Scene {
     std::vector<Tri> tris;
     std::vector<int64_t> triOrder;
     std::vector<Prim> prims;
+    // with per-primitive transforms: the object-space records (tris and prims
+    // above then hold the world-space images) and the matrices
+    std::vector<Tri> objTris;
+    std::vector<Prim> objPrims;
+    std::vector<Xf> xf;
     std::vector<int> analytic;   // primitives that are not meshes
     std::vector<dfr_material> mats;
     std::vector<Light> lights;
@@ -434,14 +446,18 @@ static bool sceneClosestF(const Scene& sc, const Ray& ray, float tMin, float tMa
 // doCreateSurfaceIntersection / doCreateBarycentricCoordinates (:1075-1150);
 // SurfaceIntersection3F.transform flips S.w to agree with G.w
 // (M/geometry/SurfaceIntersection3F.java:372-392).
-Surface makeSurface(const Scene& sc, const Ray& ray, const Hit& h) {
+// The surface of a hit on primitives given in one space (tris, prims): the
+// frames and point in that space.  `flip` turns the shading normal to the
+// geometric normal's side.
+Surface makeSurfaceIn(const std::vector<Tri>& tris, const std::vector<Prim>& prims, const Ray& ray, const Hit& h,
+                      bool flip) {
     Surface s;
     s.t = h.t; s.prim = h.prim; s.tri = h.tri; s.rayDir = ray.d;
     s.p = ray.o + ray.d * h.t;
     s.tu = s.tv = 0.0f;
-    const Prim& p = sc.prims[h.prim];
+    const Prim& p = prims[h.prim];
     if (h.tri >= 0) {
-        const Tri& tr = sc.tris[h.tri];
+        const Tri& tr = tris[h.tri];
         V3 eAB = tr.b - tr.a, eCA = tr.a - tr.c;
         V3 d1 = cross(eAB, eCA);
         float det = dot(ray.d, d1);
@@ -493,6 +509,54 @@ Surface makeSurface(const Scene& sc, const Ray& ray, const Hit& h) {
         s.g.w = normalize(V3(p.data[9], p.data[10], p.data[11]));
         s.s = s.g;
     }
+    if (flip && dot(s.s.w, s.g.w) < 0.0f) s.s.w = -s.s.w;
+    return s;
+}
+
+// Point3F.transformAndDivide (the last row is 0 0 0 1 here, the divide exact)
+V3 xfPoint(const float* m, V3 p) {
+    return V3(m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+              m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]);
+}
+// Vector3F.transform: the upper 3x3 times v
+V3 xfVector(const float* m, V3 v) {
+    return V3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z, m[8] * v.x + m[9] * v.y + m[10] * v.z);
+}
+// Vector3F.transformTranspose: the transposed upper 3x3 times v
+V3 xfVectorT(const float* m, V3 v) {
+    return V3(m[0] * v.x + m[4] * v.y + m[8] * v.z, m[1] * v.x + m[5] * v.y + m[9] * v.z, m[2] * v.x + m[6] * v.y + m[10] * v.z);
+}
+// OrthonormalBasis33F.transformTranspose: each vector through the transposed
+// inverse, normalised
+ONB xfBasis(const float* inv, const ONB& b) {
+    ONB r;
+    r.u = normalize(xfVectorT(inv, b.u));
+    r.v = normalize(xfVectorT(inv, b.v));
+    r.w = normalize(xfVectorT(inv, b.w));
+    return r;
+}
+
+// Intersection(primitive, surfaceIntersectionObjectSpace) (Intersection.java:
+// 67-74): a transformed primitive's surface is computed in its object space
+// and carried to world space by SurfaceIntersection3F.transform
+// (SurfaceIntersection3F.java:372-392): both frames through the normal
+// matrix, the shading normal turned to the geometric one's side there, the
+// point through the object-to-world matrix.  The hit itself is found on the
+// world-space image of the primitive (an affine image: the same point of the
+// surface); the object-space ray is the world ray mapped by the inverse
+// without renormalisation, so its parameter t names the same point.
+Surface makeSurface(const Scene& sc, const Ray& ray, const Hit& h) {
+    if (sc.xf.empty() || sc.xf[h.prim].identity) return makeSurfaceIn(sc.tris, sc.prims, ray, h, true);
+    const Xf& x = sc.xf[h.prim];
+    Ray ro;
+    ro.o = xfPoint(x.inv, ray.o);
+    ro.d = xfVector(x.inv, ray.d);
+    Surface so = makeSurfaceIn(sc.objTris, sc.objPrims, ro, h, false);
+    Surface s = so;
+    s.rayDir = ray.d;
+    s.p = xfPoint(x.m, so.p);
+    s.g = xfBasis(x.inv, so.g);
+    s.s = xfBasis(x.inv, so.s);
     if (dot(s.s.w, s.g.w) < 0.0f) s.s.w = -s.s.w;
     return s;
 }
@@ -1782,6 +1846,83 @@ void orc_scene_destroy(orc_scene* s) { delete s; }
 int32_t orc_scene_set_camera(orc_scene* s, const float* camera) {
     if (!s || !camera) return -1;
     std::memcpy(s->s.cam, camera, sizeof(s->s.cam));
+    return 0;
+}
+
+// updateMatrix44Fs (AbstractSceneKernel.java:292-300): one row-major 4x4
+// object-to-world matrix per primitive.  The primitives' records as created
+// are their object-space forms; tris and prims become their world-space
+// images (for intersection and light sampling) and the BVH is built again.
+int32_t orc_scene_set_transforms(orc_scene* h, const float* m, int32_t n) {
+    if (!h || !m) return -1;
+    Scene& sc = h->s;
+    if (n != (int32_t)sc.prims.size()) return -1;
+    if (sc.objPrims.empty() && !sc.prims.empty()) { sc.objTris = sc.tris; sc.objPrims = sc.prims; }
+    sc.xf.assign(n, Xf());
+    for (int p = 0; p < n; p++) {
+        Xf& x = sc.xf[p];
+        std::memcpy(x.m, m + 16 * p, sizeof(x.m));
+        if (x.m[12] != 0.0f || x.m[13] != 0.0f || x.m[14] != 0.0f || x.m[15] != 1.0f) return -1;
+        // inverse by Gauss-Jordan elimination with partial pivoting, in double
+        double a[4][8];
+        for (int r = 0; r < 4; r++)
+            for (int c = 0; c < 8; c++) a[r][c] = c < 4 ? x.m[4 * r + c] : (c - 4 == r ? 1.0 : 0.0);
+        for (int c = 0; c < 4; c++) {
+            int piv = c;
+            for (int r = c + 1; r < 4; r++) if (std::fabs(a[r][c]) > std::fabs(a[piv][c])) piv = r;
+            if (a[piv][c] == 0.0) return -1;
+            for (int k = 0; k < 8; k++) std::swap(a[c][k], a[piv][k]);
+            const double d = a[c][c];
+            for (int k = 0; k < 8; k++) a[c][k] /= d;
+            for (int r = 0; r < 4; r++) {
+                if (r == c) continue;
+                const double f = a[r][c];
+                for (int k = 0; k < 8; k++) a[r][k] -= f * a[c][k];
+            }
+        }
+        for (int r = 0; r < 4; r++)
+            for (int c = 0; c < 4; c++) x.inv[4 * r + c] = (float)a[r][c + 4];
+        x.identity = true;
+        for (int k = 0; k < 16; k++) x.identity = x.identity && x.m[k] == ((k % 5) == 0 ? 1.0f : 0.0f);
+    }
+    for (size_t t = 0; t < sc.tris.size(); t++) {
+        const Tri& ot = sc.objTris[t];
+        Tri& wt = sc.tris[t];
+        wt = ot;
+        const Xf& x = sc.xf[ot.prim];
+        if (x.identity) continue;
+        wt.a = xfPoint(x.m, ot.a); wt.b = xfPoint(x.m, ot.b); wt.c = xfPoint(x.m, ot.c);
+        wt.na = normalize(xfVectorT(x.inv, ot.na));
+        wt.nb = normalize(xfVectorT(x.inv, ot.nb));
+        wt.nc = normalize(xfVectorT(x.inv, ot.nc));
+        wt.ng = normalize(xfVectorT(x.inv, ot.ng));
+    }
+    for (int p = 0; p < n; p++) {
+        const Prim& op = sc.objPrims[p];
+        Prim& wp = sc.prims[p];
+        wp = op;
+        const Xf& x = sc.xf[p];
+        if (x.identity) continue;
+        if (op.shape == DFR_SHAPE_SPHERE) {
+            const V3 c = xfPoint(x.m, V3(op.data[0], op.data[1], op.data[2]));
+            const float sx = length(V3(x.m[0], x.m[4], x.m[8]));  // the uniform scale
+            wp.data[0] = c.x; wp.data[1] = c.y; wp.data[2] = c.z; wp.data[3] = op.data[3] * sx;
+        } else if (op.shape == DFR_SHAPE_RECTANGLE) {
+            for (int k = 0; k < 3; k++) {
+                const V3 q = xfPoint(x.m, V3(op.data[3 * k], op.data[3 * k + 1], op.data[3 * k + 2]));
+                wp.data[3 * k] = q.x; wp.data[3 * k + 1] = q.y; wp.data[3 * k + 2] = q.z;
+            }
+            wp.rectNormal = normalize(xfVectorT(x.inv, op.rectNormal));
+        } else if (op.shape == DFR_SHAPE_PLANE) {
+            const V3 o = xfPoint(x.m, V3(op.data[0], op.data[1], op.data[2]));
+            wp.data[0] = o.x; wp.data[1] = o.y; wp.data[2] = o.z;
+            for (int k = 1; k < 4; k++) {
+                const V3 v = normalize(xfVectorT(x.inv, V3(op.data[3 * k], op.data[3 * k + 1], op.data[3 * k + 2])));
+                wp.data[3 * k] = v.x; wp.data[3 * k + 1] = v.y; wp.data[3 * k + 2] = v.z;
+            }
+        }
+    }
+    bvhBuild(sc);
     return 0;
 }
 

@@ -26,6 +26,15 @@
  *   Perez sun       attenuated spectrum against Preetham et al.'s
  *                   transmittance formulas in float64, colour against an
  *                   independent CIE integration and sRGB matrix
+ *   transforms      identity matrices change no bit; a rigid motion of
+ *                   every primitive and the camera leaves each path the same
+ *                   (frames built in object space), which moving the
+ *                   vertices themselves does not; the shading normal of a
+ *                   non-uniformly scaled mesh equals the normal matrix's
+ *                   image in closed form (tests/test_transforms.py).  The hit
+ *                   is found on the primitive's world-space image, with the
+ *                   0.001 minimum distance in world space (the reference
+ *                   applies it to the object-space ray)
  *   image stage     textbook sRGB bytes (linear 0.5 -> 188, 0.18 -> 118),
  *                   tone curves at closed-form points (Reinhard 1 -> 1/2,
  *                   white point 4 -> 1, 1 - e^-ln2 = 1/2, ACES fit at 1),
@@ -59,6 +68,10 @@ typedef struct orc_params {
 orc_scene* orc_scene_create(const dfr_scene_desc* desc);
 void orc_scene_destroy(orc_scene* scene);
 int32_t orc_scene_set_camera(orc_scene* scene, const float* camera19);
+/* Per-primitive object-to-world matrices (16 floats each, row major); the
+ * records the scene was created with are the object-space forms.  Returns
+ * -1 on a count mismatch, a non-affine or a singular matrix. */
+int32_t orc_scene_set_transforms(orc_scene* scene, const float* m, int32_t n);
 
 /* Per-pixel sum and sum of squares of the accepted samples and their count
  * (unfiltered film, one bin per pixel).  Arrays are width*height*3 doubles

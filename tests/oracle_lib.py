@@ -45,6 +45,8 @@ def load():
     lib.orc_scene_create.argtypes = [P(abi.SceneDesc)]
     lib.orc_scene_destroy.argtypes = [vp]
     lib.orc_scene_set_camera.argtypes = [vp, vp]
+    lib.orc_scene_set_transforms.argtypes = [vp, vp, ctypes.c_int32]
+    lib.orc_scene_set_transforms.restype = ctypes.c_int32
     lib.orc_render.argtypes = [vp, P(Params), vp, vp, vp]
     lib.orc_trace_f32.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
     lib.orc_trace_f64.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp, vp, vp]
@@ -97,6 +99,13 @@ class OracleScene:
         """The camera record later renders use (like updateCamera)."""
         cam = np.ascontiguousarray(camera, np.float32)
         self.lib.orc_scene_set_camera(self.h, _ptr(cam))
+
+    def set_transforms(self, object_to_world):
+        """Per-primitive object-to-world matrices, (n, 4, 4) or (n, 16)."""
+        m = np.ascontiguousarray(np.asarray(object_to_world, np.float32).reshape(-1, 16))
+        rc = self.lib.orc_scene_set_transforms(self.h, _ptr(m), m.shape[0])
+        if rc != 0:
+            raise ValueError("oracle refused the transforms")
 
     def close(self):
         if self.h:
