@@ -39,6 +39,7 @@ KEY_WF_SORT_KEYS = 16
 KEY_BVH_WIDTH = 17
 KEY_USE_GRAPHS = 18
 KEY_RAY_SAMPLE = 19
+KEY_FILM_FILTER = 20
 KEY_MAXIMUM_DISTANCE = 101
 
 E_OK, E_NULL, E_ARGUMENT, E_STATE, E_DEVICE, E_MEMORY, E_STOPPED, E_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6, -7
@@ -135,6 +136,8 @@ ENTRY_POINTS = [
     ("dfr_update_camera", c_int32, [_c_void_p, P(c_float)]),
     ("dfr_read_film", c_int32, [_c_void_p, P(c_float), P(c_int32)]),
     ("dfr_read_rgba8", c_int32, [_c_void_p, P(ctypes.c_uint8)]),
+    ("dfr_read_film_xyz", c_int32, [_c_void_p, _c_void_p]),
+    ("dfr_read_film_filtered", c_int32, [_c_void_p, _c_void_p]),
     ("dfr_get_stats", c_int32, [_c_void_p, P(Stats)]),
     ("dfr_trace_rays", c_int32, [_c_void_p, _c_void_p, c_int64, c_int32, _c_void_p, _c_void_p, _c_void_p, c_int32]),
     ("dfr_trace_rays_noalloc", c_int32, [_c_void_p, _c_void_p, c_int64, c_int32, _c_void_p, _c_void_p, _c_void_p]),

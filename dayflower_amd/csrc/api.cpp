@@ -844,6 +844,69 @@ int Ctx::ensure_overflow(int64_t threads) {
     return DFR_OK;
 }
 
+// ColorSpaceF.S_R_G_B (M/color/ColorSpaceF.java:85, 116-127, 328-392) and
+// GaussianFilter2F(2, 2, 2)'s 16 x 16 table (M/filter/Filter2F.java:119-136,
+// M/filter/GaussianFilter2F.java:51-53, 117-119, 142-144), in float as the
+// reference evaluates them (exp and pow in double, rounded).
+static void make_film_consts(FilmConsts& f) {
+    auto fpow = [](float a, float b) { return (float)std::pow((double)a, (double)b); };
+    const float bp = 0.00304f, g = 2.4f;
+    const float xR = 0.6400f, yR = 0.3300f, xG = 0.3000f, yG = 0.6000f, xB = 0.1500f, yB = 0.0600f;
+    const float xW = 0.31271f, yW = 0.32902f;
+    f.break_point = bp;
+    f.gamma_recip = 1.0f / g;
+    f.slope = 1.0f / (g / fpow(bp, 1.0f / g - 1.0f) - g * bp + bp);
+    f.slope_match = g * f.slope / fpow(bp, 1.0f / g - 1.0f);
+    f.segment_offset = f.slope_match * fpow(bp, 1.0f / g) - f.slope * bp;
+    const float zR = 1.0f - (xR + yR), zG = 1.0f - (xG + yG), zB = 1.0f - (xB + yB), zW = 1.0f - (xW + yW);
+    // rows of XYZ -> RGB: each primary's row is the cross product of the other
+    // two primaries' chromaticities, scaled to map the white point to one
+    const float row[3][3] = {{yG * zB - yB * zG, xB * zG - xG * zB, xG * yB - xB * yG},
+                             {yB * zR - yR * zB, xR * zB - xB * zR, xB * yR - xR * yB},
+                             {yR * zG - yG * zR, xG * zR - xR * zG, xR * yG - xG * yR}};
+    float* m = f.xyz_to_rgb;
+    for (int r = 0; r < 3; r++) {
+        const float wr = (row[r][0] * xW + row[r][1] * yW + row[r][2] * zW) / yW;
+        for (int k = 0; k < 3; k++) m[3 * r + k] = row[r][k] / wr;
+        m[9 + r] = wr;
+    }
+    // RGB -> XYZ, the inverse by cofactors, stored by primary (X, Y, Z of red, ...)
+    const float a = m[0] * (m[4] * m[8] - m[7] * m[5]);
+    const float b = m[1] * (m[3] * m[8] - m[6] * m[5]);
+    const float cc = m[2] * (m[3] * m[7] - m[6] * m[4]);
+    const float sc = 1.0f / (a - b + cc);
+    float* r = f.rgb_to_xyz;
+    r[0] = sc * (m[4] * m[8] - m[5] * m[7]); r[1] = sc * (m[5] * m[6] - m[3] * m[8]); r[2] = sc * (m[3] * m[7] - m[4] * m[6]);
+    r[3] = sc * (m[2] * m[7] - m[1] * m[8]); r[4] = sc * (m[0] * m[8] - m[2] * m[6]); r[5] = sc * (m[1] * m[6] - m[0] * m[7]);
+    r[6] = sc * (m[1] * m[5] - m[2] * m[4]); r[7] = sc * (m[2] * m[3] - m[0] * m[5]); r[8] = sc * (m[0] * m[4] - m[1] * m[3]);
+    r[9] = xW; r[10] = yW; r[11] = zW;
+    const float radius = 2.0f, falloff = 2.0f;
+    const float edge = (float)std::exp((double)(-falloff * radius * radius));
+    auto gauss = [&](float v) { return std::max(0.0f, (float)std::exp((double)(-falloff * v * v)) - edge); };
+    for (int i = 0, y = 0; y < 16; y++)
+        for (int x = 0; x < 16; x++)
+            f.table[i++] = gauss((x + 0.5f) * radius * (1.0f / 16)) * gauss((y + 0.5f) * radius * (1.0f / 16));
+}
+
+int Ctx::ensure_filtered_film() {
+    if (!film_consts.ptr) {
+        FilmConsts fc;
+        make_film_consts(fc);
+        if (film_consts.alloc(sizeof(FilmConsts)) != hipSuccess) return fail(DFR_E_MEMORY, "film constants");
+        hipError_t e = hipMemcpyAsync(film_consts.ptr, &fc, sizeof(fc), hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_fail(e, "film constants");
+    }
+    const int64_t n = (int64_t)width * height;
+    if (xyz_pixels != n) {
+        if (xyz_film.alloc((size_t)n * sizeof(float4)) != hipSuccess) return fail(DFR_E_MEMORY, "XYZ film");
+        hipError_t e = hipMemsetAsync(xyz_film.ptr, 0, (size_t)n * sizeof(float4), stream);
+        if (e != hipSuccess) return hip_fail(e, "XYZ film clear");
+        xyz_pixels = n;
+    }
+    return DFR_OK;
+}
+
 int Ctx::ensure_film(int64_t pixels) {
     size_t need = (size_t)pixels * sizeof(float4);
     if (film.bytes < need && film.alloc(need) != hipSuccess) return fail(DFR_E_MEMORY, "film");
@@ -1186,6 +1249,10 @@ int32_t dfr_set_i32(dfr_ctx* h, int32_t key, int32_t v) {
         case DFR_KEY_RAY_SAMPLE:
             if (v < 0) return c->fail(DFR_E_ARGUMENT, "ray sample stride must be >= 0");
             c->ray_sample_stride = v; break;
+        case DFR_KEY_FILM_FILTER:
+            if (v != 0 && v != 1) return c->fail(DFR_E_ARGUMENT, "film filter is 0 (running mean) or 1 (Gaussian XYZ)");
+            if (v != c->film_filter) c->clearing = true;  // both films start again
+            c->film_filter = v; break;
         default: return c->fail(DFR_E_ARGUMENT, "unknown integer key");
     }
     return DFR_OK;
@@ -1214,6 +1281,7 @@ int32_t dfr_get_i32(const dfr_ctx* h, int32_t key, int32_t* v) {
         case DFR_KEY_BVH_WIDTH: *v = BVH_WIDTH; break;
         case DFR_KEY_USE_GRAPHS: *v = c->use_graphs; break;
         case DFR_KEY_RAY_SAMPLE: *v = c->ray_sample_stride; break;
+        case DFR_KEY_FILM_FILTER: *v = c->film_filter; break;
         default: return DFR_E_ARGUMENT;
     }
     return DFR_OK;
@@ -1258,6 +1326,7 @@ static int32_t setup_impl(Ctx* c, int32_t compile_scene_flag) {
     hipError_t e = hipMemsetAsync(c->film.ptr, 0, (size_t)c->width * c->height * sizeof(float4), c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "film clear");
+    c->xyz_pixels = -1;
     c->render_pass = 0;
     c->clearing = false;
     c->film_dirty = false;
@@ -1285,51 +1354,84 @@ int32_t dfr_render(dfr_ctx* h, int32_t passes) {
     if (c->clearing) {
         e = hipMemsetAsync(c->film.ptr, 0, (size_t)npix * sizeof(float4), c->stream);
         if (e != hipSuccess) return c->hip_fail(e, "film clear");
+        c->xyz_pixels = -1;  // the XYZ film is cleared when next used
         c->render_pass = 0;
         c->clearing = false;
     }
     c->stop.store(false);
     const int spp = passes * c->spp_per_pass;
-    DevJob job = c->main_job(spp, c->render_pass * c->spp_per_pass);
-    int rc = c->upload_jobs({job});
-    if (rc) return rc;
+    const int first_sample = c->render_pass * c->spp_per_pass;
     if (c->counters.alloc(128) != hipSuccess) return c->fail(DFR_E_MEMORY, "counters");
     hipMemsetAsync(c->counters.ptr, 0, 64, c->stream);
-    hipEventRecord(c->ev0, c->stream);
     dfr_stats st{};
     bool plain = c->kernel == 1 || c->algorithm != DFR_ALGO_PATH_TRACING;
+    int rc;
     if (plain) {
-        // one launch per pass: each launch adds spp_per_pass samples to every pixel
         int64_t threads = ((npix + 255) / 256) * 256;
         if ((rc = c->ensure_overflow(threads))) return rc;
-        for (int p = 0; p < passes; p++) {
-            DevJob jp = c->main_job(c->spp_per_pass, (c->render_pass + p) * c->spp_per_pass);
-            e = hipMemcpyAsync(c->job_buf.ptr, &jp, sizeof(DevJob), hipMemcpyHostToDevice, c->stream);
-            if (e != hipSuccess) return c->hip_fail(e, "job upload");
-            launch_megakernel((const DevScene*)c->scene_table.ptr, (const DevJob*)c->job_buf.ptr, (float4*)c->film.ptr,
-                              (int*)c->overflow.ptr, (int)npix, 256, c->any_ext_features(), (int64_t*)c->counters.ptr,
-                              c->stream);
-            if (c->stop.load()) break;
+    }
+    // Renders samples [base, base + n) into `target`: the (sum, count) film,
+    // or per-sample records (sample_film) of a chunk starting at base.
+    auto run = [&](int n, int base, float4* target, bool sample_film) -> int {
+        if (plain) {
+            // one launch per pass: each launch adds spp_per_pass samples to every pixel
+            for (int p = 0; p < n / c->spp_per_pass; p++) {
+                DevJob jp = c->main_job(c->spp_per_pass, base + p * c->spp_per_pass);
+                jp.sample_film = sample_film ? 1 : 0;
+                jp.film_sample_base = base;
+                hipError_t ee = hipMemcpyAsync(c->job_buf.ptr, &jp, sizeof(DevJob), hipMemcpyHostToDevice, c->stream);
+                if (ee != hipSuccess) return c->hip_fail(ee, "job upload");
+                launch_megakernel((const DevScene*)c->scene_table.ptr, (const DevJob*)c->job_buf.ptr, target,
+                                  (int*)c->overflow.ptr, (int)npix, 256, c->any_ext_features(), (int64_t*)c->counters.ptr,
+                                  c->stream);
+                if (c->stop.load()) break;
+            }
+            return DFR_OK;
         }
-        hipEventRecord(c->ev1, c->stream);
-        e = hipEventSynchronize(c->ev1);
-        if (e != hipSuccess) return c->hip_fail(e, "megakernel");
+        DevJob job = c->main_job(n, base);
+        job.sample_film = sample_film ? 1 : 0;
+        job.film_sample_base = base;
+        int r2 = c->upload_jobs({job});
+        if (r2) return r2;
+        Counters out{};
+        if ((r2 = c->run_wavefront(1, job.work_count, target, &out))) return r2;
+        st.rays_closest += out.rays_closest; st.rays_any += out.rays_any; st.bounces += out.bounces;
+        st.extend_seconds += c->prof_stats.extend_seconds; st.shade_seconds += c->prof_stats.shade_seconds;
+        st.shadow_seconds += c->prof_stats.shadow_seconds; st.extend_launches += c->prof_stats.extend_launches;
+        st.extend_rays += c->prof_stats.extend_rays; st.extend_nodes += c->prof_stats.extend_nodes;
+        st.extend_tris += c->prof_stats.extend_tris; st.shadow_rays += c->prof_stats.shadow_rays;
+        st.shadow_nodes += c->prof_stats.shadow_nodes; st.shadow_tris += c->prof_stats.shadow_tris;
+        return DFR_OK;
+    };
+    if (c->upload_jobs({c->main_job(c->spp_per_pass, first_sample)})) return c->fail(DFR_E_MEMORY, "job table");
+    hipEventRecord(c->ev0, c->stream);
+    if (c->film_filter) {
+        // CPURenderer's Gaussian film: render chunks of samples into
+        // per-sample records, then gather each chunk into the XYZ film
+        if ((rc = c->ensure_filtered_film())) return rc;
+        const int64_t rec = npix * (int64_t)sizeof(float4);
+        int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(spp, SAMPLE_FILM_BYTES / rec));
+        chunk = std::max(c->spp_per_pass, chunk / c->spp_per_pass * c->spp_per_pass);
+        if (c->sample_records.alloc((size_t)rec * chunk) != hipSuccess) return c->fail(DFR_E_MEMORY, "sample records");
+        for (int done = 0; done < spp && !c->stop.load(); done += chunk) {
+            const int n = std::min(chunk, spp - done);
+            hipError_t ee = hipMemsetAsync(c->sample_records.ptr, 0, (size_t)rec * n, c->stream);
+            if (ee != hipSuccess) return c->hip_fail(ee, "sample records clear");
+            if ((rc = run(n, first_sample + done, (float4*)c->sample_records.ptr, true))) return rc;
+            launch_film_gauss((const float4*)c->sample_records.ptr, c->width, c->height, n, c->seed, first_sample + done,
+                              (const FilmConsts*)c->film_consts.ptr, (float4*)c->xyz_film.ptr, (float4*)c->film.ptr,
+                              c->stream);
+        }
+    } else if ((rc = run(spp, first_sample, (float4*)c->film.ptr, false))) {
+        return rc;
+    }
+    hipEventRecord(c->ev1, c->stream);
+    e = hipEventSynchronize(c->ev1);
+    if (e != hipSuccess) return c->hip_fail(e, plain ? "megakernel" : "wavefront");
+    if (plain) {
         int64_t cnt[2];
         hipMemcpy(cnt, c->counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost);
         st.rays_closest = cnt[0]; st.rays_any = cnt[1];
-    } else {
-        Counters out{};
-        rc = c->run_wavefront(1, job.work_count, (float4*)c->film.ptr, &out);
-        if (rc) return rc;
-        hipEventRecord(c->ev1, c->stream);
-        e = hipEventSynchronize(c->ev1);
-        if (e != hipSuccess) return c->hip_fail(e, "wavefront");
-        st.rays_closest = out.rays_closest; st.rays_any = out.rays_any; st.bounces = out.bounces;
-        st.extend_seconds = c->prof_stats.extend_seconds; st.shade_seconds = c->prof_stats.shade_seconds;
-        st.shadow_seconds = c->prof_stats.shadow_seconds; st.extend_launches = c->prof_stats.extend_launches;
-        st.extend_rays = c->prof_stats.extend_rays; st.extend_nodes = c->prof_stats.extend_nodes;
-        st.extend_tris = c->prof_stats.extend_tris; st.shadow_rays = c->prof_stats.shadow_rays;
-        st.shadow_nodes = c->prof_stats.shadow_nodes; st.shadow_tris = c->prof_stats.shadow_tris;
     }
     float ms = 0.0f;
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
@@ -1382,6 +1484,39 @@ int32_t dfr_read_film(dfr_ctx* h, float* rgb, int32_t* count) {
         if (rgb) { rgb[3 * i] = f[i].x * inv; rgb[3 * i + 1] = f[i].y * inv; rgb[3 * i + 2] = f[i].z * inv; }
         if (count) count[i] = (int32_t)w;
     }
+    return DFR_OK;
+}
+
+int32_t dfr_read_film_xyz(dfr_ctx* h, float* xyzw) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (!xyzw) return c->fail(DFR_E_NULL, "xyzw == null");
+    if (!c->is_setup) return c->fail(DFR_E_STATE, "setup before read");
+    if (!c->film_filter) return c->fail(DFR_E_STATE, "the Gaussian film is off (DFR_KEY_FILM_FILTER)");
+    int rc = c->ensure_filtered_film();
+    if (rc) return rc;
+    const int64_t n = (int64_t)c->width * c->height;
+    hipError_t e = hipMemcpyAsync(xyzw, c->xyz_film.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "XYZ film readback");
+    return DFR_OK;
+}
+
+int32_t dfr_read_film_filtered(dfr_ctx* h, float* rgb) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (!rgb) return c->fail(DFR_E_NULL, "rgb == null");
+    if (!c->is_setup) return c->fail(DFR_E_STATE, "setup before read");
+    if (!c->film_filter) return c->fail(DFR_E_STATE, "the Gaussian film is off (DFR_KEY_FILM_FILTER)");
+    int rc = c->ensure_filtered_film();
+    if (rc) return rc;
+    const int64_t n = (int64_t)c->width * c->height;
+    if (c->film_rgb.alloc((size_t)n * 3 * sizeof(float)) != hipSuccess) return c->fail(DFR_E_MEMORY, "film RGB");
+    launch_film_rgb((const float4*)c->xyz_film.ptr, (float*)c->film_rgb.ptr, (int)n, (const FilmConsts*)c->film_consts.ptr,
+                    c->stream);
+    hipError_t e = hipMemcpyAsync(rgb, c->film_rgb.ptr, n * 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "filtered film readback");
     return DFR_OK;
 }
 

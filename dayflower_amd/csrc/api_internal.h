@@ -83,6 +83,9 @@ struct WavefrontBuffers;
 // Allocates (or grows) the 17 wavefront buffers of a path pool of `capacity`
 // paths into `mem` and points `wf` at them; false when an allocation fails.
 bool alloc_wavefront_buffers(WavefrontBuffers& wf, DeviceBuffer* mem, int capacity);
+// Bytes of per-sample records one chunk of a filtered render may use.
+constexpr int64_t SAMPLE_FILM_BYTES = int64_t(2) << 30;
+
 int make_xforms(const SceneHost& in, const std::vector<float>& m, std::vector<Xform>& out, std::string& err);
 int compile_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledScene& out, int light_pdf_mode,
                   hipStream_t s, std::string& err);
@@ -146,6 +149,15 @@ struct Ctx {
     std::vector<uintptr_t> graph_key;
     int replay_iterations(const std::vector<uintptr_t>& key, const DevScene* scenes, const DevJob* jobs, int n_jobs,
                           float4* film, const WavefrontParams& p, int iterations);
+
+    // CPURenderer's Gaussian XYZ film (DFR_KEY_FILM_FILTER)
+    int film_filter = 0;
+    DeviceBuffer sample_records;   // per-sample records of one chunk of a filtered render
+    DeviceBuffer xyz_film;         // X, Y, Z, filter weight sum per pixel
+    DeviceBuffer film_consts;      // FilmConsts
+    DeviceBuffer film_rgb;         // filmRender output staging
+    int64_t xyz_pixels = -1;       // pixels xyz_film holds, -1 when stale
+    int ensure_filtered_film();
 
     dfr_stats stats{};
     dfr_stats prof_stats{};   // instrumentation of the last wavefront run

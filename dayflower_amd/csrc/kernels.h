@@ -24,6 +24,11 @@ struct DevJob {
     int64_t work_count;       // width * height * spp
     float cam[19];
     int32_t scene;            // index into the scene table
+    // 1: the film holds one record per sample, sample k of pixel p at
+    // film_offset + (k - film_sample_base) * width * height + p (the
+    // filtered-film mode); 0: one running sum per pixel
+    int32_t sample_film;
+    int32_t film_sample_base;
 };
 
 // Wavefront path state (structure of arrays).
@@ -103,6 +108,23 @@ void launch_trace_rays(const DevScene* scene, const float4* rays, int64_t n, int
 void launch_cast_primary(const DevScene* scene, const DevJob* job, float* rays, int32_t* prim, int64_t* tri,
                          float* t, int* overflow, int n_pixels, hipStream_t s);
 void launch_rgba8(const float4* film, uint8_t* out, int n, int tone_mapper, hipStream_t s);
+
+// CPURenderer's film (PixelImageF with GaussianFilter2F(2, 2, 2), sRGB).
+// FilmConsts carries the 16 x 16 filter table, the colour-space arrays
+// (ColorSpaceF storage order, 12 floats each) and the gamma constants.
+struct FilmConsts {
+    float table[256];
+    float rgb_to_xyz[12], xyz_to_rgb[12];
+    float break_point, slope, slope_match, segment_offset, gamma_recip;
+};
+// Splats the per-sample records of `spp` samples (sample_base ..) of a
+// width x height job into the XYZ film (X, Y, Z, weight sum per pixel), in
+// the reference's order per pixel, and adds each pixel's own samples to the
+// (sum, count) film.
+void launch_film_gauss(const float4* samples, int width, int height, int spp, uint64_t seed, int sample_base,
+                       const FilmConsts* consts, float4* xyz, float4* film, hipStream_t s);
+// filmRender(): XYZ film -> gamma-encoded RGB (3 floats per pixel).
+void launch_film_rgb(const float4* xyz, float* rgb, int n, const FilmConsts* consts, hipStream_t s);
 
 // Ray queries with the traversal overflow in the library's static device
 // array (no allocation).  noalloc_scratch_init takes the array's address on

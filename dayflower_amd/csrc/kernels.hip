@@ -361,13 +361,18 @@ __global__ __launch_bounds__(BLOCK) void k_megakernel(const DevScene* __restrict
             // AbstractCPURenderer.java:229: drop NaN/Inf and negative-luminance samples
             float lum = 0.212671f * L.x + 0.715160f * L.y + 0.072169f * L.z;
             if (!(isfinite(L.x) && isfinite(L.y) && isfinite(L.z)) || lum < -1.0e-5f) continue;
+            if (job.sample_film)
+                film[job.film_offset + (int64_t)(sample - (uint32_t)job.film_sample_base) * n_pixels + pix] =
+                    make_float4(L.x, L.y, L.z, 1.0f);
             sum = sum + L;
             count++;
         }
-        int64_t fi = job.film_offset + pix;
-        float4 f = film[fi];
-        f.x += sum.x; f.y += sum.y; f.z += sum.z; f.w += (float)count;
-        film[fi] = f;
+        if (!job.sample_film) {
+            int64_t fi = job.film_offset + pix;
+            float4 f = film[fi];
+            f.x += sum.x; f.y += sum.y; f.z += sum.z; f.w += (float)count;
+            film[fi] = f;
+        }
     }
     // per-wave reduction of the ray counters, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
@@ -554,6 +559,92 @@ void launch_rgba8(const float4* film, uint8_t* out, int n, int tone_mapper, hipS
 }
 
 // ---------------------------------------------------------------------------
+// CPURenderer's film: PixelImageF.filmAddColorXYZ with GaussianFilter2F(2, 2,
+// 2) (M/image/PixelImageF.java:569-617) and filmRender (:691-705) in
+// ColorSpaceF.S_R_G_B.  The reference splats each sample into every pixel
+// whose centre lies within two pixels of it; here each output pixel gathers
+// the samples of its 5 x 5 neighbourhood, in the order the reference adds
+// them to that pixel (pass by pass, samples in row order), from per-sample
+// records (rgb, 1 when the sample exists).  A sample's position is its
+// pixel plus the camera offset drawn from its stream's dimensions 0 and 1.
+// Products and sums are kept unfused so each term rounds as the
+// reference's does.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_film_gauss(const float4* __restrict__ smp, int W, int H, int spp, uint64_t seed,
+                                                    int sample_base, const FilmConsts* __restrict__ fc,
+                                                    float4* __restrict__ xyz, float4* __restrict__ film) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= W || y >= H) return;
+    const int64_t npix = (int64_t)W * H;
+    const int p = y * W + x;
+    const float* m = fc->rgb_to_xyz;
+    float4 acc = xyz[p];
+    float4 own = film[p];
+    for (int k = 0; k < spp; k++) {
+        const uint32_t sample = (uint32_t)(sample_base + k);
+        const float4* rec = smp + (int64_t)k * npix;
+        for (int qy = max(y - 2, 0); qy <= min(y + 2, H - 1); qy++) {
+            for (int qx = max(x - 2, 0); qx <= min(x + 2, W - 1); qx++) {
+                const int q = qy * W + qx;
+                const float4 r = rec[q];
+                if (r.w == 0.0f) continue;
+                const Rng rng = make_rng(seed, (uint32_t)q, sample);
+                const float dx = __fadd_rn((float)qx, rng.get(0)) - 0.5f;
+                const float dy = __fadd_rn((float)qy, rng.get(1)) - 0.5f;
+                const int x0 = (int)fmaxf(ceilf(dx - 2.0f), 0.0f), x1 = (int)fminf(floorf(dx + 2.0f), (float)(W - 1));
+                const int y0 = (int)fmaxf(ceilf(dy - 2.0f), 0.0f), y1 = (int)fminf(floorf(dy + 2.0f), (float)(H - 1));
+                if (x < x0 || x > x1 || y < y0 || y > y1) continue;
+                const int ox = min((int)floorf(fabsf(__fmul_rn(__fmul_rn((float)x - dx, 0.5f), 16.0f))), 15);
+                const int oy = min((int)floorf(fabsf(__fmul_rn(__fmul_rn((float)y - dy, 0.5f), 16.0f))), 15);
+                const float w = fc->table[oy * 16 + ox];
+                const float X = __fadd_rn(__fadd_rn(__fmul_rn(m[0], r.x), __fmul_rn(m[3], r.y)), __fmul_rn(m[6], r.z));
+                const float Y = __fadd_rn(__fadd_rn(__fmul_rn(m[1], r.x), __fmul_rn(m[4], r.y)), __fmul_rn(m[7], r.z));
+                const float Z = __fadd_rn(__fadd_rn(__fmul_rn(m[2], r.x), __fmul_rn(m[5], r.y)), __fmul_rn(m[8], r.z));
+                acc.x = __fadd_rn(acc.x, __fmul_rn(X, w));
+                acc.y = __fadd_rn(acc.y, __fmul_rn(Y, w));
+                acc.z = __fadd_rn(acc.z, __fmul_rn(Z, w));
+                acc.w = __fadd_rn(acc.w, w);
+                if (q == p) { own.x += r.x; own.y += r.y; own.z += r.z; own.w += 1.0f; }
+            }
+        }
+    }
+    xyz[p] = acc;
+    film[p] = own;
+}
+
+void launch_film_gauss(const float4* samples, int width, int height, int spp, uint64_t seed, int sample_base,
+                       const FilmConsts* consts, float4* xyz, float4* film, hipStream_t s) {
+    if (spp <= 0 || width <= 0 || height <= 0) return;
+    hipLaunchKernelGGL(k_film_gauss, dim3((width + 15) / 16, (height + 15) / 16), dim3(256), 0, s, samples, width, height,
+                       spp, seed, sample_base, consts, xyz, film);
+}
+
+__global__ void k_film_rgb(const float4* __restrict__ xyz, float* __restrict__ rgb, int n, const FilmConsts* __restrict__ fc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 f = xyz[i];
+    const float* m = fc->xyz_to_rgb;
+    float c[3];
+    for (int k = 0; k < 3; k++)
+        c[k] = __fadd_rn(__fadd_rn(__fmul_rn(m[3 * k], f.x), __fmul_rn(m[3 * k + 1], f.y)), __fmul_rn(m[3 * k + 2], f.z));
+    if (f.w != 0.0f) {
+        const float r = 1.0f / f.w;
+        for (int k = 0; k < 3; k++) c[k] = fmaxf(__fmul_rn(c[k], r), 0.0f) + 0.0f;
+    }
+    for (int k = 0; k < 3; k++) {
+        const float v = c[k];
+        rgb[3 * i + k] = v <= fc->break_point
+                             ? __fmul_rn(v, fc->slope)
+                             : __fmul_rn(fc->slope_match, (float)pow((double)v, (double)fc->gamma_recip)) - fc->segment_offset;
+    }
+}
+
+void launch_film_rgb(const float4* xyz, float* rgb, int n, const FilmConsts* consts, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_film_rgb, dim3((n + 255) / 256), dim3(256), 0, s, xyz, rgb, n, consts);
+}
+
+// ---------------------------------------------------------------------------
 // Wavefront path tracer
 // ---------------------------------------------------------------------------
 DEV int find_job(const DevJob* jobs, int n_jobs, int64_t item) {
@@ -618,7 +709,8 @@ __global__ void k_generate(const DevScene* __restrict__ scenes, const DevJob* __
         flags = 0x80000000u;  // rejected camera sample: the path is dropped at shade
         o = mk(0, 0, 0); d = mk(0, 0, 1);
     } else {
-        unsafeAtomicAdd(&film[job.film_offset + pix].w, 1.0f);
+        const int64_t fi = job.sample_film ? (int64_t)(sample - (uint32_t)job.film_sample_base) * npix + pix : pix;
+        unsafeAtomicAdd(&film[job.film_offset + fi].w, 1.0f);
     }
     flags |= nee_light_type(scenes[job.scene], rng, 0u) << INFO_LT_SHIFT;
     out.o[slot] = make_float4(o.x, o.y, o.z, __uint_as_float(flags & ORIGIN_W_BITS));
@@ -1003,6 +1095,8 @@ __global__ __launch_bounds__(BLOCK, 4) void k_shade(const DevScene* __restrict__
             const DevJob& job = UNI == 2 ? jobs[0] : jobs[info.z];
             const DevScene& sc = UNI ? *uscene : scenes[job.scene];
             film_idx = job.film_offset + info.x;
+            if (job.sample_film)
+                film_idx += (int64_t)(info.y - (uint32_t)job.film_sample_base) * (job.width * job.height);
             float4 o4 = q.o[i], d4 = q.d[i], b4 = q.beta[i];
             V o = f4v(o4), d = f4v(d4), beta = f4v(b4);
             float eta_scale = d4.w;

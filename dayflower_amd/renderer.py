@@ -286,6 +286,33 @@ class GPURenderer:
         self._c(self.lib.dfr_read_rgba8(self.ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
         return out
 
+    def setFilmFilter(self, on):
+        """Keeps CPURenderer's Gaussian XYZ film (PixelImageF with
+        GaussianFilter2F) beside the running-mean film; a change clears both."""
+        self._seti(abi.KEY_FILM_FILTER, 1 if on else 0)
+
+    def film_xyz(self, width, height):
+        """CPURenderer's XYZ film: (H, W, 4) float32 of X, Y, Z and the filter
+        weight sum."""
+        out = np.zeros((height, width, 4), np.float32)
+        self._c(self.lib.dfr_read_film_xyz(self.ctx, out.ctypes.data))
+        return out
+
+    def film_filtered(self, width, height):
+        """filmRender() of CPURenderer's film: (H, W, 3) gamma-encoded sRGB."""
+        out = np.zeros((height, width, 3), np.float32)
+        self._c(self.lib.dfr_read_film_filtered(self.ctx, out.ctypes.data))
+        return out
+
+    def save(self, path, width, height, filtered=False):
+        """Writes the image as PNG: GPURenderer's tone-mapped RGBA8 image, or
+        with filtered=True CPURenderer's filtered film in bytes."""
+        from dayflower_amd.png import float_to_bytes, write_png
+        if filtered:
+            write_png(path, float_to_bytes(self.film_filtered(width, height)))
+        else:
+            write_png(path, self.rgba8(width, height))
+
     def stats(self):
         st = abi.Stats()
         self._c(self.lib.dfr_get_stats(self.ctx, ctypes.byref(st)))

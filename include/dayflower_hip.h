@@ -152,8 +152,9 @@ typedef struct dfr_material {
     float disney_flatness;
 } dfr_material;
 
-/* One primitive (M/scene/Primitive.java).  Transforms are folded into world
- * space by the caller: every shape is given in world coordinates.
+/* One primitive (M/scene/Primitive.java), given in its object space; its
+ * object-to-world matrix is the identity unless dfr_update_transforms sets
+ * one (a caller may also fold transforms in and leave the identity).
  *   TRIANGLE_MESH  triangles [first_triangle, first_triangle+triangle_count)
  *   SPHERE         data = center.xyz, radius
  *   RECTANGLE      data = corners a.xyz, b.xyz, c.xyz  (Rectangle3F(a,b,c))
@@ -238,6 +239,9 @@ typedef struct dfr_ctx dfr_ctx;
 #define DFR_KEY_WF_SORT_KEYS           16  /* 1: shade in (material, light type) order, 0: queue order */
 #define DFR_KEY_BVH_WIDTH              17  /* read-only: children per BVH node (2 or 4) */
 #define DFR_KEY_USE_GRAPHS             18  /* 1 (default): replay captured HIP graphs of 4 wavefront iterations */
+#define DFR_KEY_FILM_FILTER            20  /* 0 (default): GPURenderer's running-mean film only;
+                                              1: also CPURenderer's Gaussian XYZ film (see
+                                              dfr_read_film_filtered); a change clears both films */
 #define DFR_KEY_RAY_SAMPLE             19  /* n > 0 with COUNT_TRAVERSAL: record every n-th traced ray */
 #define DFR_KEY_MAXIMUM_DISTANCE      101  /* float, default 20                */
 
@@ -313,6 +317,21 @@ int32_t dfr_update_camera(dfr_ctx* ctx, const float camera[DFR_CAMERA_LENGTH]);
  * :960-995).  rgb: 3 floats per pixel (row-major, y down), count: samples
  * per pixel.  Either pointer may be NULL. */
 int32_t dfr_read_film(dfr_ctx* ctx, float* rgb, int32_t* count);
+
+/* CPURenderer's film: PixelImageF's XYZ film with GaussianFilter2F(2, 2, 2)
+ * (M/image/PixelImageF.java:569-617, 662-705; M/filter/GaussianFilter2F.java)
+ * in ColorSpaceF.S_R_G_B, kept beside the running-mean film while
+ * DFR_KEY_FILM_FILTER is 1.  Every sample splats into the pixels whose
+ * centres lie within two pixels of its position (pixel + camera offset),
+ * weighted by the 16 x 16 Gaussian table; samples the reference would drop
+ * (no camera ray; non-finite or negative-luminance radiance on the plain
+ * kernel) add nothing.  dfr_read_film_xyz copies width * height * 4 floats
+ * (X, Y, Z, filter weight sum); dfr_read_film_filtered copies width *
+ * height * 3 floats of filmRender()'s output: linear sRGB divided by the
+ * weight sum, clamped at zero and gamma encoded.  DFR_E_STATE when the
+ * filter is off. */
+int32_t dfr_read_film_xyz(dfr_ctx* ctx, float* xyzw);
+int32_t dfr_read_film_filtered(dfr_ctx* ctx, float* rgb);
 
 /* RGBA8 image (imageBegin..imageEnd, AbstractImageKernel.java:1104-1324;
  * GPURenderer.java:312-335): the tone map selected by DFR_KEY_TONE_MAPPER

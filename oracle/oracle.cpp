@@ -1926,6 +1926,143 @@ int32_t orc_scene_set_transforms(orc_scene* h, const float* m, int32_t n) {
     return 0;
 }
 
+}  // extern "C"
+
+// One sample of the CPU render loop (AbstractCPURenderer.java:215-236): the
+// camera sample at pixel (x, y), its radiance, and whether it is kept (a
+// camera ray exists and the radiance is finite with luminance >= -1e-5).
+bool renderSample(const Scene& sc, const orc_params* prm, const float* cam, int x, int y, int k, C3& L) {
+    const int W = prm->width;
+    uint32_t pix = (uint32_t)(y * W + x);
+    Rng rng = makeRng(prm->seed, pix, (uint32_t)(prm->spp_offset + k));
+    Ray ray;
+    if (!primaryRay(cam, (float)x, (float)y, rng.get(0), rng.get(1), rng.get(DIM_LENS), rng.get(DIM_LENS + 1), &ray))
+        return false;
+    switch (prm->algorithm) {
+        case DFR_ALGO_PATH_TRACING:
+            L = radiancePT(sc, ray, rng, prm->max_bounce, prm->min_rr, prm->light_pdf_mode, prm->preview != 0);
+            break;
+        case DFR_ALGO_AMBIENT_OCCLUSION:
+            L = radianceAO(sc, ray, rng, prm->max_distance, prm->samples, prm->preview != 0);
+            break;
+        case DFR_ALGO_RAY_CASTING: L = radianceRC(sc, ray, prm->preview != 0); break;
+        default: L = C3(0.0f); break;
+    }
+    // AbstractCPURenderer.java:229: drop NaN/Inf/negative-luminance samples
+    return !(hasInf(L) || hasNaN(L) || relativeLuminance(L) < -1.0e-5f);
+}
+
+// ---------------------------------------------------------------------------
+// CPURenderer's film: PixelImageF's XYZ film with GaussianFilter2F(2, 2, 2)
+// (M/image/PixelImageF.java:569-617, 662-705, M/filter/Filter2F.java:33,
+// 119-136, M/filter/GaussianFilter2F.java:51-53, 117-119, 142-144) in
+// ColorSpaceF.S_R_G_B (M/color/ColorSpaceF.java:85, 116-194, 318-392).
+// Floats.exp/pow/ceil/floor are the double routines rounded to float.
+// ---------------------------------------------------------------------------
+struct ColorSpace {
+    float breakPoint, gamma, gammaRecip, slope, slopeMatch, segmentOffset;
+    float xyzToRgb[12], rgbToXyz[12];
+};
+
+inline float jpow(float a, float b) { return (float)std::pow((double)a, (double)b); }
+
+ColorSpace makeColorSpace(float breakPoint, float gamma, float xR, float yR, float xG, float yG, float xB, float yB,
+                          float xW, float yW) {
+    ColorSpace c;
+    c.breakPoint = breakPoint;
+    c.gamma = gamma;
+    c.gammaRecip = 1.0f / gamma;
+    c.slope = breakPoint > 0.0f ? 1.0f / (gamma / jpow(breakPoint, 1.0f / gamma - 1.0f) - gamma * breakPoint + breakPoint) : 1.0f;
+    c.slopeMatch = breakPoint > 0.0f ? gamma * c.slope / jpow(breakPoint, 1.0f / gamma - 1.0f) : 1.0f;
+    c.segmentOffset = breakPoint > 0.0f ? c.slopeMatch * jpow(breakPoint, 1.0f / gamma) - c.slope * breakPoint : 0.0f;
+    // XYZ -> RGB from the primaries' and white point's chromaticities
+    const float zR = 1.0f - (xR + yR), zG = 1.0f - (xG + yG), zB = 1.0f - (xB + yB), zW = 1.0f - (xW + yW);
+    const float rX = yG * zB - yB * zG, rY = xB * zG - xG * zB, rZ = xG * yB - xB * yG;
+    const float rW = (rX * xW + rY * yW + rZ * zW) / yW;
+    const float gX = yB * zR - yR * zB, gY = xR * zB - xB * zR, gZ = xB * yR - xR * yB;
+    const float gW = (gX * xW + gY * yW + gZ * zW) / yW;
+    const float bX = yR * zG - yG * zR, bY = xG * zR - xR * zG, bZ = xR * yG - xG * yR;
+    const float bW = (bX * xW + bY * yW + bZ * zW) / yW;
+    float* m = c.xyzToRgb;
+    m[0] = rX / rW; m[1] = rY / rW; m[2] = rZ / rW;
+    m[3] = gX / gW; m[4] = gY / gW; m[5] = gZ / gW;
+    m[6] = bX / bW; m[7] = bY / bW; m[8] = bZ / bW;
+    m[9] = rW; m[10] = gW; m[11] = bW;
+    // RGB -> XYZ: the inverse by cofactors, stored column-wise
+    const float a = m[0] * (m[4] * m[8] - m[7] * m[5]);
+    const float b = m[1] * (m[3] * m[8] - m[6] * m[5]);
+    const float cc = m[2] * (m[3] * m[7] - m[6] * m[4]);
+    const float sc = 1.0f / (a - b + cc);
+    float* r = c.rgbToXyz;
+    r[0] = sc * (m[4] * m[8] - m[5] * m[7]); r[1] = sc * (m[5] * m[6] - m[3] * m[8]); r[2] = sc * (m[3] * m[7] - m[4] * m[6]);
+    r[3] = sc * (m[2] * m[7] - m[1] * m[8]); r[4] = sc * (m[0] * m[8] - m[2] * m[6]); r[5] = sc * (m[1] * m[6] - m[0] * m[7]);
+    r[6] = sc * (m[1] * m[5] - m[2] * m[4]); r[7] = sc * (m[2] * m[3] - m[0] * m[5]); r[8] = sc * (m[0] * m[4] - m[1] * m[3]);
+    r[9] = xW; r[10] = yW; r[11] = zW;
+    return c;
+}
+
+const ColorSpace& srgbSpace() {
+    static const ColorSpace c = makeColorSpace(0.00304f, 2.4f, 0.6400f, 0.3300f, 0.3000f, 0.6000f, 0.1500f, 0.0600f,
+                                               0.31271f, 0.32902f);
+    return c;
+}
+
+C3 rgbToXyz(const ColorSpace& c, C3 v) {
+    const float* m = c.rgbToXyz;
+    return C3(m[0] * v.r + m[3] * v.g + m[6] * v.b, m[1] * v.r + m[4] * v.g + m[7] * v.b, m[2] * v.r + m[5] * v.g + m[8] * v.b);
+}
+C3 xyzToRgb(const ColorSpace& c, C3 v) {
+    const float* m = c.xyzToRgb;
+    return C3(m[0] * v.r + m[1] * v.g + m[2] * v.b, m[3] * v.r + m[4] * v.g + m[5] * v.b, m[6] * v.r + m[7] * v.g + m[8] * v.b);
+}
+float redoGamma(const ColorSpace& c, float v) {
+    return v <= c.breakPoint ? v * c.slope : c.slopeMatch * jpow(v, c.gammaRecip) - c.segmentOffset;
+}
+
+constexpr int FILTER_TABLE = 16;
+constexpr float FILTER_RADIUS = 2.0f, FILTER_FALLOFF = 2.0f;
+
+float gaussian1(float a, float b, float c) { return std::max(0.0f, (float)std::exp((double)(-c * a * a)) - b); }
+
+void filterTable(float* t) {
+    const float ex = (float)std::exp((double)(-FILTER_FALLOFF * FILTER_RADIUS * FILTER_RADIUS));
+    const float rcp = 1.0f / FILTER_TABLE;
+    for (int i = 0, y = 0; y < FILTER_TABLE; y++)
+        for (int x = 0; x < FILTER_TABLE; x++) {
+            const float fx = (x + 0.5f) * FILTER_RADIUS * rcp, fy = (y + 0.5f) * FILTER_RADIUS * rcp;
+            t[i++] = gaussian1(fx, ex, FILTER_FALLOFF) * gaussian1(fy, ex, FILTER_FALLOFF);
+        }
+}
+
+// filmAddColorXYZ(x, y, colorXYZ) with sample weight 1: every pixel whose
+// centre lies within the filter radius of the sample takes the colour times
+// the tabulated weight, and the weight into its weight sum.  film holds
+// (X, Y, Z, weight sum) per pixel.
+void filmSplat(int W, int H, const float* table, float x, float y, C3 xyz, float* film) {
+    const float dX = x - 0.5f, dY = y - 0.5f;
+    const float rr = 1.0f / FILTER_RADIUS;
+    const int minX = (int)std::max((float)std::ceil((double)(dX - FILTER_RADIUS)), 0.0f);
+    const int maxX = (int)std::min((float)std::floor((double)(dX + FILTER_RADIUS)), (float)(W - 1));
+    const int minY = (int)std::max((float)std::ceil((double)(dY - FILTER_RADIUS)), 0.0f);
+    const int maxY = (int)std::min((float)std::floor((double)(dY + FILTER_RADIUS)), (float)(H - 1));
+    if (maxX < minX || maxY < minY) return;
+    int offX[8], offY[8];
+    for (int fx = minX; fx <= maxX; fx++)
+        offX[fx - minX] = std::min((int)std::floor((double)std::fabs((fx - dX) * rr * FILTER_TABLE)), FILTER_TABLE - 1);
+    for (int fy = minY; fy <= maxY; fy++)
+        offY[fy - minY] = std::min((int)std::floor((double)std::fabs((fy - dY) * rr * FILTER_TABLE)), FILTER_TABLE - 1);
+    for (int fy = minY; fy <= maxY; fy++)
+        for (int fx = minX; fx <= maxX; fx++) {
+            const float w = table[offY[fy - minY] * FILTER_TABLE + offX[fx - minX]];
+            float* px = film + 4 * ((int64_t)fy * W + fx);
+            const float sw = 1.0f * w;
+            px[0] = px[0] + xyz.r * sw; px[1] = px[1] + xyz.g * sw; px[2] = px[2] + xyz.b * sw;
+            px[3] = px[3] + w;
+        }
+}
+
+extern "C" {
+
 int32_t orc_render(const orc_scene* h, const orc_params* prm, double* sum, double* sumsq, int32_t* count) {
     if (!h || !prm) return -1;
     const Scene& sc = h->s;
@@ -1937,27 +2074,11 @@ int32_t orc_render(const orc_scene* h, const orc_params* prm, double* sum, doubl
     int64_t n = (int64_t)(x1 - x0) * (y1 - y0);
     parallelFor(n, prm->threads, [&](int64_t i) {
         int x = x0 + (int)(i % (x1 - x0)), y = y0 + (int)(i / (x1 - x0));
-        uint32_t pix = (uint32_t)(y * W + x);
         double s[3] = {0, 0, 0}, q[3] = {0, 0, 0};
         int c = 0;
         for (int k = 0; k < prm->spp; k++) {
-            Rng rng = makeRng(prm->seed, pix, (uint32_t)(prm->spp_offset + k));
-            Ray ray;
-            if (!primaryRay(cam, (float)x, (float)y, rng.get(0), rng.get(1), rng.get(DIM_LENS), rng.get(DIM_LENS + 1), &ray))
-                continue;
             C3 L;
-            switch (prm->algorithm) {
-                case DFR_ALGO_PATH_TRACING:
-                    L = radiancePT(sc, ray, rng, prm->max_bounce, prm->min_rr, prm->light_pdf_mode, prm->preview != 0);
-                    break;
-                case DFR_ALGO_AMBIENT_OCCLUSION:
-                    L = radianceAO(sc, ray, rng, prm->max_distance, prm->samples, prm->preview != 0);
-                    break;
-                case DFR_ALGO_RAY_CASTING: L = radianceRC(sc, ray, prm->preview != 0); break;
-                default: L = C3(0.0f); break;
-            }
-            // AbstractCPURenderer.java:229: drop NaN/Inf/negative-luminance samples
-            if (hasInf(L) || hasNaN(L) || relativeLuminance(L) < -1.0e-5f) continue;
+            if (!renderSample(sc, prm, cam, x, y, k, L)) continue;
             s[0] += L.r; s[1] += L.g; s[2] += L.b;
             q[0] += (double)L.r * L.r; q[1] += (double)L.g * L.g; q[2] += (double)L.b * L.b;
             c++;
@@ -1969,6 +2090,78 @@ int32_t orc_render(const orc_scene* h, const orc_params* prm, double* sum, doubl
         }
         if (count) count[o] = c;
     });
+    return 0;
+}
+
+// The CPU render loop into PixelImageF's film: pass by pass, each pixel's
+// sample in row order is converted to XYZ and splatted (accumulates into
+// film: W*H*4 floats of X, Y, Z, weight sum).  Radiance is computed in
+// parallel per pass, the splats in the reference's order.
+int32_t orc_render_film(const orc_scene* h, const orc_params* prm, float* film) {
+    if (!h || !prm || !film) return -1;
+    const Scene& sc = h->s;
+    const int W = prm->width, H = prm->height;
+    float cam[DFR_CAMERA_LENGTH];
+    std::memcpy(cam, sc.cam, sizeof(cam));
+    cam[DFR_CAMERA_RES_X] = (float)W; cam[DFR_CAMERA_RES_Y] = (float)H;
+    float table[FILTER_TABLE * FILTER_TABLE];
+    filterTable(table);
+    const ColorSpace& cs = srgbSpace();
+    const int64_t n = (int64_t)W * H;
+    std::vector<C3> L(n);
+    std::vector<uint8_t> keep(n);
+    for (int k = 0; k < prm->spp; k++) {
+        parallelFor(n, prm->threads, [&](int64_t i) {
+            keep[i] = renderSample(sc, prm, cam, (int)(i % W), (int)(i / W), k, L[i]) ? 1 : 0;
+        });
+        for (int64_t i = 0; i < n; i++) {
+            if (!keep[i]) continue;
+            const int x = (int)(i % W), y = (int)(i / W);
+            Rng rng = makeRng(prm->seed, (uint32_t)i, (uint32_t)(prm->spp_offset + k));
+            filmSplat(W, H, table, (float)x + rng.get(0), (float)y + rng.get(1), rgbToXyz(cs, L[i]), film);
+        }
+    }
+    return 0;
+}
+
+// filmRender(): per pixel XYZ -> linear sRGB, divided by the weight sum and
+// clamped at zero when the sum is not zero, then gamma encoded (the splat
+// term is zero: nothing splats into it on this path).
+int32_t orc_film_render(int64_t n, const float* film, float* rgb) {
+    if (!film || !rgb) return -1;
+    const ColorSpace& cs = srgbSpace();
+    for (int64_t i = 0; i < n; i++) {
+        const float* p = film + 4 * i;
+        C3 c = xyzToRgb(cs, C3(p[0], p[1], p[2]));
+        if (!isZero(p[3])) {
+            const float r = 1.0f / p[3];
+            c = C3(std::max(c.r * r, 0.0f), std::max(c.g * r, 0.0f), std::max(c.b * r, 0.0f));
+        }
+        const C3 splat = xyzToRgb(cs, C3(0.0f, 0.0f, 0.0f)) * 1.0f;
+        c = c + splat;
+        rgb[3 * i] = redoGamma(cs, c.r); rgb[3 * i + 1] = redoGamma(cs, c.g); rgb[3 * i + 2] = redoGamma(cs, c.b);
+    }
+    return 0;
+}
+
+// filmAddColorXYZ of one sample, for tests.
+int32_t orc_film_splat(int32_t W, int32_t H, float x, float y, const float* xyz, float* film) {
+    if (!xyz || !film) return -1;
+    float table[FILTER_TABLE * FILTER_TABLE];
+    filterTable(table);
+    filmSplat(W, H, table, x, y, C3(xyz[0], xyz[1], xyz[2]), film);
+    return 0;
+}
+
+// The 16 x 16 filter table, the sRGB matrices (XYZ->RGB row-major, RGB->XYZ
+// as stored, 12 floats each) and the gamma constants (break point, slope,
+// slope match, segment offset).
+int32_t orc_film_constants(float* table, float* xyz_to_rgb, float* rgb_to_xyz, float* gamma4) {
+    if (table) filterTable(table);
+    const ColorSpace& c = srgbSpace();
+    if (xyz_to_rgb) std::memcpy(xyz_to_rgb, c.xyzToRgb, sizeof(c.xyzToRgb));
+    if (rgb_to_xyz) std::memcpy(rgb_to_xyz, c.rgbToXyz, sizeof(c.rgbToXyz));
+    if (gamma4) { gamma4[0] = c.breakPoint; gamma4[1] = c.slope; gamma4[2] = c.slopeMatch; gamma4[3] = c.segmentOffset; }
     return 0;
 }
 

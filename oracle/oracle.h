@@ -35,6 +35,13 @@
  *                   is found on the primitive's world-space image, with the
  *                   0.001 minimum distance in world space (the reference
  *                   applies it to the object-space ray)
+ *   CPU film        filter table against the Gaussian in closed form; a
+ *                   splat covers exactly the pixels within the radius, with
+ *                   weights symmetric and largest at the nearest centre; the
+ *                   sRGB matrices against IEC 61966-2-1's; the gamma
+ *                   constants against the standard curve (12.92, 1.055,
+ *                   0.055); a constant-radiance furnace film renders to the
+ *                   encoded constant (tests/test_film.py)
  *   image stage     textbook sRGB bytes (linear 0.5 -> 188, 0.18 -> 118),
  *                   tone curves at closed-form points (Reinhard 1 -> 1/2,
  *                   white point 4 -> 1, 1 - e^-ln2 = 1/2, ACES fit at 1),
@@ -78,6 +85,19 @@ int32_t orc_scene_set_transforms(orc_scene* scene, const float* m, int32_t n);
  * and width*height ints; any may be NULL. */
 int32_t orc_render(const orc_scene* scene, const orc_params* params, double* sum, double* sumsq,
                    int32_t* count);
+
+/* CPURenderer's film (PixelImageF with GaussianFilter2F(2, 2, 2), sRGB):
+ * orc_render_film accumulates `params->spp` passes into film (W*H*4 floats:
+ * X, Y, Z, filter weight sum; the caller zeroes it); orc_film_render turns
+ * it into the displayed gamma-encoded RGB (filmRender); orc_film_splat is
+ * filmAddColorXYZ of one sample; orc_film_constants returns the 16 x 16
+ * filter table, the XYZ->RGB and RGB->XYZ arrays (12 floats each, as the
+ * reference stores them) and the gamma constants (break point, slope, slope
+ * match, segment offset).  Any output may be NULL. */
+int32_t orc_render_film(const orc_scene* scene, const orc_params* params, float* film);
+int32_t orc_film_render(int64_t n, const float* film, float* rgb);
+int32_t orc_film_splat(int32_t width, int32_t height, float x, float y, const float* xyz, float* film);
+int32_t orc_film_constants(float* table256, float* xyz_to_rgb12, float* rgb_to_xyz12, float* gamma4);
 
 /* Closest/any hit for rays of 8 floats (origin, tmin, direction, tmax). */
 int32_t orc_trace_f32(const orc_scene* scene, const float* rays, int64_t n, int32_t any_hit,

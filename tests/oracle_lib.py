@@ -46,6 +46,10 @@ def load():
     lib.orc_scene_destroy.argtypes = [vp]
     lib.orc_scene_set_camera.argtypes = [vp, vp]
     lib.orc_scene_set_transforms.argtypes = [vp, vp, ctypes.c_int32]
+    lib.orc_render_film.argtypes = [vp, vp, vp]
+    lib.orc_film_render.argtypes = [ctypes.c_int64, vp, vp]
+    lib.orc_film_splat.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_float, vp, vp]
+    lib.orc_film_constants.argtypes = [vp, vp, vp, vp]
     lib.orc_scene_set_transforms.restype = ctypes.c_int32
     lib.orc_render.argtypes = [vp, P(Params), vp, vp, vp]
     lib.orc_trace_f32.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
@@ -135,6 +139,23 @@ class OracleScene:
             raise RuntimeError("orc_render failed")
         return s, q, c
 
+    def render_film(self, width, height, spp, film=None, algorithm=abi.ALGO_PATH_TRACING, max_bounce=20, min_rr=5,
+                    samples=10, max_distance=20.0, seed=0, spp_offset=0, light_pdf_mode=0, threads=THREADS):
+        """CPURenderer's Gaussian XYZ film: (H, W, 4) float32 of X, Y, Z and
+        the filter weight sum, accumulated into `film` when given."""
+        p = Params()
+        p.width, p.height, p.spp, p.spp_offset = width, height, spp, spp_offset
+        p.algorithm, p.max_bounce, p.min_rr, p.samples = algorithm, max_bounce, min_rr, samples
+        p.light_pdf_mode, p.threads = light_pdf_mode, threads
+        p.max_distance = max_distance
+        p.seed = seed
+        if film is None:
+            film = np.zeros((height, width, 4), np.float32)
+        rc = self.lib.orc_render_film(self.h, ctypes.byref(p), _ptr(film))
+        if rc != 0:
+            raise RuntimeError("orc_render_film failed")
+        return film
+
     def trace_f32(self, rays, any_hit=False, threads=THREADS):
         rays = np.ascontiguousarray(rays, np.float32)
         n = rays.shape[0]
@@ -215,3 +236,28 @@ def image_rgba8(rgb, tone_mapper, algorithm=abi.ALGO_PATH_TRACING):
     out = np.empty(x.shape[:-1] + (4,), np.uint8)
     load().orc_image_rgba8(int(algorithm), int(tone_mapper), n, _ptr(x), _ptr(out))
     return out
+
+
+def film_render(film):
+    """filmRender(): (H, W, 4) XYZ + weight film -> (H, W, 3) encoded RGB."""
+    film = np.ascontiguousarray(film, np.float32)
+    rgb = np.zeros(film.shape[:-1] + (3,), np.float32)
+    load().orc_film_render(film.size // 4, _ptr(film), _ptr(rgb))
+    return rgb
+
+
+def film_splat(width, height, x, y, xyz, film=None):
+    if film is None:
+        film = np.zeros((height, width, 4), np.float32)
+    v = np.ascontiguousarray(xyz, np.float32)
+    load().orc_film_splat(width, height, float(x), float(y), _ptr(v), _ptr(film))
+    return film
+
+
+def film_constants():
+    t = np.zeros(256, np.float32)
+    a = np.zeros(12, np.float32)
+    b = np.zeros(12, np.float32)
+    g = np.zeros(4, np.float32)
+    load().orc_film_constants(_ptr(t), _ptr(a), _ptr(b), _ptr(g))
+    return t.reshape(16, 16), a, b, g
