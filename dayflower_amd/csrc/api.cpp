@@ -952,20 +952,31 @@ size_t wavefront_bytes(int capacity) {
 // from the cached one.
 int Ctx::replay_iterations(const std::vector<uintptr_t>& key, const DevScene* scenes, const DevJob* jobs, int n_jobs,
                            float4* film, const WavefrontParams& p, int iterations) {
-    if (!graph_exec || key != graph_key) {
-        if (graph_exec) { hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
-        hipGraph_t g = nullptr;
+    GraphEntry* g = nullptr;
+    for (GraphEntry& e : graphs)
+        if (e.key == key) { g = &e; break; }
+    if (!g) {
+        if (graphs.size() >= MAX_GRAPHS) {
+            auto lru = std::min_element(graphs.begin(), graphs.end(),
+                                        [](const GraphEntry& a, const GraphEntry& b) { return a.used < b.used; });
+            hipGraphExecDestroy(lru->exec);
+            graphs.erase(lru);
+        }
+        hipGraph_t gr = nullptr;
         hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return hip_fail(e, "graph capture");
         for (int k = 0; k < iterations; k++) wavefront_iteration(scenes, jobs, n_jobs, film, wf, p, nullptr, stream);
-        e = hipStreamEndCapture(stream, &g);
+        e = hipStreamEndCapture(stream, &gr);
         if (e != hipSuccess) return hip_fail(e, "graph capture");
-        e = hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
-        if (e != hipSuccess) { graph_exec = nullptr; return hip_fail(e, "graph instantiate"); }
-        graph_key = key;
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+        hipGraphDestroy(gr);
+        if (e != hipSuccess) return hip_fail(e, "graph instantiate");
+        graphs.push_back(GraphEntry{key, exec, 0});
+        g = &graphs.back();
     }
-    hipError_t e = hipGraphLaunch(graph_exec, stream);
+    g->used = ++graph_clock;
+    hipError_t e = hipGraphLaunch(g->exec, stream);
     if (e != hipSuccess) return hip_fail(e, "graph launch");
     return DFR_OK;
 }
@@ -984,6 +995,17 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
     wavefront_init(wf, (const DevJob*)job_buf.ptr, n_jobs, work_total, stream);
     WavefrontParams p{wf_params.block, wf_params.sort_keys, (capacity + 255) / 256, any_area_lights(), uniform_scene(),
                       any_ext_features()};
+    // After a poll that finds every work item generated, the next
+    // iterations' queues hold at most the paths waiting then: size their
+    // grids to that count, rounded up to capacity / 4^k (a few captured
+    // graphs serve every run).
+    auto bound_grids = [&](const Counters& h) {
+        if (h.work_next < h.work_total) { p.queue_bound = 0; p.generate = true; return; }
+        int b = capacity;
+        while (b / 4 >= std::max<int64_t>(h.n_next, 1) && b / 4 >= 16384) b /= 4;
+        p.queue_bound = b;
+        p.generate = false;
+    };
     Counters host{};
     const int poll = 4;
     const int max_iterations = 1 << 20;  // far above max_bounce * (work / capacity + 1)
@@ -1016,13 +1038,17 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
             (uintptr_t)p.ext_features};
         for (int it = 0;; it += poll) {
             if (it >= max_iterations) return fail(DFR_E_DEVICE, "wavefront did not drain");
-            if ((rc = replay_iterations(key, scenes, jobs, n_jobs, film_ptr, p, poll))) return rc;
+            std::vector<uintptr_t> k2 = key;
+            k2.push_back((uintptr_t)p.queue_bound);
+            k2.push_back((uintptr_t)p.generate);
+            if ((rc = replay_iterations(k2, scenes, jobs, n_jobs, film_ptr, p, poll))) return rc;
             iterations = it + poll;
             hipError_t e = hipMemcpyAsync(&host, wf.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream);
             if (e == hipSuccess) e = hipStreamSynchronize(stream);
             if (e != hipSuccess) return hip_fail(e, "wavefront counters");
             if (host.n_next == 0 && host.work_next >= host.work_total) break;
             if (stop.load()) return fail(DFR_E_STOPPED, "render stopped");
+            bound_grids(host);
         }
     }
     for (int it = 0; iterations == 0; it++) {
@@ -1046,6 +1072,7 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
             // after an iteration, the paths of the next queue wait in n_next
             if (host.n_next == 0 && host.work_next >= host.work_total) { iterations = it + 1; break; }
             if (stop.load()) return fail(DFR_E_STOPPED, "render stopped");
+            bound_grids(host);
         }
     }
     prof_stats = dfr_stats{};
@@ -1147,7 +1174,7 @@ void dfr_destroy(dfr_ctx* h) {
     hipSetDevice(c->device);
     c->scenes.clear();
     for (hipEvent_t e : c->prof_events) hipEventDestroy(e);
-    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    for (Ctx::GraphEntry& g : c->graphs) hipGraphExecDestroy(g.exec);
     hipEventDestroy(c->ev0);
     hipEventDestroy(c->ev1);
     hipStreamDestroy(c->stream);

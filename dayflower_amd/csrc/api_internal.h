@@ -142,11 +142,18 @@ struct Ctx {
     DeviceBuffer wf_mem[17];
     int wf_capacity = 0;
     std::vector<hipEvent_t> prof_events;   // 6 per wavefront iteration, reused
-    // A captured run of WAVEFRONT_POLL iterations, replayed between counter
-    // polls.  `graph_key` holds every pointer and size its launches bake in.
+    // Captured runs of WAVEFRONT_POLL iterations, replayed between counter
+    // polls; each key holds every pointer, size and grid bound its launches
+    // bake in.
     bool use_graphs = true;
-    hipGraphExec_t graph_exec = nullptr;
-    std::vector<uintptr_t> graph_key;
+    struct GraphEntry {
+        std::vector<uintptr_t> key;
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+    static constexpr size_t MAX_GRAPHS = 8;  // least recently used evicted
+    std::vector<GraphEntry> graphs;
+    uint64_t graph_clock = 0;
     int replay_iterations(const std::vector<uintptr_t>& key, const DevScene* scenes, const DevJob* jobs, int n_jobs,
                           float4* film, const WavefrontParams& p, int iterations);
 

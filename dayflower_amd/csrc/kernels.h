@@ -162,6 +162,12 @@ struct WavefrontParams {
     // by work_begin (batches)
     const GenEntry* order = nullptr;
     int n_order = 0;
+    // Bound on the paths of every queue of the next iterations, from the
+    // host's last counter poll (0: the capacity).  Once every work item has
+    // been generated the queues only shrink, so the grids of those
+    // iterations are sized to it and `generate` is off.
+    int queue_bound = 0;
+    bool generate = true;
 };
 
 // Optional instrumentation of one wavefront iteration: HIP events around the

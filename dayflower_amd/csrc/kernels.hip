@@ -1222,17 +1222,17 @@ void wavefront_init(WavefrontBuffers& w, const DevJob* jobs, int n_jobs, int64_t
 // One traversal launch over the path queue (SHADOW false) or the any-hit
 // shadow queue (SHADOW true); the scene-uniform variant when p.uscene is set.
 template <int B, bool COUNT, bool SHADOW>
-static void launch_trace_queue(int grid, const DevScene* scenes, const DevJob* jobs, float4* film, WavefrontBuffers& w,
-                               const WavefrontParams& p, unsigned long long* trav, hipStream_t s) {
+static void launch_trace_queue(int grid, int tiles, const DevScene* scenes, const DevJob* jobs, float4* film,
+                               WavefrontBuffers& w, const WavefrontParams& p, unsigned long long* trav, hipStream_t s) {
     const RaySample smp = COUNT ? w.sample : RaySample{};
     if (BVH_WIDTH == 4 && p.uscene)
         hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, SHADOW ? SHADOW_RPL : EXTEND_RPL, COUNT, SHADOW, true>), dim3(grid), dim3(B), 0, s,
                            scenes, jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav, p.uscene,
-                           SHADOW || !p.sort_keys ? nullptr : w.keys, w.bhist, (w.capacity + SORT_TILE - 1) / SORT_TILE, smp);
+                           SHADOW || !p.sort_keys ? nullptr : w.keys, w.bhist, tiles, smp);
     else
         hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, SHADOW ? SHADOW_RPL : EXTEND_RPL, COUNT, SHADOW, false>), dim3(grid), dim3(B), 0, s,
                            scenes, jobs, w.a, w.sh, w.hit, film, w.ctr, w.overflow, trav, p.uscene,
-                           SHADOW || !p.sort_keys ? nullptr : w.keys, w.bhist, (w.capacity + SORT_TILE - 1) / SORT_TILE, smp);
+                           SHADOW || !p.sort_keys ? nullptr : w.keys, w.bhist, tiles, smp);
 }
 
 __global__ void k_zero_i32(int32_t* __restrict__ p, int n) {
@@ -1244,14 +1244,16 @@ template <int B>
 static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, float4* film, WavefrontBuffers& w,
                         const WavefrontParams& p, const IterProfile* prof, hipStream_t s) {
     const int cap = w.capacity;
-    const int grid256 = (cap + 255) / 256;
+    // the queues' layout follows the capacity; the grids cover `qb` paths
+    const int qb = p.queue_bound > 0 && p.queue_bound < cap ? p.queue_bound : cap;
+    const int grid256 = p.generate ? (cap + 255) / 256 : 1;
     // top up the queue with new camera paths, then commit the counters
     hipLaunchKernelGGL(k_generate, dim3(grid256), dim3(256), 0, s, scenes, jobs, n_jobs, p.order, p.n_order, w.a, w.ctr,
                        film, cap);
     hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, s, w.ctr, cap);
     const bool timing = prof && prof->timing;
     const int use_perm = p.sort_keys > 0;
-    const int tiles = (cap + SORT_TILE - 1) / SORT_TILE;  // <= 4096 (k_sort_scan_rows)
+    const int tiles = (qb + SORT_TILE - 1) / SORT_TILE;  // <= 4096 (k_sort_scan_rows)
     // the scene-uniform extension kernel writes the sort keys and adds their
     // per-tile counts: clear the counts first
     const bool fused_hist = use_perm && BVH_WIDTH == 4 && p.uscene;
@@ -1259,9 +1261,9 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
         hipLaunchKernelGGL(k_zero_i32, dim3((NSORT_KEYS * tiles + 255) / 256), dim3(256), 0, s, w.bhist, NSORT_KEYS * tiles);
     if (timing) hipEventRecord(prof->ev[0], s);
     unsigned long long* trav = prof ? prof->trav : nullptr;
-    const int egrid = (cap + EXTEND_RPL * B - 1) / (EXTEND_RPL * B);
-    if (trav) launch_trace_queue<B, true, false>(egrid, scenes, jobs, film, w, p, trav, s);
-    else launch_trace_queue<B, false, false>(egrid, scenes, jobs, film, w, p, trav, s);
+    const int egrid = (qb + EXTEND_RPL * B - 1) / (EXTEND_RPL * B);
+    if (trav) launch_trace_queue<B, true, false>(egrid, tiles, scenes, jobs, film, w, p, trav, s);
+    else launch_trace_queue<B, false, false>(egrid, tiles, scenes, jobs, film, w, p, trav, s);
     if (timing) hipEventRecord(prof->ev[1], s);
     if (use_perm) {
         if (!fused_hist)
@@ -1274,7 +1276,7 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     // shading in 512-thread blocks whatever the traversal block: one queue
     // reservation (two same-address atomics) per 512 paths
     {
-        const dim3 grid((cap + SHADE_BLOCK - 1) / SHADE_BLOCK), block(SHADE_BLOCK);
+        const dim3 grid((qb + SHADE_BLOCK - 1) / SHADE_BLOCK), block(SHADE_BLOCK);
         const DevScene* us = p.uscene;
         const int uni = us ? (n_jobs == 1 ? 2 : 1) : 0;
 #define DFR_SHADE(E, U)                                                                                                  \
@@ -1290,19 +1292,19 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     if (timing) hipEventRecord(prof->ev[3], s);
     hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(1), 0, s, w.ctr);
     if (timing) hipEventRecord(prof->ev[4], s);
-    const int sgrid = (int)((2 * (int64_t)cap + B - 1) / B);  // shadow requests <= 2 per path
+    const int sgrid = (int)((2 * (int64_t)qb + B - 1) / B);  // shadow requests <= 2 per path
     // the any-hit queue runs in blocks of at most 128 threads (−0.8 % against
     // 256; the extension rays prefer 256); the thread count, and so the
     // overflow stack, is the same
     constexpr int SB = B < 128 ? B : 128;
-    const int agrid = (int)((2 * (int64_t)cap + SHADOW_RPL * SB - 1) / (SHADOW_RPL * SB));
+    const int agrid = (int)((2 * (int64_t)qb + SHADOW_RPL * SB - 1) / (SHADOW_RPL * SB));
     if (trav) {
-        launch_trace_queue<SB, true, true>(agrid, scenes, jobs, film, w, p, trav, s);
+        launch_trace_queue<SB, true, true>(agrid, tiles, scenes, jobs, film, w, p, trav, s);
         if (p.closest_shadow)
             hipLaunchKernelGGL((k_shadow_closest<B, SHORT_STACK, true>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh,
                                w.ctr, film, w.overflow, trav, cap);
     } else {
-        launch_trace_queue<SB, false, true>(agrid, scenes, jobs, film, w, p, trav, s);
+        launch_trace_queue<SB, false, true>(agrid, tiles, scenes, jobs, film, w, p, trav, s);
         if (p.closest_shadow)
             hipLaunchKernelGGL((k_shadow_closest<B, SHORT_STACK, false>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh,
                                w.ctr, film, w.overflow, trav, cap);
