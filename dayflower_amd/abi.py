@@ -17,7 +17,7 @@ ALGO_AMBIENT_OCCLUSION, ALGO_DEPTH_CAMERA, ALGO_PATH_TRACING, ALGO_RAY_CASTING, 
 SHAPE_TRIANGLE_MESH, SHAPE_SPHERE, SHAPE_RECTANGLE, SHAPE_PLANE = range(4)
 (MAT_MATTE, MAT_MIRROR, MAT_GLASS, MAT_METAL, MAT_PLASTIC, MAT_DISNEY, MAT_GLOSSY, MAT_SUBSTRATE, MAT_TRANSLUCENT,
  MAT_CLEAR_COAT, MAT_UBER) = range(11)
-LIGHT_DIFFUSE_AREA, LIGHT_POINT, LIGHT_PEREZ, LIGHT_SPOT, LIGHT_DIRECTIONAL = range(5)
+LIGHT_DIFFUSE_AREA, LIGHT_POINT, LIGHT_PEREZ, LIGHT_SPOT, LIGHT_DIRECTIONAL, LIGHT_IMAGE = range(6)
 LENS_FISHEYE, LENS_THIN = 0, 1
 
 KEY_MAXIMUM_BOUNCE = 1
@@ -77,6 +77,10 @@ class Light(ctypes.Structure):
     ]
 
 
+class Image(ctypes.Structure):
+    _fields_ = [("width", c_int32), ("height", c_int32), ("rgb", ctypes.POINTER(c_float))]
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [
         ("n_vertices", c_int64), ("positions", ctypes.POINTER(c_float)), ("normals", ctypes.POINTER(c_float)),
@@ -85,6 +89,7 @@ class SceneDesc(ctypes.Structure):
         ("n_materials", c_int32), ("materials", ctypes.POINTER(Material)),
         ("n_lights", c_int32), ("lights", ctypes.POINTER(Light)),
         ("camera", c_float * CAMERA_LENGTH),
+        ("n_images", c_int32), ("images", ctypes.POINTER(Image)),
     ]
 
 

@@ -438,6 +438,8 @@ struct Baked {
     std::vector<DevLight> lights;
     std::vector<DevPerez> perez;
     std::vector<float4> xforms;   // per primitive: DevScene::xforms
+    std::vector<DevImageLight> images;
+    std::vector<float> texels;
 };
 
 static int bake_scene(const SceneHost& in, const std::vector<Xform>& xf, const std::vector<int32_t>& tri_prim,
@@ -542,6 +544,8 @@ static int bake_scene(const SceneHost& in, const std::vector<Xform>& xf, const s
     }
     b.lights.assign(in.lights.size(), DevLight{});
     b.perez.clear();
+    b.images.clear();
+    b.texels.clear();
     for (size_t l = 0; l < in.lights.size(); l++) {
         const dfr_light& L = in.lights[l];
         DevLight& d = b.lights[l];
@@ -578,6 +582,20 @@ static int bake_scene(const SceneHost& in, const std::vector<Xform>& xf, const s
             perez_build(pz, L.turbidity, L.direction);
             d.perez = (int32_t)b.perez.size();
             b.perez.push_back(pz);
+        } else if (L.type == DFR_LIGHT_IMAGE) {
+            if (L.primitive < 0 || L.primitive >= (int)in.images.size()) {
+                err = "light " + std::to_string(l) + ": no such image"; return DFR_E_ARGUMENT;
+            }
+            const SceneHost::Image& im = in.images[L.primitive];
+            const int64_t off = (int64_t)b.texels.size();
+            b.texels.insert(b.texels.end(), im.rgb.begin(), im.rgb.end());
+            float su = L.direction[0], sv = L.direction[1];
+            if (su == 0.0f && sv == 0.0f) su = sv = 1.0f;
+            DevImageLight il;
+            image_light_build(il, im.width, im.height, off, b.texels.data(), L.cone_angle, su, sv,
+                              L.radius > 0.0f ? L.radius : 10.0f);
+            d.perez = (int32_t)b.images.size();
+            b.images.push_back(il);
         } else if (L.type == DFR_LIGHT_SPOT || L.type == DFR_LIGHT_DIRECTIONAL) {
             const double x = L.direction[0], y = L.direction[1], z = L.direction[2], n = std::sqrt(x * x + y * y + z * z);
             if (!(n > 0.0)) { err = "light " + std::to_string(l) + ": zero direction"; return DFR_E_ARGUMENT; }
@@ -605,6 +623,8 @@ static int upload_baked(const Baked& bk, CompiledScene& out, hipStream_t s) {
     if ((rc = upload(out.lights, bk.lights, s))) return rc;
     if ((rc = upload(out.perez, bk.perez, s))) return rc;
     if ((rc = upload(out.xforms, bk.xforms, s))) return rc;
+    if ((rc = upload(out.images, bk.images, s))) return rc;
+    if ((rc = upload(out.texels, bk.texels, s))) return rc;
     return DFR_OK;
 }
 
@@ -691,17 +711,21 @@ int compile_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledSce
     ds.lights = (const DevLight*)out.lights.ptr;
     ds.perez = (const DevPerez*)out.perez.ptr;
     ds.xforms = (const float4*)out.xforms.ptr;
+    ds.images = (const DevImageLight*)out.images.ptr;
+    ds.texels = (const float*)out.texels.ptr;
     ds.n_analytic = (int32_t)bk.analytic.size();
     ds.n_lights = (int32_t)bk.lights.size();
     for (const DevLight& L : bk.lights) {
         out.area_lights = out.area_lights || L.type == DFR_LIGHT_DIFFUSE_AREA;
-        out.ext_features = out.ext_features || L.type == DFR_LIGHT_SPOT || L.type == DFR_LIGHT_DIRECTIONAL;
+        out.ext_features = out.ext_features || L.type == DFR_LIGHT_SPOT || L.type == DFR_LIGHT_DIRECTIONAL ||
+                           L.type == DFR_LIGHT_IMAGE;
     }
     ds.has_bvh = nt > 0 ? 1 : 0;
     ds.light_pdf_mode = light_pdf_mode;
     std::memcpy(out.camera, in.camera, sizeof(out.camera));
     out.bytes = out.nodes.bytes + out.tris.bytes + out.tshade.bytes + out.prims.bytes + out.analytic.bytes +
-                out.mats.bytes + out.lobes.bytes + out.lights.bytes + out.perez.bytes + out.xforms.bytes;
+                out.mats.bytes + out.lobes.bytes + out.lights.bytes + out.perez.bytes + out.xforms.bytes +
+                out.images.bytes + out.texels.bytes;
     if (hipStreamSynchronize(s) != hipSuccess) { err = "upload failed"; return DFR_E_DEVICE; }
     return DFR_OK;
 }
@@ -786,6 +810,17 @@ int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err) {
     h.mats.assign(d->materials, d->materials + d->n_materials);
     h.lights.assign(d->lights, d->lights + d->n_lights);
     std::memcpy(h.camera, d->camera, sizeof(h.camera));
+    if (d->n_images < 0 || (d->n_images && !d->images)) { err = "images == null"; return DFR_E_NULL; }
+    h.images.assign(d->n_images, SceneHost::Image{});
+    for (int32_t i = 0; i < d->n_images; i++) {
+        const dfr_image& im = d->images[i];
+        if (im.width < 1 || im.height < 1) { err = "image " + std::to_string(i) + ": empty"; return DFR_E_ARGUMENT; }
+        if (!im.rgb) { err = "image " + std::to_string(i) + ": rgb == null"; return DFR_E_NULL; }
+        if ((int64_t)im.width * im.height > (int64_t(1) << 28)) { err = "image " + std::to_string(i) + ": too large"; return DFR_E_ARGUMENT; }
+        h.images[i].width = im.width;
+        h.images[i].height = im.height;
+        h.images[i].rgb.assign(im.rgb, im.rgb + 3 * (size_t)im.width * im.height);
+    }
     return DFR_OK;
 }
 

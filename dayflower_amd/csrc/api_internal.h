@@ -50,6 +50,8 @@ struct SceneHost {
     std::vector<dfr_material> mats;
     std::vector<dfr_light> lights;
     float camera[DFR_CAMERA_LENGTH];
+    struct Image { int32_t width = 0, height = 0; std::vector<float> rgb; };
+    std::vector<Image> images;
 };
 
 // A validated object-to-world map of one primitive: the linear part `a`,
@@ -65,7 +67,7 @@ struct Xform {
 
 // A scene compiled into SoA device buffers, resident in HBM.
 struct CompiledScene {
-    DeviceBuffer nodes, tris, tshade, prims, prim_class, analytic, mats, lobes, lights, perez, xforms;
+    DeviceBuffer nodes, tris, tshade, prims, prim_class, analytic, mats, lobes, lights, perez, xforms, images, texels;
     DevScene dev{};
     int max_depth = 1;
     bool area_lights = false;  // any DiffuseAreaLight (the only lights whose NEE needs closest hits)

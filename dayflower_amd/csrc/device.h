@@ -117,6 +117,8 @@ struct DevScene {
     // its object-space direction, rows 3-5 an object-space frame vector to
     // world space (read for primitives with `xformed` set)
     const float4* xforms;
+    const DevImageLight* images;  // image lights (DevLight::perez indexes them)
+    const float* texels;          // their images' RGB texels
     int32_t n_analytic;
     int32_t n_lights;
     int32_t has_bvh;
@@ -1390,6 +1392,7 @@ DEV bool sample_light(const DevScene& sc, const DevLight& L, const Surf& si, flo
     }
     if constexpr (EXT) {
         if (L.type == DFR_LIGHT_SPOT || L.type == DFR_LIGHT_DIRECTIONAL) return sample_delta_light_ext(L, si, out);
+        if (L.type == DFR_LIGHT_IMAGE) return image_sample(sc, sc.images[L.perez], si, u, v, out);
     }
     if (L.type == DFR_LIGHT_PEREZ) return perez_sample(sc.perez[L.perez], si, u, v, usun, out);
     if (L.tri < 0) return sample_sphere_light(sc, L, si, u, v, out);
@@ -1415,9 +1418,14 @@ DEV bool sample_light(const DevScene& sc, const DevLight& L, const Surf& si, flo
     return false;
 }
 
-// Shape3F.evaluateProbabilityDensityFunction for a triangle or sphere light
+// Shape3F.evaluateProbabilityDensityFunction for a triangle or sphere light;
+// the infinite lights' solid-angle pdfs (image lights in EXT kernels only)
+template <bool EXT>
 DEV float light_pdf(const DevScene& sc, const DevLight& L, const Surf& si, V wi) {
     if (L.type == DFR_LIGHT_PEREZ) return perez_pdf(sc.perez[L.perez], wi);
+    if constexpr (EXT) {
+        if (L.type == DFR_LIGHT_IMAGE) return image_pdf(sc.images[L.perez], wi);
+    }
     if (L.type != DFR_LIGHT_DIFFUSE_AREA) return 0.0f;
     if (L.tri < 0) return sphere_light_pdf(sc, L, si, wi);
     const size_t rb = 8 * (size_t)L.tri;
@@ -1444,8 +1452,14 @@ DEV V emitted(const DevScene& sc, const Surf& si, V dir) {
     return ld3(sc.mats[p.material].emission);
 }
 
+// Light.evaluateRadianceEmitted(ray): what a ray that leaves the scene sees
+// (image lights in EXT instantiations only)
+template <bool EXT>
 DEV V emitted_ray(const DevScene& sc, const DevLight& L, V d) {
     if (L.type == DFR_LIGHT_PEREZ) return perez_sky(sc.perez[L.perez], d);
+    if constexpr (EXT) {
+        if (L.type == DFR_LIGHT_IMAGE) return image_radiance(sc, sc.images[L.perez], d);
+    }
     return mk(0, 0, 0);
 }
 

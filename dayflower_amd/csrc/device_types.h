@@ -109,4 +109,25 @@ struct DevPerez {
     float cond_int[PEREZ_RES];
 };
 
+// ImageLight record (M/scene/light/ImageLight.java:134-150, 544-584): the
+// image (width x height RGB texels at `texel` in the scene's texel array),
+// its rotation and scale in texture space, the light's object frame and the
+// 32 x 32 importance distribution of luminance x sin(theta), laid out as the
+// Perez record's.
+struct DevImageLight {
+    int32_t width, height;
+    int64_t texel;               // first texel (3 floats each) in DevScene::texels
+    float cos_angle, sin_angle;
+    float scale_u, scale_v;
+    float radius;
+    float world_to_object[9];    // rotation, row major
+    float object_to_world[9];
+    float func_int;
+    float marginal_func[PEREZ_RES];
+    float marginal_cdf[PEREZ_RES + 1];
+    float cond_func[PEREZ_RES][PEREZ_RES];
+    float cond_cdf[PEREZ_RES][PEREZ_RES + 1];
+    float cond_int[PEREZ_RES];
+};
+
 }  // namespace dfr

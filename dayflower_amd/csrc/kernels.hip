@@ -166,7 +166,7 @@ DEV void nee_requests(const DevScene& sc, const Bsdf& b, const Surf& si, int li,
             const V wd = nrm(bs.wi);
             const V io = sky ? perez_to_object(sc.perez[L.perez], wd) : wd;
             if (!spec) {
-                float lp = sky ? perez_pdf_object(sc.perez[L.perez], io) : light_pdf(sc, L, si, bs.wi);
+                float lp = sky ? perez_pdf_object(sc.perez[L.perez], io) : light_pdf<EXT>(sc, L, si, bs.wi);
                 if (lp == 0.0f) ok = false;
                 else w = power_heuristic(bs.pdf, lp);
             }
@@ -179,7 +179,7 @@ DEV void nee_requests(const DevScene& sc, const Bsdf& b, const Surf& si, int li,
                 } else {
                     // an infinite light is seen exactly when nothing is hit:
                     // its radiance folds into c and the query is any-hit
-                    r.c = r.c * (sky ? perez_sky_object(sc.perez[L.perez], io) : emitted_ray(sc, L, r.d));
+                    r.c = r.c * (sky ? perez_sky_object(sc.perez[L.perez], io) : emitted_ray<EXT>(sc, L, r.d));
                     r.mode = black(r.c) ? -1 : 0;
                 }
             }
@@ -197,7 +197,8 @@ DEV V resolve_hit(const DevScene& sc, const ShadowReq& r, bool hit, const HitRec
         V Li = emitted(sc, s2, -r.d);
         return r.c * Li;
     }
-    return r.c * emitted_ray(sc, sc.lights[r.light], r.d);
+    // closest-hit requests aim at area lights, which a miss does not see
+    return r.c * emitted_ray<false>(sc, sc.lights[r.light], r.d);
 }
 
 template <int SHORT>
@@ -270,7 +271,7 @@ DEV V radiance_pt(const DevScene& sc, V o, V d, const Rng& rng, int max_bounce, 
             // the scene is white
             if (bounce == 0 && preview) L = mk(1.0f, 1.0f, 1.0f);
             else if (bounce == 0 || spec)
-                for (int i = 0; i < sc.n_lights; i++) L = L + beta * emitted_ray(sc, sc.lights[i], d);
+                for (int i = 0; i < sc.n_lights; i++) L = L + beta * emitted_ray<EXT>(sc, sc.lights[i], d);
             break;
         }
     }
@@ -328,7 +329,7 @@ DEV V radiance_rc(const DevScene& sc, V o, V d, bool preview, TraversalStack<SHO
     }
     if (preview) return mk(1.0f, 1.0f, 1.0f);  // Scene.java:692-693
     V L = mk(0, 0, 0);
-    for (int i = 0; i < sc.n_lights; i++) L = L + emitted_ray(sc, sc.lights[i], d);
+    for (int i = 0; i < sc.n_lights; i++) L = L + emitted_ray<true>(sc, sc.lights[i], d);
     return L;
 }
 
@@ -668,7 +669,8 @@ DEV uint32_t nee_light_type(const DevScene& sc, const Rng& rng, uint32_t bounce)
     if (sc.n_lights == 0) return 3u;
     const int li = min((int)(rng.get(DIM_BOUNCE + DIMS_PER_BOUNCE * bounce) * sc.n_lights), sc.n_lights - 1);
     const int t = sc.lights[li].type;
-    return t == DFR_LIGHT_PEREZ ? 2u : (t == DFR_LIGHT_DIFFUSE_AREA ? 0u : 1u);  // delta lights share class 1
+    // infinite lights share class 2, delta lights class 1
+    return t == DFR_LIGHT_PEREZ || t == DFR_LIGHT_IMAGE ? 2u : (t == DFR_LIGHT_DIFFUSE_AREA ? 0u : 1u);
 }
 
 // The job of work item `item` through a generation-order table: entries in
@@ -1150,7 +1152,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_shade(const DevScene* __restrict__
             } else if (bounce == 0 && job.preview) {
                 L = mk(1.0f, 1.0f, 1.0f);  // Scene.java:603-606
             } else if (bounce == 0 || spec) {
-                for (int l = 0; l < sc.n_lights; l++) L = L + beta * emitted_ray(sc, sc.lights[l], d);
+                for (int l = 0; l < sc.n_lights; l++) L = L + beta * emitted_ray<EXT>(sc, sc.lights[l], d);
             }
             if (!black(L)) film_add(film, film_idx, L);
             // each NEE request carries the film index of its path

@@ -221,4 +221,84 @@ void perez_build(DevPerez& p, float turbidity, const float sun_direction_world[3
     p.func_int = dist1(p.marginal_func, R, p.marginal_cdf);
 }
 
+// AngleF.degrees(d): wrapped into [0, 360] (Floats.wrapAround), then radians.
+static float wrap_degrees(float d) {
+    const float span = std::nextafter(360.0f, 1e9f);
+    if (d >= 0.0f && d <= 360.0f) return d;
+    if (d < 0.0f) return 360.0f - std::fmod(0.0f - d, span);
+    return 0.0f + std::fmod(d - 360.0f, span);
+}
+
+// Floats.positiveModulo
+static float pmodf(float x, float y) { return x < 0.0f ? std::fmod(std::fmod(x, y) + y, y) : std::fmod(x, y); }
+static int pmodi(int x, int y) { return x < 0 ? (x % y + y) % y : x % y; }
+
+// ImageLight.doRadianceSky for an object-space direction: its spherical
+// coordinates, rotated counterclockwise, scaled, moved to image space and
+// looked up bilinearly with wrap-around (ImageF.getColorRGB(x, y,
+// WRAP_AROUND)).
+void image_light_radiance(const DevImageLight& L, const float* texels, const float d[3], float rgb[3]) {
+    float phi = (float)std::atan2((double)d[1], (double)d[0]);
+    if (phi < 0.0f) phi += 2.0f * PI_F;
+    const float u = phi * (1.0f / (2.0f * PI_F));
+    const float v = (float)std::acos((double)std::min(std::max(d[2], -1.0f), 1.0f)) * (1.0f / PI_F);
+    const float ru = (u * L.cos_angle - v * L.sin_angle) * L.scale_u;
+    const float rv = (v * L.cos_angle + u * L.sin_angle) * L.scale_v;
+    const float x = pmodf(ru * (float)L.width - 0.5f, (float)L.width);
+    const float y = pmodf(rv * (float)L.height - 0.5f, (float)L.height);
+    const int x0 = (int)std::floor(x), x1 = (int)std::ceil(x), y0 = (int)std::floor(y), y1 = (int)std::ceil(y);
+    auto texel = [&](int tx, int ty, float* c) {
+        const float* t = texels + L.texel + 3 * ((int64_t)pmodi(ty, L.height) * L.width + pmodi(tx, L.width));
+        c[0] = t[0]; c[1] = t[1]; c[2] = t[2];
+    };
+    if (x0 == x1 && y0 == y1) { texel(x0, y0, rgb); return; }
+    float c00[3], c01[3], c10[3], c11[3];
+    texel(x0, y0, c00); texel(x1, y0, c01); texel(x0, y1, c10); texel(x1, y1, c11);
+    const float fx = x - (float)x0, fy = y - (float)y0;
+    for (int k = 0; k < 3; k++) {
+        const float a = (1.0f - fx) * c00[k] + fx * c01[k];
+        const float b = (1.0f - fx) * c10[k] + fx * c11[k];
+        rgb[k] = (1.0f - fy) * a + fy * b;
+    }
+}
+
+void image_light_build(DevImageLight& L, int width, int height, int64_t texel_offset, const float* texels,
+                       float angle_degrees, float scale_u, float scale_v, float radius) {
+    std::memset(&L, 0, sizeof(L));
+    L.width = width;
+    L.height = height;
+    L.texel = texel_offset;
+    const float rad = (float)(wrap_degrees(angle_degrees) * (3.14159265358979323846 / 180.0));
+    L.cos_angle = (float)std::cos((double)rad);
+    L.sin_angle = (float)std::sin((double)rad);
+    L.scale_u = scale_u;
+    L.scale_v = scale_v;
+    L.radius = radius;
+    // the same object frame as PerezLight's: Matrix44F.rotate(OrthonormalBasis33F(y, x))
+    const float o2w[9] = {0, 1, 0, 0, 0, 1, 1, 0, 0};
+    const float w2o[9] = {0, 0, 1, 1, 0, 0, 0, 1, 0};
+    std::memcpy(L.object_to_world, o2w, sizeof(o2w));
+    std::memcpy(L.world_to_object, w2o, sizeof(w2o));
+    // doCreateDistribution: luminance x sin(theta) at the cell centres
+    const int R = PEREZ_RES;
+    float fn[PEREZ_RES];
+    for (int u = 0; u < R; u++) {
+        const float su = (u + 0.5f) * (1.0f / R);
+        for (int v = 0; v < R; v++) {
+            const float sv = (v + 0.5f) * (1.0f / R);
+            const float st = (float)std::sin((double)(PI_F * sv));
+            const float phi = su * 2.0f * PI_F, t = sv * PI_F;
+            const float sth = (float)std::sin((double)t);
+            const float d[3] = {sth * (float)std::cos((double)phi), sth * (float)std::sin((double)phi), (float)std::cos((double)t)};
+            float rgb[3];
+            image_light_radiance(L, texels, d, rgb);
+            L.cond_func[u][v] = (rgb[0] * 0.212671f + rgb[1] * 0.715160f + rgb[2] * 0.072169f) * st;
+        }
+        L.cond_int[u] = dist1(L.cond_func[u], R, L.cond_cdf[u]);
+        fn[u] = L.cond_int[u];
+    }
+    std::memcpy(L.marginal_func, fn, sizeof(fn));
+    L.func_int = dist1(L.marginal_func, R, L.marginal_cdf);
+}
+
 }  // namespace dfr

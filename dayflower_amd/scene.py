@@ -156,6 +156,7 @@ class Scene:
         self.primitives = []
         self.materials = []
         self.lights = []
+        self.images = []
         self.camera = camera_record((0, 0, -3), (0, 0, 0), 800, 800)
         self._keep = None
 
@@ -281,6 +282,24 @@ class Scene:
         self.lights.append(L)
         return len(self.lights) - 1
 
+    def add_image_light(self, image, angle_deg=0.0, scale=(1.0, 1.0), radius=10.0):
+        """ImageLight(image, angle, scale) (M/scene/light/ImageLight.java:134):
+        `image` is an (H, W, 3) array of linear RGB, row 0 first."""
+        im = np.ascontiguousarray(image, np.float32)
+        if im.ndim != 3 or im.shape[2] != 3 or im.shape[0] < 1 or im.shape[1] < 1:
+            raise ValueError("image must be (H, W, 3)")
+        self.images.append(im)
+        li = len(self.lights)
+        L = abi.Light()
+        L.type = abi.LIGHT_IMAGE
+        L.primitive = len(self.images) - 1
+        L.triangle = -1
+        L.cone_angle = float(angle_deg)
+        L.direction[0], L.direction[1] = float(scale[0]), float(scale[1])
+        L.radius = float(radius)
+        self.lights.append(L)
+        return li
+
     def add_perez_light(self, turbidity=2.0, sun_direction=(1.0, 1.0, -1.0)):
         """PerezLight(turbidity, sunDirectionWorldSpace) (M/scene/light/PerezLight.java)."""
         L = abi.Light()
@@ -325,7 +344,14 @@ class Scene:
         d.lights = lights
         for i in range(abi.CAMERA_LENGTH):
             d.camera[i] = float(self.camera[i])
-        self._keep = (pos, nrm, idx, prims, mats, lights)
+        imgs = [np.ascontiguousarray(im, np.float32) for im in self.images]
+        recs = (abi.Image * max(1, len(imgs)))()
+        for k, im in enumerate(imgs):
+            recs[k].height, recs[k].width = int(im.shape[0]), int(im.shape[1])
+            recs[k].rgb = im.ctypes.data_as(fp)
+        d.n_images = len(imgs)
+        d.images = recs if imgs else None
+        self._keep = (pos, nrm, idx, prims, mats, lights, imgs, recs)
         return d
 
 

@@ -78,12 +78,13 @@ extern "C" {
 #define DFR_MAT_UBER       10
 
 /* Lights: M/scene/light/{DiffuseAreaLight,PointLight,PerezLight,SpotLight,
- * DirectionalLight}.java */
+ * DirectionalLight,ImageLight}.java */
 #define DFR_LIGHT_DIFFUSE_AREA 0
 #define DFR_LIGHT_POINT        1
 #define DFR_LIGHT_PEREZ        2
 #define DFR_LIGHT_SPOT         3
 #define DFR_LIGHT_DIRECTIONAL  4
+#define DFR_LIGHT_IMAGE        5
 
 /* Camera lens ordinals: M/scene/Lens.java:31-36 */
 #define DFR_LENS_FISHEYE 0
@@ -181,7 +182,14 @@ typedef struct dfr_primitive {
  *                 (world space), cone_angle and cone_delta in degrees
  *   DIRECTIONAL   radiance, direction = towards the light (world space,
  *                 normalised by the compiler), radius = scene radius (the
- *                 shadow ray's length is twice it) */
+ *                 shadow ray's length is twice it)
+ *   IMAGE         an environment image (ImageLight.java:134-150, 544-584):
+ *                 primitive = index into the scene's images, cone_angle =
+ *                 the image's rotation in degrees (AngleF), direction[0..1]
+ *                 = its scale (u, v; (1, 1) when both are 0), radius (10 when
+ *                 0; sampled directions end at twice it); looked up at the
+ *                 spherical coordinates of the direction in the light's
+ *                 object frame with bilinear filtering and wrap-around */
 typedef struct dfr_light {
     int32_t type;
     int32_t primitive;
@@ -198,6 +206,13 @@ typedef struct dfr_light {
     float reserved1;
 } dfr_light;
 
+/* One image: width * height linear RGB texels (3 floats each), row 0 first
+ * (an ImageF's colours; an 8-bit image's bytes divided by 255). */
+typedef struct dfr_image {
+    int32_t width, height;
+    const float* rgb;
+} dfr_image;
+
 typedef struct dfr_scene_desc {
     int64_t n_vertices;
     const float* positions;   /* 3 * n_vertices                         */
@@ -212,6 +227,8 @@ typedef struct dfr_scene_desc {
     int32_t n_lights;
     const dfr_light* lights;
     float camera[DFR_CAMERA_LENGTH];
+    int32_t n_images;
+    const struct dfr_image* images;  /* images of IMAGE lights, or NULL     */
 } dfr_scene_desc;
 
 /* ------------------------------------------------------------------------ */

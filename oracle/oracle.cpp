@@ -78,8 +78,9 @@ struct Light {
     bool twoSided;
     C3 radiance;
     V3 position;
-    // Perez sky (filled by perezInit)
+    // Perez sky (filled by perezInit), image light (imageInit)
     struct Perez* perez;
+    struct ImageL* image = nullptr;
     // SpotLight: cone axis, cos(cone angle), cos(cone angle - delta);
     // DirectionalLight: unit direction towards the light, scene radius
     V3 axis;
@@ -1251,7 +1252,9 @@ struct LightSample { C3 L; V3 p; V3 wi; float pdf; };
 
 namespace {
 
-Scene::~Scene() { for (auto& l : lights) delete l.perez; }
+Scene::~Scene() {
+    for (auto& l : lights) { delete l.perez; delete l.image; }
+}
 
 float triArea(const Tri& t) { return 0.5f * length(cross(t.b - t.a, t.c - t.a)); }
 
@@ -1379,6 +1382,7 @@ bool sampleLight(const Scene& sc, const Light& L, const Surface& si, float u, fl
         return true;
     }
     if (L.type == DFR_LIGHT_PEREZ) return perezSample(*L.perez, si, u, v, uSun, out);
+    if (L.type == DFR_LIGHT_IMAGE) return imageSample(*L.image, si, u, v, out);
     if (L.tri < 0) {
         if (L.prim < 0 || sc.prims[L.prim].shape != DFR_SHAPE_SPHERE) return false;
         return sampleSphereLight(sc.prims[L.prim], L, si, u, v, pdfMode, out);
@@ -1406,6 +1410,7 @@ bool sampleLight(const Scene& sc, const Light& L, const Surface& si, float u, fl
 // Shape3F.evaluateProbabilityDensityFunction (M/geometry/Shape3F.java:199-217).
 float lightPdf(const Scene& sc, const Light& L, const Surface& si, V3 wi, int pdfMode) {
     if (L.type == DFR_LIGHT_PEREZ) return perezPdf(*L.perez, wi);
+    if (L.type == DFR_LIGHT_IMAGE) return imagePdf(*L.image, wi);
     if (L.type != DFR_LIGHT_DIFFUSE_AREA) return 0.0f;
     if (L.tri < 0) {
         if (L.prim < 0 || sc.prims[L.prim].shape != DFR_SHAPE_SPHERE) return 0.0f;
@@ -1432,9 +1437,10 @@ C3 emitted(const Scene& sc, const Surface& si, V3 dir) {
     return col(sc.mats[p.material].emission);
 }
 
-// Light.evaluateRadianceEmitted(ray): only the Perez sky emits on a miss.
+// Light.evaluateRadianceEmitted(ray): the Perez sky and image lights emit on a miss.
 C3 emittedRay(const Light& L, V3 d) {
     if (L.type == DFR_LIGHT_PEREZ) return perezSky(*L.perez, d);
+    if (L.type == DFR_LIGHT_IMAGE) return imageSky(*L.image, d);
     return C3(0.0f);
 }
 
@@ -1822,6 +1828,15 @@ orc_scene* orc_scene_create(const dfr_scene_desc* d) {
             L.perez = new Perez();
             perezInit(*L.perez, l.turbidity, V3(l.direction[0], l.direction[1], l.direction[2]));
         }
+        if (l.type == DFR_LIGHT_IMAGE) {
+            if (l.primitive < 0 || l.primitive >= d->n_images || !d->images) return nullptr;
+            const dfr_image& im = d->images[l.primitive];
+            if (im.width < 1 || im.height < 1 || !im.rgb) return nullptr;
+            float su = l.direction[0], sv = l.direction[1];
+            if (su == 0.0f && sv == 0.0f) su = sv = 1.0f;
+            L.image = new ImageL();
+            imageInit(*L.image, im.width, im.height, im.rgb, l.cone_angle, su, sv, l.radius > 0.0f ? l.radius : 10.0f);
+        }
         if (l.type == DFR_LIGHT_SPOT || l.type == DFR_LIGHT_DIRECTIONAL) {
             const double x = l.direction[0], y = l.direction[1], z = l.direction[2], n = std::sqrt(x * x + y * y + z * z);
             L.axis = V3((float)(x / n), (float)(y / n), (float)(z / n));
@@ -2140,6 +2155,35 @@ int32_t orc_film_render(int64_t n, const float* film, float* rgb) {
         const C3 splat = xyzToRgb(cs, C3(0.0f, 0.0f, 0.0f)) * 1.0f;
         c = c + splat;
         rgb[3 * i] = redoGamma(cs, c.r); rgb[3 * i + 1] = redoGamma(cs, c.g); rgb[3 * i + 2] = redoGamma(cs, c.b);
+    }
+    return 0;
+}
+
+// An ImageLight alone, for tests: radiance and solid-angle pdf along world
+// directions (n x 3), and samples for (u, v) pairs: direction, pdf, radiance
+// (pdf 0 where the reference returns no sample).
+int32_t orc_image_light(int32_t w, int32_t h, const float* rgb, float angle_degrees, float su, float sv, int64_t n,
+                        const float* dirs, float* radiance, float* pdf, int64_t n_samples, const float* uv,
+                        float* sample_dirs, float* sample_pdf, float* sample_radiance) {
+    if (w < 1 || h < 1 || !rgb) return -1;
+    ImageL im;
+    imageInit(im, w, h, rgb, angle_degrees, su, sv, 10.0f);
+    for (int64_t i = 0; i < n; i++) {
+        const V3 d(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
+        const C3 c = imageSky(im, d);
+        radiance[3 * i] = c.r; radiance[3 * i + 1] = c.g; radiance[3 * i + 2] = c.b;
+        pdf[i] = imagePdf(im, d);
+    }
+    Surface si{};
+    for (int64_t i = 0; i < n_samples; i++) {
+        LightSample ls;
+        if (imageSample(im, si, uv[2 * i], uv[2 * i + 1], &ls)) {
+            sample_dirs[3 * i] = ls.wi.x; sample_dirs[3 * i + 1] = ls.wi.y; sample_dirs[3 * i + 2] = ls.wi.z;
+            sample_pdf[i] = ls.pdf;
+            sample_radiance[3 * i] = ls.L.r; sample_radiance[3 * i + 1] = ls.L.g; sample_radiance[3 * i + 2] = ls.L.b;
+        } else {
+            sample_pdf[i] = 0.0f;
+        }
     }
     return 0;
 }

@@ -35,6 +35,13 @@
  *                   is found on the primitive's world-space image, with the
  *                   0.001 minimum distance in world space (the reference
  *                   applies it to the object-space ray)
+ *   image light     a constant image: constant radiance and a pdf that
+ *                   integrates to one; texel lookup at directions whose image
+ *                   coordinates are known in closed form (wrap-around at the
+ *                   seam, the object frame, rotation and scale); samples carry
+ *                   the evaluated pdf and follow it; a Lambertian plane under
+ *                   a constant sky renders to albedo x radiance
+ *                   (tests/test_image_light.py)
  *   CPU film        filter table against the Gaussian in closed form; a
  *                   splat covers exactly the pixels within the radius, with
  *                   weights symmetric and largest at the nearest centre; the
@@ -95,6 +102,12 @@ int32_t orc_render(const orc_scene* scene, const orc_params* params, double* sum
  * reference stores them) and the gamma constants (break point, slope, slope
  * match, segment offset).  Any output may be NULL. */
 int32_t orc_render_film(const orc_scene* scene, const orc_params* params, float* film);
+/* An ImageLight alone (tests): radiance and pdf along n world directions;
+ * samples for n_samples (u, v) pairs (direction, pdf, radiance; pdf 0 when
+ * there is no sample). */
+int32_t orc_image_light(int32_t w, int32_t h, const float* rgb, float angle_degrees, float su, float sv, int64_t n,
+                        const float* dirs, float* radiance, float* pdf, int64_t n_samples, const float* uv,
+                        float* sample_dirs, float* sample_pdf, float* sample_radiance);
 int32_t orc_film_render(int64_t n, const float* film, float* rgb);
 int32_t orc_film_splat(int32_t width, int32_t height, float x, float y, const float* xyz, float* film);
 int32_t orc_film_constants(float* table256, float* xyz_to_rgb12, float* rgb_to_xyz12, float* gamma4);

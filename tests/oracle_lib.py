@@ -47,6 +47,8 @@ def load():
     lib.orc_scene_set_camera.argtypes = [vp, vp]
     lib.orc_scene_set_transforms.argtypes = [vp, vp, ctypes.c_int32]
     lib.orc_render_film.argtypes = [vp, vp, vp]
+    lib.orc_image_light.argtypes = [ctypes.c_int32, ctypes.c_int32, vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_int64, vp, vp, vp, ctypes.c_int64, vp, vp, vp, vp]
     lib.orc_film_render.argtypes = [ctypes.c_int64, vp, vp]
     lib.orc_film_splat.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_float, vp, vp]
     lib.orc_film_constants.argtypes = [vp, vp, vp, vp]
@@ -261,3 +263,19 @@ def film_constants():
     g = np.zeros(4, np.float32)
     load().orc_film_constants(_ptr(t), _ptr(a), _ptr(b), _ptr(g))
     return t.reshape(16, 16), a, b, g
+
+
+def image_light(image, dirs, uv=None, angle=0.0, scale=(1.0, 1.0)):
+    """The oracle's ImageLight alone: (radiance, pdf) along world `dirs`
+    (n, 3), and for `uv` (m, 2) the samples (dirs, pdf, radiance)."""
+    im = np.ascontiguousarray(image, np.float32)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    rad = np.zeros_like(d)
+    pdf = np.zeros(len(d), np.float32)
+    u = np.ascontiguousarray(np.zeros((0, 2)) if uv is None else uv, np.float32).reshape(-1, 2)
+    sd = np.zeros((len(u), 3), np.float32)
+    sp = np.zeros(len(u), np.float32)
+    sr = np.zeros((len(u), 3), np.float32)
+    load().orc_image_light(im.shape[1], im.shape[0], _ptr(im), float(angle), float(scale[0]), float(scale[1]), len(d),
+                           _ptr(d), _ptr(rad), _ptr(pdf), len(u), _ptr(u), _ptr(sd), _ptr(sp), _ptr(sr))
+    return rad, pdf, sd, sp, sr
