@@ -57,7 +57,19 @@ class Material(ctypes.Structure):
         ("disney_anisotropic", c_float), ("disney_sheen", c_float), ("disney_sheen_tint", c_float),
         ("disney_clearcoat", c_float), ("disney_clearcoat_gloss", c_float), ("disney_spec_trans", c_float),
         ("disney_diff_trans", c_float), ("disney_flatness", c_float),
+        ("textures", c_int32 * 4),
     ]
+
+
+SLOT_KD, SLOT_KS, SLOT_KR, SLOT_KT = range(4)
+(TEX_CONSTANT, TEX_CHECKERBOARD, TEX_UV, TEX_SURFACE_NORMAL, TEX_DOT_PRODUCT, TEX_BULLSEYE, TEX_POLKA_DOT, TEX_BLEND,
+ TEX_IMAGE) = range(9)
+
+
+class Texture(ctypes.Structure):
+    _fields_ = [("type", c_int32), ("a", c_int32), ("b", c_int32), ("image", c_int32),
+                ("color", c_float * 3), ("angle", c_float), ("scale", c_float * 2), ("origin", c_float * 3),
+                ("cell", c_float), ("radius", c_float), ("t", c_float * 3)]
 
 
 class Primitive(ctypes.Structure):
@@ -90,6 +102,7 @@ class SceneDesc(ctypes.Structure):
         ("n_lights", c_int32), ("lights", ctypes.POINTER(Light)),
         ("camera", c_float * CAMERA_LENGTH),
         ("n_images", c_int32), ("images", ctypes.POINTER(Image)),
+        ("n_textures", c_int32), ("textures", ctypes.POINTER(Texture)),
     ]
 
 

@@ -4,6 +4,7 @@
 #include "api_internal.h"
 
 #include <algorithm>
+#include <functional>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -36,6 +37,10 @@ void compile_material(const dfr_material& m, DevMaterial& out, std::vector<DevLo
     out.type = m.type;
     out.eta = 1.0f;
     for (int k = 0; k < 3; k++) out.emission[k] = m.emission[k];
+    // a textured material keeps every lobe whatever its constant colours:
+    // the lobes present at a hit follow from the textures there
+    const bool textured = m.textures[0] || m.textures[1] || m.textures[2] || m.textures[3];
+    for (int k = 0; k < 4; k++) out.tex[k] = m.textures[k];
     auto add = [&](const DevLobe& l) { lobes.push_back(l); out.n_lobes++; };
     auto tr = [](DevLobe& l, float ax, float ay, bool sep) {
         l.alphaX = std::max(ax, 0.001f); l.alphaY = std::max(ay, 0.001f); l.separable = sep ? 1 : 0;
@@ -54,14 +59,14 @@ void compile_material(const dfr_material& m, DevMaterial& out, std::vector<DevLo
         }
         case DFR_MAT_MIRROR: {
             DevLobe l{}; l.kind = LOBE_SPECULAR_R; l.type = BX_SPEC | BX_REFL; sat3(l.R, m.kr); l.fresnel = FR_CONST;
-            if (black3(l.R)) { out.has_bsdf = 0; break; }
+            if (black3(l.R) && !textured) { out.has_bsdf = 0; break; }
             add(l);
             break;
         }
         case DFR_MAT_GLASS: {
             float kr[3], kt[3]; sat3(kr, m.kr); sat3(kt, m.kt);
             float eta = m.eta[0];
-            if (black3(kr) && black3(kt)) { out.has_bsdf = 0; break; }
+            if (black3(kr) && black3(kt) && !textured) { out.has_bsdf = 0; break; }
             out.eta = eta;
             if (m.roughness_u == 0.0f && m.roughness_v == 0.0f) {
                 DevLobe l{}; l.kind = LOBE_FRESNEL_SPECULAR; l.type = BX_SPEC | BX_REFL | BX_TRANS;
@@ -71,12 +76,12 @@ void compile_material(const dfr_material& m, DevMaterial& out, std::vector<DevLo
             }
             float ru = m.remap_roughness ? roughness_to_alpha(m.roughness_u) : m.roughness_u;
             float rv = m.remap_roughness ? roughness_to_alpha(m.roughness_v) : m.roughness_v;
-            if (!black3(kr)) {
+            if (!black3(kr) || textured) {
                 DevLobe l{}; l.kind = LOBE_TS_R; l.type = BX_GLOSSY | BX_REFL; std::memcpy(l.R, kr, 12);
                 tr(l, ru, rv, false); l.fresnel = FR_DIELECTRIC; l.frEtaI = 1.0f; l.frEtaT = eta;
                 add(l);
             }
-            if (!black3(kt)) {
+            if (!black3(kt) || textured) {
                 DevLobe l{}; l.kind = LOBE_TS_T; l.type = BX_GLOSSY | BX_TRANS; std::memcpy(l.T, kt, 12);
                 tr(l, ru, rv, false); l.etaA = 1.0f; l.etaB = eta;
                 add(l);
@@ -94,8 +99,8 @@ void compile_material(const dfr_material& m, DevMaterial& out, std::vector<DevLo
         }
         case DFR_MAT_PLASTIC: {
             float kd[3], ks[3]; sat3(kd, m.kd); sat3(ks, m.ks);
-            if (!black3(kd)) { DevLobe l{}; l.kind = LOBE_LAMBERT; l.type = BX_DIFF | BX_REFL; std::memcpy(l.R, kd, 12); add(l); }
-            if (!black3(ks)) {
+            if (!black3(kd) || textured) { DevLobe l{}; l.kind = LOBE_LAMBERT; l.type = BX_DIFF | BX_REFL; std::memcpy(l.R, kd, 12); add(l); }
+            if (!black3(ks) || textured) {
                 float r = m.remap_roughness ? roughness_to_alpha(m.roughness_u) : m.roughness_u;
                 DevLobe l{}; l.kind = LOBE_TS_R; l.type = BX_GLOSSY | BX_REFL; std::memcpy(l.R, ks, 12);
                 tr(l, r, r, false); l.fresnel = FR_DIELECTRIC; l.frEtaI = 1.5f; l.frEtaT = 1.0f;
@@ -440,6 +445,7 @@ struct Baked {
     std::vector<float4> xforms;   // per primitive: DevScene::xforms
     std::vector<DevImageLight> images;
     std::vector<float> texels;
+    std::vector<DevTexture> textures;
 };
 
 static int bake_scene(const SceneHost& in, const std::vector<Xform>& xf, const std::vector<int32_t>& tri_prim,
@@ -468,7 +474,15 @@ static int bake_scene(const SceneHost& in, const std::vector<Xform>& xf, const s
         r[2] = f4(P[6], P[7], P[8], N[2]);
         r[3] = f4(N[3], N[4], N[5], ng[0]);
         r[4] = f4(N[6], N[7], N[8], ng[1]);
-        r[5] = f4(ng[2], 0.0f, 0.0f, 0.0f);
+        // texture coordinates of a, b, c (zero without them)
+        float uv[6] = {0, 0, 0, 0, 0, 0};
+        if (!in.uvs.empty())
+            for (int k = 0; k < 3; k++) {
+                const int64_t v = in.indices[3 * t + k];
+                uv[2 * k] = in.uvs[2 * v]; uv[2 * k + 1] = in.uvs[2 * v + 1];
+            }
+        r[5] = f4(ng[2], uv[0], uv[1], uv[2]);
+        r[6] = f4(uv[3], uv[4], uv[5], 0.0f);
     }
     b.tris.resize(3 * nt);
     for (int64_t i = 0; i < nt; i++) {
@@ -546,6 +560,61 @@ static int bake_scene(const SceneHost& in, const std::vector<Xform>& xf, const s
     b.perez.clear();
     b.images.clear();
     b.texels.clear();
+    // textures: image textures' texels go into the shared texel array
+    b.textures.assign(std::max<size_t>(1, in.textures.size()), DevTexture{});
+    for (size_t t = 0; t < in.textures.size(); t++) {
+        const dfr_texture& T = in.textures[t];
+        DevTexture& d = b.textures[t];
+        std::memset(&d, 0, sizeof(d));
+        d.type = T.type; d.a = T.a; d.b = T.b;
+        std::memcpy(d.color, T.color, 12);
+        std::memcpy(d.origin, T.origin, 12);
+        std::memcpy(d.t, T.t, 12);
+        d.cell = T.cell; d.radius = T.radius;
+        d.scale_u = T.scale[0]; d.scale_v = T.scale[1];
+        // AngleF.degrees: wrapped into [0, 360], radians, cos and sin in double
+        float deg = T.angle;
+        if (!(deg >= 0.0f && deg <= 360.0f)) {
+            const float span = std::nextafter(360.0f, 1e9f);
+            deg = deg < 0.0f ? 360.0f - std::fmod(0.0f - deg, span) : 0.0f + std::fmod(deg - 360.0f, span);
+        }
+        const float rad = (float)(deg * (3.14159265358979323846 / 180.0));
+        d.cos_angle = (float)std::cos((double)rad);
+        d.sin_angle = (float)std::sin((double)rad);
+        if (T.type == DFR_TEX_IMAGE) {
+            const SceneHost::Image& im = in.images[T.image];
+            d.width = im.width; d.height = im.height;
+            d.texel = (int64_t)b.texels.size();
+            b.texels.insert(b.texels.end(), im.rgb.begin(), im.rgb.end());
+        }
+    }
+    // a textured material's coordinates on a transformed rectangle or plane,
+    // and a bullseye's object-space distance, need the inverse map: refused
+    for (size_t p = 0; p < in.prims.size(); p++) {
+        if (xf.empty() || xf[p].identity) continue;
+        const dfr_material& M = in.mats[in.prims[p].material];
+        if (!(M.textures[0] || M.textures[1] || M.textures[2] || M.textures[3])) continue;
+        const int shape = in.prims[p].shape;
+        if (shape == DFR_SHAPE_RECTANGLE || shape == DFR_SHAPE_PLANE) {
+            err = "primitive " + std::to_string(p) + ": a textured rectangle or plane takes no transform";
+            return DFR_E_UNSUPPORTED;
+        }
+        for (int k = 0; k < 4; k++) {
+            std::vector<int> todo;
+            if (M.textures[k]) todo.push_back(M.textures[k] - 1);
+            while (!todo.empty()) {
+                const dfr_texture& T = in.textures[todo.back()];
+                todo.pop_back();
+                if (T.type == DFR_TEX_BULLSEYE) {
+                    err = "primitive " + std::to_string(p) + ": a bullseye texture on a transformed primitive";
+                    return DFR_E_UNSUPPORTED;
+                }
+                if (T.type == DFR_TEX_CHECKERBOARD || T.type == DFR_TEX_POLKA_DOT || T.type == DFR_TEX_BLEND) {
+                    todo.push_back(T.a); todo.push_back(T.b);
+                }
+            }
+        }
+    }
     for (size_t l = 0; l < in.lights.size(); l++) {
         const dfr_light& L = in.lights[l];
         DevLight& d = b.lights[l];
@@ -625,6 +694,7 @@ static int upload_baked(const Baked& bk, CompiledScene& out, hipStream_t s) {
     if ((rc = upload(out.xforms, bk.xforms, s))) return rc;
     if ((rc = upload(out.images, bk.images, s))) return rc;
     if ((rc = upload(out.texels, bk.texels, s))) return rc;
+    if ((rc = upload(out.textures, bk.textures, s))) return rc;
     return DFR_OK;
 }
 
@@ -677,7 +747,10 @@ int compile_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledSce
     std::vector<DevMaterial> mats(in.mats.size());
     std::vector<DevLobe> lobes;
     for (size_t m = 0; m < in.mats.size(); m++) compile_material(in.mats[m], mats[m], lobes);
-    for (const dfr_material& m : in.mats) out.ext_features = out.ext_features || m.type >= DFR_MAT_GLOSSY;
+    // the extended materials and every textured material run in the EXT kernels
+    for (const dfr_material& m : in.mats)
+        out.ext_features = out.ext_features || m.type >= DFR_MAT_GLOSSY || m.textures[0] || m.textures[1] ||
+                           m.textures[2] || m.textures[3];
     // traversal addresses nodes and triangles with 32-bit byte offsets
     const size_t node_bytes = BVH_WIDTH == 4 ? bvh.nodes4.size() * sizeof(bvh.nodes4[0]) : bvh.nodes.size() * sizeof(bvh.nodes[0]);
     if (node_bytes >= (size_t(1) << 32) || bk.tris.size() * sizeof(float4) >= (size_t(1) << 32)) {
@@ -713,6 +786,7 @@ int compile_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledSce
     ds.xforms = (const float4*)out.xforms.ptr;
     ds.images = (const DevImageLight*)out.images.ptr;
     ds.texels = (const float*)out.texels.ptr;
+    ds.textures = (const DevTexture*)out.textures.ptr;
     ds.n_analytic = (int32_t)bk.analytic.size();
     ds.n_lights = (int32_t)bk.lights.size();
     for (const DevLight& L : bk.lights) {
@@ -725,7 +799,7 @@ int compile_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledSce
     std::memcpy(out.camera, in.camera, sizeof(out.camera));
     out.bytes = out.nodes.bytes + out.tris.bytes + out.tshade.bytes + out.prims.bytes + out.analytic.bytes +
                 out.mats.bytes + out.lobes.bytes + out.lights.bytes + out.perez.bytes + out.xforms.bytes +
-                out.images.bytes + out.texels.bytes;
+                out.images.bytes + out.texels.bytes + out.textures.bytes;
     if (hipStreamSynchronize(s) != hipSuccess) { err = "upload failed"; return DFR_E_DEVICE; }
     return DFR_OK;
 }
@@ -810,6 +884,10 @@ int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err) {
     h.mats.assign(d->materials, d->materials + d->n_materials);
     h.lights.assign(d->lights, d->lights + d->n_lights);
     std::memcpy(h.camera, d->camera, sizeof(h.camera));
+    h.uvs.clear();
+    if (d->uvs && d->n_vertices > 0) h.uvs.assign(d->uvs, d->uvs + 2 * d->n_vertices);
+    if (d->n_textures < 0 || (d->n_textures && !d->textures)) { err = "textures == null"; return DFR_E_NULL; }
+    h.textures.assign(d->textures, d->textures + d->n_textures);
     if (d->n_images < 0 || (d->n_images && !d->images)) { err = "images == null"; return DFR_E_NULL; }
     h.images.assign(d->n_images, SceneHost::Image{});
     for (int32_t i = 0; i < d->n_images; i++) {
@@ -820,6 +898,49 @@ int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err) {
         h.images[i].width = im.width;
         h.images[i].height = im.height;
         h.images[i].rgb.assign(im.rgb, im.rgb + 3 * (size_t)im.width * im.height);
+    }
+    // textures: indices in range, blends of non-blends, no cycles; material
+    // slots only where the kernels colour a lobe from them
+    const int nt = (int)h.textures.size();
+    for (int t = 0; t < nt; t++) {
+        const dfr_texture& T = h.textures[t];
+        const std::string who = "texture " + std::to_string(t);
+        if (T.type < DFR_TEX_CONSTANT || T.type > DFR_TEX_IMAGE) { err = who + ": type out of range"; return DFR_E_UNSUPPORTED; }
+        const bool two = T.type == DFR_TEX_CHECKERBOARD || T.type == DFR_TEX_BULLSEYE || T.type == DFR_TEX_POLKA_DOT ||
+                         T.type == DFR_TEX_BLEND;
+        if (two && (T.a < 0 || T.a >= nt || T.b < 0 || T.b >= nt)) { err = who + ": sub-texture out of range"; return DFR_E_ARGUMENT; }
+        if (T.type == DFR_TEX_BLEND && (h.textures[T.a].type == DFR_TEX_BLEND || h.textures[T.b].type == DFR_TEX_BLEND)) {
+            err = who + ": a blend of blends"; return DFR_E_UNSUPPORTED;
+        }
+        if (T.type == DFR_TEX_IMAGE && (T.image < 0 || T.image >= (int)h.images.size())) { err = who + ": no such image"; return DFR_E_ARGUMENT; }
+    }
+    {   // a chain of sub-textures must end: depth-first search for cycles
+        std::vector<int> state(nt, 0);
+        std::function<bool(int)> cyclic = [&](int t) {
+            if (state[t] == 1) return true;
+            if (state[t] == 2) return false;
+            state[t] = 1;
+            const dfr_texture& T = h.textures[t];
+            const bool two = T.type == DFR_TEX_CHECKERBOARD || T.type == DFR_TEX_BULLSEYE || T.type == DFR_TEX_POLKA_DOT ||
+                             T.type == DFR_TEX_BLEND;
+            if (two && (cyclic(T.a) || cyclic(T.b))) return true;
+            state[t] = 2;
+            return false;
+        };
+        for (int t = 0; t < nt; t++)
+            if (cyclic(t)) { err = "texture " + std::to_string(t) + ": its sub-textures form a cycle"; return DFR_E_ARGUMENT; }
+    }
+    for (size_t m = 0; m < h.mats.size(); m++) {
+        const dfr_material& M = h.mats[m];
+        for (int k = 0; k < 4; k++) {
+            if (!M.textures[k]) continue;
+            if (M.textures[k] < 0 || M.textures[k] > nt) { err = "material " + std::to_string(m) + ": texture out of range"; return DFR_E_ARGUMENT; }
+            const bool ok = (M.type == DFR_MAT_MATTE && k == DFR_SLOT_KD) ||
+                            (M.type == DFR_MAT_PLASTIC && (k == DFR_SLOT_KD || k == DFR_SLOT_KS)) ||
+                            (M.type == DFR_MAT_MIRROR && k == DFR_SLOT_KR) ||
+                            (M.type == DFR_MAT_GLASS && (k == DFR_SLOT_KR || k == DFR_SLOT_KT));
+            if (!ok) { err = "material " + std::to_string(m) + ": this slot takes no texture"; return DFR_E_UNSUPPORTED; }
+        }
     }
     return DFR_OK;
 }

@@ -52,6 +52,8 @@ struct SceneHost {
     float camera[DFR_CAMERA_LENGTH];
     struct Image { int32_t width = 0, height = 0; std::vector<float> rgb; };
     std::vector<Image> images;
+    std::vector<float> uvs;            // 2 per vertex, or empty
+    std::vector<dfr_texture> textures;
 };
 
 // A validated object-to-world map of one primitive: the linear part `a`,
@@ -67,7 +69,8 @@ struct Xform {
 
 // A scene compiled into SoA device buffers, resident in HBM.
 struct CompiledScene {
-    DeviceBuffer nodes, tris, tshade, prims, prim_class, analytic, mats, lobes, lights, perez, xforms, images, texels;
+    DeviceBuffer nodes, tris, tshade, prims, prim_class, analytic, mats, lobes, lights, perez, xforms, images, texels,
+        textures;
     DevScene dev{};
     int max_depth = 1;
     bool area_lights = false;  // any DiffuseAreaLight (the only lights whose NEE needs closest hits)

@@ -118,7 +118,8 @@ struct DevScene {
     // world space (read for primitives with `xformed` set)
     const float4* xforms;
     const DevImageLight* images;  // image lights (DevLight::perez indexes them)
-    const float* texels;          // their images' RGB texels
+    const float* texels;          // RGB texels of image lights and image textures
+    const DevTexture* textures;
     int32_t n_analytic;
     int32_t n_lights;
     int32_t has_bvh;
@@ -628,6 +629,9 @@ struct Surf {
     ONB g, s;
     int prim;
     int tri;
+    // triangles: the hit's barycentric u, v (weights of b and c); spheres:
+    // the texture coordinates (Sphere3F.java:340-372)
+    float bu, bv;
 };
 
 // A row-vector product: (dot(X[0], v), dot(X[1], v), dot(X[2], v)).
@@ -650,6 +654,7 @@ DEV ONB frame_xformed(const float4* X, V w) {
 DEV Surf make_surface(const DevScene& sc, V o, V d, const HitRec& h) {
     Surf s;
     s.prim = h.prim; s.tri = h.tri;
+    s.bu = 0.0f; s.bv = 0.0f;
     s.p = o + d * h.t;
     const DevPrim& p = sc.prims[h.prim];
     const float4* X = p.xformed ? sc.xforms + 6 * (size_t)h.prim : nullptr;
@@ -663,6 +668,7 @@ DEV Surf make_surface(const DevScene& sc, V o, V d, const HitRec& h) {
         float rdet = __builtin_amdgcn_rcpf(dot(d, d1));
         V d3 = cross(a - o, d);
         float u = dot(d3, eca) * rdet, v = dot(d3, eab) * rdet, w = 1.0f - u - v;
+        s.bu = u; s.bv = v;
         V ns = nrm(mk(r0.w, r1.w, r2.w) * w + f4v(r3) * u + f4v(r4) * v);
         if (X) {
             s.g = frame_xformed(X, mk(r3.w, r4.w, r5.x));
@@ -686,6 +692,10 @@ DEV Surf make_surface(const DevScene& sc, V o, V d, const HitRec& h) {
         V dpdv = mk(ph.z * cphi, ph.z * sphi, -radius * sinf(theta)) * (0.0f - PI_F);
         V n = nrm(cross(dpdu, dpdv));
         s.g.w = nrm(n); s.g.v = nrm(dpdv); s.g.u = nrm(dpdu);
+        float phi = atan2f(ph.y, ph.x);
+        if (phi < 0.0f) phi += 2.0f * PI_F;
+        s.bu = phi * (1.0f / (2.0f * PI_F));
+        s.bv = (theta - PI_F) / (0.0f - PI_F);
         if (X) {
             s.g.u = nrm(mul_rows(X + 3, s.g.u)); s.g.v = nrm(mul_rows(X + 3, s.g.v)); s.g.w = nrm(mul_rows(X + 3, s.g.w));
             s.p = pw + c;
@@ -1159,12 +1169,21 @@ DEV bool lobe_sample(const DevLobe& l, V wo, float u, float v, LSample& out) {
 }
 
 // BSDF over a compiled material (M/scene/BSDF.java:272-540)
+// Per-hit texture colours of a textured material: c[0], c[1] replace the
+// R of lobes 0 and 1, c[2], c[3] their T (bits 0-3 of `set`); lobe i is
+// absent at this hit when bit i of `off` is set.
+struct TexHit {
+    uint32_t set, off;
+    V c[4];
+};
+
 struct Bsdf {
     const DevLobe* lobes;
     int n;
     float eta;
     ONB s;
     V ng, wo_w, wo;
+    TexHit th;
 };
 // ClearCoatMaterial.java:292-333: the material picks one layer per hit by
 // Russian roulette on the dielectric (1 -> 1.5) reflectance of the shading
@@ -1192,10 +1211,11 @@ DEV float clear_coat_layer(Bsdf& b, const Surf& si, V d, float u) {
 // `beta` is the path throughput at the hit (after its emission is added).
 template <bool EXT>
 DEV Bsdf make_bsdf(const DevScene& sc, const DevMaterial& m, const Surf& si, V ray_d, const Rng& rng, uint32_t dim,
-                   V& beta) {
+                   V& beta, const TexHit& th) {
     Bsdf b;
     b.lobes = sc.lobes + m.first_lobe;
     b.n = m.n_lobes;
+    if constexpr (EXT) b.th = th; else { b.th.set = 0u; b.th.off = 0u; }
     b.eta = m.eta;
     b.s = si.s; b.ng = si.g.w;
     b.wo_w = -ray_d;
@@ -1208,9 +1228,35 @@ DEV Bsdf make_bsdf(const DevScene& sc, const DevMaterial& m, const Surf& si, V r
 DEV bool matches(uint32_t lt, uint32_t t) { return (lt & t) == lt; }
 DEV int count_nonspecular(const Bsdf& b) {
     int c = 0;
-    for (int i = 0; i < b.n; i++) if (!(b.lobes[i].type & BX_SPEC)) c++;
+    for (int i = 0; i < b.n; i++) if (!((b.th.off >> i) & 1u) && !(b.lobes[i].type & BX_SPEC)) c++;
     return c;
 }
+// lobe i of the BSDF with the hit's texture colours (EXT kernels only)
+template <bool EXT>
+DEV V lobe_f_at(const Bsdf& b, int i, V wo, V wi) {
+    if constexpr (EXT) {
+        if (b.th.set & (5u << i)) {
+            DevLobe c = b.lobes[i];
+            if (b.th.set & (1u << i)) { const V r = b.th.c[i]; c.R[0] = r.x; c.R[1] = r.y; c.R[2] = r.z; }
+            if (b.th.set & (4u << i)) { const V r = b.th.c[2 + i]; c.T[0] = r.x; c.T[1] = r.y; c.T[2] = r.z; }
+            return lobe_f<EXT>(c, wo, wi);
+        }
+    }
+    return lobe_f<EXT>(b.lobes[i], wo, wi);
+}
+template <bool EXT>
+DEV bool lobe_sample_at(const Bsdf& b, int i, V wo, float u, float v, LSample& out) {
+    if constexpr (EXT) {
+        if (b.th.set & (5u << i)) {
+            DevLobe c = b.lobes[i];
+            if (b.th.set & (1u << i)) { const V r = b.th.c[i]; c.R[0] = r.x; c.R[1] = r.y; c.R[2] = r.z; }
+            if (b.th.set & (4u << i)) { const V r = b.th.c[2 + i]; c.T[0] = r.x; c.T[1] = r.y; c.T[2] = r.z; }
+            return lobe_sample<EXT>(c, wo, u, v, out);
+        }
+    }
+    return lobe_sample<EXT>(b.lobes[i], wo, u, v, out);
+}
+DEV bool lobe_on(const Bsdf& b, int i) { return !((b.th.off >> i) & 1u); }
 template <bool EXT>
 DEV V bsdf_f(const Bsdf& b, uint32_t t, V wi_w) {
     V wi = to_local(wi_w, b.s);
@@ -1218,8 +1264,8 @@ DEV V bsdf_f(const Bsdf& b, uint32_t t, V wi_w) {
     V r = mk(0, 0, 0);
     for (int i = 0; i < b.n; i++) {
         const DevLobe& l = b.lobes[i];
-        if (matches(l.type, t) && ((refl && (l.type & BX_REFL)) || (!refl && (l.type & BX_TRANS))))
-            r = r + lobe_f<EXT>(l, b.wo, wi);
+        if (lobe_on(b, i) && matches(l.type, t) && ((refl && (l.type & BX_REFL)) || (!refl && (l.type & BX_TRANS))))
+            r = r + lobe_f_at<EXT>(b, i, b.wo, wi);
     }
     return r;
 }
@@ -1231,7 +1277,7 @@ DEV float bsdf_pdf(const Bsdf& b, uint32_t t, V wi_w) {
     float p = 0.0f; int m = 0;
     for (int i = 0; i < b.n; i++) {
         const DevLobe& l = b.lobes[i];
-        if (matches(l.type, t)) { m++; p += lobe_pdf<EXT>(l, b.wo, wi); }
+        if (lobe_on(b, i) && matches(l.type, t)) { m++; p += lobe_pdf<EXT>(l, b.wo, wi); }
     }
     if (m > 1) p /= m;
     return p;
@@ -1240,16 +1286,16 @@ template <bool EXT>
 DEV bool bsdf_sample(const Bsdf& b, uint32_t t, float u, float v, LSample& out) {
     if (b.wo.z == 0.0f) return false;
     int m = 0;
-    for (int i = 0; i < b.n; i++) if (matches(b.lobes[i].type, t)) m++;
+    for (int i = 0; i < b.n; i++) if (lobe_on(b, i) && matches(b.lobes[i].type, t)) m++;
     if (m == 0) return false;
     int k = min((int)floorf(u * m), m - 1);
     int sel = -1, c = 0;
     for (int i = 0; i < b.n; i++)
-        if (matches(b.lobes[i].type, t)) { if (c == k) sel = i; c++; }
+        if (lobe_on(b, i) && matches(b.lobes[i].type, t)) { if (c == k) sel = i; c++; }
     const DevLobe& L = b.lobes[sel];
     float ur = fminf(u * m - k, 0.99999994f);
     LSample ls;
-    if (!lobe_sample<EXT>(L, b.wo, ur, v, ls)) return false;
+    if (!lobe_sample_at<EXT>(b, sel, b.wo, ur, v, ls)) return false;
     V wi_w = to_world(ls.wi, b.s);
     V f = ls.f;
     float pdf = ls.pdf;
@@ -1258,9 +1304,9 @@ DEV bool bsdf_sample(const Bsdf& b, uint32_t t, float u, float v, LSample& out) 
         f = mk(0, 0, 0);
         for (int i = 0; i < b.n; i++) {
             const DevLobe& B = b.lobes[i];
-            if (!matches(B.type, t)) continue;
+            if (!lobe_on(b, i) || !matches(B.type, t)) continue;
             if (i != sel) pdf += lobe_pdf<EXT>(B, b.wo, ls.wi);
-            if ((refl && (B.type & BX_REFL)) || (!refl && (B.type & BX_TRANS))) f = f + lobe_f<EXT>(B, b.wo, ls.wi);
+            if ((refl && (B.type & BX_REFL)) || (!refl && (B.type & BX_TRANS))) f = f + lobe_f_at<EXT>(b, i, b.wo, ls.wi);
         }
     }
     if (m > 1) pdf /= m;
@@ -1274,6 +1320,164 @@ DEV bool bsdf_sample(const Bsdf& b, uint32_t t, float u, float v, LSample& out) 
 struct LightSmp { V L; V p; V wi; float pdf; };
 
 #include "perez_device.inc"
+
+// ---------------------------------------------------------------------------
+// Textures (M/scene/texture/*Texture.java getColor) and the texture
+// coordinates of a hit (Triangle3F: interpolated vertex coordinates;
+// Sphere3F: (phi / 2 pi, (theta - pi) / -pi); Rectangle3F.java:448-475;
+// Plane3F.java:280-290).  Only EXT kernel instantiations reach them.
+// ---------------------------------------------------------------------------
+DEV void hit_uv(const DevScene& sc, const Surf& si, float& u, float& v) {
+    const DevPrim& p = sc.prims[si.prim];
+    if (si.tri >= 0) {
+        const size_t rb = 8 * (size_t)si.tri;
+        const float4 r5 = ld4g(sc.tshade, rb + 5), r6 = ld4g(sc.tshade, rb + 6);
+        const float w = 1.0f - si.bu - si.bv;
+        u = r5.y * w + r5.w * si.bu + r6.y * si.bv;
+        v = r5.z * w + r6.x * si.bu + r6.z * si.bv;
+    } else if (p.shape == DFR_SHAPE_SPHERE) {
+        u = si.bu; v = si.bv;
+    } else if (p.shape == DFR_SHAPE_RECTANGLE) {
+        // projection on the plane of the normal's largest component, from
+        // corners a, d = a + c - b and b
+        const V n = mk(fabsf(p.normal[0]), fabsf(p.normal[1]), fabsf(p.normal[2]));
+        const bool ix = n.x > n.y && n.x > n.z, iy = n.y > n.z;
+        auto pr = [&](float x, float y, float z, float& a, float& b) {
+            if (ix) { a = y; b = z; } else if (iy) { a = z; b = x; } else { a = x; b = y; }
+        };
+        const float* d = p.data;
+        const V A = mk(d[0], d[1], d[2]), B = mk(d[3], d[4], d[5]), C = mk(d[6], d[7], d[8]);
+        const V D = A + C - B;
+        float ax, ay, dx, dy, bx, by, hu, hv;
+        pr(A.x, A.y, A.z, ax, ay); pr(D.x, D.y, D.z, dx, dy); pr(B.x, B.y, B.z, bx, by);
+        pr(si.p.x, si.p.y, si.p.z, hu, hv);
+        const float abx = dx - ax, aby = dy - ay, acx = bx - ax, acy = by - ay;
+        const float det = abx * acy - aby * acx, r = 1.0f / det;
+        u = hu * (-aby * r) + hv * (abx * r) + (ax * aby - ay * abx) * r;
+        v = hu * (acy * r) + hv * (-acx * r) + (acx * ay - acy * ax) * r;
+    } else {
+        const V q = si.p - ld3(p.data);
+        u = dot(nrm(ld3(p.data + 3)), q);
+        v = dot(nrm(ld3(p.data + 6)), q);
+    }
+}
+
+DEV float frac_part(float x) { return x - floorf(x); }
+
+DEV V texture_image(const DevScene& sc, const DevTexture& T, float u, float v) {
+    const float ru = (u * T.cos_angle - v * T.sin_angle) * T.scale_u;
+    const float rv = (v * T.cos_angle + u * T.sin_angle) * T.scale_v;
+    const float x = pmodf_dev(ru * (float)T.width - 0.5f, (float)T.width);
+    const float y = pmodf_dev(rv * (float)T.height - 0.5f, (float)T.height);
+    const int x0 = (int)floorf(x), x1 = (int)ceilf(x), y0 = (int)floorf(y), y1 = (int)ceilf(y);
+    auto tx = [&](int a, int b) {
+        const float* t = sc.texels + T.texel + 3 * ((int64_t)pmodi_dev(b, T.height) * T.width + pmodi_dev(a, T.width));
+        return mk(t[0], t[1], t[2]);
+    };
+    if (x0 == x1 && y0 == y1) return tx(x0, y0);
+    const float fx = x - (float)x0, fy = y - (float)y0;
+    const V a = tx(x0, y0) * (1.0f - fx) + tx(x1, y0) * fx, b = tx(x0, y1) * (1.0f - fx) + tx(x1, y1) * fx;
+    return a * (1.0f - fy) + b * fy;
+}
+
+// A texture that is not a blend, following its chain of sub-textures.
+DEV V texture_chain(const DevScene& sc, int t, const Surf& si, V d, float u, float v) {
+    for (int guard = 0; guard < 64; guard++) {
+        const DevTexture& T = sc.textures[t];
+        switch (T.type) {
+            case DFR_TEX_CONSTANT: return ld3(T.color);
+            case DFR_TEX_CHECKERBOARD: {
+                const bool iu = frac_part((u * T.cos_angle - v * T.sin_angle) * T.scale_u) > 0.5f;
+                const bool iv = frac_part((v * T.cos_angle + u * T.sin_angle) * T.scale_v) > 0.5f;
+                t = (iu ^ iv) ? T.a : T.b;
+                break;
+            }
+            case DFR_TEX_UV: return mk(u, v, 0.0f);
+            case DFR_TEX_SURFACE_NORMAL: {
+                const V n = dot(si.s.w, d) > 0.0f ? -si.s.w : si.s.w;
+                return mk((n.x + 1.0f) / 2.0f, (n.y + 1.0f) / 2.0f, (n.z + 1.0f) / 2.0f);
+            }
+            case DFR_TEX_DOT_PRODUCT: {
+                const float k = fabsf(dot(si.s.w, d));
+                return mk(k, k, k);
+            }
+            case DFR_TEX_BULLSEYE: {
+                const float r = fmodf(len(si.p - ld3(T.origin)) * T.scale_u, 1.0f);
+                t = r > 0.5f ? T.a : T.b;
+                break;
+            }
+            case DFR_TEX_POLKA_DOT: {
+                const float x = frac_part((u * T.cos_angle - v * T.sin_angle) * T.cell) - 0.5f;
+                const float y = frac_part((v * T.cos_angle + u * T.sin_angle) * T.cell) - 0.5f;
+                t = x * x + y * y < T.radius * T.radius ? T.a : T.b;
+                break;
+            }
+            case DFR_TEX_IMAGE: return texture_image(sc, T, u, v);
+            default: return mk(0.0f, 0.0f, 0.0f);
+        }
+    }
+    return mk(0.0f, 0.0f, 0.0f);
+}
+
+DEV V texture_color(const DevScene& sc, int t, const Surf& si, V d) {
+    float u, v;
+    hit_uv(sc, si, u, v);
+    const DevTexture& T = sc.textures[t];
+    if (T.type == DFR_TEX_BLEND) {
+        const V a = texture_chain(sc, T.a, si, d, u, v), b = texture_chain(sc, T.b, si, d, u, v);
+        return mk((1.0f - T.t[0]) * a.x + T.t[0] * b.x, (1.0f - T.t[1]) * a.y + T.t[1] * b.y,
+                  (1.0f - T.t[2]) * a.z + T.t[2] * b.z);
+    }
+    return texture_chain(sc, t, si, d, u, v);
+}
+
+// A material's colours at a hit: the textured slots evaluated and clamped at
+// zero, the lobes whose colour is black dropped as the material's
+// computeBSDF leaves them out (MatteMaterial, PlasticMaterial,
+// MirrorMaterial, GlassMaterial); returns false when the material has no
+// BSDF here (the ray passes through).
+DEV V slot_color(const DevScene& sc, const DevMaterial& m, int slot, V fallback, const Surf& si, V d) {
+    if (!m.tex[slot]) return fallback;
+    const V c = texture_color(sc, m.tex[slot] - 1, si, d);
+    return mk(fmaxf(c.x, 0.0f), fmaxf(c.y, 0.0f), fmaxf(c.z, 0.0f));
+}
+template <bool EXT>
+DEV bool shade_material(const DevScene& sc, const DevMaterial& m, const Surf& si, V d, TexHit& th) {
+    th.set = 0u; th.off = 0u;
+    if constexpr (EXT) {
+        if (m.tex[0] | m.tex[1] | m.tex[2] | m.tex[3]) {
+            const DevLobe* L = sc.lobes + m.first_lobe;
+            switch (m.type) {
+                case DFR_MAT_MATTE:
+                    th.c[0] = slot_color(sc, m, DFR_SLOT_KD, ld3(L[0].R), si, d); th.set = 1u;
+                    return true;
+                case DFR_MAT_PLASTIC:
+                    th.c[0] = slot_color(sc, m, DFR_SLOT_KD, ld3(L[0].R), si, d);
+                    th.c[1] = slot_color(sc, m, DFR_SLOT_KS, ld3(L[1].R), si, d);
+                    th.set = 3u;
+                    th.off = (black(th.c[0]) ? 1u : 0u) | (black(th.c[1]) ? 2u : 0u);
+                    return th.off != 3u;
+                case DFR_MAT_MIRROR:
+                    th.c[0] = slot_color(sc, m, DFR_SLOT_KR, ld3(L[0].R), si, d); th.set = 1u;
+                    return !black(th.c[0]);
+                case DFR_MAT_GLASS:
+                    if (m.n_lobes == 1) {  // smooth: one Fresnel-specular lobe
+                        th.c[0] = slot_color(sc, m, DFR_SLOT_KR, ld3(L[0].R), si, d);
+                        th.c[2] = slot_color(sc, m, DFR_SLOT_KT, ld3(L[0].T), si, d);
+                        th.set = 1u | 4u;
+                        return !(black(th.c[0]) && black(th.c[2]));
+                    }
+                    th.c[0] = slot_color(sc, m, DFR_SLOT_KR, ld3(L[0].R), si, d);
+                    th.c[3] = slot_color(sc, m, DFR_SLOT_KT, ld3(L[1].T), si, d);
+                    th.set = 1u | 8u;
+                    th.off = (black(th.c[0]) ? 1u : 0u) | (black(th.c[3]) ? 2u : 0u);
+                    return th.off != 3u;
+                default: break;
+            }
+        }
+    }
+    return m.has_bsdf != 0;
+}
 
 // SpotLight and DirectionalLight (delta distributions); reached only from
 // kernels instantiated with EXT = true.

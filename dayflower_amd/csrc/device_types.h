@@ -49,13 +49,31 @@ struct DevLobe {           // 128 bytes
 };
 static_assert(sizeof(DevLobe) == 128, "lobe record size");
 
-struct DevMaterial {       // 32 bytes
+struct DevMaterial {       // 48 bytes
     int32_t first_lobe;
     int32_t n_lobes;       // 0 with has_bsdf = 0: pass-through surface
     int32_t has_bsdf;
     int32_t type;          // DFR_MAT_* (material-sort key)
     float eta;
     float emission[3];
+    // per colour slot (DFR_SLOT_*): texture index + 1, or 0.  A textured
+    // material keeps its full lobe list; which lobes exist at a hit, and
+    // their colours, follow from the textures there.
+    int32_t tex[4];
+};
+
+// A texture (M/scene/texture/*Texture.java); angles as cos/sin.
+struct DevTexture {        // 80 bytes
+    int32_t type, a, b;
+    int32_t width, height; // image textures
+    int32_t pad;
+    int64_t texel;         // image textures: first texel in DevScene::texels
+    float color[3];
+    float cos_angle, sin_angle;
+    float scale_u, scale_v;
+    float origin[3];
+    float cell, radius;
+    float t[3];
 };
 
 struct DevPrim {           // 80 bytes

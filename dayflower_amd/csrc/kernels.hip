@@ -229,14 +229,15 @@ DEV V radiance_pt(const DevScene& sc, V o, V d, const Rng& rng, int max_bounce, 
             if (bounce == 0 || spec) L = L + beta * emitted(sc, si, wo);
             if (bounce >= max_bounce) break;
             const DevMaterial& m = sc.mats[sc.prims[h.prim].material];
-            if (!m.has_bsdf) {
+            TexHit th;
+            if (!shade_material<EXT>(sc, m, si, d, th)) {
                 o = si.p;
                 bounce--;
                 if (++pass > 256) break;
                 continue;
             }
             const uint32_t dim = DIM_BOUNCE + DIMS_PER_BOUNCE * (uint32_t)bounce;
-            Bsdf b = make_bsdf<EXT>(sc, m, si, d, rng, dim, beta);
+            Bsdf b = make_bsdf<EXT>(sc, m, si, d, rng, dim, beta, th);
             if (sc.n_lights > 0 && count_nonspecular(b) > 0) {
                 int n = sc.n_lights;
                 int li = min((int)(rng.get(dim) * n), n - 1);
@@ -1114,7 +1115,8 @@ __global__ __launch_bounds__(BLOCK, 4) void k_shade(const DevScene* __restrict__
                 if (bounce == 0 || spec) L = L + beta * emitted(sc, si, wo);
                 if (bounce < job.max_bounce) {
                     const DevMaterial& m = sc.mats[sc.prims[prim].material];
-                    if (!m.has_bsdf) {
+                    TexHit th;
+                    if (!shade_material<EXT>(sc, m, si, d, th)) {
                         if (pass + 1 <= 256) {
                             cont = true; no = si.p; nd = d; nbeta = beta; neta = eta_scale;
                             ninfo = make_uint4(info.x, info.y, info.z, (uint32_t)bounce | ((uint32_t)spec << 8) | ((uint32_t)(pass + 1) << 16) |
@@ -1122,7 +1124,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_shade(const DevScene* __restrict__
                         }
                     } else {
                         const uint32_t dim = DIM_BOUNCE + DIMS_PER_BOUNCE * (uint32_t)bounce;
-                        Bsdf b = make_bsdf<EXT>(sc, m, si, d, rng, dim, beta);
+                        Bsdf b = make_bsdf<EXT>(sc, m, si, d, rng, dim, beta, th);
                         if (sc.n_lights > 0 && count_nonspecular(b) > 0) {
                             int nl = sc.n_lights;
                             int li = min((int)(rng.get(dim) * nl), nl - 1);

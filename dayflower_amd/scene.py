@@ -152,11 +152,14 @@ class Scene:
         self.positions = np.zeros((0, 3), np.float32)
         self.normals = np.zeros((0, 3), np.float32)
         self.has_normals = False
+        self.uvs = np.zeros((0, 2), np.float32)
+        self.has_uvs = False
         self.indices = np.zeros((0, 3), np.int32)
         self.primitives = []
         self.materials = []
         self.lights = []
         self.images = []
+        self.textures = []
         self.camera = camera_record((0, 0, -3), (0, 0, 0), 800, 800)
         self._keep = None
 
@@ -165,7 +168,7 @@ class Scene:
         self.materials.append(m)
         return len(self.materials) - 1
 
-    def add_mesh(self, positions, indices, material, normals=None, area_light=-1):
+    def add_mesh(self, positions, indices, material, normals=None, area_light=-1, uvs=None):
         positions = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
         indices = np.ascontiguousarray(indices, np.int32).reshape(-1, 3)
         base_v = self.positions.shape[0]
@@ -174,6 +177,14 @@ class Scene:
             normals = _flat_vertex_normals(positions, indices)
         else:
             self.has_normals = True
+        if uvs is None:
+            uvs = np.zeros((positions.shape[0], 2), np.float32)
+        else:
+            uvs = np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)
+            if uvs.shape[0] != positions.shape[0]:
+                raise ValueError("one (u, v) per vertex")
+            self.has_uvs = True
+        self.uvs = np.concatenate([self.uvs, uvs])
         self.positions = np.concatenate([self.positions, positions])
         self.normals = np.concatenate([self.normals, np.ascontiguousarray(normals, np.float32).reshape(-1, 3)])
         self.indices = np.concatenate([self.indices, indices + base_v])
@@ -282,6 +293,59 @@ class Scene:
         self.lights.append(L)
         return len(self.lights) - 1
 
+    # -- textures (M/scene/texture/*Texture.java) ---------------------------
+    def _add_texture(self, type_, **kw):
+        t = abi.Texture()
+        t.type = type_
+        t.a = t.b = t.image = -1
+        t.scale[0] = t.scale[1] = 1.0
+        for k, v in kw.items():
+            f = getattr(t, k)
+            if isinstance(f, (int, float)):
+                setattr(t, k, v)
+            else:
+                for i, x in enumerate(v):
+                    f[i] = float(x)
+        self.textures.append(t)
+        return len(self.textures) - 1
+
+    def constant_texture(self, color):
+        return self._add_texture(abi.TEX_CONSTANT, color=color)
+
+    def checkerboard_texture(self, a, b, angle_deg=0.0, scale=(1.0, 1.0)):
+        return self._add_texture(abi.TEX_CHECKERBOARD, a=a, b=b, angle=float(angle_deg), scale=scale)
+
+    def uv_texture(self):
+        return self._add_texture(abi.TEX_UV)
+
+    def surface_normal_texture(self):
+        return self._add_texture(abi.TEX_SURFACE_NORMAL)
+
+    def dot_product_texture(self):
+        return self._add_texture(abi.TEX_DOT_PRODUCT)
+
+    def bullseye_texture(self, a, b, origin=(0.0, 0.0, 0.0), scale=1.0):
+        return self._add_texture(abi.TEX_BULLSEYE, a=a, b=b, origin=origin, scale=(scale, scale))
+
+    def polka_dot_texture(self, a, b, angle_deg=0.0, cell=10.0, radius=0.25):
+        return self._add_texture(abi.TEX_POLKA_DOT, a=a, b=b, angle=float(angle_deg), cell=float(cell),
+                                 radius=float(radius))
+
+    def blend_texture(self, a, b, t=(0.5, 0.5, 0.5)):
+        return self._add_texture(abi.TEX_BLEND, a=a, b=b, t=t)
+
+    def image_texture(self, image, angle_deg=0.0, scale=(1.0, 1.0)):
+        """LDRImageTexture: `image` (H, W, 3) linear RGB in [0, 1], row 0 first."""
+        im = np.ascontiguousarray(image, np.float32)
+        if im.ndim != 3 or im.shape[2] != 3:
+            raise ValueError("image must be (H, W, 3)")
+        self.images.append(im)
+        return self._add_texture(abi.TEX_IMAGE, image=len(self.images) - 1, angle=float(angle_deg), scale=scale)
+
+    def set_texture(self, material, slot, texture):
+        """Replaces a colour slot (abi.SLOT_KD, KS, KR, KT) of a material by a texture."""
+        self.materials[material].textures[slot] = texture + 1
+
     def add_image_light(self, image, angle_deg=0.0, scale=(1.0, 1.0), radius=10.0):
         """ImageLight(image, angle, scale) (M/scene/light/ImageLight.java:134):
         `image` is an (H, W, 3) array of linear RGB, row 0 first."""
@@ -333,7 +397,8 @@ class Scene:
         fp = ctypes.POINTER(ctypes.c_float)
         d.positions = pos.ctypes.data_as(fp)
         d.normals = nrm.ctypes.data_as(fp) if self.has_normals else None
-        d.uvs = None
+        uv = np.ascontiguousarray(self.uvs, np.float32)
+        d.uvs = uv.ctypes.data_as(fp) if self.has_uvs else None
         d.n_triangles = idx.shape[0]
         d.indices = idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
         d.n_primitives = len(self.primitives)
@@ -351,7 +416,10 @@ class Scene:
             recs[k].rgb = im.ctypes.data_as(fp)
         d.n_images = len(imgs)
         d.images = recs if imgs else None
-        self._keep = (pos, nrm, idx, prims, mats, lights, imgs, recs)
+        texs = (abi.Texture * max(1, len(self.textures)))(*self.textures)
+        d.n_textures = len(self.textures)
+        d.textures = texs if self.textures else None
+        self._keep = (pos, nrm, idx, prims, mats, lights, imgs, recs, uv, texs)
         return d
 
 

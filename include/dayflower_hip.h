@@ -124,7 +124,14 @@ extern "C" {
  *   CLEAR_COAT  kd, ks, emission
  *   UBER        kd, ks, kr, kt, k (opacity), eta[0], roughness_u/v,
  *               remap_roughness, emission
- * Record size is fixed at 128 bytes. */
+ * textures[k] (k = DFR_SLOT_KD, KS, KR, KT) replaces that colour by a texture
+ * of the scene (index + 1; 0 keeps the constant): MATTE kd; PLASTIC kd, ks;
+ * MIRROR kr; GLASS kr, kt.  A texture on another material or slot is
+ * refused with DFR_E_UNSUPPORTED. */
+#define DFR_SLOT_KD 0
+#define DFR_SLOT_KS 1
+#define DFR_SLOT_KR 2
+#define DFR_SLOT_KT 3
 typedef struct dfr_material {
     int32_t type;
     int32_t remap_roughness;
@@ -151,7 +158,42 @@ typedef struct dfr_material {
     float disney_spec_trans;
     float disney_diff_trans;
     float disney_flatness;
+    int32_t textures[4];
 } dfr_material;
+
+/* Textures: M/scene/texture/{Constant,Checkerboard,UV,SurfaceNormal,
+ * DotProduct,Bullseye,PolkaDot,Blend,LDRImage}Texture.java.
+ *   CONSTANT        color
+ *   CHECKERBOARD    a, b (texture indices), angle (degrees), scale (u, v)
+ *   UV              (u, v, 0)
+ *   SURFACE_NORMAL  (n + 1) / 2 of the shading normal facing the ray
+ *   DOT_PRODUCT     |dot(shading normal, ray direction)|
+ *   BULLSEYE        a, b, origin, scale[0]
+ *   POLKA_DOT       a, b, angle (degrees), cell (resolution), radius
+ *   BLEND           a, b (not blends themselves), t (per component)
+ *   IMAGE           image (index into the scene's images), angle (degrees),
+ *                   scale (u, v); bilinear with wrap-around */
+#define DFR_TEX_CONSTANT       0
+#define DFR_TEX_CHECKERBOARD   1
+#define DFR_TEX_UV             2
+#define DFR_TEX_SURFACE_NORMAL 3
+#define DFR_TEX_DOT_PRODUCT    4
+#define DFR_TEX_BULLSEYE       5
+#define DFR_TEX_POLKA_DOT      6
+#define DFR_TEX_BLEND          7
+#define DFR_TEX_IMAGE          8
+typedef struct dfr_texture {
+    int32_t type;
+    int32_t a, b;
+    int32_t image;
+    float color[3];
+    float angle;
+    float scale[2];
+    float origin[3];
+    float cell;
+    float radius;
+    float t[3];
+} dfr_texture;
 
 /* One primitive (M/scene/Primitive.java), given in its object space; its
  * object-to-world matrix is the identity unless dfr_update_transforms sets
@@ -207,7 +249,8 @@ typedef struct dfr_light {
 } dfr_light;
 
 /* One image: width * height linear RGB texels (3 floats each), row 0 first
- * (an ImageF's colours; an 8-bit image's bytes divided by 255). */
+ * (an ImageF's colours; an 8-bit image's bytes divided by 255, as
+ * Color3F.unpack gives an LDRImageTexture's texels). */
 typedef struct dfr_image {
     int32_t width, height;
     const float* rgb;
@@ -228,7 +271,9 @@ typedef struct dfr_scene_desc {
     const dfr_light* lights;
     float camera[DFR_CAMERA_LENGTH];
     int32_t n_images;
-    const struct dfr_image* images;  /* images of IMAGE lights, or NULL     */
+    const struct dfr_image* images;  /* images of IMAGE lights and textures */
+    int32_t n_textures;
+    const dfr_texture* textures;
 } dfr_scene_desc;
 
 /* ------------------------------------------------------------------------ */

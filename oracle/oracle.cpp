@@ -133,6 +133,9 @@ struct Scene {
     std::vector<dfr_material> mats;
     std::vector<Light> lights;
     std::vector<BvhNode> nodes;
+    std::vector<dfr_texture> textures;
+    struct Img { int w = 0, h = 0; std::vector<float> rgb; };
+    std::vector<Img> images;
     float cam[DFR_CAMERA_LENGTH];
     ~Scene();
 };
@@ -504,11 +507,29 @@ Surface makeSurfaceIn(const std::vector<Tri>& tris, const std::vector<Prim>& pri
     } else if (p.shape == DFR_SHAPE_RECTANGLE) {
         s.g = onbFromW(p.rectNormal);
         s.s = s.g;
+        // Rectangle3F.doCreateTextureCoordinates (Rectangle3F.java:448-475)
+        // with corners a, b, c and d = a + c - b
+        const V3 A(p.data[0], p.data[1], p.data[2]), B(p.data[3], p.data[4], p.data[5]), Cc(p.data[6], p.data[7], p.data[8]);
+        const V3 D = A + Cc - B;
+        const float nx = std::fabs(p.rectNormal.x), ny = std::fabs(p.rectNormal.y), nz = std::fabs(p.rectNormal.z);
+        const bool isX = nx > ny && nx > nz, isY = ny > nz;
+        auto pr = [&](V3 q, float* a, float* b) {
+            if (isX) { *a = q.y; *b = q.z; } else if (isY) { *a = q.z; *b = q.x; } else { *a = q.x; *b = q.y; }
+        };
+        float ax, ay, bx, by, cx, cy, hU, hV;
+        pr(A, &ax, &ay); pr(D, &bx, &by); pr(B, &cx, &cy); pr(s.p, &hU, &hV);
+        const float abx = bx - ax, aby = by - ay, acx = cx - ax, acy = cy - ay;
+        const float det = abx * acy - aby * acx, dr = 1.0f / det;
+        s.tu = hU * (-aby * dr) + hV * (+abx * dr) + (ax * aby - ay * abx) * dr;
+        s.tv = hU * (+acy * dr) + hV * (-acx * dr) + (acx * ay - acy * ax) * dr;
     } else {  // plane: G = (w, v, u) of the world-space frame
         s.g.u = normalize(V3(p.data[3], p.data[4], p.data[5]));
         s.g.v = normalize(V3(p.data[6], p.data[7], p.data[8]));
         s.g.w = normalize(V3(p.data[9], p.data[10], p.data[11]));
         s.s = s.g;
+        // Plane3F: (u . P, v . P) of the point relative to the plane's origin
+        const V3 q = s.p - V3(p.data[0], p.data[1], p.data[2]);
+        s.tu = dot(s.g.u, q); s.tv = dot(s.g.v, q);
     }
     if (flip && dot(s.s.w, s.g.w) < 0.0f) s.s.w = -s.s.w;
     return s;
@@ -1547,6 +1568,89 @@ bool primaryRay(const float* cam, float imageX, float imageY, float px, float py
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Textures (M/scene/texture/*Texture.java getColor) and a material whose
+// textured colours take the values at a hit (Material.computeBSDF reads
+// every colour through its texture).
+// ---------------------------------------------------------------------------
+C3 imgLookup(const Scene::Img& im, float x, float y) {  // LDRImageTexture.doGetColorRGB
+    auto tx = [&](int a, int b) {
+        const float* t = &im.rgb[3 * ((size_t)positiveModuloI(b, im.h) * im.w + positiveModuloI(a, im.w))];
+        return C3(t[0], t[1], t[2]);
+    };
+    const int x0 = (int)std::floor(x), x1 = (int)std::ceil(x), y0 = (int)std::floor(y), y1 = (int)std::ceil(y);
+    if (x0 == x1 && y0 == y1) return tx(x0, y0);
+    const float fx = x - x0, fy = y - y0;
+    return blend(blend(tx(x0, y0), tx(x1, y0), fx), blend(tx(x0, y1), tx(x1, y1), fx), fy);
+}
+
+float texRadians(float deg) { return (float)(wrapDegrees(deg) * (3.14159265358979323846 / 180.0)); }
+float fractionalPart(float x) { return x - (float)std::floor((double)x); }
+
+C3 texChain(const Scene& sc, int t, const Surface& si) {
+    const float u = si.tu, v = si.tv;
+    for (int guard = 0; guard < 64; guard++) {
+        const dfr_texture& T = sc.textures[t];
+        switch (T.type) {
+            case DFR_TEX_CONSTANT: return C3(T.color[0], T.color[1], T.color[2]);
+            case DFR_TEX_CHECKERBOARD: {  // CheckerboardTexture.java:217-228
+                const float c = fcos(texRadians(T.angle)), sn = fsin(texRadians(T.angle));
+                const bool iu = fractionalPart((u * c - v * sn) * T.scale[0]) > 0.5f;
+                const bool iv = fractionalPart((v * c + u * sn) * T.scale[1]) > 0.5f;
+                t = (iu ^ iv) ? T.a : T.b;
+                break;
+            }
+            case DFR_TEX_UV: return C3(u, v, 0.0f);  // UVTexture.java:64-72
+            case DFR_TEX_SURFACE_NORMAL: {  // SurfaceNormalTexture.java:64-72
+                const V3 n = dot(si.s.w, si.rayDir) > 0.0f ? -si.s.w : si.s.w;
+                return C3((n.x + 1.0f) / 2.0f, (n.y + 1.0f) / 2.0f, (n.z + 1.0f) / 2.0f);
+            }
+            case DFR_TEX_DOT_PRODUCT: return C3(std::fabs(dot(si.s.w, si.rayDir)));  // DotProductTexture.java:64-66
+            case DFR_TEX_BULLSEYE: {  // BullseyeTexture.java:206-214
+                const float r = std::fmod(length(si.p - V3(T.origin[0], T.origin[1], T.origin[2])) * T.scale[0], 1.0f);
+                t = r > 0.5f ? T.a : T.b;
+                break;
+            }
+            case DFR_TEX_POLKA_DOT: {  // PolkaDotTexture.java:265-282
+                const float c = fcos(texRadians(T.angle)), sn = fsin(texRadians(T.angle));
+                const float x = fractionalPart((u * c - v * sn) * T.cell) - 0.5f;
+                const float y = fractionalPart((v * c + u * sn) * T.cell) - 0.5f;
+                t = x * x + y * y < T.radius * T.radius ? T.a : T.b;
+                break;
+            }
+            case DFR_TEX_IMAGE: {  // LDRImageTexture.java:234-241
+                const Scene::Img& im = sc.images[T.image];
+                const float c = fcos(texRadians(T.angle)), sn = fsin(texRadians(T.angle));
+                const float ru = (u * c - v * sn) * T.scale[0], rv = (v * c + u * sn) * T.scale[1];
+                return imgLookup(im, positiveModuloF(ru * im.w - 0.5f, (float)im.w), positiveModuloF(rv * im.h - 0.5f, (float)im.h));
+            }
+            default: return C3(0.0f);
+        }
+    }
+    return C3(0.0f);
+}
+
+C3 texColor(const Scene& sc, int t, const Surface& si) {
+    const dfr_texture& T = sc.textures[t];
+    if (T.type == DFR_TEX_BLEND) {  // BlendTexture.java:130-138
+        const C3 a = texChain(sc, T.a, si), b = texChain(sc, T.b, si);
+        return C3(lerpf(a.r, b.r, T.t[0]), lerpf(a.g, b.g, T.t[1]), lerpf(a.b, b.b, T.t[2]));
+    }
+    return texChain(sc, t, si);
+}
+
+dfr_material materialAt(const Scene& sc, const dfr_material& m, const Surface& si) {
+    if (!(m.textures[0] || m.textures[1] || m.textures[2] || m.textures[3])) return m;
+    dfr_material r = m;
+    float* slot[4] = {r.kd, r.ks, r.kr, r.kt};
+    for (int k = 0; k < 4; k++) {
+        if (!m.textures[k]) continue;
+        const C3 c = texColor(sc, m.textures[k] - 1, si);
+        slot[k][0] = c.r; slot[k][1] = c.g; slot[k][2] = c.b;
+    }
+    return r;
+}
+
 // Scene.radiancePathTracer (M/scene/Scene.java:451-636)
 C3 radiancePT(const Scene& sc, Ray ray, const Rng& rng, int maxBounce, int minRR, int pdfMode, bool preview) {
     C3 L, beta(1.0f);
@@ -1562,7 +1666,7 @@ C3 radiancePT(const Scene& sc, Ray ray, const Rng& rng, int maxBounce, int minRR
             if (bounce >= maxBounce) break;
             BSDF bsdf;
             const uint32_t dim = DIM_BOUNCE + DIMS_PER_BOUNCE * (uint32_t)bounce;
-            if (!buildBSDF(sc.mats[sc.prims[h.prim].material], si, rng.get(dim + 9), &bsdf)) {
+            if (!buildBSDF(materialAt(sc, sc.mats[sc.prims[h.prim].material], si), si, rng.get(dim + 9), &bsdf)) {
                 ray = Ray{si.p, ray.d};
                 bounce--;
                 if (++passThrough > 256) break;  // guard against an endless chain of null surfaces
@@ -1850,6 +1954,13 @@ orc_scene* orc_scene_create(const dfr_scene_desc* d) {
         sc.lights.push_back(L);
     }
     std::memcpy(sc.cam, d->camera, sizeof(sc.cam));
+    if (d->n_textures > 0 && d->textures) sc.textures.assign(d->textures, d->textures + d->n_textures);
+    for (int32_t i = 0; i < d->n_images && d->images; i++) {
+        Scene::Img im;
+        im.w = d->images[i].width; im.h = d->images[i].height;
+        im.rgb.assign(d->images[i].rgb, d->images[i].rgb + 3 * (size_t)im.w * im.h);
+        sc.images.push_back(std::move(im));
+    }
     bvhBuild(sc);
     return h.release();
 }

@@ -1,0 +1,299 @@
+"""Textures on material colours (M/scene/texture/*Texture.java getColor; the
+materials' computeBSDF read every colour through its texture) and the
+texture coordinates of hits (Triangle3F interpolation, Sphere3F, Rectangle3F
+.java:448-475, Plane3F.java:280-290).
+
+Oracle pins (CPU): a Matte quad lit by a directional light along the view
+renders kd / pi x E x cos per pixel, deterministically, so the texture's value
+at every hit is read back and compared with the closed form of the texture at
+the hit's coordinates: the interpolated vertex coordinates of a triangle
+pair, a sphere's (phi / 2 pi, (theta - pi) / -pi), an axis-aligned
+rectangle's linear map, a checkerboard's cells, an image's bilinear texels,
+a blend.  A texture that is a constant renders bit-identically to the
+constant itself.
+
+GPU (through the C-ABI): textured Matte, Plastic, Mirror and Glass over
+triangles, spheres, rectangles and planes on the same random streams as the
+oracle, both kernels; the textures the kernels cannot follow are refused.
+"""
+import numpy as np
+import pytest
+
+from dayflower_amd import abi
+from dayflower_amd import scene as S
+
+E = 2.0   # directional light radiance
+
+
+def _lit_scene(res, add_geometry):
+    sc = S.Scene()
+    add_geometry(sc)
+    sc.add_directional_light((0.0, 0.0, -1.0), (E, E, E), radius=10.0)
+    sc.set_camera(S.camera_record((0.0, 0.0, -4.0), (0.0, 0.0, 0.0), res, res, fov_x_deg=40.0))
+    return sc
+
+
+def _direct(sc, res, seed=7):
+    """Per-pixel radiance with the direct light only (max bounce 1: the
+    continuation ray leaves the scene and sees nothing), the primary rays and
+    their hits."""
+    from tests.oracle_lib import OracleScene
+    o = OracleScene(sc)
+    s, _, c = o.render(res, res, 1, seed=seed, max_bounce=1, min_rr=50)
+    rays = o.primary_rays(res, res, seed, 0)
+    prim, _, t = o.trace_f32(rays)
+    o.close()
+    img = (s / np.maximum(c, 1)[..., None]).reshape(-1, 3)
+    p = rays[:, 0:3] + rays[:, 4:7] * t[:, None]
+    return img, rays, prim, p
+
+
+def _kd_from(img, rays, normal):
+    """kd = L pi / (E cos), cos between the surface normal and the light
+    direction (the light shines along +z onto a surface facing -z)."""
+    cos = np.abs(normal[..., 2]) if normal.ndim > 1 else abs(normal[2])
+    return img * np.pi / (E * cos if np.ndim(cos) == 0 else (E * cos)[:, None])
+
+
+def _quad(sc, material, uv=True):
+    p = np.array([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], np.float32)
+    t = np.array([[0, 2, 1], [0, 3, 2]], np.int32)   # facing -z, toward the camera
+    uvs = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32) if uv else None
+    sc.add_mesh(p, t, material, normals=np.tile([0, 0, -1], (4, 1)).astype(np.float32), uvs=uvs)
+
+
+def test_uv_texture_on_triangles_reads_the_interpolated_coordinates():
+    res = 32
+
+    def geo(sc):
+        m = sc.add_material(S.matte((1, 1, 1)))
+        sc.set_texture(m, abi.SLOT_KD, sc.uv_texture())
+        _quad(sc, m)
+    img, rays, prim, p = _direct(_lit_scene(res, geo), res)
+    hit = prim >= 0
+    kd = _kd_from(img[hit], rays[hit], np.array([0, 0, -1.0]))
+    expect = np.stack([(p[hit, 0] + 1) / 2, (p[hit, 1] + 1) / 2, np.zeros(hit.sum())], -1)
+    print(f"hits {hit.sum()}; max |kd - (u, v, 0)| {np.abs(kd - expect).max():.2e}")
+    assert hit.sum() > res * res // 4
+    np.testing.assert_allclose(kd, expect, atol=2e-5)
+
+
+def test_checkerboard_cells():
+    res = 48
+
+    def geo(sc):
+        m = sc.add_material(S.matte((1, 1, 1)))
+        a = sc.constant_texture((0.9, 0.2, 0.1))
+        b = sc.constant_texture((0.1, 0.3, 0.8))
+        sc.set_texture(m, abi.SLOT_KD, sc.checkerboard_texture(a, b, 0.0, (4.0, 4.0)))
+        _quad(sc, m)
+    img, rays, prim, p = _direct(_lit_scene(res, geo), res)
+    hit = prim >= 0
+    kd = _kd_from(img[hit], rays[hit], np.array([0, 0, -1.0]))
+    u, v = (p[hit, 0] + 1) / 2, (p[hit, 1] + 1) / 2
+    fu, fv = np.mod(u * 4, 1.0), np.mod(v * 4, 1.0)
+    away = (np.abs(fu - 0.5) > 1e-3) & (np.abs(fv - 0.5) > 1e-3) & (fu > 1e-3) & (fv > 1e-3)
+    is_a = (fu > 0.5) ^ (fv > 0.5)
+    expect = np.where(is_a[:, None], [0.9, 0.2, 0.1], [0.1, 0.3, 0.8])
+    print(f"cells away from the edges: {away.sum()} of {hit.sum()}; cells of a: {is_a[away].mean():.3f}")
+    np.testing.assert_allclose(kd[away], expect[away], rtol=1e-5)
+    assert 0.3 < is_a[away].mean() < 0.7
+
+
+def test_image_texture_bilinear_and_blend():
+    """A 2 x 2 image over the quad: u * 2 - 0.5 and v * 2 - 0.5 are the image
+    coordinates, wrapped; a blend of it with a constant mixes per component."""
+    res = 32
+    img2 = np.array([[[0.2, 0.4, 0.6], [0.8, 0.1, 0.3]], [[0.5, 0.9, 0.0], [0.1, 0.6, 0.7]]], np.float32)
+    t3 = (0.25, 0.5, 0.75)
+
+    def geo(sc):
+        m = sc.add_material(S.matte((1, 1, 1)))
+        im = sc.image_texture(img2)
+        k = sc.constant_texture((1.0, 1.0, 1.0))
+        sc.set_texture(m, abi.SLOT_KD, sc.blend_texture(im, k, t3))
+        _quad(sc, m)
+    img, rays, prim, p = _direct(_lit_scene(res, geo), res)
+    hit = prim >= 0
+    kd = _kd_from(img[hit], rays[hit], np.array([0, 0, -1.0]))
+    u, v = (p[hit, 0] + 1) / 2, (p[hit, 1] + 1) / 2
+    x, y = np.mod(u * 2 - 0.5, 2), np.mod(v * 2 - 0.5, 2)
+    x0, y0 = np.floor(x).astype(int), np.floor(y).astype(int)
+    fx, fy = x - x0, y - y0
+    T = lambda a, b: img2[b % 2, a % 2]
+    c = ((1 - fy)[:, None] * ((1 - fx)[:, None] * T(x0, y0) + fx[:, None] * T(x0 + 1, y0)) +
+         fy[:, None] * ((1 - fx)[:, None] * T(x0, y0 + 1) + fx[:, None] * T(x0 + 1, y0 + 1)))
+    expect = (1 - np.array(t3)) * c + np.array(t3) * 1.0
+    print(f"max |kd - blend(bilinear, 1)| {np.abs(kd - expect).max():.2e}")
+    np.testing.assert_allclose(kd, expect, atol=5e-5)
+
+
+def test_sphere_and_rectangle_coordinates():
+    """UV texture on a sphere: (phi / 2 pi, (theta - pi) / -pi) of the hit
+    relative to the centre; on the rectangle a(-1,-1,0), b(-1,1,0),
+    c(1,1,0) (d = a + c - b): the projection formula gives u = (y + 1) / 2,
+    v = (x + 1) / 2."""
+    res = 32
+    # sphere
+    def geo_s(sc):
+        m = sc.add_material(S.matte((1, 1, 1)))
+        sc.set_texture(m, abi.SLOT_KD, sc.uv_texture())
+        sc.add_sphere((0.0, 0.0, 0.0), 1.0, m)
+    img, rays, prim, p = _direct(_lit_scene(res, geo_s), res)
+    hit = prim >= 0
+    n = p[hit] / np.linalg.norm(p[hit], axis=1, keepdims=True)
+    kd = img[hit] * np.pi / (E * np.abs(n[:, 2]))[:, None]
+    phi = np.mod(np.arctan2(p[hit, 1], p[hit, 0]), 2 * np.pi)
+    th = np.arccos(np.clip(p[hit, 2] / np.linalg.norm(p[hit], axis=1), -1, 1))
+    expect = np.stack([phi / (2 * np.pi), (th - np.pi) / -np.pi, np.zeros(hit.sum())], -1)
+    ok = np.abs(n[:, 2]) > 0.2
+    print(f"sphere: max |kd - uv| {np.abs(kd - expect)[ok].max():.2e} over {ok.sum()} hits")
+    np.testing.assert_allclose(kd[ok], expect[ok], atol=5e-4)
+    # rectangle
+    def geo_r(sc):
+        m = sc.add_material(S.matte((1, 1, 1)))
+        sc.set_texture(m, abi.SLOT_KD, sc.uv_texture())
+        sc.add_rectangle((-1, -1, 0), (-1, 1, 0), (1, 1, 0), m)
+    img, rays, prim, p = _direct(_lit_scene(res, geo_r), res)
+    hit = prim >= 0
+    kd = _kd_from(img[hit], rays[hit], np.array([0, 0, -1.0]))
+    expect = np.stack([(p[hit, 1] + 1) / 2, (p[hit, 0] + 1) / 2, np.zeros(hit.sum())], -1)
+    print(f"rectangle: max |kd - uv| {np.abs(kd - expect).max():.2e} over {hit.sum()} hits")
+    np.testing.assert_allclose(kd, expect, atol=5e-5)
+
+
+def test_constant_texture_equals_the_constant():
+    from tests.oracle_lib import OracleScene
+    res = 24
+    a = S.cornell_box(res, res)
+    b = S.cornell_box(res, res)
+    for m in range(len(b.materials)):
+        mat = b.materials[m]
+        slots = {abi.MAT_MATTE: [(abi.SLOT_KD, mat.kd)], abi.MAT_MIRROR: [(abi.SLOT_KR, mat.kr)],
+                 abi.MAT_GLASS: [(abi.SLOT_KR, mat.kr), (abi.SLOT_KT, mat.kt)]}.get(mat.type, [])
+        for slot, col in slots:
+            b.set_texture(m, slot, b.constant_texture(tuple(col)))
+    out = []
+    for sc in (a, b):
+        o = OracleScene(sc)
+        s, _, c = o.render(res, res, 4, seed=5, max_bounce=6)
+        o.close()
+        out.append((s, c))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
+# ---------------------------------------------------------------------------
+# GPU
+# ---------------------------------------------------------------------------
+def _textured_scene(res):
+    sc = S.Scene()
+    white = sc.constant_texture((0.8, 0.8, 0.75))
+    dark = sc.constant_texture((0.1, 0.12, 0.15))
+    floor = sc.add_material(S.matte((0.8, 0.8, 0.8)))
+    sc.set_texture(floor, abi.SLOT_KD, sc.checkerboard_texture(white, dark, 15.0, (6.0, 6.0)))
+    sc.add_rectangle((-3, 0, -3), (-3, 0, 3), (3, 0, 3), floor)
+    # an icosphere with spherical texture coordinates and an image texture on Plastic kd
+    p, t, n = S.displaced_icosphere(2000, 0x5EED, radius=0.5, center=(-0.6, 0.55, 0.2))
+    q = p - np.array([-0.6, 0.55, 0.2], np.float32)
+    uv = np.stack([np.mod(np.arctan2(q[:, 2], q[:, 0]) / (2 * np.pi), 1.0),
+                   np.arccos(np.clip(q[:, 1] / np.linalg.norm(q, axis=1), -1, 1)) / np.pi], -1)
+    rng = np.random.default_rng(9)
+    image = rng.random((16, 32, 3)).astype(np.float32)
+    image[:, ::4] = 0.0          # black columns: the Lambertian lobe drops out there
+    plast = sc.add_material(S.plastic((0.8, 0.8, 0.8), (0.25, 0.25, 0.25), 0.3))
+    sc.set_texture(plast, abi.SLOT_KD, sc.image_texture(image, 10.0, (2.0, 1.0)))
+    sc.add_mesh(p, t, plast, normals=n, uvs=uv)
+    # a sphere with polka dots on Plastic kd and ks
+    pd = sc.add_material(S.plastic((0.8, 0.8, 0.8), (0.3, 0.3, 0.3), 0.2))
+    red, blue = sc.constant_texture((0.8, 0.1, 0.1)), sc.constant_texture((0.1, 0.2, 0.7))
+    sc.set_texture(pd, abi.SLOT_KD, sc.polka_dot_texture(red, blue, 30.0, 8.0, 0.3))
+    sc.set_texture(pd, abi.SLOT_KS, sc.dot_product_texture())
+    sc.add_sphere((0.6, 0.4, -0.2), 0.4, pd)
+    # glass with a blended transmittance, a mirror with a bullseye and a UV-coloured plane behind
+    gl = sc.add_material(S.glass(1.5))
+    sc.set_texture(gl, abi.SLOT_KT, sc.blend_texture(sc.surface_normal_texture(), white, (0.3, 0.6, 0.9)))
+    sc.add_sphere((0.1, 0.25, -0.8), 0.25, gl)
+    mi = sc.add_material(S.mirror((1, 1, 1)))
+    sc.set_texture(mi, abi.SLOT_KR, sc.bullseye_texture(white, dark, (0.0, 1.0, 1.5), 3.0))
+    sc.add_rectangle((-1.5, 0.1, 1.4), (-1.5, 1.6, 1.6), (0.0, 1.6, 1.6), mi)
+    back = sc.add_material(S.matte((0.8, 0.8, 0.8)))
+    sc.set_texture(back, abi.SLOT_KD, sc.uv_texture())
+    sc.add_plane((0.0, 0.0, 3.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, -1.0), back)
+    sc.add_point_light((1.5, 3.0, -1.5), (18.0, 18.0, 18.0))
+    sc.add_perez_light(2.5, (0.5, 1.0, -0.7))
+    sc.set_camera(S.camera_record((0.0, 1.5, -3.2), (0.0, 0.4, 0.0), res, res, fov_x_deg=50.0))
+    return sc
+
+
+def _same_streams(sc, res, seed, kernel, spp_list=(1, 16)):
+    from dayflower_amd.renderer import GPURenderer
+    from tests.oracle_lib import OracleScene
+    o = OracleScene(sc)
+    refs = []
+    for spp in spp_list:
+        s, _, c = o.render(res, res, spp, seed=seed, max_bounce=8)
+        refs.append((s / np.maximum(c, 1)[..., None], c))
+    o.close()
+    out = []
+    for spp, (ref, c) in zip(spp_list, refs):
+        r = GPURenderer()
+        r.setScene(sc)
+        r.setSeed(seed)
+        r.setMaximumBounce(8)
+        r.setKernel(kernel)
+        r.setup()
+        r.render(spp)
+        g, k = r.film(res, res)
+        r.dispose()
+        np.testing.assert_array_equal(k, c)
+        out.append((np.isclose(g, ref, rtol=1e-3, atol=1e-4).all(-1).mean(), abs(g.mean() - ref.mean()) / ref.mean()))
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_textured_scene_same_streams(kernel):
+    """The glass and mirror send rounding-level differences into other paths
+    (the same scene with every texture removed measures 95.3 % of 1-spp
+    paths equal), so the textured scene is held to that baseline: its paths
+    agree as often (measured 95.4 %) and its 16-spp mean as closely
+    (measured 3.0e-3)."""
+    res, seed = 48, 51
+    sc = _textured_scene(res)
+    (p1, _), (_, rel) = _same_streams(sc, res, seed, kernel)
+    plain = _textured_scene(res)
+    for m in plain.materials:
+        for k in range(4):
+            m.textures[k] = 0
+    (q1, _), (_, qrel) = _same_streams(plain, res, seed, kernel)
+    print(f"kernel {kernel}: 1-spp paths equal {p1:.5f} textured, {q1:.5f} without textures "
+          f"(bar: no more than 0.005 below, and 0.94); 16-spp mean rel diff {rel:.2e} textured, {qrel:.2e} without "
+          f"(bar: 2e-3 above, and 6e-3)", flush=True)
+    assert p1 >= 0.94 and p1 >= q1 - 0.005
+    assert rel < 6e-3 and rel <= qrel + 2e-3
+
+
+@pytest.mark.gpu
+def test_textures_the_kernels_cannot_follow_are_refused():
+    from dayflower_amd.renderer import GPURenderer, IllegalArgumentException, UnsupportedOperationException
+    cases = []
+    sc = S.cornell_box(16, 16)                     # a texture on a metal
+    m = sc.add_material(S.metal())
+    sc.set_texture(m, abi.SLOT_KD, sc.uv_texture())
+    cases.append((sc, UnsupportedOperationException))
+    sc = S.cornell_box(16, 16)                     # a blend of blends
+    k = sc.constant_texture((1, 1, 1))
+    b1 = sc.blend_texture(k, k)
+    sc.set_texture(0, abi.SLOT_KD, sc.blend_texture(b1, k))
+    cases.append((sc, UnsupportedOperationException))
+    sc = S.cornell_box(16, 16)                     # a checkerboard of itself
+    c = sc.checkerboard_texture(0, 0)
+    sc.textures[c].a = c
+    sc.set_texture(0, abi.SLOT_KD, c)
+    cases.append((sc, IllegalArgumentException))
+    r = GPURenderer()
+    for s, exc in cases:
+        with pytest.raises(exc):
+            r.setScene(s)
+    r.dispose()
