@@ -8,8 +8,15 @@ reference's order and grouping:
   (:1344-1358);
 * indices are 1-based; a vertex is the triple (position, texture
   coordinates, normal) and equal triples share one index (:1219-1263);
-* a new mesh starts where the group name (`g`) or the material name
-  (`usemtl`) changes; an object name (`o`) alone does not split (:629-660);
+* each face vertex records the group, material and object names current
+  where it is read, and a vertex triple keeps the names of its FIRST
+  occurrence in the file (the indexed model's computeIfAbsent, :1250-1263);
+  a triangle takes the names of its vertex A (:651-655), so a face whose
+  vertex A appeared earlier under another `usemtl` or `g` belongs to that
+  earlier group and material;
+* a new mesh starts where a triangle's group or material name differs from
+  the previous triangle's; an object name alone does not split, and a mesh
+  carries the object name of its last triangle (:657-683, :705-709);
 * a file without any `vn` gets per-position normals: the normalised sum of
   the unit normals cross(B - A, C - A) of the faces around each position
   (:1567-1583), shared by every vertex with that position;
@@ -115,14 +122,18 @@ def parse_wavefront_object(text, flip_texture_y=False, scale=1.0):
     if any(has_n) and not all(has_n):
         raise ValueError("some face vertices have normals and others do not")
     pos_n = None if any(has_n) else _position_normals(pos, verts)
+    first = {}                                   # vertex triple -> names where it first occurs
+    for vtx, g, mt, o in zip(verts, groups, mats, objs):
+        first.setdefault(vtx, (g, mt, o))
     meshes = []
     cur, key = None, None
     for i in range(0, len(verts), 3):
-        k = (groups[i], mats[i])
-        if cur is None or k != key:
-            cur = {"names": (groups[i], mats[i], objs[i]), "map": {}, "p": [], "n": [], "t": [], "idx": []}
+        g, mt, o = first[verts[i]]
+        if cur is None or (g, mt) != key:
+            cur = {"names": [g, mt, o], "map": {}, "p": [], "n": [], "t": [], "idx": []}
             meshes.append(cur)
-            key = k
+            key = (g, mt)
+        cur["names"][2] = o
         for vtx in verts[i:i + 3]:
             j = cur["map"].get(vtx)
             if j is None:

@@ -76,13 +76,27 @@ def test_polygon_is_a_fan_from_its_first_vertex():
 
 
 def test_meshes_split_on_group_or_material_not_object():
+    """Names come from the first occurrence of each triangle's vertex A
+    triple: the face after `usemtl blue` starts at 1/1, first read under
+    (top, red), so it stays in the red mesh; the last face's vertex A 1/2 is
+    new and opens (bottom, blue).  The red mesh carries the object name of
+    its last triangle (vertex A 1/1, first read before `o other`)."""
     ms = obj.parse_wavefront_object(GROUPS, flip_texture_y=True)
-    assert [(m.group_name, m.material_name, len(m.indices)) for m in ms] == [("top", "red", 3), ("top", "blue", 1),
-                                                                            ("bottom", "blue", 1)]
+    assert [(m.group_name, m.material_name, m.object_name, len(m.indices)) for m in ms] == [
+        ("top", "red", "", 4), ("bottom", "blue", "other", 1)]
     # equal (position, texture) pairs share a vertex; a new pair does not
     assert len(ms[0].positions) == 4
-    assert ms[2].uvs[0].tolist() == [1.0, 1.0]          # vt 2 = (1, 0), flipped
-    assert ms[0].object_name == "" and ms[1].object_name == "other"
+    assert ms[1].uvs[0].tolist() == [1.0, 1.0]          # vt 2 = (1, 0), flipped
+
+
+def test_mesh_object_name_is_the_last_triangles():
+    src = "v 0 0 0\nv 1 0 0\nv 0 1 0\nv 1 1 0\no a\nf 1 2 3\no b\nf 4 3 2\n"
+    (m,) = obj.parse_wavefront_object(src)
+    assert (m.object_name, len(m.indices)) == ("b", 2)
+    # a vertex A read again under a later group keeps its first group
+    src = "v 0 0 0\nv 1 0 0\nv 0 1 0\nv 1 1 0\ng x\nf 1 2 3\ng y\nf 1 3 4\nf 4 3 2\n"
+    ms = obj.parse_wavefront_object(src)
+    assert [(m.group_name, len(m.indices)) for m in ms] == [("x", 2), ("y", 1)]
 
 
 def test_scale_errors_and_compilation():
@@ -99,6 +113,6 @@ def test_scale_errors_and_compilation():
     sc = S.Scene()
     mat = sc.add_material(S.matte((0.5, 0.5, 0.5)))
     prims = obj.add_to_scene(sc, obj.parse_wavefront_object(GROUPS), {"": mat})
-    assert len(prims) == 3 and sc.indices.shape == (5, 3)
+    assert len(prims) == 2 and sc.indices.shape == (5, 3)
     with pytest.raises(KeyError):
         obj.add_to_scene(S.Scene(), obj.parse_wavefront_object(GROUPS), {"red": 0})
