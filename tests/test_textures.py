@@ -247,8 +247,24 @@ def _same_streams(sc, res, seed, kernel, spp_list=(1, 16)):
         g, k = r.film(res, res)
         r.dispose()
         np.testing.assert_array_equal(k, c)
-        out.append((np.isclose(g, ref, rtol=1e-3, atol=1e-4).all(-1).mean(), abs(g.mean() - ref.mean()) / ref.mean()))
+        gc, rc = np.minimum(g, CLAMP), np.minimum(ref, CLAMP)
+        diff = (gc - rc).reshape(-1)
+        z = diff.mean() / max(diff.std() / np.sqrt(diff.size), 1e-30)
+        out.append((np.isclose(g, ref, rtol=1e-3, atol=1e-4).all(-1).mean(), abs(g.mean() - ref.mean()) / ref.mean(),
+                    abs(gc.mean() - rc.mean()) / rc.mean(), z))
     return out
+
+
+# Per-channel clamp of the mean comparison.  The untextured scene's pixels stay
+# below 2.0 at 16 spp, so the clamp leaves it alone.  In the textured scene the
+# UV texture on the infinite back plane gives it albedos above 1 (a plane's
+# texture coordinates are unbounded projections, Plane3F.java:288, and
+# UVTexture returns them as red and green), and the glass sphere focuses the
+# sun onto the floor: one pixel holds about half of the image's total (a
+# single sample of radiance near 1.6e5), so a rounding-level difference in one
+# diverged path moves the raw mean by 1e-3 while saying nothing about the
+# textures.
+CLAMP = 8.0
 
 
 @pytest.mark.gpu
@@ -257,21 +273,26 @@ def test_textured_scene_same_streams(kernel):
     """The glass and mirror send rounding-level differences into other paths
     (the same scene with every texture removed measures 95.3 % of 1-spp
     paths equal), so the textured scene is held to that baseline: its paths
-    agree as often (measured 95.4 %) and its 16-spp mean as closely
-    (measured 3.0e-3)."""
+    agree as often (measured 95.4 %; a wrong texture value moves every path
+    that sees it by more than the rtol 1e-3 of path equality).  What the
+    diverged paths add to the 16-spp image is noise, not bias: the paired
+    per-pixel difference of the clamped images has a mean within 4 standard
+    errors of zero (measured |z| 0.9 textured, 0.2 without) and within 1 %."""
     res, seed = 48, 51
     sc = _textured_scene(res)
-    (p1, _), (_, rel) = _same_streams(sc, res, seed, kernel)
+    (p1, _, _, _), (_, rel, crel, z) = _same_streams(sc, res, seed, kernel)
     plain = _textured_scene(res)
     for m in plain.materials:
         for k in range(4):
             m.textures[k] = 0
-    (q1, _), (_, qrel) = _same_streams(plain, res, seed, kernel)
+    (q1, _, _, _), (_, qrel, qcrel, qz) = _same_streams(plain, res, seed, kernel)
     print(f"kernel {kernel}: 1-spp paths equal {p1:.5f} textured, {q1:.5f} without textures "
-          f"(bar: no more than 0.005 below, and 0.94); 16-spp mean rel diff {rel:.2e} textured, {qrel:.2e} without "
-          f"(bar: 2e-3 above, and 6e-3)", flush=True)
+          f"(bar: no more than 0.005 below, and 0.94); 16-spp clamped mean rel diff {crel:.2e} textured, "
+          f"{qcrel:.2e} without (bar 1e-2), paired z {z:+.2f} textured, {qz:+.2f} without (bar |z| < 4); "
+          f"raw means {rel:.2e}, {qrel:.2e}", flush=True)
     assert p1 >= 0.94 and p1 >= q1 - 0.005
-    assert rel < 6e-3 and rel <= qrel + 2e-3
+    assert abs(z) < 4 and abs(qz) < 4
+    assert crel < 1e-2 and qcrel < 1e-3
 
 
 @pytest.mark.gpu
