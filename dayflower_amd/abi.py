@@ -106,6 +106,43 @@ class SceneDesc(ctypes.Structure):
     ]
 
 
+class FArray(ctypes.Structure):
+    _fields_ = [("data", ctypes.POINTER(c_float)), ("length", c_int64)]
+
+
+class IArray(ctypes.Structure):
+    _fields_ = [("data", ctypes.POINTER(c_int32)), ("length", c_int64)]
+
+
+# dfr_compiled_scene: (field, kind) in header order; kind "f" float[], "i" int[]
+COMPILED_SCENE_ARRAYS = [
+    ("axis_aligned_bounding_boxes", "f"), ("bounding_spheres", "f"),
+    ("cones", "f"), ("cylinders", "f"), ("disks", "f"), ("hyperboloids", "f"), ("paraboloids", "f"),
+    ("polygons", "f"), ("rectangles", "f"), ("rectangular_cuboids", "f"), ("tori", "f"), ("triangles", "f"),
+    ("triangle_meshes", "i"),
+    ("blend_textures", "f"), ("bullseye_textures", "f"), ("checkerboard_textures", "f"), ("constant_textures", "f"),
+    ("ldr_image_textures", "f"), ("ldr_image_texture_offsets", "i"),
+    ("marble_textures", "f"), ("polka_dot_textures", "f"), ("simplex_fbm_textures", "f"),
+    ("bullseye_materials", "f"), ("checkerboard_materials", "f"),
+    ("clear_coat_materials", "i"), ("disney_materials", "i"), ("glass_materials", "i"), ("glossy_materials", "i"),
+    ("matte_materials", "i"), ("metal_materials", "i"), ("mirror_materials", "i"), ("plastic_materials", "i"),
+    ("polka_dot_materials", "f"),
+    ("substrate_materials", "i"), ("translucent_materials", "i"), ("uber_materials", "i"),
+    ("normal_map_ldr_image_modifiers", "f"), ("normal_map_ldr_image_modifier_offsets", "i"),
+    ("simplex_noise_normal_map_modifiers", "f"),
+    ("light_ids_and_offsets", "i"),
+    ("diffuse_area_lights", "f"), ("directional_lights", "f"), ("image_lights", "f"), ("image_light_offsets", "i"),
+    ("perez_lights", "f"), ("perez_light_offsets", "i"), ("point_lights", "f"), ("spot_lights", "f"),
+    ("camera", "f"),
+    ("primitives", "i"), ("primitive_matrices", "f"),
+]
+
+
+class CompiledScene(ctypes.Structure):
+    _fields_ = [(n, FArray if k == "f" else IArray) for n, k in COMPILED_SCENE_ARRAYS] + [
+        ("primitive_count", c_int32), ("light_count", c_int32)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("seconds", c_double), ("path_samples", c_int64), ("rays_closest", c_int64),
                 ("rays_any", c_int64), ("bounces", c_int64), ("extend_seconds", c_double),
@@ -141,6 +178,12 @@ ENTRY_POINTS = [
     ("dfr_last_error", ctypes.c_char_p, [_c_void_p]),
     ("dfr_set_scene", c_int32, [_c_void_p, P(SceneDesc)]),
     ("dfr_update_transforms", c_int32, [_c_void_p, _c_void_p, c_int32, c_int32]),
+    ("dfr_update_matrix44fs", c_int32, [_c_void_p, _c_void_p, c_int64]),
+    ("dfr_set_compiled_scene", c_int32, [_c_void_p, P(CompiledScene)]),
+    ("dfr_transcode_compiled_scene", c_int32, [P(CompiledScene), P(_c_void_p), ctypes.c_char_p, c_int32]),
+    ("dfr_transcoded_desc", P(SceneDesc), [_c_void_p]),
+    ("dfr_transcoded_transforms", P(c_float), [_c_void_p]),
+    ("dfr_transcoded_free", None, [_c_void_p]),
     ("dfr_set_resolution", c_int32, [_c_void_p, c_int32, c_int32]),
     ("dfr_set_i32", c_int32, [_c_void_p, c_int32, c_int32]),
     ("dfr_get_i32", c_int32, [_c_void_p, c_int32, P(c_int32)]),

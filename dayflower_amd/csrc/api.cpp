@@ -1342,9 +1342,8 @@ const char* dfr_last_error(const dfr_ctx* h) {
     return c ? c->error.c_str() : "context == null";
 }
 
-int32_t dfr_set_scene(dfr_ctx* h, const dfr_scene_desc* scene) {
-    Ctx* c = reinterpret_cast<Ctx*>(h);
-    CTX_GUARD(c);
+// setScene with the context's lock held
+static int32_t set_scene_locked(Ctx* c, const dfr_scene_desc* scene) {
     if (!scene) return c->fail(DFR_E_NULL, "scene == null");
     int rc = copy_scene_desc(scene, c->scene_host, c->error);
     if (rc) return rc;
@@ -1355,9 +1354,14 @@ int32_t dfr_set_scene(dfr_ctx* h, const dfr_scene_desc* scene) {
     return DFR_OK;
 }
 
-int32_t dfr_update_transforms(dfr_ctx* h, const float* object_to_world, int32_t n_primitives, int32_t rebuild) {
+int32_t dfr_set_scene(dfr_ctx* h, const dfr_scene_desc* scene) {
     Ctx* c = reinterpret_cast<Ctx*>(h);
     CTX_GUARD(c);
+    return set_scene_locked(c, scene);
+}
+
+// updateTransforms with the context's lock held
+static int32_t update_transforms_locked(Ctx* c, const float* object_to_world, int32_t n_primitives, int32_t rebuild) {
     if (!c->has_scene) return c->fail(DFR_E_STATE, "setScene before updateTransforms");
     if (!object_to_world) return c->fail(DFR_E_NULL, "object_to_world == null");
     if (n_primitives != (int32_t)c->scene_host.prims.size())
@@ -1385,6 +1389,41 @@ int32_t dfr_update_transforms(dfr_ctx* h, const float* object_to_world, int32_t 
     }
     c->xform_host = std::move(m);
     return DFR_OK;
+}
+
+int32_t dfr_update_transforms(dfr_ctx* h, const float* object_to_world, int32_t n_primitives, int32_t rebuild) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    return update_transforms_locked(c, object_to_world, n_primitives, rebuild);
+}
+
+int32_t dfr_update_matrix44fs(dfr_ctx* h, const float* matrices, int64_t length) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (!c->has_scene) return c->fail(DFR_E_STATE, "setScene before updateMatrix44Fs");
+    if (!matrices) return c->fail(DFR_E_NULL, "matrices == null");
+    if (length != 32 * (int64_t)c->scene_host.prims.size())
+        return c->fail(DFR_E_ARGUMENT, "32 floats per primitive of the scene: " +
+                                           std::to_string(32 * c->scene_host.prims.size()));
+    std::vector<float> o2w(16 * (size_t)(length / 32));
+    for (int64_t p = 0; p < length / 32; p++) std::memcpy(&o2w[16 * p], matrices + 32 * p, 16 * sizeof(float));
+    return update_transforms_locked(c, o2w.data(), (int32_t)(length / 32), 0);
+}
+
+int32_t dfr_set_compiled_scene(dfr_ctx* h, const dfr_compiled_scene* scene) {
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_GUARD(c);
+    if (!scene) return c->fail(DFR_E_NULL, "scene == null");
+    const dfr_scene_desc* desc = nullptr;
+    const float* xf = nullptr;
+    void* holder = nullptr;
+    std::string err;
+    int rc = transcode_compiled_scene(scene, &desc, &xf, &holder, err);
+    if (rc) return c->fail(rc, err);
+    rc = set_scene_locked(c, desc);
+    if (!rc && desc->n_primitives > 0) rc = update_transforms_locked(c, xf, desc->n_primitives, 0);
+    free_transcoded(holder);
+    return rc;
 }
 
 int32_t dfr_set_resolution(dfr_ctx* h, int32_t w, int32_t hh) {

@@ -84,6 +84,12 @@ struct CompiledScene {
 };
 
 int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err);
+// The reference's CompiledScene arrays as a scene description and one
+// object-to-world matrix per primitive (compiled_scene.cpp); *holder owns
+// both until free_transcoded.
+int transcode_compiled_scene(const dfr_compiled_scene* s, const dfr_scene_desc** desc, const float** xforms,
+                             void** holder, std::string& err);
+void free_transcoded(void* holder);
 struct WavefrontBuffers;
 // Allocates (or grows) the 17 wavefront buffers of a path pool of `capacity`
 // paths into `mem` and points `wf` at them; false when an allocation fails.

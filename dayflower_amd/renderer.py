@@ -123,6 +123,27 @@ class GPURenderer:
         self._xforms = m
         self._c(self.lib.dfr_update_transforms(self.ctx, m.ctypes.data, m.shape[0], 1 if rebuild else 0))
 
+    def setCompiledScene(self, arrays):
+        """setScene for a caller holding SceneCompiler.compile's arrays
+        (AbstractSceneKernel.java:1777-1855): `arrays` maps the field names of
+        dfr_compiled_scene to float32 / int32 arrays, plus "primitive_count"
+        and "light_count".  The resolution comes from the compiled camera."""
+        if arrays is None:
+            raise NullPointerException("scene == null")
+        cs, keep = compiled_scene_struct(arrays)
+        self._compiled = (cs, keep)
+        self._scene = None
+        self._c(self.lib.dfr_set_compiled_scene(self.ctx, ctypes.byref(cs)))
+        cam = keep["camera"]
+        self._c(self.lib.dfr_set_resolution(self.ctx, int(cam[abi.CAMERA_LENGTH - 2]), int(cam[abi.CAMERA_LENGTH - 1])))
+
+    def updateMatrix44Fs(self, matrices):
+        """updateMatrix44Fs with CompiledPrimitiveCache.getMatrix44Fs()'s
+        array: 32 floats per primitive (object to world, world to object)."""
+        import numpy as np
+        m = np.ascontiguousarray(np.asarray(matrices, dtype=np.float32).reshape(-1))
+        self._c(self.lib.dfr_update_matrix44fs(self.ctx, m.ctypes.data, m.size))
+
     def setImage(self, width=None, height=None):
         """setImage(): resolution from the scene camera; setImage(w, h) explicit."""
         if width is None:
@@ -376,6 +397,68 @@ class GPURenderer:
         ptrs = (fp * n)(*[f.ctypes.data_as(fp) for f in films]) if readback else None
         self._c(self.lib.dfr_batch_render(self.ctx, arr, n, int(mode), ptrs))
         return films
+
+
+def compiled_scene_struct(arrays):
+    """A dfr_compiled_scene pointing into `arrays` (see setCompiledScene) and
+    the contiguous copies it points at, which the caller keeps alive."""
+    import numpy as np
+    cs = abi.CompiledScene()
+    keep = {}
+    for name, kind in abi.COMPILED_SCENE_ARRAYS:
+        a = arrays.get(name)
+        if a is None:
+            continue
+        dt, ct = (np.float32, ctypes.c_float) if kind == "f" else (np.int32, ctypes.c_int32)
+        a = np.ascontiguousarray(a, dt).reshape(-1)
+        keep[name] = a
+        f = getattr(cs, name)
+        f.data = a.ctypes.data_as(ctypes.POINTER(ct))
+        f.length = a.size
+    cs.primitive_count = int(arrays.get("primitive_count", 0))
+    cs.light_count = int(arrays.get("light_count", 0))
+    return cs, keep
+
+
+def transcode_compiled_scene(arrays):
+    """The scene the library reads from compiled arrays, without a device:
+    a dict of the dfr_scene_desc's arrays and records (numpy copies), with
+    "transforms" (n, 4, 4).  Raises like the renderer does."""
+    import numpy as np
+    lib = abi.load()
+    cs, keep = compiled_scene_struct(arrays)
+    h = ctypes.c_void_p()
+    err = ctypes.create_string_buffer(512)
+    rc = lib.dfr_transcode_compiled_scene(ctypes.byref(cs), ctypes.byref(h), err, 512)
+    if rc != abi.E_OK:
+        msg = err.value.decode(errors="replace")
+        if rc == abi.E_UNSUPPORTED:
+            raise UnsupportedOperationException(msg)
+        if rc == abi.E_NULL:
+            raise NullPointerException(msg)
+        raise IllegalArgumentException(msg)
+    try:
+        d = lib.dfr_transcoded_desc(h).contents
+        nv, nt = d.n_vertices, d.n_triangles
+        arr = lambda p, n, dt: np.ctypeslib.as_array(p, shape=(n,)).astype(dt).copy() if n else np.zeros(0, dt)
+        out = {
+            "positions": arr(d.positions, 3 * nv, np.float32).reshape(-1, 3),
+            "normals": arr(d.normals, 3 * nv, np.float32).reshape(-1, 3),
+            "uvs": arr(d.uvs, 2 * nv, np.float32).reshape(-1, 2),
+            "indices": arr(d.indices, 3 * nt, np.int32).reshape(-1, 3),
+            "primitives": [abi.Primitive.from_buffer_copy(d.primitives[i]) for i in range(d.n_primitives)],
+            "materials": [abi.Material.from_buffer_copy(d.materials[i]) for i in range(d.n_materials)],
+            "lights": [abi.Light.from_buffer_copy(d.lights[i]) for i in range(d.n_lights)],
+            "textures": [abi.Texture.from_buffer_copy(d.textures[i]) for i in range(d.n_textures)],
+            "images": [np.ctypeslib.as_array(d.images[i].rgb, shape=(d.images[i].height, d.images[i].width, 3)).copy()
+                       for i in range(d.n_images)],
+            "camera": np.array(d.camera[:], np.float32),
+        }
+        xf = lib.dfr_transcoded_transforms(h)
+        out["transforms"] = arr(xf, 16 * d.n_primitives, np.float32).reshape(-1, 4, 4)
+    finally:
+        lib.dfr_transcoded_free(h)
+    return out
 
 
 def make_job(scene_id, camera, width, height, spp, seed, maximum_bounce=8, minimum_bounce_rr=5):

@@ -343,6 +343,93 @@ int32_t dfr_set_scene(dfr_ctx* ctx, const dfr_scene_desc* scene);
  * the scene.  dfr_set_scene resets every primitive to the identity. */
 int32_t dfr_update_transforms(dfr_ctx* ctx, const float* object_to_world, int32_t n_primitives, int32_t rebuild);
 
+/* updateMatrix44Fs() with the array the reference uploads verbatim:
+ * CompiledPrimitiveCache.getMatrix44Fs() (M/scene/compiler/
+ * CompiledPrimitiveCache.java:323-350), 32 floats per primitive (object to
+ * world, then world to object, each row major).  The reference only takes an
+ * array of the length it already has (AbstractSceneKernel.java:292-300);
+ * here `length` must be 32 x the scene's primitive count (DFR_E_ARGUMENT
+ * otherwise).  The object-to-world halves go to dfr_update_transforms (BVH
+ * refit); the world-to-object halves are not read (the inverse is computed in
+ * double precision). */
+int32_t dfr_update_matrix44fs(dfr_ctx* ctx, const float* matrices, int64_t length);
+
+/* ------------------------------------------------------------------------ */
+/* The reference's compiled scene, verbatim                                 */
+/* ------------------------------------------------------------------------ */
+
+/* One array of a CompiledScene as the reference uploads it
+ * (doGetCompatibleArray: an empty array travels as one zero). */
+typedef struct dfr_farray { const float* data; int64_t length; } dfr_farray;
+typedef struct dfr_iarray { const int32_t* data; int64_t length; } dfr_iarray;
+
+/* Every array doSetupScene() puts on the device (M/renderer/gpu/
+ * AbstractSceneKernel.java:1777-1855), by the getter that produced it, plus
+ * the two counts it keeps (primitiveCount, lightCount).  A Java caller fills
+ * it from SceneCompiler.compile(scene) without touching the arrays. */
+typedef struct dfr_compiled_scene {
+    /* CompiledBoundingVolume3FCache (read for bounds only: the library
+     * builds its own BVH) */
+    dfr_farray axis_aligned_bounding_boxes, bounding_spheres;
+    /* CompiledShape3FCache */
+    dfr_farray cones, cylinders, disks, hyperboloids, paraboloids, polygons, rectangles, rectangular_cuboids, tori,
+        triangles;
+    dfr_iarray triangle_meshes;
+    /* CompiledTextureCache */
+    dfr_farray blend_textures, bullseye_textures, checkerboard_textures, constant_textures, ldr_image_textures;
+    dfr_iarray ldr_image_texture_offsets;
+    dfr_farray marble_textures, polka_dot_textures, simplex_fbm_textures;
+    /* CompiledMaterialCache */
+    dfr_farray bullseye_materials, checkerboard_materials;
+    dfr_iarray clear_coat_materials, disney_materials, glass_materials, glossy_materials, matte_materials,
+        metal_materials, mirror_materials, plastic_materials;
+    dfr_farray polka_dot_materials;
+    dfr_iarray substrate_materials, translucent_materials, uber_materials;
+    /* CompiledModifierCache */
+    dfr_farray normal_map_ldr_image_modifiers;
+    dfr_iarray normal_map_ldr_image_modifier_offsets;
+    dfr_farray simplex_noise_normal_map_modifiers;
+    /* CompiledLightCache */
+    dfr_iarray light_ids_and_offsets;
+    dfr_farray diffuse_area_lights, directional_lights, image_lights;
+    dfr_iarray image_light_offsets;
+    dfr_farray perez_lights;
+    dfr_iarray perez_light_offsets;
+    dfr_farray point_lights, spot_lights;
+    /* CompiledCameraCache (19 floats) */
+    dfr_farray camera;
+    /* CompiledPrimitiveCache: 8 ints and 32 floats per primitive */
+    dfr_iarray primitives;
+    dfr_farray primitive_matrices;
+    int32_t primitive_count, light_count;
+} dfr_compiled_scene;
+
+/* setScene(Scene) for a Java caller that hands over SceneCompiler.compile's
+ * arrays (the drop-in path of M/renderer/gpu/AbstractSceneKernel.java
+ * :1777-1855): the arrays are transcoded into a dfr_scene_desc plus one
+ * object-to-world matrix per primitive, then set as dfr_set_scene and
+ * dfr_update_transforms would.  What the kernels cannot follow is refused
+ * with DFR_E_UNSUPPORTED and a message naming it: Cone3F, Cylinder3F, Disk3F,
+ * Hyperboloid3F, Paraboloid3F, Polygon3F, RectangularCuboid3F and Torus3F
+ * primitives; the Bullseye, Checkerboard and PolkaDot materials; Marble and
+ * simplex fBm textures; any modifier other than NoOpModifier; a texture on a
+ * parameter other than the colour slots of dfr_material (a non-constant
+ * texture elsewhere); an area light on a shape other than a Triangle3F or a
+ * Sphere3F, or one no primitive carries.  Malformed arrays (an offset outside
+ * its array, an unknown type ID) give DFR_E_ARGUMENT. */
+int32_t dfr_set_compiled_scene(dfr_ctx* ctx, const dfr_compiled_scene* scene);
+
+/* The same transcoding without a context or a device: on success *out holds
+ * the scene (dfr_transcoded_desc, dfr_transcoded_transforms) until
+ * dfr_transcoded_free; on failure *out is NULL and `error` (capacity bytes,
+ * may be NULL) holds the message. */
+typedef struct dfr_transcoded dfr_transcoded;
+int32_t dfr_transcode_compiled_scene(const dfr_compiled_scene* scene, dfr_transcoded** out, char* error,
+                                     int32_t error_capacity);
+const dfr_scene_desc* dfr_transcoded_desc(const dfr_transcoded* t);
+const float* dfr_transcoded_transforms(const dfr_transcoded* t);  /* 16 per primitive, row major */
+void dfr_transcoded_free(dfr_transcoded* t);
+
 /* setImage(): M/renderer/gpu/AbstractGPURenderer.java:417-440.  Sets the
  * film resolution; takes effect at the next setup/render. */
 int32_t dfr_set_resolution(dfr_ctx* ctx, int32_t width, int32_t height);
