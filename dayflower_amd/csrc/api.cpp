@@ -432,6 +432,88 @@ static int world_geometry(const SceneHost& in, const std::vector<Xform>& xf, std
     return DFR_OK;
 }
 
+// Scenes with at least this many spheres and rectangles walk a BVH over them;
+// fewer are tested linearly with the infinite planes (a handful of shape
+// tests costs less than the tree's root node).
+constexpr size_t ITEM_TREE_MIN = 8;
+
+static bool items_in_tree(const SceneHost& in) {
+    size_t n = 0;
+    for (const dfr_primitive& pr : in.prims) n += pr.shape == DFR_SHAPE_SPHERE || pr.shape == DFR_SHAPE_RECTANGLE;
+    return n >= ITEM_TREE_MIN;
+}
+
+// The spheres and rectangles of a scene, as items of their own BVH: each
+// primitive's world-space box, padded by a few ulps of its coordinates (the
+// box test must not cull a hit the shape's own test reports).  `prim[k]` is
+// the primitive of item k, `box[6k..6k+5]` its lo.xyz, hi.xyz.
+static void bounded_items(const SceneHost& in, const std::vector<Xform>& xf, std::vector<int32_t>& prim,
+                          std::vector<float>& box) {
+    prim.clear();
+    box.clear();
+    for (size_t p = 0; p < in.prims.size(); p++) {
+        const dfr_primitive& pr = in.prims[p];
+        if (pr.shape != DFR_SHAPE_SPHERE && pr.shape != DFR_SHAPE_RECTANGLE) continue;
+        const Xform* x = xf.empty() || xf[p].identity ? nullptr : &xf[p];
+        double lo[3], hi[3];
+        if (pr.shape == DFR_SHAPE_SPHERE) {
+            const D3v c = xpoint(x, pr.data);
+            const double r = std::fabs((double)pr.data[3] * (x ? x->scale : 1.0));
+            const double cc[3] = {c.x, c.y, c.z};
+            for (int k = 0; k < 3; k++) { lo[k] = cc[k] - r; hi[k] = cc[k] + r; }
+        } else {
+            // corners A, B, C and D = A + C - B
+            D3v q[4];
+            for (int k = 0; k < 3; k++) q[k] = xpoint(x, pr.data + 3 * k);
+            q[3] = {q[0].x + q[2].x - q[1].x, q[0].y + q[2].y - q[1].y, q[0].z + q[2].z - q[1].z};
+            for (int k = 0; k < 3; k++) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+            for (const D3v& v : q) {
+                const double vv[3] = {v.x, v.y, v.z};
+                for (int k = 0; k < 3; k++) { lo[k] = std::min(lo[k], vv[k]); hi[k] = std::max(hi[k], vv[k]); }
+            }
+        }
+        prim.push_back((int32_t)p);
+        float b[6];
+        for (int k = 0; k < 3; k++) {
+            const double pad = 4.0 * 1.1920928955078125e-07 * std::max(std::fabs(lo[k]), std::fabs(hi[k])) + 1e-30;
+            b[k] = std::nextafter((float)(lo[k] - pad), -INFINITY);
+            b[3 + k] = std::nextafter((float)(hi[k] + pad), INFINITY);
+        }
+        box.insert(box.end(), b, b + 6);
+    }
+}
+
+// Refits a four-wide tree in place: each child box becomes the union of its
+// subtree, leaves bounded by `leaf(first, count, box6)`; children are
+// numbered after their parents, so one backward pass suffices.
+static int refit_nodes(std::vector<Node4>& nodes, const std::function<void(uint32_t, uint32_t, float*)>& leaf,
+                       std::string& err) {
+    const int64_t nn = (int64_t)nodes.size();
+    std::vector<std::array<float, 6>> bound(nn);
+    for (int64_t q = nn - 1; q >= 0; q--) {
+        Node4& w = nodes[q];
+        std::array<float, 6> nb = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (int k = 0; k < 4; k++) {
+            uint32_t ref;
+            std::memcpy(&ref, &w.f[24 + k], 4);
+            if (ref == EMPTY_REF) continue;
+            std::array<float, 6> cb = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+            if (ref & LEAF_FLAG) {
+                leaf((ref & 0x7fffffffu) >> 3, (ref & 7u) + 1u, cb.data());
+            } else {
+                if ((int64_t)ref <= q || (int64_t)ref >= nn) { err = "BVH child numbered before its parent"; return DFR_E_STATE; }
+                cb = bound[ref];
+            }
+            w.f[0 + k] = cb[0]; w.f[4 + k] = cb[3];
+            w.f[8 + k] = cb[1]; w.f[12 + k] = cb[4];
+            w.f[16 + k] = cb[2]; w.f[20 + k] = cb[5];
+            for (int a = 0; a < 3; a++) { nb[a] = std::min(nb[a], cb[a]); nb[3 + a] = std::max(nb[3 + a], cb[3 + a]); }
+        }
+        bound[q] = nb;
+    }
+    return DFR_OK;
+}
+
 // Bakes the per-triangle and per-primitive device records of a scene from
 // its world-space geometry (`wpos`, `wnrm`: three vertices per triangle) in
 // the BVH's triangle order: traversal triangles, shading lines, primitives,
@@ -498,6 +580,7 @@ static int bake_scene(const SceneHost& in, const std::vector<Xform>& xf, const s
     b.prims.assign(in.prims.size(), DevPrim{});
     b.xforms.assign(6 * std::max<size_t>(1, in.prims.size()), f4(0.0f, 0.0f, 0.0f, 0.0f));
     b.analytic.clear();
+    const bool tree = items_in_tree(in);
     for (size_t p = 0; p < in.prims.size(); p++) {
         const dfr_primitive& pr = in.prims[p];
         DevPrim& d = b.prims[p];
@@ -505,7 +588,9 @@ static int bake_scene(const SceneHost& in, const std::vector<Xform>& xf, const s
         d.shape = pr.shape; d.material = pr.material; d.area_light = pr.area_light;
         std::memcpy(d.data, pr.data, sizeof(d.data));
         const Xform* x = xf.empty() || xf[p].identity ? nullptr : &xf[p];
-        if (pr.shape != DFR_SHAPE_TRIANGLE_MESH) b.analytic.push_back((int32_t)p);
+        // infinite planes, and spheres and rectangles too few for a tree of their own
+        if (pr.shape == DFR_SHAPE_PLANE || (!tree && (pr.shape == DFR_SHAPE_SPHERE || pr.shape == DFR_SHAPE_RECTANGLE)))
+            b.analytic.push_back((int32_t)p);
         if (x && !x->linear_identity && pr.shape != DFR_SHAPE_PLANE) {
             // surface frames are built in object space and carried to world
             // space: rows 0-2 map a world normal to its object-space
@@ -742,6 +827,34 @@ int compile_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledSce
     out.n_tris = nt;
     out.sah_cost = bvh.sah_cost;
 
+    // spheres and rectangles: a BVH of their own over their boxes, each box
+    // given to the builder as a degenerate triangle (lo, hi, lo)
+    {
+        std::vector<int32_t> item_prim;
+        std::vector<float> item_box;
+        if (items_in_tree(in)) bounded_items(in, xf, item_prim, item_box);
+        const int64_t ni = (int64_t)item_prim.size();
+        out.item_nodes_host.clear();
+        out.item_prim_order.clear();
+        if (ni > 0) {
+            std::vector<float> ipos(9 * ni);
+            std::vector<int32_t> iidx(3 * ni);
+            for (int64_t k = 0; k < ni; k++) {
+                const float* B = &item_box[6 * k];
+                const float v[9] = {B[0], B[1], B[2], B[3], B[4], B[5], B[0], B[1], B[2]};
+                std::memcpy(&ipos[9 * k], v, sizeof(v));
+                for (int j = 0; j < 3; j++) iidx[3 * k + j] = (int32_t)(3 * k + j);
+            }
+            BvhResult ib = build_bvh(ipos.data(), iidx.data(), ni, 1, 4);
+            out.item_nodes_host = std::move(ib.nodes4);
+            out.item_prim_order.resize(ni);
+            for (int64_t k = 0; k < ni; k++) out.item_prim_order[k] = item_prim[ib.tri_order[k]];
+            out.max_depth = std::max(out.max_depth, ib.max_stack4 + 1);
+            if ((rc = upload(out.item_nodes, out.item_nodes_host, s))) return rc;
+            if ((rc = upload(out.item_prims, out.item_prim_order, s))) return rc;
+        }
+        out.n_items = ni;
+    }
     Baked bk;
     if ((rc = bake_scene(in, xf, tri_prim, wpos, wnrm, bvh.tri_order, bk, err))) return rc;
     std::vector<DevMaterial> mats(in.mats.size());
@@ -788,6 +901,9 @@ int compile_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledSce
     ds.texels = (const float*)out.texels.ptr;
     ds.textures = (const DevTexture*)out.textures.ptr;
     ds.n_analytic = (int32_t)bk.analytic.size();
+    ds.item_nodes = (const float4*)out.item_nodes.ptr;
+    ds.item_prims = (const int32_t*)out.item_prims.ptr;
+    ds.n_items = (int32_t)out.n_items;
     ds.n_lights = (int32_t)bk.lights.size();
     for (const DevLight& L : bk.lights) {
         out.area_lights = out.area_lights || L.type == DFR_LIGHT_DIFFUSE_AREA;
@@ -798,6 +914,7 @@ int compile_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledSce
     ds.light_pdf_mode = light_pdf_mode;
     std::memcpy(out.camera, in.camera, sizeof(out.camera));
     out.bytes = out.nodes.bytes + out.tris.bytes + out.tshade.bytes + out.prims.bytes + out.analytic.bytes +
+                out.item_nodes.bytes + out.item_prims.bytes +
                 out.mats.bytes + out.lobes.bytes + out.lights.bytes + out.perez.bytes + out.xforms.bytes +
                 out.images.bytes + out.texels.bytes + out.textures.bytes;
     if (hipStreamSynchronize(s) != hipSuccess) { err = "upload failed"; return DFR_E_DEVICE; }
@@ -821,34 +938,29 @@ int refit_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledScene
     if ((rc = bake_scene(in, xf, tri_prim, wpos, wnrm, out.tri_order, bk, err))) return rc;
     // refit: each node's bound is the union of its children's
     std::vector<Node4>& nodes = out.nodes4_host;
-    const int64_t nn = (int64_t)nodes.size();
-    std::vector<std::array<float, 6>> bound(nn);
-    for (int64_t q = nn - 1; q >= 0; q--) {
-        Node4& w = nodes[q];
-        std::array<float, 6> nb = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-        for (int k = 0; k < 4; k++) {
-            uint32_t ref;
-            std::memcpy(&ref, &w.f[24 + k], 4);
-            if (ref == EMPTY_REF) continue;
-            std::array<float, 6> cb = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-            if (ref & LEAF_FLAG) {
-                const uint32_t first = (ref & 0x7fffffffu) >> 3, count = (ref & 7u) + 1u;
-                for (uint32_t i = first; i < first + count; i++)
-                    for (int v = 0; v < 3; v++) {
-                        const float4& P = bk.tris[3 * (size_t)i + v];
-                        const float c[3] = {P.x, P.y, P.z};
-                        for (int a = 0; a < 3; a++) { cb[a] = std::min(cb[a], c[a]); cb[3 + a] = std::max(cb[3 + a], c[a]); }
-                    }
-            } else {
-                if ((int64_t)ref <= q || (int64_t)ref >= nn) { err = "BVH child numbered before its parent"; return DFR_E_STATE; }
-                cb = bound[ref];
+    rc = refit_nodes(nodes, [&](uint32_t first, uint32_t count, float* cb) {
+        for (uint32_t i = first; i < first + count; i++)
+            for (int v = 0; v < 3; v++) {
+                const float4& P = bk.tris[3 * (size_t)i + v];
+                const float c[3] = {P.x, P.y, P.z};
+                for (int a = 0; a < 3; a++) { cb[a] = std::min(cb[a], c[a]); cb[3 + a] = std::max(cb[3 + a], c[a]); }
             }
-            w.f[0 + k] = cb[0]; w.f[4 + k] = cb[3];
-            w.f[8 + k] = cb[1]; w.f[12 + k] = cb[4];
-            w.f[16 + k] = cb[2]; w.f[20 + k] = cb[5];
-            for (int a = 0; a < 3; a++) { nb[a] = std::min(nb[a], cb[a]); nb[3 + a] = std::max(nb[3 + a], cb[3 + a]); }
-        }
-        bound[q] = nb;
+    }, err);
+    if (rc) return rc;
+    if (out.n_items > 0) {
+        std::vector<int32_t> item_prim;
+        std::vector<float> item_box;
+        bounded_items(in, xf, item_prim, item_box);
+        std::vector<int64_t> box_of(in.prims.size(), -1);
+        for (size_t k = 0; k < item_prim.size(); k++) box_of[item_prim[k]] = (int64_t)k;
+        rc = refit_nodes(out.item_nodes_host, [&](uint32_t first, uint32_t count, float* cb) {
+            for (uint32_t i = first; i < first + count; i++) {
+                const float* B = &item_box[6 * box_of[out.item_prim_order[i]]];
+                for (int a = 0; a < 3; a++) { cb[a] = std::min(cb[a], B[a]); cb[3 + a] = std::max(cb[3 + a], B[3 + a]); }
+            }
+        }, err);
+        if (rc) return rc;
+        if ((rc = upload(out.item_nodes, out.item_nodes_host, s))) return rc;
     }
     // same sizes as before: the device buffers are rewritten in place, so the
     // scene record's pointers stay valid
@@ -979,6 +1091,11 @@ const DevScene* Ctx::uniform_scene() const {
 
 bool Ctx::any_area_lights() const {
     for (auto& s : scenes) if (s && s->area_lights) return true;
+    return false;
+}
+
+bool Ctx::any_items() const {
+    for (auto& s : scenes) if (s && s->n_items > 0) return true;
     return false;
 }
 
@@ -1151,6 +1268,7 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
     wavefront_init(wf, (const DevJob*)job_buf.ptr, n_jobs, work_total, stream);
     WavefrontParams p{wf_params.block, wf_params.sort_keys, (capacity + 255) / 256, any_area_lights(), uniform_scene(),
                       any_ext_features()};
+    p.items = any_items();
     // After a poll that finds every work item generated, the next
     // iterations' queues hold at most the paths waiting then: size their
     // grids to that count, rounded up to capacity / 4^k (a few captured
@@ -1191,7 +1309,7 @@ int Ctx::run_wavefront(int n_jobs, int64_t work_total, float4* film_ptr, Counter
             (uintptr_t)wf.bhist, (uintptr_t)wf.sh.o, (uintptr_t)wf.sh.d, (uintptr_t)wf.sh.c, (uintptr_t)wf.sh.aux,
             (uintptr_t)wf.overflow, (uintptr_t)wf.ctr, (uintptr_t)wf.job_done, (uintptr_t)wf.capacity,
             (uintptr_t)p.block, (uintptr_t)p.sort_keys, (uintptr_t)p.closest_shadow, (uintptr_t)p.uscene,
-            (uintptr_t)p.ext_features};
+            (uintptr_t)p.ext_features, (uintptr_t)p.items};
         for (int it = 0;; it += poll) {
             if (it >= max_iterations) return fail(DFR_E_DEVICE, "wavefront did not drain");
             std::vector<uintptr_t> k2 = key;

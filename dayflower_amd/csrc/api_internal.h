@@ -70,7 +70,7 @@ struct Xform {
 // A scene compiled into SoA device buffers, resident in HBM.
 struct CompiledScene {
     DeviceBuffer nodes, tris, tshade, prims, prim_class, analytic, mats, lobes, lights, perez, xforms, images, texels,
-        textures;
+        textures, item_nodes, item_prims;
     DevScene dev{};
     int max_depth = 1;
     bool area_lights = false;  // any DiffuseAreaLight (the only lights whose NEE needs closest hits)
@@ -81,6 +81,10 @@ struct CompiledScene {
     float camera[DFR_CAMERA_LENGTH];
     std::vector<Node4> nodes4_host;    // the BVH, kept for refits after transform updates
     std::vector<uint32_t> tri_order;   // BVH position -> triangle
+    // the spheres' and rectangles' BVH, kept for refits
+    std::vector<Node4> item_nodes_host;
+    std::vector<int32_t> item_prim_order;  // item BVH position -> primitive
+    int64_t n_items = 0;
 };
 
 int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err);
@@ -186,6 +190,7 @@ struct Ctx {
     int max_depth() const;
     bool any_area_lights() const;
     bool any_ext_features() const;
+    bool any_items() const;
     int ensure_overflow(int64_t threads);
     int ensure_film(int64_t pixels);
     int ensure_wavefront(int capacity);

@@ -108,7 +108,7 @@ struct DevScene {
     const float4* tshade;
     const DevPrim* prims;
     const int32_t* prim_class;  // per primitive: 4 x the material's sort class
-    const int32_t* analytic;  // indices of non-mesh primitives
+    const int32_t* analytic;  // infinite planes (and spheres and rectangles too few for a tree)
     const DevMaterial* mats;
     const DevLobe* lobes;
     const DevLight* lights;
@@ -120,7 +120,14 @@ struct DevScene {
     const DevImageLight* images;  // image lights (DevLight::perez indexes them)
     const float* texels;          // RGB texels of image lights and image textures
     const DevTexture* textures;
-    int32_t n_analytic;
+    // spheres and rectangles: a four-wide BVH of their own over their world
+    // boxes (item_nodes), whose leaves index item_prims (primitives in BVH
+    // order); walked ahead of the triangles' BVH.  n_items 0: a scene with
+    // few of them tests them in the analytic list instead
+    const float4* item_nodes;
+    const int32_t* item_prims;
+    int32_t n_items;
+    int32_t n_analytic;       // entries of analytic, tested linearly
     int32_t n_lights;
     int32_t has_bvh;
     int32_t light_pdf_mode;   // 0 reference (Shape3F pdf without 1/area), 1 area
@@ -523,6 +530,35 @@ DEV void load_tri(const float4* tris, uint32_t i, float4& a, float4& b, float4& 
     a = ld4off(tris, tb, 0); b = ld4off(tris, tb, 16); c = ld4off(tris, tb, 32);
 }
 
+// Closest (ANY false) or any hit among the scene's spheres and rectangles:
+// an ordered walk of their BVH, one primitive test per item, starting and
+// ending with an empty stack.  Updates tbest and prim on a nearer hit.
+template <int SHORT, bool ANY>
+DEV bool trace_items(const DevScene& s, const PRay& r, const PlaneSel& ps, float tmin, float& tbest, int& prim,
+                     TraversalStack<SHORT>& st) {
+    bool found = false;
+    uint32_t cur = 0;
+    for (;;) {
+        if (cur & LEAF_FLAG_DEV) {
+            const uint32_t first = (cur & 0x7fffffffu) >> 3, count = (cur & 7u) + 1u;
+            for (uint32_t k = 0; k < count; k++) {
+                const int pi = s.item_prims[first + k];
+                const float t = analytic_t(s.prims[pi], r.o, r.d, tmin, tbest);
+                if (t != 0.0f && t < tbest) {
+                    tbest = t; prim = pi; found = true;
+                    if (ANY) { st.sp = 0; return true; }
+                }
+            }
+            if (st.sp == 0) break;
+            cur = (uint32_t)st.pop();
+        } else {
+            cur = visit_node<SHORT, !ANY>(s.item_nodes, cur, r, ps, tmin, tbest, st);
+            if (cur == TRAVERSAL_DONE) break;
+        }
+    }
+    return found;
+}
+
 // With REFINE (closest hit only), the winner's distance is recomputed in
 // FP64 for the ray-query outputs; the path tracer keeps the float distance of
 // the reference intersection routines, from which its next rays start.
@@ -531,7 +567,7 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
                VisitCount& vc) {
     bool found = false;
     hit.t = tmax; hit.prim = -1; hit.tri = -1;
-    // analytic primitives (few): linear, in primitive order
+    // infinite planes (and a handful of spheres and rectangles): linear, in primitive order
     for (int i = 0; i < s.n_analytic; i++) {
         int pi = s.analytic[i];
         float t = analytic_t(s.prims[pi], o, d, tmin, hit.t);
@@ -540,10 +576,14 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
             if (ANY) return true;
         }
     }
-    if (!s.has_bvh) return found;
     PRay r = prep_ray(o, d);
     const PlaneSel ps = plane_sel(r);
     st.sp = 0;
+    if (s.n_items > 0 && trace_items<SHORT, ANY>(s, r, ps, tmin, hit.t, hit.prim, st)) {
+        hit.tri = -1; found = true;
+        if (ANY) return true;
+    }
+    if (!s.has_bvh) return found;
     // Speculative while-while traversal: a lane that reaches a leaf parks it
     // and keeps descending inner nodes until all but TRAVERSAL_STRAGGLERS
     // lanes of the wave have a parked leaf (or are done); then the wave tests

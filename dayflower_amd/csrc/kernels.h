@@ -158,6 +158,9 @@ struct WavefrontParams {
     // material or a spot or directional light: launch the shade kernel
     // built with their code
     bool ext_features = false;
+    // some scene has a BVH over its spheres and rectangles: each traversal
+    // launch is preceded by the item walk over its queue
+    bool items = false;
     // generation-order table, or null when the job table itself is ordered
     // by work_begin (batches)
     const GenEntry* order = nullptr;

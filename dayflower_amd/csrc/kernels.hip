@@ -820,12 +820,24 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
             fresh = false;
             prim = -1; tri = -1; leaf = 0; st.sp = 0; cur = DONE;
             if (COUNT) { ray_nodes = 0; ray_tris = 0; entered = false; }
-            // the ray, the analytic primitives, then the BVH root
-            // (the scene-uniform kernel skips the job -> scene hops)
+            // the ray, the item walk's result, the infinite planes, then the
+            // triangles' BVH root (the scene-uniform kernel skips the job ->
+            // scene hops)
             auto setup = [&](const float4 o4, const float4 d4, uint32_t job) {
                 const DevScene& sc = UNI ? *uscene : scenes[jobs[job].scene];
                 const V o = f4v(o4), d = f4v(d4);
                 r = prep_ray(o, d);
+                if (sc.n_items > 0) {
+                    // k_items_queue ran ahead: the nearest sphere or
+                    // rectangle, or an occluded shadow ray's tmax of -inf
+                    if (SHADOW) {
+                        if (tbest == -INFINITY) prim = 0;
+                    } else {
+                        const float4 h = hits[i];
+                        tbest = h.x;
+                        prim = __float_as_int(h.y);
+                    }
+                }
                 for (int k = 0; k < sc.n_analytic; k++) {
                     const int pi = sc.analytic[k];
                     const float t = analytic_t(sc.prims[pi], o, d, T_MIN, tbest);
@@ -1223,11 +1235,68 @@ void wavefront_init(WavefrontBuffers& w, const DevJob* jobs, int n_jobs, int64_t
     hipMemcpyAsync(w.ctr, &c, sizeof(Counters), hipMemcpyHostToDevice, s);
 }
 
+// Spheres and rectangles ahead of a traversal launch over the path queue
+// (SHADOW false) or the any-hit shadow queue (SHADOW true), one ray per
+// thread: each ray of a scene with an item BVH walks it, and the closest hit
+// goes to hits[i] as (t, primitive, -1) for k_trace_queue to start from; an
+// occluded shadow ray's tmax becomes -inf.  Kept out of k_trace_queue, whose
+// register budget (72 at 7 waves per SIMD) the walk's code would spill, and
+// whose rays set up one at a time as lanes free up; here a whole wave walks
+// together.
+template <int BLOCK, int SHORT, bool SHADOW>
+__global__ __launch_bounds__(BLOCK) void k_items_queue(const DevScene* __restrict__ scenes,
+                                                       const DevJob* __restrict__ jobs, PathSoA q, ShadowSoA sh,
+                                                       float4* __restrict__ hits, const Counters* __restrict__ ctr,
+                                                       int* __restrict__ overflow, const DevScene* __restrict__ uscene) {
+    __shared__ int lds[SHORT * BLOCK];
+    const int n = SHADOW ? (int)(ctr->n_shadow2 & 0xffffffffull) : ctr->n_active;
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    float4 o4, d4;
+    uint32_t job = 0;
+    if (SHADOW) {
+        o4 = sh.o[i];
+        d4 = sh.d[i];
+        if (!uscene) job = (uint32_t)sh.aux[i].y;
+    } else {
+        o4 = q.o[i];
+        d4 = q.d[i];
+        uint32_t flags;
+        if (uscene) {
+            flags = __float_as_uint(o4.w);
+        } else {
+            const uint4 info = q.info[i];
+            flags = info.w;
+            job = info.z;
+        }
+        if (flags & 0x80000000u) return;  // dropped camera sample
+    }
+    const DevScene& sc = uscene ? *uscene : scenes[jobs[job].scene];
+    if (sc.n_items == 0) return;
+    TraversalStack<SHORT> st;
+    st.init(lds, overflow, BLOCK);
+    const PRay r = prep_ray(f4v(o4), f4v(d4));
+    float tbest = SHADOW ? o4.w : FLT_MAX;
+    int prim = -1;
+    const bool hit = trace_items<SHORT, SHADOW>(sc, r, plane_sel(r), T_MIN, tbest, prim, st);
+    if (SHADOW) {
+        if (hit) sh.o[i] = make_float4(o4.x, o4.y, o4.z, -INFINITY);
+    } else {
+        hits[i] = make_float4(tbest, __int_as_float(prim), __int_as_float(-1), 0.0f);
+    }
+}
+
 // One traversal launch over the path queue (SHADOW false) or the any-hit
 // shadow queue (SHADOW true); the scene-uniform variant when p.uscene is set.
+// With p.items, the item walk over the same queue runs first, in `igrid`
+// blocks of B threads (one per ray).
 template <int B, bool COUNT, bool SHADOW>
-static void launch_trace_queue(int grid, int tiles, const DevScene* scenes, const DevJob* jobs, float4* film,
-                               WavefrontBuffers& w, const WavefrontParams& p, unsigned long long* trav, hipStream_t s) {
+static void launch_trace_queue(int grid, int igrid, int tiles, const DevScene* scenes, const DevJob* jobs,
+                               float4* film, WavefrontBuffers& w, const WavefrontParams& p, unsigned long long* trav,
+                               hipStream_t s) {
+    if (p.items)
+        hipLaunchKernelGGL((k_items_queue<B, SHORT_STACK, SHADOW>), dim3(igrid), dim3(B), 0, s, scenes, jobs, w.a, w.sh,
+                           w.hit, w.ctr, w.overflow, p.uscene);
     const RaySample smp = COUNT ? w.sample : RaySample{};
     if (BVH_WIDTH == 4 && p.uscene)
         hipLaunchKernelGGL((k_trace_queue<B, SHORT_STACK, SHADOW ? SHADOW_RPL : EXTEND_RPL, COUNT, SHADOW, true>), dim3(grid), dim3(B), 0, s,
@@ -1266,8 +1335,9 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     if (timing) hipEventRecord(prof->ev[0], s);
     unsigned long long* trav = prof ? prof->trav : nullptr;
     const int egrid = (qb + EXTEND_RPL * B - 1) / (EXTEND_RPL * B);
-    if (trav) launch_trace_queue<B, true, false>(egrid, tiles, scenes, jobs, film, w, p, trav, s);
-    else launch_trace_queue<B, false, false>(egrid, tiles, scenes, jobs, film, w, p, trav, s);
+    const int eigrid = (qb + B - 1) / B;
+    if (trav) launch_trace_queue<B, true, false>(egrid, eigrid, tiles, scenes, jobs, film, w, p, trav, s);
+    else launch_trace_queue<B, false, false>(egrid, eigrid, tiles, scenes, jobs, film, w, p, trav, s);
     if (timing) hipEventRecord(prof->ev[1], s);
     if (use_perm) {
         if (!fused_hist)
@@ -1302,13 +1372,14 @@ static void iteration_b(const DevScene* scenes, const DevJob* jobs, int n_jobs, 
     // overflow stack, is the same
     constexpr int SB = B < 128 ? B : 128;
     const int agrid = (int)((2 * (int64_t)qb + SHADOW_RPL * SB - 1) / (SHADOW_RPL * SB));
+    const int aigrid = (int)((2 * (int64_t)qb + SB - 1) / SB);
     if (trav) {
-        launch_trace_queue<SB, true, true>(agrid, tiles, scenes, jobs, film, w, p, trav, s);
+        launch_trace_queue<SB, true, true>(agrid, aigrid, tiles, scenes, jobs, film, w, p, trav, s);
         if (p.closest_shadow)
             hipLaunchKernelGGL((k_shadow_closest<B, SHORT_STACK, true>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh,
                                w.ctr, film, w.overflow, trav, cap);
     } else {
-        launch_trace_queue<SB, false, true>(agrid, tiles, scenes, jobs, film, w, p, trav, s);
+        launch_trace_queue<SB, false, true>(agrid, aigrid, tiles, scenes, jobs, film, w, p, trav, s);
         if (p.closest_shadow)
             hipLaunchKernelGGL((k_shadow_closest<B, SHORT_STACK, false>), dim3(sgrid), dim3(B), 0, s, scenes, jobs, w.sh,
                                w.ctr, film, w.overflow, trav, cap);

@@ -468,6 +468,7 @@ static void server_loop(Server* S) {
             hipMemcpyAsync(&S->wf.ctr->work_total, S->total_host[cur], sizeof(int64_t), hipMemcpyHostToDevice, S->stream);
             p.closest_shadow = c->any_area_lights();
             p.ext_features = c->any_ext_features();
+            p.items = c->any_items();
             p.uscene = scene_uniform >= 0 ? (const DevScene*)S->table_mem.ptr + scene_uniform : nullptr;
             p.order = (const GenEntry*)S->order_mem.ptr;
             p.n_order = (int)ord.size();
