@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libdayflower_hip.so")
 BUILD = os.path.join(HERE, "_build")
 
-SOURCES = ["kernels.hip", "api.cpp", "serve.cpp", "bvh.cpp", "perez_host.cpp", "compiled_scene.cpp"]
+SOURCES = ["kernels.hip", "api.cpp", "serve.cpp", "bvh.cpp", "perez_host.cpp", "compiled_scene.cpp", "instances.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DFR_OFFLOAD_ARCH", "gfx950")
 

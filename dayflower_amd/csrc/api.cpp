@@ -381,7 +381,7 @@ inline std::array<float, 3> fl_unit(D3v v) {
 // positions of every triangle (9 floats each), the world-space primitive
 // records and the owning primitive of each triangle.
 static int world_geometry(const SceneHost& in, const std::vector<Xform>& xf, std::vector<int32_t>& tri_prim,
-                          std::vector<float>& wpos, std::vector<float>& wnrm, std::string& err) {
+                          const InstanceSet& is, std::vector<float>& wpos, std::vector<float>& wnrm, std::string& err) {
     const int64_t nt = (int64_t)in.indices.size() / 3;
     const int64_t nv = (int64_t)in.positions.size() / 3;
     // object-space vertex normals: given, or the normalised sum of the
@@ -415,7 +415,8 @@ static int world_geometry(const SceneHost& in, const std::vector<Xform>& xf, std
     wpos.resize(9 * nt);
     wnrm.resize(9 * nt);
     for (int64_t t = 0; t < nt; t++) {
-        const Xform* x = xf.empty() ? nullptr : &xf[tri_prim[t]];
+        // a shared mesh's triangles stay in object space
+        const Xform* x = xf.empty() || is.tri_group[t] >= 0 ? nullptr : &xf[tri_prim[t]];
         for (int k = 0; k < 3; k++) {
             const int64_t v = in.indices[3 * t + k];
             const auto p = fl(xpoint(x, &in.positions[3 * v]));
@@ -486,8 +487,8 @@ static void bounded_items(const SceneHost& in, const std::vector<Xform>& xf, std
 // Refits a four-wide tree in place: each child box becomes the union of its
 // subtree, leaves bounded by `leaf(first, count, box6)`; children are
 // numbered after their parents, so one backward pass suffices.
-static int refit_nodes(std::vector<Node4>& nodes, const std::function<void(uint32_t, uint32_t, float*)>& leaf,
-                       std::string& err) {
+int refit_nodes(std::vector<Node4>& nodes, const std::function<void(uint32_t, uint32_t, float*)>& leaf,
+                std::string& err) {
     const int64_t nn = (int64_t)nodes.size();
     std::vector<std::array<float, 6>> bound(nn);
     for (int64_t q = nn - 1; q >= 0; q--) {
@@ -531,7 +532,7 @@ struct Baked {
 };
 
 static int bake_scene(const SceneHost& in, const std::vector<Xform>& xf, const std::vector<int32_t>& tri_prim,
-                      const std::vector<float>& wpos, const std::vector<float>& wnrm,
+                      const InstanceSet& is, const std::vector<float>& wpos, const std::vector<float>& wnrm,
                       const std::vector<uint32_t>& tri_order, Baked& b, std::string& err) {
     const int64_t nt = (int64_t)in.indices.size() / 3;
     // Triangle3F surface normal: normalize(cross(normalize(b-a), normalize(c-a)))
@@ -549,7 +550,7 @@ static int bake_scene(const SceneHost& in, const std::vector<Xform>& xf, const s
         auto ng = nz({ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]});
         // a reflection turns the world-space winding: the normal matrix
         // keeps the object-space side
-        if (!xf.empty() && xf[tri_prim[t]].det < 0.0) ng = {-ng[0], -ng[1], -ng[2]};
+        if (!xf.empty() && is.tri_group[t] < 0 && xf[tri_prim[t]].det < 0.0) ng = {-ng[0], -ng[1], -ng[2]};
         float4* r = &b.tshade[8 * t];
         r[0] = f4(P[0], P[1], P[2], N[0]);
         r[1] = f4(P[3], P[4], P[5], N[1]);
@@ -566,8 +567,9 @@ static int bake_scene(const SceneHost& in, const std::vector<Xform>& xf, const s
         r[5] = f4(ng[2], uv[0], uv[1], uv[2]);
         r[6] = f4(uv[3], uv[4], uv[5], 0.0f);
     }
-    b.tris.resize(3 * nt);
-    for (int64_t i = 0; i < nt; i++) {
+    // the BVH's triangles (those of unshared meshes), in its order
+    b.tris.resize(3 * tri_order.size());
+    for (size_t i = 0; i < tri_order.size(); i++) {
         const uint32_t t = tri_order[i];
         const float* P = &wpos[9 * (int64_t)t];
         float tid, pid;
@@ -604,7 +606,16 @@ static int bake_scene(const SceneHost& in, const std::vector<Xform>& xf, const s
                 X[r] = f4((float)(x->a[0][r] * k0), (float)(x->a[1][r] * k0), (float)(x->a[2][r] * k0), 0.0f);
                 X[3 + r] = f4((float)(x->n[r][0] * k1), (float)(x->n[r][1] * k1), (float)(x->n[r][2] * k1), 0.0f);
             }
-            d.xformed = 1;
+            d.xformed = PRIM_FRAME_XFORM;
+        }
+        if (pr.shape == DFR_SHAPE_TRIANGLE_MESH && is.prim_group[p] >= 0) {
+            // an instance: its mesh's records are in object space, the
+            // kernels carry them to world space through this matrix
+            d.xformed |= PRIM_INSTANCE;
+            for (int r = 0; r < 3; r++) {
+                for (int c = 0; c < 3; c++) d.data[4 * r + c] = x ? (float)x->a[r][c] : (r == c ? 1.0f : 0.0f);
+                d.data[4 * r + 3] = x ? (float)x->t[r] : 0.0f;
+            }
         }
         if (pr.shape == DFR_SHAPE_SPHERE && x) {
             const auto c = fl(xpoint(x, pr.data));
@@ -783,7 +794,7 @@ static int upload_baked(const Baked& bk, CompiledScene& out, hipStream_t s) {
     return DFR_OK;
 }
 
-static int validate_scene(const SceneHost& in, std::vector<int32_t>& tri_prim, std::string& err) {
+static int validate_scene(const SceneHost& in, std::vector<int32_t>& tri_prim, InstanceSet& is, std::string& err) {
     const int64_t nt = (int64_t)in.indices.size() / 3;
     const int64_t nv = (int64_t)in.positions.size() / 3;
     for (int64_t i = 0; i < 3 * nt; i++)
@@ -803,28 +814,44 @@ static int validate_scene(const SceneHost& in, std::vector<int32_t>& tri_prim, s
     }
     for (int64_t t = 0; t < nt; t++)
         if (tri_prim[t] < 0) { err = "triangle not owned by any mesh primitive"; return DFR_E_ARGUMENT; }
-    return DFR_OK;
+    return find_instances(in, is, err);
 }
 
 int compile_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledScene& out, int light_pdf_mode,
                   hipStream_t s, std::string& err) {
     const int64_t nt = (int64_t)in.indices.size() / 3;
     std::vector<int32_t> tri_prim;
-    int rc = validate_scene(in, tri_prim, err);
+    InstanceSet& is = out.inst;
+    int rc = validate_scene(in, tri_prim, is, err);
     if (rc) return rc;
+    if (is.any() && BVH_WIDTH != 4) { err = "shared meshes need four-wide trees"; return DFR_E_UNSUPPORTED; }
     std::vector<float> wpos, wnrm;
-    if ((rc = world_geometry(in, xf, tri_prim, wpos, wnrm, err))) return rc;
+    if ((rc = world_geometry(in, xf, tri_prim, is, wpos, wnrm, err))) return rc;
 
     auto t0 = std::chrono::steady_clock::now();
     int threads = (int)std::max(1u, std::thread::hardware_concurrency());
-    std::vector<int32_t> flat_idx(3 * nt);
-    for (int64_t i = 0; i < 3 * nt; i++) flat_idx[i] = (int32_t)i;
-    BvhResult bvh = build_bvh(wpos.data(), flat_idx.data(), nt, threads, BVH_WIDTH);
+    // the scene's BVH holds the triangles of unshared meshes (all of them
+    // when no mesh is shared)
+    std::vector<int32_t> flat_idx;
+    std::vector<uint32_t> own;  // BVH input position -> triangle
+    flat_idx.reserve(3 * nt);
+    for (int64_t t = 0; t < nt; t++) {
+        if (is.tri_group[t] >= 0) continue;
+        own.push_back((uint32_t)t);
+        for (int j = 0; j < 3; j++) flat_idx.push_back((int32_t)(3 * t + j));
+    }
+    const int64_t n_own = (int64_t)own.size();
+    BvhResult bvh = build_bvh(wpos.data(), flat_idx.data(), n_own, threads, BVH_WIDTH);
+    if (n_own < nt)
+        for (uint32_t& t : bvh.tri_order) t = own[t];
+    if (is.any()) build_instances(wpos, xf, is);
     out.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     // deepest traversal stack (entries) the kernels can need
     out.max_depth = BVH_WIDTH == 4 ? bvh.max_stack4 + 1 : bvh.max_depth + 1;
+    if (is.any()) out.max_depth = std::max(out.max_depth, is.max_stack + 1);
     out.n_nodes = BVH_WIDTH == 4 ? (int64_t)bvh.nodes4.size() : (int64_t)bvh.nodes.size();
     out.n_tris = nt;
+    out.n_bvh_tris = n_own;
     out.sah_cost = bvh.sah_cost;
 
     // spheres and rectangles: a BVH of their own over their boxes, each box
@@ -856,7 +883,21 @@ int compile_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledSce
         out.n_items = ni;
     }
     Baked bk;
-    if ((rc = bake_scene(in, xf, tri_prim, wpos, wnrm, bvh.tri_order, bk, err))) return rc;
+    if ((rc = bake_scene(in, xf, tri_prim, is, wpos, wnrm, bvh.tri_order, bk, err))) return rc;
+    // (host sources of the asynchronous uploads live until the final synchronisation)
+    std::vector<Node4> inodes;
+    std::vector<DevInstance> irecs;
+    if (is.any()) {
+        inodes = instance_nodes(is);
+        irecs = instance_records(xf, is);
+        if (inodes.size() * sizeof(Node4) >= (size_t(1) << 32) || is.tris.size() * sizeof(float4) >= (size_t(1) << 32)) {
+            err = "scene too large: shared meshes' nodes and triangles must each stay below 4 GiB";
+            return DFR_E_UNSUPPORTED;
+        }
+        if ((rc = upload(out.inst_nodes, inodes, s))) return rc;
+        if ((rc = upload(out.inst_tris, is.tris, s))) return rc;
+        if ((rc = upload(out.instances, irecs, s))) return rc;
+    }
     std::vector<DevMaterial> mats(in.mats.size());
     std::vector<DevLobe> lobes;
     for (size_t m = 0; m < in.mats.size(); m++) compile_material(in.mats[m], mats[m], lobes);
@@ -904,17 +945,23 @@ int compile_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledSce
     ds.item_nodes = (const float4*)out.item_nodes.ptr;
     ds.item_prims = (const int32_t*)out.item_prims.ptr;
     ds.n_items = (int32_t)out.n_items;
+    ds.inst_nodes = (const float4*)out.inst_nodes.ptr;
+    ds.inst_tris = (const float4*)out.inst_tris.ptr;
+    ds.instances = (const DevInstance*)out.instances.ptr;
+    ds.n_instances = (int32_t)is.order.size();
+    ds.prepass = (ds.n_items > 0 || ds.n_instances > 0) ? 1 : 0;
     ds.n_lights = (int32_t)bk.lights.size();
     for (const DevLight& L : bk.lights) {
         out.area_lights = out.area_lights || L.type == DFR_LIGHT_DIFFUSE_AREA;
         out.ext_features = out.ext_features || L.type == DFR_LIGHT_SPOT || L.type == DFR_LIGHT_DIRECTIONAL ||
                            L.type == DFR_LIGHT_IMAGE;
     }
-    ds.has_bvh = nt > 0 ? 1 : 0;
+    ds.has_bvh = n_own > 0 ? 1 : 0;
     ds.light_pdf_mode = light_pdf_mode;
     std::memcpy(out.camera, in.camera, sizeof(out.camera));
     out.bytes = out.nodes.bytes + out.tris.bytes + out.tshade.bytes + out.prims.bytes + out.analytic.bytes +
-                out.item_nodes.bytes + out.item_prims.bytes +
+                out.item_nodes.bytes + out.item_prims.bytes + out.inst_nodes.bytes + out.inst_tris.bytes +
+                out.instances.bytes +
                 out.mats.bytes + out.lobes.bytes + out.lights.bytes + out.perez.bytes + out.xforms.bytes +
                 out.images.bytes + out.texels.bytes + out.textures.bytes;
     if (hipStreamSynchronize(s) != hipSuccess) { err = "upload failed"; return DFR_E_DEVICE; }
@@ -928,14 +975,26 @@ int compile_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledSce
 int refit_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledScene& out, hipStream_t s, std::string& err) {
     if (BVH_WIDTH != 4) { err = "transform updates refit four-wide trees"; return DFR_E_UNSUPPORTED; }
     const int64_t nt = (int64_t)in.indices.size() / 3;
-    if (nt != out.n_tris || (int64_t)out.tri_order.size() != nt) { err = "scene changed since it was compiled"; return DFR_E_STATE; }
+    if (nt != out.n_tris) { err = "scene changed since it was compiled"; return DFR_E_STATE; }
     std::vector<int32_t> tri_prim;
-    int rc = validate_scene(in, tri_prim, err);
+    InstanceSet is;
+    int rc = validate_scene(in, tri_prim, is, err);
     if (rc) return rc;
+    if (is.prim_group != out.inst.prim_group) { err = "scene changed since it was compiled"; return DFR_E_STATE; }
     std::vector<float> wpos, wnrm;
-    if ((rc = world_geometry(in, xf, tri_prim, wpos, wnrm, err))) return rc;
+    if ((rc = world_geometry(in, xf, tri_prim, is, wpos, wnrm, err))) return rc;
     Baked bk;
-    if ((rc = bake_scene(in, xf, tri_prim, wpos, wnrm, out.tri_order, bk, err))) return rc;
+    if ((rc = bake_scene(in, xf, tri_prim, is, wpos, wnrm, out.tri_order, bk, err))) return rc;
+    std::vector<Node4> inodes;
+    std::vector<DevInstance> irecs;
+    if (out.inst.any()) {
+        // the shared meshes' trees are object space: only the top level moves
+        refit_instances(xf, out.inst);
+        inodes = instance_nodes(out.inst);
+        irecs = instance_records(xf, out.inst);
+        if ((rc = upload(out.inst_nodes, inodes, s))) return rc;
+        if ((rc = upload(out.instances, irecs, s))) return rc;
+    }
     // refit: each node's bound is the union of its children's
     std::vector<Node4>& nodes = out.nodes4_host;
     rc = refit_nodes(nodes, [&](uint32_t first, uint32_t count, float* cb) {
@@ -1095,7 +1154,7 @@ bool Ctx::any_area_lights() const {
 }
 
 bool Ctx::any_items() const {
-    for (auto& s : scenes) if (s && s->n_items > 0) return true;
+    for (auto& s : scenes) if (s && s->dev.prepass) return true;
     return false;
 }
 
@@ -1946,15 +2005,15 @@ int32_t dfr_read_bvh(dfr_ctx* h, float* nodes, int64_t* n_nodes, float* tris, in
     if (BVH_WIDTH != 4) return c->fail(DFR_E_UNSUPPORTED, "the BVH export describes four-wide nodes");
     const int64_t want_nodes = *n_nodes, want_tris = *n_tris;
     *n_nodes = cs.n_nodes;
-    *n_tris = cs.n_tris;
+    *n_tris = cs.n_bvh_tris;
     hipError_t e = hipSuccess;
     if (nodes && cs.n_nodes > 0) {
         if (want_nodes < cs.n_nodes) return c->fail(DFR_E_ARGUMENT, "node buffer too small");
         e = hipMemcpy(nodes, cs.nodes.ptr, (size_t)cs.n_nodes * 128, hipMemcpyDeviceToHost);
     }
-    if (e == hipSuccess && tris && cs.n_tris > 0) {
-        if (want_tris < cs.n_tris) return c->fail(DFR_E_ARGUMENT, "triangle buffer too small");
-        e = hipMemcpy(tris, cs.tris.ptr, (size_t)cs.n_tris * 48, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && tris && cs.n_bvh_tris > 0) {
+        if (want_tris < cs.n_bvh_tris) return c->fail(DFR_E_ARGUMENT, "triangle buffer too small");
+        e = hipMemcpy(tris, cs.tris.ptr, (size_t)cs.n_bvh_tris * 48, hipMemcpyDeviceToHost);
     }
     if (e != hipSuccess) return c->hip_fail(e, "BVH readback");
     return DFR_OK;

@@ -67,15 +67,53 @@ struct Xform {
     bool identity = true, linear_identity = true, similarity = true;
 };
 
+// Mesh primitives that name the same triangle range are instances of one
+// mesh: its triangles stay in object space, in a four-wide BVH of their own
+// (one per shared range), and a top-level BVH over the instances' world
+// boxes leads rays to them.  Built by instances.cpp.
+struct InstanceSet {
+    struct Group {
+        int64_t first = 0, count = 0;  // the shared triangle range
+        uint32_t root = 0;             // its tree's root within `bottom`
+        float box[6];                  // object-space bounds: lo.xyz, hi.xyz
+    };
+    std::vector<int32_t> prim_group;   // per primitive: its group, or -1
+    std::vector<int32_t> tri_group;    // per triangle: its group, or -1
+    std::vector<Group> groups;
+    std::vector<Node4> top;            // top-level tree over instances (root 0)
+    std::vector<Node4> bottom;         // the groups' trees, references relative to `bottom`
+    std::vector<float4> tris;          // the groups' triangles, object space, tree order
+    std::vector<int32_t> order;        // top-level position -> primitive
+    int max_stack = 0;                 // deepest stack of a top-level plus a group walk
+    bool any() const { return !groups.empty(); }
+};
+
+// Finds the shared triangle ranges of `in` (identical ranges of two or more
+// mesh primitives; ranges that overlap otherwise are an error, as is an area
+// light on a shared range) into is.prim_group, tri_group and groups.
+int find_instances(const SceneHost& in, InstanceSet& is, std::string& err);
+// Builds every group's tree from `opos` (object-space vertices, nine floats
+// per triangle) and the top-level tree over the instances' world boxes.
+void build_instances(const std::vector<float>& opos, const std::vector<Xform>& xf, InstanceSet& is);
+// Refits the top-level tree to new transforms (the groups' trees are object
+// space and stay as they are).
+void refit_instances(const std::vector<Xform>& xf, InstanceSet& is);
+// The device records of the instances in top-level order.
+std::vector<DevInstance> instance_records(const std::vector<Xform>& xf, const InstanceSet& is);
+// The top-level tree followed by the groups' trees, with the latter's
+// references moved past the former: the node array the kernels walk.
+std::vector<Node4> instance_nodes(const InstanceSet& is);
+
 // A scene compiled into SoA device buffers, resident in HBM.
 struct CompiledScene {
     DeviceBuffer nodes, tris, tshade, prims, prim_class, analytic, mats, lobes, lights, perez, xforms, images, texels,
-        textures, item_nodes, item_prims;
+        textures, item_nodes, item_prims, inst_nodes, inst_tris, instances;
     DevScene dev{};
     int max_depth = 1;
     bool area_lights = false;  // any DiffuseAreaLight (the only lights whose NEE needs closest hits)
     bool ext_features = false;  // any glossy, substrate, translucent, clear coat or uber material, or spot or directional light
     int64_t n_nodes = 0, n_tris = 0;
+    int64_t n_bvh_tris = 0;            // triangles in the scene's BVH (those of unshared meshes)
     double sah_cost = 0.0, build_seconds = 0.0;
     size_t bytes = 0;
     float camera[DFR_CAMERA_LENGTH];
@@ -85,6 +123,7 @@ struct CompiledScene {
     std::vector<Node4> item_nodes_host;
     std::vector<int32_t> item_prim_order;  // item BVH position -> primitive
     int64_t n_items = 0;
+    InstanceSet inst;                  // shared meshes, kept for refits
 };
 
 int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err);

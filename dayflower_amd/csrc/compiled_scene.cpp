@@ -169,6 +169,7 @@ private:
     const dfr_compiled_scene& s_;
     Out& o_;
     std::map<std::pair<int, int>, int> tex_memo_, mat_memo_;
+    std::map<int64_t, std::pair<int64_t, int64_t>> mesh_memo_;  // mesh offset -> (first triangle, count)
     std::map<int, int> area_light_index_;   // DiffuseAreaLight offset -> IR light index
     std::vector<int> area_light_prim_;      // per IR light: the primitive that carries it (-1 none)
     std::vector<std::pair<int, int>> area_light_shape_;  // per IR light: (shape ID, shape offset)
@@ -611,11 +612,22 @@ private:
                 triangle(shape_off);
                 p.triangle_count = 1;
                 break;
-            case SHAPE_TRIANGLE_MESH:
+            case SHAPE_TRIANGLE_MESH: {
+                // the compiler stores a mesh once however many primitives
+                // name it (Shape3FCache.java:176-177, 298-300): those
+                // primitives share its triangle range and become instances
                 p.shape = DFR_SHAPE_TRIANGLE_MESH;
-                p.first_triangle = (int64_t)o_.indices.size() / 3;
-                p.triangle_count = mesh(shape_off);
+                auto it = mesh_memo_.find(shape_off);
+                if (it != mesh_memo_.end()) {
+                    p.first_triangle = it->second.first;
+                    p.triangle_count = it->second.second;
+                } else {
+                    p.first_triangle = (int64_t)o_.indices.size() / 3;
+                    p.triangle_count = mesh(shape_off);
+                    mesh_memo_[shape_off] = {p.first_triangle, p.triangle_count};
+                }
                 break;
+            }
             case SHAPE_SPHERE: {
                 // the kernels' unit sphere at the origin (AbstractShape3FKernel
                 // .java:2025-2038); a uniform scale s with no rotation is folded

@@ -126,7 +126,18 @@ struct DevScene {
     // few of them tests them in the analytic list instead
     const float4* item_nodes;
     const int32_t* item_prims;
+    // instances of shared meshes: the top-level tree over their world boxes
+    // (root 0; its leaves index `instances`) followed by every shared mesh's
+    // own tree (leaves index inst_tris, object-space triangles in the
+    // layout of `tris`, a.w the original triangle)
+    const float4* inst_nodes;
+    const float4* inst_tris;
+    const DevInstance* instances;
+    int32_t n_instances;
     int32_t n_items;
+    // items or instances present: the wavefront's traversal starts from the
+    // result k_items_queue left in the hit record
+    int32_t prepass;
     int32_t n_analytic;       // entries of analytic, tested linearly
     int32_t n_lights;
     int32_t has_bvh;
@@ -423,9 +434,12 @@ DEV uint32_t visit2(const float4* nodes, uint32_t cur, const PRay& r, float tmin
 // four-wide node -> the next reference (ORDERED: the nearest entered child,
 // the other entered children pushed farthest first, through a five-exchange
 // sorting network; otherwise, for any-hit queries, the entered children in
-// slot order); with none entered, the top of the stack.
+// slot order); with none entered, the top of the stack, or TRAVERSAL_DONE
+// when the stack is down to `base` entries (a walk nested on top of another
+// owns only the entries above its base).
 template <int SHORT, bool ORDERED>
-DEV uint32_t visit4_next(const float (&tn)[4], const float (&tf)[4], const float4 rf, TraversalStack<SHORT>& st) {
+DEV uint32_t visit4_next(const float (&tn)[4], const float (&tf)[4], const float4 rf, TraversalStack<SHORT>& st,
+                         int base = 0) {
     const uint32_t R[4] = {__float_as_uint(rf.x), __float_as_uint(rf.y), __float_as_uint(rf.z), __float_as_uint(rf.w)};
     float key[4];
     uint32_t ref[4];
@@ -458,10 +472,10 @@ DEV uint32_t visit4_next(const float (&tn)[4], const float (&tf)[4], const float
             else st.push((int)ref[c]);
         }
         if (next != TRAVERSAL_DONE) return next;
-        return st.sp == 0 ? TRAVERSAL_DONE : (uint32_t)st.pop();
+        return st.sp == base ? TRAVERSAL_DONE : (uint32_t)st.pop();
     }
     ce(0, 1); ce(2, 3); ce(0, 2); ce(1, 3); ce(1, 2);
-    if (hits == 0) return st.sp == 0 ? TRAVERSAL_DONE : (uint32_t)st.pop();
+    if (hits == 0) return st.sp == base ? TRAVERSAL_DONE : (uint32_t)st.pop();
     if (hits >= 4) st.push((int)ref[3]);
     if (hits >= 3) st.push((int)ref[2]);
     if (hits >= 2) st.push((int)ref[1]);
@@ -496,7 +510,7 @@ DEV float4 ld4off(const float4* base, uint32_t byte_off, uint32_t imm) {
 // its base from scalar registers.
 template <int SHORT, bool ORDERED = true>
 DEV uint32_t visit4_signed(const float4* nodes, uint32_t cur, const PRay& r, const PlaneSel& ps, float tmin, float tmax,
-                           TraversalStack<SHORT>& st) {
+                           TraversalStack<SHORT>& st, int base = 0) {
     const uint32_t nb = cur << 7;  // 128 bytes per node
     const uint32_t ox = nb | ps.x, oy = nb | ps.y, oz = nb | ps.z;
     const float4 nx = ld4off(nodes, ox, 0), fx = ld4off(nodes, ox ^ 16u, 0);
@@ -514,13 +528,13 @@ DEV uint32_t visit4_signed(const float4* nodes, uint32_t cur, const PRay& r, con
         tn[c] = fmaxf(fmaxf(ax, ay), fmaxf(az, tmin));
         tf[c] = fminf(fminf(bx, by), fminf(bz, tmax));
     }
-    return visit4_next<SHORT, ORDERED>(tn, tf, rf, st);
+    return visit4_next<SHORT, ORDERED>(tn, tf, rf, st, base);
 }
 
 template <int SHORT, bool ORDERED = true>
 DEV uint32_t visit_node(const float4* nodes, uint32_t cur, const PRay& r, const PlaneSel& ps, float tmin, float tmax,
-                        TraversalStack<SHORT>& st) {
-    if (BVH_WIDTH == 4) return visit4_signed<SHORT, ORDERED>(nodes, cur, r, ps, tmin, tmax, st);
+                        TraversalStack<SHORT>& st, int base = 0) {
+    if (BVH_WIDTH == 4) return visit4_signed<SHORT, ORDERED>(nodes, cur, r, ps, tmin, tmax, st, base);
     return visit2<SHORT>(nodes, cur, r, tmin, tmax, st);
 }
 
@@ -559,6 +573,88 @@ DEV bool trace_items(const DevScene& s, const PRay& r, const PlaneSel& ps, float
     return found;
 }
 
+// A point or a direction through a 3 x 4 row-major matrix.
+DEV V xf34_point(const float* m, V p) {
+    return mk(fmaf(m[0], p.x, fmaf(m[1], p.y, fmaf(m[2], p.z, m[3]))),
+              fmaf(m[4], p.x, fmaf(m[5], p.y, fmaf(m[6], p.z, m[7]))),
+              fmaf(m[8], p.x, fmaf(m[9], p.y, fmaf(m[10], p.z, m[11]))));
+}
+DEV V xf34_vec(const float* m, V v) {
+    return mk(fmaf(m[0], v.x, fmaf(m[1], v.y, m[2] * v.z)), fmaf(m[4], v.x, fmaf(m[5], v.y, m[6] * v.z)),
+              fmaf(m[8], v.x, fmaf(m[9], v.y, m[10] * v.z)));
+}
+
+// Closest (ANY false) or any hit among the instances of shared meshes: an
+// ordered walk of the top-level tree over their world boxes; at an instance
+// the ray is carried into its mesh's object space, where its direction keeps
+// its scale so that distances along it are the world ray's, and the mesh's
+// tree is walked on the same stack above the top level's pending entries.
+// The world ray's traversal form is rebuilt at each top-level step rather
+// than kept beside the object-space one (fewer live registers in every
+// kernel that traces).  Updates tbest, prim and tri on a nearer hit.
+template <int SHORT, bool ANY>
+DEV bool trace_instances(const DevScene& s, V o, V d, float tmin, float& tbest, int& prim, int& tri,
+                         TraversalStack<SHORT>& st) {
+    bool found = false;
+    uint32_t cur = 0;
+    for (;;) {
+        if (cur & LEAF_FLAG_DEV) {
+            const uint32_t first = (cur & 0x7fffffffu) >> 3, count = (cur & 7u) + 1u;
+            const PRay rw = prep_ray(o, d);
+            for (uint32_t k = 0; k < count; k++) {
+                const DevInstance& in = s.instances[first + k];
+                // the instance's own world box against the current segment
+                // (widened as the node test widens its exit distance)
+                const float ax = in.lo[0] * rw.inv.x - rw.oinv.x, bx = in.hi[0] * rw.inv.x - rw.oinv.x;
+                const float ay = in.lo[1] * rw.inv.y - rw.oinv.y, by = in.hi[1] * rw.inv.y - rw.oinv.y;
+                const float az = in.lo[2] * rw.inv.z - rw.oinv.z, bz = in.hi[2] * rw.inv.z - rw.oinv.z;
+                const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
+                const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tbest));
+                if (!(tn <= tf * 1.00000048f)) continue;
+                const PRay ro = prep_ray(xf34_point(in.w2o, o), xf34_vec(in.w2o, d));
+                const PlaneSel pso = plane_sel(ro);
+                const int base = st.sp;
+                uint32_t c2 = (uint32_t)in.root;
+                for (;;) {
+                    if (c2 & LEAF_FLAG_DEV) {
+                        const uint32_t f2 = (c2 & 0x7fffffffu) >> 3, n2 = (c2 & 7u) + 1u;
+                        for (uint32_t j = 0; j < n2; j++) {
+                            float4 a, b, c;
+                            load_tri(s.inst_tris, f2 + j, a, b, c);
+                            const float t = tri_t(f4v(a), f4v(b), f4v(c), ro.o, ro.d, ro.kx, ro.ky, ro.kz, ro.sx, ro.sy,
+                                                  ro.sz, tmin, tbest);
+                            if (t != 0.0f && t < tbest) {
+                                tbest = t; prim = in.prim; tri = __float_as_int(a.w); found = true;
+                                if (ANY) { st.sp = 0; return true; }
+                            }
+                        }
+                        if (st.sp == base) break;
+                        c2 = (uint32_t)st.pop();
+                    } else {
+                        c2 = visit_node<SHORT, !ANY>(s.inst_nodes, c2, ro, pso, tmin, tbest, st, base);
+                        if (c2 == TRAVERSAL_DONE) break;
+                    }
+                }
+            }
+            if (st.sp == 0) break;
+            cur = (uint32_t)st.pop();
+        } else {
+            const PRay rw = prep_ray(o, d);
+            cur = visit_node<SHORT, !ANY>(s.inst_nodes, cur, rw, plane_sel(rw), tmin, tbest, st);
+            if (cur == TRAVERSAL_DONE) break;
+        }
+    }
+    return found;
+}
+
+// The three vertices of triangle record `tri` in world space (an instance's
+// records are in its mesh's object space).
+DEV void tri_world(const DevScene& s, const DevPrim& p, int tri, V& a, V& b, V& c) {
+    const size_t rb = 8 * (size_t)tri;
+    a = f4v(ld4g(s.tshade, rb)); b = f4v(ld4g(s.tshade, rb + 1)); c = f4v(ld4g(s.tshade, rb + 2));
+    if (p.xformed & PRIM_INSTANCE) { a = xf34_point(p.data, a); b = xf34_point(p.data, b); c = xf34_point(p.data, c); }
+}
+
 // With REFINE (closest hit only), the winner's distance is recomputed in
 // FP64 for the ray-query outputs; the path tracer keeps the float distance of
 // the reference intersection routines, from which its next rays start.
@@ -581,6 +677,10 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
     st.sp = 0;
     if (s.n_items > 0 && trace_items<SHORT, ANY>(s, r, ps, tmin, hit.t, hit.prim, st)) {
         hit.tri = -1; found = true;
+        if (ANY) return true;
+    }
+    if (s.n_instances > 0 && trace_instances<SHORT, ANY>(s, o, d, tmin, hit.t, hit.prim, hit.tri, st)) {
+        found = true;
         if (ANY) return true;
     }
     if (!s.has_bvh) return found;
@@ -625,8 +725,8 @@ DEV bool trace(const DevScene& s, V o, V d, float tmin, float tmax, HitRec& hit,
     if (REFINE && !ANY && found && hit.tri >= 0) {
         // The float watertight t loses digits on small, oblique triangles;
         // the winner's distance to its plane is recomputed in FP64.
-        const size_t rb = 8 * (size_t)hit.tri;
-        float4 a = ld4g(s.tshade, rb), b = ld4g(s.tshade, rb + 1), c = ld4g(s.tshade, rb + 2);
+        V a, b, c;
+        tri_world(s, s.prims[hit.prim], hit.tri, a, b, c);
         double e1x = (double)b.x - a.x, e1y = (double)b.y - a.y, e1z = (double)b.z - a.z;
         double e2x = (double)c.x - a.x, e2y = (double)c.y - a.y, e2z = (double)c.z - a.z;
         double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
@@ -697,24 +797,37 @@ DEV Surf make_surface(const DevScene& sc, V o, V d, const HitRec& h) {
     s.bu = 0.0f; s.bv = 0.0f;
     s.p = o + d * h.t;
     const DevPrim& p = sc.prims[h.prim];
-    const float4* X = p.xformed ? sc.xforms + 6 * (size_t)h.prim : nullptr;
+    const float4* X = (p.xformed & PRIM_FRAME_XFORM) ? sc.xforms + 6 * (size_t)h.prim : nullptr;
     if (h.tri >= 0) {
         const size_t rb = 8 * (size_t)h.tri;
         const float4 r0 = ld4g(sc.tshade, rb), r1 = ld4g(sc.tshade, rb + 1), r2 = ld4g(sc.tshade, rb + 2);
         const float4 r3 = ld4g(sc.tshade, rb + 3), r4 = ld4g(sc.tshade, rb + 4), r5 = ld4g(sc.tshade, rb + 5);
         V a = f4v(r0), b = f4v(r1), c = f4v(r2);
+        V na = mk(r0.w, r1.w, r2.w), nb = f4v(r3), nc = f4v(r4), ng = mk(r3.w, r4.w, r5.x);
+        if (__builtin_expect(p.xformed & PRIM_INSTANCE, 0)) {
+            // an instance: its mesh's record is in object space; vertices
+            // through the matrix, normals through the normal matrix N (the
+            // scene compiler's world-space geometric normal of an unshared
+            // mesh, cross(A e1, A e2) = det(A) N cross(e1, e2) flipped back to
+            // the object-space side under a reflection, is N ng normalised)
+            a = xf34_point(p.data, a); b = xf34_point(p.data, b); c = xf34_point(p.data, c);
+            if (X) {
+                na = mul_rows(X + 3, na); nb = mul_rows(X + 3, nb); nc = mul_rows(X + 3, nc);
+                ng = nrm(mul_rows(X + 3, ng));
+            }
+        }
         V eab = b - a, eca = a - c;
         V d1 = cross(eab, eca);
         float rdet = __builtin_amdgcn_rcpf(dot(d, d1));
         V d3 = cross(a - o, d);
         float u = dot(d3, eca) * rdet, v = dot(d3, eab) * rdet, w = 1.0f - u - v;
         s.bu = u; s.bv = v;
-        V ns = nrm(mk(r0.w, r1.w, r2.w) * w + f4v(r3) * u + f4v(r4) * v);
+        V ns = nrm(na * w + nb * u + nc * v);
         if (X) {
-            s.g = frame_xformed(X, mk(r3.w, r4.w, r5.x));
+            s.g = frame_xformed(X, ng);
             s.s = frame_xformed(X, ns);
         } else {
-            s.g = onb_w(mk(r3.w, r4.w, r5.x));
+            s.g = onb_w(ng);
             s.s = onb_w(ns);  // texture coordinates are absent: UV determinant 0
         }
     } else if (p.shape == DFR_SHAPE_SPHERE) {

@@ -78,9 +78,27 @@ struct DevTexture {        // 80 bytes
 
 struct DevPrim {           // 80 bytes
     int32_t shape, material, area_light;
-    int32_t xformed;       // 1: the surface frame is built in object space (DevScene::xforms)
+    // bit 0: the surface frame is built in object space (DevScene::xforms);
+    // bit 1 (PRIM_INSTANCE): a mesh instance whose triangle records are in
+    // object space, `data` its object-to-world matrix (3 x 4, row-major)
+    int32_t xformed;
     float data[12];
     float normal[4];       // rectangle: normalize(cross(normalize(b-a), normalize(c-a)))
+};
+constexpr int32_t PRIM_FRAME_XFORM = 1;
+constexpr int32_t PRIM_INSTANCE = 2;
+
+// One instance of a shared mesh, a leaf item of the top-level tree (80
+// bytes): the world-to-object map that carries a ray into the mesh's object
+// space (rows of a 3 x 4 matrix), the instance's world box (a top-level leaf
+// bounds up to eight instances), the root of the mesh's tree in
+// DevScene::inst_nodes and the instance's primitive.
+struct DevInstance {
+    float w2o[12];
+    float lo[3];
+    int32_t root;
+    float hi[3];
+    int32_t prim;
 };
 
 struct DevLight {          // 64 bytes

@@ -158,8 +158,9 @@ struct WavefrontParams {
     // material or a spot or directional light: launch the shade kernel
     // built with their code
     bool ext_features = false;
-    // some scene has a BVH over its spheres and rectangles: each traversal
-    // launch is preceded by the item walk over its queue
+    // some scene has a BVH over its spheres and rectangles or instances of
+    // shared meshes: each traversal launch is preceded by their walk over
+    // its queue (k_items_queue)
     bool items = false;
     // generation-order table, or null when the job table itself is ordered
     // by work_begin (batches)

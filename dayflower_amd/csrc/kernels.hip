@@ -827,15 +827,17 @@ __global__ __launch_bounds__(BLOCK, 7) void k_trace_queue(const DevScene* __rest
                 const DevScene& sc = UNI ? *uscene : scenes[jobs[job].scene];
                 const V o = f4v(o4), d = f4v(d4);
                 r = prep_ray(o, d);
-                if (sc.n_items > 0) {
-                    // k_items_queue ran ahead: the nearest sphere or
-                    // rectangle, or an occluded shadow ray's tmax of -inf
+                if (sc.prepass) {
+                    // k_items_queue ran ahead: the nearest sphere, rectangle
+                    // or instance triangle, or an occluded shadow ray's tmax
+                    // of -inf
                     if (SHADOW) {
                         if (tbest == -INFINITY) prim = 0;
                     } else {
                         const float4 h = hits[i];
                         tbest = h.x;
                         prim = __float_as_int(h.y);
+                        tri = __float_as_int(h.z);
                     }
                 }
                 for (int k = 0; k < sc.n_analytic; k++) {
@@ -1235,11 +1237,12 @@ void wavefront_init(WavefrontBuffers& w, const DevJob* jobs, int n_jobs, int64_t
     hipMemcpyAsync(w.ctr, &c, sizeof(Counters), hipMemcpyHostToDevice, s);
 }
 
-// Spheres and rectangles ahead of a traversal launch over the path queue
-// (SHADOW false) or the any-hit shadow queue (SHADOW true), one ray per
-// thread: each ray of a scene with an item BVH walks it, and the closest hit
-// goes to hits[i] as (t, primitive, -1) for k_trace_queue to start from; an
-// occluded shadow ray's tmax becomes -inf.  Kept out of k_trace_queue, whose
+// Spheres, rectangles and instances of shared meshes ahead of a traversal
+// launch over the path queue (SHADOW false) or the any-hit shadow queue
+// (SHADOW true), one ray per thread: each ray of a scene with an item BVH or
+// instances walks them, and the closest hit goes to hits[i] as (t,
+// primitive, triangle) for k_trace_queue to start from; an occluded shadow
+// ray's tmax becomes -inf.  Kept out of k_trace_queue, whose
 // register budget (72 at 7 waves per SIMD) the walk's code would spill, and
 // whose rays set up one at a time as lanes free up; here a whole wave walks
 // together.
@@ -1272,17 +1275,21 @@ __global__ __launch_bounds__(BLOCK) void k_items_queue(const DevScene* __restric
         if (flags & 0x80000000u) return;  // dropped camera sample
     }
     const DevScene& sc = uscene ? *uscene : scenes[jobs[job].scene];
-    if (sc.n_items == 0) return;
+    if (!sc.prepass) return;
     TraversalStack<SHORT> st;
     st.init(lds, overflow, BLOCK);
-    const PRay r = prep_ray(f4v(o4), f4v(d4));
+    const V o = f4v(o4), d = f4v(d4);
+    const PRay r = prep_ray(o, d);
+    const PlaneSel ps = plane_sel(r);
     float tbest = SHADOW ? o4.w : FLT_MAX;
-    int prim = -1;
-    const bool hit = trace_items<SHORT, SHADOW>(sc, r, plane_sel(r), T_MIN, tbest, prim, st);
+    int prim = -1, tri = -1;
+    bool hit = false;
+    if (sc.n_items > 0) hit = trace_items<SHORT, SHADOW>(sc, r, ps, T_MIN, tbest, prim, st);
+    if (sc.n_instances > 0 && !(SHADOW && hit)) hit = trace_instances<SHORT, SHADOW>(sc, o, d, T_MIN, tbest, prim, tri, st) || hit;
     if (SHADOW) {
         if (hit) sh.o[i] = make_float4(o4.x, o4.y, o4.z, -INFINITY);
     } else {
-        hits[i] = make_float4(tbest, __int_as_float(prim), __int_as_float(-1), 0.0f);
+        hits[i] = make_float4(tbest, __int_as_float(prim), __int_as_float(tri), 0.0f);
     }
 }
 

@@ -197,6 +197,24 @@ class Scene:
         self.primitives.append(p)
         return len(self.primitives) - 1
 
+    def add_instance(self, prim, material=None):
+        """Another primitive over mesh primitive `prim`'s triangles (the same
+        triangle range), with `material` or prim's.  Place it with a
+        per-primitive object-to-world matrix (GPURenderer.updateTransforms);
+        the library keeps one object-space copy of the triangles and a
+        top-level tree over the instances.  Returns the new primitive."""
+        src = self.primitives[prim]
+        if src.shape != abi.SHAPE_TRIANGLE_MESH:
+            raise ValueError("instances are of mesh primitives")
+        p = abi.Primitive()
+        p.shape = abi.SHAPE_TRIANGLE_MESH
+        p.material = src.material if material is None else material
+        p.area_light = -1
+        p.first_triangle = src.first_triangle
+        p.triangle_count = src.triangle_count
+        self.primitives.append(p)
+        return len(self.primitives) - 1
+
     def _add_analytic(self, shape, material, data):
         p = abi.Primitive()
         p.shape = shape

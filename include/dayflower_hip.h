@@ -198,7 +198,15 @@ typedef struct dfr_texture {
 /* One primitive (M/scene/Primitive.java), given in its object space; its
  * object-to-world matrix is the identity unless dfr_update_transforms sets
  * one (a caller may also fold transforms in and leave the identity).
- *   TRIANGLE_MESH  triangles [first_triangle, first_triangle+triangle_count)
+ *   TRIANGLE_MESH  triangles [first_triangle, first_triangle+triangle_count);
+ *                  mesh primitives that name the same range are instances
+ *                  of one mesh (its triangles are kept once, in object
+ *                  space, under a BVH of their own, with a top-level BVH
+ *                  over the instances' world boxes; the reference keeps one
+ *                  record per distinct TriangleMesh3F, Shape3FCache.java
+ *                  :176-177, 298-300); ranges that overlap without being
+ *                  equal are DFR_E_ARGUMENT, an area light on a shared range
+ *                  DFR_E_UNSUPPORTED
  *   SPHERE         data = center.xyz, radius
  *   RECTANGLE      data = corners a.xyz, b.xyz, c.xyz  (Rectangle3F(a,b,c))
  *   PLANE          data = origin.xyz, u.xyz, v.xyz, w.xyz (w = normal)
@@ -521,7 +529,8 @@ int32_t dfr_get_stats(const dfr_ctx* ctx, dfr_stats* stats);
  * last render with DFR_KEY_COUNT_TRAVERSAL and DFR_KEY_RAY_SAMPLE set (at
  * most 2^20).  With records NULL only *n_records is set.
  *
- * dfr_read_bvh: the main scene's four-wide BVH, 32 floats per node (lo.x[4],
+ * dfr_read_bvh: the main scene's four-wide BVH (over the triangles of the
+ * meshes no other primitive shares), 32 floats per node (lo.x[4],
  * hi.x[4], lo.y[4], hi.y[4], lo.z[4], hi.z[4], child references[4] as uint32
  * bits, 4 unused), and its triangles in BVH order, 12 floats each (a.xyz,
  * original triangle id as int bits, b.xyz, primitive as int bits, c.xyz, 0).

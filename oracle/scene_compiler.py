@@ -116,6 +116,7 @@ class SceneCompiler:
                                                          "mirror", "plastic", "substrate", "translucent", "uber")}
         self.triangles = []
         self.meshes = []          # int records, concatenated
+        self.mesh_offsets = {}    # (first, count) -> offset of its mesh record
         self.aabbs = []
         self.rects = []
 
@@ -252,7 +253,13 @@ class SceneCompiler:
         return off
 
     def mesh(self, first, count):
-        """Element offset of a TriangleMesh3F over triangles [first, first + count)."""
+        """Element offset of a TriangleMesh3F over triangles [first, first + count)
+        (compiled once per distinct mesh: primitives that share a triangle
+        range share one TriangleMesh3F, as primitives sharing a mesh object
+        share one record in the reference's distinct lists, Shape3FCache.java
+        :176-177, 298-300)."""
+        if (first, count) in self.mesh_offsets:
+            return self.mesh_offsets[(first, count)]
         sc = self.sc
         tris = list(range(first, first + count))
         offs = {t: self.triangle(t) for t in tris}
@@ -289,6 +296,7 @@ class SceneCompiler:
             out += rec
         base = sum(len(m) for m in self.meshes)
         self.meshes.append(out)
+        self.mesh_offsets[(first, count)] = base
         return base
 
     # -- the whole scene ----------------------------------------------------

@@ -288,6 +288,47 @@ def test_transformed_primitives_keep_their_matrices():
     _same_world(_world_prims(sc, m), _world_prims(t, t["transforms"]), tol=5e-6)
 
 
+def test_shared_mesh_compiles_once_and_comes_back_as_instances():
+    """Primitives that share a mesh (Scene.add_instance) compile to one
+    TriangleMesh3F record, as the reference's distinct lists keep one record
+    per mesh object (Shape3FCache.java:176-177, 298-300); the transcoder
+    gives every primitive that names the record the same triangle range,
+    which the library renders as instances, and the world-space scene under
+    the matrices is unchanged."""
+    sc = S.Scene()
+    mat = sc.add_material(S.matte((0.5, 0.5, 0.5)))
+    red = sc.add_material(S.matte((0.8, 0.1, 0.1)))
+    p, tris, n = S.displaced_icosphere(300, 5)
+    base = sc.add_mesh(p, tris, mat, normals=n)
+    other = sc.add_mesh(p + 2.0, tris[:20], mat)
+    inst = [sc.add_instance(base, red), sc.add_instance(base)]
+    sc.add_point_light((0, 3, 0), (1, 1, 1))
+    m = np.tile(np.eye(4), (len(sc.primitives), 1, 1))
+    c, s = np.cos(0.7), np.sin(0.7)
+    m[inst[0]][:3, :3] = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]]) @ np.diag([0.5, 0.7, 0.5])
+    m[inst[0]][:3, 3] = (1.5, 0.0, 0.0)
+    m[inst[1]][:3, :3] = np.diag([-0.4, 0.4, 0.4])
+    m[inst[1]][:3, 3] = (-1.5, 0.2, 0.3)
+    arr = _compiled(sc, m)
+    # one record each for the two distinct meshes
+    once = _compiled(_single(sc, base, other))
+    assert len(arr["triangle_meshes"]) == len(once["triangle_meshes"])
+    t = R.transcode_compiled_scene(arr)
+    prims = t["primitives"]
+    ranges = [(q.first_triangle, q.triangle_count) for q in prims[:4]]
+    assert ranges[2] == ranges[0] and ranges[3] == ranges[0] and ranges[1] != ranges[0]
+    assert t["positions"].shape[0] // 3 == len(tris) + 20   # each distinct mesh once
+    _same_world(_world_prims(sc, m), _world_prims(t, t["transforms"]), tol=5e-6)
+
+
+def _single(sc, *keep):
+    """The scene with only primitives `keep` (and its lights)."""
+    out = S.Scene()
+    out.__dict__.update(sc.__dict__)
+    out.primitives = [sc.primitives[i] for i in keep]
+    return out
+
+
 @pytest.mark.parametrize("n", [1, 2, 5, 9, 300, 2000])
 def test_mesh_bvh_walk_reaches_every_leaf(n):
     sc = S.Scene()
