@@ -63,7 +63,7 @@ class Material(ctypes.Structure):
 
 SLOT_KD, SLOT_KS, SLOT_KR, SLOT_KT = range(4)
 (TEX_CONSTANT, TEX_CHECKERBOARD, TEX_UV, TEX_SURFACE_NORMAL, TEX_DOT_PRODUCT, TEX_BULLSEYE, TEX_POLKA_DOT, TEX_BLEND,
- TEX_IMAGE) = range(9)
+ TEX_IMAGE, TEX_MARBLE, TEX_SIMPLEX_FBM) = range(11)
 
 
 class Texture(ctypes.Structure):

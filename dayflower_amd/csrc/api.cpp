@@ -1076,7 +1076,12 @@ int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err) {
     for (int t = 0; t < nt; t++) {
         const dfr_texture& T = h.textures[t];
         const std::string who = "texture " + std::to_string(t);
-        if (T.type < DFR_TEX_CONSTANT || T.type > DFR_TEX_IMAGE) { err = who + ": type out of range"; return DFR_E_UNSUPPORTED; }
+        if (T.type < DFR_TEX_CONSTANT || T.type > DFR_TEX_SIMPLEX_FBM) { err = who + ": type out of range"; return DFR_E_UNSUPPORTED; }
+        // noise octaves: a whole number the kernels loop over
+        if ((T.type == DFR_TEX_MARBLE || T.type == DFR_TEX_SIMPLEX_FBM) &&
+            !(T.cell >= 0.0f && T.cell <= 64.0f && T.cell == std::floor(T.cell))) {
+            err = who + ": octaves must be a whole number from 0 to 64"; return DFR_E_ARGUMENT;
+        }
         const bool two = T.type == DFR_TEX_CHECKERBOARD || T.type == DFR_TEX_BULLSEYE || T.type == DFR_TEX_POLKA_DOT ||
                          T.type == DFR_TEX_BLEND;
         if (two && (T.a < 0 || T.a >= nt || T.b < 0 || T.b >= nt)) { err = who + ": sub-texture out of range"; return DFR_E_ARGUMENT; }

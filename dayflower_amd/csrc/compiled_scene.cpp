@@ -70,6 +70,11 @@ constexpr int CLEAR_COAT_LENGTH = 2, DISNEY_LENGTH = 16, GLASS_LENGTH = 8, GLOSS
 // CompiledTextureCache.java
 constexpr int BLEND_TEXTURE_LENGTH = 8, BULLSEYE_TEXTURE_LENGTH = 8, CHECKERBOARD_TEXTURE_LENGTH = 8,
               CONSTANT_TEXTURE_LENGTH = 4, POLKA_DOT_TEXTURE_LENGTH = 8;
+constexpr int MARBLE_TEXTURE_LENGTH = 8, MARBLE_TEXTURE_OFFSET_COLOR_A = 0, MARBLE_TEXTURE_OFFSET_COLOR_B = 1,
+              MARBLE_TEXTURE_OFFSET_COLOR_C = 2, MARBLE_TEXTURE_OFFSET_FREQUENCY = 3, MARBLE_TEXTURE_OFFSET_SCALE = 4,
+              MARBLE_TEXTURE_OFFSET_OCTAVES = 6;
+constexpr int SIMPLEX_FBM_TEXTURE_LENGTH = 4, SIMPLEX_FBM_TEXTURE_OFFSET_COLOR = 0, SIMPLEX_FBM_TEXTURE_OFFSET_FREQUENCY = 1,
+              SIMPLEX_FBM_TEXTURE_OFFSET_GAIN = 2, SIMPLEX_FBM_TEXTURE_OFFSET_OCTAVES = 3;
 constexpr int LDR_IMAGE_OFFSET_ANGLE_RADIANS = 0, LDR_IMAGE_OFFSET_SCALE = 1, LDR_IMAGE_OFFSET_RESOLUTION_X = 3,
               LDR_IMAGE_OFFSET_RESOLUTION_Y = 4, LDR_IMAGE_OFFSET_IMAGE = 5;
 // CompiledLightCache.java:68-263
@@ -194,6 +199,14 @@ private:
         s = s + c[2];
         return s / 3.0f;
     }
+    // a colour packed as 0xAARRGGBB int bits stored in a float: byte / 255
+    // (colorRGBIntToRFloat and kin)
+    static void argb(float packed, float* rgb) {
+        const uint32_t c = (uint32_t)(int32_t)packed;
+        rgb[0] = (float)((c >> 16) & 0xffu) / 255.0f;
+        rgb[1] = (float)((c >> 8) & 0xffu) / 255.0f;
+        rgb[2] = (float)(c & 0xffu) / 255.0f;
+    }
     static float degrees(float radians) { return (float)((double)radians * (180.0 / 3.14159265358979323846)); }
 
     int texture(int id, int off, int depth = 0) {
@@ -287,8 +300,30 @@ private:
                 t.scale[1] = a.at(b + LDR_IMAGE_OFFSET_SCALE + 1);
                 break;
             }
-            case TEX_MARBLE: fail(DFR_E_UNSUPPORTED, "MarbleTexture has no counterpart in the kernels");
-            case TEX_SIMPLEX_FBM: fail(DFR_E_UNSUPPORTED, "SimplexFractionalBrownianMotionTexture has no counterpart in the kernels");
+            case TEX_MARBLE: {  // :1885-1909 (colours A, B, C packed ARGB, frequency, scale, stripes, octaves)
+                const FA a = fa(s_.marble_textures, "marble_textures");
+                const int64_t b = (int64_t)off * MARBLE_TEXTURE_LENGTH;
+                a.need(b, MARBLE_TEXTURE_LENGTH);
+                t.type = DFR_TEX_MARBLE;
+                argb(a.at(b + MARBLE_TEXTURE_OFFSET_COLOR_A), t.color);
+                argb(a.at(b + MARBLE_TEXTURE_OFFSET_COLOR_B), t.origin);
+                argb(a.at(b + MARBLE_TEXTURE_OFFSET_COLOR_C), t.t);
+                t.scale[0] = a.at(b + MARBLE_TEXTURE_OFFSET_FREQUENCY);
+                t.scale[1] = a.at(b + MARBLE_TEXTURE_OFFSET_SCALE);
+                t.cell = a.at(b + MARBLE_TEXTURE_OFFSET_OCTAVES);
+                break;
+            }
+            case TEX_SIMPLEX_FBM: {  // :2025-2043 (colour packed ARGB, frequency, gain, octaves)
+                const FA a = fa(s_.simplex_fbm_textures, "simplex_fbm_textures");
+                const int64_t b = (int64_t)off * SIMPLEX_FBM_TEXTURE_LENGTH;
+                a.need(b, SIMPLEX_FBM_TEXTURE_LENGTH);
+                t.type = DFR_TEX_SIMPLEX_FBM;
+                argb(a.at(b + SIMPLEX_FBM_TEXTURE_OFFSET_COLOR), t.color);
+                t.scale[0] = a.at(b + SIMPLEX_FBM_TEXTURE_OFFSET_FREQUENCY);
+                t.scale[1] = a.at(b + SIMPLEX_FBM_TEXTURE_OFFSET_GAIN);
+                t.cell = a.at(b + SIMPLEX_FBM_TEXTURE_OFFSET_OCTAVES);
+                break;
+            }
             case TEX_FUNCTION: fail(DFR_E_UNSUPPORTED, "FunctionTexture has no compiled form");
             default: fail(DFR_E_ARGUMENT, "unknown texture type ID " + std::to_string(id));
         }

@@ -1533,6 +1533,112 @@ DEV V texture_image(const DevScene& sc, const DevTexture& T, float u, float v) {
     return a * (1.0f - fy) + b * fy;
 }
 
+// ---------------------------------------------------------------------------
+// Noise of the Marble and simplex fBm textures: the reference's Perlin noise
+// (M/noise/PerlinNoiseF.java:113-200: improved noise with its own gradient
+// choice, u = x for hashes below 8 and 12, 13, v = y for hashes below 4 and
+// 12, 13) and turbulence (:231-250), and its 3D simplex noise (M/noise
+// /SimplexNoiseF.java:405-507, clamped to [-1, 1]) and fBm (:246-267).
+// ---------------------------------------------------------------------------
+#include "noise_tables.inc"
+static __device__ const uint8_t NOISE_PERM_DEV[512] = {NOISE_PERM_256, NOISE_PERM_256};
+static __device__ const float NOISE_GRAD3_DEV[36] = {NOISE_GRAD3_36};
+
+DEV float perlin_grad(int h, float x, float y, float z) {
+    const float u = (h < 8 || h == 12 || h == 13) ? x : y;
+    const float v = (h < 4 || h == 12 || h == 13) ? y : z;
+    return ((h & 1) == 0 ? u : -u) + ((h & 2) == 0 ? v : -v);
+}
+
+DEV float perlin_noise(float x, float y, float z) {
+    if (!isfinite(x) || !isfinite(y) || !isfinite(z)) return 0.0f;
+    const uint8_t* P = NOISE_PERM_DEV;
+    const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+    const int x0 = (int)fx & 0xff, y0 = (int)fy & 0xff, z0 = (int)fz & 0xff;
+    const float x1 = x - fx, y1 = y - fy, z1 = z - fz;
+    const float u = x1 * x1 * x1 * (x1 * (x1 * 6.0f - 15.0f) + 10.0f);
+    const float v = y1 * y1 * y1 * (y1 * (y1 * 6.0f - 15.0f) + 10.0f);
+    const float w = z1 * z1 * z1 * (z1 * (z1 * 6.0f - 15.0f) + 10.0f);
+    const int a0 = P[x0] + y0, a1 = P[a0] + z0, a2 = P[a0 + 1] + z0;
+    const int b0 = P[x0 + 1] + y0, b1 = P[b0] + z0, b2 = P[b0 + 1] + z0;
+    const float g0 = perlin_grad(P[a1] & 15, x1, y1, z1);
+    const float g1 = perlin_grad(P[b1] & 15, x1 - 1.0f, y1, z1);
+    const float g2 = perlin_grad(P[a2] & 15, x1, y1 - 1.0f, z1);
+    const float g3 = perlin_grad(P[b2] & 15, x1 - 1.0f, y1 - 1.0f, z1);
+    const float g4 = perlin_grad(P[a1 + 1] & 15, x1, y1, z1 - 1.0f);
+    const float g5 = perlin_grad(P[b1 + 1] & 15, x1 - 1.0f, y1, z1 - 1.0f);
+    const float g6 = perlin_grad(P[a2 + 1] & 15, x1, y1 - 1.0f, z1 - 1.0f);
+    const float g7 = perlin_grad(P[b2 + 1] & 15, x1 - 1.0f, y1 - 1.0f, z1 - 1.0f);
+    const float l0 = g0 + u * (g1 - g0), l1 = g2 + u * (g3 - g2), l2 = g4 + u * (g5 - g4), l3 = g6 + u * (g7 - g6);
+    const float l4 = l0 + v * (l1 - l0), l5 = l2 + v * (l3 - l2);
+    return l4 + w * (l5 - l4);
+}
+
+// |noise| summed over octaves at doubling frequencies, each divided by its
+// frequency (the first octave at the point itself)
+DEV float perlin_turbulence(float x, float y, float z, int octaves) {
+    float noise = fabsf(perlin_noise(x, y, z)), weight = 1.0f;
+    for (int i = 1; i < octaves; i++) {
+        weight *= 2.0f;
+        noise += fabsf(perlin_noise(x * weight, y * weight, z * weight)) / weight;
+    }
+    return noise;
+}
+
+DEV int fast_floor(float v) {
+    const int i = (int)v;
+    return v < (float)i ? i - 1 : i;
+}
+
+DEV float simplex_noise(float x, float y, float z) {
+    if (!isfinite(x) || !isfinite(y) || !isfinite(z)) return 0.0f;
+    const uint8_t* P = NOISE_PERM_DEV;
+    const float* G = NOISE_GRAD3_DEV;
+    constexpr float F3 = 1.0f / 3.0f, G3 = 1.0f / 6.0f;
+    const float s = (x + y + z) * F3;
+    const int i = fast_floor(x + s), j = fast_floor(y + s), k = fast_floor(z + s);
+    const float t = (float)(i + j + k) * G3;
+    const float x0 = x - ((float)i - t), y0 = y - ((float)j - t), z0 = z - ((float)k - t);
+    int i1, j1, k1, i2, j2, k2;
+    if (x0 >= y0) {
+        if (y0 >= z0) { i1 = 1; j1 = 0; k1 = 0; i2 = 1; j2 = 1; k2 = 0; }
+        else if (x0 >= z0) { i1 = 1; j1 = 0; k1 = 0; i2 = 1; j2 = 0; k2 = 1; }
+        else { i1 = 0; j1 = 0; k1 = 1; i2 = 1; j2 = 0; k2 = 1; }
+    } else {
+        if (y0 < z0) { i1 = 0; j1 = 0; k1 = 1; i2 = 0; j2 = 1; k2 = 1; }
+        else if (x0 < z0) { i1 = 0; j1 = 1; k1 = 0; i2 = 0; j2 = 1; k2 = 1; }
+        else { i1 = 0; j1 = 1; k1 = 0; i2 = 1; j2 = 1; k2 = 0; }
+    }
+    const float x1 = x0 - (float)i1 + G3, y1 = y0 - (float)j1 + G3, z1 = z0 - (float)k1 + G3;
+    const float x2 = x0 - (float)i2 + 2.0f * G3, y2 = y0 - (float)j2 + 2.0f * G3, z2 = z0 - (float)k2 + 2.0f * G3;
+    const float x3 = x0 - 1.0f + 3.0f * G3, y3 = y0 - 1.0f + 3.0f * G3, z3 = z0 - 1.0f + 3.0f * G3;
+    const int ii = i & 0xff, jj = j & 0xff, kk = k & 0xff;
+    const int gi0 = P[ii + P[jj + P[kk]]] % 12;
+    const int gi1 = P[ii + i1 + P[jj + j1 + P[kk + k1]]] % 12;
+    const int gi2 = P[ii + i2 + P[jj + j2 + P[kk + k2]]] % 12;
+    const int gi3 = P[ii + 1 + P[jj + 1 + P[kk + 1]]] % 12;
+    auto corner = [&](int gi, float a, float b, float c) {
+        const float tt = 0.6f - a * a - b * b - c * c;
+        return tt < 0.0f ? 0.0f : (tt * tt) * (tt * tt) * (G[gi * 3] * a + G[gi * 3 + 1] * b + G[gi * 3 + 2] * c);
+    };
+    const float n = 32.0f * (corner(gi0, x0, y0, z0) + corner(gi1, x1, y1, z1) + corner(gi2, x2, y2, z2) +
+                             corner(gi3, x3, y3, z3));
+    return fminf(fmaxf(n, -1.0f), 1.0f);
+}
+
+// fBm of simplex noise mapped to [0, 1] (minimum 0, maximum 1)
+DEV float simplex_fbm(float x, float y, float z, float frequency, float gain, int octaves) {
+    float amplitude = 1.0f, total = 0.0f, f = frequency, noise = 0.0f;
+    for (int i = 0; i < octaves; i++) {
+        noise += simplex_noise(x * f, y * f, z * f) * amplitude;
+        total += amplitude;
+        amplitude *= gain;
+        f *= 2.0f;
+    }
+    noise /= total;
+    return noise * (1.0f - 0.0f) / 2.0f + (1.0f + 0.0f) / 2.0f;
+}
+
 // A texture that is not a blend, following its chain of sub-textures.
 DEV V texture_chain(const DevScene& sc, int t, const Surf& si, V d, float u, float v) {
     for (int guard = 0; guard < 64; guard++) {
@@ -1566,6 +1672,22 @@ DEV V texture_chain(const DevScene& sc, int t, const Surf& si, V d, float u, flo
                 break;
             }
             case DFR_TEX_IMAGE: return texture_image(sc, T, u, v);
+            case DFR_TEX_MARBLE: {
+                // MarbleTexture.getColor (MarbleTexture.java:131-146): colours A
+                // (color), B (origin), C (t); frequency, scale, octaves in
+                // scale_u, scale_v, cell
+                const float x = si.p.x * T.scale_u, y = si.p.y * T.scale_u, z = si.p.z * T.scale_u;
+                const float r = T.scale_v * perlin_turbulence(x, y, z, (int)T.cell);
+                const float sv = 2.0f * fabsf(sinf(x + r));
+                const float tv = sv < 1.0f ? sv : sv - 1.0f;
+                const V lo = sv < 1.0f ? ld3(T.t) : ld3(T.origin), hi = sv < 1.0f ? ld3(T.origin) : ld3(T.color);
+                return lo * (1.0f - tv) + hi * tv;
+            }
+            case DFR_TEX_SIMPLEX_FBM: {
+                // SimplexFractionalBrownianMotionTexture.getColor (:127-137):
+                // frequency, gain, octaves in scale_u, scale_v, cell
+                return ld3(T.color) * simplex_fbm(si.p.x, si.p.y, si.p.z, T.scale_u, T.scale_v, (int)T.cell);
+            }
             default: return mk(0.0f, 0.0f, 0.0f);
         }
     }

@@ -352,6 +352,21 @@ class Scene:
     def blend_texture(self, a, b, t=(0.5, 0.5, 0.5)):
         return self._add_texture(abi.TEX_BLEND, a=a, b=b, t=t)
 
+    def marble_texture(self, a=(0.8, 0.8, 0.8), b=(0.4, 0.2, 0.1), c=(0.06, 0.04, 0.02), scale=5.0, stripes=0.15,
+                       octaves=8):
+        """MarbleTexture(colorA, colorB, colorC, scale, stripes, octaves)
+        (M/scene/texture/MarbleTexture.java:109-117; defaults as its
+        no-argument constructor): the record's frequency is PI * stripes in
+        float."""
+        frequency = np.float32(np.float32(np.pi) * np.float32(stripes))
+        return self._add_texture(abi.TEX_MARBLE, color=a, origin=b, t=c, scale=(float(frequency), scale),
+                                 cell=float(octaves))
+
+    def simplex_fbm_texture(self, color=(0.75, 0.5, 0.75), frequency=5.0, gain=0.5, octaves=16):
+        """SimplexFractionalBrownianMotionTexture(color, frequency, gain,
+        octaves) (defaults as its no-argument constructor)."""
+        return self._add_texture(abi.TEX_SIMPLEX_FBM, color=color, scale=(frequency, gain), cell=float(octaves))
+
     def image_texture(self, image, angle_deg=0.0, scale=(1.0, 1.0)):
         """LDRImageTexture: `image` (H, W, 3) linear RGB in [0, 1], row 0 first."""
         im = np.ascontiguousarray(image, np.float32)

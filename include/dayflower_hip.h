@@ -162,7 +162,8 @@ typedef struct dfr_material {
 } dfr_material;
 
 /* Textures: M/scene/texture/{Constant,Checkerboard,UV,SurfaceNormal,
- * DotProduct,Bullseye,PolkaDot,Blend,LDRImage}Texture.java.
+ * DotProduct,Bullseye,PolkaDot,Blend,LDRImage,Marble,
+ * SimplexFractionalBrownianMotion}Texture.java.
  *   CONSTANT        color
  *   CHECKERBOARD    a, b (texture indices), angle (degrees), scale (u, v)
  *   UV              (u, v, 0)
@@ -172,7 +173,14 @@ typedef struct dfr_material {
  *   POLKA_DOT       a, b, angle (degrees), cell (resolution), radius
  *   BLEND           a, b (not blends themselves), t (per component)
  *   IMAGE           image (index into the scene's images), angle (degrees),
- *                   scale (u, v); bilinear with wrap-around */
+ *                   scale (u, v); bilinear with wrap-around
+ *   MARBLE          colours A = color, B = origin, C = t; scale[0] the
+ *                   frequency, scale[1] the scale, cell the octaves (an
+ *                   integer, 0..64) of the Perlin turbulence at the
+ *                   world-space hit point
+ *   SIMPLEX_FBM     color times simplex-noise fBm in [0, 1] at the
+ *                   world-space hit point; scale[0] the frequency, scale[1]
+ *                   the gain, cell the octaves (an integer, 0..64) */
 #define DFR_TEX_CONSTANT       0
 #define DFR_TEX_CHECKERBOARD   1
 #define DFR_TEX_UV             2
@@ -182,6 +190,8 @@ typedef struct dfr_material {
 #define DFR_TEX_POLKA_DOT      6
 #define DFR_TEX_BLEND          7
 #define DFR_TEX_IMAGE          8
+#define DFR_TEX_MARBLE         9
+#define DFR_TEX_SIMPLEX_FBM   10
 typedef struct dfr_texture {
     int32_t type;
     int32_t a, b;
@@ -419,8 +429,8 @@ typedef struct dfr_compiled_scene {
  * dfr_update_transforms would.  What the kernels cannot follow is refused
  * with DFR_E_UNSUPPORTED and a message naming it: Cone3F, Cylinder3F, Disk3F,
  * Hyperboloid3F, Paraboloid3F, Polygon3F, RectangularCuboid3F and Torus3F
- * primitives; the Bullseye, Checkerboard and PolkaDot materials; Marble and
- * simplex fBm textures; any modifier other than NoOpModifier; a texture on a
+ * primitives; the Bullseye, Checkerboard and PolkaDot materials;
+ * FunctionTexture; any modifier other than NoOpModifier; a texture on a
  * parameter other than the colour slots of dfr_material (a non-constant
  * texture elsewhere); an area light on a shape other than a Triangle3F or a
  * Sphere3F, or one no primitive carries.  Malformed arrays (an offset outside

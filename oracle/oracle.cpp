@@ -1585,6 +1585,115 @@ C3 imgLookup(const Scene::Img& im, float x, float y) {  // LDRImageTexture.doGet
 }
 
 float texRadians(float deg) { return (float)(wrapDegrees(deg) * (3.14159265358979323846 / 180.0)); }
+
+// Noise (M/noise/PerlinNoiseF.java, M/noise/SimplexNoiseF.java) over the
+// lattice tables of dayflower_amd/csrc/noise_tables.inc (data shared with the
+// kernels; the steps are written here again).
+#include "../dayflower_amd/csrc/noise_tables.inc"
+const int kNoisePerm[512] = {NOISE_PERM_256, NOISE_PERM_256};
+const float kNoiseGrad3[36] = {NOISE_GRAD3_36};
+
+// PerlinNoiseF.noiseXYZ (:113-200): improved noise; the gradient of hash h is
+// +-u +-v with u = x for h < 8, 12 and 13 (else y) and v = y for h < 4, 12
+// and 13 (else z), signs from bits 0 and 1.
+float perlinGradient(int h, float x, float y, float z) {
+    const bool uIsX = h < 8 || h == 12 || h == 13;
+    const bool vIsY = h < 4 || h == 12 || h == 13;
+    float u = uIsX ? x : y, v = vIsY ? y : z;
+    if (h & 1) u = -u;
+    if (h & 2) v = -v;
+    return u + v;
+}
+float perlinNoise(float x, float y, float z) {
+    if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z)) return 0.0f;
+    const float fl[3] = {(float)std::floor((double)x), (float)std::floor((double)y), (float)std::floor((double)z)};
+    const int c[3] = {(int)fl[0] & 255, (int)fl[1] & 255, (int)fl[2] & 255};
+    const float f[3] = {x - fl[0], y - fl[1], z - fl[2]};
+    float fade[3];
+    for (int k = 0; k < 3; k++) fade[k] = f[k] * f[k] * f[k] * (f[k] * (f[k] * 6.0f - 15.0f) + 10.0f);
+    const int* P = kNoisePerm;
+    // corner hashes: bit 0 of the corner index steps x, bit 1 y, bit 2 z
+    const int A = P[c[0]] + c[1], B = P[c[0] + 1] + c[1];
+    const int AA = P[A] + c[2], AB = P[A + 1] + c[2], BA = P[B] + c[2], BB = P[B + 1] + c[2];
+    const int base[4] = {AA, BA, AB, BB};  // (x, y) = (0,0), (1,0), (0,1), (1,1)
+    float g[8];
+    for (int corner = 0; corner < 8; corner++) {
+        const int dx = corner & 1, dy = (corner >> 1) & 1, dz = (corner >> 2) & 1;
+        const int h = P[base[dx + 2 * dy] + dz] & 15;
+        g[corner] = perlinGradient(h, f[0] - dx, f[1] - dy, f[2] - dz);
+    }
+    auto lerpT = [](float t, float a, float b) { return a + t * (b - a); };
+    const float x00 = lerpT(fade[0], g[0], g[1]), x10 = lerpT(fade[0], g[2], g[3]);
+    const float x01 = lerpT(fade[0], g[4], g[5]), x11 = lerpT(fade[0], g[6], g[7]);
+    return lerpT(fade[2], lerpT(fade[1], x00, x10), lerpT(fade[1], x01, x11));
+}
+// PerlinNoiseF.turbulenceXYZ(x, y, z, octaves) (:231-250)
+float perlinTurbulence(float x, float y, float z, int octaves) {
+    float sum = std::fabs(perlinNoise(x, y, z));
+    float weight = 1.0f;
+    for (int i = 1; i < octaves; i++) {
+        weight *= 2.0f;
+        sum += std::fabs(perlinNoise(x * weight, y * weight, z * weight)) / weight;
+    }
+    return sum;
+}
+// SimplexNoiseF.noiseXYZ (:405-507, doFastFloorToInt :775-779): 3D simplex
+// noise, 32 x the four corners' contributions, clamped to [-1, 1]
+int fastFloor(float v) {
+    const int i = (int)v;
+    return v < (float)i ? i - 1 : i;
+}
+float simplexNoise(float x, float y, float z) {
+    if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z)) return 0.0f;
+    const float F3 = 1.0f / 3.0f, G3 = 1.0f / 6.0f;
+    const float s = (x + y + z) * F3;
+    const int cell[3] = {fastFloor(x + s), fastFloor(y + s), fastFloor(z + s)};
+    const float t = (float)(cell[0] + cell[1] + cell[2]) * G3;
+    const float p0[3] = {x - ((float)cell[0] - t), y - ((float)cell[1] - t), z - ((float)cell[2] - t)};
+    // the simplex's corners after the first: unit steps along the axes in
+    // decreasing order of the offsets
+    int o1[3] = {0, 0, 0}, o2[3] = {0, 0, 0};
+    if (p0[0] >= p0[1]) {
+        if (p0[1] >= p0[2]) { o1[0] = 1; o2[0] = 1; o2[1] = 1; }
+        else if (p0[0] >= p0[2]) { o1[0] = 1; o2[0] = 1; o2[2] = 1; }
+        else { o1[2] = 1; o2[0] = 1; o2[2] = 1; }
+    } else {
+        if (p0[1] < p0[2]) { o1[2] = 1; o2[1] = 1; o2[2] = 1; }
+        else if (p0[0] < p0[2]) { o1[1] = 1; o2[1] = 1; o2[2] = 1; }
+        else { o1[1] = 1; o2[0] = 1; o2[1] = 1; }
+    }
+    float p1[3], p2[3], p3[3];
+    for (int k = 0; k < 3; k++) {
+        p1[k] = p0[k] - (float)o1[k] + G3;
+        p2[k] = p0[k] - (float)o2[k] + 2.0f * G3;
+        p3[k] = p0[k] - 1.0f + 3.0f * G3;
+    }
+    const int* P = kNoisePerm;
+    const int ii = cell[0] & 255, jj = cell[1] & 255, kk = cell[2] & 255;
+    auto gradIndex = [&](int a, int b, int c) { return P[ii + a + P[jj + b + P[kk + c]]] % 12; };
+    auto contribution = [&](int gi, const float* q) {
+        const float r = 0.6f - q[0] * q[0] - q[1] * q[1] - q[2] * q[2];
+        if (r < 0.0f) return 0.0f;
+        const float* gv = &kNoiseGrad3[3 * gi];
+        return (r * r) * (r * r) * (gv[0] * q[0] + gv[1] * q[1] + gv[2] * q[2]);
+    };
+    const float n = contribution(gradIndex(0, 0, 0), p0) + contribution(gradIndex(o1[0], o1[1], o1[2]), p1) +
+                    contribution(gradIndex(o2[0], o2[1], o2[2]), p2) + contribution(gradIndex(1, 1, 1), p3);
+    return std::min(std::max(32.0f * n, -1.0f), 1.0f);
+}
+// SimplexNoiseF.fractionalBrownianMotionXYZ (:246-267) with minimum 0 and
+// maximum 1
+float simplexFbm(float x, float y, float z, float frequency, float gain, int octaves) {
+    float amplitude = 1.0f, amplitudeSum = 0.0f, f = frequency, sum = 0.0f;
+    for (int i = 0; i < octaves; i++) {
+        sum += simplexNoise(x * f, y * f, z * f) * amplitude;
+        amplitudeSum += amplitude;
+        amplitude *= gain;
+        f *= 2.0f;
+    }
+    sum /= amplitudeSum;
+    return sum * (1.0f - 0.0f) / 2.0f + (1.0f + 0.0f) / 2.0f;
+}
 float fractionalPart(float x) { return x - (float)std::floor((double)x); }
 
 C3 texChain(const Scene& sc, int t, const Surface& si) {
@@ -1623,6 +1732,19 @@ C3 texChain(const Scene& sc, int t, const Surface& si) {
                 const float c = fcos(texRadians(T.angle)), sn = fsin(texRadians(T.angle));
                 const float ru = (u * c - v * sn) * T.scale[0], rv = (v * c + u * sn) * T.scale[1];
                 return imgLookup(im, positiveModuloF(ru * im.w - 0.5f, (float)im.w), positiveModuloF(rv * im.h - 0.5f, (float)im.h));
+            }
+            case DFR_TEX_MARBLE: {  // MarbleTexture.java:131-146 at the world-space point
+                const float x = si.p.x * T.scale[0], y = si.p.y * T.scale[0], z = si.p.z * T.scale[0];
+                const float r = T.scale[1] * perlinTurbulence(x, y, z, (int)T.cell);
+                const float s = 2.0f * std::fabs(fsin(x + r));
+                const float tt = s < 1.0f ? s : s - 1.0f;
+                const C3 A(T.color[0], T.color[1], T.color[2]), Bc(T.origin[0], T.origin[1], T.origin[2]);
+                const C3 Cc(T.t[0], T.t[1], T.t[2]);
+                return s < 1.0f ? blend(Cc, Bc, tt) : blend(Bc, A, tt);
+            }
+            case DFR_TEX_SIMPLEX_FBM: {  // SimplexFractionalBrownianMotionTexture.java:127-137
+                const float n = simplexFbm(si.p.x, si.p.y, si.p.z, T.scale[0], T.scale[1], (int)T.cell);
+                return C3(T.color[0] * n, T.color[1] * n, T.color[2] * n);
             }
             default: return C3(0.0f);
         }
@@ -2639,6 +2761,23 @@ int32_t orc_perez_sun(float turbidity, const float* sun, float* theta, float* sp
         if (p.sunObject.z > 0.0f) sunSpectrum(p.theta, turbidity, spectrum);
     }
     if (rgb) { rgb[0] = p.sunColor.r; rgb[1] = p.sunColor.g; rgb[2] = p.sunColor.b; }
+    return 0;
+}
+
+// The texture noise at n points (xyz, 3 floats each): kind 0 Perlin noise, 1
+// Perlin turbulence (octaves), 2 simplex noise, 3 simplex fBm (frequency,
+// gain, octaves).
+int32_t orc_noise(int32_t kind, int64_t n, const float* xyz, float frequency, float gain, int32_t octaves, float* out) {
+    for (int64_t i = 0; i < n; i++) {
+        const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        switch (kind) {
+            case 0: out[i] = perlinNoise(x, y, z); break;
+            case 1: out[i] = perlinTurbulence(x, y, z, octaves); break;
+            case 2: out[i] = simplexNoise(x, y, z); break;
+            case 3: out[i] = simplexFbm(x, y, z, frequency, gain, octaves); break;
+            default: return -1;
+        }
+    }
     return 0;
 }
 

@@ -155,6 +155,11 @@ int32_t orc_perez_sample_many(float turbidity, const float* sun, int64_t n, cons
  * floats; orc_image_rgba8 turns n linear RGB means into RGBA8 as the
  * renderer does for `algorithm` (a DFR_ALGO_* ordinal). */
 int32_t orc_tone_map(int32_t tone_mapper, int64_t n, const float* in, float* out);
+
+/* The texture noise at n points (3 floats each): kind 0 Perlin noise, 1
+ * Perlin turbulence over `octaves`, 2 simplex noise, 3 simplex fBm in [0, 1]
+ * (frequency, gain, octaves). */
+int32_t orc_noise(int32_t kind, int64_t n, const float* xyz, float frequency, float gain, int32_t octaves, float* out);
 int32_t orc_image_rgba8(int32_t algorithm, int32_t tone_mapper, int64_t n, const float* rgb, uint8_t* rgba);
 
 #ifdef __cplusplus

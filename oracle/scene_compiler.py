@@ -76,6 +76,16 @@ def _face_normal(a, b, c):
     return n / np.float32(np.linalg.norm(n))
 
 
+def _argb(rgb):
+    """Color3F.pack(): 0xFFRRGGBB as int bits in a float (exact: alpha 255
+    keeps the value within 2^24 of zero); components rounded from x * 255,
+    so colours that are multiples of 1/255 pack exactly whatever the
+    reference's rounding rule (Ints.saturateAndScaleToInt, absent)."""
+    q = [int(np.clip(np.round(np.float32(x) * 255.0), 0, 255)) for x in rgb]
+    v = (255 << 24) | (q[0] << 16) | (q[1] << 8) | q[2]
+    return float(v - (1 << 32) if v >= (1 << 31) else v)
+
+
 class _Distinct:
     """Distinct records of one type in first-use order (the NodeCache's
     distinct lists) with their offsets."""
@@ -110,7 +120,7 @@ class SceneCompiler:
         if self.m is not None:
             self.m = np.asarray(self.m, np.float64).reshape(n, 4, 4)
         self.const = _Distinct()
-        self.tex = {k: _Distinct() for k in ("blend", "bullseye", "checkerboard", "polka_dot")}
+        self.tex = {k: _Distinct() for k in ("blend", "bullseye", "checkerboard", "polka_dot", "marble", "simplex_fbm")}
         self.ldr = _Distinct()
         self.mats = {k: _Distinct(length=True) for k in ("clear_coat", "disney", "glass", "glossy", "matte", "metal",
                                                          "mirror", "plastic", "substrate", "translucent", "uber")}
@@ -151,6 +161,24 @@ class SceneCompiler:
                 rec = [rad, T.scale[0], T.scale[1], float(w), float(h)] + [float(v) for v in packed]
                 return rec + [0.0] * padding(len(rec))
             return ID["LDRImageTexture"], self.ldr.add(("img", t), make)
+        if T.type == abi.TEX_MARBLE:
+            # CompiledTextureCache.toArray(MarbleTexture): colours as Color3F.pack()
+            rec = [0.0] * C["MARBLE_TEXTURE_LENGTH"]
+            rec[C["MARBLE_TEXTURE_OFFSET_COLOR_A"]] = _argb(T.color)
+            rec[C["MARBLE_TEXTURE_OFFSET_COLOR_B"]] = _argb(T.origin)
+            rec[C["MARBLE_TEXTURE_OFFSET_COLOR_C"]] = _argb(T.t)
+            rec[C["MARBLE_TEXTURE_OFFSET_FREQUENCY"]] = T.scale[0]
+            rec[C["MARBLE_TEXTURE_OFFSET_SCALE"]] = T.scale[1]
+            rec[C["MARBLE_TEXTURE_OFFSET_STRIPES"]] = float(np.float32(T.scale[0]) / np.float32(math.pi))
+            rec[C["MARBLE_TEXTURE_OFFSET_OCTAVES"]] = T.cell
+            return ID["MarbleTexture"], self.tex["marble"].add(("mb", t), lambda: rec)
+        if T.type == abi.TEX_SIMPLEX_FBM:
+            rec = [0.0] * C["SIMPLEX_FRACTIONAL_BROWNIAN_MOTION_TEXTURE_LENGTH"]
+            rec[C["SIMPLEX_FRACTIONAL_BROWNIAN_MOTION_TEXTURE_OFFSET_COLOR"]] = _argb(T.color)
+            rec[C["SIMPLEX_FRACTIONAL_BROWNIAN_MOTION_TEXTURE_OFFSET_FREQUENCY"]] = T.scale[0]
+            rec[C["SIMPLEX_FRACTIONAL_BROWNIAN_MOTION_TEXTURE_OFFSET_GAIN"]] = T.scale[1]
+            rec[C["SIMPLEX_FRACTIONAL_BROWNIAN_MOTION_TEXTURE_OFFSET_OCTAVES"]] = T.cell
+            return ID["SimplexFractionalBrownianMotionTexture"], self.tex["simplex_fbm"].add(("fbm", t), lambda: rec)
         a = pack16(*self.texture(T.a))
         b = pack16(*self.texture(T.b))
         if T.type == abi.TEX_CHECKERBOARD:
@@ -421,6 +449,8 @@ class SceneCompiler:
             "blend_textures": self.tex["blend"].array(np.float32),
             "bullseye_textures": self.tex["bullseye"].array(np.float32),
             "polka_dot_textures": self.tex["polka_dot"].array(np.float32),
+            "marble_textures": self.tex["marble"].array(np.float32),
+            "simplex_fbm_textures": self.tex["simplex_fbm"].array(np.float32),
             "ldr_image_textures": self.ldr.array(np.float32),
             "ldr_image_texture_offsets": np.asarray(
                 np.concatenate([[0], np.cumsum([len(r) for r in self.ldr.records])[:-1]]) if self.ldr.records else [],

@@ -82,6 +82,7 @@ def load():
     lib.orc_perez_pdf_many.argtypes = [f, vp, i64, vp, vp]
     lib.orc_perez_sample_many.argtypes = [f, vp, i64, vp, vp, vp]
     lib.orc_tone_map.argtypes = [i32, i64, vp, vp]
+    lib.orc_noise.argtypes = [i32, i64, vp, f, f, i32, vp]
     lib.orc_set_material_u.argtypes = [f]
     lib.orc_set_material_u.restype = None
     lib.orc_image_rgba8.argtypes = [i32, i32, i64, vp, vp]
@@ -221,6 +222,17 @@ def film_agreement(g, n_g, s, q, c):
     z = np.abs(np.asarray(g, np.float64) - mean) / np.maximum(se, 1e-12)
     rrmse = float(np.sqrt(np.mean((g - mean) ** 2)) / np.sqrt(np.mean(mean ** 2)))
     return float((z <= 3.0).mean()), float((z <= 3.0).all(-1).mean()), rrmse
+
+
+def noise(kind, xyz, frequency=1.0, gain=0.5, octaves=1):
+    """The oracle's texture noise at points (n, 3): kind "perlin",
+    "turbulence", "simplex" or "fbm"."""
+    k = {"perlin": 0, "turbulence": 1, "simplex": 2, "fbm": 3}[kind]
+    p = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    out = np.empty(p.shape[0], np.float32)
+    rc = load().orc_noise(k, p.shape[0], _ptr(p), float(frequency), float(gain), int(octaves), _ptr(out))
+    assert rc == 0
+    return out
 
 
 def tone_map(tone_mapper, values):

@@ -369,13 +369,14 @@ def test_shapes_materials_textures_the_kernels_cannot_follow_are_refused():
     mm[0] = SC.pack8(I["ConstantTexture"], 0, I["NormalMapLDRImageModifier"], 0)
     arr["matte_materials"] = mm
     _refused(arr, R.UnsupportedOperationException, "modifier")
-    # a marble texture as a matte colour, a non-constant texture as a glass eta
-    for name, text in (("MarbleTexture", "MarbleTexture"), ("SimplexFractionalBrownianMotionTexture", "fBm|Fractional")):
+    # a marble or fBm texture record past the end of its (empty) array
+    for name, text in (("MarbleTexture", "marble_textures"), ("SimplexFractionalBrownianMotionTexture", "simplex_fbm")):
         arr = dict(base)
         mm = arr["matte_materials"].copy()
         mm[1] = SC.pack8(I["ConstantTexture"], 0, I[name], 0)
         arr["matte_materials"] = mm
-        _refused(arr, R.UnsupportedOperationException, text)
+        _refused(arr, R.IllegalArgumentException, text)
+    # a non-constant texture as a glass eta
     arr = dict(base)
     gm = arr["glass_materials"].copy()
     gm[1] = SC.pack8(I["UVTexture"], 0, 0, 0)
