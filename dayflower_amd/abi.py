@@ -49,7 +49,7 @@ BATCH_PLAIN, BATCH_PACKED = 0, 1
 
 class Material(ctypes.Structure):
     _fields_ = [
-        ("type", c_int32), ("remap_roughness", c_int32), ("thin", c_int32), ("reserved0", c_int32),
+        ("type", c_int32), ("remap_roughness", c_int32), ("thin", c_int32), ("emission_texture", c_int32),
         ("emission", c_float * 3), ("kd", c_float * 3), ("ks", c_float * 3), ("kr", c_float * 3),
         ("kt", c_float * 3), ("eta", c_float * 3), ("k", c_float * 3),
         ("angle", c_float), ("roughness_u", c_float), ("roughness_v", c_float),

@@ -41,6 +41,7 @@ void compile_material(const dfr_material& m, DevMaterial& out, std::vector<DevLo
     // the lobes present at a hit follow from the textures there
     const bool textured = m.textures[0] || m.textures[1] || m.textures[2] || m.textures[3];
     for (int k = 0; k < 4; k++) out.tex[k] = m.textures[k];
+    out.tex_emission = m.emission_texture;
     auto add = [&](const DevLobe& l) { lobes.push_back(l); out.n_lobes++; };
     auto tr = [](DevLobe& l, float ax, float ay, bool sep) {
         l.alphaX = std::max(ax, 0.001f); l.alphaY = std::max(ay, 0.001f); l.separable = sep ? 1 : 0;
@@ -904,7 +905,7 @@ int compile_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledSce
     // the extended materials and every textured material run in the EXT kernels
     for (const dfr_material& m : in.mats)
         out.ext_features = out.ext_features || m.type >= DFR_MAT_GLOSSY || m.textures[0] || m.textures[1] ||
-                           m.textures[2] || m.textures[3];
+                           m.textures[2] || m.textures[3] || m.emission_texture;
     // traversal addresses nodes and triangles with 32-bit byte offsets
     const size_t node_bytes = BVH_WIDTH == 4 ? bvh.nodes4.size() * sizeof(bvh.nodes4[0]) : bvh.nodes.size() * sizeof(bvh.nodes[0]);
     if (node_bytes >= (size_t(1) << 32) || bk.tris.size() * sizeof(float4) >= (size_t(1) << 32)) {
@@ -1108,6 +1109,9 @@ int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err) {
     }
     for (size_t m = 0; m < h.mats.size(); m++) {
         const dfr_material& M = h.mats[m];
+        if (M.emission_texture < 0 || M.emission_texture > nt) {
+            err = "material " + std::to_string(m) + ": emission texture out of range"; return DFR_E_ARGUMENT;
+        }
         for (int k = 0; k < 4; k++) {
             if (!M.textures[k]) continue;
             if (M.textures[k] < 0 || M.textures[k] > nt) { err = "material " + std::to_string(m) + ": texture out of range"; return DFR_E_ARGUMENT; }

@@ -369,7 +369,10 @@ private:
         int tid, toff;
         auto emission_and_modifier = [&](int32_t word) {
             pair(word, 0, tid, toff);
-            value_color(m.emission, tid, toff, "emission");
+            // textureEmission: a constant fills the colour, anything else is
+            // the material's emission texture
+            if (tid == TEX_CONSTANT) value_color(m.emission, tid, toff, "emission");
+            else { m.emission_texture = texture(tid, toff) + 1; m.emission[0] = m.emission[1] = m.emission[2] = 0.0f; }
             int mid, moff;
             pair(word, 1, mid, moff);
             if (mid != MODIFIER_NO_OP && mid != 0)

@@ -1921,14 +1921,21 @@ DEV float light_pdf(const DevScene& sc, const DevLight& L, const Surf& si, V wi)
     return pdf;
 }
 
-// Intersection.evaluateRadianceEmitted (M/scene/Intersection.java:89-101)
+// Intersection.evaluateRadianceEmitted (M/scene/Intersection.java:89-101);
+// a textured emission (Material.emittance reads textureEmission) only in
+// the EXT instantiations, where texture code lives
+template <bool EXT = false>
 DEV V emitted(const DevScene& sc, const Surf& si, V dir) {
     const DevPrim& p = sc.prims[si.prim];
     if (p.area_light >= 0) {
         const DevLight& L = sc.lights[p.area_light];
         return (L.two_sided || dot(si.s.w, dir) > 0.0f) ? ld3(L.radiance) : mk(0, 0, 0);
     }
-    return ld3(sc.mats[p.material].emission);
+    const DevMaterial& m = sc.mats[p.material];
+    if constexpr (EXT) {
+        if (m.tex_emission) return texture_color(sc, m.tex_emission - 1, si, -dir);
+    }
+    return ld3(m.emission);
 }
 
 // Light.evaluateRadianceEmitted(ray): what a ray that leaves the scene sees

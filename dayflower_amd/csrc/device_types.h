@@ -49,7 +49,7 @@ struct DevLobe {           // 128 bytes
 };
 static_assert(sizeof(DevLobe) == 128, "lobe record size");
 
-struct DevMaterial {       // 48 bytes
+struct DevMaterial {       // 64 bytes
     int32_t first_lobe;
     int32_t n_lobes;       // 0 with has_bsdf = 0: pass-through surface
     int32_t has_bsdf;
@@ -60,6 +60,8 @@ struct DevMaterial {       // 48 bytes
     // material keeps its full lobe list; which lobes exist at a hit, and
     // their colours, follow from the textures there.
     int32_t tex[4];
+    int32_t tex_emission;  // texture index + 1 replacing `emission`, or 0
+    int32_t pad[3];
 };
 
 // A texture (M/scene/texture/*Texture.java); angles as cos/sin.

@@ -226,7 +226,7 @@ DEV V radiance_pt(const DevScene& sc, V o, V d, const Rng& rng, int max_bounce, 
         if (trace<false, SHORT>(sc, o, d, T_MIN, FLT_MAX, h, st)) {
             Surf si = make_surface(sc, o, d, h);
             V wo = -d;
-            if (bounce == 0 || spec) L = L + beta * emitted(sc, si, wo);
+            if (bounce == 0 || spec) L = L + beta * emitted<EXT>(sc, si, wo);
             if (bounce >= max_bounce) break;
             const DevMaterial& m = sc.mats[sc.prims[h.prim].material];
             TexHit th;
@@ -1126,7 +1126,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_shade(const DevScene* __restrict__
                 HitRec h; h.t = hv.x; h.prim = prim; h.tri = __float_as_int(hv.z);
                 Surf si = make_surface(sc, o, d, h);
                 V wo = -d;
-                if (bounce == 0 || spec) L = L + beta * emitted(sc, si, wo);
+                if (bounce == 0 || spec) L = L + beta * emitted<EXT>(sc, si, wo);
                 if (bounce < job.max_bounce) {
                     const DevMaterial& m = sc.mats[sc.prims[prim].material];
                     TexHit th;

@@ -375,6 +375,10 @@ class Scene:
         self.images.append(im)
         return self._add_texture(abi.TEX_IMAGE, image=len(self.images) - 1, angle=float(angle_deg), scale=scale)
 
+    def set_emission_texture(self, material, texture):
+        """The material's emission from `texture` (textureEmission)."""
+        self.materials[material].emission_texture = texture + 1
+
     def set_texture(self, material, slot, texture):
         """Replaces a colour slot (abi.SLOT_KD, KS, KR, KT) of a material by a texture."""
         self.materials[material].textures[slot] = texture + 1

@@ -127,7 +127,8 @@ extern "C" {
  * textures[k] (k = DFR_SLOT_KD, KS, KR, KT) replaces that colour by a texture
  * of the scene (index + 1; 0 keeps the constant): MATTE kd; PLASTIC kd, ks;
  * MIRROR kr; GLASS kr, kt.  A texture on another material or slot is
- * refused with DFR_E_UNSUPPORTED. */
+ * refused with DFR_E_UNSUPPORTED.  emission_texture textures the emission
+ * of every material type (Material.emittance reads textureEmission). */
 #define DFR_SLOT_KD 0
 #define DFR_SLOT_KS 1
 #define DFR_SLOT_KR 2
@@ -136,7 +137,7 @@ typedef struct dfr_material {
     int32_t type;
     int32_t remap_roughness;
     int32_t thin;
-    int32_t reserved0;
+    int32_t emission_texture;   /* any material: a texture replacing emission (index + 1; 0: the constant) */
     float emission[3];
     float kd[3];
     float ks[3];
@@ -431,8 +432,8 @@ typedef struct dfr_compiled_scene {
  * Hyperboloid3F, Paraboloid3F, Polygon3F, RectangularCuboid3F and Torus3F
  * primitives; the Bullseye, Checkerboard and PolkaDot materials;
  * FunctionTexture; any modifier other than NoOpModifier; a texture on a
- * parameter other than the colour slots of dfr_material (a non-constant
- * texture elsewhere); an area light on a shape other than a Triangle3F or a
+ * parameter other than the colour slots and the emission of dfr_material
+ * (a non-constant texture elsewhere); an area light on a shape other than a Triangle3F or a
  * Sphere3F, or one no primitive carries.  Malformed arrays (an offset outside
  * its array, an unknown type ID) give DFR_E_ARGUMENT. */
 int32_t dfr_set_compiled_scene(dfr_ctx* ctx, const dfr_compiled_scene* scene);

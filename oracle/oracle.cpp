@@ -1448,14 +1448,20 @@ float lightPdf(const Scene& sc, const Light& L, const Surface& si, V3 wi, int pd
     return pdf;
 }
 
-// Intersection.evaluateRadianceEmitted (M/scene/Intersection.java:89-101)
+C3 texColor(const Scene& sc, int t, const Surface& si);
+
+// Intersection.evaluateRadianceEmitted (M/scene/Intersection.java:89-101);
+// without an area light the material's emittance, its emission texture's
+// colour at the hit when it has one
 C3 emitted(const Scene& sc, const Surface& si, V3 dir) {
     const Prim& p = sc.prims[si.prim];
     if (p.areaLight >= 0) {
         const Light& L = sc.lights[p.areaLight];
         return (L.twoSided || dot(si.s.w, dir) > 0.0f) ? L.radiance : C3(0.0f);
     }
-    return col(sc.mats[p.material].emission);
+    const dfr_material& m = sc.mats[p.material];
+    if (m.emission_texture > 0) return texColor(sc, m.emission_texture - 1, si);
+    return col(m.emission);
 }
 
 // Light.evaluateRadianceEmitted(ray): the Perez sky and image lights emit on a miss.

@@ -201,7 +201,8 @@ class SceneCompiler:
     # -- materials ----------------------------------------------------------
     def material(self, i):
         M = self.sc.materials[i]
-        em = pack8(*self.constant(tuple(M.emission)), ID["NoOpModifier"], 0)
+        emission = self.texture(M.emission_texture - 1) if M.emission_texture else self.constant(tuple(M.emission))
+        em = pack8(*emission, ID["NoOpModifier"], 0)
         t = M.type
         if t == abi.MAT_MATTE:
             w = [em, pack8(*self.value(M.angle), *self.slot(M, abi.SLOT_KD, M.kd))]

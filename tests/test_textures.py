@@ -318,3 +318,82 @@ def test_textures_the_kernels_cannot_follow_are_refused():
         with pytest.raises(exc):
             r.setScene(s)
     r.dispose()
+
+
+def _glowing_quad(res, kind):
+    """A black Matte quad whose emission is a texture (textureEmission):
+    UV colours or a checkerboard of two constants."""
+    def geo(sc):
+        m = sc.add_material(S.matte((0.0, 0.0, 0.0)))
+        if kind == "uv":
+            t = sc.uv_texture()
+        else:
+            t = sc.checkerboard_texture(sc.constant_texture((3.0, 0.5, 0.25)), sc.constant_texture((0.2, 1.5, 0.0)),
+                                        scale=(4.0, 4.0))
+        sc.set_emission_texture(m, t)
+        _quad(sc, m)
+    return _lit_scene(res, geo)
+
+
+@pytest.mark.parametrize("kind", ["uv", "checkerboard"])
+def test_emission_texture_is_what_a_camera_ray_sees(kind):
+    """With no bounce the image is the emission at each primary hit: the quad's
+    UV coordinates ((x + 1) / 2, (y + 1) / 2, 0) or the checkerboard's two
+    colours by cell."""
+    from tests.oracle_lib import OracleScene
+    res, seed = 48, 3
+    sc = _glowing_quad(res, kind)
+    o = OracleScene(sc)
+    s, _, c = o.render(res, res, 1, seed=seed, max_bounce=0)
+    rays = o.primary_rays(res, res, seed, 0)
+    prim, _, t = o.trace_f32(rays)
+    o.close()
+    img = (s / np.maximum(c, 1)[..., None]).reshape(-1, 3)
+    hit = prim >= 0
+    p = (rays[:, 0:3] + rays[:, 4:7] * t[:, None])[hit]
+    u, v = (p[:, 0] + 1) / 2, (p[:, 1] + 1) / 2
+    if kind == "uv":
+        want = np.stack([u, v, np.zeros_like(u)], 1)
+        err = np.abs(img[hit] - want).max()
+        assert err < 2e-6, err
+    else:
+        cu, cv = np.modf(u * 4)[0] > 0.5, np.modf(v * 4)[0] > 0.5
+        away = (np.abs(np.modf(u * 4)[0] - 0.5) > 0.01) & (np.abs(np.modf(v * 4)[0] - 0.5) > 0.01) & \
+               (np.modf(u * 4)[0] > 0.01) & (np.modf(v * 4)[0] > 0.01)
+        want = np.where((cu ^ cv)[:, None], [3.0, 0.5, 0.25], [0.2, 1.5, 0.0])
+        assert np.array_equal(img[hit][away], want[away].astype(np.float32))
+    assert np.all(img[~hit] == 0.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_emission_texture_on_the_gpu(kernel):
+    """The glowing quads on both kernels against the oracle per pixel, and
+    the same scene through the reference's compiled arrays."""
+    from dayflower_amd import renderer as R
+    from oracle import scene_compiler as SC
+    from tests.oracle_lib import OracleScene
+    res, seed = 48, 3
+    for kind in ("uv", "checkerboard"):
+        sc = _glowing_quad(res, kind)
+        o = OracleScene(sc)
+        s, _, c = o.render(res, res, 4, seed=seed, max_bounce=2)
+        o.close()
+        ref = s / np.maximum(c, 1)[..., None]
+        for compiled in (False, True):
+            r = R.GPURenderer()
+            if compiled:
+                r.setCompiledScene(SC.compatible(SC.compile_scene(sc)))
+            else:
+                r.setScene(sc)
+            r.setSeed(seed)
+            r.setMaximumBounce(2)
+            r.setKernel(kernel)
+            r.setup()
+            r.render(4)
+            g, k = r.film(res, res)
+            r.dispose()
+            np.testing.assert_array_equal(k, c)
+            eq = np.isclose(g, ref, rtol=1e-4, atol=1e-5).all(-1).mean()
+            print(f"emission {kind} kernel {kernel} compiled {compiled}: pixels equal {eq:.4f}", flush=True)
+            assert eq >= 0.999
