@@ -586,62 +586,69 @@ DEV V xf34_vec(const float* m, V v) {
 
 // Closest (ANY false) or any hit among the instances of shared meshes: an
 // ordered walk of the top-level tree over their world boxes; at an instance
-// the ray is carried into its mesh's object space, where its direction keeps
-// its scale so that distances along it are the world ray's, and the mesh's
-// tree is walked on the same stack above the top level's pending entries.
-// The world ray's traversal form is rebuilt at each top-level step rather
-// than kept beside the object-space one (fewer live registers in every
-// kernel that traces).  Updates tbest, prim and tri on a nearer hit.
+// whose own box the segment meets, the ray is carried into its mesh's object
+// space, where its direction keeps its scale so that distances along it are
+// the world ray's, and the mesh's tree is walked on the same stack above the
+// top level's pending entries (the rest of a top-level leaf waits there as a
+// leaf reference stepped to its next instance).  One loop serves both levels,
+// so a wave's node steps share one code path whichever tree each lane is in.
+// Updates tbest, prim and tri on a nearer hit.
 template <int SHORT, bool ANY>
 DEV bool trace_instances(const DevScene& s, V o, V d, float tmin, float& tbest, int& prim, int& tri,
                          TraversalStack<SHORT>& st) {
     bool found = false;
+    PRay r = prep_ray(o, d);
+    PlaneSel ps = plane_sel(r);
+    int inst_prim = -1;  // the instance whose mesh the walk is in (-1: the top level)
+    int base = 0;        // stack entries below the mesh's
     uint32_t cur = 0;
     for (;;) {
-        if (cur & LEAF_FLAG_DEV) {
-            const uint32_t first = (cur & 0x7fffffffu) >> 3, count = (cur & 7u) + 1u;
-            const PRay rw = prep_ray(o, d);
-            for (uint32_t k = 0; k < count; k++) {
-                const DevInstance& in = s.instances[first + k];
-                // the instance's own world box against the current segment
-                // (widened as the node test widens its exit distance)
-                const float ax = in.lo[0] * rw.inv.x - rw.oinv.x, bx = in.hi[0] * rw.inv.x - rw.oinv.x;
-                const float ay = in.lo[1] * rw.inv.y - rw.oinv.y, by = in.hi[1] * rw.inv.y - rw.oinv.y;
-                const float az = in.lo[2] * rw.inv.z - rw.oinv.z, bz = in.hi[2] * rw.inv.z - rw.oinv.z;
-                const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
-                const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tbest));
-                if (!(tn <= tf * 1.00000048f)) continue;
-                const PRay ro = prep_ray(xf34_point(in.w2o, o), xf34_vec(in.w2o, d));
-                const PlaneSel pso = plane_sel(ro);
-                const int base = st.sp;
-                uint32_t c2 = (uint32_t)in.root;
-                for (;;) {
-                    if (c2 & LEAF_FLAG_DEV) {
-                        const uint32_t f2 = (c2 & 0x7fffffffu) >> 3, n2 = (c2 & 7u) + 1u;
-                        for (uint32_t j = 0; j < n2; j++) {
-                            float4 a, b, c;
-                            load_tri(s.inst_tris, f2 + j, a, b, c);
-                            const float t = tri_t(f4v(a), f4v(b), f4v(c), ro.o, ro.d, ro.kx, ro.ky, ro.kz, ro.sx, ro.sy,
-                                                  ro.sz, tmin, tbest);
-                            if (t != 0.0f && t < tbest) {
-                                tbest = t; prim = in.prim; tri = __float_as_int(a.w); found = true;
-                                if (ANY) { st.sp = 0; return true; }
-                            }
-                        }
-                        if (st.sp == base) break;
-                        c2 = (uint32_t)st.pop();
-                    } else {
-                        c2 = visit_node<SHORT, !ANY>(s.inst_nodes, c2, ro, pso, tmin, tbest, st, base);
-                        if (c2 == TRAVERSAL_DONE) break;
-                    }
+        if (!(cur & LEAF_FLAG_DEV)) {
+            cur = visit_node<SHORT, !ANY>(s.inst_nodes, cur, r, ps, tmin, tbest, st, base);
+        } else if (inst_prim >= 0) {
+            // a leaf of the mesh: its triangles, in object space
+            const uint32_t f2 = (cur & 0x7fffffffu) >> 3, n2 = (cur & 7u) + 1u;
+            for (uint32_t j = 0; j < n2; j++) {
+                float4 a, b, c;
+                load_tri(s.inst_tris, f2 + j, a, b, c);
+                const float t = tri_t(f4v(a), f4v(b), f4v(c), r.o, r.d, r.kx, r.ky, r.kz, r.sx, r.sy, r.sz, tmin, tbest);
+                if (t != 0.0f && t < tbest) {
+                    tbest = t; prim = inst_prim; tri = __float_as_int(a.w); found = true;
+                    if (ANY) { st.sp = 0; return true; }
                 }
             }
+            cur = st.sp == base ? TRAVERSAL_DONE : (uint32_t)st.pop();
+        } else {
+            // a top-level leaf: its first instance; the others wait
+            const uint32_t first = (cur & 0x7fffffffu) >> 3;
+            if (cur & 7u) st.push((int)(cur + 7u));
+            const DevInstance& in = s.instances[first];
+            // the instance's own world box against the current segment
+            // (widened as the node test widens its exit distance)
+            const float ax = in.lo[0] * r.inv.x - r.oinv.x, bx = in.hi[0] * r.inv.x - r.oinv.x;
+            const float ay = in.lo[1] * r.inv.y - r.oinv.y, by = in.hi[1] * r.inv.y - r.oinv.y;
+            const float az = in.lo[2] * r.inv.z - r.oinv.z, bz = in.hi[2] * r.inv.z - r.oinv.z;
+            const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
+            const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tbest));
+            if (tn <= tf * 1.00000048f) {
+                inst_prim = in.prim;
+                base = st.sp;
+                r = prep_ray(xf34_point(in.w2o, o), xf34_vec(in.w2o, d));
+                ps = plane_sel(r);
+                cur = (uint32_t)in.root;
+            } else {
+                cur = st.sp == 0 ? TRAVERSAL_DONE : (uint32_t)st.pop();
+            }
+        }
+        if (cur == TRAVERSAL_DONE) {
+            if (inst_prim < 0) break;
+            // the mesh is done: back to the top level's pending entries
+            inst_prim = -1;
+            base = 0;
+            r = prep_ray(o, d);
+            ps = plane_sel(r);
             if (st.sp == 0) break;
             cur = (uint32_t)st.pop();
-        } else {
-            const PRay rw = prep_ray(o, d);
-            cur = visit_node<SHORT, !ANY>(s.inst_nodes, cur, rw, plane_sel(rw), tmin, tbest, st);
-            if (cur == TRAVERSAL_DONE) break;
         }
     }
     return found;
