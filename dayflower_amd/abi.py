@@ -57,11 +57,11 @@ class Material(ctypes.Structure):
         ("disney_anisotropic", c_float), ("disney_sheen", c_float), ("disney_sheen_tint", c_float),
         ("disney_clearcoat", c_float), ("disney_clearcoat_gloss", c_float), ("disney_spec_trans", c_float),
         ("disney_diff_trans", c_float), ("disney_flatness", c_float),
-        ("textures", c_int32 * 4),
+        ("textures", c_int32 * 8),
     ]
 
 
-SLOT_KD, SLOT_KS, SLOT_KR, SLOT_KT = range(4)
+SLOT_KD, SLOT_KS, SLOT_KR, SLOT_KT, SLOT_ROUGHNESS_U, SLOT_ROUGHNESS_V, SLOT_ETA, SLOT_K = range(8)
 (TEX_CONSTANT, TEX_CHECKERBOARD, TEX_UV, TEX_SURFACE_NORMAL, TEX_DOT_PRODUCT, TEX_BULLSEYE, TEX_POLKA_DOT, TEX_BLEND,
  TEX_IMAGE, TEX_MARBLE, TEX_SIMPLEX_FBM) = range(11)
 

@@ -39,9 +39,10 @@ void compile_material(const dfr_material& m, DevMaterial& out, std::vector<DevLo
     for (int k = 0; k < 3; k++) out.emission[k] = m.emission[k];
     // a textured material keeps every lobe whatever its constant colours:
     // the lobes present at a hit follow from the textures there
-    const bool textured = m.textures[0] || m.textures[1] || m.textures[2] || m.textures[3];
-    for (int k = 0; k < 4; k++) out.tex[k] = m.textures[k];
+    bool textured = false;
+    for (int k = 0; k < 8; k++) { out.tex[k] = m.textures[k]; textured = textured || m.textures[k]; }
     out.tex_emission = m.emission_texture;
+    out.remap = m.remap_roughness;
     auto add = [&](const DevLobe& l) { lobes.push_back(l); out.n_lobes++; };
     auto tr = [](DevLobe& l, float ax, float ay, bool sep) {
         l.alphaX = std::max(ax, 0.001f); l.alphaY = std::max(ay, 0.001f); l.separable = sep ? 1 : 0;
@@ -690,13 +691,13 @@ static int bake_scene(const SceneHost& in, const std::vector<Xform>& xf, const s
     for (size_t p = 0; p < in.prims.size(); p++) {
         if (xf.empty() || xf[p].identity) continue;
         const dfr_material& M = in.mats[in.prims[p].material];
-        if (!(M.textures[0] || M.textures[1] || M.textures[2] || M.textures[3])) continue;
+        if (std::none_of(M.textures, M.textures + 8, [](int32_t t) { return t != 0; })) continue;
         const int shape = in.prims[p].shape;
         if (shape == DFR_SHAPE_RECTANGLE || shape == DFR_SHAPE_PLANE) {
             err = "primitive " + std::to_string(p) + ": a textured rectangle or plane takes no transform";
             return DFR_E_UNSUPPORTED;
         }
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < 8; k++) {
             std::vector<int> todo;
             if (M.textures[k]) todo.push_back(M.textures[k] - 1);
             while (!todo.empty()) {
@@ -904,8 +905,8 @@ int compile_scene(const SceneHost& in, const std::vector<Xform>& xf, CompiledSce
     for (size_t m = 0; m < in.mats.size(); m++) compile_material(in.mats[m], mats[m], lobes);
     // the extended materials and every textured material run in the EXT kernels
     for (const dfr_material& m : in.mats)
-        out.ext_features = out.ext_features || m.type >= DFR_MAT_GLOSSY || m.textures[0] || m.textures[1] ||
-                           m.textures[2] || m.textures[3] || m.emission_texture;
+        out.ext_features = out.ext_features || m.type >= DFR_MAT_GLOSSY || m.emission_texture ||
+                           std::any_of(m.textures, m.textures + 8, [](int32_t t) { return t != 0; });
     // traversal addresses nodes and triangles with 32-bit byte offsets
     const size_t node_bytes = BVH_WIDTH == 4 ? bvh.nodes4.size() * sizeof(bvh.nodes4[0]) : bvh.nodes.size() * sizeof(bvh.nodes[0]);
     if (node_bytes >= (size_t(1) << 32) || bk.tris.size() * sizeof(float4) >= (size_t(1) << 32)) {
@@ -1112,13 +1113,15 @@ int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err) {
         if (M.emission_texture < 0 || M.emission_texture > nt) {
             err = "material " + std::to_string(m) + ": emission texture out of range"; return DFR_E_ARGUMENT;
         }
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < 8; k++) {
             if (!M.textures[k]) continue;
             if (M.textures[k] < 0 || M.textures[k] > nt) { err = "material " + std::to_string(m) + ": texture out of range"; return DFR_E_ARGUMENT; }
             const bool ok = (M.type == DFR_MAT_MATTE && k == DFR_SLOT_KD) ||
-                            (M.type == DFR_MAT_PLASTIC && (k == DFR_SLOT_KD || k == DFR_SLOT_KS)) ||
+                            (M.type == DFR_MAT_PLASTIC && (k == DFR_SLOT_KD || k == DFR_SLOT_KS || k == DFR_SLOT_ROUGHNESS_U)) ||
                             (M.type == DFR_MAT_MIRROR && k == DFR_SLOT_KR) ||
-                            (M.type == DFR_MAT_GLASS && (k == DFR_SLOT_KR || k == DFR_SLOT_KT));
+                            (M.type == DFR_MAT_GLASS && (k == DFR_SLOT_KR || k == DFR_SLOT_KT)) ||
+                            (M.type == DFR_MAT_METAL && (k == DFR_SLOT_ROUGHNESS_U || k == DFR_SLOT_ROUGHNESS_V ||
+                                                         k == DFR_SLOT_ETA || k == DFR_SLOT_K));
             if (!ok) { err = "material " + std::to_string(m) + ": this slot takes no texture"; return DFR_E_UNSUPPORTED; }
         }
     }

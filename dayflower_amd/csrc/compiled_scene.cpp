@@ -383,6 +383,14 @@ private:
             pair(word, k, tid, toff);
             return constant_float(tid, toff, what);
         };
+        // a roughness: a constant gives the value, anything else becomes the
+        // slot's texture (its colour average, read at each hit)
+        auto float_slot = [&](dfr_material& mm, int slot, int32_t word, int k, const char* what) {
+            pair(word, k, tid, toff);
+            if (tid == TEX_CONSTANT) return constant_float(tid, toff, what);
+            mm.textures[slot] = texture(tid, toff) + 1;
+            return 0.0f;
+        };
         switch (id) {
             case MAT_MATTE: {  // CompiledMaterialCache.java:3010-3027: (emission, modifier) (angle, KD)
                 auto w = rec(s_.matte_materials, "matte_materials", MATTE_LENGTH);
@@ -420,11 +428,11 @@ private:
                 m.type = DFR_MAT_METAL;
                 emission_and_modifier(w[0]);
                 pair(w[1], 0, tid, toff);
-                value_color(m.eta, tid, toff, "metal eta");
+                color_slot(m, DFR_SLOT_ETA, m.eta, tid, toff);
                 pair(w[1], 1, tid, toff);
-                value_color(m.k, tid, toff, "metal k");
-                m.roughness_u = fval(w[2], 0, "metal roughness U");
-                m.roughness_v = fval(w[2], 1, "metal roughness V");
+                color_slot(m, DFR_SLOT_K, m.k, tid, toff);
+                m.roughness_u = float_slot(m, DFR_SLOT_ROUGHNESS_U, w[2], 0, "metal roughness U");
+                m.roughness_v = float_slot(m, DFR_SLOT_ROUGHNESS_V, w[2], 1, "metal roughness V");
                 m.remap_roughness = w[3] != 0;
                 break;
             }
@@ -436,7 +444,7 @@ private:
                 color_slot(m, DFR_SLOT_KD, m.kd, tid, toff);
                 pair(w[1], 1, tid, toff);
                 color_slot(m, DFR_SLOT_KS, m.ks, tid, toff);
-                m.roughness_u = m.roughness_v = fval(w[2], 0, "plastic roughness");
+                m.roughness_u = m.roughness_v = float_slot(m, DFR_SLOT_ROUGHNESS_U, w[2], 0, "plastic roughness");
                 m.remap_roughness = w[3] != 0;
                 break;
             }

@@ -1335,6 +1335,11 @@ DEV bool lobe_sample(const DevLobe& l, V wo, float u, float v, LSample& out) {
 struct TexHit {
     uint32_t set, off;
     V c[4];
+    // textured roughness and conductor Fresnel: lobes (bit i: lobe i) whose
+    // alphas become (ax, ay), whose conductor eta and k become (eta, k)
+    uint32_t rough, cond;
+    float ax, ay;
+    V eta, k;
 };
 
 struct Bsdf {
@@ -1375,7 +1380,7 @@ DEV Bsdf make_bsdf(const DevScene& sc, const DevMaterial& m, const Surf& si, V r
     Bsdf b;
     b.lobes = sc.lobes + m.first_lobe;
     b.n = m.n_lobes;
-    if constexpr (EXT) b.th = th; else { b.th.set = 0u; b.th.off = 0u; }
+    if constexpr (EXT) b.th = th; else { b.th.set = 0u; b.th.off = 0u; b.th.rough = 0u; b.th.cond = 0u; }
     b.eta = m.eta;
     b.s = si.s; b.ng = si.g.w;
     b.wo_w = -ray_d;
@@ -1391,30 +1396,41 @@ DEV int count_nonspecular(const Bsdf& b) {
     for (int i = 0; i < b.n; i++) if (!((b.th.off >> i) & 1u) && !(b.lobes[i].type & BX_SPEC)) c++;
     return c;
 }
-// lobe i of the BSDF with the hit's texture colours (EXT kernels only)
+// lobe i of the BSDF with the hit's texture values (EXT kernels only)
+DEV bool lobe_textured(const Bsdf& b, int i) {
+    return ((b.th.set & (5u << i)) | ((b.th.rough | b.th.cond) & (1u << i))) != 0u;
+}
+DEV DevLobe textured_lobe(const Bsdf& b, int i) {
+    DevLobe c = b.lobes[i];
+    if (b.th.set & (1u << i)) { const V r = b.th.c[i]; c.R[0] = r.x; c.R[1] = r.y; c.R[2] = r.z; }
+    if (b.th.set & (4u << i)) { const V r = b.th.c[2 + i]; c.T[0] = r.x; c.T[1] = r.y; c.T[2] = r.z; }
+    if (b.th.rough & (1u << i)) { c.alphaX = b.th.ax; c.alphaY = b.th.ay; }
+    if (b.th.cond & (1u << i)) {
+        c.condEta[0] = b.th.eta.x; c.condEta[1] = b.th.eta.y; c.condEta[2] = b.th.eta.z;
+        c.condK[0] = b.th.k.x; c.condK[1] = b.th.k.y; c.condK[2] = b.th.k.z;
+    }
+    return c;
+}
 template <bool EXT>
 DEV V lobe_f_at(const Bsdf& b, int i, V wo, V wi) {
     if constexpr (EXT) {
-        if (b.th.set & (5u << i)) {
-            DevLobe c = b.lobes[i];
-            if (b.th.set & (1u << i)) { const V r = b.th.c[i]; c.R[0] = r.x; c.R[1] = r.y; c.R[2] = r.z; }
-            if (b.th.set & (4u << i)) { const V r = b.th.c[2 + i]; c.T[0] = r.x; c.T[1] = r.y; c.T[2] = r.z; }
-            return lobe_f<EXT>(c, wo, wi);
-        }
+        if (lobe_textured(b, i)) return lobe_f<EXT>(textured_lobe(b, i), wo, wi);
     }
     return lobe_f<EXT>(b.lobes[i], wo, wi);
 }
 template <bool EXT>
 DEV bool lobe_sample_at(const Bsdf& b, int i, V wo, float u, float v, LSample& out) {
     if constexpr (EXT) {
-        if (b.th.set & (5u << i)) {
-            DevLobe c = b.lobes[i];
-            if (b.th.set & (1u << i)) { const V r = b.th.c[i]; c.R[0] = r.x; c.R[1] = r.y; c.R[2] = r.z; }
-            if (b.th.set & (4u << i)) { const V r = b.th.c[2 + i]; c.T[0] = r.x; c.T[1] = r.y; c.T[2] = r.z; }
-            return lobe_sample<EXT>(c, wo, u, v, out);
-        }
+        if (lobe_textured(b, i)) return lobe_sample<EXT>(textured_lobe(b, i), wo, u, v, out);
     }
     return lobe_sample<EXT>(b.lobes[i], wo, u, v, out);
+}
+template <bool EXT>
+DEV float lobe_pdf_at(const Bsdf& b, int i, V wo, V wi) {
+    if constexpr (EXT) {
+        if (lobe_textured(b, i)) return lobe_pdf<EXT>(textured_lobe(b, i), wo, wi);
+    }
+    return lobe_pdf<EXT>(b.lobes[i], wo, wi);
 }
 DEV bool lobe_on(const Bsdf& b, int i) { return !((b.th.off >> i) & 1u); }
 template <bool EXT>
@@ -1437,7 +1453,9 @@ DEV float bsdf_pdf(const Bsdf& b, uint32_t t, V wi_w) {
     float p = 0.0f; int m = 0;
     for (int i = 0; i < b.n; i++) {
         const DevLobe& l = b.lobes[i];
-        if (lobe_on(b, i) && matches(l.type, t)) { m++; p += lobe_pdf<EXT>(l, b.wo, wi); }
+        if (!(lobe_on(b, i) && matches(l.type, t))) continue;
+        m++;
+        p += lobe_pdf_at<EXT>(b, i, b.wo, wi);
     }
     if (m > 1) p /= m;
     return p;
@@ -1465,7 +1483,7 @@ DEV bool bsdf_sample(const Bsdf& b, uint32_t t, float u, float v, LSample& out) 
         for (int i = 0; i < b.n; i++) {
             const DevLobe& B = b.lobes[i];
             if (!lobe_on(b, i) || !matches(B.type, t)) continue;
-            if (i != sel) pdf += lobe_pdf<EXT>(B, b.wo, ls.wi);
+            if (i != sel) pdf += lobe_pdf_at<EXT>(b, i, b.wo, ls.wi);
             if ((refl && (B.type & BX_REFL)) || (!refl && (B.type & BX_TRANS))) f = f + lobe_f_at<EXT>(b, i, b.wo, ls.wi);
         }
     }
@@ -1723,11 +1741,26 @@ DEV V slot_color(const DevScene& sc, const DevMaterial& m, int slot, V fallback,
     const V c = texture_color(sc, m.tex[slot] - 1, si, d);
     return mk(fmaxf(c.x, 0.0f), fmaxf(c.y, 0.0f), fmaxf(c.z, 0.0f));
 }
+// A textured roughness: the texture's colour average (Texture.getFloat),
+// remapped as the material asks (MicrofacetDistribution
+// .convertRoughnessToAlpha), floored at 0.001 as the lobe's alphas are;
+// `fallback` (an alpha) when the slot has no texture.
+DEV float slot_alpha(const DevScene& sc, const DevMaterial& m, int slot, float fallback, const Surf& si, V d) {
+    if (!m.tex[slot]) return fallback;
+    const V c = texture_color(sc, m.tex[slot] - 1, si, d);
+    float r = (c.x + c.y) + c.z;
+    r = r / 3.0f;
+    if (m.remap) {
+        const float y = (float)log((double)fmaxf(r, 1.0e-3f));
+        r = 1.62142f + 0.819955f * y + 0.1734f * y * y + 0.0171201f * y * y * y + 0.000640711f * y * y * y * y;
+    }
+    return fmaxf(r, 0.001f);
+}
 template <bool EXT>
 DEV bool shade_material(const DevScene& sc, const DevMaterial& m, const Surf& si, V d, TexHit& th) {
-    th.set = 0u; th.off = 0u;
+    th.set = 0u; th.off = 0u; th.rough = 0u; th.cond = 0u;
     if constexpr (EXT) {
-        if (m.tex[0] | m.tex[1] | m.tex[2] | m.tex[3]) {
+        if (m.tex[0] | m.tex[1] | m.tex[2] | m.tex[3] | m.tex[4] | m.tex[5] | m.tex[6] | m.tex[7]) {
             const DevLobe* L = sc.lobes + m.first_lobe;
             switch (m.type) {
                 case DFR_MAT_MATTE:
@@ -1738,7 +1771,18 @@ DEV bool shade_material(const DevScene& sc, const DevMaterial& m, const Surf& si
                     th.c[1] = slot_color(sc, m, DFR_SLOT_KS, ld3(L[1].R), si, d);
                     th.set = 3u;
                     th.off = (black(th.c[0]) ? 1u : 0u) | (black(th.c[1]) ? 2u : 0u);
+                    if (m.tex[DFR_SLOT_ROUGHNESS_U]) {  // the glossy lobe's one roughness
+                        th.ax = th.ay = slot_alpha(sc, m, DFR_SLOT_ROUGHNESS_U, 0.0f, si, d);
+                        th.rough = 2u;
+                    }
                     return th.off != 3u;
+                case DFR_MAT_METAL:  // one conductor Torrance-Sparrow lobe (MetalMaterial.computeBSDF)
+                    th.ax = slot_alpha(sc, m, DFR_SLOT_ROUGHNESS_U, L[0].alphaX, si, d);
+                    th.ay = slot_alpha(sc, m, DFR_SLOT_ROUGHNESS_V, L[0].alphaY, si, d);
+                    th.eta = m.tex[DFR_SLOT_ETA] ? texture_color(sc, m.tex[DFR_SLOT_ETA] - 1, si, d) : ld3(L[0].condEta);
+                    th.k = m.tex[DFR_SLOT_K] ? texture_color(sc, m.tex[DFR_SLOT_K] - 1, si, d) : ld3(L[0].condK);
+                    th.rough = 1u; th.cond = 1u;
+                    return true;
                 case DFR_MAT_MIRROR:
                     th.c[0] = slot_color(sc, m, DFR_SLOT_KR, ld3(L[0].R), si, d); th.set = 1u;
                     return !black(th.c[0]);

@@ -49,7 +49,7 @@ struct DevLobe {           // 128 bytes
 };
 static_assert(sizeof(DevLobe) == 128, "lobe record size");
 
-struct DevMaterial {       // 64 bytes
+struct DevMaterial {       // 80 bytes
     int32_t first_lobe;
     int32_t n_lobes;       // 0 with has_bsdf = 0: pass-through surface
     int32_t has_bsdf;
@@ -59,9 +59,10 @@ struct DevMaterial {       // 64 bytes
     // per colour slot (DFR_SLOT_*): texture index + 1, or 0.  A textured
     // material keeps its full lobe list; which lobes exist at a hit, and
     // their colours, follow from the textures there.
-    int32_t tex[4];
+    int32_t tex[8];
     int32_t tex_emission;  // texture index + 1 replacing `emission`, or 0
-    int32_t pad[3];
+    int32_t remap;         // remap_roughness (for textured roughness)
+    int32_t pad[2];
 };
 
 // A texture (M/scene/texture/*Texture.java); angles as cos/sin.

@@ -124,15 +124,21 @@ extern "C" {
  *   CLEAR_COAT  kd, ks, emission
  *   UBER        kd, ks, kr, kt, k (opacity), eta[0], roughness_u/v,
  *               remap_roughness, emission
- * textures[k] (k = DFR_SLOT_KD, KS, KR, KT) replaces that colour by a texture
- * of the scene (index + 1; 0 keeps the constant): MATTE kd; PLASTIC kd, ks;
- * MIRROR kr; GLASS kr, kt.  A texture on another material or slot is
- * refused with DFR_E_UNSUPPORTED.  emission_texture textures the emission
+ * textures[k] (k = DFR_SLOT_*) replaces that parameter by a texture of the
+ * scene (index + 1; 0 keeps the constant): MATTE kd; PLASTIC kd, ks,
+ * roughness_u; MIRROR kr; GLASS kr, kt; METAL eta, k, roughness_u,
+ * roughness_v.  A roughness slot reads the texture's colour average
+ * (Texture.getFloat), remapped when remap_roughness is set.  A texture on
+ * another material or slot is refused with DFR_E_UNSUPPORTED.  emission_texture textures the emission
  * of every material type (Material.emittance reads textureEmission). */
 #define DFR_SLOT_KD 0
 #define DFR_SLOT_KS 1
 #define DFR_SLOT_KR 2
 #define DFR_SLOT_KT 3
+#define DFR_SLOT_ROUGHNESS_U 4
+#define DFR_SLOT_ROUGHNESS_V 5
+#define DFR_SLOT_ETA 6
+#define DFR_SLOT_K 7
 typedef struct dfr_material {
     int32_t type;
     int32_t remap_roughness;
@@ -159,7 +165,7 @@ typedef struct dfr_material {
     float disney_spec_trans;
     float disney_diff_trans;
     float disney_flatness;
-    int32_t textures[4];
+    int32_t textures[8];
 } dfr_material;
 
 /* Textures: M/scene/texture/{Constant,Checkerboard,UV,SurfaceNormal,

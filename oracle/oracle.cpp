@@ -1767,14 +1767,23 @@ C3 texColor(const Scene& sc, int t, const Surface& si) {
     return texChain(sc, t, si);
 }
 
+// Colours take the texture's colour, roughness its colour average
+// (Texture.getFloat, Color3F.average), Metal's eta and k its colour.
 dfr_material materialAt(const Scene& sc, const dfr_material& m, const Surface& si) {
-    if (!(m.textures[0] || m.textures[1] || m.textures[2] || m.textures[3])) return m;
+    bool any = false;
+    for (int k = 0; k < 8; k++) any = any || m.textures[k];
+    if (!any) return m;
     dfr_material r = m;
-    float* slot[4] = {r.kd, r.ks, r.kr, r.kt};
-    for (int k = 0; k < 4; k++) {
+    float* slot[8] = {r.kd, r.ks, r.kr, r.kt, &r.roughness_u, &r.roughness_v, r.eta, r.k};
+    for (int k = 0; k < 8; k++) {
         if (!m.textures[k]) continue;
         const C3 c = texColor(sc, m.textures[k] - 1, si);
-        slot[k][0] = c.r; slot[k][1] = c.g; slot[k][2] = c.b;
+        if (k == DFR_SLOT_ROUGHNESS_U || k == DFR_SLOT_ROUGHNESS_V) {
+            *slot[k] = (c.r + c.g + c.b) / 3.0f;
+            if (m.type == DFR_MAT_PLASTIC && k == DFR_SLOT_ROUGHNESS_U) r.roughness_v = *slot[k];
+        } else {
+            slot[k][0] = c.r; slot[k][1] = c.g; slot[k][2] = c.b;
+        }
     }
     return r;
 }

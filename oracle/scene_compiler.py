@@ -198,6 +198,9 @@ class SceneCompiler:
     def slot(self, M, k, color):
         return self.texture(M.textures[k] - 1) if M.textures[k] else self.constant(tuple(color))
 
+    def float_slot(self, M, k, value):
+        return self.texture(M.textures[k] - 1) if M.textures[k] else self.value(value)
+
     # -- materials ----------------------------------------------------------
     def material(self, i):
         M = self.sc.materials[i]
@@ -216,12 +219,13 @@ class SceneCompiler:
                  pack8(*self.value(M.roughness_u), *self.value(M.roughness_v)), int(M.remap_roughness), 0, 0, 0]
             name = "glass"
         elif t == abi.MAT_METAL:
-            w = [em, pack8(*self.constant(tuple(M.eta)), *self.constant(tuple(M.k))),
-                 pack8(*self.value(M.roughness_u), *self.value(M.roughness_v)), int(M.remap_roughness)]
+            w = [em, pack8(*self.slot(M, abi.SLOT_ETA, M.eta), *self.slot(M, abi.SLOT_K, M.k)),
+                 pack8(*self.float_slot(M, abi.SLOT_ROUGHNESS_U, M.roughness_u),
+                       *self.float_slot(M, abi.SLOT_ROUGHNESS_V, M.roughness_v)), int(M.remap_roughness)]
             name = "metal"
         elif t == abi.MAT_PLASTIC:
             w = [em, pack8(*self.slot(M, abi.SLOT_KD, M.kd), *self.slot(M, abi.SLOT_KS, M.ks)),
-                 pack8(*self.value(M.roughness_u), 0, 0), int(M.remap_roughness)]
+                 pack8(*self.float_slot(M, abi.SLOT_ROUGHNESS_U, M.roughness_u), 0, 0), int(M.remap_roughness)]
             name = "plastic"
         elif t == abi.MAT_DISNEY:
             v = self.value
