@@ -305,3 +305,51 @@ def test_compiled_arrays_of_a_shared_mesh_render_as_instances():
     print(f"compiled instances: paths equal {eq['compiled']:.4f} (native {eq['native']:.4f})", flush=True)
     np.testing.assert_array_equal(films["compiled"][1], c)
     assert eq["compiled"] >= eq["native"] - 0.01
+
+
+def test_several_shared_meshes_of_few_triangles():
+    """Two shared ranges (a single triangle named by three primitives, a
+    five-triangle strip by two) beside an unshared mesh: ray casting equals
+    the expansion's against the FP64 intersector."""
+    from dayflower_amd.renderer import GPURenderer
+    from tests.oracle_lib import OracleScene
+    res = 96
+    sc = S.Scene()
+    m = sc.add_material(S.matte((0.6, 0.6, 0.6)))
+    one = sc.add_mesh(np.array([[-0.5, -0.5, 0], [0.5, -0.5, 0], [0.0, 0.5, 0]], np.float32),
+                      np.array([[0, 2, 1]], np.int32), m)
+    xs = np.linspace(-1, 1, 6, dtype=np.float32)
+    strip_p = np.array([[x, y, 0.0] for x in xs for y in (-0.2, 0.2)], np.float32)
+    strip_t = np.array([[2 * i, 2 * i + 2, 2 * i + 1] for i in range(5)], np.int32)
+    strip = sc.add_mesh(strip_p, strip_t, m)
+    p, t, n = S.displaced_icosphere(500, 5, radius=0.3, center=(0.0, -0.6, 1.0))
+    sc.add_mesh(p, t, m, normals=n)
+    extra = [sc.add_instance(one), sc.add_instance(one), sc.add_instance(strip)]
+    sc.add_point_light((0.0, 2.0, -2.0), (10.0, 10.0, 10.0))
+    sc.set_camera(S.camera_record((0.0, 0.0, -3.0), (0.0, 0.0, 0.5), res, res, fov_x_deg=60.0))
+    xf = np.tile(np.eye(4, dtype=np.float32), (len(sc.primitives), 1, 1))
+    xf[one][:3, 3] = (-1.2, 0.6, 0.5)
+    xf[extra[0]][:3, :3] = _rot((0, 1, 0), 50.0) * 0.8
+    xf[extra[0]][:3, 3] = (1.1, 0.7, 0.8)
+    xf[extra[1]][:3, :3] = np.diag([1.0, -1.0, 1.0])
+    xf[extra[1]][:3, 3] = (0.0, 0.8, 1.5)
+    xf[strip][:3, 3] = (0.0, 0.0, 0.2)
+    xf[extra[2]][:3, :3] = _rot((1, 0, 0), 60.0)
+    xf[extra[2]][:3, 3] = (0.3, -0.3, 2.0)
+    ex, offset = _expanded(sc)
+    r = GPURenderer()
+    r.setScene(sc)
+    r.updateTransforms(xf)
+    r.setup()
+    rays, prim, tri, tt = r.cast_primary(res, res, sample_index=0)
+    r.dispose()
+    o = OracleScene(ex)
+    o.set_transforms(xf)
+    po, tro, to = o.trace_f64(rays)
+    o.close()
+    frac, q, mx, nhit = _hit_parity(prim, tri, tt, po, tro, to, offset)
+    on_shared = np.isin(po, [one, strip] + extra).sum()
+    print(f"few-triangle instances: id match {frac:.6f}, rel t 99.99 % {q:.2e}, hits {nhit} ({on_shared} on shared meshes)",
+          flush=True)
+    assert on_shared > 200
+    assert frac >= 0.9999 and q <= 1e-5
