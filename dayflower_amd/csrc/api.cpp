@@ -166,7 +166,7 @@ void compile_material(const dfr_material& m, DevMaterial& out, std::vector<DevLo
         }
         case DFR_MAT_SUBSTRATE: {
             float kd[3], ks[3]; sat3(kd, m.kd); sat3(ks, m.ks);
-            if (black3(kd) && black3(ks)) { out.has_bsdf = 0; break; }
+            if (black3(kd) && black3(ks) && !textured) { out.has_bsdf = 0; break; }
             float ru = m.remap_roughness ? roughness_to_alpha(m.roughness_u) : m.roughness_u;
             float rv = m.remap_roughness ? roughness_to_alpha(m.roughness_v) : m.roughness_v;
             DevLobe l{}; l.kind = LOBE_FRESNEL_BLEND; l.type = BX_GLOSSY | BX_REFL;
@@ -1121,7 +1121,10 @@ int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err) {
                             (M.type == DFR_MAT_MIRROR && k == DFR_SLOT_KR) ||
                             (M.type == DFR_MAT_GLASS && (k == DFR_SLOT_KR || k == DFR_SLOT_KT)) ||
                             (M.type == DFR_MAT_METAL && (k == DFR_SLOT_ROUGHNESS_U || k == DFR_SLOT_ROUGHNESS_V ||
-                                                         k == DFR_SLOT_ETA || k == DFR_SLOT_K));
+                                                         k == DFR_SLOT_ETA || k == DFR_SLOT_K)) ||
+                            (M.type == DFR_MAT_SUBSTRATE && (k == DFR_SLOT_KD || k == DFR_SLOT_KS)) ||
+                            (M.type == DFR_MAT_GLOSSY && k == DFR_SLOT_KR) ||
+                            (M.type == DFR_MAT_CLEAR_COAT && (k == DFR_SLOT_KD || k == DFR_SLOT_KS));
             if (!ok) { err = "material " + std::to_string(m) + ": this slot takes no texture"; return DFR_E_UNSUPPORTED; }
         }
     }

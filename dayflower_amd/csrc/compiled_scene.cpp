@@ -475,7 +475,7 @@ private:
                 m.type = DFR_MAT_GLOSSY;
                 emission_and_modifier(w[0]);
                 pair(w[1], 0, tid, toff);
-                value_color(m.kr, tid, toff, "glossy KR");
+                color_slot(m, DFR_SLOT_KR, m.kr, tid, toff);
                 m.roughness_u = m.roughness_v = fval(w[1], 1, "glossy roughness");
                 break;
             }
@@ -484,9 +484,9 @@ private:
                 m.type = DFR_MAT_SUBSTRATE;
                 emission_and_modifier(w[0]);
                 pair(w[1], 0, tid, toff);
-                value_color(m.kd, tid, toff, "substrate KD");
+                color_slot(m, DFR_SLOT_KD, m.kd, tid, toff);
                 pair(w[1], 1, tid, toff);
-                value_color(m.ks, tid, toff, "substrate KS");
+                color_slot(m, DFR_SLOT_KS, m.ks, tid, toff);
                 m.roughness_u = fval(w[2], 0, "substrate roughness U");
                 m.roughness_v = fval(w[2], 1, "substrate roughness V");
                 m.remap_roughness = w[3] != 0;
@@ -513,9 +513,9 @@ private:
                 m.type = DFR_MAT_CLEAR_COAT;
                 emission_and_modifier(w[0]);
                 pair(w[1], 0, tid, toff);
-                value_color(m.kd, tid, toff, "clear coat KD");
+                color_slot(m, DFR_SLOT_KD, m.kd, tid, toff);
                 pair(w[1], 1, tid, toff);
-                value_color(m.ks, tid, toff, "clear coat KS");
+                color_slot(m, DFR_SLOT_KS, m.ks, tid, toff);
                 break;
             }
             case MAT_UBER: {  // :3550-3580: (emission, modifier) (KD, KR) (KS, KT) (eta, opacity) (roughness U, V) remap

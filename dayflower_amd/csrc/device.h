@@ -1371,6 +1371,10 @@ DEV float clear_coat_layer(Bsdf& b, const Surf& si, V d, float u) {
     const float prr = 0.25f + 0.5f * refl;
     if (u < prr) return refl / prr;
     b.lobes += 1;
+    // the diffuse layer is lobe 0 now: so is its textured colour
+    b.th.c[0] = b.th.c[1];
+    b.th.set >>= 1;
+    b.th.off >>= 1;
     return (1.0f - refl) / (1.0f - prr);
 }
 // `beta` is the path throughput at the hit (after its emission is added).
@@ -1736,18 +1740,44 @@ DEV V texture_color(const DevScene& sc, int t, const Surf& si, V d) {
 // computeBSDF leaves them out (MatteMaterial, PlasticMaterial,
 // MirrorMaterial, GlassMaterial); returns false when the material has no
 // BSDF here (the ray passes through).
-DEV V slot_color(const DevScene& sc, const DevMaterial& m, int slot, V fallback, const Surf& si, V d) {
+// The values of a material's textured slots at a hit, every slot through one
+// call of texture_color (the texture code is inlined once per kernel, not
+// once per slot and material).
+struct SlotVals {
+    V v[8];
+};
+DEV SlotVals slot_values(const DevScene& sc, const DevMaterial& m, const Surf& si, V d) {
+    SlotVals sv;
+    V t0 = mk(0, 0, 0), t1 = t0, t2 = t0, t3 = t0, t4 = t0, t5 = t0, t6 = t0, t7 = t0;
+#pragma unroll 1
+    for (int k = 0; k < 8; k++) {
+        if (!m.tex[k]) continue;
+        const V c = texture_color(sc, m.tex[k] - 1, si, d);
+        switch (k) {
+            case 0: t0 = c; break; case 1: t1 = c; break; case 2: t2 = c; break; case 3: t3 = c; break;
+            case 4: t4 = c; break; case 5: t5 = c; break; case 6: t6 = c; break; default: t7 = c; break;
+        }
+    }
+    sv.v[0] = t0; sv.v[1] = t1; sv.v[2] = t2; sv.v[3] = t3; sv.v[4] = t4; sv.v[5] = t5; sv.v[6] = t6; sv.v[7] = t7;
+    return sv;
+}
+// a colour slot: the texture's colour clamped at zero, else `fallback`
+DEV V slot_color(const DevMaterial& m, const SlotVals& sv, int slot, V fallback) {
     if (!m.tex[slot]) return fallback;
-    const V c = texture_color(sc, m.tex[slot] - 1, si, d);
+    const V c = sv.v[slot];
     return mk(fmaxf(c.x, 0.0f), fmaxf(c.y, 0.0f), fmaxf(c.z, 0.0f));
+}
+// a colour slot read unclamped
+DEV V slot_raw(const DevMaterial& m, const SlotVals& sv, int slot, V fallback) {
+    return m.tex[slot] ? sv.v[slot] : fallback;
 }
 // A textured roughness: the texture's colour average (Texture.getFloat),
 // remapped as the material asks (MicrofacetDistribution
 // .convertRoughnessToAlpha), floored at 0.001 as the lobe's alphas are;
 // `fallback` (an alpha) when the slot has no texture.
-DEV float slot_alpha(const DevScene& sc, const DevMaterial& m, int slot, float fallback, const Surf& si, V d) {
+DEV float slot_alpha(const DevMaterial& m, const SlotVals& sv, int slot, float fallback) {
     if (!m.tex[slot]) return fallback;
-    const V c = texture_color(sc, m.tex[slot] - 1, si, d);
+    const V c = sv.v[slot];
     float r = (c.x + c.y) + c.z;
     r = r / 3.0f;
     if (m.remap) {
@@ -1762,39 +1792,60 @@ DEV bool shade_material(const DevScene& sc, const DevMaterial& m, const Surf& si
     if constexpr (EXT) {
         if (m.tex[0] | m.tex[1] | m.tex[2] | m.tex[3] | m.tex[4] | m.tex[5] | m.tex[6] | m.tex[7]) {
             const DevLobe* L = sc.lobes + m.first_lobe;
+            const SlotVals sv = slot_values(sc, m, si, d);
             switch (m.type) {
                 case DFR_MAT_MATTE:
-                    th.c[0] = slot_color(sc, m, DFR_SLOT_KD, ld3(L[0].R), si, d); th.set = 1u;
+                    th.c[0] = slot_color(m, sv, DFR_SLOT_KD, ld3(L[0].R)); th.set = 1u;
                     return true;
                 case DFR_MAT_PLASTIC:
-                    th.c[0] = slot_color(sc, m, DFR_SLOT_KD, ld3(L[0].R), si, d);
-                    th.c[1] = slot_color(sc, m, DFR_SLOT_KS, ld3(L[1].R), si, d);
+                    th.c[0] = slot_color(m, sv, DFR_SLOT_KD, ld3(L[0].R));
+                    th.c[1] = slot_color(m, sv, DFR_SLOT_KS, ld3(L[1].R));
                     th.set = 3u;
                     th.off = (black(th.c[0]) ? 1u : 0u) | (black(th.c[1]) ? 2u : 0u);
                     if (m.tex[DFR_SLOT_ROUGHNESS_U]) {  // the glossy lobe's one roughness
-                        th.ax = th.ay = slot_alpha(sc, m, DFR_SLOT_ROUGHNESS_U, 0.0f, si, d);
+                        th.ax = th.ay = slot_alpha(m, sv, DFR_SLOT_ROUGHNESS_U, 0.0f);
                         th.rough = 2u;
                     }
                     return th.off != 3u;
+                case DFR_MAT_SUBSTRATE: {  // one FresnelBlend lobe: Rd = kd in R, Rs = ks in T
+                    th.c[0] = slot_color(m, sv, DFR_SLOT_KD, ld3(L[0].R));
+                    th.c[2] = slot_color(m, sv, DFR_SLOT_KS, ld3(L[0].T));
+                    th.set = 1u | 4u;
+                    return !(black(th.c[0]) && black(th.c[2]));
+                }
+                case DFR_MAT_GLOSSY: {  // TorranceSparrowBRDF(KR, ConductorFresnel(KR, 1, 1)): eta = k = 1 / KR
+                    // GlossyMaterial reads KR unclamped
+                    const V kr = slot_raw(m, sv, DFR_SLOT_KR, ld3(L[0].R));
+                    th.c[0] = kr; th.set = 1u;
+                    th.eta = mk(1.0f / kr.x, 1.0f / kr.y, 1.0f / kr.z); th.k = th.eta;
+                    th.ax = L[0].alphaX; th.ay = L[0].alphaY;
+                    th.cond = 1u;
+                    return true;
+                }
+                case DFR_MAT_CLEAR_COAT:  // the specular layer (ks) and the diffuse one (kd), unclamped
+                    th.c[0] = slot_raw(m, sv, DFR_SLOT_KS, ld3(L[0].R));
+                    th.c[1] = slot_raw(m, sv, DFR_SLOT_KD, ld3(L[1].R));
+                    th.set = 3u;
+                    return true;
                 case DFR_MAT_METAL:  // one conductor Torrance-Sparrow lobe (MetalMaterial.computeBSDF)
-                    th.ax = slot_alpha(sc, m, DFR_SLOT_ROUGHNESS_U, L[0].alphaX, si, d);
-                    th.ay = slot_alpha(sc, m, DFR_SLOT_ROUGHNESS_V, L[0].alphaY, si, d);
-                    th.eta = m.tex[DFR_SLOT_ETA] ? texture_color(sc, m.tex[DFR_SLOT_ETA] - 1, si, d) : ld3(L[0].condEta);
-                    th.k = m.tex[DFR_SLOT_K] ? texture_color(sc, m.tex[DFR_SLOT_K] - 1, si, d) : ld3(L[0].condK);
+                    th.ax = slot_alpha(m, sv, DFR_SLOT_ROUGHNESS_U, L[0].alphaX);
+                    th.ay = slot_alpha(m, sv, DFR_SLOT_ROUGHNESS_V, L[0].alphaY);
+                    th.eta = slot_raw(m, sv, DFR_SLOT_ETA, ld3(L[0].condEta));
+                    th.k = slot_raw(m, sv, DFR_SLOT_K, ld3(L[0].condK));
                     th.rough = 1u; th.cond = 1u;
                     return true;
                 case DFR_MAT_MIRROR:
-                    th.c[0] = slot_color(sc, m, DFR_SLOT_KR, ld3(L[0].R), si, d); th.set = 1u;
+                    th.c[0] = slot_color(m, sv, DFR_SLOT_KR, ld3(L[0].R)); th.set = 1u;
                     return !black(th.c[0]);
                 case DFR_MAT_GLASS:
                     if (m.n_lobes == 1) {  // smooth: one Fresnel-specular lobe
-                        th.c[0] = slot_color(sc, m, DFR_SLOT_KR, ld3(L[0].R), si, d);
-                        th.c[2] = slot_color(sc, m, DFR_SLOT_KT, ld3(L[0].T), si, d);
+                        th.c[0] = slot_color(m, sv, DFR_SLOT_KR, ld3(L[0].R));
+                        th.c[2] = slot_color(m, sv, DFR_SLOT_KT, ld3(L[0].T));
                         th.set = 1u | 4u;
                         return !(black(th.c[0]) && black(th.c[2]));
                     }
-                    th.c[0] = slot_color(sc, m, DFR_SLOT_KR, ld3(L[0].R), si, d);
-                    th.c[3] = slot_color(sc, m, DFR_SLOT_KT, ld3(L[1].T), si, d);
+                    th.c[0] = slot_color(m, sv, DFR_SLOT_KR, ld3(L[0].R));
+                    th.c[3] = slot_color(m, sv, DFR_SLOT_KT, ld3(L[1].T));
                     th.set = 1u | 8u;
                     th.off = (black(th.c[0]) ? 1u : 0u) | (black(th.c[3]) ? 2u : 0u);
                     return th.off != 3u;

@@ -239,10 +239,10 @@ class SceneCompiler:
                  pack8(*v(M.disney_spec_trans), int(M.thin), 0)] + [0] * 7
             name = "disney"
         elif t == abi.MAT_GLOSSY:
-            w = [em, pack8(*self.constant(tuple(M.kr)), *self.value(M.roughness_u))]
+            w = [em, pack8(*self.slot(M, abi.SLOT_KR, M.kr), *self.value(M.roughness_u))]
             name = "glossy"
         elif t == abi.MAT_SUBSTRATE:
-            w = [em, pack8(*self.constant(tuple(M.kd)), *self.constant(tuple(M.ks))),
+            w = [em, pack8(*self.slot(M, abi.SLOT_KD, M.kd), *self.slot(M, abi.SLOT_KS, M.ks)),
                  pack8(*self.value(M.roughness_u), *self.value(M.roughness_v)), int(M.remap_roughness)]
             name = "substrate"
         elif t == abi.MAT_TRANSLUCENT:
@@ -251,7 +251,7 @@ class SceneCompiler:
                  pack8(*self.value(M.roughness_u), int(M.remap_roughness), 0)]
             name = "translucent"
         elif t == abi.MAT_CLEAR_COAT:
-            w = [em, pack8(*self.constant(tuple(M.kd)), *self.constant(tuple(M.ks)))]
+            w = [em, pack8(*self.slot(M, abi.SLOT_KD, M.kd), *self.slot(M, abi.SLOT_KS, M.ks))]
             name = "clear_coat"
         elif t == abi.MAT_UBER:
             w = [em, pack8(*self.constant(tuple(M.kd)), *self.constant(tuple(M.kr))),

@@ -495,3 +495,71 @@ def test_textured_roughness_eta_and_k_on_the_gpu(kernel):
     eq = np.isclose(x, y, rtol=1e-4, atol=1e-5).all(-1).mean()
     print(f"compiled textured roughness: pixels equal to dfr_set_scene's {eq:.4f}", flush=True)
     assert eq >= 0.97
+
+
+def _ext_textured_scene(res):
+    """Substrate (checkerboard kd, UV ks), Glossy (UV kr) and ClearCoat
+    (checkerboard kd, constant-texture ks) spheres over a Matte floor."""
+    sc = S.Scene()
+    floor = sc.add_material(S.matte((0.6, 0.6, 0.6)))
+    sub = sc.add_material(S.substrate((0.5, 0.3, 0.2), (0.1, 0.1, 0.1), 0.3))
+    glo = sc.add_material(S.glossy((0.8, 0.6, 0.4), 0.2))
+    cc = sc.add_material(S.clear_coat((0.4, 0.5, 0.6), (0.2, 0.2, 0.2)))
+    red, blue = sc.constant_texture((0.8, 0.1, 0.1)), sc.constant_texture((0.1, 0.2, 0.9))
+    sc.set_texture(sub, abi.SLOT_KD, sc.checkerboard_texture(red, blue, scale=(8.0, 4.0)))
+    sc.set_texture(sub, abi.SLOT_KS, sc.uv_texture())
+    sc.set_texture(glo, abi.SLOT_KR, sc.blend_texture(sc.uv_texture(), sc.constant_texture((0.9, 0.9, 0.9))))
+    sc.set_texture(cc, abi.SLOT_KD, sc.checkerboard_texture(blue, red, scale=(6.0, 6.0)))
+    sc.set_texture(cc, abi.SLOT_KS, sc.constant_texture((0.3, 0.3, 0.3)))
+    sc.add_plane((0.0, -1.0, 0.0), (1.0, 0.0, 0.0), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0), floor)
+    sc.add_sphere((-1.0, -0.4, 0.8), 0.55, sub)
+    sc.add_sphere((0.1, -0.4, 0.5), 0.55, glo)
+    sc.add_sphere((1.1, -0.4, 0.9), 0.55, cc)
+    sc.add_point_light((1.0, 3.0, -1.5), (30.0, 30.0, 30.0))
+    sc.add_perez_light(2.5, (0.5, 1.0, -0.5))
+    sc.set_camera(S.camera_record((0.0, 0.4, -3.0), (0.0, -0.2, 0.6), res, res, fov_x_deg=55.0))
+    return sc
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_textured_substrate_glossy_and_clear_coat(kernel):
+    """Same streams as the oracle (>= 98 % of 1-spp paths, 16-spp mean
+    within 0.5 %, counts exact) and the compiled arrays like dfr_set_scene."""
+    from dayflower_amd import renderer as R
+    from oracle import scene_compiler as SC
+    from tests.oracle_lib import OracleScene
+    res, seed = 48, 31
+    sc = _ext_textured_scene(res)
+    o = OracleScene(sc)
+
+    def gpu(setter, spp):
+        r = R.GPURenderer()
+        setter(r)
+        r.setSeed(seed)
+        r.setMaximumBounce(8)
+        r.setKernel(kernel)
+        r.setup()
+        r.render(spp)
+        g, k = r.film(res, res)
+        r.dispose()
+        return g, k
+    for spp in (1, 16):
+        s, _, c = o.render(res, res, spp, seed=seed, max_bounce=8)
+        ref = s / np.maximum(c, 1)[..., None]
+        g, k = gpu(lambda r: r.setScene(sc), spp)
+        np.testing.assert_array_equal(k, c)
+        eq = np.isclose(g, ref, rtol=1e-3, atol=1e-4).all(-1).mean()
+        rel = abs(g.mean() - ref.mean()) / ref.mean()
+        print(f"textured substrate/glossy/clear coat kernel {kernel} spp {spp}: paths equal {eq:.4f}, "
+              f"mean rel {rel:.2e}", flush=True)
+        if spp == 1:
+            assert eq >= 0.98
+        else:
+            assert rel < 5e-3
+    o.close()
+    x, _ = gpu(lambda r: r.setCompiledScene(SC.compatible(SC.compile_scene(sc))), 4)
+    y, _ = gpu(lambda r: r.setScene(sc), 4)
+    eq = np.isclose(x, y, rtol=1e-4, atol=1e-5).all(-1).mean()
+    print(f"compiled: pixels equal to dfr_set_scene's {eq:.4f}", flush=True)
+    assert eq >= 0.97
