@@ -42,7 +42,7 @@ void compile_material(const dfr_material& m, DevMaterial& out, std::vector<DevLo
     bool textured = false;
     for (int k = 0; k < 8; k++) { out.tex[k] = m.textures[k]; textured = textured || m.textures[k]; }
     out.tex_emission = m.emission_texture;
-    out.remap = m.remap_roughness;
+    out.remap = m.type == DFR_MAT_GLOSSY ? 0 : m.remap_roughness;  // Glossy uses its roughness as is
     auto add = [&](const DevLobe& l) { lobes.push_back(l); out.n_lobes++; };
     auto tr = [](DevLobe& l, float ax, float ay, bool sep) {
         l.alphaX = std::max(ax, 0.001f); l.alphaY = std::max(ay, 0.001f); l.separable = sep ? 1 : 0;
@@ -1122,8 +1122,9 @@ int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err) {
                             (M.type == DFR_MAT_GLASS && (k == DFR_SLOT_KR || k == DFR_SLOT_KT)) ||
                             (M.type == DFR_MAT_METAL && (k == DFR_SLOT_ROUGHNESS_U || k == DFR_SLOT_ROUGHNESS_V ||
                                                          k == DFR_SLOT_ETA || k == DFR_SLOT_K)) ||
-                            (M.type == DFR_MAT_SUBSTRATE && (k == DFR_SLOT_KD || k == DFR_SLOT_KS)) ||
-                            (M.type == DFR_MAT_GLOSSY && k == DFR_SLOT_KR) ||
+                            (M.type == DFR_MAT_SUBSTRATE && (k == DFR_SLOT_KD || k == DFR_SLOT_KS ||
+                                                             k == DFR_SLOT_ROUGHNESS_U || k == DFR_SLOT_ROUGHNESS_V)) ||
+                            (M.type == DFR_MAT_GLOSSY && (k == DFR_SLOT_KR || k == DFR_SLOT_ROUGHNESS_U)) ||
                             (M.type == DFR_MAT_CLEAR_COAT && (k == DFR_SLOT_KD || k == DFR_SLOT_KS));
             if (!ok) { err = "material " + std::to_string(m) + ": this slot takes no texture"; return DFR_E_UNSUPPORTED; }
         }

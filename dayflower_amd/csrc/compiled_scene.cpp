@@ -476,7 +476,7 @@ private:
                 emission_and_modifier(w[0]);
                 pair(w[1], 0, tid, toff);
                 color_slot(m, DFR_SLOT_KR, m.kr, tid, toff);
-                m.roughness_u = m.roughness_v = fval(w[1], 1, "glossy roughness");
+                m.roughness_u = m.roughness_v = float_slot(m, DFR_SLOT_ROUGHNESS_U, w[1], 1, "glossy roughness");
                 break;
             }
             case MAT_SUBSTRATE: {  // :3362-3385: (emission, modifier) (KD, KS) (roughness U, V) remap
@@ -487,8 +487,8 @@ private:
                 color_slot(m, DFR_SLOT_KD, m.kd, tid, toff);
                 pair(w[1], 1, tid, toff);
                 color_slot(m, DFR_SLOT_KS, m.ks, tid, toff);
-                m.roughness_u = fval(w[2], 0, "substrate roughness U");
-                m.roughness_v = fval(w[2], 1, "substrate roughness V");
+                m.roughness_u = float_slot(m, DFR_SLOT_ROUGHNESS_U, w[2], 0, "substrate roughness U");
+                m.roughness_v = float_slot(m, DFR_SLOT_ROUGHNESS_V, w[2], 1, "substrate roughness V");
                 m.remap_roughness = w[3] != 0;
                 break;
             }

@@ -1811,6 +1811,9 @@ DEV bool shade_material(const DevScene& sc, const DevMaterial& m, const Surf& si
                     th.c[0] = slot_color(m, sv, DFR_SLOT_KD, ld3(L[0].R));
                     th.c[2] = slot_color(m, sv, DFR_SLOT_KS, ld3(L[0].T));
                     th.set = 1u | 4u;
+                    th.ax = slot_alpha(m, sv, DFR_SLOT_ROUGHNESS_U, L[0].alphaX);
+                    th.ay = slot_alpha(m, sv, DFR_SLOT_ROUGHNESS_V, L[0].alphaY);
+                    th.rough = 1u;
                     return !(black(th.c[0]) && black(th.c[2]));
                 }
                 case DFR_MAT_GLOSSY: {  // TorranceSparrowBRDF(KR, ConductorFresnel(KR, 1, 1)): eta = k = 1 / KR
@@ -1818,8 +1821,8 @@ DEV bool shade_material(const DevScene& sc, const DevMaterial& m, const Surf& si
                     const V kr = slot_raw(m, sv, DFR_SLOT_KR, ld3(L[0].R));
                     th.c[0] = kr; th.set = 1u;
                     th.eta = mk(1.0f / kr.x, 1.0f / kr.y, 1.0f / kr.z); th.k = th.eta;
-                    th.ax = L[0].alphaX; th.ay = L[0].alphaY;
-                    th.cond = 1u;
+                    th.ax = th.ay = slot_alpha(m, sv, DFR_SLOT_ROUGHNESS_U, L[0].alphaX);
+                    th.cond = 1u; th.rough = 1u;
                     return true;
                 }
                 case DFR_MAT_CLEAR_COAT:  // the specular layer (ks) and the diffuse one (kd), unclamped

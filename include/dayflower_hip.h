@@ -127,7 +127,8 @@ extern "C" {
  * textures[k] (k = DFR_SLOT_*) replaces that parameter by a texture of the
  * scene (index + 1; 0 keeps the constant): MATTE kd; PLASTIC kd, ks,
  * roughness_u; MIRROR kr; GLASS kr, kt; METAL eta, k, roughness_u,
- * roughness_v; SUBSTRATE kd, ks; GLOSSY kr; CLEAR_COAT kd, ks.  A roughness slot reads the texture's colour average
+ * roughness_v; SUBSTRATE kd, ks, roughness_u, roughness_v; GLOSSY kr,
+ * roughness_u (as is); CLEAR_COAT kd, ks.  A roughness slot reads the texture's colour average
  * (Texture.getFloat), remapped when remap_roughness is set.  A texture on
  * another material or slot is refused with DFR_E_UNSUPPORTED.  emission_texture textures the emission
  * of every material type (Material.emittance reads textureEmission). */

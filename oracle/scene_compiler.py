@@ -239,11 +239,12 @@ class SceneCompiler:
                  pack8(*v(M.disney_spec_trans), int(M.thin), 0)] + [0] * 7
             name = "disney"
         elif t == abi.MAT_GLOSSY:
-            w = [em, pack8(*self.slot(M, abi.SLOT_KR, M.kr), *self.value(M.roughness_u))]
+            w = [em, pack8(*self.slot(M, abi.SLOT_KR, M.kr), *self.float_slot(M, abi.SLOT_ROUGHNESS_U, M.roughness_u))]
             name = "glossy"
         elif t == abi.MAT_SUBSTRATE:
             w = [em, pack8(*self.slot(M, abi.SLOT_KD, M.kd), *self.slot(M, abi.SLOT_KS, M.ks)),
-                 pack8(*self.value(M.roughness_u), *self.value(M.roughness_v)), int(M.remap_roughness)]
+                 pack8(*self.float_slot(M, abi.SLOT_ROUGHNESS_U, M.roughness_u),
+                       *self.float_slot(M, abi.SLOT_ROUGHNESS_V, M.roughness_v)), int(M.remap_roughness)]
             name = "substrate"
         elif t == abi.MAT_TRANSLUCENT:
             w = [em, pack8(*self.constant(tuple(M.kd)), *self.constant(tuple(M.ks))),

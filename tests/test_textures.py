@@ -498,8 +498,9 @@ def test_textured_roughness_eta_and_k_on_the_gpu(kernel):
 
 
 def _ext_textured_scene(res):
-    """Substrate (checkerboard kd, UV ks), Glossy (UV kr) and ClearCoat
-    (checkerboard kd, constant-texture ks) spheres over a Matte floor."""
+    """Substrate (checkerboard kd, UV ks, checkerboard roughness v), Glossy
+    (UV kr, checkerboard roughness) and ClearCoat (checkerboard kd,
+    constant-texture ks) spheres over a Matte floor."""
     sc = S.Scene()
     floor = sc.add_material(S.matte((0.6, 0.6, 0.6)))
     sub = sc.add_material(S.substrate((0.5, 0.3, 0.2), (0.1, 0.1, 0.1), 0.3))
@@ -510,6 +511,9 @@ def _ext_textured_scene(res):
     sc.set_texture(sub, abi.SLOT_KS, sc.uv_texture())
     sc.set_texture(glo, abi.SLOT_KR, sc.blend_texture(sc.uv_texture(), sc.constant_texture((0.9, 0.9, 0.9))))
     sc.set_texture(cc, abi.SLOT_KD, sc.checkerboard_texture(blue, red, scale=(6.0, 6.0)))
+    rough_lo, rough_hi = sc.constant_texture((0.05, 0.05, 0.05)), sc.constant_texture((0.5, 0.5, 0.5))
+    sc.set_texture(sub, abi.SLOT_ROUGHNESS_V, sc.checkerboard_texture(rough_lo, rough_hi, scale=(5.0, 5.0)))
+    sc.set_texture(glo, abi.SLOT_ROUGHNESS_U, sc.checkerboard_texture(rough_hi, rough_lo, scale=(4.0, 8.0)))
     sc.set_texture(cc, abi.SLOT_KS, sc.constant_texture((0.3, 0.3, 0.3)))
     sc.add_plane((0.0, -1.0, 0.0), (1.0, 0.0, 0.0), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0), floor)
     sc.add_sphere((-1.0, -0.4, 0.8), 0.55, sub)
