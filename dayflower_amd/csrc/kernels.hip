@@ -1245,9 +1245,10 @@ void wavefront_init(WavefrontBuffers& w, const DevJob* jobs, int n_jobs, int64_t
 // ray's tmax becomes -inf.  Kept out of k_trace_queue, whose
 // register budget (72 at 7 waves per SIMD) the walk's code would spill, and
 // whose rays set up one at a time as lanes free up; here a whole wave walks
-// together.
+// together.  At most 72 VGPRs (7 waves per SIMD): 5.4 % faster on 5000
+// spheres and rectangles, 2 % on 64 mesh instances, than 90 VGPRs at 5.
 template <int BLOCK, int SHORT, bool SHADOW>
-__global__ __launch_bounds__(BLOCK) void k_items_queue(const DevScene* __restrict__ scenes,
+__global__ __launch_bounds__(BLOCK, 7) void k_items_queue(const DevScene* __restrict__ scenes,
                                                        const DevJob* __restrict__ jobs, PathSoA q, ShadowSoA sh,
                                                        float4* __restrict__ hits, const Counters* __restrict__ ctr,
                                                        int* __restrict__ overflow, const DevScene* __restrict__ uscene) {
