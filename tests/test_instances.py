@@ -353,3 +353,25 @@ def test_several_shared_meshes_of_few_triangles():
           flush=True)
     assert on_shared > 200
     assert frac >= 0.9999 and q <= 1e-5
+
+
+def test_items_and_instances_in_a_packed_batch():
+    """Jobs on a scene with an item tree (hundreds of spheres and
+    rectangles) and on a scene with shared meshes render the same alone and
+    inside a packed batch with other scenes, where the pre-pass finds each
+    ray's scene through its job."""
+    from dayflower_amd.renderer import GPURenderer, make_job
+    from tests.test_item_bvh import _crowd
+    r = GPURenderer()
+    crowd = _crowd(32, mesh=False)
+    shared, _, _ = _field(32, n=6)           # instances at the identity: coincident copies
+    a, b, c = r.addScene(crowd), r.addScene(shared), r.addScene(S.cornell_box(32, 32))
+    ja = make_job(a, crowd.camera, 32, 32, 4, seed=5)
+    jb = make_job(b, shared.camera, 32, 32, 4, seed=6)
+    alone = [r.batchRender([j])[0] for j in (ja, jb)]
+    films = r.batchRender([make_job(c, S.cornell_box(32, 32).camera, 24, 24, 4, seed=1), ja,
+                           make_job(a, crowd.camera, 16, 16, 8, seed=9), jb])
+    r.dispose()
+    for got, want in ((films[1], alone[0]), (films[3], alone[1])):
+        assert want.mean() > 0
+        np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-5)
