@@ -1119,7 +1119,7 @@ int copy_scene_desc(const dfr_scene_desc* d, SceneHost& h, std::string& err) {
             const bool ok = (M.type == DFR_MAT_MATTE && k == DFR_SLOT_KD) ||
                             (M.type == DFR_MAT_PLASTIC && (k == DFR_SLOT_KD || k == DFR_SLOT_KS || k == DFR_SLOT_ROUGHNESS_U)) ||
                             (M.type == DFR_MAT_MIRROR && k == DFR_SLOT_KR) ||
-                            (M.type == DFR_MAT_GLASS && (k == DFR_SLOT_KR || k == DFR_SLOT_KT)) ||
+                            (M.type == DFR_MAT_GLASS && (k == DFR_SLOT_KR || k == DFR_SLOT_KT || k == DFR_SLOT_ETA)) ||
                             (M.type == DFR_MAT_METAL && (k == DFR_SLOT_ROUGHNESS_U || k == DFR_SLOT_ROUGHNESS_V ||
                                                          k == DFR_SLOT_ETA || k == DFR_SLOT_K)) ||
                             (M.type == DFR_MAT_SUBSTRATE && (k == DFR_SLOT_KD || k == DFR_SLOT_KS ||

@@ -413,7 +413,7 @@ private:
                 auto w = rec(s_.glass_materials, "glass_materials", GLASS_LENGTH);
                 m.type = DFR_MAT_GLASS;
                 emission_and_modifier(w[0]);
-                m.eta[0] = fval(w[1], 0, "glass eta");
+                m.eta[0] = float_slot(m, DFR_SLOT_ETA, w[1], 0, "glass eta");
                 pair(w[2], 0, tid, toff);
                 color_slot(m, DFR_SLOT_KR, m.kr, tid, toff);
                 pair(w[2], 1, tid, toff);

@@ -1340,6 +1340,10 @@ struct TexHit {
     uint32_t rough, cond;
     float ax, ay;
     V eta, k;
+    // a textured dielectric eta (GlassMaterial's textureEta): lobes (bit i)
+    // whose dielectric eta becomes `geta`, and the BSDF's eta
+    uint32_t deta;
+    float geta;
 };
 
 struct Bsdf {
@@ -1384,8 +1388,9 @@ DEV Bsdf make_bsdf(const DevScene& sc, const DevMaterial& m, const Surf& si, V r
     Bsdf b;
     b.lobes = sc.lobes + m.first_lobe;
     b.n = m.n_lobes;
-    if constexpr (EXT) b.th = th; else { b.th.set = 0u; b.th.off = 0u; b.th.rough = 0u; b.th.cond = 0u; }
+    if constexpr (EXT) b.th = th; else { b.th.set = 0u; b.th.off = 0u; b.th.rough = 0u; b.th.cond = 0u; b.th.deta = 0u; }
     b.eta = m.eta;
+    if constexpr (EXT) { if (th.deta) b.eta = th.geta; }
     b.s = si.s; b.ng = si.g.w;
     b.wo_w = -ray_d;
     b.wo = to_local(b.wo_w, si.s);
@@ -1402,7 +1407,7 @@ DEV int count_nonspecular(const Bsdf& b) {
 }
 // lobe i of the BSDF with the hit's texture values (EXT kernels only)
 DEV bool lobe_textured(const Bsdf& b, int i) {
-    return ((b.th.set & (5u << i)) | ((b.th.rough | b.th.cond) & (1u << i))) != 0u;
+    return ((b.th.set & (5u << i)) | ((b.th.rough | b.th.cond | b.th.deta) & (1u << i))) != 0u;
 }
 DEV DevLobe textured_lobe(const Bsdf& b, int i) {
     DevLobe c = b.lobes[i];
@@ -1413,6 +1418,7 @@ DEV DevLobe textured_lobe(const Bsdf& b, int i) {
         c.condEta[0] = b.th.eta.x; c.condEta[1] = b.th.eta.y; c.condEta[2] = b.th.eta.z;
         c.condK[0] = b.th.k.x; c.condK[1] = b.th.k.y; c.condK[2] = b.th.k.z;
     }
+    if (b.th.deta & (1u << i)) { c.etaB = b.th.geta; c.frEtaT = b.th.geta; }
     return c;
 }
 template <bool EXT>
@@ -1788,7 +1794,7 @@ DEV float slot_alpha(const DevMaterial& m, const SlotVals& sv, int slot, float f
 }
 template <bool EXT>
 DEV bool shade_material(const DevScene& sc, const DevMaterial& m, const Surf& si, V d, TexHit& th) {
-    th.set = 0u; th.off = 0u; th.rough = 0u; th.cond = 0u;
+    th.set = 0u; th.off = 0u; th.rough = 0u; th.cond = 0u; th.deta = 0u;
     if constexpr (EXT) {
         if (m.tex[0] | m.tex[1] | m.tex[2] | m.tex[3] | m.tex[4] | m.tex[5] | m.tex[6] | m.tex[7]) {
             const DevLobe* L = sc.lobes + m.first_lobe;
@@ -1841,6 +1847,11 @@ DEV bool shade_material(const DevScene& sc, const DevMaterial& m, const Surf& si
                     th.c[0] = slot_color(m, sv, DFR_SLOT_KR, ld3(L[0].R)); th.set = 1u;
                     return !black(th.c[0]);
                 case DFR_MAT_GLASS:
+                    if (m.tex[DFR_SLOT_ETA]) {  // textureEta: the colour average, every lobe
+                        const V e = sv.v[DFR_SLOT_ETA];
+                        th.geta = (e.x + e.y + e.z) / 3.0f;
+                        th.deta = m.n_lobes == 1 ? 1u : 3u;
+                    }
                     if (m.n_lobes == 1) {  // smooth: one Fresnel-specular lobe
                         th.c[0] = slot_color(m, sv, DFR_SLOT_KR, ld3(L[0].R));
                         th.c[2] = slot_color(m, sv, DFR_SLOT_KT, ld3(L[0].T));

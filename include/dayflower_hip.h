@@ -126,7 +126,8 @@ extern "C" {
  *               remap_roughness, emission
  * textures[k] (k = DFR_SLOT_*) replaces that parameter by a texture of the
  * scene (index + 1; 0 keeps the constant): MATTE kd; PLASTIC kd, ks,
- * roughness_u; MIRROR kr; GLASS kr, kt; METAL eta, k, roughness_u,
+ * roughness_u; MIRROR kr; GLASS kr, kt, eta (the colour average);
+ * METAL eta, k, roughness_u,
  * roughness_v; SUBSTRATE kd, ks, roughness_u, roughness_v; GLOSSY kr,
  * roughness_u (as is); CLEAR_COAT kd, ks.  A roughness slot reads the texture's colour average
  * (Texture.getFloat), remapped when remap_roughness is set.  A texture on

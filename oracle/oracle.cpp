@@ -1778,7 +1778,9 @@ dfr_material materialAt(const Scene& sc, const dfr_material& m, const Surface& s
     for (int k = 0; k < 8; k++) {
         if (!m.textures[k]) continue;
         const C3 c = texColor(sc, m.textures[k] - 1, si);
-        if (k == DFR_SLOT_ROUGHNESS_U || k == DFR_SLOT_ROUGHNESS_V) {
+        if (k == DFR_SLOT_ETA && m.type == DFR_MAT_GLASS) {  // GlassMaterial: textureEta.getFloat
+            r.eta[0] = r.eta[1] = r.eta[2] = (c.r + c.g + c.b) / 3.0f;
+        } else if (k == DFR_SLOT_ROUGHNESS_U || k == DFR_SLOT_ROUGHNESS_V) {
             *slot[k] = (c.r + c.g + c.b) / 3.0f;
             if (m.type == DFR_MAT_PLASTIC && k == DFR_SLOT_ROUGHNESS_U) r.roughness_v = *slot[k];
         } else {

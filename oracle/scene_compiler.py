@@ -214,7 +214,7 @@ class SceneCompiler:
             w = [em, pack8(*self.slot(M, abi.SLOT_KR, M.kr), 0, 0)]
             name = "mirror"
         elif t == abi.MAT_GLASS:
-            w = [em, pack8(*self.value(M.eta[0]), 0, 0),
+            w = [em, pack8(*self.float_slot(M, abi.SLOT_ETA, M.eta[0]), 0, 0),
                  pack8(*self.slot(M, abi.SLOT_KR, M.kr), *self.slot(M, abi.SLOT_KT, M.kt)),
                  pack8(*self.value(M.roughness_u), *self.value(M.roughness_v)), int(M.remap_roughness), 0, 0, 0]
             name = "glass"

@@ -376,10 +376,10 @@ def test_shapes_materials_textures_the_kernels_cannot_follow_are_refused():
         mm[1] = SC.pack8(I["ConstantTexture"], 0, I[name], 0)
         arr["matte_materials"] = mm
         _refused(arr, R.IllegalArgumentException, text)
-    # a non-constant texture as a glass eta
+    # a non-constant texture as a glass roughness
     arr = dict(base)
     gm = arr["glass_materials"].copy()
-    gm[1] = SC.pack8(I["UVTexture"], 0, 0, 0)
+    gm[3] = SC.pack8(I["UVTexture"], 0, I["ConstantTexture"], 0)
     arr["glass_materials"] = gm
     _refused(arr, R.UnsupportedOperationException, "constant")
 

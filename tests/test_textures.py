@@ -500,7 +500,8 @@ def test_textured_roughness_eta_and_k_on_the_gpu(kernel):
 def _ext_textured_scene(res):
     """Substrate (checkerboard kd, UV ks, checkerboard roughness v), Glossy
     (UV kr, checkerboard roughness) and ClearCoat (checkerboard kd,
-    constant-texture ks) spheres over a Matte floor."""
+    constant-texture ks) spheres, and smooth and rough Glass spheres with a
+    checkerboard eta, over a Matte floor."""
     sc = S.Scene()
     floor = sc.add_material(S.matte((0.6, 0.6, 0.6)))
     sub = sc.add_material(S.substrate((0.5, 0.3, 0.2), (0.1, 0.1, 0.1), 0.3))
@@ -511,6 +512,15 @@ def _ext_textured_scene(res):
     sc.set_texture(sub, abi.SLOT_KS, sc.uv_texture())
     sc.set_texture(glo, abi.SLOT_KR, sc.blend_texture(sc.uv_texture(), sc.constant_texture((0.9, 0.9, 0.9))))
     sc.set_texture(cc, abi.SLOT_KD, sc.checkerboard_texture(blue, red, scale=(6.0, 6.0)))
+    # smooth and rough glass whose eta is a checkerboard of 1.3 and 1.7
+    e13, e17 = sc.constant_texture((1.3, 1.3, 1.3)), sc.constant_texture((1.7, 1.7, 1.7))
+    smooth = sc.add_material(S.glass(1.5))
+    rough = sc.add_material(S.glass(1.5))
+    sc.materials[rough].roughness_u = sc.materials[rough].roughness_v = 0.2
+    sc.set_texture(smooth, abi.SLOT_ETA, sc.checkerboard_texture(e13, e17, scale=(6.0, 3.0)))
+    sc.set_texture(rough, abi.SLOT_ETA, sc.checkerboard_texture(e17, e13, scale=(4.0, 4.0)))
+    sc.add_sphere((-0.5, 0.5, 1.6), 0.45, smooth)
+    sc.add_sphere((0.6, 0.5, 1.7), 0.45, rough)
     rough_lo, rough_hi = sc.constant_texture((0.05, 0.05, 0.05)), sc.constant_texture((0.5, 0.5, 0.5))
     sc.set_texture(sub, abi.SLOT_ROUGHNESS_V, sc.checkerboard_texture(rough_lo, rough_hi, scale=(5.0, 5.0)))
     sc.set_texture(glo, abi.SLOT_ROUGHNESS_U, sc.checkerboard_texture(rough_hi, rough_lo, scale=(4.0, 8.0)))
